@@ -1,0 +1,298 @@
+/*
+ * CPU oracle for the CMPMC track-transport path (TEST INFRASTRUCTURE ONLY).
+ *
+ * A from-scratch FP64 restatement, in plain C, of the reference algorithm:
+ *   core/rng.hpp, core/math.hpp            RNG (xoshiro256** + splitmix64), Vec3 helpers
+ *   lattice/crystal_lattice.hpp            octahedral v_g LUT, decay inverse-CDF tables
+ *   phonon/phonon_kernels.hpp              rates, DOS samplers, Tamura PDFs, Jacobi solver
+ *   geometry/solids.hpp                    CylinderSolid::intersect_exit
+ *   geometry/surface_property.hpp          reflection_split
+ *   superconductor/kaplan_film.hpp         KaplanFilm::absorb_phonon (MeanFreePath loss)
+ *   transport/engine.hpp                   transport, transport_charges, helpers
+ *   transport/parallel.hpp                 run_events (chunked, ordered merge)
+ *   charge/charge_transport.hpp, charge/charge_kernels.hpp
+ * with the same random-number draw order, so that (compiled with gcc, glibc libm,
+ * -ffp-contract=off) it is bit-identical to the reference on the same seeds.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
+ * library, and only as the checker or the timed CPU baseline: the product path never
+ * calls it.
+ *
+ * Parity status of each function is listed in oracle/README.md together with the check
+ * that pins it.
+ */
+#ifndef CMPMC_ORACLE_H
+#define CMPMC_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- RNG: core/rng.hpp:17-97 ---------------------------------------------------- */
+typedef struct { uint64_t s[4]; } orc_rng;
+
+uint64_t orc_mix64(uint64_t x);
+void     orc_rng_seed(orc_rng* r, uint64_t seed);                  /* rng.hpp:20-29 */
+void     orc_rng_for_stream(orc_rng* r, uint64_t seed, uint64_t stream); /* rng.hpp:34-36 */
+uint64_t orc_next_u64(orc_rng* r);                                 /* rng.hpp:49-59 */
+double   orc_uniform(orc_rng* r);                                  /* rng.hpp:62-64 */
+double   orc_normal(orc_rng* r);                                   /* rng.hpp:71-75 */
+void     orc_isotropic_dir(orc_rng* r, double out[3]);             /* rng.hpp:78-83 */
+void     orc_lambertian_dir(orc_rng* r, const double n[3], double out[3]); /* rng.hpp:86-96 */
+
+/* Sub-stream scheme of the batched form (no reference counterpart: the reference draws
+ * every track of an event from one sequential stream). Track `key` of event `event`
+ * gets Rng(event_base ^ mix64(key + 0xd1b54a32d192ed03)), with
+ * event_base = mix64(mix64(seed) ^ mix64(event + 0x632be59bd9b4e019)).  A daughter
+ * spawned as the n-th child (n = 1, 2, ...) of a track with key K gets key
+ * mix64(K ^ (n * 0x9e3779b97f4a7c15)). */
+uint64_t orc_event_base(uint64_t seed, uint64_t event);
+void     orc_rng_for_track(orc_rng* r, uint64_t event_base, uint64_t key);
+uint64_t orc_child_key(uint64_t parent_key, uint32_t spawn_no);
+
+/* ---- Materials: materials/crystal_config.hpp:43-257, presets.hpp:18-25, 360-362 ---- */
+typedef struct {
+    double density, C11, C12, C44;
+    double beta, gamma, lambda_, mu;
+    double B_iso, A_anh, fTT_frac, polycrystal_mfp_m;
+    double fLDOS, fSTDOS, fFTDOS;
+    double debye_freq, v_sound, v_trans;
+    double bandgap, h_mass_ratio;
+    double e_mass[3];
+    int    n_valleys;
+    double valley_dirs[8][3];
+    double ac_deform_e, ac_deform_h;
+    double iv_min_energy;
+    int    iv_model;                 /* 0 Quadratic, 1 Linear, 2 Matrix (unsupported) */
+    double iv_quad_rate, iv_quad_field, iv_quad_power;
+    double iv_lin_rate0, iv_lin_rate1, iv_lin_power;
+    double e_trapping_mfp_m, h_trapping_mfp_m;
+    double e_trap_ion_d0_mfp_m, e_trap_ion_a_mfp_m, h_trap_ion_d_mfp_m, h_trap_ion_a0_mfp_m;
+    double orient[3][3];
+} orc_crystal;
+
+typedef struct {
+    double film_absorption, film_thickness, v_sound, gap_energy, phonon_lifetime,
+           phonon_lifetime_slope, low_qp_limit, high_qp_limit, subgap_absorption,
+           absorber_gap, absorber_eff, absorber_eff_slope, temperature;
+    int keep_all_phonons;
+} orc_film;
+
+void orc_crystal_germanium(orc_crystal* c);
+void orc_film_aluminum(orc_film* f);
+
+/* ---- Lattice: lattice/crystal_lattice.hpp:24-478 ------------------------------------ */
+#define ORC_N_OCT 256
+#define ORC_N_CDF 1024
+typedef struct {
+    orc_crystal cfg;
+    double C[3][3][3][3];
+    float* oct;                      /* [3][256][256][4] = vx, vy, vz, |v| */
+    double cdf_lt[ORC_N_CDF], cdf_tt[ORC_N_CDF];
+    /* charge tensors (crystal_lattice.hpp:394-438) */
+    double e_cond_mass_eV, h_mass_eV;
+    double mass_ratio[3], hv_sqrt[3], hv_sqrt_inv[3];
+    double valley_rot[8][3][3];
+    double l0_e, l0_h, k_sound_e, k_sound_h;
+} orc_lattice;
+
+int  orc_lattice_init(orc_lattice* L, const orc_crystal* cfg);
+void orc_lattice_free(orc_lattice* L);
+void orc_solve_christoffel(const double C[3][3][3][3], double rho, const double k[3],
+                           double vg[3][3], double vp[3]);        /* phonon_kernels.hpp:182-247 */
+void orc_map_k_to_vg(const orc_lattice* L, int mode, const double k[3],
+                     double vg_dir[3], double* vg_mag);          /* crystal_lattice.hpp:144-176 */
+double orc_tamura_lt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:98-106 */
+double orc_tamura_tt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:115-129 */
+double orc_theta_L_to_LT(const orc_crystal* c, double x);
+double orc_theta_T_from_LT(const orc_crystal* c, double x);
+double orc_theta_TT(const orc_crystal* c, double x);
+double orc_sample_lt_fraction(const orc_lattice* L, orc_rng* r);
+double orc_sample_tt_fraction(const orc_lattice* L, orc_rng* r);
+
+/* ---- Detector: transport/engine.hpp:96-156, geometry/solids.hpp, surface_property.hpp------------------------- */
+#define ORC_MAX_POLY 8
+typedef struct {
+    double prob_abs, prob_refl, refl_cutoff, anh_cutoff;
+    double prob_charge_abs, prob_charge_spec, min_k_elec, min_k_hole;
+    int    n_spec, n_diff, n_anh;
+    double spec[ORC_MAX_POLY], diff[ORC_MAX_POLY], anh[ORC_MAX_POLY];
+    double spec_prob_const;
+    int    has_film;
+    orc_film film;
+} orc_surface;
+
+typedef struct {
+    double radius, half_z;
+    double efield[3];
+    orc_surface surf[3];
+    uint32_t max_bounces;
+    double min_phonon_eV;
+    double luke_sample_fraction;
+    double charge_max_dk_frac, charge_dt_cap_s;
+    double fermi_level_above_Ev_eV;
+    uint32_t russian_roulette_bounces;
+    double russian_roulette_survival;
+} orc_detector;
+
+void orc_detector_default(orc_detector* d);
+void orc_reflection_split(const orc_surface* s, double energy_eV, double out[3]); /* spec, diff, anh */
+double orc_intersect_exit(const orc_detector* d, const double p[3], const double v[3],
+                          double n_in[3], int* surface);
+
+/* ---- Records: core/track.hpp, transport/engine.hpp:29-93 ---------------------------- */
+typedef struct {
+    double pos[3], k[3], vg[3];
+    double energy, vg_mag, time, weight;
+    uint32_t bounces;
+    int32_t mode;
+    /* batched form only */
+    orc_rng rng;
+    uint64_t key;
+    uint32_t spawn_no;
+    uint32_t hit_seq;
+    int32_t first_flag;              /* record the branch code of this track's first step */
+} orc_track;
+
+typedef struct {
+    double pos[3], p[3];
+    double ekin, time, weight;
+    int32_t valley;
+    int32_t type;                    /* 1 electron, 2 hole */
+    /* batched form only */
+    orc_rng rng;
+    uint64_t key;
+    uint32_t spawn_no;
+    uint32_t hit_seq;
+} orc_charge;
+
+typedef struct {                     /* SensorHit field values (not its C++ layout) */
+    double pos[3];
+    double time, edep, incident;
+    int32_t surface, excitation, mode, channel;
+    uint64_t key;                    /* batched form: genealogy key of the producing track */
+    uint32_t seq;                    /* batched form: hit sequence number on that track */
+} orc_hit;
+
+typedef struct {                     /* same field order and size as SimStats (208 B) */
+    uint64_t total_steps, isotope_scatters, polycrystal_scatters, decays_L_to_TT,
+             decays_L_to_LT, surface_refls, surface_downconversions, sensor_hits,
+             killed_tracks;
+    uint64_t charge_steps, luke_phonons_emitted, intervalley_scatters, charges_trapped,
+             trap_ionizations, charge_surface_refls, charges_collected;
+    double   total_luke_phonon_eV, charge_recomb_edep_eV;
+    uint64_t sc_pair_breakings, qp_diffusion_steps, qp_phonons_radiated,
+             qp_recombinations, qp_local_traps;
+    double   total_injected_eV, total_deposited_eV, escaped_or_killed_eV;
+} orc_stats;
+
+void orc_stats_add(orc_stats* a, const orc_stats* b);   /* engine.hpp:64-92 */
+
+/* ---- Kaplan film: superconductor/kaplan_film.hpp:106-323 ------------------------------ */
+/* Returns edep; writes up to 256 re-emitted energies into refl[], count into *n_refl. */
+double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_refl, orc_rng* r);
+double orc_sample_qp_energy(const orc_film* f, double E, orc_rng* r);
+double orc_sample_phonon_energy(const orc_film* f, double E, orc_rng* r);
+
+/* ---- Engine ------------------------------------------------------------------------ */
+/* Growable containers used by the transport entry points. */
+typedef struct { orc_track* v; size_t n, cap; } orc_track_vec;
+typedef struct { orc_charge* v; size_t n, cap; } orc_charge_vec;
+typedef struct { orc_hit* v; size_t n, cap; } orc_hit_vec;
+void orc_track_push(orc_track_vec* s, const orc_track* t);
+void orc_charge_push(orc_charge_vec* s, const orc_charge* c);
+void orc_hit_push(orc_hit_vec* s, const orc_hit* h);
+
+typedef struct {
+    const orc_lattice* L;
+    const orc_detector* D;
+} orc_engine;
+
+/* SimulationEngine::make_track (engine.hpp:171-187) */
+void orc_make_track(const orc_engine* e, const double pos[3], const double k[3], int mode,
+                    double energy, double time, double weight, orc_track* out);
+/* ChargePhysics::make_charge (charge_transport.hpp:49-70) */
+void orc_make_charge(const orc_engine* e, int type, const double pos[3], const double dir[3],
+                     double ekin, int valley, double time, orc_charge* out);
+
+/* First-step branch codes of a phonon track (recorded for the first popped track). */
+enum {
+    ORC_BR_NONE = 0, ORC_BR_KILLED = 1, ORC_BR_ISOTOPE = 2, ORC_BR_POLY = 3,
+    ORC_BR_DECAY_TT = 4, ORC_BR_DECAY_LT = 5, ORC_BR_FILM = 6, ORC_BR_ESCAPE = 7,
+    ORC_BR_SURF_ANH = 8, ORC_BR_SPECULAR = 9, ORC_BR_DIFFUSE = 10, ORC_BR_ROULETTE = 11,
+    ORC_BR_FILM_MISS = 12
+};
+
+/* SimulationEngine::transport (engine.hpp:403-582). per_track_rng = 0 follows the
+ * reference exactly (one stream `rng`, LIFO). per_track_rng = 1 is the batched form:
+ * every track draws from its own stream (orc_track.rng), daughters get child streams,
+ * and hits carry (key, seq). first_branch (may be NULL) receives the branch code of the
+ * first step of the first track popped. */
+void orc_transport(const orc_engine* e, orc_track_vec* stack, orc_hit_vec* hits,
+                   orc_stats* stats, orc_rng* rng, int per_track_rng, uint64_t event_base,
+                   int* first_branch);
+
+/* SimulationEngine::transport_charges (engine.hpp:192-380), first-order stepping.
+ * per_track_rng as for orc_transport: charges and the Luke phonons they push get their
+ * own streams (initial charge j of an event has key 0x8000000000000000 | j). */
+void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_track_vec* phonons,
+                           orc_hit_vec* hits, orc_stats* stats, orc_rng* rng,
+                           int per_track_rng, uint64_t event_base);
+
+/* ---- Workloads and run_events (transport/parallel.hpp:76-148) ------------------------ */
+enum {
+    ORC_WL_PHONON_POINT = 1,   /* one phonon of (mode, energy) at `origin`, isotropic k */
+    ORC_WL_PHONON_LOGN  = 2,   /* lognormal count, uniform volume, DOS modes, U[e_lo, e_hi] */
+    ORC_WL_PAIR         = 3,   /* one e-/h+ pair at a uniform position in the volume */
+    ORC_WL_MIXED        = 4,   /* tests/test_parallel.cpp:17-31 MixedGenerator */
+    ORC_WL_PHONON_UNIF  = 5    /* n ~ U{n_lo..n_hi} phonons of (mode, energy) at uniform positions */
+};
+
+typedef struct {
+    int kind;
+    int mode;                  /* phonon mode (POINT, UNIF, MIXED) */
+    double energy_eV;          /* POINT, UNIF, MIXED */
+    double origin[3];          /* POINT */
+    double logn_median, logn_sigma;
+    uint32_t n_min, n_max;     /* LOGN clip, UNIF range */
+    double e_lo_eV, e_hi_eV;   /* LOGN energy range */
+    int pair_at_origin;        /* PAIR: 1 = origin (the reference bench), 0 = uniform volume */
+} orc_workload;
+
+typedef struct {
+    orc_stats stats;
+    orc_hit* hits; size_t n_hits;
+    uint64_t* hit_offsets;         /* n_events + 1 */
+    int32_t* first_branch;         /* n_events (may be NULL if not requested) */
+    uint32_t* hits_per_event;      /* internal */
+    unsigned threads_used;
+    double wall_seconds;
+} orc_run_result;
+
+/* batched = 0: the reference semantics (one stream per event, LIFO). batched = 1: the
+ * sub-stream scheme above (initial track j of an event has key j). */
+int  orc_run_events(const orc_engine* e, const orc_workload* wl, uint64_t n_events,
+                    uint64_t seed, unsigned threads, uint32_t events_per_chunk, int keep_hits,
+                    int batched, int want_first_branch, orc_run_result* out);
+void orc_run_result_free(orc_run_result* r);
+
+/* Initial tracks of one event of a workload (the generator), exposed for tests. For the
+ * batched form, tracks come out with their rng/key set. Returns the track count. */
+size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t seed,
+                          uint64_t event, int batched, orc_track_vec* phonons,
+                          orc_charge_vec* charges, double* injected, orc_rng* event_rng);
+
+/* Heap-allocated engine (lattice tables + detector copy) for ctypes callers. */
+orc_engine* orc_engine_new(const orc_crystal* cfg, const orc_detector* det);
+void orc_engine_free(orc_engine* e);
+const orc_lattice* orc_engine_lattice(const orc_engine* e);
+const float* orc_engine_oct(const orc_engine* e);
+size_t orc_sizeof(int which);   /* struct sizes, for binding checks */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,428 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) and of the in-place
+reference build (oracle/_ref/libcmpmc_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the checker
+and by bench.py's cpu_baseline leg.  The product (cmpmc_amd) never imports this module.
+The Structure layouts mirror oracle/oracle.h field by field; ``check_layout`` compares
+their sizes against the C library's own ``orc_sizeof``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libcmpmc_ref.so")
+
+u64 = C.c_uint64
+u32 = C.c_uint32
+i32 = C.c_int32
+dbl = C.c_double
+
+
+class Rng(C.Structure):
+    _fields_ = [("s", u64 * 4)]
+
+
+class Crystal(C.Structure):
+    _fields_ = [
+        ("density", dbl), ("C11", dbl), ("C12", dbl), ("C44", dbl),
+        ("beta", dbl), ("gamma", dbl), ("lambda_", dbl), ("mu", dbl),
+        ("B_iso", dbl), ("A_anh", dbl), ("fTT_frac", dbl), ("polycrystal_mfp_m", dbl),
+        ("fLDOS", dbl), ("fSTDOS", dbl), ("fFTDOS", dbl),
+        ("debye_freq", dbl), ("v_sound", dbl), ("v_trans", dbl),
+        ("bandgap", dbl), ("h_mass_ratio", dbl),
+        ("e_mass", dbl * 3),
+        ("n_valleys", C.c_int),
+        ("valley_dirs", (dbl * 3) * 8),
+        ("ac_deform_e", dbl), ("ac_deform_h", dbl),
+        ("iv_min_energy", dbl),
+        ("iv_model", C.c_int),
+        ("iv_quad_rate", dbl), ("iv_quad_field", dbl), ("iv_quad_power", dbl),
+        ("iv_lin_rate0", dbl), ("iv_lin_rate1", dbl), ("iv_lin_power", dbl),
+        ("e_trapping_mfp_m", dbl), ("h_trapping_mfp_m", dbl),
+        ("e_trap_ion_d0_mfp_m", dbl), ("e_trap_ion_a_mfp_m", dbl),
+        ("h_trap_ion_d_mfp_m", dbl), ("h_trap_ion_a0_mfp_m", dbl),
+        ("orient", (dbl * 3) * 3),
+    ]
+
+
+class Film(C.Structure):
+    _fields_ = [(n, dbl) for n in (
+        "film_absorption", "film_thickness", "v_sound", "gap_energy", "phonon_lifetime",
+        "phonon_lifetime_slope", "low_qp_limit", "high_qp_limit", "subgap_absorption",
+        "absorber_gap", "absorber_eff", "absorber_eff_slope", "temperature")] + [
+        ("keep_all_phonons", C.c_int)]
+
+
+class Surface(C.Structure):
+    _fields_ = [
+        ("prob_abs", dbl), ("prob_refl", dbl), ("refl_cutoff", dbl), ("anh_cutoff", dbl),
+        ("prob_charge_abs", dbl), ("prob_charge_spec", dbl), ("min_k_elec", dbl), ("min_k_hole", dbl),
+        ("n_spec", C.c_int), ("n_diff", C.c_int), ("n_anh", C.c_int),
+        ("spec", dbl * 8), ("diff", dbl * 8), ("anh", dbl * 8),
+        ("spec_prob_const", dbl),
+        ("has_film", C.c_int),
+        ("film", Film),
+    ]
+
+
+class Detector(C.Structure):
+    _fields_ = [
+        ("radius", dbl), ("half_z", dbl), ("efield", dbl * 3),
+        ("surf", Surface * 3),
+        ("max_bounces", u32), ("min_phonon_eV", dbl), ("luke_sample_fraction", dbl),
+        ("charge_max_dk_frac", dbl), ("charge_dt_cap_s", dbl), ("fermi_level_above_Ev_eV", dbl),
+        ("russian_roulette_bounces", u32), ("russian_roulette_survival", dbl),
+    ]
+
+
+class Workload(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int), ("mode", C.c_int), ("energy_eV", dbl), ("origin", dbl * 3),
+        ("logn_median", dbl), ("logn_sigma", dbl), ("n_min", u32), ("n_max", u32),
+        ("e_lo_eV", dbl), ("e_hi_eV", dbl), ("pair_at_origin", C.c_int),
+    ]
+
+
+class Hit(C.Structure):
+    _fields_ = [
+        ("pos", dbl * 3), ("time", dbl), ("edep", dbl), ("incident", dbl),
+        ("surface", i32), ("excitation", i32), ("mode", i32), ("channel", i32),
+        ("key", u64), ("seq", u32),
+    ]
+
+
+STATS_FIELDS = [
+    ("total_steps", u64), ("isotope_scatters", u64), ("polycrystal_scatters", u64),
+    ("decays_L_to_TT", u64), ("decays_L_to_LT", u64), ("surface_refls", u64),
+    ("surface_downconversions", u64), ("sensor_hits", u64), ("killed_tracks", u64),
+    ("charge_steps", u64), ("luke_phonons_emitted", u64), ("intervalley_scatters", u64),
+    ("charges_trapped", u64), ("trap_ionizations", u64), ("charge_surface_refls", u64),
+    ("charges_collected", u64),
+    ("total_luke_phonon_eV", dbl), ("charge_recomb_edep_eV", dbl),
+    ("sc_pair_breakings", u64), ("qp_diffusion_steps", u64), ("qp_phonons_radiated", u64),
+    ("qp_recombinations", u64), ("qp_local_traps", u64),
+    ("total_injected_eV", dbl), ("total_deposited_eV", dbl), ("escaped_or_killed_eV", dbl),
+]
+
+
+class Stats(C.Structure):
+    _fields_ = STATS_FIELDS
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Track(C.Structure):
+    _fields_ = [
+        ("pos", dbl * 3), ("k", dbl * 3), ("vg", dbl * 3),
+        ("energy", dbl), ("vg_mag", dbl), ("time", dbl), ("weight", dbl),
+        ("bounces", u32), ("mode", i32), ("rng", Rng), ("key", u64),
+        ("spawn_no", u32), ("hit_seq", u32), ("first_flag", i32),
+    ]
+
+
+class Charge(C.Structure):
+    _fields_ = [
+        ("pos", dbl * 3), ("p", dbl * 3), ("ekin", dbl), ("time", dbl), ("weight", dbl),
+        ("valley", i32), ("type", i32), ("rng", Rng), ("key", u64),
+        ("spawn_no", u32), ("hit_seq", u32),
+    ]
+
+
+class RunResult(C.Structure):
+    _fields_ = [
+        ("stats", Stats), ("hits", C.POINTER(Hit)), ("n_hits", C.c_size_t),
+        ("hit_offsets", C.POINTER(u64)), ("first_branch", C.POINTER(i32)),
+        ("hits_per_event", C.POINTER(u32)), ("threads_used", C.c_uint), ("wall_seconds", dbl),
+    ]
+
+
+class TrackVec(C.Structure):
+    _fields_ = [("v", C.POINTER(Track)), ("n", C.c_size_t), ("cap", C.c_size_t)]
+
+
+class HitVec(C.Structure):
+    _fields_ = [("v", C.POINTER(Hit)), ("n", C.c_size_t), ("cap", C.c_size_t)]
+
+
+WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_MIXED, WL_PHONON_UNIF = 1, 2, 3, 4, 5
+BRANCH_NAMES = {0: "none", 1: "killed", 2: "isotope", 3: "poly", 4: "decay_TT", 5: "decay_LT",
+                6: "film", 7: "escape", 8: "surf_anh", 9: "specular", 10: "diffuse",
+                11: "roulette", 12: "film_miss"}
+
+HIT_DTYPE = np.dtype([
+    ("pos", "<f8", (3,)), ("time", "<f8"), ("edep", "<f8"), ("incident", "<f8"),
+    ("surface", "<i4"), ("excitation", "<i4"), ("mode", "<i4"), ("channel", "<i4"),
+    ("key", "<u8"), ("seq", "<u4"), ("_pad", "<u4"),
+])
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the oracle (and the reference build where the reference tree exists)."""
+    out = subprocess.run(["make", "-C", HERE, "all"], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_engine_new.restype = C.c_void_p
+        L.orc_engine_new.argtypes = [C.POINTER(Crystal), C.POINTER(Detector)]
+        L.orc_engine_free.argtypes = [C.c_void_p]
+        L.orc_engine_oct.restype = C.POINTER(C.c_float)
+        L.orc_engine_oct.argtypes = [C.c_void_p]
+        L.orc_engine_lattice.restype = C.c_void_p
+        L.orc_engine_lattice.argtypes = [C.c_void_p]
+        L.orc_sizeof.restype = C.c_size_t
+        L.orc_sizeof.argtypes = [C.c_int]
+        L.orc_run_events.argtypes = [C.c_void_p, C.POINTER(Workload), u64, u64, C.c_uint, u32,
+                                     C.c_int, C.c_int, C.c_int, C.POINTER(RunResult)]
+        L.orc_run_result_free.argtypes = [C.POINTER(RunResult)]
+        L.orc_mix64.restype = u64
+        L.orc_mix64.argtypes = [u64]
+        L.orc_next_u64.restype = u64
+        L.orc_next_u64.argtypes = [C.POINTER(Rng)]
+        L.orc_uniform.restype = dbl
+        L.orc_uniform.argtypes = [C.POINTER(Rng)]
+        L.orc_normal.restype = dbl
+        L.orc_normal.argtypes = [C.POINTER(Rng)]
+        L.orc_rng_seed.argtypes = [C.POINTER(Rng), u64]
+        L.orc_rng_for_stream.argtypes = [C.POINTER(Rng), u64, u64]
+        L.orc_isotropic_dir.argtypes = [C.POINTER(Rng), C.POINTER(dbl)]
+        L.orc_lambertian_dir.argtypes = [C.POINTER(Rng), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_event_base.restype = u64
+        L.orc_event_base.argtypes = [u64, u64]
+        L.orc_child_key.restype = u64
+        L.orc_child_key.argtypes = [u64, u32]
+        L.orc_rng_for_track.argtypes = [C.POINTER(Rng), u64, u64]
+        L.orc_map_k_to_vg.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_solve_christoffel.argtypes = [C.c_void_p, dbl, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_sample_lt_fraction.restype = dbl
+        L.orc_sample_lt_fraction.argtypes = [C.c_void_p, C.POINTER(Rng)]
+        L.orc_sample_tt_fraction.restype = dbl
+        L.orc_sample_tt_fraction.argtypes = [C.c_void_p, C.POINTER(Rng)]
+        for n in ("orc_theta_TT", "orc_theta_L_to_LT", "orc_theta_T_from_LT",
+                  "orc_tamura_lt_pdf", "orc_tamura_tt_pdf"):
+            getattr(L, n).restype = dbl
+            getattr(L, n).argtypes = [C.POINTER(Crystal), dbl]
+        L.orc_absorb_phonon.restype = dbl
+        L.orc_absorb_phonon.argtypes = [C.POINTER(Film), dbl, C.POINTER(dbl), C.POINTER(C.c_int), C.POINTER(Rng)]
+        L.orc_sample_qp_energy.restype = dbl
+        L.orc_sample_qp_energy.argtypes = [C.POINTER(Film), dbl, C.POINTER(Rng)]
+        L.orc_sample_phonon_energy.restype = dbl
+        L.orc_sample_phonon_energy.argtypes = [C.POINTER(Film), dbl, C.POINTER(Rng)]
+        L.orc_intersect_exit.restype = dbl
+        L.orc_intersect_exit.argtypes = [C.POINTER(Detector), C.POINTER(dbl), C.POINTER(dbl),
+                                         C.POINTER(dbl), C.POINTER(C.c_int)]
+        L.orc_reflection_split.argtypes = [C.POINTER(Surface), dbl, C.POINTER(dbl)]
+        L.orc_transport.argtypes = [C.c_void_p, C.POINTER(TrackVec), C.POINTER(HitVec), C.POINTER(Stats),
+                                    C.POINTER(Rng), C.c_int, u64, C.POINTER(C.c_int)]
+        L.orc_make_track.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(dbl), C.c_int, dbl, dbl, dbl,
+                                     C.POINTER(Track)]
+        L.orc_track_push.argtypes = [C.POINTER(TrackVec), C.POINTER(Track)]
+        L.orc_crystal_germanium.argtypes = [C.POINTER(Crystal)]
+        L.orc_detector_default.argtypes = [C.POINTER(Detector)]
+        L.orc_film_aluminum.argtypes = [C.POINTER(Film)]
+        _lib = L
+        check_layout(L)
+    return _lib
+
+
+def check_layout(L) -> None:
+    for idx, cls in enumerate([Crystal, Detector, Workload, Hit, Stats, Track, Surface, Film, RunResult, Charge]):
+        n = L.orc_sizeof(idx)
+        if n != C.sizeof(cls):
+            raise RuntimeError(f"oracle struct {cls.__name__}: C {n} B vs ctypes {C.sizeof(cls)} B")
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        R = C.CDLL(REF_PATH)
+        R.ref_rng_stream.argtypes = [u64, u64, C.c_int, C.POINTER(u64)]
+        R.ref_rng_seeded.argtypes = [u64, C.c_int, C.POINTER(u64)]
+        R.ref_rng_samplers.argtypes = [u64, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_lattice_info.argtypes = [C.POINTER(dbl)]
+        R.ref_decay_fractions.argtypes = [u64, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_map_k_to_vg.argtypes = [C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_christoffel.argtypes = [C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_absorb_phonon.argtypes = [dbl, u64, C.c_int, C.POINTER(dbl), C.POINTER(C.c_int),
+                                        C.POINTER(dbl), C.POINTER(u64)]
+        R.ref_intersect.argtypes = [C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_run_events.restype = C.c_long
+        R.ref_run_events.argtypes = [C.POINTER(Crystal), C.POINTER(Detector), C.POINTER(Workload), u64, u64,
+                                     C.c_uint, u32, C.c_int, C.c_void_p, C.POINTER(Hit), C.c_long,
+                                     C.POINTER(u64), C.POINTER(dbl), C.c_char_p, C.c_int]
+        R.ref_transport_stack.restype = C.c_long
+        R.ref_transport_stack.argtypes = [C.POINTER(Detector), C.c_int, C.POINTER(dbl), u64, u64,
+                                          C.c_void_p, C.POINTER(Hit), C.c_long, C.POINTER(u64)]
+        R.ref_sizeof_simstats.restype = C.c_int
+        _ref = R
+    return _ref
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(dbl))
+
+
+# ---------------------------------------------------------------- configuration helpers
+def germanium() -> Crystal:
+    c = Crystal()
+    lib().orc_crystal_germanium(C.byref(c))
+    return c
+
+
+def default_detector() -> Detector:
+    d = Detector()
+    lib().orc_detector_default(C.byref(d))
+    return d
+
+
+def set_bias(det: Detector, volts: float, thickness_m: float = 2.54e-2) -> None:
+    """UniformElectricField::from_bias (charge_transport.hpp:22-24)."""
+    det.efield[0] = 0.0
+    det.efield[1] = 0.0
+    det.efield[2] = -volts / thickness_m
+
+
+def demo_polynomial_surfaces(det: Detector) -> None:
+    """The reference bench detector (cmpmc_bench.cpp:250-258): demo polynomials
+    (surface_property.hpp:254-272), film-entry 0.2 on both faces, 100 bounces."""
+    spec = [0.928, -2.03e-4, -3.21e-6, 3.1e-9, 2.9e-13]
+    diff = [5.88e-2, 7.83e-4, -2.47e-6, 1.71e-8, -2.98e-11]
+    anh = [0.0, 0.0, 0.0, 0.0, 0.0, 1.51e-14]
+    for i in range(3):
+        s = det.surf[i]
+        s.n_spec, s.n_diff, s.n_anh = len(spec), len(diff), len(anh)
+        for j, v in enumerate(spec):
+            s.spec[j] = v
+        for j, v in enumerate(diff):
+            s.diff[j] = v
+        for j, v in enumerate(anh):
+            s.anh[j] = v
+        s.anh_cutoff, s.refl_cutoff = 520.0, 350.0
+    det.max_bounces = 100
+
+
+def workload(kind: int, **kw) -> Workload:
+    w = Workload()
+    w.kind = kind
+    w.mode = kw.get("mode", 0)
+    w.energy_eV = kw.get("energy_eV", 7.5e-3)
+    o = kw.get("origin", (0.0, 0.0, 0.0))
+    for i in range(3):
+        w.origin[i] = o[i]
+    w.logn_median = kw.get("logn_median", 200.0)
+    w.logn_sigma = kw.get("logn_sigma", 1.0)
+    w.n_min = kw.get("n_min", 1)
+    w.n_max = kw.get("n_max", 20000)
+    w.e_lo_eV = kw.get("e_lo_eV", 1e-3)
+    w.e_hi_eV = kw.get("e_hi_eV", 31e-3)
+    w.pair_at_origin = kw.get("pair_at_origin", 0)
+    return w
+
+
+class Engine:
+    """Oracle engine: lattice tables + detector copy."""
+
+    def __init__(self, crystal: Crystal | None = None, detector: Detector | None = None):
+        self.crystal = crystal if crystal is not None else germanium()
+        self.detector = detector if detector is not None else default_detector()
+        self.ptr = lib().orc_engine_new(C.byref(self.crystal), C.byref(self.detector))
+        if not self.ptr:
+            raise MemoryError("orc_engine_new failed")
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().orc_engine_free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+    def oct_table(self) -> np.ndarray:
+        p = lib().orc_engine_oct(self.ptr)
+        return np.ctypeslib.as_array(p, shape=(3, 256, 256, 4)).copy()
+
+    def lattice_ptr(self):
+        return lib().orc_engine_lattice(self.ptr)
+
+    def map_k_to_vg(self, mode: int, k) -> np.ndarray:
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+        out = np.empty((k.shape[0], 4))
+        vg = np.empty(3)
+        mag = C.c_double()
+        L = lib()
+        lp = self.lattice_ptr()
+        for i in range(k.shape[0]):
+            row = np.ascontiguousarray(k[i])
+            L.orc_map_k_to_vg(lp, mode, dptr(row), dptr(vg), C.byref(mag))
+            out[i, :3] = vg
+            out[i, 3] = mag.value
+        return out
+
+    def run(self, wl: Workload, n_events: int, seed: int, threads: int = 0, chunk: int = 16,
+            keep_hits: bool = True, batched: bool = False, first_branch: bool = False) -> dict:
+        r = RunResult()
+        lib().orc_run_events(self.ptr, C.byref(wl), n_events, seed, threads, chunk, int(keep_hits),
+                             int(batched), int(first_branch), C.byref(r))
+        try:
+            out = {"stats": r.stats.as_dict(), "stats_raw": bytes(r.stats),
+                   "wall_seconds": r.wall_seconds, "threads": r.threads_used}
+            n = r.n_hits
+            if n:
+                buf = (Hit * n).from_address(C.addressof(r.hits.contents))
+                out["hits"] = np.frombuffer(bytes(buf), dtype=HIT_DTYPE).copy()
+            else:
+                out["hits"] = np.zeros(0, dtype=HIT_DTYPE)
+            out["hit_offsets"] = np.ctypeslib.as_array(r.hit_offsets, shape=(n_events + 1,)).copy()
+            if first_branch:
+                out["first_branch"] = np.ctypeslib.as_array(r.first_branch, shape=(n_events,)).copy()
+            return out
+        finally:
+            lib().orc_run_result_free(C.byref(r))
+
+
+def ref_run(wl: Workload, n_events: int, seed: int, detector: Detector | None = None,
+            crystal: Crystal | None = None, threads: int = 0, chunk: int = 16,
+            keep_hits: bool = True, hit_cap: int | None = None) -> dict:
+    """run_events of the reference build on the same workload."""
+    R = ref()
+    det = detector if detector is not None else default_detector()
+    cr = crystal if crystal is not None else germanium()
+    stats = Stats()
+    cap = hit_cap if hit_cap is not None else 64 * n_events + 1024
+    hits = (Hit * cap)()
+    offs = np.zeros(n_events + 1, dtype=np.uint64)
+    wall = C.c_double()
+    err = C.create_string_buffer(512)
+    n = R.ref_run_events(C.byref(cr), C.byref(det), C.byref(wl), n_events, seed, threads, chunk,
+                         int(keep_hits), C.byref(stats), hits, cap,
+                         offs.ctypes.data_as(C.POINTER(u64)), C.byref(wall), err, 512)
+    if n < 0:
+        raise RuntimeError(err.value.decode())
+    if n > cap:   # rerun with room for every hit (same seeds, same result)
+        return ref_run(wl, n_events, seed, det, cr, threads, chunk, keep_hits, hit_cap=n)
+    arr = np.frombuffer(bytes(hits), dtype=HIT_DTYPE)[:n].copy()
+    return {"stats": stats.as_dict(), "stats_raw": bytes(stats), "hits": arr, "hit_offsets": offs,
+            "wall_seconds": wall.value}

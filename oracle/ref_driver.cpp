@@ -1,0 +1,334 @@
+// Driver that exposes the reference implementation (the header-only C++20 library under
+// /root/reference/include, compiled in place by oracle/Makefile) through a small C ABI,
+// so that tests can pin the CPU oracle against it bit for bit.
+//
+// TEST INFRASTRUCTURE ONLY: the built library lands in oracle/_ref/ and is loaded by
+// tests/ and by bench.py's cpu_baseline leg, never by the product.  The POD structs of
+// oracle/oracle.h describe the workload and the detector; this file converts them into
+// the reference's DetectorModel / generator lambdas.
+#include "cmpmc/cmpmc.hpp"
+#include "oracle.h"
+
+#include <cstring>
+#include <vector>
+
+using namespace cmpmc;
+
+namespace {
+
+SurfaceProperty to_surface(const orc_surface& s) {
+    SurfaceProperty p;
+    p.prob_abs = s.prob_abs;
+    p.prob_refl = s.prob_refl;
+    p.refl_cutoff = s.refl_cutoff;
+    p.anh_cutoff = s.anh_cutoff;
+    p.prob_charge_abs = s.prob_charge_abs;
+    p.prob_charge_spec = s.prob_charge_spec;
+    p.min_k_elec = s.min_k_elec;
+    p.min_k_hole = s.min_k_hole;
+    p.spec_coeffs.assign(s.spec, s.spec + s.n_spec);
+    p.diff_coeffs.assign(s.diff, s.diff + s.n_diff);
+    p.anh_coeffs.assign(s.anh, s.anh + s.n_anh);
+    p.spec_prob_const = s.spec_prob_const;
+    if (s.has_film) {
+        SuperconductorFilmConfig f;
+        f.film_absorption = s.film.film_absorption;
+        f.film_thickness = s.film.film_thickness;
+        f.v_sound = s.film.v_sound;
+        f.gap_energy = s.film.gap_energy;
+        f.phonon_lifetime = s.film.phonon_lifetime;
+        f.phonon_lifetime_slope = s.film.phonon_lifetime_slope;
+        f.low_qp_limit = s.film.low_qp_limit;
+        f.high_qp_limit = s.film.high_qp_limit;
+        f.subgap_absorption = s.film.subgap_absorption;
+        f.absorber_gap = s.film.absorber_gap;
+        f.absorber_eff = s.film.absorber_eff;
+        f.absorber_eff_slope = s.film.absorber_eff_slope;
+        f.temperature = s.film.temperature;
+        f.keep_all_phonons = s.film.keep_all_phonons != 0;
+        p.kaplan_sensor = KaplanFilm(f);
+    }
+    return p;
+}
+
+DetectorModel to_detector(const orc_detector* d) {
+    DetectorModel m;
+    m.solid.radius = d->radius;
+    m.solid.half_z = d->half_z;
+    m.e_field.field_Vm = Vec3{d->efield[0], d->efield[1], d->efield[2]};
+    for (int i = 0; i < 3; ++i) m.surfaces[i] = to_surface(d->surf[i]);
+    m.max_bounces = d->max_bounces;
+    m.min_phonon_eV = d->min_phonon_eV;
+    m.luke_sample_fraction = d->luke_sample_fraction;
+    m.charge_max_dk_frac = d->charge_max_dk_frac;
+    m.charge_dt_cap_s = d->charge_dt_cap_s;
+    m.fermi_level_above_Ev_eV = d->fermi_level_above_Ev_eV;
+    m.russian_roulette_bounces = d->russian_roulette_bounces;
+    m.russian_roulette_survival = d->russian_roulette_survival;
+    return m;
+}
+
+CrystalConfig to_crystal(const orc_crystal* c) {
+    CrystalConfig g = materials::Germanium();
+    g.e_trapping_mfp_m = c->e_trapping_mfp_m;
+    g.h_trapping_mfp_m = c->h_trapping_mfp_m;
+    g.e_trap_ion_d0_mfp_m = c->e_trap_ion_d0_mfp_m;
+    g.e_trap_ion_a_mfp_m = c->e_trap_ion_a_mfp_m;
+    g.h_trap_ion_d_mfp_m = c->h_trap_ion_d_mfp_m;
+    g.h_trap_ion_a0_mfp_m = c->h_trap_ion_a0_mfp_m;
+    g.B_iso = c->B_iso;
+    g.A_anh = c->A_anh;
+    return g;
+}
+
+void to_hit(const SensorHit& h, orc_hit* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->pos[0] = h.pos.x; o->pos[1] = h.pos.y; o->pos[2] = h.pos.z;
+    o->time = h.time; o->edep = h.edep; o->incident = h.incident_energy;
+    o->surface = static_cast<int32_t>(h.surface_id);
+    o->excitation = static_cast<int32_t>(h.excitation);
+    o->mode = static_cast<int32_t>(h.mode);
+    o->channel = h.channel_id;
+}
+
+Vec3 uniform_volume(const DetectorModel& d, Rng& r) {
+    const double rr = d.solid.radius * std::sqrt(r.uniform());
+    const double phi = units::two_pi * r.uniform();
+    const double z = d.solid.half_z * (2.0 * r.uniform() - 1.0);
+    return {rr * std::cos(phi), rr * std::sin(phi), z};
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_sizeof_simstats(void) { return static_cast<int>(sizeof(SimStats)); }
+
+void ref_rng_stream(uint64_t seed, uint64_t stream, int n, uint64_t* out) {
+    Rng r = Rng::for_stream(seed, stream);
+    for (int i = 0; i < n; ++i) out[i] = r.next_u64();
+}
+
+void ref_rng_seeded(uint64_t seed, int n, uint64_t* out) {
+    Rng r(seed);
+    for (int i = 0; i < n; ++i) out[i] = r.next_u64();
+}
+
+// Normal / isotropic / lambertian draws of Rng(seed): out gets n*4 doubles
+// (normal, iso x, iso y, iso z) followed by n*3 lambertian components about `nrm`.
+void ref_rng_samplers(uint64_t seed, int n, const double* nrm, double* out) {
+    Rng r(seed);
+    const Vec3 nv{nrm[0], nrm[1], nrm[2]};
+    for (int i = 0; i < n; ++i) {
+        out[4 * i] = r.normal();
+        const Vec3 d = r.isotropic_dir();
+        out[4 * i + 1] = d.x; out[4 * i + 2] = d.y; out[4 * i + 3] = d.z;
+    }
+    for (int i = 0; i < n; ++i) {
+        const Vec3 d = r.lambertian_dir(nv);
+        out[4 * n + 3 * i] = d.x; out[4 * n + 3 * i + 1] = d.y; out[4 * n + 3 * i + 2] = d.z;
+    }
+}
+
+// Derived constants of CrystalLattice(Germanium()) (charge tensors, rates, decay
+// angles). The private octahedral LUT is pinned through map_k_to_vg queries instead
+// (ref_map_k_to_vg), and the decay tables through sample_*_energy_fraction draws
+// (ref_decay_fractions).
+void ref_lattice_info(double* out /* 16 */) {
+    const CrystalLattice lat(materials::Germanium());
+    out[0] = lat.electron_mass_eV();
+    out[1] = lat.hole_mass_eV();
+    out[2] = lat.l0_electron();
+    out[3] = lat.l0_hole();
+    out[4] = lat.k_sound_electron();
+    out[5] = lat.k_sound_hole();
+    out[6] = lat.hv_max_scale();
+    out[7] = lat.debye_energy();
+    out[8] = lat.isotope_rate(7.5e-3);
+    out[9] = lat.downconversion_rate(PhononMode::L, 7.5e-3);
+    out[10] = lat.theta_TT(0.5);
+    out[11] = lat.theta_L_to_LT(0.8);
+    out[12] = lat.theta_T_from_LT(0.8);
+    out[13] = static_cast<double>(lat.num_valleys());
+    out[14] = 0.0;
+    out[15] = 0.0;
+}
+
+// Energy fractions drawn by sample_lt / sample_tt_energy_fraction from Rng(seed).
+void ref_decay_fractions(uint64_t seed, int n, double* lt, double* tt) {
+    const CrystalLattice lat(materials::Germanium());
+    Rng r(seed);
+    for (int i = 0; i < n; ++i) { lt[i] = lat.sample_lt_energy_fraction(r); tt[i] = lat.sample_tt_energy_fraction(r); }
+}
+
+// map_k_to_vg (octahedral default) for n directions: out = n*4 (vx, vy, vz, |v|).
+void ref_map_k_to_vg(int mode, int n, const double* k, double* out) {
+    static const CrystalLattice lat(materials::Germanium());
+    for (int i = 0; i < n; ++i) {
+        const auto kin = lat.map_k_to_vg(static_cast<PhononMode>(mode), Vec3{k[3 * i], k[3 * i + 1], k[3 * i + 2]});
+        out[4 * i] = kin.vg_dir.x; out[4 * i + 1] = kin.vg_dir.y; out[4 * i + 2] = kin.vg_dir.z; out[4 * i + 3] = kin.vg_mag;
+    }
+}
+
+// physics::solve_christoffel_exact for n directions: out = n*12 (3 x (vgx, vgy, vgz, vp)).
+void ref_christoffel(int n, const double* k, double* out) {
+    const CrystalConfig cfg = materials::Germanium();
+    const auto V = cfg.build_voigt_matrix();
+    auto vi = [](int i, int j) { if (i == j) return i; if ((i == 1 && j == 2) || (i == 2 && j == 1)) return 3;
+                                 if ((i == 0 && j == 2) || (i == 2 && j == 0)) return 4; return 5; };
+    double C[3][3][3][3];
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) for (int l = 0; l < 3; ++l) for (int m = 0; m < 3; ++m)
+        C[i][j][l][m] = V[vi(i, j)][vi(l, m)];
+    for (int t = 0; t < n; ++t) {
+        Vec3 vg[3]; double vp[3];
+        physics::solve_christoffel_exact(C, cfg.density, Vec3{k[3 * t], k[3 * t + 1], k[3 * t + 2]}, vg, vp);
+        for (int m = 0; m < 3; ++m) {
+            out[12 * t + 4 * m] = vg[m].x; out[12 * t + 4 * m + 1] = vg[m].y;
+            out[12 * t + 4 * m + 2] = vg[m].z; out[12 * t + 4 * m + 3] = vp[m];
+        }
+    }
+}
+
+// n calls of KaplanFilm::absorb_phonon(E) on the default Al film from Rng(seed):
+// edep[i], nrefl[i], refl_sum[i]; the rng state after the calls is written to state_out.
+void ref_absorb_phonon(double E, uint64_t seed, int n, double* edep, int* nrefl, double* refl_sum,
+                       uint64_t* state_out) {
+    const KaplanFilm film = KaplanFilm::aluminum_sensor();
+    Rng r(seed);
+    KaplanBuffer refl;
+    for (int i = 0; i < n; ++i) {
+        edep[i] = film.absorb_phonon(E, refl, r);
+        nrefl[i] = static_cast<int>(refl.size());
+        double s = 0.0;
+        for (double x : refl) s += x;
+        refl_sum[i] = s;
+    }
+    for (int i = 0; i < 4; ++i) state_out[i] = r.s[i];
+}
+
+// intersect_exit for n (p, v) pairs of the default cylinder: out = n*5 (t, nx, ny, nz, id)
+void ref_intersect(int n, const double* p, const double* v, double* out) {
+    const CylinderSolid s;
+    for (int i = 0; i < n; ++i) {
+        const BoundaryHit h = s.intersect_exit(Vec3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}, Vec3{v[3 * i], v[3 * i + 1], v[3 * i + 2]});
+        out[5 * i] = h.distance; out[5 * i + 1] = h.inward_normal.x; out[5 * i + 2] = h.inward_normal.y;
+        out[5 * i + 3] = h.inward_normal.z; out[5 * i + 4] = static_cast<double>(h.surface_id);
+    }
+}
+
+// run_events of a workload (plain reference semantics). stats_out receives the raw
+// 208-byte SimStats; hits (capacity hit_cap) and hit_offsets (n_events + 1) are filled
+// when non-NULL. Returns the number of hits, or -1 on an exception (message copied).
+long ref_run_events(const orc_crystal* crystal, const orc_detector* det, const orc_workload* wl,
+                    uint64_t n_events, uint64_t seed, unsigned threads, uint32_t chunk, int keep_hits,
+                    void* stats_out, orc_hit* hits, long hit_cap, uint64_t* hit_offsets,
+                    double* wall_seconds, char* err, int err_len) {
+    try {
+        const CrystalLattice lattice(to_crystal(crystal));
+        const DetectorModel detector = to_detector(det);
+        const SimulationEngine engine(lattice, detector);
+        const ChargePhysics charges(lattice);
+        const size_t n_valleys = lattice.config().valley_dirs.size();
+        const orc_workload w = *wl;
+        auto pair = [&](EventInput& in, const Vec3& pos) {
+            const auto v = static_cast<uint16_t>(in.rng.uniform() * n_valleys) % n_valleys;
+            in.charges.push_back(charges.make_charge(ExcitationType::Electron, pos, in.rng.isotropic_dir(), 1e-6, v));
+            in.charges.push_back(charges.make_charge(ExcitationType::Hole, pos, in.rng.isotropic_dir(), 1e-6, 0));
+        };
+        auto gen = [&](EventInput& in) {
+            switch (w.kind) {
+                case ORC_WL_PHONON_POINT:
+                    in.phonons.push_back(engine.make_track({w.origin[0], w.origin[1], w.origin[2]}, in.rng.isotropic_dir(),
+                                                           static_cast<PhononMode>(w.mode), w.energy_eV));
+                    in.injected_eV += w.energy_eV;
+                    break;
+                case ORC_WL_MIXED:
+                    if (in.event_id % 2 == 0) {
+                        in.phonons.push_back(engine.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::FT, 7.5e-3));
+                        in.injected_eV += 7.5e-3;
+                    } else {
+                        pair(in, Vec3{0, 0, 0});
+                    }
+                    break;
+                case ORC_WL_PAIR: {
+                    const Vec3 pos = w.pair_at_origin ? Vec3{0, 0, 0} : uniform_volume(detector, in.rng);
+                    pair(in, pos);
+                    break;
+                }
+                case ORC_WL_PHONON_LOGN:
+                case ORC_WL_PHONON_UNIF: {
+                    uint32_t n;
+                    if (w.kind == ORC_WL_PHONON_LOGN) {
+                        const double g = in.rng.normal();
+                        const double v = std::exp(std::log(w.logn_median) + w.logn_sigma * g);
+                        if (!(v < static_cast<double>(w.n_max))) n = w.n_max;
+                        else if (v < static_cast<double>(w.n_min)) n = w.n_min;
+                        else n = std::clamp(static_cast<uint32_t>(std::llround(v)), w.n_min, w.n_max);
+                    } else {
+                        n = w.n_min + static_cast<uint32_t>(in.rng.uniform() * static_cast<double>(w.n_max - w.n_min + 1));
+                        if (n > w.n_max) n = w.n_max;
+                    }
+                    for (uint32_t j = 0; j < n; ++j) {
+                        const Vec3 pos = uniform_volume(detector, in.rng);
+                        PhononMode m = static_cast<PhononMode>(w.mode);
+                        double E = w.energy_eV;
+                        if (w.kind == ORC_WL_PHONON_LOGN) {
+                            m = lattice.sample_dos_mode(in.rng);
+                            E = w.e_lo_eV + (w.e_hi_eV - w.e_lo_eV) * in.rng.uniform();
+                        }
+                        in.phonons.push_back(engine.make_track(pos, in.rng.isotropic_dir(), m, E));
+                        in.injected_eV += E;
+                    }
+                    break;
+                }
+                default:
+                    break;
+            }
+        };
+        RunOptions opt;
+        opt.threads = threads;
+        opt.seed = seed;
+        opt.events_per_chunk = chunk;
+        opt.keep_hits = keep_hits != 0;
+        const RunResult r = run_events(engine, n_events, gen, opt);
+        static_assert(sizeof(SimStats) == sizeof(orc_stats), "SimStats layout");
+        std::memcpy(stats_out, &r.stats, sizeof(SimStats));
+        if (hits) {
+            const long n = std::min<long>(hit_cap, static_cast<long>(r.hits.size()));
+            for (long i = 0; i < n; ++i) to_hit(r.hits[static_cast<size_t>(i)], &hits[i]);
+        }
+        if (hit_offsets)
+            for (size_t i = 0; i < r.hit_offsets.size(); ++i) hit_offsets[i] = r.hit_offsets[i];
+        if (wall_seconds) *wall_seconds = r.wall_seconds;
+        return static_cast<long>(r.hits.size());
+    } catch (const std::exception& ex) {
+        if (err && err_len > 0) { std::strncpy(err, ex.what(), static_cast<size_t>(err_len - 1)); err[err_len - 1] = 0; }
+        return -1;
+    }
+}
+
+// Direct SimulationEngine::transport of one stack (the drop-in's serial semantics):
+// tracks are made by make_track from (pos, k, mode, E) rows; the rng is Rng::for_stream(seed, stream).
+long ref_transport_stack(const orc_detector* det, int n, const double* rows /* n*8: pos3 k3 mode E */,
+                         uint64_t seed, uint64_t stream, void* stats_out, orc_hit* hits, long hit_cap,
+                         uint64_t* rng_state_out) {
+    const CrystalLattice lattice(materials::Germanium());
+    const DetectorModel detector = to_detector(det);
+    const SimulationEngine engine(lattice, detector);
+    std::vector<PhononTrack> stack;
+    for (int i = 0; i < n; ++i) {
+        const double* r = rows + 8 * i;
+        stack.push_back(engine.make_track({r[0], r[1], r[2]}, {r[3], r[4], r[5]}, static_cast<PhononMode>(static_cast<int>(r[6])), r[7]));
+    }
+    std::vector<SensorHit> out;
+    SimStats st;
+    Rng rng = Rng::for_stream(seed, stream);
+    engine.transport(stack, out, st, rng);
+    std::memcpy(stats_out, &st, sizeof(SimStats));
+    const long m = std::min<long>(hit_cap, static_cast<long>(out.size()));
+    for (long i = 0; i < m; ++i) to_hit(out[static_cast<size_t>(i)], &hits[i]);
+    for (int i = 0; i < 4; ++i) rng_state_out[i] = rng.s[i];
+    return static_cast<long>(out.size());
+}
+
+}  // extern "C"
