@@ -1,0 +1,335 @@
+"""cmpmc_amd: MI355X (gfx950) implementation of CMPMC's track-transport path.
+
+Python front end over the C ABI (include/cmpmc_amd.h, library cmpmc_amd/_build/
+libcmpmc_amd.so).  The reference's C++ API has the same shape:
+
+    CrystalLattice(cfg)            -> Lattice(crystal)            (host table build)
+    DetectorModel{}                -> default_detector()
+    SimulationEngine(lat, det)     -> Engine(lattice, detector)   (tables uploaded to HBM)
+    engine.transport(stack, ...)   -> Engine.transport(tracks, rng_state)
+    run_events(engine, n, gen, o)  -> Engine.run_events(batch, ...) (host-generated stacks)
+                                      Engine.run_workload(wl, n, ...) (device generator)
+
+Every compute call goes to the HIP library; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _capi as capi
+from ._capi import (CHARGE_DTYPE, FORM_BATCHED, FORM_PLAIN, HIT_DTYPE, PHONON_DTYPE, WL_PAIR, WL_PHONON_LOGN,
+                    WL_PHONON_POINT, WL_PHONON_UNIF, CmpmcError, check)
+
+__all__ = ["Lattice", "Engine", "Rng", "germanium", "default_detector", "set_bias", "demo_polynomial_surfaces",
+           "workload", "make_tracks", "CmpmcError", "FORM_PLAIN", "FORM_BATCHED", "PHONON_DTYPE", "CHARGE_DTYPE",
+           "HIT_DTYPE", "WL_PHONON_POINT", "WL_PHONON_LOGN", "WL_PAIR", "WL_PHONON_UNIF", "device_info"]
+
+MASK64 = (1 << 64) - 1
+
+
+# ------------------------------------------------------------------ host RNG (core/rng.hpp)
+def _mix64(x: int) -> int:
+    x = (x + 0x9E3779B97F4A7C15) & MASK64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & MASK64
+    return x ^ (x >> 31)
+
+
+class Rng:
+    """xoshiro256** with splitmix64 seeding and Rng::for_stream (rng.hpp:17-97), for
+    host-side generators that hand post-generator stream states to the device."""
+
+    def __init__(self, seed: int = 0x853C49E6748FEA9B):
+        z = seed & MASK64
+        s = []
+        for _ in range(4):
+            z = (z + 0x9E3779B97F4A7C15) & MASK64
+            x = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+            x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & MASK64
+            s.append(x ^ (x >> 31))
+        self.s = s
+
+    @classmethod
+    def for_stream(cls, seed: int, stream: int) -> "Rng":
+        return cls(_mix64(seed & MASK64) ^ _mix64((stream + 0x632BE59BD9B4E019) & MASK64))
+
+    def next_u64(self) -> int:
+        s = self.s
+        r = ((((s[1] * 5) & MASK64) << 7 | ((s[1] * 5) & MASK64) >> 57) & MASK64) * 9 & MASK64
+        t = (s[1] << 17) & MASK64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = ((s[3] << 45) | (s[3] >> 19)) & MASK64
+        return r
+
+    def uniform(self) -> float:
+        return float((self.next_u64() >> 11) | 1) * 2.0 ** -53
+
+    def isotropic_dir(self) -> tuple:
+        ct = 2.0 * self.uniform() - 1.0
+        st = max(0.0, 1.0 - ct * ct) ** 0.5
+        phi = 2.0 * 3.141592653589793 * self.uniform()
+        return (st * np.cos(phi), st * np.sin(phi), ct)
+
+    def state(self) -> np.ndarray:
+        return np.array(self.s, dtype=np.uint64)
+
+
+# ------------------------------------------------------------------ configuration
+def device_info() -> str:
+    buf = C.create_string_buffer(256)
+    check(capi.load().cmpmc_device_info(buf, 256))
+    return buf.value.decode()
+
+
+def germanium() -> capi.Crystal:
+    c = capi.Crystal()
+    check(capi.load().cmpmc_crystal_germanium(C.byref(c)))
+    return c
+
+
+def default_detector() -> capi.Detector:
+    d = capi.Detector()
+    check(capi.load().cmpmc_detector_default(C.byref(d)))
+    return d
+
+
+def set_bias(det: capi.Detector, volts: float, thickness_m: float = 2.54e-2) -> None:
+    """UniformElectricField::from_bias(V, d) = (0, 0, -V/d) (charge_transport.hpp:22-24)."""
+    det.efield[0], det.efield[1], det.efield[2] = 0.0, 0.0, -volts / thickness_m
+
+
+def demo_polynomial_surfaces(det: capi.Detector, max_bounces: int = 100) -> None:
+    """The reference benchmark's surfaces (cmpmc_bench.cpp:250-258; the demonstration
+    polynomials of surface_property.hpp:254-272)."""
+    spec = [0.928, -2.03e-4, -3.21e-6, 3.1e-9, 2.9e-13]
+    diff = [5.88e-2, 7.83e-4, -2.47e-6, 1.71e-8, -2.98e-11]
+    anh = [0.0, 0.0, 0.0, 0.0, 0.0, 1.51e-14]
+    for s in det.surf:
+        s.n_spec, s.n_diff, s.n_anh = len(spec), len(diff), len(anh)
+        for i, v in enumerate(spec):
+            s.spec[i] = v
+        for i, v in enumerate(diff):
+            s.diff[i] = v
+        for i, v in enumerate(anh):
+            s.anh[i] = v
+        s.anh_cutoff, s.refl_cutoff = 520.0, 350.0
+    det.max_bounces = max_bounces
+
+
+def workload(kind: int, mode: int = 0, energy_eV: float = 7.5e-3, origin=(0.0, 0.0, 0.0), logn_median: float = 200.0,
+             logn_sigma: float = 1.0, n_min: int = 1, n_max: int = 20000, e_lo_eV: float = 1e-3, e_hi_eV: float = 31e-3,
+             pair_at_origin: bool = False) -> capi.Workload:
+    w = capi.Workload()
+    w.kind, w.mode, w.energy_eV = kind, mode, energy_eV
+    for i in range(3):
+        w.origin[i] = origin[i]
+    w.logn_median, w.logn_sigma, w.n_min, w.n_max = logn_median, logn_sigma, n_min, n_max
+    w.e_lo_eV, w.e_hi_eV, w.pair_at_origin = e_lo_eV, e_hi_eV, int(pair_at_origin)
+    return w
+
+
+class Lattice:
+    """CrystalLattice(cfg): builds the octahedral v_g table and the decay inverse CDFs
+    on the host (crystal_lattice.hpp:45-55)."""
+
+    def __init__(self, crystal: capi.Crystal | None = None):
+        self.crystal = crystal if crystal is not None else germanium()
+        self.oct = np.zeros((3, 256, 256, 4), dtype=np.float32)
+        self.cdf_lt = np.zeros(1024)
+        self.cdf_tt = np.zeros(1024)
+        self.c = capi.Lattice()
+        check(capi.load().cmpmc_lattice_build(C.byref(self.crystal), self.oct.ctypes.data, self.cdf_lt.ctypes.data,
+                                               self.cdf_tt.ctypes.data, C.byref(self.c)))
+
+    def map_k_to_vg(self, mode: int, k) -> tuple[np.ndarray, np.ndarray]:
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+        d = np.empty_like(k)
+        m = np.empty(k.shape[0])
+        check(capi.load().cmpmc_map_k_to_vg(C.byref(self.c), mode, k.shape[0], k.ctypes.data, d.ctypes.data,
+                                             m.ctypes.data))
+        return d, m
+
+
+def make_tracks(lattice: Lattice, pos, k, mode, energy, time_s=0.0, weight=1.0) -> np.ndarray:
+    """SimulationEngine::make_track for arrays of rows (engine.hpp:171-187)."""
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    k = np.asarray(k, dtype=np.float64).reshape(-1, 3)
+    n = pos.shape[0]
+    mode = np.broadcast_to(np.asarray(mode, dtype=np.int64), (n,))
+    nrm2 = (k[:, 0] * k[:, 0] + k[:, 1] * k[:, 1]) + k[:, 2] * k[:, 2]
+    inv = np.where(nrm2 > 0, 1.0 / np.sqrt(np.where(nrm2 > 0, nrm2, 1.0)), 0.0)
+    ku = np.where(nrm2[:, None] > 0, k * inv[:, None], np.array([0.0, 0.0, 1.0]))
+    t = np.zeros(n, dtype=PHONON_DTYPE)
+    t["pos"] = pos
+    t["k_dir"] = ku
+    for m in (0, 1, 2):
+        sel = np.nonzero(mode == m)[0]
+        if sel.size:
+            d, mag = lattice.map_k_to_vg(m, ku[sel])
+            t["vg_dir"][sel] = d
+            t["vg_mag"][sel] = mag
+    t["energy"] = energy
+    t["time"] = time_s
+    t["weight"] = weight
+    t["mode"] = mode
+    return t
+
+
+class Engine:
+    """SimulationEngine(lattice, detector) on the current GPU (engine.hpp:158-169)."""
+
+    def __init__(self, lattice: Lattice | None = None, detector: capi.Detector | None = None):
+        self.lattice = lattice if lattice is not None else Lattice()
+        self.detector = detector if detector is not None else default_detector()
+        p = C.c_void_p()
+        check(capi.load().cmpmc_model_create(C.byref(self.lattice.c), C.byref(self.detector), C.byref(p)))
+        self.ptr = p
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            capi.load().cmpmc_model_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cancel(self, value: bool = True) -> None:
+        check(capi.load().cmpmc_model_cancel(self.ptr, int(value)))
+
+    def trim(self) -> None:
+        check(capi.load().cmpmc_model_trim(self.ptr))
+
+    @staticmethod
+    def _opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
+              poll_cancel) -> capi.RunOpts:
+        o = capi.RunOpts()
+        o.form, o.keep_hits, o.want_first_branch, o.block_size = form, int(keep_hits), int(first_branch), block
+        o.seed, o.stack_headroom, o.lane_stack_depth = seed, headroom, lane_depth
+        o.grid_blocks, o.refill_period, o.poll_cancel = grid_blocks, refill_period, int(poll_cancel)
+        return o
+
+    def _run(self, call, n_events: int, keep_hits: bool, hit_capacity: int | None, first_branch: bool) -> dict:
+        cap = hit_capacity if hit_capacity is not None else (max(1024, 64 * n_events) if keep_hits else 0)
+        for _ in range(3):
+            out = capi.RunOut()
+            hits = np.zeros(max(cap, 1), dtype=HIT_DTYPE)
+            offs = np.zeros(n_events + 1, dtype=np.uint64)
+            fb = np.zeros(max(n_events, 1), dtype=np.int32)
+            out.hits = hits.ctypes.data if keep_hits else None
+            out.hit_capacity = cap if keep_hits else 0
+            out.hit_offsets = offs.ctypes.data_as(C.POINTER(capi.u64))
+            out.first_branch = fb.ctypes.data_as(C.POINTER(capi.i32)) if first_branch else None
+            t0 = time.perf_counter()
+            rc = call(out)
+            wall = time.perf_counter() - t0
+            if rc == capi.ERR_CAPACITY and keep_hits and out.hits_needed > cap:
+                cap = int(out.hits_needed)      # deterministic: rerun with room for every hit
+                continue
+            check(rc)
+            res = {"stats": out.stats.as_dict(), "stats_raw": bytes(out.stats), "hits": hits[:out.n_hits].copy(),
+                   "hit_offsets": offs, "kernel_ms": out.kernel_ms, "wall_s": wall}
+            if first_branch:
+                res["first_branch"] = fb[:n_events].copy()
+            return res
+        raise CmpmcError(capi.ERR_CAPACITY, "hit capacity retries exhausted")
+
+    def run_workload(self, wl: capi.Workload, n_events: int, seed: int = 1, form: int = FORM_PLAIN,
+                     keep_hits: bool = True, hit_capacity: int | None = None, first_branch: bool = False,
+                     block: int = 0, headroom: int = 0, lane_depth: int = 0, grid_blocks: int = 0,
+                     refill_period: int = 0, poll_cancel: bool = False, stream=None) -> dict:
+        """run_events with the generator on the device (parallel.hpp:76-148)."""
+        o = self._opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
+                       poll_cancel)
+        L = capi.load()
+        return self._run(lambda out: L.cmpmc_run_workload(self.ptr, C.byref(wl), n_events, C.byref(o), C.byref(out),
+                                                          stream),
+                         n_events, keep_hits, hit_capacity, first_branch)
+
+    def run_events(self, tracks: np.ndarray, track_offsets: np.ndarray, rng_state: np.ndarray,
+                   injected: np.ndarray | None = None, charges: np.ndarray | None = None,
+                   charge_offsets: np.ndarray | None = None, seed: int = 1, form: int = FORM_PLAIN,
+                   keep_hits: bool = True, hit_capacity: int | None = None, first_branch: bool = False,
+                   first_event_id: int = 0, **kw) -> dict:
+        """run_events for stacks produced by a host generator: tracks of event i are rows
+        track_offsets[i]:track_offsets[i+1]; rng_state[i] is its post-generator stream."""
+        n = len(track_offsets) - 1
+        tracks = np.ascontiguousarray(tracks, dtype=PHONON_DTYPE)
+        toff = np.ascontiguousarray(track_offsets, dtype=np.uint64)
+        rng = np.ascontiguousarray(rng_state, dtype=np.uint64).reshape(n, 4)
+        b = capi.EventBatch()
+        b.n_events = n
+        b.tracks = tracks.ctypes.data if tracks.size else None
+        b.track_offsets = toff.ctypes.data_as(C.POINTER(capi.u64))
+        keep = [tracks, toff, rng]
+        if charges is not None:
+            charges = np.ascontiguousarray(charges, dtype=CHARGE_DTYPE)
+            coff = np.ascontiguousarray(charge_offsets, dtype=np.uint64)
+            b.charges = charges.ctypes.data
+            b.charge_offsets = coff.ctypes.data_as(C.POINTER(capi.u64))
+            keep += [charges, coff]
+        b.rng_state = rng.ctypes.data_as(C.POINTER(capi.u64))
+        if injected is not None:
+            inj = np.ascontiguousarray(injected, dtype=np.float64)
+            b.injected_eV = inj.ctypes.data_as(C.POINTER(capi.dbl))
+            keep.append(inj)
+        b.first_event_id = first_event_id
+        o = self._opts(form, seed, keep_hits, first_branch, kw.get("block", 0), kw.get("headroom", 0),
+                       kw.get("lane_depth", 0), kw.get("grid_blocks", 0), kw.get("refill_period", 0),
+                       kw.get("poll_cancel", False))
+        L = capi.load()
+        return self._run(lambda out: L.cmpmc_run_events(self.ptr, C.byref(b), C.byref(o), C.byref(out), None),
+                         n, keep_hits, hit_capacity, first_branch)
+
+    def transport(self, tracks: np.ndarray, rng_state: np.ndarray, hit_capacity: int = 1 << 16) -> dict:
+        """SimulationEngine::transport(stack, hits, stats, rng) (engine.hpp:403-582)."""
+        tracks = np.ascontiguousarray(tracks, dtype=PHONON_DTYPE)
+        rng = np.ascontiguousarray(rng_state, dtype=np.uint64).copy()
+        L = capi.load()
+        while True:
+            hits = np.zeros(hit_capacity, dtype=HIT_DTYPE)
+            nh = capi.u64(0)
+            st = capi.Stats()
+            r = rng.copy()
+            rc = L.cmpmc_transport_serial(self.ptr, tracks.ctypes.data, len(tracks),
+                                          r.ctypes.data_as(C.POINTER(capi.u64)), hits.ctypes.data, hit_capacity,
+                                          C.byref(nh), C.byref(st), None)
+            if rc == capi.ERR_CAPACITY and hit_capacity < (1 << 26):
+                hit_capacity *= 8
+                continue
+            check(rc)
+            return {"stats": st.as_dict(), "stats_raw": bytes(st), "hits": hits[:nh.value].copy(), "rng": r}
+
+    def transport_charges(self, charges: np.ndarray, rng_state: np.ndarray, phonon_capacity: int = 1 << 16,
+                          hit_capacity: int = 1 << 12) -> dict:
+        """SimulationEngine::transport_charges (engine.hpp:192-380): returns the hits and the
+        Luke phonons it pushed (to be transported by a following transport call)."""
+        charges = np.ascontiguousarray(charges, dtype=CHARGE_DTYPE)
+        rng = np.ascontiguousarray(rng_state, dtype=np.uint64).copy()
+        L = capi.load()
+        while True:
+            ph = np.zeros(phonon_capacity, dtype=PHONON_DTYPE)
+            hits = np.zeros(hit_capacity, dtype=HIT_DTYPE)
+            nph, nh = capi.u64(0), capi.u64(0)
+            st = capi.Stats()
+            r = rng.copy()
+            rc = L.cmpmc_transport_charges_serial(self.ptr, charges.ctypes.data, len(charges),
+                                                  r.ctypes.data_as(C.POINTER(capi.u64)), ph.ctypes.data,
+                                                  phonon_capacity, C.byref(nph), hits.ctypes.data, hit_capacity,
+                                                  C.byref(nh), C.byref(st), None)
+            if rc == capi.ERR_CAPACITY and phonon_capacity < (1 << 24):
+                phonon_capacity *= 8
+                hit_capacity *= 8
+                continue
+            check(rc)
+            return {"stats": st.as_dict(), "stats_raw": bytes(st), "hits": hits[:nh.value].copy(),
+                    "phonons": ph[:nph.value].copy(), "rng": r}

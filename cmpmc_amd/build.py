@@ -1,0 +1,68 @@
+"""Builds libcmpmc_amd.so (gfx950) in-tree: cmpmc_amd/_build/libcmpmc_amd.so.
+
+Every HIP translation unit is compiled for gfx950 with packed FP32 instructions
+disabled. The plain-form kernel file is compiled without floating-point contraction
+(FP64 operation order of the reference); the batched kernels allow contraction. The
+host table builder is compiled by g++ without contraction.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+LIB = os.path.join(OUT, "libcmpmc_amd.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("CMPMC_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"), "-I", SRC,
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+UNITS = [
+    ("transport_plain.hip", ["-ffp-contract=off"]),
+    ("transport_batched.hip", ["-ffp-contract=fast"]),
+    ("capi.hip", []),
+]
+HOST_UNITS = [("lattice_build.cpp", ["-ffp-contract=off"])]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r.stdout + r.stderr
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OUT, exist_ok=True)
+    jobs = []
+    objs = []
+    for src, extra in UNITS:
+        obj = os.path.join(OUT, src.replace(".hip", ".o"))
+        objs.append(obj)
+        jobs.append([HIPCC, "-c", "-x", "hip", os.path.join(SRC, src), "-o", obj] + HIP_FLAGS + extra)
+    for src, extra in HOST_UNITS:
+        obj = os.path.join(OUT, src.replace(".cpp", ".host.o"))
+        objs.append(obj)
+        jobs.append(["g++", "-c", os.path.join(SRC, src), "-o", obj, "-O2", "-std=c++17", "-fPIC",
+                     "-I", os.path.join(ROOT, "include"), "-I", SRC, "-Wall"] + extra)
+    with ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+        logs = list(ex.map(_run, jobs))
+    tmp = LIB + ".tmp"
+    logs.append(_run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-fPIC"]))
+    os.replace(tmp, LIB)
+    if verbose:
+        print("\n".join(l for l in logs if l.strip()))
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,1090 @@
+// C ABI implementation: model upload, run orchestration (plain and batched forms),
+// hit ordering and statistics assembly.  Host code of the library; the kernels live in
+// transport_plain.hip and transport_batched.hip.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "capi_internal.h"
+#include "cmpmc_amd.h"
+#include "kernels_common.hpp"
+#include "transport_kernels.h"
+
+using namespace cmpmc_dev;
+
+// ------------------------------------------------------------------ errors
+namespace cmpmc_internal {
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+void clear_error() { g_err.clear(); }
+}  // namespace cmpmc_internal
+
+using cmpmc_internal::fail;
+
+#define HIPCHK(expr)                                                                                  \
+    do {                                                                                              \
+        const hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return fail(CMPMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+#define CHK(expr)                         \
+    do {                                  \
+        const int rc_ = (expr);           \
+        if (rc_ != CMPMC_OK) return rc_;  \
+    } while (0)
+
+extern "C" const char* cmpmc_last_error(void) { return cmpmc_internal::g_err.c_str(); }
+
+extern "C" int cmpmc_device_info(char* buf, size_t len) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CMPMC_ERR_NO_DEVICE, "no GPU visible");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, dev));
+    if (buf && len) std::snprintf(buf, len, "%s, %d CUs, %.1f GiB, clock %d kHz", p.gcnArchName, p.multiProcessorCount,
+                                  p.totalGlobalMem / 1073741824.0, p.clockRate);
+    return CMPMC_OK;
+}
+
+extern "C" size_t cmpmc_abi_sizeof(int which) {
+    switch (which) {
+        case 0: return sizeof(cmpmc_crystal);
+        case 1: return sizeof(cmpmc_film);
+        case 2: return sizeof(cmpmc_surface);
+        case 3: return sizeof(cmpmc_detector);
+        case 4: return sizeof(cmpmc_lattice);
+        case 5: return sizeof(cmpmc_phonon_track);
+        case 6: return sizeof(cmpmc_charge_track);
+        case 7: return sizeof(cmpmc_sensor_hit);
+        case 8: return sizeof(cmpmc_sim_stats);
+        case 9: return sizeof(cmpmc_event_batch);
+        case 10: return sizeof(cmpmc_workload);
+        case 11: return sizeof(cmpmc_run_opts);
+        case 12: return sizeof(cmpmc_run_out);
+        default: return 0;
+    }
+}
+
+// ------------------------------------------------------------------ presets
+extern "C" int cmpmc_crystal_germanium(cmpmc_crystal* c) {
+    if (!c) return fail(CMPMC_ERR_INVALID, "null crystal");
+    std::memset(c, 0, sizeof(*c));
+    c->density = 5323.0;
+    c->C11 = 126.0 * 1e9; c->C12 = 44.0 * 1e9; c->C44 = 67.0 * 1e9;   // units::GPa products
+    c->beta = -73.2 * 1e9; c->gamma = -70.8 * 1e9; c->lambda = 37.6 * 1e9; c->mu = 56.1 * 1e9;
+    c->B_iso = 3.67e-41; c->A_anh = 6.43e-55; c->fTT_frac = 0.74; c->polycrystal_mfp_m = 0.0;
+    c->fLDOS = 0.097834; c->fSTDOS = 0.535390; c->fFTDOS = 0.366770;
+    c->debye_freq = 8.64 * 1e12; c->v_sound = 5324.2077; c->v_trans = 3258.7879;
+    c->bandgap = 0.74; c->h_mass_ratio = 0.350;
+    c->e_mass[0] = 1.588; c->e_mass[1] = 0.081; c->e_mass[2] = 0.081;
+    c->n_valleys = 4;
+    c->iv_model = 0;
+    const double vd[4][3] = {{1, 1, 1}, {-1, 1, 1}, {-1, -1, 1}, {1, -1, 1}};
+    std::memcpy(c->valley_dirs, vd, sizeof(vd));
+    c->ac_deform_e = 11.0; c->ac_deform_h = 3.4;
+    c->iv_min_energy = 10.3 * 1e-3;
+    c->iv_quad_rate = 6.72e-2; c->iv_quad_field = 217.0; c->iv_quad_power = 3.24;
+    c->iv_lin_rate0 = 1.5e6; c->iv_lin_rate1 = 1.5; c->iv_lin_power = 4.0;
+    for (int i = 0; i < 3; ++i) c->orient[i][i] = 1.0;
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_film_aluminum(cmpmc_film* f) {
+    if (!f) return fail(CMPMC_ERR_INVALID, "null film");
+    std::memset(f, 0, sizeof(*f));
+    f->film_absorption = 0.20; f->film_thickness = 600.0 * 1e-9; f->v_sound = 3260.0; f->gap_energy = 173.715 * 1e-6;
+    f->phonon_lifetime = 242.0 * 1e-12; f->phonon_lifetime_slope = 0.29; f->low_qp_limit = 3.0; f->high_qp_limit = 0.0;
+    f->subgap_absorption = 0.10; f->absorber_gap = 0.0; f->absorber_eff = 1.0; f->absorber_eff_slope = 0.0;
+    f->temperature = 0.05e-3; f->keep_all_phonons = 1; f->loss_model = 0;
+    return CMPMC_OK;
+}
+
+static void bare_surface(cmpmc_surface& s) {
+    std::memset(&s, 0, sizeof(s));
+    s.prob_abs = 0.0; s.prob_refl = 1.0; s.refl_cutoff = 350.0; s.anh_cutoff = 520.0;
+    s.prob_charge_abs = 1.0; s.prob_charge_spec = 0.0;
+    s.n_spec = 1; s.spec[0] = 0.0;
+    s.n_diff = 1; s.diff[0] = 1.0;
+    s.n_anh = 0;
+}
+
+extern "C" int cmpmc_detector_default(cmpmc_detector* d) {
+    if (!d) return fail(CMPMC_ERR_INVALID, "null detector");
+    std::memset(d, 0, sizeof(*d));
+    d->radius = 3.81 * 1e-2; d->half_z = 1.27 * 1e-2;
+    bare_surface(d->surf[0]);
+    d->surf[0].prob_abs = 1.0;
+    d->surf[0].has_film = 1;
+    cmpmc_film_aluminum(&d->surf[0].film);
+    d->surf[1] = d->surf[0];
+    bare_surface(d->surf[2]);
+    d->max_bounces = 5000; d->russian_roulette_bounces = 0;
+    d->min_phonon_eV = 10.0 * 1e-6; d->luke_sample_fraction = 1.0;
+    d->charge_max_dk_frac = 0.05; d->charge_dt_cap_s = 100.0 * 1e-12;
+    d->fermi_level_above_Ev_eV = -1.0; d->russian_roulette_survival = 0.5;
+    return CMPMC_OK;
+}
+
+// ------------------------------------------------------------------ model
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+};
+
+int ensure(DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return CMPMC_OK;
+    b.release();
+    const hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return fail(CMPMC_ERR_HIP, "hipMalloc(%zu B): %s", bytes, hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return CMPMC_OK;
+}
+
+template <class T> T* as(DevBuf& b) { return static_cast<T*>(b.p); }
+
+}  // namespace
+
+struct cmpmc_model {
+    int device = 0;
+    int n_cu = 0;
+    DevModel dm{};
+    cmpmc_detector det{};
+    cmpmc_crystal cfg{};
+    float4* d_oct = nullptr;
+    double* d_cdf = nullptr;
+    volatile int32_t* h_cancel = nullptr;   // host-mapped
+    int32_t* d_cancel = nullptr;
+    std::mutex mu;
+    // cached run buffers
+    DevBuf ws, pst, cst, poff, coff, pinit, cinit, rng, inj, tin, cin, tin_off, cin_off, hits, hits_tmp, keys_a, keys_b,
+        idx_a, idx_b, sorted, counts, fb, acc, queue, cqueue, lstack, clstack, qoff, wl, cub_tmp;
+};
+
+static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc_detector* det) {
+    DevModel& m = M->dm;
+    const cmpmc_crystal& c = lat->cfg;
+    m.oct = M->d_oct;
+    m.cdf_lt = M->d_cdf;
+    m.cdf_tt = M->d_cdf + CMPMC_N_CDF;
+    std::memcpy(m.orient, c.orient, sizeof(m.orient));
+    m.B_iso = c.B_iso; m.A_anh = c.A_anh; m.fTT = c.fTT_frac; m.poly_mfp = c.polycrystal_mfp_m;
+    m.fL = c.fLDOS; m.fST = c.fSTDOS; m.fFT = c.fFTDOS; m.v_sound = c.v_sound; m.v_trans = c.v_trans;
+    const double h = kHPlanck;
+    m.c_iso_f = static_cast<float>(c.B_iso / (h * h * h * h));
+    m.c_anh_f = static_cast<float>(c.A_anh / (h * h * h * h * h));
+    m.bandgap = c.bandgap; m.e_mass_eV = lat->e_cond_mass_eV; m.h_mass_eV = lat->h_mass_eV;
+    for (int i = 0; i < 3; ++i) {
+        m.mass_ratio[i] = lat->mass_ratio[i];
+        m.hv_sqrt[i] = lat->hv_sqrt[i];
+        m.hv_sqrt_inv[i] = lat->hv_sqrt_inv[i];
+    }
+    std::memcpy(m.valley_rot, lat->valley_rot, sizeof(m.valley_rot));
+    m.l0_e = lat->l0_e; m.l0_h = lat->l0_h; m.k_sound_e = lat->k_sound_e; m.k_sound_h = lat->k_sound_h;
+    m.hv_max_scale = std::max({lat->hv_sqrt_inv[0], lat->hv_sqrt_inv[1], lat->hv_sqrt_inv[2]});
+    m.n_valleys = c.n_valleys; m.iv_model = c.iv_model;
+    m.iv_min_energy = c.iv_min_energy; m.iv_quad_rate = c.iv_quad_rate; m.iv_quad_field = c.iv_quad_field;
+    m.iv_quad_power = c.iv_quad_power; m.iv_lin_rate0 = c.iv_lin_rate0; m.iv_lin_rate1 = c.iv_lin_rate1;
+    m.iv_lin_power = c.iv_lin_power;
+    m.trap_mfp[0] = c.e_trapping_mfp_m; m.trap_mfp[1] = c.h_trapping_mfp_m;
+    m.ion_e_mfp[0] = c.e_trap_ion_d0_mfp_m; m.ion_e_mfp[1] = c.h_trap_ion_d_mfp_m;
+    m.ion_h_mfp[0] = c.e_trap_ion_a_mfp_m; m.ion_h_mfp[1] = c.h_trap_ion_a0_mfp_m;
+    m.radius = det->radius; m.half_z = det->half_z;
+    for (int i = 0; i < 3; ++i) m.efield[i] = det->efield[i];
+    for (int s = 0; s < 3; ++s) {
+        const cmpmc_surface& a = det->surf[s];
+        DevSurface& b = m.surf[s];
+        b.prob_abs = a.prob_abs; b.prob_refl = a.prob_refl; b.refl_cutoff = a.refl_cutoff; b.anh_cutoff = a.anh_cutoff;
+        b.prob_charge_abs = a.prob_charge_abs; b.prob_charge_spec = a.prob_charge_spec;
+        b.min_k_elec = a.min_k_elec; b.min_k_hole = a.min_k_hole;
+        b.n_spec = a.n_spec; b.n_diff = a.n_diff; b.n_anh = a.n_anh; b.has_film = a.has_film;
+        std::memcpy(b.spec, a.spec, sizeof(b.spec));
+        std::memcpy(b.diff, a.diff, sizeof(b.diff));
+        std::memcpy(b.anh, a.anh, sizeof(b.anh));
+        b.spec_prob_const = a.spec_prob_const;
+        const cmpmc_film& f = a.film;
+        b.film = DevFilm{f.film_absorption, f.film_thickness, f.v_sound, f.gap_energy, f.phonon_lifetime,
+                         f.phonon_lifetime_slope, f.low_qp_limit, f.high_qp_limit, f.subgap_absorption, f.absorber_gap,
+                         f.absorber_eff, f.absorber_eff_slope, f.temperature, f.keep_all_phonons, 0};
+    }
+    m.max_bounces = det->max_bounces; m.rr_bounces = det->russian_roulette_bounces;
+    m.min_phonon_eV = det->min_phonon_eV; m.luke_fraction = det->luke_sample_fraction;
+    m.dk_frac = det->charge_max_dk_frac; m.dt_cap = det->charge_dt_cap_s;
+    m.fermi = det->fermi_level_above_Ev_eV; m.rr_survival = det->russian_roulette_survival;
+}
+
+static int check_detector(const cmpmc_detector* d) {
+    if (!(d->radius > 0.0) || !(d->half_z > 0.0)) return fail(CMPMC_ERR_INVALID, "detector: radius and half_z must be positive");
+    for (int s = 0; s < 3; ++s) {
+        const cmpmc_surface& a = d->surf[s];
+        if (a.n_spec < 0 || a.n_spec > CMPMC_MAX_POLY || a.n_diff < 0 || a.n_diff > CMPMC_MAX_POLY || a.n_anh < 0 ||
+            a.n_anh > CMPMC_MAX_POLY)
+            return fail(CMPMC_ERR_INVALID, "surface %d: polynomial length must be in [0, %d]", s, CMPMC_MAX_POLY);
+        if (a.has_film && a.film.loss_model != 0)
+            return fail(CMPMC_ERR_INVALID, "surface %d: only the MeanFreePath film-loss model is supported", s);
+    }
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector* det, cmpmc_model** out) {
+    if (!lat || !det || !out || !lat->oct_lut || !lat->cdf_lt || !lat->cdf_tt)
+        return fail(CMPMC_ERR_INVALID, "cmpmc_model_create: null argument");
+    *out = nullptr;
+    CHK(check_detector(det));
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CMPMC_ERR_NO_DEVICE, "cmpmc_model_create: no GPU visible");
+    cmpmc_model* M = new cmpmc_model();
+    HIPCHK(hipGetDevice(&M->device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, M->device));
+    M->n_cu = prop.multiProcessorCount;
+    const size_t oct_bytes = sizeof(float) * 3 * CMPMC_N_OCT * CMPMC_N_OCT * 4;
+    if (hipMalloc(&M->d_oct, oct_bytes) != hipSuccess || hipMalloc(&M->d_cdf, 2 * CMPMC_N_CDF * sizeof(double)) != hipSuccess) {
+        delete M;
+        return fail(CMPMC_ERR_HIP, "cmpmc_model_create: table allocation failed");
+    }
+    HIPCHK(hipMemcpy(M->d_oct, lat->oct_lut, oct_bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(M->d_cdf, lat->cdf_lt, CMPMC_N_CDF * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(M->d_cdf + CMPMC_N_CDF, lat->cdf_tt, CMPMC_N_CDF * sizeof(double), hipMemcpyHostToDevice));
+    void* hc = nullptr;
+    HIPCHK(hipHostMalloc(&hc, sizeof(int32_t), hipHostMallocMapped));
+    M->h_cancel = static_cast<volatile int32_t*>(hc);
+    *M->h_cancel = 0;
+    HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&M->d_cancel), hc, 0));
+    M->det = *det;
+    M->cfg = lat->cfg;
+    fill_dev_model(M, lat, det);
+    *out = M;
+    return CMPMC_OK;
+}
+
+static void trim(cmpmc_model* M) {
+    for (DevBuf* b : {&M->ws, &M->pst, &M->cst, &M->poff, &M->coff, &M->pinit, &M->cinit, &M->rng, &M->inj, &M->tin,
+                      &M->cin, &M->tin_off, &M->cin_off, &M->hits, &M->hits_tmp, &M->keys_a, &M->keys_b, &M->idx_a,
+                      &M->idx_b, &M->sorted, &M->counts, &M->fb, &M->acc, &M->queue, &M->cqueue, &M->lstack,
+                      &M->clstack, &M->qoff, &M->wl, &M->cub_tmp})
+        b->release();
+}
+
+extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
+    if (!M) return CMPMC_OK;
+    trim(M);
+    if (M->d_oct) (void)hipFree(M->d_oct);
+    if (M->d_cdf) (void)hipFree(M->d_cdf);
+    if (M->h_cancel) (void)hipHostFree(const_cast<int32_t*>(M->h_cancel));
+    delete M;
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_model_cancel(cmpmc_model* M, int value) {
+    if (!M) return fail(CMPMC_ERR_INVALID, "null model");
+    *M->h_cancel = value ? 1 : 0;
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_model_trim(cmpmc_model* M) {
+    if (!M) return fail(CMPMC_ERR_INVALID, "null model");
+    std::lock_guard<std::mutex> lk(M->mu);
+    trim(M);
+    return CMPMC_OK;
+}
+
+// ------------------------------------------------------------------ accumulators
+namespace {
+
+// Device accumulator block: counters, ledger (hi, lo) pairs, status words, hit counter,
+// Luke counter, queue head.
+struct Acc {
+    unsigned long long counters[kNumCounters];
+    unsigned long long ledger[2 * L_N];
+    unsigned long long status[4];
+    unsigned long long hit_count;
+    unsigned long long luke_count;
+    unsigned long long head;
+    unsigned long long chead;
+};
+
+struct Totals {
+    unsigned long long counters[kNumCounters] = {};
+    unsigned __int128 ledger[L_N] = {};
+};
+
+void add_acc(Totals& t, const Acc& a) {
+    for (int i = 0; i < kNumCounters; ++i) t.counters[i] += a.counters[i];
+    for (int i = 0; i < L_N; ++i)
+        t.ledger[i] += (static_cast<unsigned __int128>(a.ledger[2 * i]) << 20) + a.ledger[2 * i + 1];
+}
+
+double ledger_eV(unsigned __int128 v) { return static_cast<double>(static_cast<long double>(v) / 17592186044416.0L); }
+
+void apply_totals(const Totals& t, cmpmc_sim_stats& s) {
+    const unsigned long long* c = t.counters;
+    s.total_steps += c[C_STEPS];
+    s.isotope_scatters += c[C_ISO];
+    s.polycrystal_scatters += c[C_POLY];
+    s.decays_L_to_TT += c[C_TT];
+    s.decays_L_to_LT += c[C_LT];
+    s.surface_refls += c[C_REFL];
+    s.surface_downconversions += c[C_SURF_ANH];
+    s.sensor_hits += c[C_HITS];
+    s.killed_tracks += c[C_KILLED];
+    const unsigned long long* q = c + C_NCOUNTERS;
+    s.charge_steps += q[CC_STEPS];
+    s.luke_phonons_emitted += q[CC_LUKE];
+    s.intervalley_scatters += q[CC_IV];
+    s.charges_trapped += q[CC_TRAPPED];
+    s.trap_ionizations += q[CC_TRAP_ION];
+    s.charge_surface_refls += q[CC_SURF_REFL];
+    s.charges_collected += q[CC_COLLECTED];
+    s.total_luke_phonon_eV += ledger_eV(t.ledger[L_LUKE]);
+    s.charge_recomb_edep_eV += ledger_eV(t.ledger[L_RECOMB]);
+    s.total_injected_eV += ledger_eV(t.ledger[L_INJ]);
+    s.total_deposited_eV += ledger_eV(t.ledger[L_DEP]);
+    s.escaped_or_killed_eV += ledger_eV(t.ledger[L_ESC]);
+}
+
+unsigned long long fixed_point(double eV) { return static_cast<unsigned long long>(std::llrint(eV * 17592186044416.0)); }
+
+// ------------------------------------------------------------------ hit ordering kernels
+__global__ void iota_kernel(uint32_t* v, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = static_cast<uint32_t>(i);
+}
+__global__ void gather_key_kernel(const DevHit* h, const uint32_t* idx, uint64_t n, int field, uint64_t* out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevHit& d = h[idx ? idx[i] : i];
+    out[i] = field == 0 ? d.k0 : (field == 1 ? d.k1 : static_cast<uint64_t>(d.k2));
+}
+__global__ void gather_hits_kernel(const DevHit* h, const uint32_t* idx, uint64_t n, int plain, uint64_t first_event,
+                                   uint64_t n_events, cmpmc_sensor_hit* out, uint32_t* counts) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const DevHit& d = h[idx ? idx[i] : i];
+    out[i] = d.h;
+    const uint64_t ev = plain ? (d.k0 >> 32) : d.k0;
+    const uint64_t e = ev - first_event;
+    if (e < n_events) atomicAdd(&counts[e], 1u);
+}
+
+inline unsigned nblk(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+// Sorts the first n device hits by (event, track key, sequence) and writes them to the
+// host buffer; per-event counts become hit_offsets.
+int order_hits(cmpmc_model* M, uint64_t n, bool plain, uint64_t first_event, uint64_t n_events, cmpmc_sensor_hit* host,
+               uint64_t* offsets, hipStream_t s) {
+    CHK(ensure(M->counts, (n_events + 1) * sizeof(uint32_t)));
+    HIPCHK(hipMemsetAsync(M->counts.p, 0, (n_events + 1) * sizeof(uint32_t), s));
+    const DevHit* hits = as<DevHit>(M->hits);
+    uint32_t* idx_final = nullptr;
+    if (n > 1) {
+        CHK(ensure(M->keys_a, n * 8));
+        CHK(ensure(M->keys_b, n * 8));
+        CHK(ensure(M->idx_a, n * 4));
+        CHK(ensure(M->idx_b, n * 4));
+        uint64_t* ka = as<uint64_t>(M->keys_a);
+        uint64_t* kb = as<uint64_t>(M->keys_b);
+        uint32_t* ia = as<uint32_t>(M->idx_a);
+        uint32_t* ib = as<uint32_t>(M->idx_b);
+        hipLaunchKernelGGL(iota_kernel, dim3(nblk(n)), dim3(256), 0, s, ia, n);
+        // least significant key first; every pass is a stable radix sort
+        const int fields_plain[1] = {0};
+        const int fields_batched[3] = {2, 1, 0};
+        const int* fields = plain ? fields_plain : fields_batched;
+        const int nf = plain ? 1 : 3;
+        for (int f = 0; f < nf; ++f) {
+            hipLaunchKernelGGL(gather_key_kernel, dim3(nblk(n)), dim3(256), 0, s, hits, ia, n, fields[f], ka);
+            const int bits = (fields[f] == 2) ? 32 : 64;
+            size_t tmp = 0;
+            HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, ka, kb, ia, ib, static_cast<int>(n), 0, bits, s));
+            CHK(ensure(M->cub_tmp, tmp));
+            HIPCHK(hipcub::DeviceRadixSort::SortPairs(M->cub_tmp.p, tmp, ka, kb, ia, ib, static_cast<int>(n), 0, bits, s));
+            std::swap(ia, ib);
+        }
+        idx_final = ia;
+    }
+    CHK(ensure(M->sorted, std::max<uint64_t>(n, 1) * sizeof(cmpmc_sensor_hit)));
+    if (n) hipLaunchKernelGGL(gather_hits_kernel, dim3(nblk(n)), dim3(256), 0, s, hits, idx_final, n, plain ? 1 : 0,
+                              first_event, n_events, as<cmpmc_sensor_hit>(M->sorted), as<uint32_t>(M->counts));
+    HIPCHK(hipGetLastError());
+    std::vector<uint32_t> counts(n_events);
+    if (n && host) HIPCHK(hipMemcpyAsync(host, M->sorted.p, n * sizeof(cmpmc_sensor_hit), hipMemcpyDeviceToHost, s));
+    if (n_events) HIPCHK(hipMemcpyAsync(counts.data(), M->counts.p, n_events * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (offsets) {
+        offsets[0] = 0;
+        for (uint64_t e = 0; e < n_events; ++e) offsets[e + 1] = offsets[e] + counts[e];
+    }
+    return CMPMC_OK;
+}
+
+// Everything a run needs, whatever its source (host stacks or device generator).
+struct Job {
+    uint64_t n_events = 0;
+    uint64_t first_event = 0;
+    const cmpmc_phonon_track* tracks = nullptr;
+    const uint64_t* toff = nullptr;
+    const cmpmc_charge_track* charges = nullptr;
+    const uint64_t* coff = nullptr;
+    const uint64_t* rng_in = nullptr;
+    uint64_t* rng_out = nullptr;
+    const double* injected = nullptr;
+    const cmpmc_workload* wl = nullptr;
+    bool skip_phonons = false;
+    // transport_charges: the event's remaining phonon stack goes here
+    cmpmc_phonon_track* ph_out = nullptr;
+    uint64_t ph_cap = 0;
+    uint64_t* n_ph = nullptr;
+};
+
+struct RunCfg {
+    int form = CMPMC_FORM_PLAIN;
+    bool keep_hits = true;
+    bool want_first_branch = false;
+    int block = 0;
+    uint64_t seed = 1;
+    uint64_t headroom = 0;
+    uint64_t lane_depth = 0;
+    uint32_t grid_blocks = 0;
+    uint32_t refill_period = 0;
+    bool poll_cancel = false;
+};
+
+struct Outputs {
+    cmpmc_sim_stats* stats;
+    cmpmc_sensor_hit* hits;       // host
+    uint64_t hit_cap;
+    uint64_t* n_hits;
+    uint64_t* hits_needed;
+    uint64_t* hit_offsets;
+    int32_t* first_branch;
+    double* kernel_ms;
+    uint64_t* stack_needed;
+};
+
+// ------------------------------------------------------------------ plain form
+int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s) {
+    const uint64_t N = J.n_events;
+    const bool keep = C.keep_hits && O.hits != nullptr;
+    const uint64_t dev_hit_cap = keep ? O.hit_cap : 0;
+    CHK(ensure(M->hits, std::max<uint64_t>(dev_hit_cap, 1) * sizeof(DevHit)));
+    CHK(ensure(M->acc, sizeof(Acc)));
+    Acc* d_acc = as<Acc>(M->acc);
+    HIPCHK(hipMemsetAsync(d_acc, 0, sizeof(Acc), s));
+    if (J.wl) {
+        CHK(ensure(M->wl, sizeof(cmpmc_workload)));
+        HIPCHK(hipMemcpyAsync(M->wl.p, J.wl, sizeof(cmpmc_workload), hipMemcpyHostToDevice, s));
+    }
+    // per-event initial phonon / charge counts
+    std::vector<uint32_t> nph(N), nch(N);
+    if (J.wl) {
+        if (J.wl->kind == CMPMC_WL_PAIR) {
+            std::fill(nph.begin(), nph.end(), 0u);
+            std::fill(nch.begin(), nch.end(), 2u);
+        } else {
+            CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * sizeof(uint32_t)));
+            HIPCHK(cmpmc_launch_workload_count(as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
+                                               as<uint32_t>(M->counts), s));
+            HIPCHK(hipMemcpyAsync(nph.data(), M->counts.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            std::fill(nch.begin(), nch.end(), 0u);
+        }
+    } else {
+        for (uint64_t e = 0; e < N; ++e) {
+            nph[e] = J.toff ? static_cast<uint32_t>(J.toff[e + 1] - J.toff[e]) : 0;
+            nch[e] = J.coff ? static_cast<uint32_t>(J.coff[e + 1] - J.coff[e]) : 0;
+        }
+    }
+    const bool any_charges = std::any_of(nch.begin(), nch.end(), [](uint32_t v) { return v > 0; });
+    uint64_t headroom = C.headroom ? C.headroom : (any_charges && M->dm.luke_fraction > 0.0 ? 8192 : 128);
+    const uint64_t c_headroom = 16;
+    const uint64_t max_lanes = 262144;
+    const uint64_t region_budget = 24ull << 30;
+    Totals tot;
+    float ms_total = 0.0f;
+    hipEvent_t ev0, ev1;
+    HIPCHK(hipEventCreate(&ev0));
+    HIPCHK(hipEventCreate(&ev1));
+    uint64_t deepest = 0;
+    int rc = CMPMC_OK;
+    std::vector<uint64_t> poff, coff;
+    for (uint64_t c0 = 0; c0 < N && rc == CMPMC_OK;) {
+        // chunk [c0, c1): bounded lane count and stack-region memory
+        uint64_t c1 = c0, bytes = 0;
+        while (c1 < N && c1 - c0 < max_lanes) {
+            const uint64_t b = (nph[c1] + headroom) * sizeof(cmpmc_phonon_track);
+            if (c1 > c0 && bytes + b > region_budget) break;
+            bytes += b;
+            ++c1;
+        }
+        const uint64_t n = c1 - c0;
+        for (int attempt = 0;; ++attempt) {
+            poff.assign(n + 1, 0);
+            coff.assign(n + 1, 0);
+            for (uint64_t e = 0; e < n; ++e) {
+                poff[e + 1] = poff[e] + nph[c0 + e] + headroom;
+                coff[e + 1] = coff[e] + (nch[c0 + e] ? nch[c0 + e] + c_headroom : 0);
+            }
+            CHK(ensure(M->pst, poff[n] * sizeof(cmpmc_phonon_track)));
+            CHK(ensure(M->poff, (n + 1) * 8));
+            CHK(ensure(M->pinit, n * 4));
+            HIPCHK(hipMemcpyAsync(M->poff.p, poff.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(M->pinit.p, nph.data() + c0, n * 4, hipMemcpyHostToDevice, s));
+            const bool ch = coff[n] > 0;
+            if (ch) {
+                CHK(ensure(M->cst, coff[n] * sizeof(cmpmc_charge_track)));
+                CHK(ensure(M->coff, (n + 1) * 8));
+                CHK(ensure(M->cinit, n * 4));
+                HIPCHK(hipMemcpyAsync(M->coff.p, coff.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(M->cinit.p, nch.data() + c0, n * 4, hipMemcpyHostToDevice, s));
+            }
+            if (!J.wl) {
+                // host stacks: copy this chunk's rows and scatter them into the regions
+                const uint64_t t0 = J.toff ? J.toff[c0] : 0, t1 = J.toff ? J.toff[c1] : 0;
+                std::vector<uint64_t> rel(n + 1);
+                if (t1 > t0) {
+                    for (uint64_t e = 0; e <= n; ++e) rel[e] = J.toff[c0 + e] - t0;
+                    CHK(ensure(M->tin, (t1 - t0) * sizeof(cmpmc_phonon_track)));
+                    CHK(ensure(M->tin_off, (n + 1) * 8));
+                    HIPCHK(hipMemcpyAsync(M->tin.p, J.tracks + t0, (t1 - t0) * sizeof(cmpmc_phonon_track),
+                                          hipMemcpyHostToDevice, s));
+                    HIPCHK(hipMemcpyAsync(M->tin_off.p, rel.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+                    HIPCHK(cmpmc_launch_scatter_tracks(as<cmpmc_phonon_track>(M->tin), as<uint64_t>(M->tin_off), n,
+                                                       t1 - t0, as<cmpmc_phonon_track>(M->pst), as<uint64_t>(M->poff), s));
+                    HIPCHK(hipStreamSynchronize(s));
+                }
+                const uint64_t q0 = J.coff ? J.coff[c0] : 0, q1 = J.coff ? J.coff[c1] : 0;
+                if (q1 > q0) {
+                    for (uint64_t e = 0; e <= n; ++e) rel[e] = J.coff[c0 + e] - q0;
+                    CHK(ensure(M->cin, (q1 - q0) * sizeof(cmpmc_charge_track)));
+                    CHK(ensure(M->cin_off, (n + 1) * 8));
+                    HIPCHK(hipMemcpyAsync(M->cin.p, J.charges + q0, (q1 - q0) * sizeof(cmpmc_charge_track),
+                                          hipMemcpyHostToDevice, s));
+                    HIPCHK(hipMemcpyAsync(M->cin_off.p, rel.data(), (n + 1) * 8, hipMemcpyHostToDevice, s));
+                    HIPCHK(cmpmc_launch_scatter_charges(as<cmpmc_charge_track>(M->cin), as<uint64_t>(M->cin_off), n,
+                                                        q1 - q0, as<cmpmc_charge_track>(M->cst), as<uint64_t>(M->coff), s));
+                    HIPCHK(hipStreamSynchronize(s));
+                }
+                CHK(ensure(M->rng, n * 32));
+                HIPCHK(hipMemcpyAsync(M->rng.p, J.rng_in + 4 * c0, n * 32, hipMemcpyHostToDevice, s));
+                if (J.injected) {
+                    CHK(ensure(M->inj, n * 8));
+                    HIPCHK(hipMemcpyAsync(M->inj.p, J.injected + c0, n * 8, hipMemcpyHostToDevice, s));
+                }
+            }
+            CHK(ensure(M->ws, static_cast<size_t>(5) * CMPMC_KCAP * n * sizeof(double)));
+            if (C.want_first_branch || J.skip_phonons) CHK(ensure(M->fb, n * 4 + 16));
+            if (J.skip_phonons) CHK(ensure(M->counts, n * 4 + 16));
+            // snapshot of the hit counter: a retried chunk rewrites its hits
+            unsigned long long hit_before = 0;
+            HIPCHK(hipMemcpyAsync(&hit_before, &d_acc->hit_count, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            Acc zero{};
+            zero.hit_count = hit_before;
+            HIPCHK(hipMemcpyAsync(d_acc, &zero, sizeof(Acc), hipMemcpyHostToDevice, s));
+            PlainArgs a{};
+            a.n_events = n;
+            a.first_event = J.first_event + c0;
+            a.pstack = as<cmpmc_phonon_track>(M->pst);
+            a.pstack_off = as<uint64_t>(M->poff);
+            a.n_init_ph = J.wl ? nullptr : as<uint32_t>(M->pinit);
+            a.cstack = ch ? as<cmpmc_charge_track>(M->cst) : nullptr;
+            a.cstack_off = ch ? as<uint64_t>(M->coff) : nullptr;
+            a.n_init_ch = (ch && !J.wl) ? as<uint32_t>(M->cinit) : nullptr;
+            a.rng_state = J.wl ? nullptr : as<uint64_t>(M->rng);
+            a.injected = (J.injected && !J.wl) ? as<double>(M->inj) : nullptr;
+            a.wl = J.wl ? as<cmpmc_workload>(M->wl) : nullptr;
+            a.seed = C.seed;
+            a.hits = as<DevHit>(M->hits);
+            a.hit_count = &d_acc->hit_count;
+            a.hit_cap = dev_hit_cap;
+            a.keep_hits = keep ? 1 : 0;
+            a.skip_phonons = J.skip_phonons ? 1 : 0;
+            a.hits_per_event = J.skip_phonons ? as<uint32_t>(M->counts) : nullptr;
+            a.first_branch = C.want_first_branch ? as<int32_t>(M->fb) : nullptr;
+            a.counters = d_acc->counters;
+            a.ledger = d_acc->ledger;
+            a.status = d_acc->status;
+            a.kaplan_ws = as<double>(M->ws);
+            a.ws_stride = static_cast<uint32_t>(n);
+            a.cancel = C.poll_cancel ? M->d_cancel : nullptr;
+            HIPCHK(hipEventRecord(ev0, s));
+            HIPCHK(cmpmc_launch_plain(&M->dm, &a, C.block, s));
+            HIPCHK(hipEventRecord(ev1, s));
+            Acc res;
+            HIPCHK(hipMemcpyAsync(&res, d_acc, sizeof(Acc), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+            ms_total += ms;
+            if (res.status[0]) {
+                // a stack region was too small: rerun the chunk (same inputs, same result)
+                deepest = std::max<uint64_t>(deepest, res.status[1]);
+                const uint64_t max_init = *std::max_element(nph.begin() + c0, nph.begin() + c1);
+                const uint64_t need = res.status[1] > max_init ? res.status[1] - max_init : headroom;
+                if (attempt >= 6 || headroom > (1ull << 22)) {
+                    if (O.stack_needed) *O.stack_needed = deepest;
+                    rc = fail(CMPMC_ERR_CAPACITY, "event stack overflow: %llu entries needed",
+                              static_cast<unsigned long long>(deepest));
+                    break;
+                }
+                headroom = std::max(headroom * 2, need * 2 + 64);
+                HIPCHK(hipMemcpyAsync(&d_acc->hit_count, &hit_before, 8, hipMemcpyHostToDevice, s));
+                continue;
+            }
+            if (C.poll_cancel && *M->h_cancel) {
+                rc = fail(CMPMC_ERR_CANCELLED, "run cancelled by the host");
+                break;
+            }
+            add_acc(tot, res);
+            if (C.want_first_branch && O.first_branch)
+                HIPCHK(hipMemcpy(O.first_branch + c0, M->fb.p, n * 4, hipMemcpyDeviceToHost));
+            if (J.rng_out) HIPCHK(hipMemcpy(J.rng_out + 4 * c0, M->rng.p, n * 32, hipMemcpyDeviceToHost));
+            if (J.skip_phonons && J.ph_out) {
+                // transport_charges: hand back the Luke phonons left on each event's stack
+                std::vector<uint32_t> left(n);
+                HIPCHK(hipMemcpy(left.data(), M->counts.p, n * 4, hipMemcpyDeviceToHost));
+                for (uint64_t e = 0; e < n; ++e) {
+                    if (*J.n_ph + left[e] > J.ph_cap) {
+                        rc = fail(CMPMC_ERR_CAPACITY, "phonon output buffer too small (%llu needed)",
+                                  static_cast<unsigned long long>(*J.n_ph + left[e]));
+                        break;
+                    }
+                    HIPCHK(hipMemcpy(J.ph_out + *J.n_ph, as<cmpmc_phonon_track>(M->pst) + poff[e],
+                                     left[e] * sizeof(cmpmc_phonon_track), hipMemcpyDeviceToHost));
+                    *J.n_ph += left[e];
+                }
+            }
+            break;
+        }
+        c0 = c1;
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (rc != CMPMC_OK) return rc;
+    if (O.kernel_ms) *O.kernel_ms += ms_total;
+    unsigned long long produced = 0;
+    HIPCHK(hipMemcpy(&produced, &d_acc->hit_count, 8, hipMemcpyDeviceToHost));
+    if (O.hits_needed) *O.hits_needed = keep ? produced : 0;
+    apply_totals(tot, *O.stats);
+    if (keep && produced > dev_hit_cap)
+        return fail(CMPMC_ERR_CAPACITY, "hit buffer too small: %llu hits produced, capacity %llu", produced,
+                    static_cast<unsigned long long>(dev_hit_cap));
+    const uint64_t nh = keep ? produced : 0;
+    CHK(order_hits(M, nh, true, J.first_event, N, O.hits, O.hit_offsets, s));
+    if (O.n_hits) *O.n_hits = nh;
+    return CMPMC_OK;
+}
+
+// ------------------------------------------------------------------ batched form
+int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block) {
+    if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
+    int per_cu = 0;
+    if (cmpmc_occupancy(which, block, &per_cu) != hipSuccess || per_cu <= 0) per_cu = 4;
+    return per_cu * M->n_cu;
+}
+
+int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s) {
+    const uint64_t N = J.n_events;
+    const bool keep = C.keep_hits && O.hits != nullptr;
+    const uint64_t dev_hit_cap = keep ? O.hit_cap : 0;
+    const int block = C.block > 0 ? C.block : 256;
+    CHK(ensure(M->hits, std::max<uint64_t>(dev_hit_cap, 1) * sizeof(DevHit)));
+    CHK(ensure(M->acc, sizeof(Acc)));
+    Acc* d_acc = as<Acc>(M->acc);
+    if (J.wl) {
+        CHK(ensure(M->wl, sizeof(cmpmc_workload)));
+        HIPCHK(hipMemcpyAsync(M->wl.p, J.wl, sizeof(cmpmc_workload), hipMemcpyHostToDevice, s));
+    }
+    Totals tot;
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    uint64_t depth = C.lane_depth ? C.lane_depth : 512;
+    const int grid = grid_for(M, 0, C, block);
+    const int cgrid = grid_for(M, 1, C, block);
+    const uint64_t lanes = static_cast<uint64_t>(grid) * block;
+    const uint64_t clanes = static_cast<uint64_t>(cgrid) * block;
+    // initial queues
+    uint64_t n_init = 0, n_charges = 0;
+    std::vector<uint64_t> off;
+    const bool pair = J.wl && J.wl->kind == CMPMC_WL_PAIR;
+    if (J.wl && !pair) {
+        CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * 4));
+        HIPCHK(cmpmc_launch_workload_count(as<cmpmc_workload>(M->wl), C.seed, J.first_event, N, as<uint32_t>(M->counts), s));
+        std::vector<uint32_t> cnt(N);
+        HIPCHK(hipMemcpyAsync(cnt.data(), M->counts.p, N * 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        off.assign(N + 1, 0);
+        for (uint64_t e = 0; e < N; ++e) off[e + 1] = off[e] + cnt[e];
+        n_init = off[N];
+    } else if (!J.wl) {
+        n_init = J.toff ? J.toff[N] : 0;
+        n_charges = J.coff ? J.coff[N] : 0;
+    } else {
+        n_charges = 2 * N;
+    }
+    const double lf = std::min(std::max(M->dm.luke_fraction, 0.0), 1.0);
+    uint64_t luke_cap = (n_charges && lf > 0.0) ? n_charges * 4096 : 0;
+    float ms_total = 0.0f;
+    int rc = CMPMC_OK;
+    for (int attempt = 0;; ++attempt) {
+        Acc zero{};
+        HIPCHK(hipMemcpyAsync(d_acc, &zero, sizeof(Acc), hipMemcpyHostToDevice, s));
+        CHK(ensure(M->queue, std::max<uint64_t>(n_init + luke_cap, 1) * sizeof(BTrack)));
+        BTrack* q = as<BTrack>(M->queue);
+        unsigned long long inj_fixed = 0;
+        if (J.wl && !pair) {
+            CHK(ensure(M->qoff, (N + 1) * 8));
+            HIPCHK(hipMemcpyAsync(M->qoff.p, off.data(), (N + 1) * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(cmpmc_launch_workload_fill(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
+                                              as<uint64_t>(M->qoff), n_init, q, d_acc->ledger, s));
+        } else if (!J.wl && n_init) {
+            CHK(ensure(M->tin, n_init * sizeof(cmpmc_phonon_track)));
+            CHK(ensure(M->tin_off, (N + 1) * 8));
+            HIPCHK(hipMemcpyAsync(M->tin.p, J.tracks, n_init * sizeof(cmpmc_phonon_track), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(M->tin_off.p, J.toff, (N + 1) * 8, hipMemcpyHostToDevice, s));
+            HIPCHK(cmpmc_launch_tracks_to_queue(as<cmpmc_phonon_track>(M->tin), as<uint64_t>(M->tin_off), N, n_init,
+                                                J.first_event, q, s));
+        }
+        if (!J.wl && J.injected)
+            for (uint64_t e = 0; e < N; ++e) inj_fixed += fixed_point(J.injected[e]);
+        if (n_charges) {
+            CHK(ensure(M->cqueue, n_charges * sizeof(BCharge)));
+            if (pair) {
+                HIPCHK(cmpmc_launch_pair_fill(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
+                                              as<BCharge>(M->cqueue), s));
+            } else {
+                CHK(ensure(M->cin, n_charges * sizeof(cmpmc_charge_track)));
+                CHK(ensure(M->cin_off, (N + 1) * 8));
+                HIPCHK(hipMemcpyAsync(M->cin.p, J.charges, n_charges * sizeof(cmpmc_charge_track), hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(M->cin_off.p, J.coff, (N + 1) * 8, hipMemcpyHostToDevice, s));
+                HIPCHK(cmpmc_launch_charges_to_queue(as<cmpmc_charge_track>(M->cin), as<uint64_t>(M->cin_off), N,
+                                                     n_charges, J.first_event, as<BCharge>(M->cqueue), s));
+            }
+        }
+        if (C.want_first_branch) {
+            CHK(ensure(M->fb, N * 4 + 16));
+            HIPCHK(hipMemsetAsync(M->fb.p, 0, N * 4, s));
+        }
+        HIPCHK(hipEventRecord(e0, s));
+        uint64_t n_luke = 0;
+        if (n_charges) {
+            CHK(ensure(M->clstack, static_cast<size_t>(64) * clanes * sizeof(BCharge)));
+            ChargeArgs ca{};
+            ca.seed = C.seed;
+            ca.first_event = J.first_event;
+            ca.init = as<BCharge>(M->cqueue);
+            ca.n_init = n_charges;
+            ca.init_head = &d_acc->chead;
+            ca.lstack = as<BCharge>(M->clstack);
+            ca.depth = 64;
+            ca.lanes = static_cast<uint32_t>(clanes);
+            ca.luke_out = q + n_init;
+            ca.luke_count = &d_acc->luke_count;
+            ca.luke_cap = luke_cap;
+            ca.keep_hits = keep ? 1 : 0;
+            ca.hits = as<DevHit>(M->hits);
+            ca.hit_count = &d_acc->hit_count;
+            ca.hit_cap = dev_hit_cap;
+            ca.counters = d_acc->counters;
+            ca.ledger = d_acc->ledger;
+            ca.status = d_acc->status;
+            ca.cancel = C.poll_cancel ? M->d_cancel : nullptr;
+            HIPCHK(cmpmc_launch_charges(&M->dm, &ca, block, cgrid, s));
+            HIPCHK(hipMemcpyAsync(&n_luke, &d_acc->luke_count, 8, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            if (n_luke > luke_cap) {              // Luke queue too small: rerun with room
+                if (attempt >= 4) { rc = fail(CMPMC_ERR_CAPACITY, "Luke phonon queue overflow"); break; }
+                luke_cap = n_luke + n_luke / 4 + 1024;
+                continue;
+            }
+        }
+        HIPCHK(hipEventRecord(e1, s));
+        const uint64_t n_queue = n_init + n_luke;
+        if (n_queue) {
+            CHK(ensure(M->lstack, static_cast<size_t>(depth) * lanes * sizeof(BTrack)));
+            CHK(ensure(M->ws, static_cast<size_t>(5) * CMPMC_KCAP * lanes * sizeof(double)));
+            BatchedArgs ba{};
+            ba.seed = C.seed;
+            ba.first_event = J.first_event;
+            ba.n_events = N;
+            ba.init = q;
+            ba.n_init = n_queue;
+            ba.init_head = &d_acc->head;
+            ba.lstack = as<BTrack>(M->lstack);
+            ba.depth = static_cast<uint32_t>(depth);
+            ba.lanes = static_cast<uint32_t>(lanes);
+            ba.refill_period = C.refill_period ? C.refill_period : 1;
+            ba.keep_hits = keep ? 1 : 0;
+            ba.hits = as<DevHit>(M->hits);
+            ba.hit_count = &d_acc->hit_count;
+            ba.hit_cap = dev_hit_cap;
+            ba.first_branch = C.want_first_branch ? as<int32_t>(M->fb) : nullptr;
+            ba.counters = d_acc->counters;
+            ba.ledger = d_acc->ledger;
+            ba.status = d_acc->status;
+            ba.kaplan_ws = as<double>(M->ws);
+            ba.cancel = C.poll_cancel ? M->d_cancel : nullptr;
+            HIPCHK(cmpmc_launch_batched(&M->dm, &ba, block, grid, s));
+        }
+        HIPCHK(hipEventRecord(e2, s));
+        Acc res;
+        HIPCHK(hipMemcpyAsync(&res, d_acc, sizeof(Acc), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e2));
+        ms_total += ms;
+        if (res.status[2]) { rc = fail(CMPMC_ERR_CANCELLED, "run cancelled by the host"); break; }
+        if (res.status[0]) {                      // a lane spawn stack overflowed: rerun deeper
+            if (attempt >= 4 || depth >= (1u << 16)) {
+                if (O.stack_needed) *O.stack_needed = res.status[1];
+                rc = fail(CMPMC_ERR_CAPACITY, "lane spawn stack overflow (depth %llu)", static_cast<unsigned long long>(depth));
+                break;
+            }
+            depth = std::max<uint64_t>(depth * 2, res.status[1] + 64);
+            continue;
+        }
+        res.ledger[2 * L_INJ] += inj_fixed >> 20;
+        res.ledger[2 * L_INJ + 1] += inj_fixed & 0xFFFFFull;
+        add_acc(tot, res);
+        if (C.want_first_branch && O.first_branch) HIPCHK(hipMemcpy(O.first_branch, M->fb.p, N * 4, hipMemcpyDeviceToHost));
+        break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (rc != CMPMC_OK) return rc;
+    if (O.kernel_ms) *O.kernel_ms += ms_total;
+    unsigned long long produced = 0;
+    HIPCHK(hipMemcpy(&produced, &d_acc->hit_count, 8, hipMemcpyDeviceToHost));
+    if (O.hits_needed) *O.hits_needed = keep ? produced : 0;
+    apply_totals(tot, *O.stats);
+    if (keep && produced > dev_hit_cap)
+        return fail(CMPMC_ERR_CAPACITY, "hit buffer too small: %llu hits produced, capacity %llu", produced,
+                    static_cast<unsigned long long>(dev_hit_cap));
+    const uint64_t nh = keep ? produced : 0;
+    CHK(order_hits(M, nh, false, J.first_event, N, O.hits, O.hit_offsets, s));
+    if (O.n_hits) *O.n_hits = nh;
+    return CMPMC_OK;
+}
+
+RunCfg cfg_from(const cmpmc_run_opts* o) {
+    RunCfg c;
+    if (!o) return c;
+    c.form = o->form;
+    c.keep_hits = o->keep_hits != 0;
+    c.want_first_branch = o->want_first_branch != 0;
+    c.block = o->block_size;
+    c.seed = o->seed;
+    c.headroom = o->stack_headroom;
+    c.lane_depth = o->lane_stack_depth;
+    c.grid_blocks = o->grid_blocks;
+    c.refill_period = o->refill_period;
+    c.poll_cancel = o->poll_cancel != 0;
+    return c;
+}
+
+int validate_tracks(const cmpmc_phonon_track* t, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) {
+        const cmpmc_phonon_track& x = t[i];
+        if (!(x.energy >= 0.0) || !(x.weight >= 0.0) || x.mode > 2 || !std::isfinite(x.energy) ||
+            !std::isfinite(x.pos[0] + x.pos[1] + x.pos[2]) || !std::isfinite(x.time))
+            return fail(CMPMC_ERR_INVALID, "phonon track %llu: invalid energy, weight, mode, position or time",
+                        static_cast<unsigned long long>(i));
+    }
+    return CMPMC_OK;
+}
+int validate_charges(const cmpmc_dev::DevModel& m, const cmpmc_charge_track* t, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) {
+        const cmpmc_charge_track& x = t[i];
+        if ((x.type != 1 && x.type != 2) || !(x.ekin >= 0.0) || !std::isfinite(x.ekin) ||
+            (x.type == 1 && static_cast<int>(x.valley) >= std::max(1, m.n_valleys)))
+            return fail(CMPMC_ERR_INVALID, "charge track %llu: invalid type, energy or valley",
+                        static_cast<unsigned long long>(i));
+    }
+    return CMPMC_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ entry points
+extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, const cmpmc_run_opts* opts,
+                                cmpmc_run_out* out, void* stream) {
+    if (!M || !b || !out) return fail(CMPMC_ERR_INVALID, "cmpmc_run_events: null argument");
+    if (b->n_events && (!b->rng_state || (b->tracks && !b->track_offsets) || (b->charges && !b->charge_offsets)))
+        return fail(CMPMC_ERR_INVALID, "cmpmc_run_events: offsets and rng_state are required");
+    std::lock_guard<std::mutex> lk(M->mu);
+    cmpmc_internal::clear_error();
+    const uint64_t nt = b->tracks ? b->track_offsets[b->n_events] : 0;
+    const uint64_t nc = b->charges ? b->charge_offsets[b->n_events] : 0;
+    CHK(validate_tracks(b->tracks, nt));
+    CHK(validate_charges(M->dm, b->charges, nc));
+    if (nc && M->dm.iv_model == 2)
+        return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
+    Job J;
+    J.n_events = b->n_events;
+    J.first_event = b->first_event_id;
+    J.tracks = b->tracks;
+    J.toff = b->tracks ? b->track_offsets : nullptr;
+    J.charges = b->charges;
+    J.coff = b->charges ? b->charge_offsets : nullptr;
+    J.rng_in = b->rng_state;
+    J.injected = b->injected_eV;
+    const RunCfg C = cfg_from(opts);
+    Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
+              out->first_branch, &out->kernel_ms, &out->stack_needed};
+    out->kernel_ms = 0.0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (C.form == CMPMC_FORM_BATCHED) return run_batched(M, J, C, O, s);
+    return run_plain(M, J, C, O, s);
+}
+
+extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events, const cmpmc_run_opts* opts,
+                                  cmpmc_run_out* out, void* stream) {
+    if (!M || !wl || !out) return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: null argument");
+    if (wl->kind != CMPMC_WL_PHONON_POINT && wl->kind != CMPMC_WL_PHONON_LOGN && wl->kind != CMPMC_WL_PAIR &&
+        wl->kind != CMPMC_WL_PHONON_UNIF)
+        return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: unknown workload kind %d", wl->kind);
+    if (wl->mode < 0 || wl->mode > 2 || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
+        (wl->kind == CMPMC_WL_PHONON_LOGN && !(wl->logn_median > 0.0)))
+        return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: invalid workload parameters");
+    if (wl->kind == CMPMC_WL_PAIR && M->dm.iv_model == 2)
+        return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
+    if (n_events >= (1ull << 32)) return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: at most 2^32 - 1 events per call");
+    std::lock_guard<std::mutex> lk(M->mu);
+    cmpmc_internal::clear_error();
+    Job J;
+    J.n_events = n_events;
+    J.first_event = 0;
+    J.wl = wl;
+    const RunCfg C = cfg_from(opts);
+    Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
+              out->first_branch, &out->kernel_ms, &out->stack_needed};
+    out->kernel_ms = 0.0;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (C.form == CMPMC_FORM_BATCHED) return run_batched(M, J, C, O, s);
+    return run_plain(M, J, C, O, s);
+}
+
+extern "C" int cmpmc_transport_serial(cmpmc_model* M, const cmpmc_phonon_track* stack, uint64_t n, uint64_t rng_state[4],
+                                      cmpmc_sensor_hit* hits, uint64_t hit_cap, uint64_t* n_hits, cmpmc_sim_stats* stats,
+                                      void* stream) {
+    if (!M || (n && !stack) || !rng_state || !n_hits || !stats) return fail(CMPMC_ERR_INVALID, "cmpmc_transport_serial: null argument");
+    if (*n_hits > hit_cap) return fail(CMPMC_ERR_INVALID, "cmpmc_transport_serial: *n_hits exceeds hit_cap");
+    if (n == 0) return CMPMC_OK;
+    std::lock_guard<std::mutex> lk(M->mu);
+    cmpmc_internal::clear_error();
+    CHK(validate_tracks(stack, n));
+    const uint64_t toff[2] = {0, n};
+    uint64_t rng_out[4];
+    Job J;
+    J.n_events = 1;
+    J.tracks = stack;
+    J.toff = toff;
+    J.rng_in = rng_state;
+    J.rng_out = rng_out;
+    RunCfg C;
+    C.keep_hits = hits != nullptr;
+    uint64_t nh = 0, need = 0, offs[2];
+    double ms = 0.0;
+    cmpmc_sim_stats st{};
+    Outputs O{&st, hits ? hits + *n_hits : nullptr, hit_cap - *n_hits, &nh, &need, offs, nullptr, &ms, nullptr};
+    const int rc = run_plain(M, J, C, O, static_cast<hipStream_t>(stream));
+    if (rc != CMPMC_OK) return rc;
+    std::memcpy(rng_state, rng_out, sizeof(rng_out));
+    *n_hits += nh;
+    // SimStats operator+= (engine.hpp:64-92)
+    uint64_t* a = reinterpret_cast<uint64_t*>(stats);
+    const uint64_t* b = reinterpret_cast<const uint64_t*>(&st);
+    for (int i = 0; i < 26; ++i) {
+        if (i == 16 || i == 17 || i >= 23) {
+            double x, y;
+            std::memcpy(&x, a + i, 8);
+            std::memcpy(&y, b + i, 8);
+            x += y;
+            std::memcpy(a + i, &x, 8);
+        } else {
+            a[i] += b[i];
+        }
+    }
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_transport_charges_serial(cmpmc_model* M, const cmpmc_charge_track* charges, uint64_t n,
+                                              uint64_t rng_state[4], cmpmc_phonon_track* phonons_out,
+                                              uint64_t phonon_cap, uint64_t* n_phonons, cmpmc_sensor_hit* hits,
+                                              uint64_t hit_cap, uint64_t* n_hits, cmpmc_sim_stats* stats, void* stream) {
+    if (!M || (n && !charges) || !rng_state || !n_hits || !stats || !n_phonons)
+        return fail(CMPMC_ERR_INVALID, "cmpmc_transport_charges_serial: null argument");
+    if (M->dm.iv_model == 2)
+        return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
+    if (n == 0) return CMPMC_OK;
+    std::lock_guard<std::mutex> lk(M->mu);
+    cmpmc_internal::clear_error();
+    CHK(validate_charges(M->dm, charges, n));
+    const uint64_t coff[2] = {0, n};
+    uint64_t rng_out[4];
+    Job J;
+    J.n_events = 1;
+    J.charges = charges;
+    J.coff = coff;
+    J.rng_in = rng_state;
+    J.rng_out = rng_out;
+    J.skip_phonons = true;
+    J.ph_out = phonons_out;
+    J.ph_cap = phonon_cap;
+    J.n_ph = n_phonons;
+    RunCfg C;
+    C.keep_hits = hits != nullptr;
+    uint64_t nh = 0, need = 0, offs[2];
+    double ms = 0.0;
+    cmpmc_sim_stats st{};
+    Outputs O{&st, hits ? hits + *n_hits : nullptr, hit_cap - *n_hits, &nh, &need, offs, nullptr, &ms, nullptr};
+    const int rc = run_plain(M, J, C, O, static_cast<hipStream_t>(stream));
+    if (rc != CMPMC_OK) return rc;
+    std::memcpy(rng_state, rng_out, sizeof(rng_out));
+    *n_hits += nh;
+    uint64_t* a = reinterpret_cast<uint64_t*>(stats);
+    const uint64_t* b = reinterpret_cast<const uint64_t*>(&st);
+    for (int i = 0; i < 26; ++i) {
+        if (i == 16 || i == 17 || i >= 23) {
+            double x, y;
+            std::memcpy(&x, a + i, 8);
+            std::memcpy(&y, b + i, 8);
+            x += y;
+            std::memcpy(a + i, &x, 8);
+        } else {
+            a[i] += b[i];
+        }
+    }
+    return CMPMC_OK;
+}

@@ -1,0 +1,351 @@
+// One step of SimulationEngine::transport_charges (engine.hpp:210-377), first-order
+// (FirstOrderDkCap) stepping, uniform field, RecordOnly release at collection, with the
+// ChargePhysics leaves of charge/charge_transport.hpp:49-241 and the Herring-Vogt
+// kinematics of lattice/crystal_lattice.hpp:239-293.
+//
+// Ctx (in addition to the phonon context):
+//   ctx.ccount(CCounter), ctx.luke_eV(double), ctx.recomb(double), ctx.escaped(double),
+//   ctx.deposited(double), ctx.push_charge(CTrack&, CTrack&), ctx.push_luke(PTrack&, CTrack&),
+//   ctx.charge_hit(CTrack&, edep_w, e_rel, surface)
+#pragma once
+
+#include "phonon_step.hpp"
+
+namespace cmpmc_dev {
+
+enum CCounter : int { CC_STEPS = 0, CC_LUKE, CC_IV, CC_TRAPPED, CC_TRAP_ION, CC_SURF_REFL, CC_COLLECTED, CC_N };
+
+template <class P> struct CTrack {
+    using R = typename P::R;
+    V3<R> pos, p;
+    R ekin;
+    double time, weight;
+    int valley;
+    int type;                    // 1 electron, 2 hole
+    Rng rng;
+    uint64_t key;
+    uint32_t spawn_no, hit_seq;
+};
+
+template <class P> __device__ inline V3<typename P::R> to_valley(const DevModel& m, int iv, V3<typename P::R> v) {
+    return mat_mul(m.valley_rot[iv], v);
+}
+template <class P> __device__ inline V3<typename P::R> from_valley(const DevModel& m, int iv, V3<typename P::R> v) {
+    return mat_mul_t(m.valley_rot[iv], v);
+}
+template <class P> __device__ inline V3<typename P::R> ell_to_sph(const DevModel& m, int iv, V3<typename P::R> v) {
+    using R = typename P::R;
+    const V3<R> t = to_valley<P>(m, iv, v);
+    return {t.x * R(m.hv_sqrt_inv[0]), t.y * R(m.hv_sqrt_inv[1]), t.z * R(m.hv_sqrt_inv[2])};
+}
+template <class P> __device__ inline V3<typename P::R> sph_to_ell(const DevModel& m, int iv, V3<typename P::R> v) {
+    using R = typename P::R;
+    const V3<R> s{v.x * R(m.hv_sqrt[0]), v.y * R(m.hv_sqrt[1]), v.z * R(m.hv_sqrt[2])};
+    return from_valley<P>(m, iv, s);
+}
+// E_kin = sqrt(p^2 + m^2) - m; the FP32 form p^2 / (sqrt(p^2 + m^2) + m) avoids the
+// cancellation (E_kin / m ~ 1e-10 here).
+template <class P> __device__ inline typename P::R kin_energy(typename P::R p2, double mass) {
+    using R = typename P::R;
+    if constexpr (P::exact) return sqrt(p2 + mass * mass) - mass;
+    else {
+        const R mm = R(mass);
+        return p2 / (P::sqrt_(p2 + mm * mm) + mm);
+    }
+}
+template <class P> __device__ inline typename P::R ekin_e(const DevModel& m, int iv, V3<typename P::R> p) {
+    return kin_energy<P>(norm2(ell_to_sph<P>(m, iv, p)), m.e_mass_eV);
+}
+template <class P> __device__ inline typename P::R ekin_h(const DevModel& m, V3<typename P::R> p) {
+    return kin_energy<P>(norm2(p), m.h_mass_eV);
+}
+template <class P> __device__ inline V3<typename P::R> vel_e(const DevModel& m, int iv, V3<typename P::R> p) {
+    using R = typename P::R;
+    const R ek = ekin_e<P>(m, iv, p);
+    const V3<R> pv = to_valley<P>(m, iv, p);
+    const V3<R> vv{pv.x / R(m.mass_ratio[0]), pv.y / R(m.mass_ratio[1]), pv.z / R(m.mass_ratio[2])};
+    return from_valley<P>(m, iv, vv) * (R(kCLight) / (ek + R(m.e_mass_eV)));
+}
+template <class P> __device__ inline V3<typename P::R> vel_h(const DevModel& m, V3<typename P::R> p) {
+    using R = typename P::R;
+    const R ek = ekin_h<P>(m, p);
+    return p * (R(kCLight) / (ek + R(m.h_mass_eV)));
+}
+template <class P> __device__ inline V3<typename P::R> ekin_to_p_e(const DevModel& m, int iv, V3<typename P::R> dir,
+                                                                   typename P::R ekin) {
+    using R = typename P::R;
+    const V3<R> u = unit<P>(dir);
+    const V3<R> uh = ell_to_sph<P>(m, iv, u);
+    const R band_u = norm2(uh);
+    if (band_u <= R(0) || ekin <= R(0)) return {R(0), R(0), R(0)};
+    const R pm = P::sqrt_((ekin * ekin + R(2) * ekin * R(m.e_mass_eV)) / band_u);
+    return u * pm;
+}
+template <class P> __device__ inline V3<typename P::R> charge_velocity(const DevModel& m, const CTrack<P>& t) {
+    return (t.type == 1) ? vel_e<P>(m, t.valley, t.p) : vel_h<P>(m, t.p);
+}
+
+// ChargePhysics::make_charge (charge_transport.hpp:49-70)
+template <class P>
+__device__ inline void make_charge(const DevModel& m, CTrack<P>& c, int type, V3<typename P::R> pos, V3<typename P::R> dir,
+                                   typename P::R ekin, int valley, double time) {
+    using R = typename P::R;
+    const V3<R> u = unit<P>(dir);
+    if (type == 1) c.p = ekin_to_p_e<P>(m, valley, u, ekin);
+    else {
+        const R mh = R(m.h_mass_eV);
+        c.p = u * P::sqrt_(tmax(R(0), ekin * ekin + R(2) * ekin * mh));
+    }
+    c.pos = pos;
+    c.ekin = ekin;
+    c.time = time;
+    c.weight = 1.0;
+    c.valley = valley;
+    c.type = type;
+    c.spawn_no = 0;
+    c.hit_seq = 0;
+}
+
+// charge_transport.hpp:82-106
+template <class P> __device__ inline typename P::R luke_rate(const DevModel& m, const CTrack<P>& t) {
+    using R = typename P::R;
+    R k_mag, k_sound, l0;
+    if (t.type == 1) {
+        k_mag = norm<P>(ell_to_sph<P>(m, t.valley, vdiv<P>(t.p, R(kHbarc))));
+        k_sound = R(m.k_sound_e);
+        l0 = R(m.l0_e);
+    } else {
+        k_mag = norm<P>(t.p) / R(kHbarc);
+        k_sound = R(m.k_sound_h);
+        l0 = R(m.l0_h);
+    }
+    if (k_mag <= k_sound) return R(0);
+    const R mach = k_mag / k_sound;
+    const R dm = mach - R(1);
+    const R tstep = (R(3) * l0) / R(m.v_sound);
+    const R tau = tstep * mach / (dm * dm * dm);
+    return (tau > R(0)) ? (R(1) / tau) : R(0);
+}
+
+// charge_kernels.hpp:82-99
+template <class P> __device__ inline typename P::R iv_rate(const DevModel& m, typename P::R ekin) {
+    using R = typename P::R;
+    if (m.n_valleys < 2) return R(0);
+    if (ekin < R(m.iv_min_energy)) return R(0);
+    const double E2 = m.efield[0] * m.efield[0] + m.efield[1] * m.efield[1] + m.efield[2] * m.efield[2];
+    if (m.iv_model == 0) {
+        const double e0 = m.iv_quad_field;
+        return R(m.iv_quad_rate * pow(e0 * e0 + E2, 0.5 * m.iv_quad_power));
+    }
+    const double evcm = sqrt(E2) * 0.01;
+    return R(m.iv_lin_rate0 + m.iv_lin_rate1 * pow(evcm, m.iv_lin_power));
+}
+
+template <class P> struct CRates {
+    typename P::R luke, iv, trap, ion_e, ion_h, tot;
+};
+template <class P> __device__ inline typename P::R mfp_rate(double mfp, typename P::R v) {
+    using R = typename P::R;
+    return (mfp > 0.0 && mfp < 1e20) ? (v / R(mfp)) : R(0);
+}
+// engine.hpp:876-886
+template <class P> __device__ inline CRates<P> charge_rates(const DevModel& m, const CTrack<P>& t) {
+    using R = typename P::R;
+    const R v = norm<P>(charge_velocity<P>(m, t));
+    const int h = (t.type == 1) ? 0 : 1;
+    CRates<P> r;
+    r.luke = luke_rate<P>(m, t);
+    r.iv = (t.type == 1) ? iv_rate<P>(m, t.ekin) : R(0);
+    r.trap = mfp_rate<P>(m.trap_mfp[h], v);
+    r.ion_e = mfp_rate<P>(m.ion_e_mfp[h], v);
+    r.ion_h = mfp_rate<P>(m.ion_h_mfp[h], v);
+    r.tot = r.luke + r.iv + r.trap + r.ion_e + r.ion_h;
+    return r;
+}
+
+// charge_transport.hpp:109-166. Returns true if a phonon was emitted.
+template <class P>
+__device__ inline bool emit_luke(const DevModel& m, CTrack<P>& t, Rng& r, V3<typename P::R>& q_dir, typename P::R& e_ph_out) {
+    using R = typename P::R;
+    V3<R> k_trk;
+    R k_sound;
+    if (t.type == 1) {
+        k_trk = ell_to_sph<P>(m, t.valley, vdiv<P>(t.p, R(kHbarc)));
+        k_sound = R(m.k_sound_e);
+    } else {
+        k_trk = vdiv<P>(t.p, R(kHbarc));
+        k_sound = R(m.k_sound_h);
+    }
+    const R k_mag = norm<P>(k_trk);
+    if (k_mag <= k_sound * R(1.0001)) return false;
+    const V3<R> k_dir = vdiv<P>(k_trk, k_mag);
+    const R v = k_sound / k_mag;
+    const R u = P::uniform(r);
+    const R cos_th = tclamp(v + (R(1) - v) * P::cbrt_(R(1) - u), v, R(1));
+    const R sin_th = P::sqrt_(tmax(R(0), R(1) - cos_th * cos_th));
+    R s_phi, c_phi;
+    P::sincos_2pi(P::uniform(r), s_phi, c_phi);
+    const R q = R(2) * (k_mag * cos_th - k_sound);
+    if (q <= R(0)) return false;
+    const V3<R> perp = orthogonal_unit<P>(k_dir);
+    const V3<R> qv = rotate_around(rotate_around(k_dir * q, perp, cos_th, sin_th), k_dir, c_phi, s_phi);
+    const V3<R> k_rec = k_trk - qv;
+    const R q_mag = norm<P>(qv);
+    const R e_ph = q_mag * R(m.v_sound) * R(kHbar);
+    if (e_ph >= t.ekin || e_ph <= R(0)) return false;
+    if (t.type == 2) {
+        t.p = k_rec * R(kHbarc);
+        t.ekin = ekin_h<P>(m, t.p);
+        q_dir = unit<P>(qv);
+    } else {
+        q_dir = unit<P>(sph_to_ell<P>(m, t.valley, qv));
+        const V3<R> prd = sph_to_ell<P>(m, t.valley, k_rec) * R(kHbarc);
+        const R e_rec = t.ekin - e_ph;
+        t.p = ekin_to_p_e<P>(m, t.valley, prd, e_rec);
+        t.ekin = e_rec;
+    }
+    e_ph_out = e_ph;
+    return true;
+}
+
+// charge_transport.hpp:169-184
+template <class P> __device__ inline void scatter_iv(const DevModel& m, CTrack<P>& t, Rng& r) {
+    using R = typename P::R;
+    const int nv = m.n_valleys;
+    if (t.type != 1 || nv < 2) return;
+    const V3<R> kv = to_valley<P>(m, t.valley, vdiv<P>(t.p, R(kHbarc)));
+    const uint64_t offset = 1 + static_cast<uint64_t>(P::uniform(r) * R(nv - 1));
+    const int nvalley = static_cast<int>((static_cast<uint64_t>(t.valley) + offset) % static_cast<uint64_t>(nv));
+    const V3<R> pnd = from_valley<P>(m, nvalley, kv) * R(kHbarc);
+    t.valley = nvalley;
+    t.p = ekin_to_p_e<P>(m, nvalley, pnd, t.ekin);
+}
+
+// charge_transport.hpp:187-199
+template <class P> __device__ inline bool should_absorb(const CTrack<P>& t, V3<typename P::R> n, const DevSurface& s, Rng& r) {
+    using R = typename P::R;
+    const V3<R> out{n.x * R(-1), n.y * R(-1), n.z * R(-1)};
+    const R kp = dot(vdiv<P>(t.p, R(kHbarc)), out);
+    const R mk = R((t.type == 1) ? s.min_k_elec : s.min_k_hole);
+    if (kp <= mk) return false;
+    return P::uniform(r) < R(s.prob_charge_abs);
+}
+
+// charge_transport.hpp:203-241
+template <class P> __device__ inline void reflect_charge(const DevModel& m, CTrack<P>& t, V3<typename P::R> n, double spec, Rng& r) {
+    using R = typename P::R;
+    if (P::uniform(r) < R(spec)) {
+        const V3<R> kd = unit<P>(t.p);
+        const V3<R> kr = unit<P>(kd - n * (R(2) * dot(kd, n)));
+        if (t.type == 2) { t.p = kr * norm<P>(t.p); return; }
+        const V3<R> pt = ekin_to_p_e<P>(m, t.valley, kr, t.ekin);
+        if (dot(vel_e<P>(m, t.valley, pt), n) > R(1e-6)) { t.p = pt; return; }
+    }
+    if (t.type == 2) {
+        const V3<R> d = lambertian_dir<P>(r, n);
+        t.p = d * norm<P>(t.p);
+    } else {
+        for (int trial = 0; trial < 16; ++trial) {
+            const V3<R> d = lambertian_dir<P>(r, n);
+            const V3<R> pt = ekin_to_p_e<P>(m, t.valley, d, t.ekin);
+            if (dot(vel_e<P>(m, t.valley, pt), n) > R(1e-6)) { t.p = pt; return; }
+        }
+        t.p = ekin_to_p_e<P>(m, t.valley, n, t.ekin);
+    }
+}
+
+// One pass of the charge step loop. Returns false when the carrier ends (collected,
+// trapped) or exceeds the 20,000,000-step cap (the caller counts steps).
+template <class P, class Ctx>
+__device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, Rng& r, Ctx& ctx) {
+    using R = typename P::R;
+    ctx.ccount(CC_STEPS);
+    const V3<R> E{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
+    const R q_sign = (trk.type == 1) ? R(-1) : R(1);
+    const CRates<P> R0 = charge_rates<P>(m, trk);
+    R dt_max = R(m.dt_cap);
+    if (m.dk_frac > 0.0) {
+        const bool is_e = (trk.type == 1);
+        const R ks = R(is_e ? m.k_sound_e : m.k_sound_h);
+        const R k_now = is_e ? norm<P>(ell_to_sph<P>(m, trk.valley, vdiv<P>(trk.p, R(kHbarc)))) : norm<P>(trk.p) / R(kHbarc);
+        const R dk_rate = norm<P>(E) * R(kCLight) / R(kHbarc) * (is_e ? R(m.hv_max_scale) : R(1));
+        if (dk_rate > R(0)) dt_max = tmin(dt_max, R(m.dk_frac) * tmax(ks, k_now) / dk_rate);
+    }
+    const R dt_phys = (R0.tot > R(1)) ? (P::neg_log(P::uniform(r)) / R0.tot) : P::inf();
+    const R dt = tmin(dt_phys, dt_max);
+    const V3<R> dp = E * (q_sign * R(kCLight) * dt);
+    const V3<R> pm = trk.p + dp * R(0.5);
+    const V3<R> vm = (trk.type == 1) ? vel_e<P>(m, trk.valley, pm) : vel_h<P>(m, pm);
+    const R vmag = norm<P>(vm);
+    const R d_step = vmag * dt;
+    if (vmag > R(1e-6)) {
+        const V3<R> vdir = vdiv<P>(vm, vmag);
+        V3<R> n_in;
+        int sid;
+        const R dist = intersect_exit<P>(m, trk.pos, vdir, n_in, sid);
+        if (dist <= d_step) {
+            const R dtb = dist / vmag;
+            trk.pos = trk.pos + vdir * dist;
+            trk.time += static_cast<double>(dtb);
+            trk.p = trk.p + E * (q_sign * R(kCLight) * dtb);
+            trk.ekin = (trk.type == 1) ? ekin_e<P>(m, trk.valley, trk.p) : ekin_h<P>(m, trk.p);
+            const DevSurface& s = m.surf[sid];
+            if (!should_absorb<P>(trk, n_in, s, r)) {
+                ctx.ccount(CC_SURF_REFL);
+                trk.pos = trk.pos + n_in * R(1e-11);
+                reflect_charge<P>(m, trk, n_in, s.prob_charge_spec, r);
+                return true;
+            }
+            const double muF = (m.fermi >= 0.0) ? m.fermi : 0.5 * m.bandgap;
+            const double e_rel = static_cast<double>(trk.ekin) + ((trk.type == 1) ? (m.bandgap - muF) : muF);
+            ctx.ccount(CC_COLLECTED);
+            const double recomb = e_rel * trk.weight;
+            ctx.recomb(recomb);
+            ctx.deposited(recomb);
+            ctx.charge_hit(trk, recomb, e_rel, sid);
+            return false;
+        }
+    }
+    trk.pos = trk.pos + vm * dt;
+    trk.time += static_cast<double>(dt);
+    trk.p = trk.p + dp;
+    trk.ekin = (trk.type == 1) ? ekin_e<P>(m, trk.valley, trk.p) : ekin_h<P>(m, trk.p);
+    if (dt_phys <= dt_max) {
+        const R u_proc = P::uniform(r) * R0.tot;
+        if (u_proc < R0.trap) {
+            ctx.ccount(CC_TRAPPED);
+            ctx.escaped((static_cast<double>(trk.ekin) + 0.5 * m.bandgap) * trk.weight);
+            return false;
+        } else if (u_proc < R0.trap + R0.ion_e + R0.ion_h) {
+            ctx.ccount(CC_TRAP_ION);
+            const bool lib_e = (u_proc < R0.trap + R0.ion_e);
+            const uint64_t nval = (m.n_valleys > 1) ? static_cast<uint64_t>(m.n_valleys) : 1;
+            const int kv = lib_e ? static_cast<int>(static_cast<uint16_t>(next_u64(r) % nval)) : 0;
+            const V3<R> d = isotropic_dir<P>(r);
+            CTrack<P> k;
+            make_charge<P>(m, k, lib_e ? 1 : 2, trk.pos, d, R(1.0e-6), kv, trk.time);
+            k.weight = trk.weight;
+            ctx.push_charge(k, trk);
+        } else if (u_proc < R0.trap + R0.ion_e + R0.ion_h + R0.iv) {
+            scatter_iv<P>(m, trk, r);
+            ctx.ccount(CC_IV);
+        } else if (R0.luke > R(0)) {
+            V3<R> qd;
+            R eph;
+            if (emit_luke<P>(m, trk, r, qd, eph)) {
+                ctx.ccount(CC_LUKE);
+                ctx.luke_eV(static_cast<double>(eph) * trk.weight);
+                const double lp = tclamp(m.luke_fraction, 0.0, 1.0);
+                if (lp > 0.0 && (lp >= 1.0 || P::uniform(r) < R(lp))) {
+                    const int md = sample_dos_mode<P>(m, r);
+                    PTrack<P> ph;
+                    make_track<P>(m, ph, trk.pos, qd, md, static_cast<double>(eph), trk.time, trk.weight * (1.0 / lp));
+                    ctx.push_luke(ph, trk);
+                }
+            }
+        }
+    }
+    return true;
+}
+
+}  // namespace cmpmc_dev

@@ -1,0 +1,135 @@
+// Pieces shared by the transport kernels: the deterministic energy ledger, the global
+// counter layout, device hit records and the device-side workload generator.
+#pragma once
+
+#include "charge_step.hpp"
+
+namespace cmpmc_dev {
+
+// Global counters (uint64): phonon counters C_* then charge counters CC_* .
+constexpr int kNumCounters = C_NCOUNTERS + CC_N;
+
+// Ledger slots: deposited, escaped/killed, Luke emitted, charge recombination, injected.
+enum Ledger : int { L_DEP = 0, L_ESC, L_LUKE, L_RECOMB, L_INJ, L_N };
+
+// Fixed-point energy ledger: every increment is rounded once to 2^-44 eV and summed as
+// an integer, so totals are independent of the order in which lanes and tracks finish.
+// Global slots hold (hi, lo) pairs: value = hi * 2^20 + lo (in units of 2^-44 eV).
+constexpr double kLedgerScale = 17592186044416.0;   // 2^44
+struct LaneLedger {
+    long long v[L_N];
+    __device__ void reset() {
+        for (int i = 0; i < L_N; ++i) v[i] = 0;
+    }
+    __device__ void add(int slot, double eV) { v[slot] += __double2ll_rn(eV * kLedgerScale); }
+    __device__ void flush(unsigned long long* g) {
+        for (int i = 0; i < L_N; ++i) {
+            if (v[i] == 0) continue;
+            const unsigned long long q = static_cast<unsigned long long>(v[i]);
+            atomicAdd(&g[2 * i], q >> 20);
+            atomicAdd(&g[2 * i + 1], q & 0xFFFFFull);
+            v[i] = 0;
+        }
+    }
+};
+
+// Hit as produced on the device, with its ordering key.
+struct DevHit {
+    uint64_t k0;    // plain form: event << 32 | seq;   batched form: event
+    uint64_t k1;    // plain form: 0;                   batched form: track key
+    uint32_t k2;    // batched form: hit sequence number on the track
+    uint32_t _pad;
+    cmpmc_sensor_hit h;
+};
+
+template <class P> __device__ inline void fill_hit(cmpmc_sensor_hit& h, V3<typename P::R> pos, double time, double edep,
+                                                   double incident, int surface, int excitation, int mode) {
+    h.pos[0] = static_cast<double>(pos.x);
+    h.pos[1] = static_cast<double>(pos.y);
+    h.pos[2] = static_cast<double>(pos.z);
+    h.time = time;
+    h.edep = edep;
+    h.incident_energy = incident;
+    h.surface_id = static_cast<uint8_t>(surface);
+    h.excitation = static_cast<uint8_t>(excitation);
+    h.mode = static_cast<uint8_t>(mode);
+    h._pad = 0;
+    h.channel_id = -1;
+}
+
+// ------------------------------------------------------------------ device generators
+// Documented draw orders (the reference leaves the generator to the user):
+//   POINT: k = isotropic_dir(event stream).
+//   LOGN:  n = clip(llround(exp(ln median + sigma * normal())), n_min, n_max) from the event
+//          stream; then per phonon j: r = R sqrt(u), phi = 2 pi u, z = H (2u - 1), mode =
+//          sample_dos_mode, E = e_lo + (e_hi - e_lo) u, k = isotropic_dir.
+//   UNIF:  n = n_min + floor(u (n_max - n_min + 1)); per phonon: position as LOGN, k.
+//   PAIR:  position (unless at origin), valley = floor(u n_valleys) mod n_valleys,
+//          electron k, hole k (cmpmc_bench.cpp:274-276 order).
+// Plain form: every draw comes from the event stream, in that order. Batched form: the
+// per-phonon draws of LOGN/UNIF come from the generator stream of track j,
+// rng_for_track(base, j | 2^62), and transport then draws from rng_for_track(base, j).
+__device__ inline uint32_t workload_count(const cmpmc_workload& w, Rng& ev) {
+    if (w.kind == CMPMC_WL_PHONON_POINT) return 1;
+    if (w.kind == CMPMC_WL_PHONON_LOGN) {
+        const double u1 = Fp64::uniform(ev);
+        const double u2 = Fp64::uniform(ev);
+        const double g = sqrt(-2.0 * log(u1)) * cos(kTwoPi * u2);
+        const double v = exp(log(w.logn_median) + w.logn_sigma * g);
+        if (!(v < static_cast<double>(w.n_max))) return w.n_max;
+        if (v < static_cast<double>(w.n_min)) return w.n_min;
+        uint32_t n = static_cast<uint32_t>(llround(v));
+        return tclamp(n, w.n_min, w.n_max);
+    }
+    if (w.kind == CMPMC_WL_PHONON_UNIF) {
+        uint32_t n = w.n_min + static_cast<uint32_t>(Fp64::uniform(ev) * static_cast<double>(w.n_max - w.n_min + 1));
+        return tmin(n, w.n_max);
+    }
+    return 0;
+}
+
+template <class P> __device__ inline V3<typename P::R> uniform_volume(const DevModel& m, Rng& r) {
+    using R = typename P::R;
+    const R rr = R(m.radius) * P::sqrt_(P::uniform(r));
+    R s, c;
+    P::sincos_2pi(P::uniform(r), s, c);
+    const R z = R(m.half_z) * (R(2) * P::uniform(r) - R(1));
+    return {rr * c, rr * s, z};
+}
+
+// Initial phonon j of an event (after workload_count consumed its draws from `ev`).
+template <class P>
+__device__ inline void workload_phonon(const DevModel& m, const cmpmc_workload& w, Rng& r, PTrack<P>& t, double& E) {
+    using R = typename P::R;
+    if (w.kind == CMPMC_WL_PHONON_POINT) {
+        const V3<R> k = isotropic_dir<P>(r);
+        const V3<R> o{R(w.origin[0]), R(w.origin[1]), R(w.origin[2])};
+        E = w.energy_eV;
+        make_track<P>(m, t, o, k, w.mode, E, 0.0, 1.0);
+        return;
+    }
+    const V3<R> pos = uniform_volume<P>(m, r);
+    int md = w.mode;
+    E = w.energy_eV;
+    if (w.kind == CMPMC_WL_PHONON_LOGN) {
+        md = sample_dos_mode<P>(m, r);
+        E = w.e_lo_eV + (w.e_hi_eV - w.e_lo_eV) * static_cast<double>(P::uniform(r));
+    }
+    const V3<R> k = isotropic_dir<P>(r);
+    make_track<P>(m, t, pos, k, md, E, 0.0, 1.0);
+}
+
+template <class P> __device__ inline void workload_pair(const DevModel& m, const cmpmc_workload& w, Rng& r,
+                                                        CTrack<P>& e, CTrack<P>& h) {
+    using R = typename P::R;
+    V3<R> pos{R(0), R(0), R(0)};
+    if (!w.pair_at_origin) pos = uniform_volume<P>(m, r);
+    const int nv = m.n_valleys;
+    const int valley = static_cast<int>(static_cast<uint16_t>(P::uniform(r) * R(nv)) % nv);
+    const V3<R> de = isotropic_dir<P>(r);
+    make_charge<P>(m, e, 1, pos, de, R(1e-6), valley, 0.0);
+    const V3<R> dh = isotropic_dir<P>(r);
+    make_charge<P>(m, h, 2, pos, dh, R(1e-6), 0, 0.0);
+}
+
+}  // namespace cmpmc_dev

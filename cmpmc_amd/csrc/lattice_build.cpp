@@ -1,0 +1,301 @@
+// Host-side CrystalLattice table build (lattice/crystal_lattice.hpp:45-55, 340-477) and
+// host map_k_to_vg, exported through the C ABI. Compiled without FMA contraction:
+// the only libm call is sqrt (correctly rounded), so the tables are bit-identical to
+// the reference's for the same CrystalConfig.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cmpmc_amd.h"
+#include "capi_internal.h"
+
+namespace {
+
+struct V {
+    double x, y, z;
+};
+inline double dot(const V& a, const V& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double nrm2(const V& a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+inline V unit(const V& a) {
+    const double n2 = nrm2(a);
+    if (n2 <= 0.0) return {0.0, 0.0, 1.0};
+    const double inv = 1.0 / std::sqrt(n2);
+    return {a.x * inv, a.y * inv, a.z * inv};
+}
+inline double comp(const V& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+inline double dmax(double a, double b) { return (a < b) ? b : a; }
+
+using Stiff = double[3][3][3][3];
+
+// Cubic Voigt matrix -> C_ijlm (crystal_config.hpp:181-191; crystal_lattice.hpp:333-348)
+void stiffness(const cmpmc_crystal& c, Stiff C) {
+    double Vm[6][6] = {};
+    for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j) {
+            if (i < 3 && j < 3) Vm[i][j] = (i == j) ? c.C11 : c.C12;
+            else if (i == j) Vm[i][i] = c.C44;
+        }
+    for (int i = 0; i < 6; ++i)
+        for (int j = i + 1; j < 6; ++j) Vm[j][i] = Vm[i][j];
+    auto voigt = [](int i, int j) {
+        if (i == j) return i;
+        if (i + j == 3) return 3;   // (1,2) / (2,1)
+        if (i + j == 2) return 4;   // (0,2) / (2,0)
+        return 5;                   // (0,1) / (1,0)
+    };
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int l = 0; l < 3; ++l)
+                for (int m = 0; m < 3; ++m) C[i][j][l][m] = Vm[voigt(i, j)][voigt(l, m)];
+}
+
+// Christoffel eigen-solve by cyclic Jacobi sweeps and group velocities
+// (phonon/phonon_kernels.hpp:134-247): v_g,d = e_i C_ijld (k_j / v_p) e_l / rho.
+void christoffel(const Stiff C, double rho, const V& k, V vg[3]) {
+    const double inv_rho = 1.0 / rho;
+    double D[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int l = 0; l < 3; ++l) {
+            double s = 0.0;
+            for (int j = 0; j < 3; ++j)
+                for (int m = 0; m < 3; ++m) s += C[i][j][l][m] * comp(k, j) * comp(k, m);
+            D[i][l] = s * inv_rho;
+        }
+    double Q[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 12; ++sweep) {
+        const double off = std::fabs(D[0][1]) + std::fabs(D[0][2]) + std::fabs(D[1][2]);
+        if (off < 1e-15 * (std::fabs(D[0][0]) + std::fabs(D[1][1]) + std::fabs(D[2][2]))) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (std::fabs(D[p][q]) < 1e-18) continue;
+                const double tau = (D[q][q] - D[p][p]) / (2.0 * D[p][q]);
+                const double t = (tau >= 0.0) ? (1.0 / (tau + std::sqrt(1.0 + tau * tau)))
+                                              : (-1.0 / (-tau + std::sqrt(1.0 + tau * tau)));
+                const double c = 1.0 / std::sqrt(1.0 + t * t);
+                const double s = t * c;
+                const double a = D[p][p], b = D[q][q], g = D[p][q];
+                D[p][p] = c * c * a - 2.0 * s * c * g + s * s * b;
+                D[q][q] = s * s * a + 2.0 * s * c * g + c * c * b;
+                D[p][q] = D[q][p] = 0.0;
+                for (int r = 0; r < 3; ++r) {
+                    if (r != p && r != q) {
+                        const double x = D[r][p], y = D[r][q];
+                        D[r][p] = D[p][r] = c * x - s * y;
+                        D[r][q] = D[q][r] = s * x + c * y;
+                    }
+                    const double x = Q[r][p], y = Q[r][q];
+                    Q[r][p] = c * x - s * y;
+                    Q[r][q] = s * x + c * y;
+                }
+            }
+    }
+    const double ev[3] = {D[0][0], D[1][1], D[2][2]};
+    const V vec[3] = {{Q[0][0], Q[1][0], Q[2][0]}, {Q[0][1], Q[1][1], Q[2][1]}, {Q[0][2], Q[1][2], Q[2][2]}};
+    // L = polarisation most parallel to k; the others ordered fast, slow by eigenvalue
+    int il = 0;
+    double best = -1.0;
+    for (int i = 0; i < 3; ++i) {
+        const double d = std::fabs(dot(vec[i], k));
+        if (d > best) { best = d; il = i; }
+    }
+    const int r1 = (il + 1) % 3, r2 = (il + 2) % 3;
+    const int order[3] = {il, ev[r1] >= ev[r2] ? r1 : r2, ev[r1] >= ev[r2] ? r2 : r1};
+    for (int mo = 0; mo < 3; ++mo) {
+        const int idx = order[mo];
+        const double vp = std::sqrt(dmax(ev[idx], 1.0));
+        const double inv = 1.0 / vp;
+        const V sl{k.x * inv, k.y * inv, k.z * inv};
+        const V& e = vec[idx];
+        double out[3];
+        for (int d = 0; d < 3; ++d) {
+            double s = 0.0;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                    for (int l = 0; l < 3; ++l) s += comp(e, i) * C[i][j][l][d] * comp(sl, j) * comp(e, l);
+            out[d] = s * inv_rho;
+        }
+        vg[mo] = {out[0], out[1], out[2]};
+    }
+}
+
+// Octahedral parameterisation (crystal_lattice.hpp:117-141)
+V oct_direction(double u, double v) {
+    const double kz = 1.0 - std::fabs(u) - std::fabs(v);
+    double kx = u, ky = v;
+    if (kz < 0.0) {
+        kx = (1.0 - std::fabs(v)) * (u >= 0.0 ? 1.0 : -1.0);
+        ky = (1.0 - std::fabs(u)) * (v >= 0.0 ? 1.0 : -1.0);
+    }
+    return unit(V{kx, ky, kz});
+}
+
+// Tamura energy-partition densities (phonon_kernels.hpp:98-129)
+double pdf_lt(const cmpmc_crystal& c, double x) {
+    const double m = c.v_sound / c.v_trans;
+    if (x <= (m - 1.0) / (m + 1.0) || x >= 1.0) return 0.0;
+    const double a = 1.0 - x * x;
+    const double b = (1.0 + x) * (1.0 + x) - m * m * (1.0 - x) * (1.0 - x);
+    const double cc = 1.0 + x * x - m * m * (1.0 - x) * (1.0 - x);
+    return (a * a / (x * x)) * b * cc * cc;
+}
+double pdf_tt(const cmpmc_crystal& c, double x) {
+    const double m = c.v_sound / c.v_trans;
+    if (x <= 0.5 * (1.0 - 1.0 / m) || x >= 0.5 * (1.0 + 1.0 / m)) return 0.0;
+    const double be = c.beta / 1e11, ga = c.gamma / 1e11, la = c.lambda / 1e11, mu = c.mu / 1e11;
+    const double A = 0.5 * (1.0 - m * m) * (be + la + (1.0 + m * m) * (ga + mu));
+    const double B = be + la + 2.0 * m * m * (ga + mu);
+    const double Cc = be + la + 2.0 * (ga + mu);
+    const double Dd = (1.0 - m * m) * (2.0 * be + 4.0 * ga + la + 3.0 * mu);
+    const double X = m * x;
+    const double t1 = A + B * m * X - B * X * X;
+    const double t2 = Cc * X * (m - X) - Dd / (m - X) * (X - m - (1.0 - m * m) / (4.0 * X));
+    return t1 * t1 + t2 * t2;
+}
+
+// Inverse CDF on 1024 nodes from a 4096-midpoint cumulative sum (crystal_lattice.hpp:448-477)
+template <class F> void inverse_cdf(double* inv, double lo, double hi, F pdf) {
+    constexpr int NI = 4096;
+    std::vector<double> cdf(NI + 1, 0.0);
+    const double dx = (hi - lo) / NI;
+    double sum = 0.0;
+    for (int i = 0; i < NI; ++i) {
+        sum += dmax(0.0, pdf(lo + (i + 0.5) * dx)) * dx;
+        cdf[i + 1] = sum;
+    }
+    if (sum <= 0.0) {
+        for (int k = 0; k < CMPMC_N_CDF; ++k) inv[k] = 0.5 * (lo + hi);
+        return;
+    }
+    for (double& c : cdf) c /= sum;
+    inv[0] = lo;
+    inv[CMPMC_N_CDF - 1] = hi;
+    int j = 0;
+    for (int k = 1; k < CMPMC_N_CDF - 1; ++k) {
+        const double target = static_cast<double>(k) / (CMPMC_N_CDF - 1);
+        while (j < NI && cdf[j + 1] < target) ++j;
+        const double den = cdf[j + 1] - cdf[j];
+        const double t = (den > 1e-16) ? ((target - cdf[j]) / den) : 0.0;
+        inv[k] = lo + (j + t) * dx;
+    }
+}
+
+}  // namespace
+
+extern "C" int cmpmc_lattice_build(const cmpmc_crystal* cfg, float* oct, double* cdf_lt, double* cdf_tt,
+                                   cmpmc_lattice* out) {
+    if (!cfg || !oct || !cdf_lt || !cdf_tt || !out)
+        return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_lattice_build: null argument");
+    if (cfg->n_valleys < 0 || cfg->n_valleys > 8)
+        return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_lattice_build: n_valleys must be in [0, 8]");
+    if (!(cfg->density > 0.0) || !(cfg->v_sound > 0.0) || !(cfg->v_trans > 0.0))
+        return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_lattice_build: density and sound speeds must be positive");
+    std::memset(out, 0, sizeof(*out));
+    out->cfg = *cfg;
+    Stiff C;
+    stiffness(*cfg, C);
+    for (int i = 0; i < CMPMC_N_OCT; ++i) {
+        const double u = -1.0 + 2.0 * (static_cast<double>(i) / (CMPMC_N_OCT - 1));
+        for (int j = 0; j < CMPMC_N_OCT; ++j) {
+            const double v = -1.0 + 2.0 * (static_cast<double>(j) / (CMPMC_N_OCT - 1));
+            V k = oct_direction(u, v);
+            if (std::fabs(k.x) < 1e-6 && std::fabs(k.y) < 1e-6) k = unit(V{1e-6, 0.0, k.z});
+            V vg[3];
+            christoffel(C, cfg->density, k, vg);
+            for (int m = 0; m < 3; ++m) {
+                float* e = oct + ((static_cast<size_t>(m) * CMPMC_N_OCT + i) * CMPMC_N_OCT + j) * 4;
+                e[0] = static_cast<float>(vg[m].x);
+                e[1] = static_cast<float>(vg[m].y);
+                e[2] = static_cast<float>(vg[m].z);
+                e[3] = static_cast<float>(std::sqrt(nrm2(vg[m])));
+            }
+        }
+    }
+    const double m = cfg->v_sound / cfg->v_trans;
+    inverse_cdf(cdf_lt, (m - 1.0) / (m + 1.0), 1.0, [cfg](double x) { return pdf_lt(*cfg, x); });
+    inverse_cdf(cdf_tt, 0.5 * (1.0 - 1.0 / m), 0.5 * (1.0 + 1.0 / m), [cfg](double x) { return pdf_tt(*cfg, x); });
+    out->oct_lut = oct;
+    out->cdf_lt = cdf_lt;
+    out->cdf_tt = cdf_tt;
+
+    // Herring-Vogt tensors (crystal_lattice.hpp:394-438)
+    constexpr double me_eV = 510998.950, me_kg = 9.1093837015e-31, e_SI = 1.602176634e-19;
+    constexpr double hbar = 6.582119569e-16, c_light = 299792458.0, hbarc = hbar * c_light;
+    const double mc = 3.0 / (1.0 / cfg->e_mass[0] + 1.0 / cfg->e_mass[1] + 1.0 / cfg->e_mass[2]);
+    out->e_cond_mass_eV = mc * me_eV;
+    out->h_mass_eV = cfg->h_mass_ratio * me_eV;
+    for (int i = 0; i < 3; ++i) out->mass_ratio[i] = cfg->e_mass[i] / mc;
+    for (int i = 0; i < 3; ++i) out->hv_sqrt[i] = std::sqrt(out->mass_ratio[i]);
+    for (int i = 0; i < 3; ++i) out->hv_sqrt_inv[i] = 1.0 / out->hv_sqrt[i];
+    for (int iv = 0; iv < cfg->n_valleys; ++iv) {
+        const V raw = unit(V{cfg->valley_dirs[iv][0], cfg->valley_dirs[iv][1], cfg->valley_dirs[iv][2]});
+        const double (*O)[3] = cfg->orient;
+        const V d = unit(V{O[0][0] * raw.x + O[0][1] * raw.y + O[0][2] * raw.z,
+                           O[1][0] * raw.x + O[1][1] * raw.y + O[1][2] * raw.z,
+                           O[2][0] * raw.x + O[2][1] * raw.y + O[2][2] * raw.z});
+        const double f = std::sqrt(dmax(0.0, 1.0 - d.z * d.z));
+        const double a = (std::fabs(d.y) < 1e-12) ? 0.0 : ((std::fabs(d.x) < 1e-12) ? -1.0 : -d.y / f);
+        const double b = (std::fabs(d.y) < 1e-12) ? 1.0 : ((std::fabs(d.x) < 1e-12) ? 0.0 : d.x / f);
+        double (*Rm)[3] = out->valley_rot[iv];
+        Rm[0][0] = d.x; Rm[0][1] = d.y; Rm[0][2] = d.z;
+        Rm[1][0] = a;   Rm[1][1] = b;   Rm[1][2] = 0.0;
+        Rm[2][0] = -d.z * b; Rm[2][1] = d.z * a; Rm[2][2] = f;
+    }
+    auto l0 = [cfg](double mass_ratio, double deform) {
+        if (deform <= 0.0) return 1e-6;
+        const double hb = hbar * e_SI, msi = mass_ratio * me_kg, ac = deform * e_SI;
+        const double hb4 = (hb * hb) * (hb * hb);
+        return 0.5 * 3.141592653589793 * hb4 * cfg->density / (msi * msi * msi * ac * ac);
+    };
+    out->l0_e = l0(mc, cfg->ac_deform_e);
+    out->l0_h = l0(cfg->h_mass_ratio, cfg->ac_deform_h);
+    out->k_sound_e = (out->e_cond_mass_eV * cfg->v_sound) / (c_light * hbarc);
+    out->k_sound_h = (out->h_mass_eV * cfg->v_sound) / (c_light * hbarc);
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_map_k_to_vg(const cmpmc_lattice* lat, int mode, uint64_t n, const double* k, double* vg_dir,
+                                 double* vg_mag) {
+    if (!lat || !lat->oct_lut || (n && (!k || !vg_dir || !vg_mag)) || mode < 0 || mode > 2)
+        return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_map_k_to_vg: bad argument");
+    const double (*O)[3] = lat->cfg.orient;
+    for (uint64_t q = 0; q < n; ++q) {
+        const V kd{k[3 * q], k[3 * q + 1], k[3 * q + 2]};
+        const V kc{O[0][0] * kd.x + O[1][0] * kd.y + O[2][0] * kd.z, O[0][1] * kd.x + O[1][1] * kd.y + O[2][1] * kd.z,
+                   O[0][2] * kd.x + O[1][2] * kd.y + O[2][2] * kd.z};
+        const double l1 = std::fabs(kc.x) + std::fabs(kc.y) + std::fabs(kc.z);
+        const double inv_l1 = (l1 > 1e-18) ? (1.0 / l1) : 1.0;
+        double px = kc.x * inv_l1, py = kc.y * inv_l1;
+        if (kc.z < 0.0) {
+            const double ox = (1.0 - std::fabs(py)) * (px >= 0.0 ? 1.0 : -1.0);
+            const double oy = (1.0 - std::fabs(px)) * (py >= 0.0 ? 1.0 : -1.0);
+            px = ox;
+            py = oy;
+        }
+        px = (px < -1.0) ? -1.0 : (1.0 < px ? 1.0 : px);
+        py = (py < -1.0) ? -1.0 : (1.0 < py ? 1.0 : py);
+        const double u = (px + 1.0) * 0.5 * (CMPMC_N_OCT - 1);
+        const double v = (py + 1.0) * 0.5 * (CMPMC_N_OCT - 1);
+        int i0 = static_cast<int>(u), j0 = static_cast<int>(v);
+        i0 = i0 < 0 ? 0 : (i0 > CMPMC_N_OCT - 2 ? CMPMC_N_OCT - 2 : i0);
+        j0 = j0 < 0 ? 0 : (j0 > CMPMC_N_OCT - 2 ? CMPMC_N_OCT - 2 : j0);
+        const float du = static_cast<float>(u - i0), dv = static_cast<float>(v - j0);
+        const float w00 = (1.0f - du) * (1.0f - dv), w10 = du * (1.0f - dv), w01 = (1.0f - du) * dv, w11 = du * dv;
+        const float* e00 = lat->oct_lut + ((static_cast<size_t>(mode) * CMPMC_N_OCT + i0) * CMPMC_N_OCT + j0) * 4;
+        const float* e10 = e00 + 4 * CMPMC_N_OCT;
+        const float* e01 = e00 + 4;
+        const float* e11 = e10 + 4;
+        const V vc{static_cast<double>(e00[0] * w00 + e10[0] * w10 + e01[0] * w01 + e11[0] * w11),
+                   static_cast<double>(e00[1] * w00 + e10[1] * w10 + e01[1] * w01 + e11[1] * w11),
+                   static_cast<double>(e00[2] * w00 + e10[2] * w10 + e01[2] * w01 + e11[2] * w11)};
+        const double mag = std::sqrt(nrm2(vc));
+        const double inv = 1.0 / dmax(mag, 1e-12);
+        const V t{vc.x * inv, vc.y * inv, vc.z * inv};
+        const V d = unit(V{O[0][0] * t.x + O[0][1] * t.y + O[0][2] * t.z, O[1][0] * t.x + O[1][1] * t.y + O[1][2] * t.z,
+                           O[2][0] * t.x + O[2][1] * t.y + O[2][2] * t.z});
+        vg_dir[3 * q] = d.x;
+        vg_dir[3 * q + 1] = d.y;
+        vg_dir[3 * q + 2] = d.z;
+        vg_mag[q] = mag;
+    }
+    return CMPMC_OK;
+}

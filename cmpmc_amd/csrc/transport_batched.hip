@@ -1,0 +1,535 @@
+// Batched form: persistent FP32 kernels over SoA-free register-resident tracks.
+//
+// Each lane owns one track at a time and steps it with FP32 kinematics (FP64 energy,
+// weight, time and ledger).  A lane whose track ends pops its own spawn stack (decay,
+// film re-emission and surface down-conversion daughters, in HBM); when that is empty
+// it takes new tracks from the global initial queue.  Refills are wave-aggregated: one
+// ballot counts the idle lanes of the wave and a single atomic claims that many queue
+// entries, every `refill_period` steps (or at once when the whole wave is idle).
+// Every track draws from its own stream rng_for_track(event_base(seed, event), key), so
+// results do not depend on which lane runs which track, or when.
+#include "kernels_common.hpp"
+#include "transport_kernels.h"
+
+namespace cmpmc_dev {
+
+using P = Fp32;
+
+__device__ inline void btrack_to_ptrack(const BTrack& b, uint64_t seed, PTrack<P>& t, uint64_t& event) {
+    t.pos = {b.pos[0], b.pos[1], b.pos[2]};
+    t.k = {b.k[0], b.k[1], b.k[2]};
+    t.vg = {b.vg[0], b.vg[1], b.vg[2]};
+    t.vg_mag = b.vg_mag;
+    t.energy = b.energy;
+    t.time = b.time;
+    t.weight = b.weight;
+    t.bounces = b.bounces_mode >> 8;
+    t.mode = static_cast<int>(b.bounces_mode & 0xFF);
+    t.key = b.key;
+    t.spawn_no = 0;
+    t.hit_seq = 0;
+    event = b.event;
+    t.rng = rng_for_track(event_base(seed, event), b.key);
+}
+
+template <class T> __device__ inline void ptrack_to_btrack(const PTrack<T>& t, uint64_t event, BTrack& b) {
+    b.pos[0] = static_cast<float>(t.pos.x); b.pos[1] = static_cast<float>(t.pos.y); b.pos[2] = static_cast<float>(t.pos.z);
+    b.k[0] = static_cast<float>(t.k.x); b.k[1] = static_cast<float>(t.k.y); b.k[2] = static_cast<float>(t.k.z);
+    b.vg[0] = static_cast<float>(t.vg.x); b.vg[1] = static_cast<float>(t.vg.y); b.vg[2] = static_cast<float>(t.vg.z);
+    b.vg_mag = static_cast<float>(t.vg_mag);
+    b.energy = t.energy;
+    b.time = t.time;
+    b.weight = t.weight;
+    b.key = t.key;
+    b.event = static_cast<uint32_t>(event);
+    b.bounces_mode = (t.bounces << 8) | static_cast<uint32_t>(t.mode);
+}
+
+// Block-level flush of per-lane counters and ledger into the global accumulators.
+struct BlockAccum {
+    unsigned long long cnt[kNumCounters];
+    unsigned long long led[2 * L_N];
+};
+
+__device__ inline void block_accum_init(BlockAccum& s) {
+    for (int i = threadIdx.x; i < kNumCounters; i += blockDim.x) s.cnt[i] = 0;
+    for (int i = threadIdx.x; i < 2 * L_N; i += blockDim.x) s.led[i] = 0;
+}
+__device__ inline void lane_to_block(BlockAccum& s, uint32_t* cnt, LaneLedger& led) {
+    for (int i = 0; i < kNumCounters; ++i)
+        if (cnt[i]) { atomicAdd(&s.cnt[i], static_cast<unsigned long long>(cnt[i])); cnt[i] = 0; }
+    led.flush(s.led);
+}
+__device__ inline void block_to_global(const BlockAccum& s, unsigned long long* counters, unsigned long long* ledger) {
+    for (int i = threadIdx.x; i < kNumCounters; i += blockDim.x)
+        if (s.cnt[i]) atomicAdd(&counters[i], s.cnt[i]);
+    for (int i = threadIdx.x; i < 2 * L_N; i += blockDim.x)
+        if (s.led[i]) atomicAdd(&ledger[i], s.led[i]);
+}
+
+struct BatchCtx {
+    uint32_t cnt[kNumCounters];
+    LaneLedger led;
+    BTrack* ls;
+    uint32_t sp, depth, lanes, lane;
+    uint32_t need;
+    bool overflow;
+    uint64_t event;
+    DevHit* hits;
+    unsigned long long* hit_count;
+    uint64_t hit_cap;
+    bool keep_hits;
+    int32_t* first_branch;
+    uint64_t first_event;
+    bool rec;
+    KList kl[5];
+    const double* lt;
+    const double* tt;
+
+    __device__ void count(Counter c) { ++cnt[c]; }
+    __device__ void ccount(CCounter c) { ++cnt[C_NCOUNTERS + c]; }
+    __device__ void escaped(double v) { led.add(L_ESC, v); }
+    __device__ void deposited(double v) { led.add(L_DEP, v); }
+    __device__ void luke_eV(double v) { led.add(L_LUKE, v); }
+    __device__ void recomb(double v) { led.add(L_RECOMB, v); }
+    __device__ void branch(int code) {
+        if (rec) {
+            rec = false;
+            if (first_branch) first_branch[event - first_event] = code;
+        }
+    }
+    __device__ const double* cdf_lt() const { return lt; }
+    __device__ const double* cdf_tt() const { return tt; }
+    __device__ KList* lists() { return kl; }
+    __device__ void push(PTrack<P>& child, PTrack<P>& parent) {
+        child.key = child_key(parent.key, ++parent.spawn_no);
+        if (sp >= depth) {               // no room: the daughter is lost (status reports it)
+            overflow = true;
+            need = tmax(need, sp + 1);
+            led.add(L_ESC, child.energy * child.weight);
+            return;
+        }
+        ptrack_to_btrack(child, event, ls[static_cast<size_t>(sp) * lanes + lane]);
+        ++sp;
+        need = tmax(need, sp);
+    }
+    __device__ void hit(PTrack<P>& t, double wed, int sid) {
+        const uint32_t seq = t.hit_seq++;
+        if (!keep_hits) return;
+        const unsigned long long i = atomicAdd(hit_count, 1ull);
+        if (i < hit_cap) {
+            DevHit& d = hits[i];
+            d.k0 = event;
+            d.k1 = t.key;
+            d.k2 = seq;
+            fill_hit<P>(d.h, t.pos, t.time, wed, t.energy, sid, 0, t.mode);
+        }
+    }
+};
+
+__global__ void __launch_bounds__(256) transport_batched_kernel(const DevModel m, const BatchedArgs a) {
+    __shared__ double s_cdf[2 * CMPMC_N_CDF];
+    __shared__ BlockAccum s_acc;
+    for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
+        s_cdf[i] = m.cdf_lt[i];
+        s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
+    }
+    block_accum_init(s_acc);
+    __syncthreads();
+
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    BatchCtx ctx;
+    for (int i = 0; i < kNumCounters; ++i) ctx.cnt[i] = 0;
+    ctx.led.reset();
+    ctx.ls = a.lstack;
+    ctx.sp = 0;
+    ctx.depth = a.depth;
+    ctx.lanes = a.lanes;
+    ctx.lane = lane;
+    ctx.need = 0;
+    ctx.overflow = false;
+    ctx.event = 0;
+    ctx.hits = a.hits;
+    ctx.hit_count = a.hit_count;
+    ctx.hit_cap = a.hit_cap;
+    ctx.keep_hits = a.keep_hits != 0;
+    ctx.first_branch = a.first_branch;
+    ctx.first_event = a.first_event;
+    ctx.rec = false;
+    for (int i = 0; i < 5; ++i) {
+        ctx.kl[i].base = a.kaplan_ws + static_cast<size_t>(i) * CMPMC_KCAP * a.lanes + lane;
+        ctx.kl[i].stride = a.lanes;
+        ctx.kl[i].n = 0;
+        ctx.kl[i].cap = CMPMC_KCAP;
+    }
+    ctx.lt = s_cdf;
+    ctx.tt = s_cdf + CMPMC_N_CDF;
+
+    PTrack<P> t;
+    bool alive = false;
+    bool exhausted = false;
+    const uint32_t period = a.refill_period ? a.refill_period : 1;
+    const uint64_t lt_mask = (1ull << __lane_id()) - 1ull;
+    for (uint32_t iter = 0;; ++iter) {
+        if (!alive && ctx.sp > 0) {            // own spawn stack first (LIFO)
+            --ctx.sp;
+            btrack_to_ptrack(ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane], a.seed, t, ctx.event);
+            alive = true;
+            ctx.rec = false;
+        }
+        const bool want = !alive && !exhausted;
+        const uint64_t want_mask = __ballot(want);
+        const uint64_t alive_mask = __ballot(alive);
+        if (want_mask && (alive_mask == 0 || iter % period == 0)) {
+            const int leader = __ffsll(static_cast<unsigned long long>(want_mask)) - 1;
+            unsigned long long base = 0;
+            if (__lane_id() == leader) base = atomicAdd(a.init_head, static_cast<unsigned long long>(__popcll(want_mask)));
+            base = __shfl(base, leader);
+            if (want) {
+                const unsigned long long idx = base + __popcll(want_mask & lt_mask);
+                if (idx < a.n_init) {
+                    btrack_to_ptrack(a.init[idx], a.seed, t, ctx.event);
+                    alive = true;
+                    ctx.rec = (t.key == 0);
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(alive) == 0) {
+            if (__ballot(!exhausted) == 0) break;
+            continue;
+        }
+        if ((iter & 1023u) == 0 && a.cancel) {
+            int c = 0;
+            if (__lane_id() == 0) c = *a.cancel;
+            if (__shfl(c, 0)) {
+                if (__lane_id() == 0) atomicOr(&a.status[2], 1ull);
+                break;
+            }
+        }
+        if (alive) {
+            alive = phonon_step<P>(m, t, t.rng, ctx);
+            if (!alive) ctx.rec = false;
+            if (ctx.cnt[C_STEPS] > 0x40000000u) lane_to_block(s_acc, ctx.cnt, ctx.led);
+        }
+    }
+    lane_to_block(s_acc, ctx.cnt, ctx.led);
+    if (ctx.overflow) {
+        atomicOr(&a.status[0], 1ull);
+        atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
+    }
+    __syncthreads();
+    block_to_global(s_acc, a.counters, a.ledger);
+}
+
+// ------------------------------------------------------------------ batched charges
+__device__ inline void bcharge_to_ctrack(const BCharge& b, uint64_t seed, CTrack<P>& c, uint64_t& event) {
+    c.pos = {b.pos[0], b.pos[1], b.pos[2]};
+    c.p = {b.p[0], b.p[1], b.p[2]};
+    c.ekin = b.ekin;
+    c.time = b.time;
+    c.weight = b.weight;
+    c.valley = static_cast<int>(b.valley_type >> 8);
+    c.type = static_cast<int>(b.valley_type & 0xFF);
+    c.key = b.key;
+    c.spawn_no = 0;
+    c.hit_seq = 0;
+    event = b.event;
+    c.rng = rng_for_track(event_base(seed, event), b.key);
+}
+template <class T> __device__ inline void ctrack_to_bcharge(const CTrack<T>& c, uint64_t event, BCharge& b) {
+    b.pos[0] = static_cast<float>(c.pos.x); b.pos[1] = static_cast<float>(c.pos.y); b.pos[2] = static_cast<float>(c.pos.z);
+    b.p[0] = static_cast<float>(c.p.x); b.p[1] = static_cast<float>(c.p.y); b.p[2] = static_cast<float>(c.p.z);
+    b.ekin = static_cast<float>(c.ekin);
+    b.event = static_cast<uint32_t>(event);
+    b.time = c.time;
+    b.weight = c.weight;
+    b.key = c.key;
+    b.valley_type = (static_cast<uint32_t>(c.valley) << 8) | static_cast<uint32_t>(c.type);
+    b._pad = 0;
+}
+
+struct ChargeCtx {
+    uint32_t cnt[kNumCounters];
+    LaneLedger led;
+    BCharge* ls;
+    uint32_t sp, depth, lanes, lane;
+    bool overflow;
+    uint64_t event;
+    BTrack* luke_out;
+    unsigned long long* luke_count;
+    uint64_t luke_cap;
+    DevHit* hits;
+    unsigned long long* hit_count;
+    uint64_t hit_cap;
+    bool keep_hits;
+
+    __device__ void count(Counter c) { ++cnt[c]; }
+    __device__ void ccount(CCounter c) { ++cnt[C_NCOUNTERS + c]; }
+    __device__ void escaped(double v) { led.add(L_ESC, v); }
+    __device__ void deposited(double v) { led.add(L_DEP, v); }
+    __device__ void luke_eV(double v) { led.add(L_LUKE, v); }
+    __device__ void recomb(double v) { led.add(L_RECOMB, v); }
+    __device__ void branch(int) {}
+    __device__ void push_charge(CTrack<P>& c, CTrack<P>& parent) {
+        c.key = child_key(parent.key, ++parent.spawn_no);
+        if (sp >= depth) { overflow = true; led.add(L_ESC, (static_cast<double>(c.ekin)) * c.weight); return; }
+        ctrack_to_bcharge(c, event, ls[static_cast<size_t>(sp) * lanes + lane]);
+        ++sp;
+    }
+    __device__ void push_luke(PTrack<P>& ph, CTrack<P>& parent) {
+        ph.key = child_key(parent.key, ++parent.spawn_no);
+        const unsigned long long i = atomicAdd(luke_count, 1ull);
+        if (i < luke_cap) ptrack_to_btrack(ph, event, luke_out[i]);
+        else { overflow = true; led.add(L_ESC, ph.energy * ph.weight); }
+    }
+    __device__ void charge_hit(const CTrack<P>& c, double recomb_w, double e_rel, int sid) {
+        if (!keep_hits) return;
+        const unsigned long long i = atomicAdd(hit_count, 1ull);
+        if (i < hit_cap) {
+            DevHit& d = hits[i];
+            d.k0 = event;
+            d.k1 = c.key;
+            d.k2 = 0;
+            fill_hit<P>(d.h, c.pos, c.time, recomb_w, e_rel, sid, c.type, 0);
+        }
+    }
+};
+
+__global__ void __launch_bounds__(256) transport_charges_kernel(const DevModel m, const ChargeArgs a) {
+    __shared__ BlockAccum s_acc;
+    block_accum_init(s_acc);
+    __syncthreads();
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    ChargeCtx ctx;
+    for (int i = 0; i < kNumCounters; ++i) ctx.cnt[i] = 0;
+    ctx.led.reset();
+    ctx.ls = a.lstack;
+    ctx.sp = 0;
+    ctx.depth = a.depth;
+    ctx.lanes = a.lanes;
+    ctx.lane = lane;
+    ctx.overflow = false;
+    ctx.event = 0;
+    ctx.luke_out = a.luke_out;
+    ctx.luke_count = a.luke_count;
+    ctx.luke_cap = a.luke_cap;
+    ctx.hits = a.hits;
+    ctx.hit_count = a.hit_count;
+    ctx.hit_cap = a.hit_cap;
+    ctx.keep_hits = a.keep_hits != 0;
+
+    CTrack<P> c;
+    bool alive = false, exhausted = false;
+    uint32_t steps = 0;
+    const uint64_t lt_mask = (1ull << __lane_id()) - 1ull;
+    for (uint32_t iter = 0;; ++iter) {
+        if (!alive && ctx.sp > 0) {
+            --ctx.sp;
+            bcharge_to_ctrack(ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane], a.seed, c, ctx.event);
+            alive = true;
+            steps = 0;
+        }
+        const bool want = !alive && !exhausted;
+        const uint64_t want_mask = __ballot(want);
+        if (want_mask) {
+            const int leader = __ffsll(static_cast<unsigned long long>(want_mask)) - 1;
+            unsigned long long base = 0;
+            if (__lane_id() == leader) base = atomicAdd(a.init_head, static_cast<unsigned long long>(__popcll(want_mask)));
+            base = __shfl(base, leader);
+            if (want) {
+                const unsigned long long idx = base + __popcll(want_mask & lt_mask);
+                if (idx < a.n_init) {
+                    bcharge_to_ctrack(a.init[idx], a.seed, c, ctx.event);
+                    alive = true;
+                    steps = 0;
+                } else {
+                    exhausted = true;
+                }
+            }
+        }
+        if (__ballot(alive) == 0) {
+            if (__ballot(!exhausted) == 0) break;
+            continue;
+        }
+        if ((iter & 1023u) == 0 && a.cancel) {
+            int cc = 0;
+            if (__lane_id() == 0) cc = *a.cancel;
+            if (__shfl(cc, 0)) {
+                if (__lane_id() == 0) atomicOr(&a.status[2], 1ull);
+                break;
+            }
+        }
+        if (alive) {
+            alive = charge_step<P>(m, c, c.rng, ctx);
+            if (alive && ++steps >= 20000000u) alive = false;   // engine.hpp:210 step cap
+            if (ctx.cnt[C_NCOUNTERS + CC_STEPS] > 0x40000000u) lane_to_block(s_acc, ctx.cnt, ctx.led);
+        }
+    }
+    lane_to_block(s_acc, ctx.cnt, ctx.led);
+    if (ctx.overflow) atomicOr(&a.status[0], 1ull);
+    __syncthreads();
+    block_to_global(s_acc, a.counters, a.ledger);
+}
+
+// ------------------------------------------------------------------ generators
+__global__ void workload_count_kernel(const cmpmc_workload* wl, uint64_t seed, uint64_t first_event, uint64_t n,
+                                      uint32_t* counts) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Rng r = rng_for_stream(seed, first_event + i);
+    counts[i] = workload_count(*wl, r);
+}
+
+__device__ inline uint64_t find_event(const uint64_t* offsets, uint64_t n_events, uint64_t g) {
+    uint64_t lo = 0, hi = n_events;                 // largest e with offsets[e] <= g
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (offsets[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void workload_fill_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
+                                     uint64_t n_events, const uint64_t* offsets, uint64_t n_tracks, BTrack* out,
+                                     unsigned long long* ledger) {
+    __shared__ unsigned long long s_inj;
+    if (threadIdx.x == 0) s_inj = 0;
+    __syncthreads();
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g < n_tracks) {
+        const uint64_t e = find_event(offsets, n_events, g);
+        const uint64_t j = g - offsets[e];
+        const uint64_t ev = first_event + e;
+        const uint64_t base = event_base(seed, ev);
+        // POINT draws its direction from the event stream (as the host generator does);
+        // LOGN / UNIF draw track j's initial state from its generator stream.
+        Rng r = (wl->kind == CMPMC_WL_PHONON_POINT) ? rng_for_stream(seed, ev) : rng_for_track(base, j | (1ull << 62));
+        PTrack<Fp64> t;
+        double E;
+        workload_phonon<Fp64>(m, *wl, r, t, E);
+        t.key = j;
+        ptrack_to_btrack(t, ev, out[g]);
+        atomicAdd(&s_inj, static_cast<unsigned long long>(__double2ll_rn(E * kLedgerScale)));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_inj) {
+        atomicAdd(&ledger[2 * L_INJ], s_inj >> 20);
+        atomicAdd(&ledger[2 * L_INJ + 1], s_inj & 0xFFFFFull);
+    }
+}
+
+__global__ void pair_fill_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
+                                 uint64_t n_events, BCharge* out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_events) return;
+    const uint64_t ev = first_event + i;
+    Rng r = rng_for_stream(seed, ev);
+    CTrack<Fp64> e, h;
+    workload_pair<Fp64>(m, *wl, r, e, h);
+    e.key = 0x8000000000000000ull;
+    h.key = 0x8000000000000001ull;
+    ctrack_to_bcharge(e, ev, out[2 * i]);
+    ctrack_to_bcharge(h, ev, out[2 * i + 1]);
+}
+
+__global__ void tracks_to_queue_kernel(const cmpmc_phonon_track* in, const uint64_t* offsets, uint64_t n_events,
+                                       uint64_t n_tracks, uint64_t first_event, BTrack* out) {
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_tracks) return;
+    const uint64_t e = find_event(offsets, n_events, g);
+    const cmpmc_phonon_track& s = in[g];
+    BTrack& b = out[g];
+    for (int i = 0; i < 3; ++i) {
+        b.pos[i] = static_cast<float>(s.pos[i]);
+        b.k[i] = static_cast<float>(s.k_dir[i]);
+        b.vg[i] = static_cast<float>(s.vg_dir[i]);
+    }
+    b.vg_mag = static_cast<float>(s.vg_mag);
+    b.energy = s.energy;
+    b.time = s.time;
+    b.weight = s.weight;
+    b.key = g - offsets[e];
+    b.event = static_cast<uint32_t>(first_event + e);
+    b.bounces_mode = (s.bounces << 8) | s.mode;
+}
+
+__global__ void charges_to_queue_kernel(const cmpmc_charge_track* in, const uint64_t* offsets, uint64_t n_events,
+                                        uint64_t n_charges, uint64_t first_event, BCharge* out) {
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_charges) return;
+    const uint64_t e = find_event(offsets, n_events, g);
+    const cmpmc_charge_track& s = in[g];
+    BCharge& b = out[g];
+    for (int i = 0; i < 3; ++i) {
+        b.pos[i] = static_cast<float>(s.pos[i]);
+        b.p[i] = static_cast<float>(s.p_eV[i]);
+    }
+    b.ekin = static_cast<float>(s.ekin);
+    b.event = static_cast<uint32_t>(first_event + e);
+    b.time = s.time;
+    b.weight = s.weight;
+    b.key = 0x8000000000000000ull | (g - offsets[e]);
+    b.valley_type = (static_cast<uint32_t>(s.valley) << 8) | s.type;
+    b._pad = 0;
+}
+
+}  // namespace cmpmc_dev
+
+using namespace cmpmc_dev;
+
+static inline unsigned blocks_for(uint64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+// Resident blocks per CU of the persistent kernels (sizes their grids).
+extern "C" hipError_t cmpmc_occupancy(int which, int block, int* blocks_per_cu) {
+    if (which == 0) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, transport_batched_kernel, block, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, transport_charges_kernel, block, 0);
+}
+
+extern "C" hipError_t cmpmc_launch_batched(const DevModel* m, const BatchedArgs* a, int block, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(transport_batched_kernel, dim3(grid), dim3(block), 0, s, *m, *a);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_charges(const DevModel* m, const ChargeArgs* a, int block, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(transport_charges_kernel, dim3(grid), dim3(block), 0, s, *m, *a);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
+                                                  uint64_t n, uint32_t* counts, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(workload_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, wl, seed, first_event, n, counts);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_workload_fill(const DevModel* m, const cmpmc_workload* wl, uint64_t seed,
+                                                 uint64_t first_event, uint64_t n_events, const uint64_t* offsets,
+                                                 uint64_t n_tracks, BTrack* out, unsigned long long* ledger,
+                                                 hipStream_t s) {
+    if (n_tracks == 0) return hipSuccess;
+    hipLaunchKernelGGL(workload_fill_kernel, dim3(blocks_for(n_tracks, 256)), dim3(256), 0, s, *m, wl, seed,
+                       first_event, n_events, offsets, n_tracks, out, ledger);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_pair_fill(const DevModel* m, const cmpmc_workload* wl, uint64_t seed,
+                                             uint64_t first_event, uint64_t n_events, BCharge* out, hipStream_t s) {
+    if (n_events == 0) return hipSuccess;
+    hipLaunchKernelGGL(pair_fill_kernel, dim3(blocks_for(n_events, 256)), dim3(256), 0, s, *m, wl, seed, first_event,
+                       n_events, out);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_tracks_to_queue(const cmpmc_phonon_track* in, const uint64_t* offsets,
+                                                   uint64_t n_events, uint64_t n_tracks, uint64_t first_event,
+                                                   BTrack* out, hipStream_t s) {
+    if (n_tracks == 0) return hipSuccess;
+    hipLaunchKernelGGL(tracks_to_queue_kernel, dim3(blocks_for(n_tracks, 256)), dim3(256), 0, s, in, offsets, n_events,
+                       n_tracks, first_event, out);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_charges_to_queue(const cmpmc_charge_track* in, const uint64_t* offsets,
+                                                    uint64_t n_events, uint64_t n_charges, uint64_t first_event,
+                                                    BCharge* out, hipStream_t s) {
+    if (n_charges == 0) return hipSuccess;
+    hipLaunchKernelGGL(charges_to_queue_kernel, dim3(blocks_for(n_charges, 256)), dim3(256), 0, s, in, offsets,
+                       n_events, n_charges, first_event, out);
+    return hipGetLastError();
+}

@@ -1,0 +1,149 @@
+// Kernel argument blocks and launchers shared between the HIP kernels and the C ABI
+// implementation (capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cmpmc_amd.h"
+
+// Capacity of each Kaplan cascade list of one lane (core/math.hpp:230, KaplanBuffer).
+#define CMPMC_KCAP 256
+
+namespace cmpmc_dev {
+
+struct DevModel;
+struct DevHit;
+
+// Plain per-event form: one lane per event, stacks in per-event regions.
+struct PlainArgs {
+    uint64_t n_events;
+    uint64_t first_event;              // event id of lane 0
+    cmpmc_phonon_track* pstack;        // phonon stack regions
+    const uint64_t* pstack_off;        // n_events + 1
+    const uint32_t* n_init_ph;         // initial phonons already in each region (NULL: 0)
+    cmpmc_charge_track* cstack;        // charge stack regions (may be NULL)
+    const uint64_t* cstack_off;
+    const uint32_t* n_init_ch;
+    uint64_t* rng_state;               // 4 per event (in/out); NULL with a device workload
+    const double* injected;            // per event (may be NULL)
+    const cmpmc_workload* wl;          // device generator (NULL: stacks are pre-filled)
+    uint64_t seed;
+    DevHit* hits;
+    unsigned long long* hit_count;
+    uint64_t hit_cap;
+    int32_t keep_hits;
+    int32_t skip_phonons;              // transport_charges only: leave the Luke phonons on the stack
+    uint32_t* hits_per_event;
+    int32_t* first_branch;
+    unsigned long long* counters;      // kNumCounters
+    unsigned long long* ledger;        // 2 * L_N
+    unsigned long long* status;        // [0] overflow flag, [1] deepest stack needed
+    double* kaplan_ws;                 // 5 * CMPMC_KCAP * ws_stride doubles
+    uint32_t ws_stride;
+    const volatile int32_t* cancel;
+};
+
+// Compact record of a queued track of the batched form (80 B). Its random stream is
+// not stored: it is rng_for_track(event_base(seed, event), key).
+struct alignas(16) BTrack {
+    float pos[3];
+    float k[3];
+    float vg[3];
+    float vg_mag;
+    double energy, time, weight;
+    uint64_t key;
+    uint32_t event;
+    uint32_t bounces_mode;             // bounces << 8 | mode
+};
+
+// Queued carrier of the batched charge kernel (64 B).
+struct alignas(16) BCharge {
+    float pos[3];
+    float p[3];
+    float ekin;
+    uint32_t event;
+    double time, weight;
+    uint64_t key;
+    uint32_t valley_type;              // valley << 8 | type
+    uint32_t _pad;
+};
+
+struct BatchedArgs {
+    uint64_t seed;
+    uint64_t first_event;
+    uint64_t n_events;
+    const BTrack* init;                // initial queue
+    uint64_t n_init;
+    unsigned long long* init_head;     // claim counter of the initial queue
+    BTrack* lstack;                    // per-lane spawn stacks: entry d of lane L at [d * lanes + L]
+    uint32_t depth;
+    uint32_t lanes;
+    uint32_t refill_period;
+    int32_t keep_hits;
+    DevHit* hits;
+    unsigned long long* hit_count;
+    uint64_t hit_cap;
+    int32_t* first_branch;             // per event (relative to first_event)
+    unsigned long long* counters;
+    unsigned long long* ledger;
+    unsigned long long* status;        // [0] overflow flag, [1] deepest stack needed, [2] cancelled
+    double* kaplan_ws;
+    const volatile int32_t* cancel;
+};
+
+struct ChargeArgs {
+    uint64_t seed;
+    uint64_t first_event;
+    const BCharge* init;
+    uint64_t n_init;
+    unsigned long long* init_head;
+    BCharge* lstack;                   // knock-on carriers: entry d of lane L at [d * lanes + L]
+    uint32_t depth;
+    uint32_t lanes;
+    BTrack* luke_out;                  // enqueued Luke phonons (append)
+    unsigned long long* luke_count;
+    uint64_t luke_cap;
+    int32_t keep_hits;
+    DevHit* hits;
+    unsigned long long* hit_count;
+    uint64_t hit_cap;
+    unsigned long long* counters;
+    unsigned long long* ledger;
+    unsigned long long* status;
+    const volatile int32_t* cancel;
+};
+
+}  // namespace cmpmc_dev
+
+extern "C" {
+hipError_t cmpmc_occupancy(int which, int block, int* blocks_per_cu);
+hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block, hipStream_t s);
+hipError_t cmpmc_launch_batched(const cmpmc_dev::DevModel* m, const cmpmc_dev::BatchedArgs* a, int block, int grid,
+                                hipStream_t s);
+hipError_t cmpmc_launch_charges(const cmpmc_dev::DevModel* m, const cmpmc_dev::ChargeArgs* a, int block, int grid,
+                                hipStream_t s);
+// Device generators of the batched form: per-event counts, then the initial queue.
+hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl_dev, uint64_t seed, uint64_t first_event,
+                                       uint64_t n_events, uint32_t* counts, hipStream_t s);
+hipError_t cmpmc_launch_workload_fill(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
+                                      uint64_t first_event, uint64_t n_events, const uint64_t* offsets,
+                                      uint64_t n_tracks, cmpmc_dev::BTrack* out, unsigned long long* ledger,
+                                      hipStream_t s);
+hipError_t cmpmc_launch_pair_fill(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
+                                  uint64_t first_event, uint64_t n_events, cmpmc_dev::BCharge* out, hipStream_t s);
+// Host-provided initial stacks -> per-event stack regions of the plain form.
+hipError_t cmpmc_launch_scatter_tracks(const cmpmc_phonon_track* in, const uint64_t* in_off, uint64_t n_events,
+                                       uint64_t n_tracks, cmpmc_phonon_track* regions, const uint64_t* region_off,
+                                       hipStream_t s);
+hipError_t cmpmc_launch_scatter_charges(const cmpmc_charge_track* in, const uint64_t* in_off, uint64_t n_events,
+                                        uint64_t n_charges, cmpmc_charge_track* regions, const uint64_t* region_off,
+                                        hipStream_t s);
+// Host-provided initial stacks -> batched queue records.
+hipError_t cmpmc_launch_tracks_to_queue(const cmpmc_phonon_track* in, const uint64_t* offsets, uint64_t n_events,
+                                        uint64_t n_tracks, uint64_t first_event, cmpmc_dev::BTrack* out,
+                                        hipStream_t s);
+hipError_t cmpmc_launch_charges_to_queue(const cmpmc_charge_track* in, const uint64_t* offsets, uint64_t n_events,
+                                         uint64_t n_charges, uint64_t first_event, cmpmc_dev::BCharge* out,
+                                         hipStream_t s);
+}

@@ -1,0 +1,268 @@
+// Plain per-event transport kernel (FP64): one lane runs one event to completion with
+// the reference's semantics (engine.hpp:192-582 driven by parallel.hpp:102-113): the
+// event draws every random number from its own stream in LIFO order, charges first,
+// their Luke phonons then join the event's phonon stack.  Compiled with
+// -ffp-contract=off so that FP64 arithmetic follows the reference operation by
+// operation; only ocml-vs-glibc transcendental ULPs differ.
+#include "kernels_common.hpp"
+#include "transport_kernels.h"
+
+namespace cmpmc_dev {
+
+using P = Fp64;
+using R = double;
+
+struct PlainCtx {
+    // stats
+    uint32_t cnt[kNumCounters];
+    LaneLedger led;
+    // stacks of this event
+    cmpmc_phonon_track* ps;
+    uint32_t psp, pcap;
+    cmpmc_charge_track* cs;
+    uint32_t csp, ccap;
+    uint32_t need;           // deepest stack requirement seen (for capacity errors)
+    bool overflow;
+    // hits
+    DevHit* hits;
+    unsigned long long* hit_count;
+    uint64_t hit_cap;
+    uint64_t ev;
+    uint32_t seq;
+    bool keep_hits;
+    // first-step branch of the first popped phonon
+    int first_branch;
+    bool first_pending;
+    bool rec;
+    // Kaplan lists (in the lane's workspace)
+    KList kl[5];
+    const double* lt;
+    const double* tt;
+
+    __device__ void count(Counter c) { ++cnt[c]; }
+    __device__ void ccount(CCounter c) { ++cnt[C_NCOUNTERS + c]; }
+    __device__ void escaped(double v) { led.add(L_ESC, v); }
+    __device__ void deposited(double v) { led.add(L_DEP, v); }
+    __device__ void luke_eV(double v) { led.add(L_LUKE, v); }
+    __device__ void recomb(double v) { led.add(L_RECOMB, v); }
+    __device__ void branch(int code) {
+        if (rec) { first_branch = code; rec = false; }
+    }
+    __device__ const double* cdf_lt() const { return lt; }
+    __device__ const double* cdf_tt() const { return tt; }
+    __device__ KList* lists() { return kl; }
+
+    __device__ void store_phonon(const PTrack<P>& t) {
+        if (psp >= pcap) { overflow = true; need = tmax(need, psp + 1); return; }
+        cmpmc_phonon_track& o = ps[psp++];
+        need = tmax(need, psp);
+        o.pos[0] = t.pos.x; o.pos[1] = t.pos.y; o.pos[2] = t.pos.z;
+        o.k_dir[0] = t.k.x; o.k_dir[1] = t.k.y; o.k_dir[2] = t.k.z;
+        o.vg_dir[0] = t.vg.x; o.vg_dir[1] = t.vg.y; o.vg_dir[2] = t.vg.z;
+        o.energy = t.energy; o.vg_mag = t.vg_mag; o.time = t.time; o.weight = t.weight;
+        o.bounces = t.bounces; o.mode = static_cast<uint8_t>(t.mode);
+    }
+    __device__ void push(PTrack<P>& child, PTrack<P>&) { store_phonon(child); }
+    __device__ void push_luke(PTrack<P>& ph, CTrack<P>&) { store_phonon(ph); }
+    __device__ void push_charge(CTrack<P>& c, CTrack<P>&) {
+        if (csp >= ccap) { overflow = true; return; }
+        cmpmc_charge_track& o = cs[csp++];
+        o.pos[0] = c.pos.x; o.pos[1] = c.pos.y; o.pos[2] = c.pos.z;
+        o.p_eV[0] = c.p.x; o.p_eV[1] = c.p.y; o.p_eV[2] = c.p.z;
+        o.ekin = c.ekin; o.time = c.time; o.weight = c.weight;
+        o.valley = static_cast<uint16_t>(c.valley); o.type = static_cast<uint8_t>(c.type); o.alive = 1;
+    }
+    __device__ void emit(const cmpmc_sensor_hit& h) {
+        if (!keep_hits) return;
+        const unsigned long long i = atomicAdd(hit_count, 1ull);
+        if (i < hit_cap) {
+            DevHit& d = hits[i];
+            d.k0 = (ev << 32) | seq;
+            d.k1 = 0;
+            d.k2 = 0;
+            d.h = h;
+        }
+        ++seq;
+    }
+    __device__ void hit(const PTrack<P>& t, double wed, int sid) {
+        cmpmc_sensor_hit h;
+        fill_hit<P>(h, t.pos, t.time, wed, t.energy, sid, 0, t.mode);
+        emit(h);
+    }
+    __device__ void charge_hit(const CTrack<P>& t, double recomb, double e_rel, int sid) {
+        cmpmc_sensor_hit h;
+        fill_hit<P>(h, t.pos, t.time, recomb, e_rel, sid, t.type, 0);
+        emit(h);
+    }
+};
+
+__device__ inline void load_phonon(const cmpmc_phonon_track& o, PTrack<P>& t) {
+    t.pos = {o.pos[0], o.pos[1], o.pos[2]};
+    t.k = {o.k_dir[0], o.k_dir[1], o.k_dir[2]};
+    t.vg = {o.vg_dir[0], o.vg_dir[1], o.vg_dir[2]};
+    t.energy = o.energy; t.vg_mag = o.vg_mag; t.time = o.time; t.weight = o.weight;
+    t.bounces = o.bounces; t.mode = o.mode;
+}
+__device__ inline void load_charge(const cmpmc_charge_track& o, CTrack<P>& c) {
+    c.pos = {o.pos[0], o.pos[1], o.pos[2]};
+    c.p = {o.p_eV[0], o.p_eV[1], o.p_eV[2]};
+    c.ekin = o.ekin; c.time = o.time; c.weight = o.weight;
+    c.valley = o.valley; c.type = o.type;
+}
+
+__global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, const PlainArgs a) {
+    __shared__ double s_cdf[2 * CMPMC_N_CDF];
+    for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
+        s_cdf[i] = m.cdf_lt[i];
+        s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
+    }
+    __syncthreads();
+    const uint64_t lane = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (lane >= a.n_events) return;
+    if (a.cancel && *a.cancel) return;
+    const uint64_t ev = a.first_event + lane;
+
+    PlainCtx ctx;
+    for (int i = 0; i < kNumCounters; ++i) ctx.cnt[i] = 0;
+    ctx.led.reset();
+    ctx.ps = a.pstack + a.pstack_off[lane];
+    ctx.pcap = static_cast<uint32_t>(a.pstack_off[lane + 1] - a.pstack_off[lane]);
+    ctx.psp = a.n_init_ph ? a.n_init_ph[lane] : 0;
+    ctx.cs = a.cstack ? a.cstack + a.cstack_off[lane] : nullptr;
+    ctx.ccap = a.cstack ? static_cast<uint32_t>(a.cstack_off[lane + 1] - a.cstack_off[lane]) : 0;
+    ctx.csp = a.n_init_ch ? a.n_init_ch[lane] : 0;
+    ctx.need = ctx.psp;
+    ctx.overflow = false;
+    ctx.hits = a.hits;
+    ctx.hit_count = a.hit_count;
+    ctx.hit_cap = a.hit_cap;
+    ctx.ev = ev;
+    ctx.seq = 0;
+    ctx.keep_hits = a.keep_hits != 0;
+    ctx.first_branch = CMPMC_BR_NONE;
+    ctx.first_pending = true;
+    ctx.rec = false;
+    for (int i = 0; i < 5; ++i) {
+        ctx.kl[i].base = a.kaplan_ws + static_cast<size_t>(i) * CMPMC_KCAP * a.ws_stride + lane;
+        ctx.kl[i].stride = a.ws_stride;
+        ctx.kl[i].n = 0;
+        ctx.kl[i].cap = CMPMC_KCAP;
+    }
+    ctx.lt = s_cdf;
+    ctx.tt = s_cdf + CMPMC_N_CDF;
+
+    Rng rng;
+    if (a.wl) {
+        // device generator: the event's stream, documented draw order
+        rng = rng_for_stream(a.seed, ev);
+        const cmpmc_workload& w = *a.wl;
+        if (w.kind == CMPMC_WL_PAIR) {
+            CTrack<P> e, h;
+            workload_pair<P>(m, w, rng, e, h);
+            ctx.csp = 0;
+            ctx.push_charge(e, e);
+            ctx.push_charge(h, h);
+        } else {
+            const uint32_t n = workload_count(w, rng);
+            ctx.psp = 0;
+            double inj = 0.0;
+            for (uint32_t j = 0; j < n; ++j) {
+                PTrack<P> t;
+                double E;
+                workload_phonon<P>(m, w, rng, t, E);
+                ctx.store_phonon(t);
+                inj += E;
+            }
+            ctx.led.add(L_INJ, inj);
+        }
+    } else {
+        rng = Rng{a.rng_state[4 * lane], a.rng_state[4 * lane + 1], a.rng_state[4 * lane + 2], a.rng_state[4 * lane + 3]};
+        if (a.injected) ctx.led.add(L_INJ, a.injected[lane]);
+    }
+
+    // transport_charges: engine.hpp:204-379
+    while (ctx.csp > 0 && !ctx.overflow) {
+        CTrack<P> c;
+        load_charge(ctx.cs[--ctx.csp], c);
+        for (uint32_t step = 0; step < 20000000u; ++step)
+            if (!charge_step<P>(m, c, rng, ctx)) break;
+    }
+    // transport: engine.hpp:412-581
+    while (!a.skip_phonons && ctx.psp > 0 && !ctx.overflow) {
+        PTrack<P> t;
+        load_phonon(ctx.ps[--ctx.psp], t);
+        if (ctx.first_pending) { ctx.rec = true; ctx.first_pending = false; }
+        while (phonon_step<P>(m, t, rng, ctx)) {
+        }
+        ctx.rec = false;
+    }
+
+    if (a.rng_state) {
+        a.rng_state[4 * lane] = rng.s0; a.rng_state[4 * lane + 1] = rng.s1;
+        a.rng_state[4 * lane + 2] = rng.s2; a.rng_state[4 * lane + 3] = rng.s3;
+    }
+    if (a.first_branch) a.first_branch[lane] = ctx.first_branch;
+    if (a.hits_per_event) a.hits_per_event[lane] = a.skip_phonons ? ctx.psp : ctx.seq;
+    for (int i = 0; i < kNumCounters; ++i)
+        if (ctx.cnt[i]) atomicAdd(&a.counters[i], static_cast<unsigned long long>(ctx.cnt[i]));
+    ctx.led.flush(a.ledger);
+    if (ctx.overflow) {
+        atomicOr(&a.status[0], 1ull);
+        atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
+    }
+}
+
+__device__ inline uint64_t region_event(const uint64_t* off, uint64_t n_events, uint64_t g) {
+    uint64_t lo = 0, hi = n_events;          // largest e with off[e] <= g
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (off[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void scatter_tracks_kernel(const cmpmc_phonon_track* in, const uint64_t* in_off, uint64_t n_events,
+                                      uint64_t n, cmpmc_phonon_track* regions, const uint64_t* region_off) {
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint64_t e = region_event(in_off, n_events, g);
+    regions[region_off[e] + (g - in_off[e])] = in[g];
+}
+
+__global__ void scatter_charges_kernel(const cmpmc_charge_track* in, const uint64_t* in_off, uint64_t n_events,
+                                       uint64_t n, cmpmc_charge_track* regions, const uint64_t* region_off) {
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint64_t e = region_event(in_off, n_events, g);
+    regions[region_off[e] + (g - in_off[e])] = in[g];
+}
+
+}  // namespace cmpmc_dev
+
+extern "C" hipError_t cmpmc_launch_scatter_tracks(const cmpmc_phonon_track* in, const uint64_t* in_off,
+                                                  uint64_t n_events, uint64_t n, cmpmc_phonon_track* regions,
+                                                  const uint64_t* region_off, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(cmpmc_dev::scatter_tracks_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                       in, in_off, n_events, n, regions, region_off);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cmpmc_launch_scatter_charges(const cmpmc_charge_track* in, const uint64_t* in_off,
+                                                   uint64_t n_events, uint64_t n, cmpmc_charge_track* regions,
+                                                   const uint64_t* region_off, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(cmpmc_dev::scatter_charges_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0,
+                       s, in, in_off, n_events, n, regions, region_off);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block,
+                                         hipStream_t stream) {
+    const uint64_t n = a->n_events;
+    if (n == 0) return hipSuccess;
+    const int bs = block > 0 ? block : 256;
+    const unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
+    hipLaunchKernelGGL(cmpmc_dev::transport_plain_kernel, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    return hipGetLastError();
+}

@@ -1,0 +1,10 @@
+// Umbrella header of the drop-in cmpmc C++ API (the reference's cmpmc/cmpmc.hpp names
+// for the track-transport path), backed by libcmpmc_amd (include/cmpmc_amd.h).
+#pragma once
+
+#include "cmpmc/core/core.hpp"
+#include "cmpmc/materials/crystal_config.hpp"
+#include "cmpmc/lattice/crystal_lattice.hpp"
+#include "cmpmc/geometry/surface_property.hpp"
+#include "cmpmc/transport/engine.hpp"
+#include "cmpmc/transport/parallel.hpp"

@@ -1,0 +1,312 @@
+/*
+ * cmpmc_amd: C ABI of the MI355X (gfx950) track-transport library.
+ *
+ * Plain pointers and sizes only. Every entry point returns an int status (CMPMC_OK = 0);
+ * cmpmc_last_error() gives the message of the last failure on the calling thread.
+ * Each function replaces one piece of the reference's C++ API, cited beside it
+ * (paths relative to the reference's include/cmpmc/).  The C++ drop-in headers
+ * (under include/cmpmc/) map status codes back to the reference's exceptions.
+ *
+ * Record layouts (cmpmc_phonon_track, cmpmc_charge_track, cmpmc_sensor_hit,
+ * cmpmc_sim_stats) are byte-compatible with the reference's PhononTrack (128 B),
+ * ChargeTrack (128 B), SensorHit (56 B) and SimStats (208 B) on x86-64.
+ */
+#ifndef CMPMC_AMD_H
+#define CMPMC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+enum {
+    CMPMC_OK = 0,
+    CMPMC_ERR_INVALID = 1,       /* bad argument (null pointer, NaN/negative energy, ...) */
+    CMPMC_ERR_UNVALIDATED = 2,   /* guarded model requested (UnvalidatedModuleError) */
+    CMPMC_ERR_CAPACITY = 3,      /* a stack, queue or hit buffer was too small (see *_needed) */
+    CMPMC_ERR_HIP = 4,           /* HIP runtime failure */
+    CMPMC_ERR_NO_DEVICE = 5,     /* no GPU visible */
+    CMPMC_ERR_CANCELLED = 6      /* the cancel flag was raised by the host */
+};
+
+/* ---------------------------------------------------------------- records */
+/* core/track.hpp:62-72 PhononTrack (alignas(64), 128 B) */
+typedef struct {
+    double pos[3];
+    double k_dir[3];
+    double vg_dir[3];
+    double energy;
+    double vg_mag;
+    double time;
+    double weight;
+    uint32_t bounces;
+    uint8_t mode;               /* 0 L, 1 FT, 2 ST */
+    uint8_t _pad[19];
+} cmpmc_phonon_track;
+
+/* core/track.hpp:75-84 ChargeTrack (alignas(64), 128 B) */
+typedef struct {
+    double pos[3];
+    double p_eV[3];
+    double ekin;
+    double time;
+    double weight;
+    uint16_t valley;
+    uint8_t type;               /* 1 electron, 2 hole */
+    uint8_t alive;
+    uint8_t _pad[52];
+} cmpmc_charge_track;
+
+/* core/track.hpp:99-108 SensorHit (56 B) */
+typedef struct {
+    double pos[3];
+    double time;
+    double edep;
+    double incident_energy;
+    uint8_t surface_id;         /* 0 top, 1 bottom, 2 side wall */
+    uint8_t excitation;         /* 0 phonon, 1 electron, 2 hole */
+    uint8_t mode;
+    uint8_t _pad;
+    int32_t channel_id;
+} cmpmc_sensor_hit;
+
+/* transport/engine.hpp:29-93 SimStats (208 B, same field order) */
+typedef struct {
+    uint64_t total_steps, isotope_scatters, polycrystal_scatters, decays_L_to_TT,
+             decays_L_to_LT, surface_refls, surface_downconversions, sensor_hits,
+             killed_tracks;
+    uint64_t charge_steps, luke_phonons_emitted, intervalley_scatters, charges_trapped,
+             trap_ionizations, charge_surface_refls, charges_collected;
+    double   total_luke_phonon_eV, charge_recomb_edep_eV;
+    uint64_t sc_pair_breakings, qp_diffusion_steps, qp_phonons_radiated,
+             qp_recombinations, qp_local_traps;
+    double   total_injected_eV, total_deposited_eV, escaped_or_killed_eV;
+} cmpmc_sim_stats;
+
+/* ---------------------------------------------------------------- model description */
+/* materials/crystal_config.hpp:43-134 (the fields the transport path reads) */
+typedef struct {
+    double density, C11, C12, C44;
+    double beta, gamma, lambda, mu;
+    double B_iso, A_anh, fTT_frac, polycrystal_mfp_m;
+    double fLDOS, fSTDOS, fFTDOS;
+    double debye_freq, v_sound, v_trans;
+    double bandgap, h_mass_ratio;
+    double e_mass[3];
+    int32_t n_valleys;          /* <= 8 */
+    int32_t iv_model;           /* 0 Quadratic, 1 Linear, 2 Matrix (guarded) */
+    double valley_dirs[8][3];
+    double ac_deform_e, ac_deform_h;
+    double iv_min_energy, iv_quad_rate, iv_quad_field, iv_quad_power;
+    double iv_lin_rate0, iv_lin_rate1, iv_lin_power;
+    double e_trapping_mfp_m, h_trapping_mfp_m;
+    double e_trap_ion_d0_mfp_m, e_trap_ion_a_mfp_m, h_trap_ion_d_mfp_m, h_trap_ion_a0_mfp_m;
+    double orient[3][3];
+} cmpmc_crystal;
+
+#define CMPMC_N_OCT 256
+#define CMPMC_N_CDF 1024
+
+/* The tables of lattice/crystal_lattice.hpp:300-331: octahedral FP32 v_g table
+ * [3][256][256][4], decay inverse CDFs [1024] and the Herring-Vogt charge tensors.
+ * Filled by cmpmc_lattice_build; the table pointers are owned by the caller. */
+typedef struct {
+    cmpmc_crystal cfg;
+    const float* oct_lut;
+    const double* cdf_lt;
+    const double* cdf_tt;
+    double e_cond_mass_eV, h_mass_eV;
+    double mass_ratio[3], hv_sqrt[3], hv_sqrt_inv[3];
+    double valley_rot[8][3][3];
+    double l0_e, l0_h, k_sound_e, k_sound_h;
+} cmpmc_lattice;
+
+/* superconductor/kaplan_film.hpp:29 KaplanFilm (crystal_config.hpp:240-257 fields) */
+typedef struct {
+    double film_absorption, film_thickness, v_sound, gap_energy, phonon_lifetime,
+           phonon_lifetime_slope, low_qp_limit, high_qp_limit, subgap_absorption,
+           absorber_gap, absorber_eff, absorber_eff_slope, temperature;
+    int32_t keep_all_phonons;
+    int32_t loss_model;         /* 0 MeanFreePath (1 KaplanRateCompetition: not supported) */
+} cmpmc_film;
+
+#define CMPMC_MAX_POLY 8
+/* geometry/surface_property.hpp:23-179 SurfaceProperty (polynomials as fixed arrays;
+ * n_* = 0 is the reference's empty coefficient vector) */
+typedef struct {
+    double prob_abs, prob_refl, refl_cutoff, anh_cutoff;
+    double prob_charge_abs, prob_charge_spec, min_k_elec, min_k_hole;
+    int32_t n_spec, n_diff, n_anh, has_film;
+    double spec[CMPMC_MAX_POLY], diff[CMPMC_MAX_POLY], anh[CMPMC_MAX_POLY];
+    double spec_prob_const;
+    cmpmc_film film;
+} cmpmc_surface;
+
+/* transport/engine.hpp:96-150 DetectorModel (cylinder solid, uniform field) */
+typedef struct {
+    double radius, half_z;
+    double efield[3];
+    cmpmc_surface surf[3];
+    uint32_t max_bounces;
+    uint32_t russian_roulette_bounces;
+    double min_phonon_eV;
+    double luke_sample_fraction;
+    double charge_max_dk_frac, charge_dt_cap_s;
+    double fermi_level_above_Ev_eV;
+    double russian_roulette_survival;
+} cmpmc_detector;
+
+typedef struct cmpmc_model cmpmc_model;
+
+/* ---------------------------------------------------------------- run description */
+enum { CMPMC_FORM_PLAIN = 0, CMPMC_FORM_BATCHED = 1 };
+
+/* Events handed over as initial stacks (what run_events' generator produced):
+ * tracks of event i are rows [offsets[i], offsets[i+1]) of `tracks`, in push order;
+ * rng_state[4*i..] is the event's stream after the generator (post-generator state). */
+typedef struct {
+    uint64_t n_events;
+    const cmpmc_phonon_track* tracks;     /* host memory */
+    const uint64_t* track_offsets;        /* n_events + 1 */
+    const cmpmc_charge_track* charges;    /* may be NULL */
+    const uint64_t* charge_offsets;       /* n_events + 1, NULL when charges is NULL */
+    const uint64_t* rng_state;            /* 4 * n_events */
+    const double* injected_eV;            /* n_events (may be NULL) */
+    uint64_t first_event_id;              /* event id of row 0 (sub-streams of the batched form) */
+} cmpmc_event_batch;
+
+/* Synthetic workloads generated on the device (the benchmark configurations). */
+enum {
+    CMPMC_WL_PHONON_POINT = 1,   /* one phonon (mode, energy) at origin, isotropic k */
+    CMPMC_WL_PHONON_LOGN = 2,    /* lognormal count, uniform volume, DOS modes, U[e_lo, e_hi] */
+    CMPMC_WL_PAIR = 3,           /* one e-/h+ pair (uniform volume unless pair_at_origin) */
+    CMPMC_WL_PHONON_UNIF = 5     /* n ~ U{n_min..n_max} phonons (mode, energy), uniform volume */
+};
+typedef struct {
+    int32_t kind;
+    int32_t mode;
+    double energy_eV;
+    double origin[3];
+    double logn_median, logn_sigma;
+    uint32_t n_min, n_max;
+    double e_lo_eV, e_hi_eV;
+    int32_t pair_at_origin;
+    int32_t _pad;
+} cmpmc_workload;
+
+typedef struct {
+    int32_t form;                 /* CMPMC_FORM_PLAIN (FP64, reference stream order) or _BATCHED (FP32) */
+    int32_t keep_hits;
+    int32_t want_first_branch;
+    int32_t block_size;           /* 0 = library default / tuned */
+    uint64_t seed;                /* run seed (sub-streams of the batched form, device workloads) */
+    uint64_t stack_headroom;      /* plain form: per-event stack entries beyond the initial ones (0 = 512) */
+    uint64_t lane_stack_depth;    /* batched form: per-lane spawn stack depth (0 = 1024) */
+    uint32_t grid_blocks;         /* batched form: persistent grid size (0 = tuned) */
+    uint32_t refill_period;       /* batched form: steps between refills of dead lanes (0 = tuned) */
+    int32_t poll_cancel;          /* kernels poll the model's cancel flag (cmpmc_model_cancel) */
+    int32_t _pad;
+} cmpmc_run_opts;
+
+typedef struct {
+    cmpmc_sim_stats stats;        /* accumulated into (like SimStats&) */
+    cmpmc_sensor_hit* hits;       /* host buffer of hit_capacity rows (may be NULL if !keep_hits) */
+    uint64_t hit_capacity;
+    uint64_t n_hits;              /* hits written */
+    uint64_t hits_needed;         /* total hits produced (> hit_capacity on CMPMC_ERR_CAPACITY) */
+    uint64_t* hit_offsets;        /* n_events + 1 (may be NULL) */
+    int32_t* first_branch;        /* n_events branch codes of the primary's first step (may be NULL) */
+    double kernel_ms;             /* device time of the transport kernels (HIP events) */
+    uint64_t stack_needed;        /* on CMPMC_ERR_CAPACITY: entries the deepest stack needed */
+} cmpmc_run_out;
+
+/* First-step branch codes */
+enum {
+    CMPMC_BR_NONE = 0, CMPMC_BR_KILLED = 1, CMPMC_BR_ISOTOPE = 2, CMPMC_BR_POLY = 3,
+    CMPMC_BR_DECAY_TT = 4, CMPMC_BR_DECAY_LT = 5, CMPMC_BR_FILM = 6, CMPMC_BR_ESCAPE = 7,
+    CMPMC_BR_SURF_ANH = 8, CMPMC_BR_SPECULAR = 9, CMPMC_BR_DIFFUSE = 10, CMPMC_BR_ROULETTE = 11
+};
+
+/* ---------------------------------------------------------------- entry points */
+/* Last error message of the calling thread (empty string if none). */
+const char* cmpmc_last_error(void);
+
+/* Library / device information: writes up to len bytes of "gfx950, <n> CUs, ..." */
+int cmpmc_device_info(char* buf, size_t len);
+
+/* sizeof of the ABI structs, for binding checks: 0 crystal, 1 film, 2 surface,
+ * 3 detector, 4 lattice, 5 phonon track, 6 charge track, 7 hit, 8 stats,
+ * 9 event batch, 10 workload, 11 run opts, 12 run out. Returns 0 for other values. */
+size_t cmpmc_abi_sizeof(int which);
+
+/* Germanium preset: materials/presets.hpp:18-25 (Germanium()) and the CrystalConfig
+ * defaults it inherits (crystal_config.hpp:43-134). */
+int cmpmc_crystal_germanium(cmpmc_crystal* out);
+/* AluminumFilm(): presets.hpp:360-362 */
+int cmpmc_film_aluminum(cmpmc_film* out);
+/* DetectorModel{} defaults: engine.hpp:96-150 with diffuse_al_sensor_surface() on both
+ * faces and diffuse_bare_surface() on the side wall (surface_property.hpp:232-248). */
+int cmpmc_detector_default(cmpmc_detector* out);
+
+/* CrystalLattice(const CrystalConfig&) table build: crystal_lattice.hpp:45-55, 340-477.
+ * oct_lut: 3*256*256*4 floats; cdf_lt, cdf_tt: 1024 doubles each (caller-owned);
+ * `out` receives cfg, the derived charge tensors and the three table pointers. */
+int cmpmc_lattice_build(const cmpmc_crystal* cfg, float* oct_lut, double* cdf_lt, double* cdf_tt,
+                        cmpmc_lattice* out);
+
+/* CrystalLattice::map_k_to_vg (octahedral, crystal_lattice.hpp:144-176) on the host, for
+ * n directions k[3*n] -> vg_dir[3*n], vg_mag[n]. Used by make_track in generators. */
+int cmpmc_map_k_to_vg(const cmpmc_lattice* lat, int mode, uint64_t n, const double* k,
+                      double* vg_dir, double* vg_mag);
+
+/* SimulationEngine(const CrystalLattice&, const DetectorModel&): engine.hpp:163-169.
+ * Uploads the tables and the detector to the current device. */
+int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector* det, cmpmc_model** out);
+int cmpmc_model_destroy(cmpmc_model* m);
+
+/* Sets (value != 0) or clears the model's host-mapped cancel flag. Runs with
+ * opts->poll_cancel stop early and return CMPMC_ERR_CANCELLED while it is set. No
+ * counterpart in the reference (its workers stop only on an exception,
+ * parallel.hpp:96, 116-120). */
+int cmpmc_model_cancel(cmpmc_model* m, int value);
+
+/* Releases the model's cached device buffers (they are grown on demand by runs). */
+int cmpmc_model_trim(cmpmc_model* m);
+
+/* SimulationEngine::transport(stack, hits, stats, rng): engine.hpp:403-582, exact
+ * reference semantics (one FP64 lane, LIFO, one stream). Consumes the n tracks, appends
+ * hits (host buffer, capacity hit_cap, *n_hits in/out), accumulates stats and advances
+ * rng_state[4]. `stream` is a hipStream_t (NULL = default stream). */
+int cmpmc_transport_serial(cmpmc_model* m, const cmpmc_phonon_track* stack, uint64_t n,
+                           uint64_t rng_state[4], cmpmc_sensor_hit* hits, uint64_t hit_cap,
+                           uint64_t* n_hits, cmpmc_sim_stats* stats, void* stream);
+
+/* SimulationEngine::transport_charges(charges, phonons, hits, stats, rng):
+ * engine.hpp:192-380 (first-order stepping, RecordOnly release). Luke phonons that
+ * are enqueued (luke_sample_fraction > 0) are written to phonons_out (capacity
+ * phonon_cap, *n_phonons in/out) for a following cmpmc_transport_serial call. */
+int cmpmc_transport_charges_serial(cmpmc_model* m, const cmpmc_charge_track* charges, uint64_t n,
+                                   uint64_t rng_state[4], cmpmc_phonon_track* phonons_out,
+                                   uint64_t phonon_cap, uint64_t* n_phonons,
+                                   cmpmc_sensor_hit* hits, uint64_t hit_cap, uint64_t* n_hits,
+                                   cmpmc_sim_stats* stats, void* stream);
+
+/* run_events(engine, n_events, generator, options): transport/parallel.hpp:76-148, for
+ * events whose initial stacks the host generator already produced (`batch`). Charges of
+ * an event are transported first and their Luke phonons join that event's phonons.
+ * Hits come back ordered by event (hit_offsets). */
+int cmpmc_run_events(cmpmc_model* m, const cmpmc_event_batch* batch, const cmpmc_run_opts* opts,
+                     cmpmc_run_out* out, void* stream);
+
+/* run_events with the generator on the device: n_events events of workload `wl`,
+ * event i drawing from Rng::for_stream(opts->seed, i) (rng.hpp:34-36). */
+int cmpmc_run_workload(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events,
+                       const cmpmc_run_opts* opts, cmpmc_run_out* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -48,7 +48,9 @@ void     orc_lambertian_dir(orc_rng* r, const double n[3], double out[3]); /* rn
  * gets Rng(event_base ^ mix64(key + 0xd1b54a32d192ed03)), with
  * event_base = mix64(mix64(seed) ^ mix64(event + 0x632be59bd9b4e019)).  A daughter
  * spawned as the n-th child (n = 1, 2, ...) of a track with key K gets key
- * mix64(K ^ (n * 0x9e3779b97f4a7c15)). */
+ * mix64(K ^ (n * 0x9e3779b97f4a7c15)).  Device-generated initial phonons of the
+ * lognormal / uniform-count workloads draw their initial state from the generator
+ * stream of key j | 2^62 and are then transported with the stream of key j. */
 uint64_t orc_event_base(uint64_t seed, uint64_t event);
 void     orc_rng_for_track(orc_rng* r, uint64_t event_base, uint64_t key);
 uint64_t orc_child_key(uint64_t parent_key, uint32_t spawn_no);

@@ -381,6 +381,33 @@ class Engine:
             out[i, 3] = mag.value
         return out
 
+    def transport_stack(self, tracks: np.ndarray, rng_state) -> dict:
+        """SimulationEngine::transport of one stack (reference semantics): `tracks` is a
+        structured array with pos, k_dir, vg_dir, energy, vg_mag, time, weight, bounces, mode."""
+        L = lib()
+        vec = TrackVec()
+        for row in tracks:
+            t = Track()
+            for i in range(3):
+                t.pos[i], t.k[i], t.vg[i] = row["pos"][i], row["k_dir"][i], row["vg_dir"][i]
+            t.energy, t.vg_mag, t.time, t.weight = row["energy"], row["vg_mag"], row["time"], row["weight"]
+            t.bounces, t.mode = int(row["bounces"]), int(row["mode"])
+            L.orc_track_push(C.byref(vec), C.byref(t))
+        hv = HitVec()
+        st = Stats()
+        r = Rng()
+        for i in range(4):
+            r.s[i] = int(rng_state[i])
+        L.orc_transport(self.ptr, C.byref(vec), C.byref(hv), C.byref(st), C.byref(r), 0, 0, None)
+        n = hv.n
+        hits = np.frombuffer(bytes((Hit * n).from_address(C.addressof(hv.v.contents))), dtype=HIT_DTYPE).copy() \
+            if n else np.zeros(0, dtype=HIT_DTYPE)
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        libc.free(C.cast(vec.v, C.c_void_p))
+        libc.free(C.cast(hv.v, C.c_void_p))
+        return {"stats": st.as_dict(), "stats_raw": bytes(st), "hits": hits, "rng": [r.s[i] for i in range(4)]}
+
     def run(self, wl: Workload, n_events: int, seed: int, threads: int = 0, chunk: int = 16,
             keep_hits: bool = True, batched: bool = False, first_branch: bool = False) -> dict:
         r = RunResult()

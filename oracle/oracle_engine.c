@@ -717,7 +717,9 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
         for (uint32_t j = 0; j < n; ++j) {
             orc_rng tr;
             orc_rng* r = ev_rng;
-            if (batched) { orc_rng_for_track(&tr, base, j); r = &tr; }
+            /* batched: the generator stream of track j is key j | 2^62; transport then
+             * draws from the track's own stream (key j) */
+            if (batched) { orc_rng_for_track(&tr, base, (uint64_t)j | (1ULL << 62)); r = &tr; }
             double pos[3], k[3];
             uniform_volume(D, r, pos);
             int m = mode;
@@ -729,7 +731,7 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
             orc_isotropic_dir(r, k);
             orc_track t;
             orc_make_track(e, pos, k, m, E, 0.0, 1.0, &t);
-            if (batched) { t.key = j; t.rng = tr; }
+            if (batched) { t.key = j; orc_rng_for_track(&t.rng, base, j); }
             /* plain: the first popped track is the last pushed; batched: key 0 */
             t.first_flag = batched ? (j == 0) : (j + 1 == n);
             orc_track_push(phonons, &t);

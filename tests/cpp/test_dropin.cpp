@@ -1,0 +1,147 @@
+// The drop-in C++ API on the GPU, in the style of the reference's tests/test_parallel.cpp
+// and tests/test_engine_energy_closure.cpp: the same user code (lattice, detector,
+// engine, generator lambda, run_events) compiles against include/cmpmc/ and runs on
+// libcmpmc_amd. Prints "name value reference rel_err PASS|FAIL" lines and returns
+// nonzero on any failure.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "cmpmc/cmpmc.hpp"
+
+using namespace cmpmc;
+
+static int g_fail = 0;
+static void check(const std::string& name, double val, double ref, double tol_rel, double tol_abs = 0.0) {
+    const double err = std::abs(val - ref);
+    const double rel = (ref != 0.0) ? err / std::abs(ref) : err;
+    const bool ok = (err <= tol_abs) || (rel <= tol_rel);
+    std::printf("%-60s %14.6g %14.6g %10.3g %s\n", name.c_str(), val, ref, rel, ok ? "PASS" : "FAIL");
+    std::fflush(stdout);
+    if (!ok) ++g_fail;
+}
+
+struct MixedGenerator {   // even events: one 7.5 meV FT phonon; odd events: one e-/h+ pair
+    const SimulationEngine* engine;
+    const ChargePhysics* charges;
+    size_t n_valleys;
+    void operator()(EventInput& in) const {
+        if (in.event_id % 2 == 0) {
+            in.phonons.push_back(engine->make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::FT, 7.5e-3));
+            in.injected_eV += 7.5e-3;
+        } else {
+            const auto v = static_cast<uint16_t>(in.rng.uniform() * n_valleys) % n_valleys;
+            in.charges.push_back(charges->make_charge(ExcitationType::Electron, {0, 0, 0}, in.rng.isotropic_dir(), 1e-6, v));
+            in.charges.push_back(charges->make_charge(ExcitationType::Hole, {0, 0, 0}, in.rng.isotropic_dir(), 1e-6, 0));
+        }
+    }
+};
+
+int main(int argc, char** argv) {
+    std::printf("# test_dropin\n");
+    const uint64_t N = argc > 1 ? std::strtoull(argv[1], nullptr, 10) : 203;
+    const CrystalLattice lattice(materials::Germanium());
+    DetectorModel det;
+    det.e_field = UniformElectricField::from_bias(2.0, 2.54 * units::cm);
+    det.luke_sample_fraction = 0.02;
+    const SimulationEngine engine(lattice, det);
+    const ChargePhysics charges(lattice);
+    const MixedGenerator gen{&engine, &charges, lattice.config().valley_dirs.size()};
+
+    RunOptions opt;
+    opt.seed = 77;
+    opt.threads = 1;
+    const RunResult r1 = run_events(engine, N, gen, opt);
+    for (unsigned t : {2u, 4u}) {
+        opt.threads = t;
+        const RunResult rt = run_events(engine, N, gen, opt);
+        check(std::to_string(t) + " generator threads: stats bit-identical",
+              std::memcmp(&r1.stats, &rt.stats, sizeof(SimStats)) == 0 ? 1 : 0, 1, 0.0);
+        bool same = rt.hits.size() == r1.hits.size() && rt.hit_offsets == r1.hit_offsets;
+        for (size_t i = 0; same && i < rt.hits.size(); ++i)
+            same = std::memcmp(&rt.hits[i], &r1.hits[i], sizeof(SensorHit)) == 0;
+        check(std::to_string(t) + " generator threads: hits identical", same ? 1 : 0, 1, 0.0);
+    }
+    // plain serial loop over the same per-event streams (SimulationEngine::transport*)
+    SimStats serial;
+    std::vector<SensorHit> serial_hits;
+    {
+        std::vector<PhononTrack> ph;
+        std::vector<ChargeTrack> ch;
+        for (uint64_t ev = 0; ev < std::min<uint64_t>(N, 24); ++ev) {
+            Rng rng = Rng::for_stream(opt.seed, ev);
+            ph.clear();
+            ch.clear();
+            EventInput in{ev, rng, ph, ch, 0.0};
+            gen(in);
+            serial.total_injected_eV += in.injected_eV;
+            if (!ch.empty()) engine.transport_charges(ch, ph, serial_hits, serial, rng);
+            if (!ph.empty()) engine.transport(ph, serial_hits, serial, rng);
+        }
+        opt.threads = 1;
+        const RunResult r24 = run_events(engine, std::min<uint64_t>(N, 24), gen, opt);
+        check("vs serial loop: phonon steps", static_cast<double>(r24.stats.total_steps),
+              static_cast<double>(serial.total_steps), 0.0);
+        check("vs serial loop: charge steps", static_cast<double>(r24.stats.charge_steps),
+              static_cast<double>(serial.charge_steps), 0.0);
+        check("vs serial loop: hit count", static_cast<double>(r24.hits.size()), static_cast<double>(serial_hits.size()), 0.0);
+        check("vs serial loop: deposited energy", r24.stats.total_deposited_eV, serial.total_deposited_eV, 1e-12);
+    }
+    check("hit_offsets has n + 1 entries", static_cast<double>(r1.hit_offsets.size()), static_cast<double>(N + 1), 0.0);
+    check("last hit offset = number of hits", static_cast<double>(r1.hit_offsets.back()), static_cast<double>(r1.hits.size()), 0.0);
+    {
+        RunOptions o;
+        o.seed = 5;
+        auto phonon_gen = [&](EventInput& in) {
+            in.phonons.push_back(engine.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::L, 4e-3));
+            in.injected_eV += 4e-3;
+        };
+        const RunResult r = run_events(engine, 500, phonon_gen, o);
+        check("phonon run: deposited + escaped = injected", r.stats.total_deposited_eV + r.stats.escaped_or_killed_eV,
+              r.stats.total_injected_eV, 1e-9);
+        o.keep_hits = false;
+        const RunResult rn = run_events(engine, 500, phonon_gen, o);
+        check("keep_hits = false: no hits stored", static_cast<double>(rn.hits.size()), 0.0, 0.0);
+        check("keep_hits = false: same statistics", std::memcmp(&r.stats, &rn.stats, sizeof(SimStats)) == 0 ? 1 : 0, 1, 0.0);
+        o.keep_hits = true;
+        o.form = CMPMC_FORM_BATCHED;
+        const RunResult rb = run_events(engine, 500, phonon_gen, o);
+        check("batched form: deposited + escaped = injected", rb.stats.total_deposited_eV + rb.stats.escaped_or_killed_eV,
+              rb.stats.total_injected_eV, 1e-9);
+        check("batched form: deposited energy vs plain (2%)", rb.stats.total_deposited_eV, r.stats.total_deposited_eV, 0.02);
+    }
+    {   // test_engine_energy_closure.cpp: 2000 L phonons at 4 meV on one stack, Rng(1)
+        Rng rng(1);
+        std::vector<PhononTrack> st;
+        std::vector<SensorHit> hits;
+        SimStats S;
+        double Ein = 0;
+        for (int i = 0; i < 2000; ++i) {
+            st.push_back(engine.make_track({0, 0, 0}, rng.isotropic_dir(), PhononMode::L, 0.004));
+            Ein += 0.004;
+        }
+        engine.transport(st, hits, S, rng);
+        check("transport(): deposited + escaped/killed = injected", S.total_deposited_eV + S.escaped_or_killed_eV, Ein, 1e-9);
+        check("transport(): stack consumed", static_cast<double>(st.size()), 0.0, 0.0);
+    }
+    {   // a worker exception reaches the caller (guarded Matrix IV model)
+        CrystalConfig cfg = materials::Germanium();
+        cfg.iv_model = IVRateModel::Matrix;
+        const CrystalLattice guarded_lattice(cfg);
+        const SimulationEngine guarded(guarded_lattice, det);
+        const ChargePhysics gc(guarded_lattice);
+        const MixedGenerator g{&guarded, &gc, cfg.valley_dirs.size()};
+        RunOptions o;
+        o.threads = 4;
+        bool caught = false;
+        try { (void)run_events(guarded, 64, g, o); } catch (const UnvalidatedModuleError&) { caught = true; }
+        check("guarded model: UnvalidatedModuleError on the caller", caught ? 1 : 0, 1, 0.0);
+        bool caught2 = false;
+        auto throwing = [](EventInput& in) { if (in.event_id == 17) throw std::runtime_error("generator failure"); };
+        try { (void)run_events(engine, 64, throwing, o); } catch (const std::runtime_error&) { caught2 = true; }
+        check("generator exception rethrown on the caller", caught2 ? 1 : 0, 1, 0.0);
+    }
+    std::printf("# %d check(s) failed\n", g_fail);
+    return g_fail ? 1 : 0;
+}

@@ -1,0 +1,98 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own implementation,
+compiled in place by oracle/Makefile into oracle/_ref/libcmpmc_ref.so.
+
+Run in a container that has the reference tree:  python tests/golden/make_golden.py
+The fixtures are data only (inputs and the reference's outputs); they pin the CPU
+oracle where the reference build is absent (for example on a GPU box).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle_ctypes as O  # noqa: E402
+
+
+def main():
+    O.build()
+    R = O.ref()
+    out = {}
+    # 1. Rng::for_stream streams: first 16 next_u64 for seeds x streams
+    seeds, streams = [1, 42, 77], [0, 1, 2, 9999, 999999]
+    rs = np.zeros((len(seeds), len(streams), 16), dtype=np.uint64)
+    for i, s in enumerate(seeds):
+        for j, t in enumerate(streams):
+            a = (O.u64 * 16)()
+            R.ref_rng_stream(s, t, 16, a)
+            rs[i, j] = list(a)
+    out["rng_seeds"] = np.array(seeds, dtype=np.uint64)
+    out["rng_streams"] = np.array(streams, dtype=np.uint64)
+    out["rng_values"] = rs
+    # 2. samplers of Rng(2024): normal, isotropic, lambertian about (0, 0, -1)
+    nrm = np.array([0.0, 0.0, -1.0])
+    samp = np.zeros(7 * 256)
+    R.ref_rng_samplers(2024, 256, O.dptr(nrm), O.dptr(samp))
+    out["sampler_values"] = samp
+    # 3. lattice-derived constants and map_k_to_vg queries
+    info = np.zeros(16)
+    R.ref_lattice_info(O.dptr(info))
+    out["lattice_info"] = info
+    k = np.random.default_rng(12345).normal(size=(512, 3))
+    out["map_k"] = k
+    vg = np.zeros((3, 512, 4))
+    for m in range(3):
+        tmp = np.zeros((512, 4))
+        R.ref_map_k_to_vg(m, 512, O.dptr(np.ascontiguousarray(k)), O.dptr(tmp))
+        vg[m] = tmp
+    out["map_vg"] = vg
+    # 4. decay energy-fraction draws from Rng(99)
+    lt, tt = np.zeros(1000), np.zeros(1000)
+    R.ref_decay_fractions(99, 1000, O.dptr(lt), O.dptr(tt))
+    out["decay_lt"], out["decay_tt"] = lt, tt
+    # 5. Kaplan cascade outcomes (200 calls from Rng(seed) at each energy)
+    for tag, E, seed in (("k75", 7.5e-3, 11), ("k15", 15e-3, 12), ("k31", 31e-3, 13)):
+        edep, ns, sm = np.zeros(200), np.zeros(200, dtype=np.int32), np.zeros(200)
+        st = (O.u64 * 4)()
+        R.ref_absorb_phonon(E, seed, 200, O.dptr(edep), ns.ctypes.data_as(O.C.POINTER(O.C.c_int)), O.dptr(sm), st)
+        out[f"{tag}_edep"], out[f"{tag}_nrefl"], out[f"{tag}_reflsum"] = edep, ns, sm
+        out[f"{tag}_state"] = np.array(list(st), dtype=np.uint64)
+    # 6. intersect_exit of the default cylinder
+    g = np.random.default_rng(7)
+    p = np.c_[g.uniform(-0.026, 0.026, (300, 2)), g.uniform(-0.0125, 0.0125, 300)]
+    v = g.normal(size=(300, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    res = np.zeros((300, 5))
+    R.ref_intersect(300, O.dptr(np.ascontiguousarray(p)), O.dptr(np.ascontiguousarray(v)), O.dptr(res))
+    out["isect_p"], out["isect_v"], out["isect_out"] = p, v, res
+    # 7. run_events end to end
+    runs = {
+        "c1": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 400, 42, O.default_detector()),
+        "lognormal": (O.workload(O.WL_PHONON_LOGN, logn_median=4.0), 40, 42, O.default_detector()),
+    }
+    det = O.default_detector()
+    O.set_bias(det, 2.0)
+    det.luke_sample_fraction = 0.02
+    runs["mixed"] = (O.workload(O.WL_MIXED), 41, 77, det)
+    d3 = O.default_detector()
+    O.set_bias(d3, 4.0)
+    d3.luke_sample_fraction = 0.0
+    runs["pairs"] = (O.workload(O.WL_PAIR), 6, 42, d3)
+    dd = O.default_detector()
+    O.demo_polynomial_surfaces(dd)
+    runs["bench_demo"] = (O.workload(O.WL_PHONON_POINT, mode=2, energy_eV=7.5e-3), 100, 12345, dd)
+    for tag, (wl, n, seed, d) in runs.items():
+        r = O.ref_run(wl, n, seed, detector=d, threads=4)
+        out[f"run_{tag}_stats"] = np.frombuffer(r["stats_raw"], dtype=np.uint8)
+        out[f"run_{tag}_offsets"] = r["hit_offsets"]
+        h = r["hits"][:600]
+        out[f"run_{tag}_hits"] = np.c_[h["pos"], h["time"], h["edep"], h["incident"], h["surface"], h["mode"]]
+    path = os.path.join(HERE, "reference_vectors.npz")
+    np.savez_compressed(path, **out)
+    print(path, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    main()

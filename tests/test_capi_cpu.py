@@ -1,0 +1,113 @@
+"""The C ABI library without a GPU: it loads, exports every symbol include/cmpmc_amd.h
+declares, its struct sizes match the bindings, and its host-side pieces (presets, the
+CrystalLattice table build, host map_k_to_vg, argument checks) equal the oracle's."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cmpmc_amd.h")
+GOLD = os.path.join(ROOT, "tests", "golden", "reference_vectors.npz")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\**\s+\**(cmpmc_\w+)\s*\(", text, re.M)))
+
+
+def test_exports_every_declared_symbol(lib_cmpmc):
+    L = lib_cmpmc.capi.load()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/cmpmc_amd.h but not exported"
+    assert set(lib_cmpmc.capi.EXPORTED_SYMBOLS) <= set(names)
+
+
+def test_struct_sizes(lib_cmpmc):
+    L = lib_cmpmc.capi.load()
+    lib_cmpmc.capi.check_abi(L)
+    # byte layouts of the reference's records (static_assert'ed in the survey)
+    assert L.cmpmc_abi_sizeof(5) == 128 and L.cmpmc_abi_sizeof(6) == 128
+    assert L.cmpmc_abi_sizeof(7) == 56 and L.cmpmc_abi_sizeof(8) == 208
+
+
+def test_presets_equal_oracle(lib_cmpmc, oracle):
+    a = lib_cmpmc.germanium()
+    b = oracle.germanium()
+    for name, _ in lib_cmpmc.capi.Crystal._fields_:
+        va, vb = getattr(a, name), getattr(b, name if name != "lambda_" else "lambda_")
+        if hasattr(va, "__len__"):
+            assert np.array_equal(np.ctypeslib.as_array(va), np.ctypeslib.as_array(vb)), name
+        else:
+            assert va == vb, name
+    da, db = lib_cmpmc.default_detector(), oracle.default_detector()
+    assert da.radius == db.radius and da.half_z == db.half_z and da.max_bounces == db.max_bounces
+    for s in range(3):
+        assert da.surf[s].has_film == db.surf[s].has_film
+        assert da.surf[s].prob_abs == db.surf[s].prob_abs
+        assert da.surf[s].film.gap_energy == db.surf[s].film.gap_energy
+
+
+def test_lattice_tables_bit_exact(lib_cmpmc, oracle):
+    lat = lib_cmpmc.Lattice()
+    assert np.array_equal(lat.oct, oracle.Engine().oct_table())
+    g = np.load(GOLD)
+    for m in range(3):
+        d, mag = lat.map_k_to_vg(m, g["map_k"])
+        assert np.array_equal(d, g["map_vg"][m][:, :3]) and np.array_equal(mag, g["map_vg"][m][:, 3])
+    # decay tables through the reference's own draws (Rng(99), sample_*_energy_fraction)
+    r = lib_cmpmc.Rng(99)
+    lt, tt = [], []
+    for _ in range(1000):
+        for cdf, acc in ((lat.cdf_lt, lt), (lat.cdf_tt, tt)):
+            u = r.uniform() * 1023
+            i = min(int(u), 1022)
+            f = u - i
+            acc.append(cdf[i] * (1.0 - f) + cdf[i + 1] * f)
+    assert np.array_equal(lt, g["decay_lt"]) and np.array_equal(tt, g["decay_tt"])
+    info = g["lattice_info"]
+    c = lat.c
+    assert [c.e_cond_mass_eV, c.h_mass_eV, c.l0_e, c.l0_h, c.k_sound_e, c.k_sound_h] == list(info[:6])
+
+
+def test_host_rng_streams(lib_cmpmc):
+    g = np.load(GOLD)
+    for i, s in enumerate(g["rng_seeds"]):
+        for j, t in enumerate(g["rng_streams"]):
+            r = lib_cmpmc.Rng.for_stream(int(s), int(t))
+            assert [r.next_u64() for _ in range(16)] == [int(x) for x in g["rng_values"][i, j]]
+
+
+def test_argument_errors(lib_cmpmc):
+    cm = lib_cmpmc
+    L = cm.capi.load()
+    lat = cm.Lattice()
+    k = np.zeros(3)
+    assert L.cmpmc_map_k_to_vg(C.byref(lat.c), 7, 1, k.ctypes.data, k.ctypes.data, k.ctypes.data) == cm.capi.ERR_INVALID
+    assert "bad argument" in L.cmpmc_last_error().decode()
+    assert L.cmpmc_lattice_build(None, None, None, None, None) == cm.capi.ERR_INVALID
+    bad = cm.germanium()
+    bad.n_valleys = 99
+    o = np.zeros((3, 256, 256, 4), dtype=np.float32)
+    a, b = np.zeros(1024), np.zeros(1024)
+    out = cm.capi.Lattice()
+    assert L.cmpmc_lattice_build(C.byref(bad), o.ctypes.data, a.ctypes.data, b.ctypes.data, C.byref(out)) == 1
+
+
+def test_model_without_device_reports_it(lib_cmpmc):
+    cm = lib_cmpmc
+    n = C.c_int(0)
+    try:
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipGetDeviceCount(C.byref(n))
+    except OSError:
+        pass
+    if n.value > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(cm.CmpmcError) as e:
+        cm.Engine()
+    assert e.value.code == cm.capi.ERR_NO_DEVICE

@@ -1,0 +1,204 @@
+"""GPU parity of the HIP transport kernels against the CPU oracle (same seeds).
+
+Plain form (FP64, reference stream order): the first-step branch of every event's
+primary track is identical to the oracle's, and because both follow the reference
+draw for draw, per-event integer outcomes agree for almost every event (only
+ocml-vs-glibc ULP differences in log/sin/cos/acos/exp can flip a later comparison).
+Batched form (FP32, per-track sub-streams): compared with the oracle run in the same
+sub-stream scheme: identical first-step branch counts, ledger closure 1e-9, and hit
+time / energy / position distributions (two-sample KS p > 0.01, means within 3 sigma).
+"""
+import numpy as np
+import pytest
+from scipy import stats as sst
+
+pytestmark = pytest.mark.gpu
+
+N_C1 = 2000
+SEED = 42
+
+
+def _closure(s):
+    inj = s["total_injected_eV"]
+    return abs(inj - s["total_deposited_eV"] - s["escaped_or_killed_eV"]) / inj
+
+
+def _per_event_counts(offsets):
+    return np.diff(offsets.astype(np.int64))
+
+
+def _ks_ok(a, b, name, pmin=0.01):
+    p = sst.ks_2samp(a, b).pvalue
+    assert p > pmin, f"{name}: KS p = {p:.3g}"
+
+
+def _mean_ok(a, b, name, nsig=3.0):
+    sa = a.std(ddof=1) / np.sqrt(len(a))
+    sb = b.std(ddof=1) / np.sqrt(len(b))
+    z = abs(a.mean() - b.mean()) / np.hypot(sa, sb)
+    assert z < nsig, f"{name}: means differ by {z:.2f} sigma"
+
+
+@pytest.fixture(scope="module")
+def c1(lib_cmpmc, oracle):
+    cm = lib_cmpmc
+    eng = cm.Engine()
+    wl = cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
+    owl = oracle.workload(oracle.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
+    oeng = oracle.Engine()
+    return cm, eng, wl, oracle, oeng, owl
+
+
+def test_plain_c1_matches_oracle(c1):
+    cm, eng, wl, O, oeng, owl = c1
+    print(f"case: plain C1 n={N_C1} seed={SEED}", flush=True)
+    g = eng.run_workload(wl, N_C1, seed=SEED, form=cm.FORM_PLAIN, first_branch=True)
+    o = oeng.run(owl, N_C1, SEED, first_branch=True)
+    assert np.array_equal(g["first_branch"], o["first_branch"]), "first-step branch codes differ"
+    gs, os_ = g["stats"], o["stats"]
+    assert gs["total_injected_eV"] == pytest.approx(N_C1 * 7.5e-3, rel=1e-12)
+    assert _closure(gs) <= 1e-9
+    # integer outcomes per event: identical for all but a few events
+    same = _per_event_counts(g["hit_offsets"]) == _per_event_counts(o["hit_offsets"])
+    assert same.mean() >= 0.97, f"per-event hit counts agree for {same.mean():.3f} of events"
+    for k in ("total_steps", "isotope_scatters", "decays_L_to_TT", "decays_L_to_LT", "sensor_hits", "surface_refls"):
+        assert gs[k] == pytest.approx(os_[k], rel=0.02), k
+    assert gs["total_deposited_eV"] == pytest.approx(os_["total_deposited_eV"], rel=0.02)
+    h, q = g["hits"], o["hits"]
+    _ks_ok(h["time"], q["time"], "hit time")
+    _ks_ok(h["edep"], q["edep"], "hit edep")
+    _ks_ok(h["pos"][:, 2], q["pos"][:, 2], "hit z")
+
+
+def test_plain_host_batch_equals_device_generator(c1):
+    """run_events with host-generated stacks (post-generator streams) = device generator."""
+    cm, eng, wl, O, oeng, owl = c1
+    n = 300
+    lat = eng.lattice
+    ks, states = [], []
+    for ev in range(n):
+        r = cm.Rng.for_stream(SEED, ev)
+        ks.append(r.isotropic_dir())
+        states.append(r.state())
+    tracks = cm.make_tracks(lat, np.zeros((n, 3)), np.array(ks), 0, 7.5e-3)
+    offs = np.arange(n + 1, dtype=np.uint64)
+    a = eng.run_events(tracks, offs, np.array(states), injected=np.full(n, 7.5e-3), first_branch=True)
+    b = eng.run_workload(wl, n, seed=SEED, first_branch=True)
+    assert a["stats_raw"] == b["stats_raw"]
+    assert np.array_equal(a["hit_offsets"], b["hit_offsets"])
+    assert np.array_equal(a["hits"], b["hits"])
+    assert np.array_equal(a["first_branch"], b["first_branch"])
+
+
+def test_transport_serial_matches_oracle(c1):
+    """SimulationEngine::transport on one stack: same hits/stats as the oracle for a
+    stack whose first steps share the reference draw order."""
+    cm, eng, wl, O, oeng, owl = c1
+    rng = cm.Rng(7)
+    k = np.array([rng.isotropic_dir() for _ in range(20)])
+    tracks = cm.make_tracks(eng.lattice, np.zeros((20, 3)), k, 0, 4e-3)
+    state = cm.Rng.for_stream(1, 3).state()
+    g = eng.transport(tracks, state)
+    o = oeng.transport_stack(tracks, state)
+    assert g["stats"]["total_steps"] == pytest.approx(o["stats"]["total_steps"], rel=0.05)
+    assert abs(g["stats"]["total_deposited_eV"] + g["stats"]["escaped_or_killed_eV"] - 20 * 4e-3) <= 1e-9 * 20 * 4e-3
+    assert len(g["hits"]) > 0
+
+
+def test_batched_c1_matches_oracle_batched(c1):
+    cm, eng, wl, O, oeng, owl = c1
+    print(f"case: batched C1 n={N_C1} seed={SEED}", flush=True)
+    g = eng.run_workload(wl, N_C1, seed=SEED, form=cm.FORM_BATCHED, first_branch=True)
+    o = oeng.run(owl, N_C1, SEED, batched=True, first_branch=True)
+    bg = np.bincount(g["first_branch"], minlength=16)
+    bo = np.bincount(o["first_branch"], minlength=16)
+    assert np.array_equal(bg, bo), f"first-step branch counts: gpu {bg} oracle {bo}"
+    gs, os_ = g["stats"], o["stats"]
+    assert _closure(gs) <= 1e-9
+    assert gs["total_injected_eV"] == pytest.approx(N_C1 * 7.5e-3, rel=1e-12)
+    h, q = g["hits"], o["hits"]
+    for f in ("time", "edep"):
+        _ks_ok(h[f], q[f], f"hit {f}")
+        _mean_ok(h[f], q[f], f"hit {f}")
+    for i, ax in enumerate("xyz"):
+        _ks_ok(h["pos"][:, i], q["pos"][:, i], f"hit {ax}")
+    assert gs["total_steps"] == pytest.approx(os_["total_steps"], rel=0.05)
+
+
+def test_batched_deterministic(c1):
+    cm, eng, wl, O, oeng, owl = c1
+    a = eng.run_workload(wl, 500, seed=3, form=cm.FORM_BATCHED)
+    b = eng.run_workload(wl, 500, seed=3, form=cm.FORM_BATCHED, refill_period=7, grid_blocks=64)
+    assert a["stats_raw"] == b["stats_raw"], "batched stats depend on the schedule"
+    assert np.array_equal(a["hit_offsets"], b["hit_offsets"])
+    assert np.array_equal(a["hits"], b["hits"])
+
+
+def test_batched_stack_overflow_retry(c1):
+    """A lane spawn stack that is too small is detected and the run is repeated with a
+    deeper one; the result is the same as with ample room."""
+    cm, eng, wl, O, oeng, owl = c1
+    a = eng.run_workload(wl, 300, seed=5, form=cm.FORM_BATCHED, lane_depth=2)
+    b = eng.run_workload(wl, 300, seed=5, form=cm.FORM_BATCHED)
+    assert a["stats_raw"] == b["stats_raw"]
+
+
+def test_cancel_flag(c1):
+    cm, eng, wl, O, oeng, owl = c1
+    eng.cancel(True)
+    try:
+        with pytest.raises(cm.CmpmcError) as ei:
+            eng.run_workload(wl, 200, seed=1, form=cm.FORM_BATCHED, poll_cancel=True)
+        assert ei.value.code == 6
+    finally:
+        eng.cancel(False)
+    r = eng.run_workload(wl, 50, seed=1, form=cm.FORM_BATCHED, poll_cancel=True)
+    assert r["stats"]["total_steps"] > 0
+
+
+def test_lognormal_plain_and_batched(lib_cmpmc, oracle):
+    cm, O = lib_cmpmc, oracle
+    eng = cm.Engine()
+    wl = cm.workload(cm.WL_PHONON_LOGN, logn_median=4.0, logn_sigma=1.0, n_max=200)
+    owl = O.workload(O.WL_PHONON_LOGN, logn_median=4.0, logn_sigma=1.0, n_max=200)
+    oeng = O.Engine()
+    n = 200
+    print(f"case: lognormal n={n}", flush=True)
+    gp = eng.run_workload(wl, n, seed=11, form=cm.FORM_PLAIN, first_branch=True)
+    op = oeng.run(owl, n, 11, first_branch=True)
+    assert gp["stats"]["total_injected_eV"] == pytest.approx(op["stats"]["total_injected_eV"], rel=1e-12)
+    assert np.mean(gp["first_branch"] == op["first_branch"]) >= 0.99
+    gb = eng.run_workload(wl, n, seed=11, form=cm.FORM_BATCHED, first_branch=True)
+    ob = oeng.run(owl, n, 11, batched=True, first_branch=True)
+    assert gb["stats"]["total_injected_eV"] == pytest.approx(ob["stats"]["total_injected_eV"], rel=1e-12)
+    assert np.array_equal(np.bincount(gb["first_branch"], minlength=16), np.bincount(ob["first_branch"], minlength=16))
+    for r in (gp, gb):
+        assert _closure(r["stats"]) <= 1e-9
+    _ks_ok(gb["hits"]["edep"], ob["hits"]["edep"], "lognormal hit edep")
+    _ks_ok(gb["hits"]["time"], ob["hits"]["time"], "lognormal hit time")
+
+
+def test_charge_pairs_plain_and_batched(lib_cmpmc, oracle):
+    cm, O = lib_cmpmc, oracle
+    det = cm.default_detector()
+    cm.set_bias(det, 4.0)
+    det.luke_sample_fraction = 0.0
+    eng = cm.Engine(detector=det)
+    odet = O.default_detector()
+    O.set_bias(odet, 4.0)
+    odet.luke_sample_fraction = 0.0
+    oeng = O.Engine(detector=odet)
+    n = 16
+    print(f"case: charge pairs n={n}", flush=True)
+    wl = cm.workload(cm.WL_PAIR)
+    owl = O.workload(O.WL_PAIR)
+    gp = eng.run_workload(wl, n, seed=42, form=cm.FORM_PLAIN)
+    op = oeng.run(owl, n, 42)
+    for k in ("charge_steps", "luke_phonons_emitted", "charges_collected"):
+        assert gp["stats"][k] == pytest.approx(op["stats"][k], rel=0.02), k
+    assert gp["stats"]["charges_collected"] == 2 * n
+    gb = eng.run_workload(wl, n, seed=42, form=cm.FORM_BATCHED)
+    ob = oeng.run(owl, n, 42, batched=True)
+    assert gb["stats"]["charges_collected"] == 2 * n
+    assert gb["stats"]["luke_phonons_emitted"] == pytest.approx(ob["stats"]["luke_phonons_emitted"], rel=0.05)
+    assert gb["stats"]["total_luke_phonon_eV"] == pytest.approx(ob["stats"]["total_luke_phonon_eV"], rel=0.05)

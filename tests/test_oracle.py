@@ -1,0 +1,371 @@
+"""Pins the CPU oracle (oracle/) before it is trusted as the checker.
+
+1. Golden vectors written by the reference's own implementation (tests/golden/
+   reference_vectors.npz, see make_golden.py): the oracle must reproduce them bit for
+   bit (RNG streams, samplers, v_g lookups, decay tables, Kaplan cascades, ray exits and
+   whole run_events results, including hits).
+2. Where the reference build itself is present (oracle/_ref), larger bit-exact runs.
+3. What the mathematics fixes independently of both: cubic Christoffel closed forms,
+   momentum closure of the decay angles, cascade energy closure, the 2D/Omega
+   pair-breaking limit, sampler moments against the Kaplan densities, ledger closure.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD)
+
+
+def test_rng_streams_golden(oracle, gold):
+    O = oracle
+    L = O.lib()
+    for i, s in enumerate(gold["rng_seeds"]):
+        for j, t in enumerate(gold["rng_streams"]):
+            r = O.Rng()
+            L.orc_rng_for_stream(C.byref(r), int(s), int(t))
+            got = [L.orc_next_u64(C.byref(r)) for _ in range(16)]
+            assert got == [int(x) for x in gold["rng_values"][i, j]], (s, t)
+
+
+def test_uniform_open_interval(oracle):
+    O = oracle
+    L = O.lib()
+    r = O.Rng()
+    L.orc_rng_seed(C.byref(r), 5)
+    u = np.array([L.orc_uniform(C.byref(r)) for _ in range(20000)])
+    assert (u > 0).all() and (u < 1).all()
+    assert abs(u.mean() - 0.5) < 0.01
+
+
+def test_samplers_golden(oracle, gold):
+    O = oracle
+    L = O.lib()
+    r = O.Rng()
+    L.orc_rng_seed(C.byref(r), 2024)
+    out = np.zeros(7 * 256)
+    d = np.zeros(3)
+    for i in range(256):
+        out[4 * i] = L.orc_normal(C.byref(r))
+        L.orc_isotropic_dir(C.byref(r), O.dptr(d))
+        out[4 * i + 1:4 * i + 4] = d
+    n = np.array([0.0, 0.0, -1.0])
+    for i in range(256):
+        L.orc_lambertian_dir(C.byref(r), O.dptr(n), O.dptr(d))
+        out[1024 + 3 * i:1024 + 3 * i + 3] = d
+    assert np.array_equal(out, gold["sampler_values"])
+
+
+def test_map_k_to_vg_golden(oracle, gold):
+    eng = oracle.Engine()
+    for m in range(3):
+        assert np.array_equal(eng.map_k_to_vg(m, gold["map_k"]), gold["map_vg"][m])
+
+
+def test_decay_fractions_golden(oracle, gold):
+    O = oracle
+    L = O.lib()
+    eng = O.Engine()
+    r = O.Rng()
+    L.orc_rng_seed(C.byref(r), 99)
+    lt, tt = [], []
+    for _ in range(1000):
+        lt.append(L.orc_sample_lt_fraction(eng.lattice_ptr(), C.byref(r)))
+        tt.append(L.orc_sample_tt_fraction(eng.lattice_ptr(), C.byref(r)))
+    assert np.array_equal(lt, gold["decay_lt"]) and np.array_equal(tt, gold["decay_tt"])
+
+
+@pytest.mark.parametrize("tag,E,seed", [("k75", 7.5e-3, 11), ("k15", 15e-3, 12), ("k31", 31e-3, 13)])
+def test_kaplan_cascade_golden(oracle, gold, tag, E, seed):
+    O = oracle
+    L = O.lib()
+    f = O.Film()
+    L.orc_film_aluminum(C.byref(f))
+    r = O.Rng()
+    L.orc_rng_seed(C.byref(r), seed)
+    refl = np.zeros(256)
+    nr = C.c_int()
+    for i in range(200):
+        edep = L.orc_absorb_phonon(C.byref(f), E, O.dptr(refl), C.byref(nr), C.byref(r))
+        assert edep == gold[f"{tag}_edep"][i]
+        assert nr.value == gold[f"{tag}_nrefl"][i]
+        s = 0.0
+        for x in refl[:nr.value]:
+            s += x
+        assert s == gold[f"{tag}_reflsum"][i]
+    assert [r.s[i] for i in range(4)] == [int(x) for x in gold[f"{tag}_state"]]
+
+
+def test_intersect_golden(oracle, gold):
+    O = oracle
+    L = O.lib()
+    det = O.default_detector()
+    n_in = np.zeros(3)
+    sid = C.c_int()
+    for i in range(len(gold["isect_p"])):
+        p = np.ascontiguousarray(gold["isect_p"][i])
+        v = np.ascontiguousarray(gold["isect_v"][i])
+        t = L.orc_intersect_exit(C.byref(det), O.dptr(p), O.dptr(v), O.dptr(n_in), C.byref(sid))
+        assert np.array_equal([t, *n_in, sid.value], gold["isect_out"][i])
+
+
+def _gold_runs(O):
+    det = O.default_detector()
+    O.set_bias(det, 2.0)
+    det.luke_sample_fraction = 0.02
+    d3 = O.default_detector()
+    O.set_bias(d3, 4.0)
+    d3.luke_sample_fraction = 0.0
+    dd = O.default_detector()
+    O.demo_polynomial_surfaces(dd)
+    return {
+        "c1": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 400, 42, O.default_detector()),
+        "lognormal": (O.workload(O.WL_PHONON_LOGN, logn_median=4.0), 40, 42, O.default_detector()),
+        "mixed": (O.workload(O.WL_MIXED), 41, 77, det),
+        "pairs": (O.workload(O.WL_PAIR), 6, 42, d3),
+        "bench_demo": (O.workload(O.WL_PHONON_POINT, mode=2, energy_eV=7.5e-3), 100, 12345, dd),
+    }
+
+
+@pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo"])
+def test_run_events_golden(oracle, gold, tag):
+    """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's."""
+    O = oracle
+    wl, n, seed, det = _gold_runs(O)[tag]
+    r = O.Engine(detector=det).run(wl, n, seed, threads=4)
+    assert r["stats_raw"] == gold[f"run_{tag}_stats"].tobytes()
+    assert np.array_equal(r["hit_offsets"], gold[f"run_{tag}_offsets"])
+    h = r["hits"][:600]
+    got = np.c_[h["pos"], h["time"], h["edep"], h["incident"], h["surface"], h["mode"]]
+    assert np.array_equal(got, gold[f"run_{tag}_hits"])
+
+
+def test_thread_count_independence(oracle):
+    """tests/test_parallel.cpp:70-77: bit-identical results for 1..4 threads."""
+    O = oracle
+    det = O.default_detector()
+    O.set_bias(det, 2.0)
+    det.luke_sample_fraction = 0.02
+    eng = O.Engine(detector=det)
+    wl = O.workload(O.WL_MIXED)
+    base = eng.run(wl, 37, 77, threads=1)
+    for t in (2, 3, 4):
+        r = eng.run(wl, 37, 77, threads=t)
+        assert r["stats_raw"] == base["stats_raw"]
+        assert np.array_equal(r["hits"], base["hits"])
+    assert base["hit_offsets"][-1] == len(base["hits"])
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                                    "oracle", "_ref", "libcmpmc_ref.so")),
+                    reason="the in-place reference build (oracle/_ref) is not present")
+def test_against_reference_build(oracle):
+    O = oracle
+    eng = O.Engine()
+    wl = O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
+    a = O.ref_run(wl, 1500, 42, threads=8)
+    b = eng.run(wl, 1500, 42, threads=8)
+    assert a["stats_raw"] == b["stats_raw"]
+    assert np.array_equal(a["hit_offsets"], b["hit_offsets"])
+    for f in ("pos", "time", "edep", "incident", "surface", "mode"):
+        assert np.array_equal(a["hits"][f], b["hits"][f])
+    d4 = O.default_detector()
+    O.set_bias(d4, 4.0)
+    a = O.ref_run(O.workload(O.WL_PAIR), 1, 42, detector=d4, threads=1, chunk=1)
+    b = O.Engine(detector=d4).run(O.workload(O.WL_PAIR), 1, 42, threads=1, chunk=1)
+    assert a["stats_raw"] == b["stats_raw"]
+
+
+# ------------------------------------------------------------------ closed forms
+def test_christoffel_closed_forms(oracle):
+    """tests/test_christoffel.cpp:29-46: cubic phase speeds along [100], [110], [111]."""
+    O = oracle
+    L = O.lib()
+    c = O.germanium()
+    eng = O.Engine(crystal=c)
+    Cijkl = eng.lattice_ptr() + C.sizeof(O.Crystal)   # orc_lattice.C follows the crystal config
+    r, c11, c12, c44 = c.density, c.C11, c.C12, c.C44
+    vg = np.zeros((3, 3))
+    vp = np.zeros(3)
+
+    def solve(k):
+        k = np.array(k, dtype=float)
+        k /= np.linalg.norm(k)
+        L.orc_solve_christoffel(Cijkl, r, O.dptr(k), O.dptr(vg), O.dptr(vp))
+        return vp.copy(), vg.copy()
+
+    p, _ = solve([1, 0, 0])
+    assert p[0] == pytest.approx(np.sqrt(c11 / r), rel=1e-10)
+    assert p[1] == pytest.approx(np.sqrt(c44 / r), rel=1e-10)
+    p, _ = solve([1, 1, 0])
+    assert p[0] == pytest.approx(np.sqrt((c11 + c12 + 2 * c44) / (2 * r)), rel=1e-10)
+    a, b = np.sqrt(c44 / r), np.sqrt((c11 - c12) / (2 * r))
+    assert p[1] == pytest.approx(max(a, b), rel=1e-10) and p[2] == pytest.approx(min(a, b), rel=1e-10)
+    p, g = solve([1, 1, 1])
+    assert p[0] == pytest.approx(np.sqrt((c11 + 2 * c12 + 4 * c44) / (3 * r)), rel=1e-10)
+    assert p[1] == pytest.approx(np.sqrt((c11 - c12 + c44) / (3 * r)), rel=1e-10)
+    # along [111] the L group velocity is parallel to k with |v_g| = v_p
+    assert np.linalg.norm(g[0]) == pytest.approx(p[0], rel=1e-10)
+
+
+def test_group_velocity_is_gradient(oracle):
+    """tests/test_christoffel.cpp:52-72: v_g = grad_k omega by central differences."""
+    O = oracle
+    L = O.lib()
+    c = O.germanium()
+    eng = O.Engine(crystal=c)
+    Cijkl = eng.lattice_ptr() + C.sizeof(O.Crystal)   # orc_lattice.C follows the crystal config
+    rng = np.random.default_rng(4)
+    vg = np.zeros((3, 3))
+    vp = np.zeros(3)
+    worst = 0.0
+    for _ in range(200):
+        k = rng.normal(size=3)
+        k /= np.linalg.norm(k)
+        L.orc_solve_christoffel(Cijkl, c.density, O.dptr(k), O.dptr(vg), O.dptr(vp))
+        g0, p0 = vg.copy(), vp.copy()
+        if abs(p0[1] - p0[2]) < 1e-3 * p0[1]:
+            continue
+        for m in range(3):
+            grad = np.zeros(3)
+            for d in range(3):
+                kp, km = k.copy(), k.copy()
+                kp[d] += 1e-6
+                km[d] -= 1e-6
+                kpu, kmu = kp / np.linalg.norm(kp), km / np.linalg.norm(km)
+                L.orc_solve_christoffel(Cijkl, c.density, O.dptr(kpu), O.dptr(vg), O.dptr(vp))
+                a = np.linalg.norm(kp) * vp[m]
+                L.orc_solve_christoffel(Cijkl, c.density, O.dptr(kmu), O.dptr(vg), O.dptr(vp))
+                b = np.linalg.norm(km) * vp[m]
+                grad[d] = (a - b) / 2e-6
+            worst = max(worst, np.linalg.norm(grad - g0[m]) / np.linalg.norm(g0[m]))
+    assert worst < 1e-5
+
+
+def test_decay_angles_momentum_closure(oracle):
+    """tests/test_anharmonic_decay.cpp:44-70."""
+    O = oracle
+    L = O.lib()
+    c = O.germanium()
+    vL, vT = c.v_sound, c.v_trans
+    d = vL / vT
+    worst_lt = worst_tt = 0.0
+    for i in range(1, 400):
+        xl = (d - 1) / (d + 1) + ((1.0 - (d - 1) / (d + 1)) * i) / 400.0
+        tL, tT = L.orc_theta_L_to_LT(C.byref(c), xl), L.orc_theta_T_from_LT(C.byref(c), xl)
+        kL, kT, k0 = xl / vL, (1 - xl) / vT, 1 / vL
+        worst_lt = max(worst_lt, np.hypot(kL * np.cos(tL) + kT * np.cos(tT) - k0, kL * np.sin(tL) - kT * np.sin(tT)) / k0)
+        xt = 0.5 * (1 - 1 / d) + (1 / d) * i / 400.0
+        t1, t2 = L.orc_theta_TT(C.byref(c), xt), L.orc_theta_TT(C.byref(c), 1 - xt)
+        k1, k2 = xt / vT, (1 - xt) / vT
+        worst_tt = max(worst_tt, np.hypot(k1 * np.cos(t1) + k2 * np.cos(t2) - k0, k1 * np.sin(t1) - k2 * np.sin(t2)) / k0)
+    assert worst_lt < 1e-9 and worst_tt < 1e-9
+    assert np.cos(L.orc_theta_TT(C.byref(c), 0.5)) == pytest.approx(1.0 / d, rel=1e-9)
+
+
+def test_decay_fraction_moments(oracle):
+    """tests/test_anharmonic_decay.cpp:72-89: sampled <x> vs the Leman (9)/(12) densities."""
+    O = oracle
+    L = O.lib()
+    c = O.germanium()
+    eng = O.Engine(crystal=c)
+    r = O.Rng()
+    L.orc_rng_seed(C.byref(r), 12345)
+    lt = np.array([L.orc_sample_lt_fraction(eng.lattice_ptr(), C.byref(r)) for _ in range(200000)])
+    tt = np.array([L.orc_sample_tt_fraction(eng.lattice_ptr(), C.byref(r)) for _ in range(200000)])
+    d = c.v_sound / c.v_trans
+
+    def moments(pdf, lo, hi):
+        x = lo + (np.arange(200000) + 0.5) * (hi - lo) / 200000
+        w = np.maximum(0.0, np.array([pdf(C.byref(c), xi) for xi in x[::50]]))
+        xs = x[::50]
+        return (w * xs).sum() / w.sum()
+
+    mlt = moments(L.orc_tamura_lt_pdf, (d - 1) / (d + 1), 1.0)
+    mtt = moments(L.orc_tamura_tt_pdf, 0.5 * (1 - 1 / d), 0.5 * (1 + 1 / d))
+    assert lt.mean() == pytest.approx(mlt, rel=3e-3)
+    assert tt.mean() == pytest.approx(mtt, rel=3e-3)
+
+
+def test_kaplan_energy_closure_and_limit(oracle):
+    """tests/test_pair_breaking.cpp:106-119 (closure) and tests/test_kaplan.cpp:60-70
+    (eta = 2 Delta / Omega below 4 Delta in the no-loss limit)."""
+    O = oracle
+    L = O.lib()
+    f = O.Film()
+    L.orc_film_aluminum(C.byref(f))
+    f.subgap_absorption = 0.0
+    f.temperature = 0.0
+    D = f.gap_energy
+    r = O.Rng()
+    L.orc_rng_seed(C.byref(r), 20261101)
+    refl = np.zeros(256)
+    nr = C.c_int()
+    for W in (50.0, 150.0, 350.0):
+        worst = 0.0
+        for _ in range(300):
+            e = L.orc_absorb_phonon(C.byref(f), W * D, O.dptr(refl), C.byref(nr), C.byref(r))
+            e += refl[:nr.value].sum()
+            worst = max(worst, abs(e / (W * D) - 1.0))
+        assert worst < 1e-12
+    thick = O.Film()
+    L.orc_film_aluminum(C.byref(thick))
+    thick.film_thickness, thick.subgap_absorption, thick.keep_all_phonons, thick.temperature = 1.0, 0.0, 0, 0.0
+    for W in (2.5, 3.0, 3.9):
+        s = sum(L.orc_absorb_phonon(C.byref(thick), W * D, O.dptr(refl), C.byref(nr), C.byref(r)) for _ in range(20000))
+        assert s / 20000 / (W * D) == pytest.approx(2.0 / W, rel=5e-3)
+
+
+def test_kaplan_sampler_moments(oracle):
+    """tests/test_kaplan.cpp:50-58: pair-breaking and QP-emission samplers vs the Kaplan
+    densities, integrated here with a sin^2 substitution."""
+    O = oracle
+    L = O.lib()
+    f = O.Film()
+    L.orc_film_aluminum(C.byref(f))
+    D = f.gap_energy
+    r = O.Rng()
+    L.orc_rng_seed(C.byref(r), 31)
+
+    def mean_pdf(p, lo, hi):
+        u = (np.arange(200000) + 0.5) / 200000
+        sn = np.sin(0.5 * np.pi * u)
+        x = lo + (hi - lo) * sn * sn
+        jac = (hi - lo) * np.pi * sn * np.cos(0.5 * np.pi * u)
+        w = p(x) * jac
+        return (w * x).sum() / w.sum()
+
+    for W in (2.5, 5.0, 20.0):
+        Om = W * D
+        s = np.mean([L.orc_sample_qp_energy(C.byref(f), Om, C.byref(r)) for _ in range(100000)])
+        ref = mean_pdf(lambda x: (x * (Om - x) + D * D) / np.sqrt((x * x - D * D) * ((Om - x) ** 2 - D * D)), D, Om - D)
+        assert s / D == pytest.approx(ref / D, rel=4e-3)
+        s2 = np.mean([L.orc_sample_phonon_energy(C.byref(f), Om, C.byref(r)) for _ in range(100000)])
+        ref2 = mean_pdf(lambda x: (Om - x) ** 2 * (x - D * D / Om) / np.sqrt(x * x - D * D), D, Om)
+        assert s2 / D == pytest.approx((Om - ref2) / D, rel=4e-3)
+
+
+def test_ledger_closure(oracle):
+    """tests/test_engine_energy_closure.cpp:19-21 / test_parallel.cpp:110-120."""
+    O = oracle
+    r = O.Engine().run(O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=4e-3), 500, 5, threads=4)
+    s = r["stats"]
+    assert abs(s["total_deposited_eV"] + s["escaped_or_killed_eV"] - s["total_injected_eV"]) <= 1e-9 * s["total_injected_eV"]
+
+
+def test_batched_scheme_is_order_independent(oracle):
+    """The batched form's per-track streams make every event a function of (seed, event)
+    only: results do not depend on chunking or thread count."""
+    O = oracle
+    eng = O.Engine()
+    wl = O.workload(O.WL_PHONON_LOGN, logn_median=3.0)
+    a = eng.run(wl, 30, 9, threads=1, chunk=1, batched=True)
+    b = eng.run(wl, 30, 9, threads=4, chunk=7, batched=True)
+    assert a["stats"]["total_steps"] == b["stats"]["total_steps"]
+    assert np.array_equal(a["hits"], b["hits"])
