@@ -211,11 +211,12 @@ class Engine:
 
     @staticmethod
     def _opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
-              poll_cancel) -> capi.RunOpts:
+              poll_cancel, waves=0) -> capi.RunOpts:
         o = capi.RunOpts()
         o.form, o.keep_hits, o.want_first_branch, o.block_size = form, int(keep_hits), int(first_branch), block
         o.seed, o.stack_headroom, o.lane_stack_depth = seed, headroom, lane_depth
         o.grid_blocks, o.refill_period, o.poll_cancel = grid_blocks, refill_period, int(poll_cancel)
+        o.waves_per_simd = waves
         return o
 
     def _run(self, call, n_events: int, keep_hits: bool, hit_capacity: int | None, first_branch: bool) -> dict:
@@ -246,10 +247,10 @@ class Engine:
     def run_workload(self, wl: capi.Workload, n_events: int, seed: int = 1, form: int = FORM_PLAIN,
                      keep_hits: bool = True, hit_capacity: int | None = None, first_branch: bool = False,
                      block: int = 0, headroom: int = 0, lane_depth: int = 0, grid_blocks: int = 0,
-                     refill_period: int = 0, poll_cancel: bool = False, stream=None) -> dict:
+                     refill_period: int = 0, poll_cancel: bool = False, waves: int = 0, stream=None) -> dict:
         """run_events with the generator on the device (parallel.hpp:76-148)."""
         o = self._opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
-                       poll_cancel)
+                       poll_cancel, waves)
         L = capi.load()
         return self._run(lambda out: L.cmpmc_run_workload(self.ptr, C.byref(wl), n_events, C.byref(o), C.byref(out),
                                                           stream),
@@ -285,7 +286,7 @@ class Engine:
         b.first_event_id = first_event_id
         o = self._opts(form, seed, keep_hits, first_branch, kw.get("block", 0), kw.get("headroom", 0),
                        kw.get("lane_depth", 0), kw.get("grid_blocks", 0), kw.get("refill_period", 0),
-                       kw.get("poll_cancel", False))
+                       kw.get("poll_cancel", False), kw.get("waves", 0))
         L = capi.load()
         return self._run(lambda out: L.cmpmc_run_events(self.ptr, C.byref(b), C.byref(o), C.byref(out), None),
                          n, keep_hits, hit_capacity, first_branch)

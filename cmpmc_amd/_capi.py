@@ -113,7 +113,7 @@ class Workload(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [("form", i32), ("keep_hits", i32), ("want_first_branch", i32), ("block_size", i32),
                 ("seed", u64), ("stack_headroom", u64), ("lane_stack_depth", u64), ("grid_blocks", u32),
-                ("refill_period", u32), ("poll_cancel", i32), ("_pad", i32)]
+                ("refill_period", u32), ("poll_cancel", i32), ("waves_per_simd", i32)]
 
 
 class RunOut(C.Structure):

@@ -474,6 +474,7 @@ struct RunCfg {
     uint64_t lane_depth = 0;
     uint32_t grid_blocks = 0;
     uint32_t refill_period = 0;
+    int waves = 0;
     bool poll_cancel = false;
 };
 
@@ -704,10 +705,12 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 }
 
 // ------------------------------------------------------------------ batched form
-int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block) {
+// Register-budget variant of the persistent kernels when the options leave it open.
+constexpr int kDefaultWaves = 2;
+int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
-    if (cmpmc_occupancy(which, block, &per_cu) != hipSuccess || per_cu <= 0) per_cu = 4;
+    if (cmpmc_occupancy(which, waves, block, &per_cu) != hipSuccess || per_cu <= 0) per_cu = 4;
     return per_cu * M->n_cu;
 }
 
@@ -729,8 +732,9 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventCreate(&e2));
     uint64_t depth = C.lane_depth ? C.lane_depth : 512;
-    const int grid = grid_for(M, 0, C, block);
-    const int cgrid = grid_for(M, 1, C, block);
+    const int waves = C.waves ? C.waves : kDefaultWaves;
+    const int grid = grid_for(M, 0, C, block, waves);
+    const int cgrid = grid_for(M, 1, C, block, waves);
     const uint64_t lanes = static_cast<uint64_t>(grid) * block;
     const uint64_t clanes = static_cast<uint64_t>(cgrid) * block;
     // initial queues
@@ -819,7 +823,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ca.ledger = d_acc->ledger;
             ca.status = d_acc->status;
             ca.cancel = C.poll_cancel ? M->d_cancel : nullptr;
-            HIPCHK(cmpmc_launch_charges(&M->dm, &ca, block, cgrid, s));
+            HIPCHK(cmpmc_launch_charges(&M->dm, &ca, block, cgrid, waves, s));
             HIPCHK(hipMemcpyAsync(&n_luke, &d_acc->luke_count, 8, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             if (n_luke > luke_cap) {              // Luke queue too small: rerun with room
@@ -854,7 +858,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.status = d_acc->status;
             ba.kaplan_ws = as<double>(M->ws);
             ba.cancel = C.poll_cancel ? M->d_cancel : nullptr;
-            HIPCHK(cmpmc_launch_batched(&M->dm, &ba, block, grid, s));
+            HIPCHK(cmpmc_launch_batched(&M->dm, &ba, block, grid, waves, s));
         }
         HIPCHK(hipEventRecord(e2, s));
         Acc res;
@@ -909,8 +913,21 @@ RunCfg cfg_from(const cmpmc_run_opts* o) {
     c.lane_depth = o->lane_stack_depth;
     c.grid_blocks = o->grid_blocks;
     c.refill_period = o->refill_period;
+    c.waves = o->waves_per_simd;
     c.poll_cancel = o->poll_cancel != 0;
     return c;
+}
+
+// Options the kernels cannot honour are rejected before anything is launched.
+int check_cfg(const RunCfg& C) {
+    if (C.form != CMPMC_FORM_PLAIN && C.form != CMPMC_FORM_BATCHED)
+        return fail(CMPMC_ERR_INVALID, "run options: unknown form %d", C.form);
+    if (C.block < 0 || C.block > kMaxBlock || C.block % 64 != 0)
+        return fail(CMPMC_ERR_INVALID, "run options: block_size %d is not 0 or a multiple of 64 up to %d", C.block,
+                    kMaxBlock);
+    if (C.waves != 0 && (C.waves < 2 || C.waves > 4))
+        return fail(CMPMC_ERR_INVALID, "run options: waves_per_simd %d is not 0, 2, 3 or 4", C.waves);
+    return CMPMC_OK;
 }
 
 int validate_tracks(const cmpmc_phonon_track* t, uint64_t n) {
@@ -960,6 +977,7 @@ extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, cons
     J.rng_in = b->rng_state;
     J.injected = b->injected_eV;
     const RunCfg C = cfg_from(opts);
+    CHK(check_cfg(C));
     Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
               out->first_branch, &out->kernel_ms, &out->stack_needed};
     out->kernel_ms = 0.0;
@@ -987,6 +1005,7 @@ extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint
     J.first_event = 0;
     J.wl = wl;
     const RunCfg C = cfg_from(opts);
+    CHK(check_cfg(C));
     Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
               out->first_branch, &out->kernel_ms, &out->stack_needed};
     out->kernel_ms = 0.0;

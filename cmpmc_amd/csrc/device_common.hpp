@@ -14,6 +14,11 @@
 
 #include "cmpmc_amd.h"
 
+// Capacity of each Kaplan cascade list of one lane (core/math.hpp:230, KaplanBuffer).
+#ifndef CMPMC_KCAP
+#define CMPMC_KCAP 256
+#endif
+
 namespace cmpmc_dev {
 
 // ------------------------------------------------------------------ constants (core/units.hpp)
@@ -502,9 +507,10 @@ template <class P> __device__ inline double film_sample_phonon(const DevFilm& f,
 }
 
 // KaplanFilm::absorb_phonon (Seed::Phonon, MeanFreePath loss): returns edep, fills `out`.
-// lists[0..3] are the qp / phon / next_phon / next_qp buffers (swapped by index).
+// lists[0..3] provide the storage of the qp / phon / next_phon / next_qp buffers; the
+// generation swaps are done on named copies so that no list is indexed at run time.
 template <class P>
-__device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KList& out, KList* lists, Rng& r) {
+__device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KList& out, const KList* lists, Rng& r) {
     using R = typename P::R;
     out.n = 0;
     if (film_direct<P>(f, energy, r)) return energy;
@@ -514,47 +520,41 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
     }
     int n_pairs = 1;
     if (f.high_qp_limit > 0.0 && f.gap > 0.0) n_pairs = static_cast<int>(ceil(energy / (2.0 * f.high_qp_limit * f.gap)));
-    int iqp = 0, iph = 1, inph = 2, inqp = 3;
-    for (int i = 0; i < 4; ++i) lists[i].n = 0;
+    KList qp = lists[0], ph = lists[1], nph = lists[2], nqp = lists[3];
+    qp.n = ph.n = nph.n = nqp.n = 0;
     const double init_e = energy / n_pairs;
-    for (int i = 0; i < n_pairs; ++i) lists[iph].push(init_e);
+    for (int i = 0; i < n_pairs; ++i) ph.push(init_e);
     double edep = 0.0;
-    while (lists[iqp].n > 0 || lists[iph].n > 0) {
-        if (lists[iph].n > 0) {                       // A: pair breaking
-            KList& nph = lists[inph];
+    while (qp.n > 0 || ph.n > 0) {
+        if (ph.n > 0) {                               // A: pair breaking
             nph.n = 0;
-            const KList& ph = lists[iph];
             for (int i = 0; i < ph.n; ++i) {
                 const double E = ph.get(i);
                 if (E < 2.0 * f.gap) { nph.push(E); continue; }
                 const double qpE = film_sample_qp<P>(f, E, r);
-                edep += film_qp_absorption<P>(f, qpE, nph, lists[iqp], r);
-                edep += film_qp_absorption<P>(f, E - qpE, nph, lists[iqp], r);
+                edep += film_qp_absorption<P>(f, qpE, nph, qp, r);
+                edep += film_qp_absorption<P>(f, E - qpE, nph, qp, r);
             }
-            const int t = iph; iph = inph; inph = t;
+            const KList t = ph; ph = nph; nph = t;
         }
-        if (lists[iqp].n > 0) {                       // B: QP phonon emission
-            KList& nqp = lists[inqp];
+        if (qp.n > 0) {                               // B: QP phonon emission
             nqp.n = 0;
-            const KList& qp = lists[iqp];
             for (int i = 0; i < qp.n; ++i) {
                 const double E = qp.get(i);
                 const double phonE = film_sample_phonon<P>(f, E, r);
                 const double qpE = E - phonE;
                 if (phonE < 2.0 * f.gap) {
                     if (film_direct<P>(f, phonE, r)) edep += phonE;
-                    else lists[iph].push(phonE);
+                    else ph.push(phonE);
                 } else {
-                    lists[iph].push(phonE);
+                    ph.push(phonE);
                 }
-                edep += film_qp_absorption<P>(f, qpE, lists[iph], nqp, r);
+                edep += film_qp_absorption<P>(f, qpE, ph, nqp, r);
             }
-            const int t = iqp; iqp = inqp; inqp = t;
+            const KList t = qp; qp = nqp; nqp = t;
         }
-        if (lists[iph].n > 0) {                       // C: escape or stay
-            KList& nph = lists[inph];
+        if (ph.n > 0) {                               // C: escape or stay
             nph.n = 0;
-            const KList& ph = lists[iph];
             for (int i = 0; i < ph.n; ++i) {
                 const double E = ph.get(i);
                 if (f.temperature > 0.0) {            // is_thermalized
@@ -567,7 +567,7 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
                 if (P::uniform(r) < R(film_escape_probability<P>(f, E, frac))) out.push(E);
                 else if (f.keep_all_phonons || !(E < 2.0 * f.gap)) nph.push(E);
             }
-            const int t = iph; iph = inph; inph = t;
+            const KList t = ph; ph = nph; nph = t;
         }
     }
     return edep;

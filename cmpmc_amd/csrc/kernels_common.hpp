@@ -16,22 +16,61 @@ enum Ledger : int { L_DEP = 0, L_ESC, L_LUKE, L_RECOMB, L_INJ, L_N };
 // an integer, so totals are independent of the order in which lanes and tracks finish.
 // Global slots hold (hi, lo) pairs: value = hi * 2^20 + lo (in units of 2^-44 eV).
 constexpr double kLedgerScale = 17592186044416.0;   // 2^44
-struct LaneLedger {
-    long long v[L_N];
-    __device__ void reset() {
-        for (int i = 0; i < L_N; ++i) v[i] = 0;
-    }
-    __device__ void add(int slot, double eV) { v[slot] += __double2ll_rn(eV * kLedgerScale); }
-    __device__ void flush(unsigned long long* g) {
-        for (int i = 0; i < L_N; ++i) {
-            if (v[i] == 0) continue;
-            const unsigned long long q = static_cast<unsigned long long>(v[i]);
-            atomicAdd(&g[2 * i], q >> 20);
-            atomicAdd(&g[2 * i + 1], q & 0xFFFFFull);
-            v[i] = 0;
-        }
-    }
+
+// Statistics of one block, kept in LDS so that they cost the lanes no registers:
+// counters are bumped once per wave with the popcount of the active lanes, ledger
+// increments go to per-lane fixed-point slots. Flushed to the global accumulators by
+// the whole block at kernel end.
+constexpr int kMaxBlock = 256;
+struct BlockStats {
+    unsigned long long cnt[kMaxBlock / 64][kNumCounters];
+    unsigned long long led[L_N][kMaxBlock];
+    unsigned long long* gledger;     // global (hi, lo) pairs, for lanes whose slot fills up
 };
+
+__device__ inline void bs_init(BlockStats& s, unsigned long long* gledger) {
+    for (int i = threadIdx.x; i < (kMaxBlock / 64) * kNumCounters; i += blockDim.x) (&s.cnt[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < L_N * kMaxBlock; i += blockDim.x) (&s.led[0][0])[i] = 0;
+    if (threadIdx.x == 0) s.gledger = gledger;
+}
+// Counts every active lane of the wave that executes this call.
+__device__ inline void bs_count(BlockStats& s, int c) {
+    const uint64_t active = __builtin_amdgcn_read_exec();
+    if (__lane_id() == __ffsll(static_cast<unsigned long long>(active)) - 1)
+        s.cnt[threadIdx.x >> 6][c] += static_cast<unsigned long long>(__popcll(active));
+}
+// Adds a non-negative energy to the lane's slot; a slot that reaches 2^62 units
+// (2.6e5 eV) is moved to the global pair at once, so long-running lanes cannot wrap.
+__device__ inline void bs_ledger(BlockStats& s, int slot, double eV) {
+    const unsigned long long v =
+        s.led[slot][threadIdx.x] + static_cast<unsigned long long>(__double2ll_rn(eV * kLedgerScale));
+    if (v >> 62) {
+        atomicAdd(&s.gledger[2 * slot], v >> 20);
+        atomicAdd(&s.gledger[2 * slot + 1], v & 0xFFFFFull);
+        s.led[slot][threadIdx.x] = 0;
+    } else {
+        s.led[slot][threadIdx.x] = v;
+    }
+}
+// Whole block: after __syncthreads, sums the block's statistics into the global
+// counters and (hi, lo) ledger pairs (value = hi * 2^20 + lo, in units of 2^-44 eV).
+__device__ inline void bs_flush(const BlockStats& s, unsigned long long* counters, unsigned long long* ledger) {
+    const int nw = (blockDim.x + 63) / 64;
+    for (int c = threadIdx.x; c < kNumCounters; c += blockDim.x) {
+        unsigned long long v = 0;
+        for (int w = 0; w < nw; ++w) v += s.cnt[w][c];
+        if (v) atomicAdd(&counters[c], v);
+    }
+    for (int l = threadIdx.x; l < L_N; l += blockDim.x) {
+        unsigned long long hi = 0, lo = 0;
+        for (int t = 0; t < static_cast<int>(blockDim.x); ++t) {
+            hi += s.led[l][t] >> 20;
+            lo += s.led[l][t] & 0xFFFFFull;
+        }
+        if (hi) atomicAdd(&ledger[2 * l], hi);
+        if (lo) atomicAdd(&ledger[2 * l + 1], lo);
+    }
+}
 
 // Hit as produced on the device, with its ordering key.
 struct DevHit {

@@ -7,7 +7,8 @@
 //   ctx.push(PTrack&, PTrack&)   push a daughter (child, parent)
 //   ctx.hit(PTrack&, edep_w, surface)
 //   ctx.branch(int)              branch code of the step (first-step bookkeeping)
-//   ctx.lists()                  the five Kaplan lists of this lane (out + 4 work lists)
+//   ctx.kaplan_base/stride()     this lane's Kaplan workspace: 5 lists of CMPMC_KCAP
+//                                doubles, element j of list i at base[(i*KCAP + j)*stride]
 //   ctx.cdf_lt() / cdf_tt()      inverse-CDF tables (LDS or global)
 #pragma once
 
@@ -166,7 +167,10 @@ __device__ inline bool phonon_step(const DevModel& m, PTrack<P>& trk, Rng& r, Ct
     if (s.has_film && P::uniform(r) < R(s.prob_abs)) {
         const DevFilm& film = s.film;
         if (P::uniform(r) < R(film.film_absorption)) {
-            KList* L = ctx.lists();
+            double* wb = ctx.kaplan_base();
+            const uint32_t ws = ctx.kaplan_stride();
+            KList L[5];
+            for (int i = 0; i < 5; ++i) L[i] = KList{wb + static_cast<size_t>(i) * CMPMC_KCAP * ws, ws, 0, CMPMC_KCAP};
             KList& out = L[4];
             const double edep = film_absorb_phonon<P>(film, trk.energy, out, L, r);
             if (edep > 0.0) {

@@ -45,31 +45,8 @@ template <class T> __device__ inline void ptrack_to_btrack(const PTrack<T>& t, u
     b.bounces_mode = (t.bounces << 8) | static_cast<uint32_t>(t.mode);
 }
 
-// Block-level flush of per-lane counters and ledger into the global accumulators.
-struct BlockAccum {
-    unsigned long long cnt[kNumCounters];
-    unsigned long long led[2 * L_N];
-};
-
-__device__ inline void block_accum_init(BlockAccum& s) {
-    for (int i = threadIdx.x; i < kNumCounters; i += blockDim.x) s.cnt[i] = 0;
-    for (int i = threadIdx.x; i < 2 * L_N; i += blockDim.x) s.led[i] = 0;
-}
-__device__ inline void lane_to_block(BlockAccum& s, uint32_t* cnt, LaneLedger& led) {
-    for (int i = 0; i < kNumCounters; ++i)
-        if (cnt[i]) { atomicAdd(&s.cnt[i], static_cast<unsigned long long>(cnt[i])); cnt[i] = 0; }
-    led.flush(s.led);
-}
-__device__ inline void block_to_global(const BlockAccum& s, unsigned long long* counters, unsigned long long* ledger) {
-    for (int i = threadIdx.x; i < kNumCounters; i += blockDim.x)
-        if (s.cnt[i]) atomicAdd(&counters[i], s.cnt[i]);
-    for (int i = threadIdx.x; i < 2 * L_N; i += blockDim.x)
-        if (s.led[i]) atomicAdd(&ledger[i], s.led[i]);
-}
-
 struct BatchCtx {
-    uint32_t cnt[kNumCounters];
-    LaneLedger led;
+    BlockStats* bs;
     BTrack* ls;
     uint32_t sp, depth, lanes, lane;
     uint32_t need;
@@ -82,16 +59,16 @@ struct BatchCtx {
     int32_t* first_branch;
     uint64_t first_event;
     bool rec;
-    KList kl[5];
+    double* kws;
     const double* lt;
     const double* tt;
 
-    __device__ void count(Counter c) { ++cnt[c]; }
-    __device__ void ccount(CCounter c) { ++cnt[C_NCOUNTERS + c]; }
-    __device__ void escaped(double v) { led.add(L_ESC, v); }
-    __device__ void deposited(double v) { led.add(L_DEP, v); }
-    __device__ void luke_eV(double v) { led.add(L_LUKE, v); }
-    __device__ void recomb(double v) { led.add(L_RECOMB, v); }
+    __device__ void count(Counter c) { bs_count(*bs, c); }
+    __device__ void ccount(CCounter c) { bs_count(*bs, C_NCOUNTERS + c); }
+    __device__ void escaped(double v) { bs_ledger(*bs, L_ESC, v); }
+    __device__ void deposited(double v) { bs_ledger(*bs, L_DEP, v); }
+    __device__ void luke_eV(double v) { bs_ledger(*bs, L_LUKE, v); }
+    __device__ void recomb(double v) { bs_ledger(*bs, L_RECOMB, v); }
     __device__ void branch(int code) {
         if (rec) {
             rec = false;
@@ -100,13 +77,14 @@ struct BatchCtx {
     }
     __device__ const double* cdf_lt() const { return lt; }
     __device__ const double* cdf_tt() const { return tt; }
-    __device__ KList* lists() { return kl; }
+    __device__ double* kaplan_base() const { return kws; }
+    __device__ uint32_t kaplan_stride() const { return lanes; }
     __device__ void push(PTrack<P>& child, PTrack<P>& parent) {
         child.key = child_key(parent.key, ++parent.spawn_no);
         if (sp >= depth) {               // no room: the daughter is lost (status reports it)
             overflow = true;
             need = tmax(need, sp + 1);
-            led.add(L_ESC, child.energy * child.weight);
+            bs_ledger(*bs, L_ESC, child.energy * child.weight);
             return;
         }
         ptrack_to_btrack(child, event, ls[static_cast<size_t>(sp) * lanes + lane]);
@@ -127,20 +105,23 @@ struct BatchCtx {
     }
 };
 
-__global__ void __launch_bounds__(256) transport_batched_kernel(const DevModel m, const BatchedArgs a) {
+// W: waves per SIMD the register allocation is budgeted for (2: up to 256 VGPRs,
+// 3: 168, 4: 128 with some spilling); the launcher picks the variant.
+template <int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
+transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     __shared__ double s_cdf[2 * CMPMC_N_CDF];
-    __shared__ BlockAccum s_acc;
+    __shared__ BlockStats s_stats;
     for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
         s_cdf[i] = m.cdf_lt[i];
         s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
     }
-    block_accum_init(s_acc);
+    bs_init(s_stats, a.ledger);
     __syncthreads();
 
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     BatchCtx ctx;
-    for (int i = 0; i < kNumCounters; ++i) ctx.cnt[i] = 0;
-    ctx.led.reset();
+    ctx.bs = &s_stats;
     ctx.ls = a.lstack;
     ctx.sp = 0;
     ctx.depth = a.depth;
@@ -156,12 +137,7 @@ __global__ void __launch_bounds__(256) transport_batched_kernel(const DevModel m
     ctx.first_branch = a.first_branch;
     ctx.first_event = a.first_event;
     ctx.rec = false;
-    for (int i = 0; i < 5; ++i) {
-        ctx.kl[i].base = a.kaplan_ws + static_cast<size_t>(i) * CMPMC_KCAP * a.lanes + lane;
-        ctx.kl[i].stride = a.lanes;
-        ctx.kl[i].n = 0;
-        ctx.kl[i].cap = CMPMC_KCAP;
-    }
+    ctx.kws = a.kaplan_ws + lane;
     ctx.lt = s_cdf;
     ctx.tt = s_cdf + CMPMC_N_CDF;
 
@@ -211,16 +187,14 @@ __global__ void __launch_bounds__(256) transport_batched_kernel(const DevModel m
         if (alive) {
             alive = phonon_step<P>(m, t, t.rng, ctx);
             if (!alive) ctx.rec = false;
-            if (ctx.cnt[C_STEPS] > 0x40000000u) lane_to_block(s_acc, ctx.cnt, ctx.led);
         }
     }
-    lane_to_block(s_acc, ctx.cnt, ctx.led);
     if (ctx.overflow) {
         atomicOr(&a.status[0], 1ull);
         atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
     }
     __syncthreads();
-    block_to_global(s_acc, a.counters, a.ledger);
+    bs_flush(s_stats, a.counters, a.ledger);
 }
 
 // ------------------------------------------------------------------ batched charges
@@ -251,8 +225,7 @@ template <class T> __device__ inline void ctrack_to_bcharge(const CTrack<T>& c, 
 }
 
 struct ChargeCtx {
-    uint32_t cnt[kNumCounters];
-    LaneLedger led;
+    BlockStats* bs;
     BCharge* ls;
     uint32_t sp, depth, lanes, lane;
     bool overflow;
@@ -265,16 +238,16 @@ struct ChargeCtx {
     uint64_t hit_cap;
     bool keep_hits;
 
-    __device__ void count(Counter c) { ++cnt[c]; }
-    __device__ void ccount(CCounter c) { ++cnt[C_NCOUNTERS + c]; }
-    __device__ void escaped(double v) { led.add(L_ESC, v); }
-    __device__ void deposited(double v) { led.add(L_DEP, v); }
-    __device__ void luke_eV(double v) { led.add(L_LUKE, v); }
-    __device__ void recomb(double v) { led.add(L_RECOMB, v); }
+    __device__ void count(Counter c) { bs_count(*bs, c); }
+    __device__ void ccount(CCounter c) { bs_count(*bs, C_NCOUNTERS + c); }
+    __device__ void escaped(double v) { bs_ledger(*bs, L_ESC, v); }
+    __device__ void deposited(double v) { bs_ledger(*bs, L_DEP, v); }
+    __device__ void luke_eV(double v) { bs_ledger(*bs, L_LUKE, v); }
+    __device__ void recomb(double v) { bs_ledger(*bs, L_RECOMB, v); }
     __device__ void branch(int) {}
     __device__ void push_charge(CTrack<P>& c, CTrack<P>& parent) {
         c.key = child_key(parent.key, ++parent.spawn_no);
-        if (sp >= depth) { overflow = true; led.add(L_ESC, (static_cast<double>(c.ekin)) * c.weight); return; }
+        if (sp >= depth) { overflow = true; bs_ledger(*bs, L_ESC, static_cast<double>(c.ekin) * c.weight); return; }
         ctrack_to_bcharge(c, event, ls[static_cast<size_t>(sp) * lanes + lane]);
         ++sp;
     }
@@ -282,7 +255,7 @@ struct ChargeCtx {
         ph.key = child_key(parent.key, ++parent.spawn_no);
         const unsigned long long i = atomicAdd(luke_count, 1ull);
         if (i < luke_cap) ptrack_to_btrack(ph, event, luke_out[i]);
-        else { overflow = true; led.add(L_ESC, ph.energy * ph.weight); }
+        else { overflow = true; bs_ledger(*bs, L_ESC, ph.energy * ph.weight); }
     }
     __device__ void charge_hit(const CTrack<P>& c, double recomb_w, double e_rel, int sid) {
         if (!keep_hits) return;
@@ -297,14 +270,15 @@ struct ChargeCtx {
     }
 };
 
-__global__ void __launch_bounds__(256) transport_charges_kernel(const DevModel m, const ChargeArgs a) {
-    __shared__ BlockAccum s_acc;
-    block_accum_init(s_acc);
+template <int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
+transport_charges_kernel(const DevModel m, const ChargeArgs a) {
+    __shared__ BlockStats s_stats;
+    bs_init(s_stats, a.ledger);
     __syncthreads();
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     ChargeCtx ctx;
-    for (int i = 0; i < kNumCounters; ++i) ctx.cnt[i] = 0;
-    ctx.led.reset();
+    ctx.bs = &s_stats;
     ctx.ls = a.lstack;
     ctx.sp = 0;
     ctx.depth = a.depth;
@@ -364,13 +338,11 @@ __global__ void __launch_bounds__(256) transport_charges_kernel(const DevModel m
         if (alive) {
             alive = charge_step<P>(m, c, c.rng, ctx);
             if (alive && ++steps >= 20000000u) alive = false;   // engine.hpp:210 step cap
-            if (ctx.cnt[C_NCOUNTERS + CC_STEPS] > 0x40000000u) lane_to_block(s_acc, ctx.cnt, ctx.led);
         }
     }
-    lane_to_block(s_acc, ctx.cnt, ctx.led);
     if (ctx.overflow) atomicOr(&a.status[0], 1ull);
     __syncthreads();
-    block_to_global(s_acc, a.counters, a.ledger);
+    bs_flush(s_stats, a.counters, a.ledger);
 }
 
 // ------------------------------------------------------------------ generators
@@ -481,19 +453,33 @@ using namespace cmpmc_dev;
 
 static inline unsigned blocks_for(uint64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
-// Resident blocks per CU of the persistent kernels (sizes their grids).
-extern "C" hipError_t cmpmc_occupancy(int which, int block, int* blocks_per_cu) {
-    if (which == 0) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, transport_batched_kernel, block, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, transport_charges_kernel, block, 0);
+template <class K> static K pick(int waves, K k2, K k3, K k4) { return waves == 3 ? k3 : (waves == 4 ? k4 : k2); }
+static const void* batched_fn(int waves) {
+    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_batched_kernel<2>),
+                             reinterpret_cast<const void*>(&transport_batched_kernel<3>),
+                             reinterpret_cast<const void*>(&transport_batched_kernel<4>));
+}
+static const void* charges_fn(int waves) {
+    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_charges_kernel<2>),
+                             reinterpret_cast<const void*>(&transport_charges_kernel<3>),
+                             reinterpret_cast<const void*>(&transport_charges_kernel<4>));
 }
 
-extern "C" hipError_t cmpmc_launch_batched(const DevModel* m, const BatchedArgs* a, int block, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(transport_batched_kernel, dim3(grid), dim3(block), 0, s, *m, *a);
-    return hipGetLastError();
+// Resident blocks per CU of the persistent kernels (sizes their grids).
+extern "C" hipError_t cmpmc_occupancy(int which, int waves, int block, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, which == 0 ? batched_fn(waves) : charges_fn(waves),
+                                                        block, 0);
 }
-extern "C" hipError_t cmpmc_launch_charges(const DevModel* m, const ChargeArgs* a, int block, int grid, hipStream_t s) {
-    hipLaunchKernelGGL(transport_charges_kernel, dim3(grid), dim3(block), 0, s, *m, *a);
-    return hipGetLastError();
+
+extern "C" hipError_t cmpmc_launch_batched(const DevModel* m, const BatchedArgs* a, int block, int grid, int waves,
+                                           hipStream_t s) {
+    void* args[] = {const_cast<DevModel*>(m), const_cast<BatchedArgs*>(a)};
+    return hipLaunchKernel(batched_fn(waves), dim3(grid), dim3(block), args, 0, s);
+}
+extern "C" hipError_t cmpmc_launch_charges(const DevModel* m, const ChargeArgs* a, int block, int grid, int waves,
+                                           hipStream_t s) {
+    void* args[] = {const_cast<DevModel*>(m), const_cast<ChargeArgs*>(a)};
+    return hipLaunchKernel(charges_fn(waves), dim3(grid), dim3(block), args, 0, s);
 }
 extern "C" hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
                                                   uint64_t n, uint32_t* counts, hipStream_t s) {

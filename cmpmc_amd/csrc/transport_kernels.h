@@ -8,7 +8,9 @@
 #include "cmpmc_amd.h"
 
 // Capacity of each Kaplan cascade list of one lane (core/math.hpp:230, KaplanBuffer).
+#ifndef CMPMC_KCAP
 #define CMPMC_KCAP 256
+#endif
 
 namespace cmpmc_dev {
 
@@ -117,12 +119,13 @@ struct ChargeArgs {
 }  // namespace cmpmc_dev
 
 extern "C" {
-hipError_t cmpmc_occupancy(int which, int block, int* blocks_per_cu);
+// which: 0 phonon kernel, 1 charge kernel; waves: register-budget variant (2, 3 or 4).
+hipError_t cmpmc_occupancy(int which, int waves, int block, int* blocks_per_cu);
 hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block, hipStream_t s);
 hipError_t cmpmc_launch_batched(const cmpmc_dev::DevModel* m, const cmpmc_dev::BatchedArgs* a, int block, int grid,
-                                hipStream_t s);
+                                int waves, hipStream_t s);
 hipError_t cmpmc_launch_charges(const cmpmc_dev::DevModel* m, const cmpmc_dev::ChargeArgs* a, int block, int grid,
-                                hipStream_t s);
+                                int waves, hipStream_t s);
 // Device generators of the batched form: per-event counts, then the initial queue.
 hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl_dev, uint64_t seed, uint64_t first_event,
                                        uint64_t n_events, uint32_t* counts, hipStream_t s);

@@ -10,12 +10,9 @@
 namespace cmpmc_dev {
 
 using P = Fp64;
-using R = double;
 
 struct PlainCtx {
-    // stats
-    uint32_t cnt[kNumCounters];
-    LaneLedger led;
+    BlockStats* bs;
     // stacks of this event
     cmpmc_phonon_track* ps;
     uint32_t psp, pcap;
@@ -34,23 +31,24 @@ struct PlainCtx {
     int first_branch;
     bool first_pending;
     bool rec;
-    // Kaplan lists (in the lane's workspace)
-    KList kl[5];
+    double* kws;             // Kaplan workspace of this lane
+    uint32_t kstride;
     const double* lt;
     const double* tt;
 
-    __device__ void count(Counter c) { ++cnt[c]; }
-    __device__ void ccount(CCounter c) { ++cnt[C_NCOUNTERS + c]; }
-    __device__ void escaped(double v) { led.add(L_ESC, v); }
-    __device__ void deposited(double v) { led.add(L_DEP, v); }
-    __device__ void luke_eV(double v) { led.add(L_LUKE, v); }
-    __device__ void recomb(double v) { led.add(L_RECOMB, v); }
+    __device__ void count(Counter c) { bs_count(*bs, c); }
+    __device__ void ccount(CCounter c) { bs_count(*bs, C_NCOUNTERS + c); }
+    __device__ void escaped(double v) { bs_ledger(*bs, L_ESC, v); }
+    __device__ void deposited(double v) { bs_ledger(*bs, L_DEP, v); }
+    __device__ void luke_eV(double v) { bs_ledger(*bs, L_LUKE, v); }
+    __device__ void recomb(double v) { bs_ledger(*bs, L_RECOMB, v); }
     __device__ void branch(int code) {
         if (rec) { first_branch = code; rec = false; }
     }
     __device__ const double* cdf_lt() const { return lt; }
     __device__ const double* cdf_tt() const { return tt; }
-    __device__ KList* lists() { return kl; }
+    __device__ double* kaplan_base() const { return kws; }
+    __device__ uint32_t kaplan_stride() const { return kstride; }
 
     __device__ void store_phonon(const PTrack<P>& t) {
         if (psp >= pcap) { overflow = true; need = tmax(need, psp + 1); return; }
@@ -73,14 +71,15 @@ struct PlainCtx {
         o.valley = static_cast<uint16_t>(c.valley); o.type = static_cast<uint8_t>(c.type); o.alive = 1;
     }
     __device__ void emit(const cmpmc_sensor_hit& h) {
-        if (!keep_hits) return;
-        const unsigned long long i = atomicAdd(hit_count, 1ull);
-        if (i < hit_cap) {
-            DevHit& d = hits[i];
-            d.k0 = (ev << 32) | seq;
-            d.k1 = 0;
-            d.k2 = 0;
-            d.h = h;
+        if (keep_hits) {
+            const unsigned long long i = atomicAdd(hit_count, 1ull);
+            if (i < hit_cap) {
+                DevHit& d = hits[i];
+                d.k0 = (ev << 32) | seq;
+                d.k1 = 0;
+                d.k2 = 0;
+                d.h = h;
+            }
         }
         ++seq;
     }
@@ -110,51 +109,12 @@ __device__ inline void load_charge(const cmpmc_charge_track& o, CTrack<P>& c) {
     c.valley = o.valley; c.type = o.type;
 }
 
-__global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, const PlainArgs a) {
-    __shared__ double s_cdf[2 * CMPMC_N_CDF];
-    for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
-        s_cdf[i] = m.cdf_lt[i];
-        s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
-    }
-    __syncthreads();
-    const uint64_t lane = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (lane >= a.n_events) return;
-    if (a.cancel && *a.cancel) return;
-    const uint64_t ev = a.first_event + lane;
-
-    PlainCtx ctx;
-    for (int i = 0; i < kNumCounters; ++i) ctx.cnt[i] = 0;
-    ctx.led.reset();
-    ctx.ps = a.pstack + a.pstack_off[lane];
-    ctx.pcap = static_cast<uint32_t>(a.pstack_off[lane + 1] - a.pstack_off[lane]);
-    ctx.psp = a.n_init_ph ? a.n_init_ph[lane] : 0;
-    ctx.cs = a.cstack ? a.cstack + a.cstack_off[lane] : nullptr;
-    ctx.ccap = a.cstack ? static_cast<uint32_t>(a.cstack_off[lane + 1] - a.cstack_off[lane]) : 0;
-    ctx.csp = a.n_init_ch ? a.n_init_ch[lane] : 0;
-    ctx.need = ctx.psp;
-    ctx.overflow = false;
-    ctx.hits = a.hits;
-    ctx.hit_count = a.hit_count;
-    ctx.hit_cap = a.hit_cap;
-    ctx.ev = ev;
-    ctx.seq = 0;
-    ctx.keep_hits = a.keep_hits != 0;
-    ctx.first_branch = CMPMC_BR_NONE;
-    ctx.first_pending = true;
-    ctx.rec = false;
-    for (int i = 0; i < 5; ++i) {
-        ctx.kl[i].base = a.kaplan_ws + static_cast<size_t>(i) * CMPMC_KCAP * a.ws_stride + lane;
-        ctx.kl[i].stride = a.ws_stride;
-        ctx.kl[i].n = 0;
-        ctx.kl[i].cap = CMPMC_KCAP;
-    }
-    ctx.lt = s_cdf;
-    ctx.tt = s_cdf + CMPMC_N_CDF;
-
+// Runs one event (generation, charges, phonons) on this lane.
+__device__ void plain_event(const DevModel& m, const PlainArgs& a, PlainCtx& ctx, uint64_t lane) {
     Rng rng;
     if (a.wl) {
         // device generator: the event's stream, documented draw order
-        rng = rng_for_stream(a.seed, ev);
+        rng = rng_for_stream(a.seed, ctx.ev);
         const cmpmc_workload& w = *a.wl;
         if (w.kind == CMPMC_WL_PAIR) {
             CTrack<P> e, h;
@@ -173,11 +133,11 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
                 ctx.store_phonon(t);
                 inj += E;
             }
-            ctx.led.add(L_INJ, inj);
+            bs_ledger(*ctx.bs, L_INJ, inj);
         }
     } else {
         rng = Rng{a.rng_state[4 * lane], a.rng_state[4 * lane + 1], a.rng_state[4 * lane + 2], a.rng_state[4 * lane + 3]};
-        if (a.injected) ctx.led.add(L_INJ, a.injected[lane]);
+        if (a.injected) bs_ledger(*ctx.bs, L_INJ, a.injected[lane]);
     }
 
     // transport_charges: engine.hpp:204-379
@@ -196,20 +156,57 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
         }
         ctx.rec = false;
     }
-
     if (a.rng_state) {
         a.rng_state[4 * lane] = rng.s0; a.rng_state[4 * lane + 1] = rng.s1;
         a.rng_state[4 * lane + 2] = rng.s2; a.rng_state[4 * lane + 3] = rng.s3;
     }
-    if (a.first_branch) a.first_branch[lane] = ctx.first_branch;
-    if (a.hits_per_event) a.hits_per_event[lane] = a.skip_phonons ? ctx.psp : ctx.seq;
-    for (int i = 0; i < kNumCounters; ++i)
-        if (ctx.cnt[i]) atomicAdd(&a.counters[i], static_cast<unsigned long long>(ctx.cnt[i]));
-    ctx.led.flush(a.ledger);
-    if (ctx.overflow) {
-        atomicOr(&a.status[0], 1ull);
-        atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
+}
+
+__global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, const PlainArgs a) {
+    __shared__ double s_cdf[2 * CMPMC_N_CDF];
+    __shared__ BlockStats s_stats;
+    for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
+        s_cdf[i] = m.cdf_lt[i];
+        s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
     }
+    bs_init(s_stats, a.ledger);
+    __syncthreads();
+    const uint64_t lane = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool active = lane < a.n_events && !(a.cancel && *a.cancel);
+    if (active) {
+        PlainCtx ctx;
+        ctx.bs = &s_stats;
+        ctx.ps = a.pstack + a.pstack_off[lane];
+        ctx.pcap = static_cast<uint32_t>(a.pstack_off[lane + 1] - a.pstack_off[lane]);
+        ctx.psp = a.n_init_ph ? a.n_init_ph[lane] : 0;
+        ctx.cs = a.cstack ? a.cstack + a.cstack_off[lane] : nullptr;
+        ctx.ccap = a.cstack ? static_cast<uint32_t>(a.cstack_off[lane + 1] - a.cstack_off[lane]) : 0;
+        ctx.csp = a.n_init_ch ? a.n_init_ch[lane] : 0;
+        ctx.need = ctx.psp;
+        ctx.overflow = false;
+        ctx.hits = a.hits;
+        ctx.hit_count = a.hit_count;
+        ctx.hit_cap = a.hit_cap;
+        ctx.ev = a.first_event + lane;
+        ctx.seq = 0;
+        ctx.keep_hits = a.keep_hits != 0;
+        ctx.first_branch = CMPMC_BR_NONE;
+        ctx.first_pending = true;
+        ctx.rec = false;
+        ctx.kws = a.kaplan_ws + lane;
+        ctx.kstride = a.ws_stride;
+        ctx.lt = s_cdf;
+        ctx.tt = s_cdf + CMPMC_N_CDF;
+        plain_event(m, a, ctx, lane);
+        if (a.first_branch) a.first_branch[lane] = ctx.first_branch;
+        if (a.hits_per_event) a.hits_per_event[lane] = a.skip_phonons ? ctx.psp : ctx.seq;
+        if (ctx.overflow) {
+            atomicOr(&a.status[0], 1ull);
+            atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
+        }
+    }
+    __syncthreads();
+    bs_flush(s_stats, a.counters, a.ledger);
 }
 
 __device__ inline uint64_t region_event(const uint64_t* off, uint64_t n_events, uint64_t g) {
@@ -261,7 +258,7 @@ extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmp
                                          hipStream_t stream) {
     const uint64_t n = a->n_events;
     if (n == 0) return hipSuccess;
-    const int bs = block > 0 ? block : 256;
+    const int bs = (block > 0 && block <= cmpmc_dev::kMaxBlock) ? block : 256;
     const unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
     hipLaunchKernelGGL(cmpmc_dev::transport_plain_kernel, dim3(grid), dim3(bs), 0, stream, *m, *a);
     return hipGetLastError();

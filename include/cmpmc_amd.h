@@ -208,7 +208,7 @@ typedef struct {
     uint32_t grid_blocks;         /* batched form: persistent grid size (0 = tuned) */
     uint32_t refill_period;       /* batched form: steps between refills of dead lanes (0 = tuned) */
     int32_t poll_cancel;          /* kernels poll the model's cancel flag (cmpmc_model_cancel) */
-    int32_t _pad;
+    int32_t waves_per_simd;       /* batched form: register-budget variant 2, 3 or 4 (0 = tuned) */
 } cmpmc_run_opts;
 
 typedef struct {
