@@ -2,8 +2,10 @@
 
 Every HIP translation unit is compiled for gfx950 with packed FP32 instructions
 disabled. The plain-form kernel file is compiled without floating-point contraction
-(FP64 operation order of the reference); the batched kernels allow contraction. The
-host table builder is compiled by g++ without contraction.
+(FP64 operation order of the reference). The batched FP32 kernels allow contraction,
+use the hardware FP32 reciprocal / square root (not the correctly rounded division and
+square-root sequences) and flush FP32 denormals; their FP64 arithmetic is unaffected.
+The host table builder is compiled by g++ without contraction.
 """
 from __future__ import annotations
 
@@ -28,7 +30,8 @@ HIP_FLAGS = [
 ]
 UNITS = [
     ("transport_plain.hip", ["-ffp-contract=off"]),
-    ("transport_batched.hip", ["-ffp-contract=fast"]),
+    ("transport_batched.hip", ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+                               "-fgpu-flush-denormals-to-zero"]),
     ("capi.hip", []),
 ]
 HOST_UNITS = [("lattice_build.cpp", ["-ffp-contract=off"])]

@@ -185,7 +185,7 @@ struct cmpmc_model {
     std::mutex mu;
     // cached run buffers
     DevBuf ws, pst, cst, poff, coff, pinit, cinit, rng, inj, tin, cin, tin_off, cin_off, hits, hits_tmp, keys_a, keys_b,
-        idx_a, idx_b, sorted, counts, fb, acc, queue, cqueue, lstack, clstack, qoff, wl, cub_tmp;
+        idx_a, idx_b, sorted, counts, fb, acc, queue, cqueue, lstack, clstack, qoff, wl, cub_tmp, fjobs;
 };
 
 static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc_detector* det) {
@@ -238,6 +238,7 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     m.min_phonon_eV = det->min_phonon_eV; m.luke_fraction = det->luke_sample_fraction;
     m.dk_frac = det->charge_max_dk_frac; m.dt_cap = det->charge_dt_cap_s;
     m.fermi = det->fermi_level_above_Ev_eV; m.rr_survival = det->russian_roulette_survival;
+    fill_fp32_constants(m);
 }
 
 static int check_detector(const cmpmc_detector* d) {
@@ -289,7 +290,7 @@ static void trim(cmpmc_model* M) {
     for (DevBuf* b : {&M->ws, &M->pst, &M->cst, &M->poff, &M->coff, &M->pinit, &M->cinit, &M->rng, &M->inj, &M->tin,
                       &M->cin, &M->tin_off, &M->cin_off, &M->hits, &M->hits_tmp, &M->keys_a, &M->keys_b, &M->idx_a,
                       &M->idx_b, &M->sorted, &M->counts, &M->fb, &M->acc, &M->queue, &M->cqueue, &M->lstack,
-                      &M->clstack, &M->qoff, &M->wl, &M->cub_tmp})
+                      &M->clstack, &M->qoff, &M->wl, &M->cub_tmp, &M->fjobs})
         b->release();
 }
 
@@ -706,7 +707,7 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 
 // ------------------------------------------------------------------ batched form
 // Register-budget variant of the persistent kernels when the options leave it open.
-constexpr int kDefaultWaves = 2;
+constexpr int kDefaultWaves = 3;
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
@@ -835,7 +836,8 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         HIPCHK(hipEventRecord(e1, s));
         const uint64_t n_queue = n_init + n_luke;
         if (n_queue) {
-            CHK(ensure(M->lstack, static_cast<size_t>(depth) * lanes * sizeof(BTrack)));
+            CHK(ensure(M->lstack, static_cast<size_t>(depth) * lanes * sizeof(SRec)));
+            CHK(ensure(M->fjobs, static_cast<size_t>(lanes / 64) * kFilmCap * sizeof(SRec)));
             CHK(ensure(M->ws, static_cast<size_t>(5) * CMPMC_KCAP * lanes * sizeof(double)));
             BatchedArgs ba{};
             ba.seed = C.seed;
@@ -844,7 +846,8 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.init = q;
             ba.n_init = n_queue;
             ba.init_head = &d_acc->head;
-            ba.lstack = as<BTrack>(M->lstack);
+            ba.lstack = as<SRec>(M->lstack);
+            ba.film_jobs = as<SRec>(M->fjobs);
             ba.depth = static_cast<uint32_t>(depth);
             ba.lanes = static_cast<uint32_t>(lanes);
             ba.refill_period = C.refill_period ? C.refill_period : 1;

@@ -69,6 +69,15 @@ struct DevModel {
     DevSurface surf[3];
     uint32_t max_bounces, rr_bounces;
     double min_phonon_eV, luke_fraction, dk_frac, dt_cap, fermi, rr_survival;
+    // FP32 copies used by the batched kernels (uniform scalar operands, no per-step
+    // conversions), filled on the host by fill_fp32_constants.
+    struct SurfF {
+        float prob_abs, prob_refl, film_abs;
+        float p_anh, p_anh_spec;          // reflection split, valid when const_split != 0
+        int32_t const_split, has_film, _pad;
+    } surf_f[3];
+    float orient_f[9];
+    float fL_f, fLST_f, fTT_f, pST_f, poly_rate_f, radius_f, half_z_f, _pad_f;
 };
 
 // ------------------------------------------------------------------ RNG (core/rng.hpp)
@@ -177,9 +186,9 @@ struct Fp32 {
     __device__ static float inf() { return __builtin_huge_valf(); }
 };
 
-template <class T> __device__ inline T tmax(T a, T b) { return (a < b) ? b : a; }   // std::max
-template <class T> __device__ inline T tmin(T a, T b) { return (b < a) ? b : a; }   // std::min
-template <class T> __device__ inline T tclamp(T v, T lo, T hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
+template <class T> __host__ __device__ inline T tmax(T a, T b) { return (a < b) ? b : a; }   // std::max
+template <class T> __host__ __device__ inline T tmin(T a, T b) { return (b < a) ? b : a; }   // std::min
+template <class T> __host__ __device__ inline T tclamp(T v, T lo, T hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
 
 // ------------------------------------------------------------------ Vec3 (core/math.hpp)
 template <class R> struct V3 {
@@ -370,13 +379,13 @@ __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P:
 }
 
 // surface_property.hpp:182-220
-__device__ inline double eval_poly(const double* c, int n, double f) {
+__host__ __device__ inline double eval_poly(const double* c, int n, double f) {
     if (n <= 0) return 0.0;
     double acc = 0.0;
     for (int i = n - 1; i >= 0; --i) acc = acc * f + c[i];
     return acc;
 }
-__device__ inline void reflection_split(const DevSurface& s, double energy, double& p_spec, double& p_diff,
+__host__ __device__ inline void reflection_split(const DevSurface& s, double energy, double& p_spec, double& p_diff,
                                         double& p_anh) {
     const double f = (energy / kHPlanck) / 1e9;
     const double anh = (s.n_anh == 0 || f > s.anh_cutoff) ? 0.0 : eval_poly(s.anh, s.n_anh, f);
@@ -394,6 +403,35 @@ __device__ inline void reflection_split(const DevSurface& s, double energy, doub
     p_spec = spec / nrm;
     p_diff = diff / nrm;
     p_anh = a / nrm;
+}
+
+// Host: the FP32 block of the model from its FP64 fields. A surface without reflection
+// polynomials has an energy-independent split, stored as two cumulative thresholds.
+inline void fill_fp32_constants(DevModel& m) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) m.orient_f[3 * i + j] = static_cast<float>(m.orient[i][j]);
+    m.fL_f = static_cast<float>(m.fL);
+    m.fLST_f = static_cast<float>(m.fL + m.fST);
+    m.fTT_f = static_cast<float>(m.fTT);
+    m.pST_f = static_cast<float>(m.fST / (m.fST + m.fFT));
+    m.poly_rate_f = (m.poly_mfp <= 0.0) ? 0.0f : static_cast<float>(1.0 / m.poly_mfp);
+    m.radius_f = static_cast<float>(m.radius);
+    m.half_z_f = static_cast<float>(m.half_z);
+    m._pad_f = 0.0f;
+    for (int k = 0; k < 3; ++k) {
+        const DevSurface& s = m.surf[k];
+        DevModel::SurfF& f = m.surf_f[k];
+        f.prob_abs = static_cast<float>(s.prob_abs);
+        f.prob_refl = static_cast<float>(s.prob_refl);
+        f.film_abs = static_cast<float>(s.film.film_absorption);
+        f.has_film = s.has_film;
+        f.const_split = (s.n_spec == 0 && s.n_diff == 0 && s.n_anh == 0) ? 1 : 0;
+        double ps = 0.0, pd = 1.0, pa = 0.0;
+        if (f.const_split) reflection_split(s, 0.0, ps, pd, pa);
+        f.p_anh = static_cast<float>(pa);
+        f.p_anh_spec = static_cast<float>(pa + ps);
+        f._pad = 0;
+    }
 }
 
 // ------------------------------------------------------------------ Kaplan film cascade

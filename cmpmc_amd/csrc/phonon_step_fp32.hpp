@@ -1,0 +1,204 @@
+// One iteration of SimulationEngine::transport's per-track loop (engine.hpp:416-580) for
+// the batched FP32 kernel, arranged for wave convergence.
+//
+// It consumes the track's random stream in exactly the order of phonon_step<Fp32> (and
+// of the reference), but:
+//  * the bulk-scatter and surface-reflection branches only choose the new (mode, k);
+//    the Christoffel LUT lookup that both need runs once, after they rejoin, so a wave
+//    whose lanes split between the two pays for one lookup instead of two;
+//  * the inward-direction acceptance test of diffuse and specular reflection runs on
+//    that shared lookup; only a rejected first proposal takes the slow path;
+//  * surface constants come from the model's FP32 block (no per-step conversions), and
+//    a surface without reflection polynomials uses its precomputed split;
+//  * a phonon entering a Kaplan film is not cascaded inline: the step returns
+//    kStepFilm with the entry normal and surface, and the kernel batches such tracks so
+//    that a whole wave runs cascades together (film_job below).
+// Rare branches (anharmonic decay, surface down-conversion) keep their own lookups.
+#pragma once
+
+#include "phonon_step.hpp"
+
+namespace cmpmc_dev {
+
+enum StepCode : int { kStepDead = 0, kStepAlive = 1, kStepFilm = 2 };
+
+template <class Ctx>
+__device__ inline int phonon_step_fp32(const DevModel& m, PTrack<Fp32>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film) {
+    using P = Fp32;
+    Rng& r = trk.rng;
+    if (trk.energy < m.min_phonon_eV || trk.bounces >= m.max_bounces) {
+        ctx.count(C_KILLED);
+        ctx.escaped(trk.energy * trk.weight);
+        ctx.branch(CMPMC_BR_KILLED);
+        return kStepDead;
+    }
+    if (m.rr_bounces != 0 && trk.bounces > m.rr_bounces && (trk.bounces - m.rr_bounces) % 32 == 0) {
+        if (P::uniform(r) < static_cast<float>(m.rr_survival)) trk.weight /= m.rr_survival;
+        else {
+            ctx.count(C_KILLED);
+            ctx.branch(CMPMC_BR_ROULETTE);
+            return kStepDead;
+        }
+    }
+    ctx.count(C_STEPS);
+    const float e = static_cast<float>(trk.energy);
+    const float e2 = e * e;
+    const float rate_iso = m.c_iso_f * (e2 * e2);
+    const float rate_poly = trk.vg_mag * m.poly_rate_f;
+    const float rate_down = (trk.mode != 0) ? 0.0f : m.c_anh_f * (e2 * e2 * e);
+    const float rate_tot = rate_iso + rate_poly + rate_down;
+    const float d_phys = (rate_tot > 1e-12f) ? (P::neg_log(P::uniform(r)) * (trk.vg_mag / rate_tot)) : P::inf();
+    V3<float> n_in;
+    int sid;
+    const float dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
+
+    int mnew = trk.mode;
+    V3<float> knew;
+    bool reflect = false, specular = false;
+    if (d_phys < dist) {                          // bulk event
+        trk.pos = trk.pos + trk.vg * d_phys;
+        trk.time += static_cast<double>(d_phys / trk.vg_mag);
+        const float u_proc = P::uniform(r) * rate_tot;
+        if (rate_down > 0.0f && u_proc < rate_down) {
+            anharmonic_decay<P>(m, trk, r, ctx);
+            return kStepAlive;
+        }
+        if (u_proc < rate_down + rate_iso) { ctx.count(C_ISO); ctx.branch(CMPMC_BR_ISOTOPE); }
+        else { ctx.count(C_POLY); ctx.branch(CMPMC_BR_POLY); }
+        const float um = P::uniform(r);             // sample_dos_mode
+        mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
+        knew = isotropic_dir<P>(r);
+    } else {                                      // boundary
+        trk.pos = trk.pos + trk.vg * dist;
+        trk.time += static_cast<double>(dist / trk.vg_mag);
+        trk.bounces++;
+        const DevModel::SurfF& sf0 = m.surf_f[0];
+        const DevModel::SurfF& sf1 = m.surf_f[1];
+        const DevModel::SurfF& sf2 = m.surf_f[2];
+        const bool has_film = (sid == 0) ? sf0.has_film : ((sid == 1) ? sf1.has_film : sf2.has_film);
+        const float prob_abs = (sid == 0) ? sf0.prob_abs : ((sid == 1) ? sf1.prob_abs : sf2.prob_abs);
+        if (has_film && P::uniform(r) < prob_abs) {
+            const float film_abs = (sid == 0) ? sf0.film_abs : ((sid == 1) ? sf1.film_abs : sf2.film_abs);
+            if (P::uniform(r) < film_abs) {
+                ctx.branch(CMPMC_BR_FILM);
+                n_film = n_in;
+                sid_film = sid;
+                return kStepFilm;
+            }
+        }
+        const float prob_refl = (sid == 0) ? sf0.prob_refl : ((sid == 1) ? sf1.prob_refl : sf2.prob_refl);
+        if (P::uniform(r) > prob_refl) {
+            ctx.count(C_KILLED);
+            ctx.escaped(trk.energy * trk.weight);
+            ctx.branch(CMPMC_BR_ESCAPE);
+            return kStepDead;
+        }
+        ctx.count(C_REFL);
+        trk.pos = trk.pos + n_in * 1e-11f;
+        const bool cs = (sid == 0) ? sf0.const_split : ((sid == 1) ? sf1.const_split : sf2.const_split);
+        float t_anh, t_spec;
+        if (cs) {
+            t_anh = (sid == 0) ? sf0.p_anh : ((sid == 1) ? sf1.p_anh : sf2.p_anh);
+            t_spec = (sid == 0) ? sf0.p_anh_spec : ((sid == 1) ? sf1.p_anh_spec : sf2.p_anh_spec);
+        } else {
+            double p_spec, p_diff, p_anh;
+            reflection_split(m.surf[sid], trk.energy, p_spec, p_diff, p_anh);
+            t_anh = static_cast<float>(p_anh);
+            t_spec = static_cast<float>(p_anh + p_spec);
+        }
+        const float u_refl = P::uniform(r);
+        if (u_refl < t_anh) {                     // surface-mediated down-conversion
+            ctx.count(C_SURF_ANH);
+            ctx.branch(CMPMC_BR_SURF_ANH);
+            const bool tt = P::uniform(r) < m.fTT_f;
+            const double x = tt ? sample_fraction<P>(ctx.cdf_tt(), r) : sample_fraction<P>(ctx.cdf_lt(), r);
+            const int m1 = tt ? sample_transverse_mode<P>(m, r) : 0;
+            const int m2 = sample_transverse_mode<P>(m, r);
+            const V3<float> k2 = sample_inward_wavevector<P>(m, m2, n_in, r);
+            PTrack<P> d;
+            make_track<P>(m, d, trk.pos, k2, m2, (1.0 - x) * trk.energy, trk.time, trk.weight);
+            ctx.push(d, trk);
+            trk.energy = x * trk.energy;
+            trk.mode = m1;
+            trk.k = sample_inward_wavevector<P>(m, m1, n_in, r);
+            map_k_to_vg<P>(m, m1, trk.k, trk.vg, trk.vg_mag);
+            return kStepAlive;
+        }
+        reflect = true;
+        if (u_refl < t_spec) {
+            ctx.branch(CMPMC_BR_SPECULAR);
+            specular = true;
+            knew = trk.k - n_in * (2.0f * dot(trk.k, n_in));
+        } else {
+            ctx.branch(CMPMC_BR_DIFFUSE);
+            knew = lambertian_dir<P>(r, n_in);
+        }
+    }
+    // converged tail: one LUT lookup for the new direction
+    V3<float> vg;
+    float mag;
+    map_k_to_vg<P>(m, mnew, knew, vg, mag);
+    if (reflect && !(dot(vg, n_in) > 1e-6f)) {
+        if (specular) {                           // engine.hpp: resample an inward direction
+            knew = sample_inward_wavevector<P>(m, mnew, n_in, r);
+        } else {                                  // diffuse: trials 2..16 of the inward sampler
+            bool ok = false;
+            for (int trial = 1; trial < 16 && !ok; ++trial) {
+                const V3<float> kd = lambertian_dir<P>(r, n_in);
+                V3<float> v2;
+                float m2;
+                map_k_to_vg<P>(m, mnew, kd, v2, m2);
+                if (dot(v2, n_in) > 1e-6f) { knew = kd; ok = true; }
+            }
+            if (!ok) knew = n_in;
+        }
+        map_k_to_vg<P>(m, mnew, knew, vg, mag);
+    }
+    trk.k = knew;
+    trk.mode = mnew;
+    trk.vg = vg;
+    trk.vg_mag = mag;
+    return kStepAlive;
+}
+
+// Kaplan film interaction of a track that phonon_step_fp32 handed over (engine.hpp:
+// 521-552): the cascade, the sensor hit, the re-emitted phonons (pushed as daughters) and
+// the continuing track, which the caller re-queues. Returns false if nothing re-emits.
+template <class Ctx>
+__device__ inline bool film_job(const DevModel& m, PTrack<Fp32>& trk, V3<float> n_in, int sid, Ctx& ctx) {
+    using P = Fp32;
+    Rng& r = trk.rng;
+    const DevFilm& film = m.surf[sid].film;
+    double* wb = ctx.kaplan_base();
+    const uint32_t ws = ctx.kaplan_stride();
+    KList L[5];
+    for (int i = 0; i < 5; ++i) L[i] = KList{wb + static_cast<size_t>(i) * CMPMC_KCAP * ws, ws, 0, CMPMC_KCAP};
+    KList& out = L[4];
+    const double edep = film_absorb_phonon<P>(film, trk.energy, out, L, r);
+    if (edep > 0.0) {
+        const double wed = edep * trk.weight;
+        ctx.count(C_HITS);
+        ctx.deposited(wed);
+        ctx.hit(trk, wed, sid);
+    }
+    double sum_refl = 0.0;
+    for (int i = 0; i < out.n; ++i) sum_refl += out.get(i);
+    ctx.escaped(tmax(0.0, trk.energy - edep - sum_refl) * trk.weight);
+    if (out.n == 0) return false;
+    const V3<float> re = trk.pos + n_in * 1e-11f;
+    for (int i = 1; i < out.n; ++i) {
+        const int md = sample_dos_mode<P>(m, r);
+        const V3<float> kd = sample_inward_wavevector<P>(m, md, n_in, r);
+        PTrack<P> d;
+        make_track<P>(m, d, re, kd, md, out.get(i), trk.time, trk.weight);
+        ctx.push(d, trk);
+    }
+    trk.pos = re;
+    trk.energy = out.get(0);
+    trk.mode = sample_dos_mode<P>(m, r);
+    trk.k = sample_inward_wavevector<P>(m, trk.mode, n_in, r);
+    map_k_to_vg<P>(m, trk.mode, trk.k, trk.vg, trk.vg_mag);
+    return true;
+}
+
+}  // namespace cmpmc_dev

@@ -9,6 +9,7 @@
 // Every track draws from its own stream rng_for_track(event_base(seed, event), key), so
 // results do not depend on which lane runs which track, or when.
 #include "kernels_common.hpp"
+#include "phonon_step_fp32.hpp"
 #include "transport_kernels.h"
 
 namespace cmpmc_dev {
@@ -45,13 +46,49 @@ template <class T> __device__ inline void ptrack_to_btrack(const PTrack<T>& t, u
     b.bounces_mode = (t.bounces << 8) | static_cast<uint32_t>(t.mode);
 }
 
+static_assert(sizeof(SRec) == 144, "SRec layout");
+
+__device__ inline void srec_to_ptrack(const SRec& b, PTrack<P>& t, uint64_t& event) {
+    t.pos = {b.pos[0], b.pos[1], b.pos[2]};
+    t.k = {b.k[0], b.k[1], b.k[2]};
+    t.vg = {b.vg[0], b.vg[1], b.vg[2]};
+    t.vg_mag = b.vg_mag;
+    t.energy = b.energy;
+    t.time = b.time;
+    t.weight = b.weight;
+    t.bounces = b.bounces_mode >> 8;
+    t.mode = static_cast<int>(b.bounces_mode & 0xFF);
+    t.key = b.key;
+    t.spawn_no = b.spawn_no;
+    t.hit_seq = b.hit_seq;
+    t.rng = Rng{b.rng[0], b.rng[1], b.rng[2], b.rng[3]};
+    event = b.event;
+}
+
+__device__ inline void ptrack_to_srec(const PTrack<P>& t, uint64_t event, SRec& b) {
+    b.pos[0] = t.pos.x; b.pos[1] = t.pos.y; b.pos[2] = t.pos.z;
+    b.k[0] = t.k.x; b.k[1] = t.k.y; b.k[2] = t.k.z;
+    b.vg[0] = t.vg.x; b.vg[1] = t.vg.y; b.vg[2] = t.vg.z;
+    b.vg_mag = t.vg_mag;
+    b.energy = t.energy;
+    b.time = t.time;
+    b.weight = t.weight;
+    b.key = t.key;
+    b.rng[0] = t.rng.s0; b.rng[1] = t.rng.s1; b.rng[2] = t.rng.s2; b.rng[3] = t.rng.s3;
+    b.event = static_cast<uint32_t>(event);
+    b.bounces_mode = (t.bounces << 8) | static_cast<uint32_t>(t.mode);
+    b.spawn_no = t.spawn_no;
+    b.hit_seq = t.hit_seq;
+}
+
 struct BatchCtx {
     BlockStats* bs;
-    BTrack* ls;
+    SRec* ls;
     uint32_t sp, depth, lanes, lane;
     uint32_t need;
     bool overflow;
     uint64_t event;
+    uint64_t seed;
     DevHit* hits;
     unsigned long long* hit_count;
     uint64_t hit_cap;
@@ -79,17 +116,26 @@ struct BatchCtx {
     __device__ const double* cdf_tt() const { return tt; }
     __device__ double* kaplan_base() const { return kws; }
     __device__ uint32_t kaplan_stride() const { return lanes; }
-    __device__ void push(PTrack<P>& child, PTrack<P>& parent) {
-        child.key = child_key(parent.key, ++parent.spawn_no);
-        if (sp >= depth) {               // no room: the daughter is lost (status reports it)
+    // Puts a complete track on this lane's stack; a full stack loses it (status reports
+    // the overflow and the host reruns with a deeper stack).
+    __device__ void store(const PTrack<P>& t) {
+        if (sp >= depth) {
             overflow = true;
             need = tmax(need, sp + 1);
-            bs_ledger(*bs, L_ESC, child.energy * child.weight);
+            bs_ledger(*bs, L_ESC, t.energy * t.weight);
             return;
         }
-        ptrack_to_btrack(child, event, ls[static_cast<size_t>(sp) * lanes + lane]);
+        ptrack_to_srec(t, event, ls[static_cast<size_t>(sp) * lanes + lane]);
         ++sp;
         need = tmax(need, sp);
+    }
+    // A daughter: key n of its parent, stream rng_for_track(event_base, key).
+    __device__ void push(PTrack<P>& child, PTrack<P>& parent) {
+        child.key = child_key(parent.key, ++parent.spawn_no);
+        child.rng = rng_for_track(event_base(seed, event), child.key);
+        child.spawn_no = 0;
+        child.hit_seq = 0;
+        store(child);
     }
     __device__ void hit(PTrack<P>& t, double wed, int sid) {
         const uint32_t seq = t.hit_seq++;
@@ -105,6 +151,15 @@ struct BatchCtx {
     }
 };
 
+// Persistent phonon kernel. Per lane: one live track in registers, a spawn stack in HBM.
+// Per wave: a buffer of deferred film interactions (wave-uniform count nf). Loop:
+//   1. a lane without a track pops its stack, else claims from the initial queue
+//      (wave-aggregated atomic every refill_period iterations, at once if the wave idles);
+//   2. with kFilmFlush deferred film interactions pending, or nothing else to do, the
+//      wave runs up to 64 of them, one per lane, pushing the continuing tracks and the
+//      re-emitted phonons onto the lanes' stacks;
+//   3. otherwise every lane with a track takes one step; tracks that enter a film are
+//      appended to the wave's buffer.
 // W: waves per SIMD the register allocation is budgeted for (2: up to 256 VGPRs,
 // 3: 168, 4: 128 with some spilling); the launcher picks the variant.
 template <int W>
@@ -120,6 +175,8 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     __syncthreads();
 
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lid = __lane_id();
+    SRec* const fbuf = a.film_jobs + static_cast<size_t>(lane / 64) * kFilmCap;
     BatchCtx ctx;
     ctx.bs = &s_stats;
     ctx.ls = a.lstack;
@@ -130,6 +187,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.need = 0;
     ctx.overflow = false;
     ctx.event = 0;
+    ctx.seed = a.seed;
     ctx.hits = a.hits;
     ctx.hit_count = a.hit_count;
     ctx.hit_cap = a.hit_cap;
@@ -144,22 +202,24 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     PTrack<P> t;
     bool alive = false;
     bool exhausted = false;
+    uint32_t nf = 0;                              // deferred film interactions of this wave
     const uint32_t period = a.refill_period ? a.refill_period : 1;
-    const uint64_t lt_mask = (1ull << __lane_id()) - 1ull;
+    const uint64_t lt_mask = (1ull << lid) - 1ull;
     for (uint32_t iter = 0;; ++iter) {
-        if (!alive && ctx.sp > 0) {            // own spawn stack first (LIFO)
+        if (!alive && ctx.sp > 0) {               // own spawn stack first (LIFO)
             --ctx.sp;
-            btrack_to_ptrack(ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane], a.seed, t, ctx.event);
+            srec_to_ptrack(ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane], t, ctx.event);
             alive = true;
             ctx.rec = false;
         }
         const bool want = !alive && !exhausted;
         const uint64_t want_mask = __ballot(want);
-        const uint64_t alive_mask = __ballot(alive);
+        uint64_t alive_mask = __ballot(alive);
         if (want_mask && (alive_mask == 0 || iter % period == 0)) {
             const int leader = __ffsll(static_cast<unsigned long long>(want_mask)) - 1;
             unsigned long long base = 0;
-            if (__lane_id() == leader) base = atomicAdd(a.init_head, static_cast<unsigned long long>(__popcll(want_mask)));
+            if (static_cast<int>(lid) == leader)
+                base = atomicAdd(a.init_head, static_cast<unsigned long long>(__popcll(want_mask)));
             base = __shfl(base, leader);
             if (want) {
                 const unsigned long long idx = base + __popcll(want_mask & lt_mask);
@@ -171,22 +231,58 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                     exhausted = true;
                 }
             }
+            alive_mask = __ballot(alive);
         }
-        if (__ballot(alive) == 0) {
+        if (nf > 0 && (nf >= kFilmFlush || alive_mask == 0)) {
+            // film phase: lane i runs deferred interaction nf - take + i
+            const uint32_t take = nf < 64u ? nf : 64u;
+            if (lid < take) {
+                const SRec& jr = fbuf[nf - take + lid];
+                PTrack<P> j;
+                uint64_t jev;
+                srec_to_ptrack(jr, j, jev);
+                const V3<float> n_in{jr.n_in[0], jr.n_in[1], jr.n_in[2]};
+                const int sid = jr.sid;
+                const uint64_t own_event = ctx.event;
+                const bool own_rec = ctx.rec;
+                ctx.event = jev;
+                ctx.rec = false;
+                if (film_job(m, j, n_in, sid, ctx)) ctx.store(j);
+                ctx.event = own_event;
+                ctx.rec = own_rec;
+            }
+            nf -= take;
+            continue;
+        }
+        if (alive_mask == 0) {
             if (__ballot(!exhausted) == 0) break;
             continue;
         }
         if ((iter & 1023u) == 0 && a.cancel) {
             int c = 0;
-            if (__lane_id() == 0) c = *a.cancel;
+            if (lid == 0) c = *a.cancel;
             if (__shfl(c, 0)) {
-                if (__lane_id() == 0) atomicOr(&a.status[2], 1ull);
+                if (lid == 0) atomicOr(&a.status[2], 1ull);
                 break;
             }
         }
+        int code = kStepDead;
+        V3<float> n_film;
+        int sid_film = 0;
         if (alive) {
-            alive = phonon_step<P>(m, t, t.rng, ctx);
+            code = phonon_step_fp32(m, t, ctx, n_film, sid_film);
+            alive = (code == kStepAlive);
             if (!alive) ctx.rec = false;
+        }
+        const uint64_t film_mask = __ballot(code == kStepFilm);
+        if (film_mask) {
+            if (code == kStepFilm) {
+                SRec& jr = fbuf[nf + __popcll(film_mask & lt_mask)];
+                ptrack_to_srec(t, ctx.event, jr);
+                jr.n_in[0] = n_film.x; jr.n_in[1] = n_film.y; jr.n_in[2] = n_film.z;
+                jr.sid = sid_film;
+            }
+            nf += static_cast<uint32_t>(__popcll(film_mask));
         }
     }
     if (ctx.overflow) {

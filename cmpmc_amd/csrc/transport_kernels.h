@@ -71,6 +71,30 @@ struct alignas(16) BCharge {
     uint32_t _pad;
 };
 
+// Lane spawn-stack entry and deferred film interaction of the batched phonon kernel
+// (144 B): the complete track state including its random stream, so that a track
+// continues exactly where it stopped. n_in / sid are the film entry normal and surface.
+struct alignas(16) SRec {
+    float pos[3];
+    float k[3];
+    float vg[3];
+    float vg_mag;
+    double energy, time, weight;
+    uint64_t key;
+    uint64_t rng[4];
+    uint32_t event;
+    uint32_t bounces_mode;             // bounces << 8 | mode
+    uint32_t spawn_no, hit_seq;
+    float n_in[3];
+    int32_t sid;
+    uint32_t _pad[2];
+};
+
+// Deferred film interactions are held per wave (kFilmCap entries) and run together once
+// kFilmFlush of them are pending, or when the wave has nothing else to do.
+constexpr uint32_t kFilmCap = 128;
+constexpr uint32_t kFilmFlush = 64;
+
 struct BatchedArgs {
     uint64_t seed;
     uint64_t first_event;
@@ -78,7 +102,8 @@ struct BatchedArgs {
     const BTrack* init;                // initial queue
     uint64_t n_init;
     unsigned long long* init_head;     // claim counter of the initial queue
-    BTrack* lstack;                    // per-lane spawn stacks: entry d of lane L at [d * lanes + L]
+    SRec* lstack;                      // per-lane spawn stacks: entry d of lane L at [d * lanes + L]
+    SRec* film_jobs;                   // per-wave deferred film interactions: [wave * kFilmCap + slot]
     uint32_t depth;
     uint32_t lanes;
     uint32_t refill_period;
