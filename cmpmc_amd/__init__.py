@@ -23,7 +23,7 @@ from . import _capi as capi
 from ._capi import (CHARGE_DTYPE, FORM_BATCHED, FORM_PLAIN, HIT_DTYPE, PHONON_DTYPE, WL_PAIR, WL_PHONON_LOGN,
                     WL_PHONON_POINT, WL_PHONON_UNIF, CmpmcError, check)
 
-__all__ = ["Lattice", "Engine", "Rng", "germanium", "default_detector", "set_bias", "demo_polynomial_surfaces",
+__all__ = ["Lattice", "Engine", "DiffTape", "apply_diff_params", "Rng", "germanium", "default_detector", "set_bias", "demo_polynomial_surfaces",
            "workload", "make_tracks", "CmpmcError", "FORM_PLAIN", "FORM_BATCHED", "PHONON_DTYPE", "CHARGE_DTYPE",
            "HIT_DTYPE", "WL_PHONON_POINT", "WL_PHONON_LOGN", "WL_PAIR", "WL_PHONON_UNIF", "device_info"]
 
@@ -98,6 +98,19 @@ def default_detector() -> capi.Detector:
     d = capi.Detector()
     check(capi.load().cmpmc_detector_default(C.byref(d)))
     return d
+
+
+def apply_diff_params(crystal: capi.Crystal, det: capi.Detector, params) -> None:
+    """Writes (B_iso, A_anh, f_diffuse, p_film) into a crystal and detector the way the
+    differentiable form applies them: every surface diffuse with probability f and
+    specular otherwise, every film absorbing with probability p."""
+    B, A, f, p = (float(v) for v in params)
+    crystal.B_iso, crystal.A_anh = B, A
+    for s in det.surf:
+        s.n_spec, s.n_diff, s.n_anh = 1, 1, 0
+        s.spec[0], s.diff[0], s.spec_prob_const = 1.0 - f, f, 1.0 - f
+        if s.has_film:
+            s.film.film_absorption = p
 
 
 def set_bias(det: capi.Detector, volts: float, thickness_m: float = 2.54e-2) -> None:
@@ -291,6 +304,25 @@ class Engine:
         return self._run(lambda out: L.cmpmc_run_events(self.ptr, C.byref(b), C.byref(o), C.byref(out), None),
                          n, keep_hits, hit_capacity, first_branch)
 
+    def diff_forward(self, wl: capi.Workload, n_events: int, params, seed: int = 1, t_max_s: float = 1e-3,
+                     waves: int = 0, grid_blocks: int = 0, stream=None) -> tuple[np.ndarray, "DiffTape", dict]:
+        """Differentiable batched transport of a phonon workload: observables y (top and
+        bottom face collected energy, 64-bin energy-weighted arrival-time histogram over
+        [0, t_max_s)), a tape for the backward pass and the run's statistics.
+        params = (B_iso, A_anh, f_diffuse, p_film)."""
+        p = capi.DiffParams(*[float(v) for v in params])
+        o = self._opts(FORM_BATCHED, seed, False, False, 0, 0, 0, grid_blocks, 0, False, waves)
+        y = np.zeros(capi.DIFF_OUT)
+        st = capi.Stats()
+        tape = C.c_void_p()
+        ms = C.c_double()
+        t0 = time.perf_counter()
+        check(capi.load().cmpmc_diff_forward(self.ptr, C.byref(wl), n_events, C.byref(p), t_max_s, C.byref(o),
+                                             y.ctypes.data_as(C.POINTER(capi.dbl)), C.byref(st), C.byref(tape),
+                                             C.byref(ms), stream))
+        info = {"stats": st.as_dict(), "kernel_ms": ms.value, "wall_s": time.perf_counter() - t0}
+        return y, DiffTape(tape, self), info
+
     def transport(self, tracks: np.ndarray, rng_state: np.ndarray, hit_capacity: int = 1 << 16) -> dict:
         """SimulationEngine::transport(stack, hits, stats, rng) (engine.hpp:403-582)."""
         tracks = np.ascontiguousarray(tracks, dtype=PHONON_DTYPE)
@@ -334,3 +366,37 @@ class Engine:
             check(rc)
             return {"stats": st.as_dict(), "stats_raw": bytes(st), "hits": hits[:nh.value].copy(),
                     "phonons": ph[:nph.value].copy(), "rng": r}
+
+
+class DiffTape:
+    """Hits of a differentiable forward pass with their scores (device memory), consumed
+    by backward(); freed with the object."""
+
+    def __init__(self, ptr: C.c_void_p, engine: Engine):
+        self.ptr = ptr
+        self.engine = engine          # the model outlives its tapes
+        self.last_kernel_ms = 0.0
+
+    def __len__(self) -> int:
+        return int(capi.load().cmpmc_diff_tape_size(self.ptr)) if self.ptr else 0
+
+    def backward(self, g, stream=None) -> np.ndarray:
+        """grad[j] = sum_k g[k] dE[y_k]/dtheta_j for theta = (B, A, f_diffuse, p_film)."""
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(capi.DIFF_OUT)
+        out = np.zeros(4)
+        ms = C.c_double()
+        check(capi.load().cmpmc_diff_backward(self.ptr, g.ctypes.data_as(C.POINTER(capi.dbl)),
+                                              out.ctypes.data_as(C.POINTER(capi.dbl)), C.byref(ms), stream))
+        self.last_kernel_ms = ms.value
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            capi.load().cmpmc_diff_tape_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

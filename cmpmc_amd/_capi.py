@@ -122,8 +122,14 @@ class RunOut(C.Structure):
                 ("kernel_ms", dbl), ("stack_needed", u64)]
 
 
+class DiffParams(C.Structure):
+    _fields_ = [("B_iso", dbl), ("A_anh", dbl), ("f_diffuse", dbl), ("p_film", dbl)]
+
+
+DIFF_OUT = 66
+
 ABI_STRUCTS = [Crystal, Film, Surface, Detector, Lattice, PHONON_DTYPE, CHARGE_DTYPE, HIT_DTYPE, Stats,
-               EventBatch, Workload, RunOpts, RunOut]
+               EventBatch, Workload, RunOpts, RunOut, DiffParams]
 
 _lib = None
 
@@ -164,6 +170,13 @@ def load():
     L.cmpmc_run_events.argtypes = [C.c_void_p, C.POINTER(EventBatch), C.POINTER(RunOpts), C.POINTER(RunOut), C.c_void_p]
     L.cmpmc_run_workload.argtypes = [C.c_void_p, C.POINTER(Workload), u64, C.POINTER(RunOpts), C.POINTER(RunOut),
                                      C.c_void_p]
+    L.cmpmc_diff_forward.argtypes = [C.c_void_p, C.POINTER(Workload), u64, C.POINTER(DiffParams), dbl,
+                                     C.POINTER(RunOpts), C.POINTER(dbl), C.POINTER(Stats), C.POINTER(C.c_void_p),
+                                     C.POINTER(dbl), C.c_void_p]
+    L.cmpmc_diff_backward.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl), C.c_void_p]
+    L.cmpmc_diff_tape_size.argtypes = [C.c_void_p]
+    L.cmpmc_diff_tape_size.restype = u64
+    L.cmpmc_diff_tape_free.argtypes = [C.c_void_p]
     check_abi(L)
     _lib = L
     return L
@@ -186,5 +199,6 @@ EXPORTED_SYMBOLS = [
     "cmpmc_last_error", "cmpmc_device_info", "cmpmc_abi_sizeof", "cmpmc_crystal_germanium", "cmpmc_film_aluminum",
     "cmpmc_detector_default", "cmpmc_lattice_build", "cmpmc_map_k_to_vg", "cmpmc_model_create",
     "cmpmc_model_destroy", "cmpmc_model_cancel", "cmpmc_model_trim", "cmpmc_transport_serial",
-    "cmpmc_transport_charges_serial", "cmpmc_run_events", "cmpmc_run_workload",
+    "cmpmc_transport_charges_serial", "cmpmc_run_events", "cmpmc_run_workload", "cmpmc_diff_forward",
+    "cmpmc_diff_backward", "cmpmc_diff_tape_size", "cmpmc_diff_tape_free",
 ]

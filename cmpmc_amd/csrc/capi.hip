@@ -78,6 +78,7 @@ extern "C" size_t cmpmc_abi_sizeof(int which) {
         case 10: return sizeof(cmpmc_workload);
         case 11: return sizeof(cmpmc_run_opts);
         case 12: return sizeof(cmpmc_run_out);
+        case 13: return sizeof(cmpmc_diff_params);
         default: return 0;
     }
 }
@@ -330,6 +331,7 @@ struct Acc {
     unsigned long long luke_count;
     unsigned long long head;
     unsigned long long chead;
+    unsigned long long tape_count;
 };
 
 struct Totals {
@@ -708,14 +710,28 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 // ------------------------------------------------------------------ batched form
 // Register-budget variant of the persistent kernels when the options leave it open.
 constexpr int kDefaultWaves = 3;
+// which: 0 phonon kernel, 1 charge kernel, 2 phonon kernel of the differentiable form
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
-    if (cmpmc_occupancy(which, waves, block, &per_cu) != hipSuccess || per_cu <= 0) per_cu = 4;
+    const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, block, &per_cu)
+                                      : cmpmc_occupancy(which, waves, block, &per_cu);
+    if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     return per_cu * M->n_cu;
 }
 
-int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s) {
+// Differentiable form of a batched run: the parameter-modified model, the likelihood
+// constants and the tape (owned by the caller's cmpmc_diff_tape).
+struct DiffCfg {
+    DevModel dm;
+    double k_f = 0.0, k_p = 0.0, bins_per_s = 0.0;
+    DevBuf* tape = nullptr;
+    uint64_t n_tape = 0;            // out: tape records written
+};
+
+int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s,
+                DiffCfg* D = nullptr) {
+    const DevModel* dmp = D ? &D->dm : &M->dm;
     const uint64_t N = J.n_events;
     const bool keep = C.keep_hits && O.hits != nullptr;
     const uint64_t dev_hit_cap = keep ? O.hit_cap : 0;
@@ -734,8 +750,9 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     HIPCHK(hipEventCreate(&e2));
     uint64_t depth = C.lane_depth ? C.lane_depth : 512;
     const int waves = C.waves ? C.waves : kDefaultWaves;
-    const int grid = grid_for(M, 0, C, block, waves);
+    const int grid = grid_for(M, D ? 2 : 0, C, block, waves);
     const int cgrid = grid_for(M, 1, C, block, waves);
+    uint64_t tape_cap = D ? std::max<uint64_t>(D->tape->bytes / sizeof(TapeRec), 1) : 0;
     const uint64_t lanes = static_cast<uint64_t>(grid) * block;
     const uint64_t clanes = static_cast<uint64_t>(cgrid) * block;
     // initial queues
@@ -770,7 +787,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         if (J.wl && !pair) {
             CHK(ensure(M->qoff, (N + 1) * 8));
             HIPCHK(hipMemcpyAsync(M->qoff.p, off.data(), (N + 1) * 8, hipMemcpyHostToDevice, s));
-            HIPCHK(cmpmc_launch_workload_fill(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
+            HIPCHK(cmpmc_launch_workload_fill(dmp, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
                                               as<uint64_t>(M->qoff), n_init, q, d_acc->ledger, s));
         } else if (!J.wl && n_init) {
             CHK(ensure(M->tin, n_init * sizeof(cmpmc_phonon_track)));
@@ -836,8 +853,9 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         HIPCHK(hipEventRecord(e1, s));
         const uint64_t n_queue = n_init + n_luke;
         if (n_queue) {
-            CHK(ensure(M->lstack, static_cast<size_t>(depth) * lanes * sizeof(SRec)));
-            CHK(ensure(M->fjobs, static_cast<size_t>(lanes / 64) * kFilmCap * sizeof(SRec)));
+            const size_t rec_bytes = D ? sizeof(SRecD) : sizeof(SRec);
+            CHK(ensure(M->lstack, static_cast<size_t>(depth) * lanes * rec_bytes));
+            CHK(ensure(M->fjobs, static_cast<size_t>(lanes / 64) * kFilmCap * rec_bytes));
             CHK(ensure(M->ws, static_cast<size_t>(5) * CMPMC_KCAP * lanes * sizeof(double)));
             BatchedArgs ba{};
             ba.seed = C.seed;
@@ -846,8 +864,8 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.init = q;
             ba.n_init = n_queue;
             ba.init_head = &d_acc->head;
-            ba.lstack = as<SRec>(M->lstack);
-            ba.film_jobs = as<SRec>(M->fjobs);
+            ba.lstack = M->lstack.p;
+            ba.film_jobs = M->fjobs.p;
             ba.depth = static_cast<uint32_t>(depth);
             ba.lanes = static_cast<uint32_t>(lanes);
             ba.refill_period = C.refill_period ? C.refill_period : 1;
@@ -861,7 +879,16 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.status = d_acc->status;
             ba.kaplan_ws = as<double>(M->ws);
             ba.cancel = C.poll_cancel ? M->d_cancel : nullptr;
-            HIPCHK(cmpmc_launch_batched(&M->dm, &ba, block, grid, waves, s));
+            if (D) {
+                CHK(ensure(*D->tape, tape_cap * sizeof(TapeRec)));
+                ba.tape = as<TapeRec>(*D->tape);
+                ba.tape_count = &d_acc->tape_count;
+                ba.tape_cap = tape_cap;
+                ba.bins_per_s = D->bins_per_s;
+                ba.k_f = D->k_f;
+                ba.k_p = D->k_p;
+            }
+            HIPCHK(cmpmc_launch_batched(dmp, &ba, block, grid, waves, D ? 1 : 0, s));
         }
         HIPCHK(hipEventRecord(e2, s));
         Acc res;
@@ -880,6 +907,12 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             depth = std::max<uint64_t>(depth * 2, res.status[1] + 64);
             continue;
         }
+        if (D && res.tape_count > tape_cap) {     // tape too small: rerun with room for every hit
+            if (attempt >= 6) { rc = fail(CMPMC_ERR_CAPACITY, "differentiable tape overflow"); break; }
+            tape_cap = res.tape_count + res.tape_count / 8 + 1024;
+            continue;
+        }
+        if (D) D->n_tape = res.tape_count;
         res.ledger[2 * L_INJ] += inj_fixed >> 20;
         res.ledger[2 * L_INJ + 1] += inj_fixed & 0xFFFFFull;
         add_acc(tot, res);
@@ -1015,6 +1048,142 @@ extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (C.form == CMPMC_FORM_BATCHED) return run_batched(M, J, C, O, s);
     return run_plain(M, J, C, O, s);
+}
+
+// ------------------------------------------------------------------ differentiable form
+struct cmpmc_diff_tape {
+    cmpmc_model* model = nullptr;
+    DevBuf tape, gbuf, partials;
+    uint64_t n = 0;
+    double inv_theta[4] = {0, 0, 0, 0};
+};
+
+extern "C" int cmpmc_diff_forward(cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events,
+                                  const cmpmc_diff_params* p, double t_max_s, const cmpmc_run_opts* opts, double* y,
+                                  cmpmc_sim_stats* stats, cmpmc_diff_tape** tape_out, double* kernel_ms, void* stream) {
+    if (!M || !wl || !p || !y || !tape_out) return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: null argument");
+    *tape_out = nullptr;
+    if (wl->kind != CMPMC_WL_PHONON_POINT && wl->kind != CMPMC_WL_PHONON_LOGN && wl->kind != CMPMC_WL_PHONON_UNIF)
+        return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: a phonon workload is required");
+    if (wl->mode < 0 || wl->mode > 2 || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
+        (wl->kind == CMPMC_WL_PHONON_LOGN && !(wl->logn_median > 0.0)))
+        return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: invalid workload parameters");
+    if (!(p->B_iso > 0.0) || !(p->A_anh > 0.0) || !(p->f_diffuse > 0.0 && p->f_diffuse < 1.0) ||
+        !(p->p_film > 0.0 && p->p_film < 1.0) || !(t_max_s > 0.0))
+        return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: need B, A > 0, f and p in (0, 1), t_max > 0");
+    if (n_events >= (1ull << 32)) return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: at most 2^32 - 1 events per call");
+    for (int k = 0; k < 3; ++k) {
+        const DevSurface& sf = M->dm.surf[k];
+        if (sf.n_spec > 1 || sf.n_diff > 1 || sf.n_anh > 0)
+            return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: surface %d has energy-dependent reflection or "
+                        "down-conversion; the diffuse fraction is defined for constant splits only", k);
+    }
+    std::lock_guard<std::mutex> lk(M->mu);
+    cmpmc_internal::clear_error();
+    RunCfg C = cfg_from(opts);
+    C.form = CMPMC_FORM_BATCHED;
+    C.keep_hits = false;
+    C.want_first_branch = false;
+    CHK(check_cfg(C));
+    DiffCfg D;
+    D.dm = M->dm;
+    DevModel& dm = D.dm;
+    const double h = kHPlanck;
+    dm.B_iso = p->B_iso;
+    dm.A_anh = p->A_anh;
+    dm.c_iso_f = static_cast<float>(p->B_iso / (h * h * h * h));
+    dm.c_anh_f = static_cast<float>(p->A_anh / (h * h * h * h * h));
+    for (int k = 0; k < 3; ++k) {
+        DevSurface& sf = dm.surf[k];
+        sf.n_spec = 1; sf.spec[0] = 1.0 - p->f_diffuse;
+        sf.n_diff = 1; sf.diff[0] = p->f_diffuse;
+        sf.n_anh = 0;
+        sf.spec_prob_const = 1.0 - p->f_diffuse;
+        if (sf.has_film) sf.film.film_absorption = p->p_film;
+    }
+    fill_fp32_constants(dm);
+    D.k_f = p->f_diffuse / (1.0 - p->f_diffuse);
+    D.k_p = p->p_film / (1.0 - p->p_film);
+    D.bins_per_s = static_cast<double>(kDiffBins) / t_max_s;
+    cmpmc_diff_tape* T = new cmpmc_diff_tape();
+    T->model = M;
+    T->inv_theta[0] = 1.0 / p->B_iso;
+    T->inv_theta[1] = 1.0 / p->A_anh;
+    T->inv_theta[2] = 1.0 / p->f_diffuse;
+    T->inv_theta[3] = 1.0 / p->p_film;
+    D.tape = &T->tape;
+    Job J;
+    J.n_events = n_events;
+    J.wl = wl;
+    cmpmc_sim_stats local{};
+    double ms = 0.0;
+    Outputs O{stats ? stats : &local, nullptr, 0, nullptr, nullptr, nullptr, nullptr, &ms, nullptr};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // a first tape capacity of 48 hits per event; run_batched grows it if needed
+    const int rc0 = ensure(T->tape, std::max<uint64_t>(n_events, 1) * 48 * sizeof(TapeRec));
+    const int rc = (rc0 == CMPMC_OK) ? run_batched(M, J, C, O, s, &D) : rc0;
+    if (rc != CMPMC_OK) { T->tape.release(); delete T; return rc; }
+    T->n = D.n_tape;
+    CHK(ensure(T->gbuf, (2 * kDiffOut + 8) * sizeof(double)));
+    unsigned long long* d_y = as<unsigned long long>(T->gbuf) + kDiffOut + 8;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(d_y, 0, kDiffOut * sizeof(unsigned long long), s));
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(cmpmc_launch_tape_forward(as<TapeRec>(T->tape), T->n, d_y, s));
+    HIPCHK(hipEventRecord(e1, s));
+    unsigned long long yf[kDiffOut];
+    HIPCHK(hipMemcpyAsync(yf, d_y, sizeof(yf), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms2 = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms2, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int k = 0; k < kDiffOut; ++k) y[k] = static_cast<double>(yf[k]) / kLedgerScale;
+    if (kernel_ms) *kernel_ms = ms + ms2;
+    *tape_out = T;
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_diff_backward(cmpmc_diff_tape* T, const double* g, double* grad, double* kernel_ms, void* stream) {
+    if (!T || !g || !grad) return fail(CMPMC_ERR_INVALID, "cmpmc_diff_backward: null argument");
+    cmpmc_internal::clear_error();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int n_blocks = static_cast<int>(std::min<uint64_t>(std::max<uint64_t>((T->n + 255) / 256, 1), 2048));
+    CHK(ensure(T->gbuf, (2 * kDiffOut + 8) * sizeof(double)));
+    CHK(ensure(T->partials, static_cast<size_t>(n_blocks) * 4 * sizeof(double)));
+    double* d_g = as<double>(T->gbuf);
+    double* d_inv = d_g + kDiffOut;
+    double* d_grad = d_inv + 4;
+    HIPCHK(hipMemcpyAsync(d_g, g, kDiffOut * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_inv, T->inv_theta, 4 * sizeof(double), hipMemcpyHostToDevice, s));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(cmpmc_launch_tape_backward(as<TapeRec>(T->tape), T->n, d_g, as<double>(T->partials), n_blocks, s));
+    HIPCHK(cmpmc_launch_tape_finish(as<double>(T->partials), n_blocks, d_inv, d_grad, s));
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipMemcpyAsync(grad, d_grad, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (kernel_ms) *kernel_ms = ms;
+    return CMPMC_OK;
+}
+
+extern "C" uint64_t cmpmc_diff_tape_size(const cmpmc_diff_tape* T) { return T ? T->n : 0; }
+
+extern "C" int cmpmc_diff_tape_free(cmpmc_diff_tape* T) {
+    if (!T) return CMPMC_OK;
+    T->tape.release();
+    T->gbuf.release();
+    T->partials.release();
+    delete T;
+    return CMPMC_OK;
 }
 
 extern "C" int cmpmc_transport_serial(cmpmc_model* M, const cmpmc_phonon_track* stack, uint64_t n, uint64_t rng_state[4],

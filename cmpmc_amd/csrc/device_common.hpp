@@ -71,9 +71,13 @@ struct DevModel {
     double min_phonon_eV, luke_fraction, dk_frac, dt_cap, fermi, rr_survival;
     // FP32 copies used by the batched kernels (uniform scalar operands, no per-step
     // conversions), filled on the host by fill_fp32_constants.
+    // Reflection split of a surface whose polynomials are constants (n <= 1): piecewise
+    // constant in energy with breaks at the two cutoffs, tabulated per region
+    // r = (E > e_refl) + 2 (E > e_anh) as cumulative thresholds t_anh[r], t_spec[r].
     struct SurfF {
+        double e_refl, e_anh;
+        float t_anh[4], t_spec[4];
         float prob_abs, prob_refl, film_abs;
-        float p_anh, p_anh_spec;          // reflection split, valid when const_split != 0
         int32_t const_split, has_film, _pad;
     } surf_f[3];
     float orient_f[9];
@@ -425,11 +429,21 @@ inline void fill_fp32_constants(DevModel& m) {
         f.prob_refl = static_cast<float>(s.prob_refl);
         f.film_abs = static_cast<float>(s.film.film_absorption);
         f.has_film = s.has_film;
-        f.const_split = (s.n_spec == 0 && s.n_diff == 0 && s.n_anh == 0) ? 1 : 0;
-        double ps = 0.0, pd = 1.0, pa = 0.0;
-        if (f.const_split) reflection_split(s, 0.0, ps, pd, pa);
-        f.p_anh = static_cast<float>(pa);
-        f.p_anh_spec = static_cast<float>(pa + ps);
+        f.const_split = (s.n_spec <= 1 && s.n_diff <= 1 && s.n_anh <= 1) ? 1 : 0;
+        f.e_refl = s.refl_cutoff * 1e9 * kHPlanck;
+        f.e_anh = s.anh_cutoff * 1e9 * kHPlanck;
+        const double lo = tmin(f.e_refl, f.e_anh), hi = tmax(f.e_refl, f.e_anh);
+        for (int r = 0; r < 4; ++r) {
+            const bool above_refl = (r & 1) != 0, above_anh = (r & 2) != 0;
+            double E;                               // an energy inside region r
+            if (!above_refl && !above_anh) E = 0.5 * lo;
+            else if (above_refl && above_anh) E = 2.0 * hi;
+            else E = 0.5 * (lo + hi);
+            double ps = 0.0, pd = 1.0, pa = 0.0;
+            if (f.const_split) reflection_split(s, E, ps, pd, pa);
+            f.t_anh[r] = static_cast<float>(pa);
+            f.t_spec[r] = static_cast<float>(pa + ps);
+        }
         f._pad = 0;
     }
 }

@@ -8,12 +8,20 @@
 //    whose lanes split between the two pays for one lookup instead of two;
 //  * the inward-direction acceptance test of diffuse and specular reflection runs on
 //    that shared lookup; only a rejected first proposal takes the slow path;
-//  * surface constants come from the model's FP32 block (no per-step conversions), and
-//    a surface without reflection polynomials uses its precomputed split;
+//  * surface constants come from the model's FP32 block (no per-step conversions, read
+//    through ctx.surf(sid), which the kernel keeps in LDS), and a surface whose
+//    reflection polynomials are constants uses its tabulated split;
 //  * a phonon entering a Kaplan film is not cascaded inline: the step returns
 //    kStepFilm with the entry normal and surface, and the kernel batches such tracks so
 //    that a whole wave runs cascades together (film_job below).
 // Rare branches (anharmonic decay, surface down-conversion) keep their own lookups.
+//
+// The context also receives the likelihood terms of every random choice that depends on
+// the differentiable parameters (no-ops outside the differentiable form):
+//   ctx.flight(rate_iso, rate_down, tau)   a free flight of duration tau at these rates
+//   ctx.process(kProcIso | kProcDecay | kProcPoly)   the bulk process chosen
+//   ctx.film_gate(absorbed)                the film-absorption choice
+//   ctx.reflect(diffuse)                   the diffuse / specular choice
 #pragma once
 
 #include "phonon_step.hpp"
@@ -21,6 +29,7 @@
 namespace cmpmc_dev {
 
 enum StepCode : int { kStepDead = 0, kStepAlive = 1, kStepFilm = 2 };
+enum BulkProcess : int { kProcIso = 0, kProcDecay = 1, kProcPoly = 2 };
 
 template <class Ctx>
 __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<Fp32>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film) {
@@ -56,38 +65,39 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<Fp32>& trk, Ctx
     V3<float> knew;
     bool reflect = false, specular = false;
     if (d_phys < dist) {                          // bulk event
+        const float tau = d_phys / trk.vg_mag;
         trk.pos = trk.pos + trk.vg * d_phys;
-        trk.time += static_cast<double>(d_phys / trk.vg_mag);
+        trk.time += static_cast<double>(tau);
+        ctx.flight(rate_iso, rate_down, tau);
         const float u_proc = P::uniform(r) * rate_tot;
         if (rate_down > 0.0f && u_proc < rate_down) {
+            ctx.process(kProcDecay);
             anharmonic_decay<P>(m, trk, r, ctx);
             return kStepAlive;
         }
-        if (u_proc < rate_down + rate_iso) { ctx.count(C_ISO); ctx.branch(CMPMC_BR_ISOTOPE); }
-        else { ctx.count(C_POLY); ctx.branch(CMPMC_BR_POLY); }
+        if (u_proc < rate_down + rate_iso) { ctx.count(C_ISO); ctx.branch(CMPMC_BR_ISOTOPE); ctx.process(kProcIso); }
+        else { ctx.count(C_POLY); ctx.branch(CMPMC_BR_POLY); ctx.process(kProcPoly); }
         const float um = P::uniform(r);             // sample_dos_mode
         mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
         knew = isotropic_dir<P>(r);
     } else {                                      // boundary
+        const float tau = dist / trk.vg_mag;
         trk.pos = trk.pos + trk.vg * dist;
-        trk.time += static_cast<double>(dist / trk.vg_mag);
+        trk.time += static_cast<double>(tau);
+        ctx.flight(rate_iso, rate_down, tau);
         trk.bounces++;
-        const DevModel::SurfF& sf0 = m.surf_f[0];
-        const DevModel::SurfF& sf1 = m.surf_f[1];
-        const DevModel::SurfF& sf2 = m.surf_f[2];
-        const bool has_film = (sid == 0) ? sf0.has_film : ((sid == 1) ? sf1.has_film : sf2.has_film);
-        const float prob_abs = (sid == 0) ? sf0.prob_abs : ((sid == 1) ? sf1.prob_abs : sf2.prob_abs);
-        if (has_film && P::uniform(r) < prob_abs) {
-            const float film_abs = (sid == 0) ? sf0.film_abs : ((sid == 1) ? sf1.film_abs : sf2.film_abs);
-            if (P::uniform(r) < film_abs) {
+        const DevModel::SurfF& sf = ctx.surf(sid);
+        if (sf.has_film && P::uniform(r) < sf.prob_abs) {
+            const bool absorbed = P::uniform(r) < sf.film_abs;
+            ctx.film_gate(absorbed);
+            if (absorbed) {
                 ctx.branch(CMPMC_BR_FILM);
                 n_film = n_in;
                 sid_film = sid;
                 return kStepFilm;
             }
         }
-        const float prob_refl = (sid == 0) ? sf0.prob_refl : ((sid == 1) ? sf1.prob_refl : sf2.prob_refl);
-        if (P::uniform(r) > prob_refl) {
+        if (P::uniform(r) > sf.prob_refl) {
             ctx.count(C_KILLED);
             ctx.escaped(trk.energy * trk.weight);
             ctx.branch(CMPMC_BR_ESCAPE);
@@ -95,11 +105,11 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<Fp32>& trk, Ctx
         }
         ctx.count(C_REFL);
         trk.pos = trk.pos + n_in * 1e-11f;
-        const bool cs = (sid == 0) ? sf0.const_split : ((sid == 1) ? sf1.const_split : sf2.const_split);
         float t_anh, t_spec;
-        if (cs) {
-            t_anh = (sid == 0) ? sf0.p_anh : ((sid == 1) ? sf1.p_anh : sf2.p_anh);
-            t_spec = (sid == 0) ? sf0.p_anh_spec : ((sid == 1) ? sf1.p_anh_spec : sf2.p_anh_spec);
+        if (sf.const_split) {
+            const int reg = (trk.energy > sf.e_refl ? 1 : 0) + (trk.energy > sf.e_anh ? 2 : 0);
+            t_anh = sf.t_anh[reg];
+            t_spec = sf.t_spec[reg];
         } else {
             double p_spec, p_diff, p_anh;
             reflection_split(m.surf[sid], trk.energy, p_spec, p_diff, p_anh);
@@ -127,10 +137,12 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<Fp32>& trk, Ctx
         reflect = true;
         if (u_refl < t_spec) {
             ctx.branch(CMPMC_BR_SPECULAR);
+            ctx.reflect(false);
             specular = true;
             knew = trk.k - n_in * (2.0f * dot(trk.k, n_in));
         } else {
             ctx.branch(CMPMC_BR_DIFFUSE);
+            ctx.reflect(true);
             knew = lambertian_dir<P>(r, n_in);
         }
     }

@@ -8,6 +8,8 @@
 // entries, every `refill_period` steps (or at once when the whole wave is idle).
 // Every track draws from its own stream rng_for_track(event_base(seed, event), key), so
 // results do not depend on which lane runs which track, or when.
+#include <algorithm>
+
 #include "kernels_common.hpp"
 #include "phonon_step_fp32.hpp"
 #include "transport_kernels.h"
@@ -47,6 +49,8 @@ template <class T> __device__ inline void ptrack_to_btrack(const PTrack<T>& t, u
 }
 
 static_assert(sizeof(SRec) == 144, "SRec layout");
+static_assert(sizeof(SRecD) == 176, "SRecD layout");
+static_assert(sizeof(TapeRec) == 32, "TapeRec layout");
 
 __device__ inline void srec_to_ptrack(const SRec& b, PTrack<P>& t, uint64_t& event) {
     t.pos = {b.pos[0], b.pos[1], b.pos[2]};
@@ -81,9 +85,27 @@ __device__ inline void ptrack_to_srec(const PTrack<P>& t, uint64_t event, SRec& 
     b.hit_seq = t.hit_seq;
 }
 
-struct BatchCtx {
+// Record access for both forms: SRec, or SRecD with the track's scores.
+template <bool Diff> struct RecOps;
+template <> struct RecOps<false> {
+    using Rec = SRec;
+    __device__ static SRec& base(Rec& r) { return r; }
+    __device__ static void load_sc(const Rec&, double*) {}
+    __device__ static void store_sc(Rec&, const double*) {}
+};
+template <> struct RecOps<true> {
+    using Rec = SRecD;
+    __device__ static SRec& base(Rec& r) { return r.r; }
+    __device__ static void load_sc(const Rec& r, double* sc) { for (int i = 0; i < 4; ++i) sc[i] = r.sc[i]; }
+    __device__ static void store_sc(Rec& r, const double* sc) { for (int i = 0; i < 4; ++i) r.sc[i] = sc[i]; }
+};
+
+template <bool Diff> struct BatchCtx {
+    using Ops = RecOps<Diff>;
+    using Rec = typename Ops::Rec;
     BlockStats* bs;
-    SRec* ls;
+    const DevModel::SurfF* surfs;        // LDS copy of the model's FP32 surface block
+    Rec* ls;
     uint32_t sp, depth, lanes, lane;
     uint32_t need;
     bool overflow;
@@ -99,6 +121,12 @@ struct BatchCtx {
     double* kws;
     const double* lt;
     const double* tt;
+    // differentiable form: relative scores of the current track and the tape
+    double sc[4];
+    double k_f, k_p, bins_per_s;
+    TapeRec* tape;
+    unsigned long long* tape_count;
+    uint64_t tape_cap;
 
     __device__ void count(Counter c) { bs_count(*bs, c); }
     __device__ void ccount(CCounter c) { bs_count(*bs, C_NCOUNTERS + c); }
@@ -116,6 +144,26 @@ struct BatchCtx {
     __device__ const double* cdf_tt() const { return tt; }
     __device__ double* kaplan_base() const { return kws; }
     __device__ uint32_t kaplan_stride() const { return lanes; }
+    __device__ const DevModel::SurfF& surf(int sid) const { return surfs[sid]; }
+    // likelihood terms (relative scores theta d log L / d theta)
+    __device__ void flight(float rate_iso, float rate_down, float tau) {
+        if constexpr (Diff) {
+            sc[0] -= static_cast<double>(rate_iso) * static_cast<double>(tau);
+            sc[1] -= static_cast<double>(rate_down) * static_cast<double>(tau);
+        }
+    }
+    __device__ void process(int p) {
+        if constexpr (Diff) {
+            if (p == kProcIso) sc[0] += 1.0;
+            else if (p == kProcDecay) sc[1] += 1.0;
+        }
+    }
+    __device__ void reflect(bool diffuse) {
+        if constexpr (Diff) sc[2] += diffuse ? 1.0 : -k_f;
+    }
+    __device__ void film_gate(bool absorbed) {
+        if constexpr (Diff) sc[3] += absorbed ? 1.0 : -k_p;
+    }
     // Puts a complete track on this lane's stack; a full stack loses it (status reports
     // the overflow and the host reruns with a deeper stack).
     __device__ void store(const PTrack<P>& t) {
@@ -125,11 +173,14 @@ struct BatchCtx {
             bs_ledger(*bs, L_ESC, t.energy * t.weight);
             return;
         }
-        ptrack_to_srec(t, event, ls[static_cast<size_t>(sp) * lanes + lane]);
+        Rec& r = ls[static_cast<size_t>(sp) * lanes + lane];
+        ptrack_to_srec(t, event, Ops::base(r));
+        Ops::store_sc(r, sc);
         ++sp;
         need = tmax(need, sp);
     }
-    // A daughter: key n of its parent, stream rng_for_track(event_base, key).
+    // A daughter: key n of its parent, stream rng_for_track(event_base, key); it inherits
+    // the parent's scores at the spawn.
     __device__ void push(PTrack<P>& child, PTrack<P>& parent) {
         child.key = child_key(parent.key, ++parent.spawn_no);
         child.rng = rng_for_track(event_base(seed, event), child.key);
@@ -139,6 +190,18 @@ struct BatchCtx {
     }
     __device__ void hit(PTrack<P>& t, double wed, int sid) {
         const uint32_t seq = t.hit_seq++;
+        if constexpr (Diff) {
+            const unsigned long long i = atomicAdd(tape_count, 1ull);
+            if (i < tape_cap) {
+                TapeRec& q = tape[i];
+                for (int j = 0; j < 4; ++j) q.s[j] = static_cast<float>(sc[j]);
+                q.e = wed;
+                const double b = t.time * bins_per_s;
+                const uint32_t bin = (b >= 0.0 && b < static_cast<double>(kDiffBins)) ? static_cast<uint32_t>(b) : kNoBin;
+                q.chan_bin = static_cast<uint32_t>(sid) | (bin << 8);
+                q._pad = 0;
+            }
+        }
         if (!keep_hits) return;
         const unsigned long long i = atomicAdd(hit_count, 1ull);
         if (i < hit_cap) {
@@ -161,25 +224,32 @@ struct BatchCtx {
 //   3. otherwise every lane with a track takes one step; tracks that enter a film are
 //      appended to the wave's buffer.
 // W: waves per SIMD the register allocation is budgeted for (2: up to 256 VGPRs,
-// 3: 168, 4: 128 with some spilling); the launcher picks the variant.
-template <int W>
+// 3: 168, 4: 128 with some spilling); the launcher picks the variant. Diff: the
+// differentiable form (scores carried with every track, hits written to the tape).
+template <int W, bool Diff>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 transport_batched_kernel(const DevModel m, const BatchedArgs a) {
+    using Ctx = BatchCtx<Diff>;
+    using Ops = typename Ctx::Ops;
+    using Rec = typename Ctx::Rec;
     __shared__ double s_cdf[2 * CMPMC_N_CDF];
     __shared__ BlockStats s_stats;
+    __shared__ DevModel::SurfF s_surf[3];
     for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
         s_cdf[i] = m.cdf_lt[i];
         s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
     }
+    if (threadIdx.x < 3) s_surf[threadIdx.x] = m.surf_f[threadIdx.x];
     bs_init(s_stats, a.ledger);
     __syncthreads();
 
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lid = __lane_id();
-    SRec* const fbuf = a.film_jobs + static_cast<size_t>(lane / 64) * kFilmCap;
-    BatchCtx ctx;
+    Rec* const fbuf = static_cast<Rec*>(a.film_jobs) + static_cast<size_t>(lane / 64) * kFilmCap;
+    Ctx ctx;
     ctx.bs = &s_stats;
-    ctx.ls = a.lstack;
+    ctx.surfs = s_surf;
+    ctx.ls = static_cast<Rec*>(a.lstack);
     ctx.sp = 0;
     ctx.depth = a.depth;
     ctx.lanes = a.lanes;
@@ -198,6 +268,13 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.kws = a.kaplan_ws + lane;
     ctx.lt = s_cdf;
     ctx.tt = s_cdf + CMPMC_N_CDF;
+    for (int i = 0; i < 4; ++i) ctx.sc[i] = 0.0;
+    ctx.k_f = a.k_f;
+    ctx.k_p = a.k_p;
+    ctx.bins_per_s = a.bins_per_s;
+    ctx.tape = a.tape;
+    ctx.tape_count = a.tape_count;
+    ctx.tape_cap = a.tape_cap;
 
     PTrack<P> t;
     bool alive = false;
@@ -208,7 +285,9 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     for (uint32_t iter = 0;; ++iter) {
         if (!alive && ctx.sp > 0) {               // own spawn stack first (LIFO)
             --ctx.sp;
-            srec_to_ptrack(ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane], t, ctx.event);
+            const Rec& r = ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane];
+            srec_to_ptrack(Ops::base(const_cast<Rec&>(r)), t, ctx.event);
+            Ops::load_sc(r, ctx.sc);
             alive = true;
             ctx.rec = false;
         }
@@ -225,6 +304,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                 const unsigned long long idx = base + __popcll(want_mask & lt_mask);
                 if (idx < a.n_init) {
                     btrack_to_ptrack(a.init[idx], a.seed, t, ctx.event);
+                    for (int i = 0; i < 4; ++i) ctx.sc[i] = 0.0;
                     alive = true;
                     ctx.rec = (t.key == 0);
                 } else {
@@ -237,19 +317,24 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
             // film phase: lane i runs deferred interaction nf - take + i
             const uint32_t take = nf < 64u ? nf : 64u;
             if (lid < take) {
-                const SRec& jr = fbuf[nf - take + lid];
+                Rec& jr = fbuf[nf - take + lid];
+                const SRec& jb = Ops::base(jr);
                 PTrack<P> j;
                 uint64_t jev;
-                srec_to_ptrack(jr, j, jev);
-                const V3<float> n_in{jr.n_in[0], jr.n_in[1], jr.n_in[2]};
-                const int sid = jr.sid;
+                srec_to_ptrack(jb, j, jev);
+                const V3<float> n_in{jb.n_in[0], jb.n_in[1], jb.n_in[2]};
+                const int sid = jb.sid;
                 const uint64_t own_event = ctx.event;
                 const bool own_rec = ctx.rec;
+                double own_sc[4];
+                for (int i = 0; i < 4; ++i) own_sc[i] = ctx.sc[i];
+                Ops::load_sc(jr, ctx.sc);
                 ctx.event = jev;
                 ctx.rec = false;
                 if (film_job(m, j, n_in, sid, ctx)) ctx.store(j);
                 ctx.event = own_event;
                 ctx.rec = own_rec;
+                for (int i = 0; i < 4; ++i) ctx.sc[i] = own_sc[i];
             }
             nf -= take;
             continue;
@@ -277,10 +362,12 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
         const uint64_t film_mask = __ballot(code == kStepFilm);
         if (film_mask) {
             if (code == kStepFilm) {
-                SRec& jr = fbuf[nf + __popcll(film_mask & lt_mask)];
-                ptrack_to_srec(t, ctx.event, jr);
-                jr.n_in[0] = n_film.x; jr.n_in[1] = n_film.y; jr.n_in[2] = n_film.z;
-                jr.sid = sid_film;
+                Rec& jr = fbuf[nf + __popcll(film_mask & lt_mask)];
+                SRec& jb = Ops::base(jr);
+                ptrack_to_srec(t, ctx.event, jb);
+                jb.n_in[0] = n_film.x; jb.n_in[1] = n_film.y; jb.n_in[2] = n_film.z;
+                jb.sid = sid_film;
+                Ops::store_sc(jr, ctx.sc);
             }
             nf += static_cast<uint32_t>(__popcll(film_mask));
         }
@@ -291,6 +378,63 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     }
     __syncthreads();
     bs_flush(s_stats, a.counters, a.ledger);
+}
+
+// ------------------------------------------------------------------ differentiable form: tape reductions
+// y[c] = sum of e over hits on face c, y[2 + b] = sum of e over hits in time bin b, as
+// 2^-44 eV fixed point (order-independent).
+__global__ void tape_forward_kernel(const TapeRec* tape, uint64_t n, unsigned long long* y_fixed) {
+    __shared__ unsigned long long s_y[kDiffOut];
+    for (int i = threadIdx.x; i < kDiffOut; i += blockDim.x) s_y[i] = 0;
+    __syncthreads();
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const TapeRec q = tape[i];
+        const unsigned long long v = static_cast<unsigned long long>(__double2ll_rn(q.e * kLedgerScale));
+        const uint32_t chan = q.chan_bin & 0xFF, bin = q.chan_bin >> 8;
+        if (chan < static_cast<uint32_t>(kDiffChannels)) atomicAdd(&s_y[chan], v);
+        if (bin < static_cast<uint32_t>(kDiffBins)) atomicAdd(&s_y[kDiffChannels + bin], v);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kDiffOut; i += blockDim.x)
+        if (s_y[i]) atomicAdd(&y_fixed[i], s_y[i]);
+}
+
+// Backward: per block, partials[block][j] = sum over its hits of (g[chan] + g[2 + bin]) e s_j,
+// in a fixed order for a given grid (grid-stride loop, then an LDS tree).
+__global__ void __launch_bounds__(256) tape_backward_kernel(const TapeRec* tape, uint64_t n, const double* g,
+                                                             double* partials) {
+    __shared__ double s_g[kDiffOut];
+    __shared__ double s_red[4][256];
+    for (int i = threadIdx.x; i < kDiffOut; i += blockDim.x) s_g[i] = g[i];
+    __syncthreads();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const TapeRec q = tape[i];
+        const uint32_t chan = q.chan_bin & 0xFF, bin = q.chan_bin >> 8;
+        double w = 0.0;
+        if (chan < static_cast<uint32_t>(kDiffChannels)) w += s_g[chan];
+        if (bin < static_cast<uint32_t>(kDiffBins)) w += s_g[kDiffChannels + bin];
+        const double we = w * q.e;
+        for (int j = 0; j < 4; ++j) acc[j] += we * static_cast<double>(q.s[j]);
+    }
+    for (int j = 0; j < 4; ++j) s_red[j][threadIdx.x] = acc[j];
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (static_cast<int>(threadIdx.x) < h)
+            for (int j = 0; j < 4; ++j) s_red[j][threadIdx.x] += s_red[j][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) partials[4 * blockIdx.x + threadIdx.x] = s_red[threadIdx.x][0];
+}
+
+__global__ void tape_finish_kernel(const double* partials, int n_blocks, const double* inv_theta, double* grad) {
+    if (threadIdx.x < 4) {
+        double s = 0.0;
+        for (int b = 0; b < n_blocks; ++b) s += partials[4 * b + threadIdx.x];
+        grad[threadIdx.x] = s * inv_theta[threadIdx.x];
+    }
 }
 
 // ------------------------------------------------------------------ batched charges
@@ -550,10 +694,14 @@ using namespace cmpmc_dev;
 static inline unsigned blocks_for(uint64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 template <class K> static K pick(int waves, K k2, K k3, K k4) { return waves == 3 ? k3 : (waves == 4 ? k4 : k2); }
-static const void* batched_fn(int waves) {
-    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_batched_kernel<2>),
-                             reinterpret_cast<const void*>(&transport_batched_kernel<3>),
-                             reinterpret_cast<const void*>(&transport_batched_kernel<4>));
+static const void* batched_fn(int waves, bool diff) {
+    if (diff)
+        return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_batched_kernel<2, true>),
+                                 reinterpret_cast<const void*>(&transport_batched_kernel<3, true>),
+                                 reinterpret_cast<const void*>(&transport_batched_kernel<4, true>));
+    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_batched_kernel<2, false>),
+                             reinterpret_cast<const void*>(&transport_batched_kernel<3, false>),
+                             reinterpret_cast<const void*>(&transport_batched_kernel<4, false>));
 }
 static const void* charges_fn(int waves) {
     return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_charges_kernel<2>),
@@ -563,14 +711,34 @@ static const void* charges_fn(int waves) {
 
 // Resident blocks per CU of the persistent kernels (sizes their grids).
 extern "C" hipError_t cmpmc_occupancy(int which, int waves, int block, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, which == 0 ? batched_fn(waves) : charges_fn(waves),
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
+                                                        which == 0 ? batched_fn(waves, false) : charges_fn(waves),
                                                         block, 0);
+}
+extern "C" hipError_t cmpmc_occupancy_diff(int waves, int block, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_fn(waves, true), block, 0);
 }
 
 extern "C" hipError_t cmpmc_launch_batched(const DevModel* m, const BatchedArgs* a, int block, int grid, int waves,
-                                           hipStream_t s) {
+                                           int diff, hipStream_t s) {
     void* args[] = {const_cast<DevModel*>(m), const_cast<BatchedArgs*>(a)};
-    return hipLaunchKernel(batched_fn(waves), dim3(grid), dim3(block), args, 0, s);
+    return hipLaunchKernel(batched_fn(waves, diff != 0), dim3(grid), dim3(block), args, 0, s);
+}
+extern "C" hipError_t cmpmc_launch_tape_forward(const TapeRec* tape, uint64_t n, unsigned long long* y_fixed,
+                                                hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>(std::min<uint64_t>((n + 255) / 256 + 1, 4096));
+    hipLaunchKernelGGL(tape_forward_kernel, dim3(grid), dim3(256), 0, s, tape, n, y_fixed);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_tape_backward(const TapeRec* tape, uint64_t n, const double* g, double* partials,
+                                                 int n_blocks, hipStream_t s) {
+    hipLaunchKernelGGL(tape_backward_kernel, dim3(n_blocks), dim3(256), 0, s, tape, n, g, partials);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_tape_finish(const double* partials, int n_blocks, const double* inv_theta,
+                                               double* grad, hipStream_t s) {
+    hipLaunchKernelGGL(tape_finish_kernel, dim3(1), dim3(64), 0, s, partials, n_blocks, inv_theta, grad);
+    return hipGetLastError();
 }
 extern "C" hipError_t cmpmc_launch_charges(const DevModel* m, const ChargeArgs* a, int block, int grid, int waves,
                                            hipStream_t s) {

@@ -90,6 +90,25 @@ struct alignas(16) SRec {
     uint32_t _pad[2];
 };
 
+// Spawn record of the differentiable form: the track plus its accumulated relative
+// scores theta * d log L / d theta for (B, A, f_diffuse, p_film).
+struct alignas(16) SRecD {
+    SRec r;
+    double sc[4];
+};
+
+// One sensor hit of the differentiable form (the tape read by the backward kernel).
+struct alignas(16) TapeRec {
+    float s[4];                        // relative scores of the hit's track at the hit
+    double e;                          // collected energy x weight [eV]
+    uint32_t chan_bin;                 // surface id | time bin << 8 (bin 0xFFFF: t >= t_max)
+    uint32_t _pad;
+};
+constexpr int kDiffChannels = 2;       // top and bottom faces (surface ids 0, 1)
+constexpr int kDiffBins = 64;
+constexpr int kDiffOut = kDiffChannels + kDiffBins;
+constexpr uint32_t kNoBin = 0xFFFF;
+
 // Deferred film interactions are held per wave (kFilmCap entries) and run together once
 // kFilmFlush of them are pending, or when the wave has nothing else to do.
 constexpr uint32_t kFilmCap = 128;
@@ -102,8 +121,9 @@ struct BatchedArgs {
     const BTrack* init;                // initial queue
     uint64_t n_init;
     unsigned long long* init_head;     // claim counter of the initial queue
-    SRec* lstack;                      // per-lane spawn stacks: entry d of lane L at [d * lanes + L]
-    SRec* film_jobs;                   // per-wave deferred film interactions: [wave * kFilmCap + slot]
+    void* lstack;                      // per-lane spawn stacks (SRec, SRecD when differentiating):
+                                       //   entry d of lane L at [d * lanes + L]
+    void* film_jobs;                   // per-wave deferred film interactions: [wave * kFilmCap + slot]
     uint32_t depth;
     uint32_t lanes;
     uint32_t refill_period;
@@ -117,6 +137,12 @@ struct BatchedArgs {
     unsigned long long* status;        // [0] overflow flag, [1] deepest stack needed, [2] cancelled
     double* kaplan_ws;
     const volatile int32_t* cancel;
+    // differentiable form
+    TapeRec* tape;
+    unsigned long long* tape_count;
+    uint64_t tape_cap;
+    double bins_per_s;                 // time bins per second of the arrival histogram
+    double k_f, k_p;                   // f / (1 - f), p / (1 - p)
 };
 
 struct ChargeArgs {
@@ -148,7 +174,16 @@ extern "C" {
 hipError_t cmpmc_occupancy(int which, int waves, int block, int* blocks_per_cu);
 hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block, hipStream_t s);
 hipError_t cmpmc_launch_batched(const cmpmc_dev::DevModel* m, const cmpmc_dev::BatchedArgs* a, int block, int grid,
-                                int waves, hipStream_t s);
+                                int waves, int diff, hipStream_t s);
+hipError_t cmpmc_occupancy_diff(int waves, int block, int* blocks_per_cu);
+// Differentiable form: y[kDiffOut] from the tape (fixed-point sums), and the backward
+// reduction grad[4] = sum_h (g[chan] + g[2 + bin]) e_h s_h / theta.
+hipError_t cmpmc_launch_tape_forward(const cmpmc_dev::TapeRec* tape, uint64_t n, unsigned long long* y_fixed,
+                                     hipStream_t s);
+hipError_t cmpmc_launch_tape_backward(const cmpmc_dev::TapeRec* tape, uint64_t n, const double* g, double* partials,
+                                      int n_blocks, hipStream_t s);
+hipError_t cmpmc_launch_tape_finish(const double* partials, int n_blocks, const double* inv_theta, double* grad,
+                                    hipStream_t s);
 hipError_t cmpmc_launch_charges(const cmpmc_dev::DevModel* m, const cmpmc_dev::ChargeArgs* a, int block, int grid,
                                 int waves, hipStream_t s);
 // Device generators of the batched form: per-event counts, then the initial queue.

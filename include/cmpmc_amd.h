@@ -239,7 +239,8 @@ int cmpmc_device_info(char* buf, size_t len);
 
 /* sizeof of the ABI structs, for binding checks: 0 crystal, 1 film, 2 surface,
  * 3 detector, 4 lattice, 5 phonon track, 6 charge track, 7 hit, 8 stats,
- * 9 event batch, 10 workload, 11 run opts, 12 run out. Returns 0 for other values. */
+ * 9 event batch, 10 workload, 11 run opts, 12 run out, 13 diff params. Returns 0 for
+ * other values. */
 size_t cmpmc_abi_sizeof(int which);
 
 /* Germanium preset: materials/presets.hpp:18-25 (Germanium()) and the CrystalConfig
@@ -305,6 +306,46 @@ int cmpmc_run_events(cmpmc_model* m, const cmpmc_event_batch* batch, const cmpmc
  * event i drawing from Rng::for_stream(opts->seed, i) (rng.hpp:34-36). */
 int cmpmc_run_workload(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events,
                        const cmpmc_run_opts* opts, cmpmc_run_out* out, void* stream);
+
+/* ---------------------------------------------------------------- differentiable form
+ * No counterpart in the reference (it has no gradients): the batched FP32 transport of a
+ * phonon workload, with the likelihood-ratio (score) derivatives of its observables with
+ * respect to four parameters. The observables y[CMPMC_DIFF_OUT] are batch totals:
+ *   y[0], y[1]   energy collected (edep x weight) by the films of the top (+z) and bottom
+ *                faces (surface ids 0, 1) - the per-channel collected energy;
+ *   y[2 + b]     energy collected in arrival-time bin b of 64 equal bins over [0, t_max_s).
+ * The parameters override the model for the call: the isotope constant B (s^3, the
+ * CrystalConfig field B_iso), the anharmonic constant A (s^4, A_anh), the diffuse
+ * fraction of every surface (its specular part is 1 - f; surfaces must have constant
+ * reflection polynomials and no down-conversion) and the absorption probability of
+ * every film (KaplanFilm film_absorption). f and p must lie strictly inside (0, 1). */
+#define CMPMC_DIFF_OUT 66
+typedef struct {
+    double B_iso;
+    double A_anh;
+    double f_diffuse;
+    double p_film;
+} cmpmc_diff_params;
+
+typedef struct cmpmc_diff_tape cmpmc_diff_tape;
+
+/* Forward pass: fills y (host, CMPMC_DIFF_OUT doubles), accumulates the run's statistics
+ * into *stats (may be NULL) and returns a tape of the hits with their scores (device
+ * memory) for cmpmc_diff_backward. opts->form is ignored (always the batched FP32 form);
+ * *kernel_ms (may be NULL) receives the device time of the transport kernels. */
+int cmpmc_diff_forward(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events, const cmpmc_diff_params* params,
+                       double t_max_s, const cmpmc_run_opts* opts, double* y, cmpmc_sim_stats* stats,
+                       cmpmc_diff_tape** tape, double* kernel_ms, void* stream);
+
+/* Backward pass: grad[j] = sum_k g[k] dE[y_k]/dtheta_j for theta = (B, A, f, p), from the
+ * tape (g and grad are host arrays of CMPMC_DIFF_OUT and 4 doubles). The estimator is
+ * sum over hits h of g . (dy/dhit) e_h S_h,j / theta_j, with S_h,j the accumulated
+ * relative score theta_j d log L / d theta_j of the hit's ancestry. */
+int cmpmc_diff_backward(cmpmc_diff_tape* tape, const double* g, double* grad, double* kernel_ms, void* stream);
+
+/* Number of hits on the tape; frees the tape. */
+uint64_t cmpmc_diff_tape_size(const cmpmc_diff_tape* tape);
+int cmpmc_diff_tape_free(cmpmc_diff_tape* tape);
 
 #ifdef __cplusplus
 }
