@@ -10,9 +10,14 @@ generated on the device from the per-event seeds before the timed region.
 value        events/s of the batched FP32 form over the timed steps (wall clock around
              the library call, inputs already in HBM)
 plain_*      the plain per-event FP64 kernel on the same 1e6 events (one run)
+grad_*       the differentiated workload: 1e5 events of U{1..64} L-mode 7.5 meV phonons
+             from uniform positions, theta = (Ge B, Ge A, f_diffuse 0.9, p_film 0.2);
+             grad_fwd_ratio = wall time of (differentiable forward + backward kernel) /
+             wall time of the batched forward of the same events at the same parameters
 lognormal_*  the lognormal workload (phonon count per event lognormal(200, 1) clipped to
              [1, 20000], uniform positions, DOS modes, E ~ U[1, 31] meV) on a bounded
-             slice of events, batched FP32
+             slice of events, batched FP32 (and the plain kernel on a smaller slice when
+             --lognormal-plain-events is given)
 roofline     200 B per phonon step (read + write of a 100 B FP32 track record) over the
              batched kernel's average device time per launch (HIP events on the launch
              stream), against 8 TB/s HBM; traffic = rocprof FETCH_SIZE + WRITE_SIZE bytes
@@ -73,6 +78,8 @@ def main():
     ap.add_argument("--events", type=int, default=1_000_000)
     ap.add_argument("--plain-events", type=int, default=1_000_000)
     ap.add_argument("--lognormal-events", type=int, default=100_000)
+    ap.add_argument("--lognormal-plain-events", type=int, default=0)
+    ap.add_argument("--grad-events", type=int, default=100_000)
     ap.add_argument("--cpu-events", type=int, default=100_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=42)
@@ -156,6 +163,36 @@ def main():
                              "achieved": rp["stats"]["total_steps"] * BYTES_PER_STEP_FP64 / (rp["kernel_ms"] / 1e3) / 1e9}
     out["plain_roofline"]["frac"] = out["plain_roofline"]["achieved"] * 1e9 / HBM_PEAK
 
+    if args.grad_events > 0:
+        import numpy as np
+        cr = cm.germanium()
+        theta = np.array([cr.B_iso, cr.A_anh, 0.9, 0.2])
+        det = cm.default_detector()
+        cm.apply_diff_params(cr, det, theta)
+        e5 = cm.Engine(cm.Lattice(cr), det)
+        w5 = cm.workload(cm.WL_PHONON_UNIF, mode=0, energy_eV=7.5e-3, n_min=1, n_max=64)
+        g = np.zeros(66)
+        g[2:] = (np.arange(64) + 0.5) / 64.0
+        e5.run_workload(w5, 2000, seed=1, keep_hits=False, form=cm.FORM_BATCHED)      # warm
+        eng.diff_forward(w5, 2000, theta, seed=1)[1].backward(g)
+        t = time.perf_counter()
+        rf = e5.run_workload(w5, args.grad_events, seed=args.seed, keep_hits=False, form=cm.FORM_BATCHED)
+        t_fwd = time.perf_counter() - t
+        t = time.perf_counter()
+        y, tape, info = eng.diff_forward(w5, args.grad_events, theta, seed=args.seed)
+        t_dfwd = time.perf_counter() - t
+        t = time.perf_counter()
+        grad = tape.backward(g)
+        t_bwd = time.perf_counter() - t
+        out["grad_fwd_ratio"] = (t_dfwd + t_bwd) / t_fwd
+        out["grad"] = {"events": args.grad_events, "forward_s": t_fwd, "diff_forward_s": t_dfwd, "backward_s": t_bwd,
+                       "backward_kernel_ms": tape.last_kernel_ms, "tape_hits": len(tape),
+                       "phonon_steps": rf["stats"]["total_steps"], "theta": theta.tolist(),
+                       "grad_time_objective": grad.tolist()}
+        _log(f"grad: forward {t_fwd:.3f} s, differentiable forward {t_dfwd:.3f} s, backward {t_bwd * 1e3:.2f} ms, "
+             f"ratio {out['grad_fwd_ratio']:.3f}")
+        del tape
+
     if not args.no_lognormal and args.lognormal_events > 0:
         lw = cm.workload(cm.WL_PHONON_LOGN)
         t = time.perf_counter()
@@ -167,6 +204,18 @@ def main():
         out["lognormal_kernel_s"] = rl["kernel_ms"] / 1e3
         _log(f"lognormal: {args.lognormal_events} events in {lwall:.2f} s, {rl['stats']['total_steps']:.3e} steps, "
              f"closure {_closure(rl['stats']):.1e}")
+        if args.lognormal_plain_events > 0:
+            n = args.lognormal_plain_events
+            t = time.perf_counter()
+            rb = eng.run_workload(lw, n, seed=args.seed + 1, **kw)
+            bw = time.perf_counter() - t
+            t = time.perf_counter()
+            rpl = eng.run_workload(lw, n, seed=args.seed + 1, keep_hits=False, form=cm.FORM_PLAIN)
+            pw = time.perf_counter() - t
+            out["lognormal_slice"] = {"events": n, "batched_s": bw, "plain_s": pw, "batched_vs_plain": pw / bw,
+                                      "batched_steps": rb["stats"]["total_steps"],
+                                      "plain_steps": rpl["stats"]["total_steps"]}
+            _log(f"lognormal slice of {n}: batched {bw:.2f} s, plain {pw:.2f} s")
 
     if not args.no_cpu:
         try:

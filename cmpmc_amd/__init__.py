@@ -251,7 +251,7 @@ class Engine:
                 continue
             check(rc)
             res = {"stats": out.stats.as_dict(), "stats_raw": bytes(out.stats), "hits": hits[:out.n_hits].copy(),
-                   "hit_offsets": offs, "kernel_ms": out.kernel_ms, "wall_s": wall}
+                   "hit_offsets": offs, "kernel_ms": out.kernel_ms, "wall_s": wall, "passes": out.passes}
             if first_branch:
                 res["first_branch"] = fb[:n_events].copy()
             return res

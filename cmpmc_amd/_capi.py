@@ -119,7 +119,7 @@ class RunOpts(C.Structure):
 class RunOut(C.Structure):
     _fields_ = [("stats", Stats), ("hits", C.c_void_p), ("hit_capacity", u64), ("n_hits", u64),
                 ("hits_needed", u64), ("hit_offsets", C.POINTER(u64)), ("first_branch", C.POINTER(i32)),
-                ("kernel_ms", dbl), ("stack_needed", u64)]
+                ("kernel_ms", dbl), ("stack_needed", u64), ("passes", u64)]
 
 
 class DiffParams(C.Structure):

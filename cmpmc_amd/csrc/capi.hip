@@ -491,6 +491,7 @@ struct Outputs {
     int32_t* first_branch;
     double* kernel_ms;
     uint64_t* stack_needed;
+    uint64_t* passes = nullptr;   // transport passes run (capacity retries add passes)
 };
 
 // ------------------------------------------------------------------ plain form
@@ -550,6 +551,7 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
         }
         const uint64_t n = c1 - c0;
         for (int attempt = 0;; ++attempt) {
+            if (O.passes) ++*O.passes;
             poff.assign(n + 1, 0);
             coff.assign(n + 1, 0);
             for (uint64_t e = 0; e < n; ++e) {
@@ -637,7 +639,6 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
             a.ledger = d_acc->ledger;
             a.status = d_acc->status;
             a.kaplan_ws = as<double>(M->ws);
-            a.ws_stride = static_cast<uint32_t>(n);
             a.cancel = C.poll_cancel ? M->d_cancel : nullptr;
             HIPCHK(hipEventRecord(ev0, s));
             HIPCHK(cmpmc_launch_plain(&M->dm, &a, C.block, s));
@@ -710,6 +711,8 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 // ------------------------------------------------------------------ batched form
 // Register-budget variant of the persistent kernels when the options leave it open.
 constexpr int kDefaultWaves = 3;
+// Spawn-pool entries per wave of the batched phonon kernel when the options leave it open.
+constexpr uint64_t kDefaultPool = 2048;
 // which: 0 phonon kernel, 1 charge kernel, 2 phonon kernel of the differentiable form
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
@@ -748,7 +751,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventCreate(&e2));
-    uint64_t depth = C.lane_depth ? C.lane_depth : 512;
+    uint64_t depth = C.lane_depth ? C.lane_depth : kDefaultPool;   // spawn-pool entries per wave
     const int waves = C.waves ? C.waves : kDefaultWaves;
     const int grid = grid_for(M, D ? 2 : 0, C, block, waves);
     const int cgrid = grid_for(M, 1, C, block, waves);
@@ -779,6 +782,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     float ms_total = 0.0f;
     int rc = CMPMC_OK;
     for (int attempt = 0;; ++attempt) {
+        if (O.passes) ++*O.passes;
         Acc zero{};
         HIPCHK(hipMemcpyAsync(d_acc, &zero, sizeof(Acc), hipMemcpyHostToDevice, s));
         CHK(ensure(M->queue, std::max<uint64_t>(n_init + luke_cap, 1) * sizeof(BTrack)));
@@ -854,7 +858,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         const uint64_t n_queue = n_init + n_luke;
         if (n_queue) {
             const size_t rec_bytes = D ? sizeof(SRecD) : sizeof(SRec);
-            CHK(ensure(M->lstack, static_cast<size_t>(depth) * lanes * rec_bytes));
+            CHK(ensure(M->lstack, static_cast<size_t>(depth) * (lanes / 64) * rec_bytes));
             CHK(ensure(M->fjobs, static_cast<size_t>(lanes / 64) * kFilmCap * rec_bytes));
             CHK(ensure(M->ws, static_cast<size_t>(5) * CMPMC_KCAP * lanes * sizeof(double)));
             BatchedArgs ba{};
@@ -864,9 +868,9 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.init = q;
             ba.n_init = n_queue;
             ba.init_head = &d_acc->head;
-            ba.lstack = M->lstack.p;
+            ba.pool = M->lstack.p;
             ba.film_jobs = M->fjobs.p;
-            ba.depth = static_cast<uint32_t>(depth);
+            ba.pool_cap = static_cast<uint32_t>(depth);
             ba.lanes = static_cast<uint32_t>(lanes);
             ba.refill_period = C.refill_period ? C.refill_period : 1;
             ba.keep_hits = keep ? 1 : 0;
@@ -899,9 +903,9 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         ms_total += ms;
         if (res.status[2]) { rc = fail(CMPMC_ERR_CANCELLED, "run cancelled by the host"); break; }
         if (res.status[0]) {                      // a lane spawn stack overflowed: rerun deeper
-            if (attempt >= 4 || depth >= (1u << 16)) {
+            if (attempt >= 4 || depth >= (1u << 22)) {
                 if (O.stack_needed) *O.stack_needed = res.status[1];
-                rc = fail(CMPMC_ERR_CAPACITY, "lane spawn stack overflow (depth %llu)", static_cast<unsigned long long>(depth));
+                rc = fail(CMPMC_ERR_CAPACITY, "spawn pool overflow (%llu entries per wave)", static_cast<unsigned long long>(depth));
                 break;
             }
             depth = std::max<uint64_t>(depth * 2, res.status[1] + 64);
@@ -1015,8 +1019,9 @@ extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, cons
     const RunCfg C = cfg_from(opts);
     CHK(check_cfg(C));
     Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
-              out->first_branch, &out->kernel_ms, &out->stack_needed};
+              out->first_branch, &out->kernel_ms, &out->stack_needed, &out->passes};
     out->kernel_ms = 0.0;
+    out->passes = 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (C.form == CMPMC_FORM_BATCHED) return run_batched(M, J, C, O, s);
     return run_plain(M, J, C, O, s);
@@ -1043,8 +1048,9 @@ extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint
     const RunCfg C = cfg_from(opts);
     CHK(check_cfg(C));
     Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
-              out->first_branch, &out->kernel_ms, &out->stack_needed};
+              out->first_branch, &out->kernel_ms, &out->stack_needed, &out->passes};
     out->kernel_ms = 0.0;
+    out->passes = 0;
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (C.form == CMPMC_FORM_BATCHED) return run_batched(M, J, C, O, s);
     return run_plain(M, J, C, O, s);

@@ -105,8 +105,11 @@ template <bool Diff> struct BatchCtx {
     using Rec = typename Ops::Rec;
     BlockStats* bs;
     const DevModel::SurfF* surfs;        // LDS copy of the model's FP32 surface block
-    Rec* ls;
-    uint32_t sp, depth, lanes, lane;
+    Rec* pool;                           // this wave's spawn pool
+    volatile uint32_t* top;              // its fill level (LDS, one word per wave; written by
+                                         // whichever lane of the wave updates it, so every
+                                         // access goes to memory)
+    uint32_t cap;
     uint32_t need;
     bool overflow;
     uint64_t event;
@@ -143,7 +146,7 @@ template <bool Diff> struct BatchCtx {
     __device__ const double* cdf_lt() const { return lt; }
     __device__ const double* cdf_tt() const { return tt; }
     __device__ double* kaplan_base() const { return kws; }
-    __device__ uint32_t kaplan_stride() const { return lanes; }
+    __device__ uint32_t kaplan_stride() const { return 1; }
     __device__ const DevModel::SurfF& surf(int sid) const { return surfs[sid]; }
     // likelihood terms (relative scores theta d log L / d theta)
     __device__ void flight(float rate_iso, float rate_down, float tau) {
@@ -164,20 +167,29 @@ template <bool Diff> struct BatchCtx {
     __device__ void film_gate(bool absorbed) {
         if constexpr (Diff) sc[3] += absorbed ? 1.0 : -k_p;
     }
-    // Puts a complete track on this lane's stack; a full stack loses it (status reports
-    // the overflow and the host reruns with a deeper stack).
+    // Appends a complete track to the wave's pool. Callable from divergent code: the
+    // lanes executing the call take consecutive slots (one LDS update by the first of
+    // them). A full pool loses the track (status reports the overflow and the host reruns
+    // with a larger pool).
     __device__ void store(const PTrack<P>& t) {
-        if (sp >= depth) {
+        const uint64_t active = __ballot(1);
+        const uint32_t rank = static_cast<uint32_t>(__popcll(active & ((1ull << __lane_id()) - 1ull)));
+        uint32_t base = 0;
+        if (rank == 0) {
+            base = *top;
+            *top = base + static_cast<uint32_t>(__popcll(active));
+        }
+        base = __builtin_amdgcn_readfirstlane(base);
+        const uint32_t slot = base + rank;
+        if (slot >= cap) {
             overflow = true;
-            need = tmax(need, sp + 1);
+            need = tmax(need, slot + 1);
             bs_ledger(*bs, L_ESC, t.energy * t.weight);
             return;
         }
-        Rec& r = ls[static_cast<size_t>(sp) * lanes + lane];
+        Rec& r = pool[slot];
         ptrack_to_srec(t, event, Ops::base(r));
         Ops::store_sc(r, sc);
-        ++sp;
-        need = tmax(need, sp);
     }
     // A daughter: key n of its parent, stream rng_for_track(event_base, key); it inherits
     // the parent's scores at the spawn.
@@ -214,15 +226,19 @@ template <bool Diff> struct BatchCtx {
     }
 };
 
-// Persistent phonon kernel. Per lane: one live track in registers, a spawn stack in HBM.
-// Per wave: a buffer of deferred film interactions (wave-uniform count nf). Loop:
-//   1. a lane without a track pops its stack, else claims from the initial queue
-//      (wave-aggregated atomic every refill_period iterations, at once if the wave idles);
+// Persistent phonon kernel. Per lane: one live track in registers. Per wave: a spawn pool
+// (LIFO, in HBM, fill level in LDS) that receives every daughter and every track a film
+// re-emits, and a buffer of deferred film interactions (wave-uniform count nf). Loop:
+//   1. lanes without a track take from the wave's pool, then claim from the initial
+//      queue (wave-aggregated atomic every refill_period iterations, at once if the wave
+//      idles);
 //   2. with kFilmFlush deferred film interactions pending, or nothing else to do, the
 //      wave runs up to 64 of them, one per lane, pushing the continuing tracks and the
-//      re-emitted phonons onto the lanes' stacks;
+//      re-emitted phonons onto the pool;
 //   3. otherwise every lane with a track takes one step; tracks that enter a film are
-//      appended to the wave's buffer.
+//      appended to the wave's film buffer.
+// A pool shared by the wave balances the spawned work across its lanes, and taking from
+// it before the initial queue keeps it small.
 // W: waves per SIMD the register allocation is budgeted for (2: up to 256 VGPRs,
 // 3: 168, 4: 128 with some spilling); the launcher picks the variant. Diff: the
 // differentiable form (scores carried with every track, hits written to the tape).
@@ -235,25 +251,26 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     __shared__ double s_cdf[2 * CMPMC_N_CDF];
     __shared__ BlockStats s_stats;
     __shared__ DevModel::SurfF s_surf[3];
+    __shared__ uint32_t s_top[kMaxBlock / 64];
     for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
         s_cdf[i] = m.cdf_lt[i];
         s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
     }
     if (threadIdx.x < 3) s_surf[threadIdx.x] = m.surf_f[threadIdx.x];
+    if (threadIdx.x < kMaxBlock / 64) s_top[threadIdx.x] = 0;
     bs_init(s_stats, a.ledger);
     __syncthreads();
 
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lid = __lane_id();
-    Rec* const fbuf = static_cast<Rec*>(a.film_jobs) + static_cast<size_t>(lane / 64) * kFilmCap;
+    const uint32_t wave = lane / 64;
+    Rec* const fbuf = static_cast<Rec*>(a.film_jobs) + static_cast<size_t>(wave) * kFilmCap;
     Ctx ctx;
     ctx.bs = &s_stats;
     ctx.surfs = s_surf;
-    ctx.ls = static_cast<Rec*>(a.lstack);
-    ctx.sp = 0;
-    ctx.depth = a.depth;
-    ctx.lanes = a.lanes;
-    ctx.lane = lane;
+    ctx.pool = static_cast<Rec*>(a.pool) + static_cast<size_t>(wave) * a.pool_cap;
+    ctx.top = &s_top[threadIdx.x / 64];
+    ctx.cap = a.pool_cap;
     ctx.need = 0;
     ctx.overflow = false;
     ctx.event = 0;
@@ -265,7 +282,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.first_branch = a.first_branch;
     ctx.first_event = a.first_event;
     ctx.rec = false;
-    ctx.kws = a.kaplan_ws + lane;
+    ctx.kws = a.kaplan_ws + static_cast<size_t>(lane) * 5 * CMPMC_KCAP;   // this lane's lists, contiguous
     ctx.lt = s_cdf;
     ctx.tt = s_cdf + CMPMC_N_CDF;
     for (int i = 0; i < 4; ++i) ctx.sc[i] = 0.0;
@@ -283,17 +300,27 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     const uint32_t period = a.refill_period ? a.refill_period : 1;
     const uint64_t lt_mask = (1ull << lid) - 1ull;
     for (uint32_t iter = 0;; ++iter) {
-        if (!alive && ctx.sp > 0) {               // own spawn stack first (LIFO)
-            --ctx.sp;
-            const Rec& r = ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane];
-            srec_to_ptrack(Ops::base(const_cast<Rec&>(r)), t, ctx.event);
-            Ops::load_sc(r, ctx.sc);
-            alive = true;
-            ctx.rec = false;
+        uint64_t idle_mask = __ballot(!alive);
+        const uint32_t top = __builtin_amdgcn_readfirstlane(*ctx.top);
+        if (idle_mask && top > 0) {               // the wave's pool first (LIFO)
+            const uint32_t n_idle = static_cast<uint32_t>(__popcll(idle_mask));
+            const uint32_t take = n_idle < top ? n_idle : top;
+            if (!alive) {
+                const uint32_t r = static_cast<uint32_t>(__popcll(idle_mask & lt_mask));
+                if (r < take) {
+                    const Rec& rr = ctx.pool[top - 1 - r];
+                    srec_to_ptrack(Ops::base(const_cast<Rec&>(rr)), t, ctx.event);
+                    Ops::load_sc(rr, ctx.sc);
+                    alive = true;
+                    ctx.rec = false;
+                }
+            }
+            if (lid == 0) *ctx.top = top - take;
+            idle_mask = __ballot(!alive);
         }
         const bool want = !alive && !exhausted;
         const uint64_t want_mask = __ballot(want);
-        uint64_t alive_mask = __ballot(alive);
+        uint64_t alive_mask = ~idle_mask;
         if (want_mask && (alive_mask == 0 || iter % period == 0)) {
             const int leader = __ffsll(static_cast<unsigned long long>(want_mask)) - 1;
             unsigned long long base = 0;
@@ -340,7 +367,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
             continue;
         }
         if (alive_mask == 0) {
-            if (__ballot(!exhausted) == 0) break;
+            if (__ballot(!exhausted) == 0 && __builtin_amdgcn_readfirstlane(*ctx.top) == 0) break;
             continue;
         }
         if ((iter & 1023u) == 0 && a.cancel) {

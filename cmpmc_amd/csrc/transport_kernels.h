@@ -41,8 +41,7 @@ struct PlainArgs {
     unsigned long long* counters;      // kNumCounters
     unsigned long long* ledger;        // 2 * L_N
     unsigned long long* status;        // [0] overflow flag, [1] deepest stack needed
-    double* kaplan_ws;                 // 5 * CMPMC_KCAP * ws_stride doubles
-    uint32_t ws_stride;
+    double* kaplan_ws;                 // 5 * CMPMC_KCAP doubles per lane, lane-contiguous
     const volatile int32_t* cancel;
 };
 
@@ -71,7 +70,7 @@ struct alignas(16) BCharge {
     uint32_t _pad;
 };
 
-// Lane spawn-stack entry and deferred film interaction of the batched phonon kernel
+// Spawn-pool entry and deferred film interaction of the batched phonon kernel
 // (144 B): the complete track state including its random stream, so that a track
 // continues exactly where it stopped. n_in / sid are the film entry normal and surface.
 struct alignas(16) SRec {
@@ -121,10 +120,10 @@ struct BatchedArgs {
     const BTrack* init;                // initial queue
     uint64_t n_init;
     unsigned long long* init_head;     // claim counter of the initial queue
-    void* lstack;                      // per-lane spawn stacks (SRec, SRecD when differentiating):
-                                       //   entry d of lane L at [d * lanes + L]
+    void* pool;                        // per-wave spawn pools (SRec, SRecD when differentiating):
+                                       //   entry e of wave w at [w * pool_cap + e]
     void* film_jobs;                   // per-wave deferred film interactions: [wave * kFilmCap + slot]
-    uint32_t depth;
+    uint32_t pool_cap;
     uint32_t lanes;
     uint32_t refill_period;
     int32_t keep_hits;
@@ -134,8 +133,8 @@ struct BatchedArgs {
     int32_t* first_branch;             // per event (relative to first_event)
     unsigned long long* counters;
     unsigned long long* ledger;
-    unsigned long long* status;        // [0] overflow flag, [1] deepest stack needed, [2] cancelled
-    double* kaplan_ws;
+    unsigned long long* status;        // [0] overflow flag, [1] largest pool needed, [2] cancelled
+    double* kaplan_ws;                 // 5 * CMPMC_KCAP doubles per lane, lane-contiguous
     const volatile int32_t* cancel;
     // differentiable form
     TapeRec* tape;

@@ -193,8 +193,8 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
         ctx.first_branch = CMPMC_BR_NONE;
         ctx.first_pending = true;
         ctx.rec = false;
-        ctx.kws = a.kaplan_ws + lane;
-        ctx.kstride = a.ws_stride;
+        ctx.kws = a.kaplan_ws + lane * 5 * CMPMC_KCAP;   // this lane's lists, contiguous
+        ctx.kstride = 1;
         ctx.lt = s_cdf;
         ctx.tt = s_cdf + CMPMC_N_CDF;
         plain_event(m, a, ctx, lane);

@@ -204,7 +204,7 @@ typedef struct {
     int32_t block_size;           /* 0 = library default / tuned */
     uint64_t seed;                /* run seed (sub-streams of the batched form, device workloads) */
     uint64_t stack_headroom;      /* plain form: per-event stack entries beyond the initial ones (0 = 512) */
-    uint64_t lane_stack_depth;    /* batched form: per-lane spawn stack depth (0 = 1024) */
+    uint64_t lane_stack_depth;    /* batched form: spawn-pool entries per wavefront (0 = 2048) */
     uint32_t grid_blocks;         /* batched form: persistent grid size (0 = tuned) */
     uint32_t refill_period;       /* batched form: steps between refills of dead lanes (0 = tuned) */
     int32_t poll_cancel;          /* kernels poll the model's cancel flag (cmpmc_model_cancel) */
@@ -221,6 +221,8 @@ typedef struct {
     int32_t* first_branch;        /* n_events branch codes of the primary's first step (may be NULL) */
     double kernel_ms;             /* device time of the transport kernels (HIP events) */
     uint64_t stack_needed;        /* on CMPMC_ERR_CAPACITY: entries the deepest stack needed */
+    uint64_t passes;              /* transport passes run: a stack or queue that proved too small
+                                   * is enlarged and the work rerun (same seeds, same result) */
 } cmpmc_run_out;
 
 /* First-step branch codes */
