@@ -67,5 +67,20 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+DROPIN_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "test_dropin")
+
+
+def build_dropin_test() -> str:
+    """The drop-in C++ API test program: user code written against include/cmpmc/,
+    linked to libcmpmc_amd.so (found through a relative rpath, so the tree can move)."""
+    os.makedirs(os.path.dirname(DROPIN_BIN), exist_ok=True)
+    rel = os.path.relpath(OUT, os.path.dirname(DROPIN_BIN))
+    _run(["g++", "-std=c++20", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), DROPIN_SRC, "-o", DROPIN_BIN,
+          "-L", OUT, "-lcmpmc_amd", "-lpthread", f"-Wl,-rpath,$ORIGIN/{rel}"])
+    return DROPIN_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_dropin_test())
