@@ -14,6 +14,13 @@ grad_*       the differentiated workload: 1e5 events of U{1..64} L-mode 7.5 meV 
              from uniform positions, theta = (Ge B, Ge A, f_diffuse 0.9, p_film 0.2);
              grad_fwd_ratio = wall time of (differentiable forward + backward kernel) /
              wall time of the batched forward of the same events at the same parameters
+charge_*     e-/h+ pairs at 4 V across the 2.54 cm height, uniform in the volume, trapping
+             on (electron and hole trapping lengths 0.5 m), Luke phonons counted: 1e5 pairs,
+             batched FP32 and plain FP64 (pairs/s, charge steps/s)
+luke_*       the same pairs (trapping off) with every Luke phonon enqueued on the device
+             and tracked to absorption: 2,000 events batched (the plain form on a slice
+             when --luke-plain-events is given: its one-event-per-lane time is set by the
+             slowest events, ~50-80 s)
 lognormal_*  the lognormal workload (phonon count per event lognormal(200, 1) clipped to
              [1, 20000], uniform positions, DOS modes, E ~ U[1, 31] meV) on a bounded
              slice of events, batched FP32 (and the plain kernel on a smaller slice when
@@ -80,6 +87,9 @@ def main():
     ap.add_argument("--lognormal-events", type=int, default=100_000)
     ap.add_argument("--lognormal-plain-events", type=int, default=0)
     ap.add_argument("--grad-events", type=int, default=100_000)
+    ap.add_argument("--charge-events", type=int, default=100_000)
+    ap.add_argument("--luke-events", type=int, default=2_000)
+    ap.add_argument("--luke-plain-events", type=int, default=0)
     ap.add_argument("--cpu-events", type=int, default=100_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=42)
@@ -192,6 +202,48 @@ def main():
         _log(f"grad: forward {t_fwd:.3f} s, differentiable forward {t_dfwd:.3f} s, backward {t_bwd * 1e3:.2f} ms, "
              f"ratio {out['grad_fwd_ratio']:.3f}")
         del tape
+
+    if args.charge_events > 0 or args.luke_events > 0:
+        pair = cm.workload(cm.WL_PAIR)
+
+        def pair_engine(luke_fraction, trap_mfp):
+            cr = cm.germanium()
+            cr.e_trapping_mfp_m = cr.h_trapping_mfp_m = trap_mfp
+            det = cm.default_detector()
+            cm.set_bias(det, 4.0)
+            det.luke_sample_fraction = luke_fraction
+            return cm.Engine(cm.Lattice(cr), det)
+
+        def timed(e, n, form):
+            t = time.perf_counter()
+            r = e.run_workload(pair, n, seed=args.seed, keep_hits=False, form=form)
+            return time.perf_counter() - t, r["stats"]
+
+        if args.charge_events > 0:
+            e3 = pair_engine(0.0, 0.5)
+            timed(e3, 1000, cm.FORM_BATCHED)
+            tb, sb = timed(e3, args.charge_events, cm.FORM_BATCHED)
+            tp, sp = timed(e3, args.charge_events, cm.FORM_PLAIN)
+            out["charge"] = {"pairs": args.charge_events, "batched_pairs_per_s": args.charge_events / tb,
+                             "plain_pairs_per_s": args.charge_events / tp, "batched_vs_plain": tp / tb,
+                             "batched_charge_steps_per_s": sb["charge_steps"] / tb,
+                             "trapped_fraction": sb["charges_trapped"] / (2.0 * args.charge_events),
+                             "luke_per_pair": sb["luke_phonons_emitted"] / args.charge_events,
+                             "trap_mfp_m": 0.5}
+            _log(f"charge drift: batched {tb:.3f} s, plain {tp:.3f} s for {args.charge_events} pairs")
+        if args.luke_events > 0:
+            e4 = pair_engine(1.0, 0.0)
+            timed(e4, 100, cm.FORM_BATCHED)
+            tb, sb = timed(e4, args.luke_events, cm.FORM_BATCHED)
+            out["luke"] = {"events": args.luke_events, "batched_events_per_s": args.luke_events / tb,
+                           "luke_phonons_per_pair": sb["luke_phonons_emitted"] / args.luke_events,
+                           "deposited_eV_per_event": sb["total_deposited_eV"] / args.luke_events,
+                           "phonon_steps_per_s": sb["total_steps"] / tb}
+            if args.luke_plain_events > 0:
+                tp, sp = timed(e4, args.luke_plain_events, cm.FORM_PLAIN)
+                out["luke"]["plain_events"] = args.luke_plain_events
+                out["luke"]["plain_events_per_s"] = args.luke_plain_events / tp
+            _log(f"Luke cascade: batched {tb:.3f} s for {args.luke_events} events")
 
     if not args.no_lognormal and args.lognormal_events > 0:
         lw = cm.workload(cm.WL_PHONON_LOGN)
