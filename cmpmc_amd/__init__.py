@@ -161,6 +161,15 @@ class Lattice:
         check(capi.load().cmpmc_lattice_build(C.byref(self.crystal), self.oct.ctypes.data, self.cdf_lt.ctypes.data,
                                                self.cdf_tt.ctypes.data, C.byref(self.c)))
 
+    KINEMATICS_OCTAHEDRAL, KINEMATICS_EXACT = 0, 1
+
+    def set_kinematics_mode(self, kin_mode: int) -> None:
+        """CrystalLattice::set_kinematics_mode: 0 Octahedral (LUT, default), 1 Exact.
+        Engines take the lattice's mode when they are created."""
+        if kin_mode not in (0, 1):
+            raise ValueError("kinematics mode: 0 (Octahedral) or 1 (Exact)")
+        self.c.kinematics_mode = kin_mode
+
     def map_k_to_vg(self, mode: int, k) -> tuple[np.ndarray, np.ndarray]:
         k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
         d = np.empty_like(k)
@@ -218,6 +227,19 @@ class Engine:
 
     def cancel(self, value: bool = True) -> None:
         check(capi.load().cmpmc_model_cancel(self.ptr, int(value)))
+
+    def set_kinematics(self, kin_mode: int) -> None:
+        """Kinematics of this engine's device model: 0 Octahedral LUT, 1 Exact."""
+        check(capi.load().cmpmc_model_set_kinematics(self.ptr, kin_mode))
+
+    def device_map_k_to_vg(self, mode: int, k, form: int = FORM_PLAIN) -> tuple[np.ndarray, np.ndarray]:
+        """map_k_to_vg as the kernels of `form` compute it (FP64 plain, FP32 batched)."""
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+        d = np.empty_like(k)
+        m = np.empty(k.shape[0])
+        check(capi.load().cmpmc_model_map_k_to_vg(self.ptr, mode, form, k.shape[0], k.ctypes.data, d.ctypes.data,
+                                                  m.ctypes.data, None))
+        return d, m
 
     def trim(self) -> None:
         check(capi.load().cmpmc_model_trim(self.ptr))

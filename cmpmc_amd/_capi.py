@@ -66,7 +66,7 @@ class Lattice(C.Structure):
                 ("cdf_tt", C.POINTER(dbl)), ("e_cond_mass_eV", dbl), ("h_mass_eV", dbl),
                 ("mass_ratio", dbl * 3), ("hv_sqrt", dbl * 3), ("hv_sqrt_inv", dbl * 3),
                 ("valley_rot", ((dbl * 3) * 3) * 8), ("l0_e", dbl), ("l0_h", dbl), ("k_sound_e", dbl),
-                ("k_sound_h", dbl)]
+                ("k_sound_h", dbl), ("kinematics_mode", i32), ("_pad", i32)]
 
 
 PHONON_DTYPE = np.dtype({"names": ["pos", "k_dir", "vg_dir", "energy", "vg_mag", "time", "weight", "bounces", "mode"],
@@ -162,6 +162,9 @@ def load():
     L.cmpmc_model_destroy.argtypes = [C.c_void_p]
     L.cmpmc_model_cancel.argtypes = [C.c_void_p, C.c_int]
     L.cmpmc_model_trim.argtypes = [C.c_void_p]
+    L.cmpmc_model_set_kinematics.argtypes = [C.c_void_p, C.c_int]
+    L.cmpmc_model_map_k_to_vg.argtypes = [C.c_void_p, C.c_int, C.c_int, u64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p]
     L.cmpmc_transport_serial.argtypes = [C.c_void_p, C.c_void_p, u64, C.POINTER(u64), C.c_void_p, u64,
                                          C.POINTER(u64), C.POINTER(Stats), C.c_void_p]
     L.cmpmc_transport_charges_serial.argtypes = [C.c_void_p, C.c_void_p, u64, C.POINTER(u64), C.c_void_p, u64,
@@ -198,7 +201,8 @@ def check(code: int) -> None:
 EXPORTED_SYMBOLS = [
     "cmpmc_last_error", "cmpmc_device_info", "cmpmc_abi_sizeof", "cmpmc_crystal_germanium", "cmpmc_film_aluminum",
     "cmpmc_detector_default", "cmpmc_lattice_build", "cmpmc_map_k_to_vg", "cmpmc_model_create",
-    "cmpmc_model_destroy", "cmpmc_model_cancel", "cmpmc_model_trim", "cmpmc_transport_serial",
+    "cmpmc_model_destroy", "cmpmc_model_cancel", "cmpmc_model_trim", "cmpmc_model_set_kinematics",
+    "cmpmc_model_map_k_to_vg", "cmpmc_transport_serial",
     "cmpmc_transport_charges_serial", "cmpmc_run_events", "cmpmc_run_workload", "cmpmc_diff_forward",
     "cmpmc_diff_backward", "cmpmc_diff_tape_size", "cmpmc_diff_tape_free",
 ]

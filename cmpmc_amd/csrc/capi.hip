@@ -181,6 +181,7 @@ struct cmpmc_model {
     cmpmc_crystal cfg{};
     float4* d_oct = nullptr;
     double* d_cdf = nullptr;
+    double* d_stiff = nullptr;              // C_ijlm: 81 doubles, then 81 floats
     volatile int32_t* h_cancel = nullptr;   // host-mapped
     int32_t* d_cancel = nullptr;
     std::mutex mu;
@@ -195,6 +196,10 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     m.oct = M->d_oct;
     m.cdf_lt = M->d_cdf;
     m.cdf_tt = M->d_cdf + CMPMC_N_CDF;
+    m.stiff = M->d_stiff;
+    m.stiff_f = reinterpret_cast<const float*>(M->d_stiff + 81);
+    m.kin_mode = lat->kinematics_mode;
+    m.density = lat->cfg.density;
     std::memcpy(m.orient, c.orient, sizeof(m.orient));
     m.B_iso = c.B_iso; m.A_anh = c.A_anh; m.fTT = c.fTT_frac; m.poly_mfp = c.polycrystal_mfp_m;
     m.fL = c.fLDOS; m.fST = c.fSTDOS; m.fFT = c.fFTDOS; m.v_sound = c.v_sound; m.v_trans = c.v_trans;
@@ -260,6 +265,9 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
         return fail(CMPMC_ERR_INVALID, "cmpmc_model_create: null argument");
     *out = nullptr;
     CHK(check_detector(det));
+    if (lat->kinematics_mode != 0 && lat->kinematics_mode != 1)
+        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0) and Exact (1) are available",
+                    lat->kinematics_mode);
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CMPMC_ERR_NO_DEVICE, "cmpmc_model_create: no GPU visible");
     cmpmc_model* M = new cmpmc_model();
@@ -268,13 +276,22 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
     HIPCHK(hipGetDeviceProperties(&prop, M->device));
     M->n_cu = prop.multiProcessorCount;
     const size_t oct_bytes = sizeof(float) * 3 * CMPMC_N_OCT * CMPMC_N_OCT * 4;
-    if (hipMalloc(&M->d_oct, oct_bytes) != hipSuccess || hipMalloc(&M->d_cdf, 2 * CMPMC_N_CDF * sizeof(double)) != hipSuccess) {
+    if (hipMalloc(&M->d_oct, oct_bytes) != hipSuccess || hipMalloc(&M->d_cdf, 2 * CMPMC_N_CDF * sizeof(double)) != hipSuccess ||
+        hipMalloc(&M->d_stiff, 81 * (sizeof(double) + sizeof(float))) != hipSuccess) {
         delete M;
         return fail(CMPMC_ERR_HIP, "cmpmc_model_create: table allocation failed");
     }
     HIPCHK(hipMemcpy(M->d_oct, lat->oct_lut, oct_bytes, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(M->d_cdf, lat->cdf_lt, CMPMC_N_CDF * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(M->d_cdf + CMPMC_N_CDF, lat->cdf_tt, CMPMC_N_CDF * sizeof(double), hipMemcpyHostToDevice));
+    {
+        double Cd[81];
+        float Cf[81];
+        cmpmc_internal::stiffness_tensor(lat->cfg, Cd);
+        for (int i = 0; i < 81; ++i) Cf[i] = static_cast<float>(Cd[i]);
+        HIPCHK(hipMemcpy(M->d_stiff, Cd, sizeof(Cd), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M->d_stiff + 81, Cf, sizeof(Cf), hipMemcpyHostToDevice));
+    }
     void* hc = nullptr;
     HIPCHK(hipHostMalloc(&hc, sizeof(int32_t), hipHostMallocMapped));
     M->h_cancel = static_cast<volatile int32_t*>(hc);
@@ -300,6 +317,7 @@ extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
     trim(M);
     if (M->d_oct) (void)hipFree(M->d_oct);
     if (M->d_cdf) (void)hipFree(M->d_cdf);
+    if (M->d_stiff) (void)hipFree(M->d_stiff);
     if (M->h_cancel) (void)hipHostFree(const_cast<int32_t*>(M->h_cancel));
     delete M;
     return CMPMC_OK;
@@ -316,6 +334,48 @@ extern "C" int cmpmc_model_trim(cmpmc_model* M) {
     std::lock_guard<std::mutex> lk(M->mu);
     trim(M);
     return CMPMC_OK;
+}
+
+extern "C" int cmpmc_model_set_kinematics(cmpmc_model* M, int mode) {
+    if (!M) return fail(CMPMC_ERR_INVALID, "null model");
+    if (mode != 0 && mode != 1)
+        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0) and Exact (1) are available", mode);
+    std::lock_guard<std::mutex> lk(M->mu);
+    M->dm.kin_mode = mode;
+    return CMPMC_OK;
+}
+
+extern "C" int cmpmc_model_map_k_to_vg(cmpmc_model* M, int mode, int form, uint64_t n, const double* k, double* vg_dir,
+                                       double* vg_mag, void* stream) {
+    if (!M || (n && (!k || !vg_dir || !vg_mag)) || mode < 0 || mode > 2 ||
+        (form != CMPMC_FORM_PLAIN && form != CMPMC_FORM_BATCHED))
+        return fail(CMPMC_ERR_INVALID, "cmpmc_model_map_k_to_vg: bad argument");
+    if (n == 0) return CMPMC_OK;
+    std::lock_guard<std::mutex> lk(M->mu);
+    cmpmc_internal::clear_error();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevBuf buf;
+    CHK(ensure(buf, n * 7 * sizeof(double)));
+    double* d_k = static_cast<double*>(buf.p);
+    double* d_out = d_k + 3 * n;
+    int rc = CMPMC_OK;
+    if (hipMemcpyAsync(d_k, k, n * 3 * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        cmpmc_launch_map_query(&M->dm, form == CMPMC_FORM_BATCHED ? 1 : 0, mode, n, d_k, d_out, s) != hipSuccess) {
+        rc = fail(CMPMC_ERR_HIP, "cmpmc_model_map_k_to_vg: launch failed");
+    } else {
+        std::vector<double> out(4 * n);
+        if (hipMemcpyAsync(out.data(), d_out, n * 4 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = fail(CMPMC_ERR_HIP, "cmpmc_model_map_k_to_vg: copy failed");
+        } else {
+            for (uint64_t i = 0; i < n; ++i) {
+                vg_dir[3 * i] = out[4 * i]; vg_dir[3 * i + 1] = out[4 * i + 1]; vg_dir[3 * i + 2] = out[4 * i + 2];
+                vg_mag[i] = out[4 * i + 3];
+            }
+        }
+    }
+    buf.release();
+    return rc;
 }
 
 // ------------------------------------------------------------------ accumulators
@@ -717,8 +777,9 @@ constexpr uint64_t kDefaultPool = 2048;
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
-    const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, block, &per_cu)
-                                      : cmpmc_occupancy(which, waves, block, &per_cu);
+    const int exact = M->dm.kin_mode == 1 ? 1 : 0;
+    const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, exact, block, &per_cu)
+                                      : cmpmc_occupancy(which, waves, exact, block, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     return per_cu * M->n_cu;
 }

@@ -51,6 +51,11 @@ struct DevSurface {
 // Passed by value as a kernel argument (read through the scalar cache).
 struct DevModel {
     const float4* oct;          // [3][256][256] (vx, vy, vz, |v|)
+    const double* stiff;        // C_ijlm [3][3][3][3] (exact kinematics)
+    const float* stiff_f;       // the same in FP32
+    int32_t kin_mode;           // CrystalLattice::KinematicsMode (host side selects the kernel policy)
+    int32_t _pad_k;
+    double density;
     const double* cdf_lt;       // [1024]
     const double* cdf_tt;       // [1024]
     double orient[3][3];
@@ -147,6 +152,7 @@ __device__ inline uint64_t next_u64(Rng& r) {            // rng.hpp:49-59 (xoshi
 struct Fp64 {
     using R = double;
     static constexpr bool exact = true;
+    static constexpr bool kin_exact = false;     // octahedral LUT kinematics
     __device__ static double uniform(Rng& r) { return static_cast<double>((next_u64(r) >> 11) | 1ULL) * 0x1.0p-53; }
     __device__ static double neg_log(double u) { return -log(u); }
     // cos/sin of phi = 2 pi u, evaluated as the reference does (units::two_pi * u)
@@ -170,6 +176,7 @@ struct Fp64 {
 struct Fp32 {
     using R = float;
     static constexpr bool exact = false;
+    static constexpr bool kin_exact = false;
     // top 24 bits of the same 64-bit draw: (0, 1), never 0 or 1
     __device__ static float uniform(Rng& r) { return static_cast<float>(static_cast<uint32_t>(next_u64(r) >> 40) | 1u) * 0x1.0p-24f; }
     __device__ static float neg_log(float u) { return -0.69314718f * __builtin_amdgcn_logf(u); }
@@ -188,6 +195,16 @@ struct Fp32 {
     __device__ static float hypot_(float x, float y) { return __builtin_sqrtf(x * x + y * y); }
     __device__ static float sin_half_pi(float u) { return __builtin_amdgcn_sinf(0.25f * u); }
     __device__ static float inf() { return __builtin_huge_valf(); }
+};
+
+// The same arithmetic with KinematicsMode::Exact (per-call Christoffel solve instead of
+// the octahedral LUT); kernels are instantiated for both so that neither pays for the
+// other's code.
+struct Fp64X : Fp64 {
+    static constexpr bool kin_exact = true;
+};
+struct Fp32X : Fp32 {
+    static constexpr bool kin_exact = true;
 };
 
 template <class T> __host__ __device__ inline T tmax(T a, T b) { return (a < b) ? b : a; }   // std::max
@@ -267,8 +284,8 @@ template <class P> __device__ inline V3<typename P::R> lambertian_dir(Rng& r, V3
 // ------------------------------------------------------------------ lattice kinematics
 // CrystalLattice::map_k_to_vg_oct: crystal_lattice.hpp:117-130, 144-176
 template <class P>
-__device__ inline void map_k_to_vg(const DevModel& m, int mode, V3<typename P::R> kdet, V3<typename P::R>& vg_dir,
-                                   typename P::R& vg_mag) {
+__device__ inline void map_k_to_vg_oct(const DevModel& m, int mode, V3<typename P::R> kdet, V3<typename P::R>& vg_dir,
+                                       typename P::R& vg_mag) {
     using R = typename P::R;
     const V3<R> k = mat_mul_t(m.orient, kdet);
     const R l1 = fabs(k.x) + fabs(k.y) + fabs(k.z);
@@ -300,6 +317,206 @@ __device__ inline void map_k_to_vg(const DevModel& m, int mode, V3<typename P::R
     const V3<R> t = vdiv<P>(vc, tmax(mag, R(1e-12)));
     vg_dir = unit<P>(mat_mul(m.orient, t));
     vg_mag = mag;
+}
+
+// ---- exact kinematics (KinematicsMode::Exact): the Christoffel matrix of k, its
+// eigen-decomposition and the group velocity of the requested mode.
+// D_il = C_ijlm k_j k_m / rho; group velocity v_d = e_i C_ijld (k_j / v_p) e_l / rho with
+// L the eigenvector most parallel to k and FT/ST ordered by phase speed
+// (phonon_kernels.hpp:134-175).
+// Every entry summed on its own (not mirrored), as the Jacobi path of the reference does:
+// D[l][i] then rounds like the reference's.
+__device__ inline void christoffel_D64(const double* C, double inv_rho, const double k[3], double D[3][3]) {
+    for (int i = 0; i < 3; ++i)
+        for (int l = 0; l < 3; ++l) {
+            double sum = 0.0;
+            for (int j = 0; j < 3; ++j)
+                for (int mm = 0; mm < 3; ++mm) sum += C[((i * 3 + j) * 3 + l) * 3 + mm] * k[j] * k[mm];
+            D[i][l] = sum * inv_rho;
+        }
+}
+
+// v_g of the requested mode from eigenpairs (evals[3], evecs[3][3] as rows).
+template <class T>
+__device__ inline void vg_from_eigen(const T* C, T inv_rho, const T k[3], const T ev[3], const T evec[3][3], int mode,
+                                     T vg[3]) {
+    int long_idx = 0;
+    T max_dot = T(-1);
+    for (int i = 0; i < 3; ++i) {
+        const T d = fabs(evec[i][0] * k[0] + evec[i][1] * k[1] + evec[i][2] * k[2]);
+        if (d > max_dot) { max_dot = d; long_idx = i; }
+    }
+    const int rem1 = (long_idx + 1) % 3, rem2 = (long_idx + 2) % 3;
+    const int ft = (ev[rem1] >= ev[rem2]) ? rem1 : rem2;
+    const int st = (ev[rem1] >= ev[rem2]) ? rem2 : rem1;
+    const int idx = (mode == 0) ? long_idx : ((mode == 1) ? ft : st);
+    const T vp = sqrt(tmax(ev[idx], T(1)));
+    const T* e = evec[idx];
+    const T inv_vp = T(1) / vp;
+    const T slow[3] = {k[0] * inv_vp, k[1] * inv_vp, k[2] * inv_vp};
+    for (int dim = 0; dim < 3; ++dim) {
+        T sum = T(0);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int l = 0; l < 3; ++l) sum += e[i] * C[((i * 3 + j) * 3 + l) * 3 + dim] * slow[j] * e[l];
+        vg[dim] = sum * inv_rho;
+    }
+}
+
+// Cyclic Jacobi sweeps (phonon_kernels.hpp:182-247), FP64.
+__device__ inline void christoffel_jacobi64(const double* C, double rho, const double k[3], int mode, double vg[3]) {
+    double D[3][3];
+    const double inv_rho = 1.0 / rho;
+    christoffel_D64(C, inv_rho, k, D);
+    double V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 12; ++sweep) {
+        const double off = fabs(D[0][1]) + fabs(D[0][2]) + fabs(D[1][2]);
+        if (off < 1e-15 * (fabs(D[0][0]) + fabs(D[1][1]) + fabs(D[2][2]))) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                if (fabs(D[p][q]) < 1e-18) continue;
+                const double tau = (D[q][q] - D[p][p]) / (2.0 * D[p][q]);
+                const double t = (tau >= 0.0) ? (1.0 / (tau + sqrt(1.0 + tau * tau)))
+                                              : (-1.0 / (-tau + sqrt(1.0 + tau * tau)));
+                const double c = 1.0 / sqrt(1.0 + t * t);
+                const double s = t * c;
+                const double dpp = D[p][p], dqq = D[q][q], dpq = D[p][q];
+                D[p][p] = c * c * dpp - 2.0 * s * c * dpq + s * s * dqq;
+                D[q][q] = s * s * dpp + 2.0 * s * c * dpq + c * c * dqq;
+                D[p][q] = D[q][p] = 0.0;
+                for (int r = 0; r < 3; ++r) {
+                    if (r != p && r != q) {
+                        const double drp = D[r][p], drq = D[r][q];
+                        D[r][p] = D[p][r] = c * drp - s * drq;
+                        D[r][q] = D[q][r] = s * drp + c * drq;
+                    }
+                    const double vrp = V[r][p], vrq = V[r][q];
+                    V[r][p] = c * vrp - s * vrq;
+                    V[r][q] = s * vrp + c * vrq;
+                }
+            }
+    }
+    const double ev[3] = {D[0][0], D[1][1], D[2][2]};
+    double evec[3][3];
+    for (int i = 0; i < 3; ++i) { evec[i][0] = V[0][i]; evec[i][1] = V[1][i]; evec[i][2] = V[2][i]; }
+    vg_from_eigen<double>(C, inv_rho, k, ev, evec, mode, vg);
+}
+
+// Eigenvalues by trigonometric Cardano roots (q = tr D / 3, p = sqrt(tr (D - qI)^2 / 6),
+// r = det((D - qI) / p) / 2, lambda = q + 2p cos(acos(r)/3 + 2 pi j / 3)); eigenvectors
+// as the largest cross product of two rows of D - lambda I. Returns false (and leaves
+// vg untouched) when a relative eigenvalue gap is below gap_tol: near FT/ST degeneracies
+// the eigenvectors are ill-conditioned. phonon_kernels.hpp:255-298.
+template <class T>
+__device__ inline bool christoffel_cardano(const T* C, T rho, const T k[3], int mode, T gap_tol, T vg[3]) {
+    T D[3][3];
+    const T inv_rho = T(1) / rho;
+    for (int i = 0; i < 3; ++i)
+        for (int l = i; l < 3; ++l) {
+            T sum = T(0);
+            for (int j = 0; j < 3; ++j)
+                for (int mm = 0; mm < 3; ++mm) sum += C[((i * 3 + j) * 3 + l) * 3 + mm] * k[j] * k[mm];
+            D[i][l] = D[l][i] = sum * inv_rho;
+        }
+    const T q = (D[0][0] + D[1][1] + D[2][2]) / T(3);
+    const T off2 = D[0][1] * D[0][1] + D[0][2] * D[0][2] + D[1][2] * D[1][2];
+    const T dev2 = (D[0][0] - q) * (D[0][0] - q) + (D[1][1] - q) * (D[1][1] - q) + (D[2][2] - q) * (D[2][2] - q) +
+                   T(2) * off2;
+    const T p = sqrt(dev2 / T(6));
+    if (!(p > T(1e-12) * fabs(q))) return false;
+    T Bm[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Bm[i][j] = (D[i][j] - (i == j ? q : T(0))) / p;
+    const T det = Bm[0][0] * (Bm[1][1] * Bm[2][2] - Bm[1][2] * Bm[2][1]) -
+                  Bm[0][1] * (Bm[1][0] * Bm[2][2] - Bm[1][2] * Bm[2][0]) +
+                  Bm[0][2] * (Bm[1][0] * Bm[2][1] - Bm[1][1] * Bm[2][0]);
+    const T r = tclamp(T(0.5) * det, T(-1), T(1));
+    const T phi = acos(r) / T(3);
+    T ev[3];
+    ev[0] = q + T(2) * p * cos(phi);
+    ev[2] = q + T(2) * p * cos(phi + T(2.0943951023931957));
+    ev[1] = T(3) * q - ev[0] - ev[2];
+    const T scale = fabs(ev[0]);
+    if (fabs(ev[0] - ev[1]) < gap_tol * scale || fabs(ev[1] - ev[2]) < gap_tol * scale) return false;
+    T evec[3][3];
+    for (int n = 0; n < 3; ++n) {
+        const V3<T> r0{D[0][0] - ev[n], D[0][1], D[0][2]};
+        const V3<T> r1{D[1][0], D[1][1] - ev[n], D[1][2]};
+        const V3<T> r2{D[2][0], D[2][1], D[2][2] - ev[n]};
+        const V3<T> c01 = cross(r0, r1), c02 = cross(r0, r2), c12 = cross(r1, r2);
+        const T n01 = norm2(c01), n02 = norm2(c02), n12 = norm2(c12);
+        const V3<T> best = (n01 >= n02 && n01 >= n12) ? c01 : ((n02 >= n12) ? c02 : c12);
+        const T b2 = norm2(best);
+        const T inv = (b2 <= T(0)) ? T(0) : T(1) / sqrt(b2);
+        evec[n][0] = (b2 <= T(0)) ? T(0) : best.x * inv;
+        evec[n][1] = (b2 <= T(0)) ? T(0) : best.y * inv;
+        evec[n][2] = (b2 <= T(0)) ? T(1) : best.z * inv;
+    }
+    vg_from_eigen<T>(C, inv_rho, k, ev, evec, mode, vg);
+    return true;
+}
+
+// CrystalLattice::map_k_to_vg_exact (crystal_lattice.hpp:77-85) in FP64, in the
+// reference's operation order: Cardano with the Jacobi fallback (gap 1e-4).
+__device__ __noinline__ void map_k_to_vg_exact64(const DevModel& m, int mode, const double kdet[3], double vg_dir[3],
+                                                 double* vg_mag) {
+    const V3<double> kc = mat_mul_t(m.orient, V3<double>{kdet[0], kdet[1], kdet[2]});
+    const V3<double> ku = unit<Fp64>(kc);
+    const double k[3] = {ku.x, ku.y, ku.z};
+    double vg[3];
+    if (!christoffel_cardano<double>(m.stiff, m.density, k, mode, 1e-4, vg))
+        christoffel_jacobi64(m.stiff, m.density, k, mode, vg);
+    const V3<double> v{vg[0], vg[1], vg[2]};
+    const double mag = norm<Fp64>(v);
+    const V3<double> t = vdiv<Fp64>(v, tmax(mag, 1e-12));
+    const V3<double> o = unit<Fp64>(mat_mul(m.orient, t));
+    vg_dir[0] = o.x; vg_dir[1] = o.y; vg_dir[2] = o.z;
+    *vg_mag = mag;
+}
+
+// The same in FP32: the cubic, eigenvectors and group velocity in single precision, with
+// the FP64 solve (above) for directions whose relative eigenvalue gap is below 1e-2,
+// where FP32 eigenvectors would lose more than ~1e-5.
+__device__ __noinline__ void map_k_to_vg_exact32(const DevModel& m, int mode, const float kdet[3], float vg_dir[3],
+                                                 float* vg_mag) {
+    const float* O = m.orient_f;
+    const V3<float> kc{O[0] * kdet[0] + O[3] * kdet[1] + O[6] * kdet[2], O[1] * kdet[0] + O[4] * kdet[1] + O[7] * kdet[2],
+                       O[2] * kdet[0] + O[5] * kdet[1] + O[8] * kdet[2]};
+    const V3<float> ku = unit<Fp32>(kc);
+    const float k[3] = {ku.x, ku.y, ku.z};
+    float vg[3];
+    if (!christoffel_cardano<float>(m.stiff_f, static_cast<float>(m.density), k, mode, 1e-2f, vg)) {
+        const double kd[3] = {kdet[0], kdet[1], kdet[2]};
+        double d[3], mg;
+        map_k_to_vg_exact64(m, mode, kd, d, &mg);
+        vg_dir[0] = static_cast<float>(d[0]); vg_dir[1] = static_cast<float>(d[1]); vg_dir[2] = static_cast<float>(d[2]);
+        *vg_mag = static_cast<float>(mg);
+        return;
+    }
+    const V3<float> v{vg[0], vg[1], vg[2]};
+    const float mag = norm<Fp32>(v);
+    const V3<float> t = vdiv<Fp32>(v, tmax(mag, 1e-12f));
+    const V3<float> o = unit<Fp32>(V3<float>{O[0] * t.x + O[1] * t.y + O[2] * t.z, O[3] * t.x + O[4] * t.y + O[5] * t.z,
+                                             O[6] * t.x + O[7] * t.y + O[8] * t.z});
+    vg_dir[0] = o.x; vg_dir[1] = o.y; vg_dir[2] = o.z;
+    *vg_mag = mag;
+}
+
+// CrystalLattice::map_k_to_vg: the octahedral LUT (default) or, for the Exact policies,
+// the per-call solve.
+template <class P>
+__device__ inline void map_k_to_vg(const DevModel& m, int mode, V3<typename P::R> kdet, V3<typename P::R>& vg_dir,
+                                   typename P::R& vg_mag) {
+    using R = typename P::R;
+    if constexpr (!P::kin_exact) {
+        map_k_to_vg_oct<P>(m, mode, kdet, vg_dir, vg_mag);
+    } else {
+        const R kk[3] = {kdet.x, kdet.y, kdet.z};
+        R d[3];
+        if constexpr (P::exact) map_k_to_vg_exact64(m, mode, kk, d, &vg_mag);
+        else map_k_to_vg_exact32(m, mode, kk, d, &vg_mag);
+        vg_dir = V3<R>{d[0], d[1], d[2]};
+    }
 }
 
 // phonon_kernels.hpp:53-63

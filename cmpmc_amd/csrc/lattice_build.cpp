@@ -23,9 +23,12 @@ inline V unit(const V& a) {
     return {a.x * inv, a.y * inv, a.z * inv};
 }
 inline double comp(const V& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+inline V cross(const V& a, const V& r) { return {a.y * r.z - a.z * r.y, a.z * r.x - a.x * r.z, a.x * r.y - a.y * r.x}; }
 inline double dmax(double a, double b) { return (a < b) ? b : a; }
 
 using Stiff = double[3][3][3][3];
+
+void modes(const Stiff C, double inv_rho, const V& k, const double ev[3], const V vec[3], V vg[3]);
 
 // Cubic Voigt matrix -> C_ijlm (crystal_config.hpp:181-191; crystal_lattice.hpp:333-348)
 void stiffness(const cmpmc_crystal& c, Stiff C) {
@@ -91,6 +94,60 @@ void christoffel(const Stiff C, double rho, const V& k, V vg[3]) {
     }
     const double ev[3] = {D[0][0], D[1][1], D[2][2]};
     const V vec[3] = {{Q[0][0], Q[1][0], Q[2][0]}, {Q[0][1], Q[1][1], Q[2][1]}, {Q[0][2], Q[1][2], Q[2][2]}};
+    modes(C, inv_rho, k, ev, vec, vg);
+}
+
+// Closed-form solve (phonon_kernels.hpp:255-298): Cardano roots, eigenvectors as the
+// largest cross product of two rows of D - lambda I; a relative gap below 1e-4 takes the
+// Jacobi sweeps.
+void christoffel_cardano(const Stiff C, double rho, const V& k, V vg[3]) {
+    const double inv_rho = 1.0 / rho;
+    double D[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int l = i; l < 3; ++l) {
+            double s = 0.0;
+            for (int j = 0; j < 3; ++j)
+                for (int m = 0; m < 3; ++m) s += C[i][j][l][m] * comp(k, j) * comp(k, m);
+            D[i][l] = D[l][i] = s * inv_rho;
+        }
+    const double q = (D[0][0] + D[1][1] + D[2][2]) / 3.0;
+    const double off2 = D[0][1] * D[0][1] + D[0][2] * D[0][2] + D[1][2] * D[1][2];
+    const double dev2 = (D[0][0] - q) * (D[0][0] - q) + (D[1][1] - q) * (D[1][1] - q) + (D[2][2] - q) * (D[2][2] - q) +
+                        2.0 * off2;
+    const double p = std::sqrt(dev2 / 6.0);
+    if (!(p > 1e-12 * std::fabs(q))) { christoffel(C, rho, k, vg); return; }
+    double Bm[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Bm[i][j] = (D[i][j] - (i == j ? q : 0.0)) / p;
+    const double det = Bm[0][0] * (Bm[1][1] * Bm[2][2] - Bm[1][2] * Bm[2][1]) -
+                       Bm[0][1] * (Bm[1][0] * Bm[2][2] - Bm[1][2] * Bm[2][0]) +
+                       Bm[0][2] * (Bm[1][0] * Bm[2][1] - Bm[1][1] * Bm[2][0]);
+    const double hd = 0.5 * det;
+    const double r = (hd < -1.0) ? -1.0 : ((1.0 < hd) ? 1.0 : hd);
+    const double phi = std::acos(r) / 3.0;
+    double ev[3];
+    ev[0] = q + 2.0 * p * std::cos(phi);
+    ev[2] = q + 2.0 * p * std::cos(phi + 2.0943951023931957);
+    ev[1] = 3.0 * q - ev[0] - ev[2];
+    const double scale = std::fabs(ev[0]);
+    if (std::fabs(ev[0] - ev[1]) < 1e-4 * scale || std::fabs(ev[1] - ev[2]) < 1e-4 * scale) {
+        christoffel(C, rho, k, vg);
+        return;
+    }
+    V vec[3];
+    for (int n = 0; n < 3; ++n) {
+        const V r0{D[0][0] - ev[n], D[0][1], D[0][2]};
+        const V r1{D[1][0], D[1][1] - ev[n], D[1][2]};
+        const V r2{D[2][0], D[2][1], D[2][2] - ev[n]};
+        const V c01 = cross(r0, r1), c02 = cross(r0, r2), c12 = cross(r1, r2);
+        const double n01 = nrm2(c01), n02 = nrm2(c02), n12 = nrm2(c12);
+        vec[n] = unit((n01 >= n02 && n01 >= n12) ? c01 : ((n02 >= n12) ? c02 : c12));
+    }
+    modes(C, inv_rho, k, ev, vec, vg);
+}
+
+// Mode labelling and group velocities from eigenpairs (phonon_kernels.hpp:134-175).
+void modes(const Stiff C, double inv_rho, const V& k, const double ev[3], const V vec[3], V vg[3]) {
     // L = polarisation most parallel to k; the others ordered fast, slow by eigenvalue
     int il = 0;
     double best = -1.0;
@@ -181,6 +238,14 @@ template <class F> void inverse_cdf(double* inv, double lo, double hi, F pdf) {
 
 }  // namespace
 
+namespace cmpmc_internal {
+void stiffness_tensor(const cmpmc_crystal& c, double out[81]) {
+    Stiff C;
+    stiffness(c, C);
+    std::memcpy(out, C, sizeof(C));
+}
+}  // namespace cmpmc_internal
+
 extern "C" int cmpmc_lattice_build(const cmpmc_crystal* cfg, float* oct, double* cdf_lt, double* cdf_tt,
                                    cmpmc_lattice* out) {
     if (!cfg || !oct || !cdf_lt || !cdf_tt || !out)
@@ -258,6 +323,29 @@ extern "C" int cmpmc_map_k_to_vg(const cmpmc_lattice* lat, int mode, uint64_t n,
     if (!lat || !lat->oct_lut || (n && (!k || !vg_dir || !vg_mag)) || mode < 0 || mode > 2)
         return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_map_k_to_vg: bad argument");
     const double (*O)[3] = lat->cfg.orient;
+    if (lat->kinematics_mode == 1) {              // KinematicsMode::Exact (crystal_lattice.hpp:77-85)
+        Stiff C;
+        stiffness(lat->cfg, C);
+        for (uint64_t q = 0; q < n; ++q) {
+            const V kd{k[3 * q], k[3 * q + 1], k[3 * q + 2]};
+            const V kc = unit(V{O[0][0] * kd.x + O[1][0] * kd.y + O[2][0] * kd.z,
+                                O[0][1] * kd.x + O[1][1] * kd.y + O[2][1] * kd.z,
+                                O[0][2] * kd.x + O[1][2] * kd.y + O[2][2] * kd.z});
+            V vg[3];
+            christoffel_cardano(C, lat->cfg.density, kc, vg);
+            const V& v = vg[mode];
+            const double mag = std::sqrt(nrm2(v));
+            const double inv = 1.0 / dmax(mag, 1e-12);
+            const V t{v.x * inv, v.y * inv, v.z * inv};
+            const V d = unit(V{O[0][0] * t.x + O[0][1] * t.y + O[0][2] * t.z, O[1][0] * t.x + O[1][1] * t.y + O[1][2] * t.z,
+                               O[2][0] * t.x + O[2][1] * t.y + O[2][2] * t.z});
+            vg_dir[3 * q] = d.x;
+            vg_dir[3 * q + 1] = d.y;
+            vg_dir[3 * q + 2] = d.z;
+            vg_mag[q] = mag;
+        }
+        return CMPMC_OK;
+    }
     for (uint64_t q = 0; q < n; ++q) {
         const V kd{k[3 * q], k[3 * q + 1], k[3 * q + 2]};
         const V kc{O[0][0] * kd.x + O[1][0] * kd.y + O[2][0] * kd.z, O[0][1] * kd.x + O[1][1] * kd.y + O[2][1] * kd.z,

@@ -31,9 +31,8 @@ namespace cmpmc_dev {
 enum StepCode : int { kStepDead = 0, kStepAlive = 1, kStepFilm = 2 };
 enum BulkProcess : int { kProcIso = 0, kProcDecay = 1, kProcPoly = 2 };
 
-template <class Ctx>
-__device__ inline int phonon_step_fp32(const DevModel& m, PTrack<Fp32>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film) {
-    using P = Fp32;
+template <class P, class Ctx>
+__device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film) {
     Rng& r = trk.rng;
     if (trk.energy < m.min_phonon_eV || trk.bounces >= m.max_bounces) {
         ctx.count(C_KILLED);
@@ -176,9 +175,8 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<Fp32>& trk, Ctx
 // Kaplan film interaction of a track that phonon_step_fp32 handed over (engine.hpp:
 // 521-552): the cascade, the sensor hit, the re-emitted phonons (pushed as daughters) and
 // the continuing track, which the caller re-queues. Returns false if nothing re-emits.
-template <class Ctx>
-__device__ inline bool film_job(const DevModel& m, PTrack<Fp32>& trk, V3<float> n_in, int sid, Ctx& ctx) {
-    using P = Fp32;
+template <class P, class Ctx>
+__device__ inline bool film_job(const DevModel& m, PTrack<P>& trk, V3<float> n_in, int sid, Ctx& ctx) {
     Rng& r = trk.rng;
     const DevFilm& film = m.surf[sid].film;
     double* wb = ctx.kaplan_base();

@@ -16,8 +16,9 @@
 
 namespace cmpmc_dev {
 
-using P = Fp32;
-
+// P: Fp32 (octahedral LUT kinematics) or Fp32X (exact kinematics); every kernel below is
+// instantiated for both.
+template <class P>
 __device__ inline void btrack_to_ptrack(const BTrack& b, uint64_t seed, PTrack<P>& t, uint64_t& event) {
     t.pos = {b.pos[0], b.pos[1], b.pos[2]};
     t.k = {b.k[0], b.k[1], b.k[2]};
@@ -52,6 +53,7 @@ static_assert(sizeof(SRec) == 144, "SRec layout");
 static_assert(sizeof(SRecD) == 176, "SRecD layout");
 static_assert(sizeof(TapeRec) == 32, "TapeRec layout");
 
+template <class P>
 __device__ inline void srec_to_ptrack(const SRec& b, PTrack<P>& t, uint64_t& event) {
     t.pos = {b.pos[0], b.pos[1], b.pos[2]};
     t.k = {b.k[0], b.k[1], b.k[2]};
@@ -69,6 +71,7 @@ __device__ inline void srec_to_ptrack(const SRec& b, PTrack<P>& t, uint64_t& eve
     event = b.event;
 }
 
+template <class P>
 __device__ inline void ptrack_to_srec(const PTrack<P>& t, uint64_t event, SRec& b) {
     b.pos[0] = t.pos.x; b.pos[1] = t.pos.y; b.pos[2] = t.pos.z;
     b.k[0] = t.k.x; b.k[1] = t.k.y; b.k[2] = t.k.z;
@@ -100,7 +103,7 @@ template <> struct RecOps<true> {
     __device__ static void store_sc(Rec& r, const double* sc) { for (int i = 0; i < 4; ++i) r.sc[i] = sc[i]; }
 };
 
-template <bool Diff> struct BatchCtx {
+template <bool Diff, class P> struct BatchCtx {
     using Ops = RecOps<Diff>;
     using Rec = typename Ops::Rec;
     BlockStats* bs;
@@ -242,10 +245,10 @@ template <bool Diff> struct BatchCtx {
 // W: waves per SIMD the register allocation is budgeted for (2: up to 256 VGPRs,
 // 3: 168, 4: 128 with some spilling); the launcher picks the variant. Diff: the
 // differentiable form (scores carried with every track, hits written to the tape).
-template <int W, bool Diff>
+template <int W, bool Diff, class P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 transport_batched_kernel(const DevModel m, const BatchedArgs a) {
-    using Ctx = BatchCtx<Diff>;
+    using Ctx = BatchCtx<Diff, P>;
     using Ops = typename Ctx::Ops;
     using Rec = typename Ctx::Rec;
     __shared__ double s_cdf[2 * CMPMC_N_CDF];
@@ -358,7 +361,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                 Ops::load_sc(jr, ctx.sc);
                 ctx.event = jev;
                 ctx.rec = false;
-                if (film_job(m, j, n_in, sid, ctx)) ctx.store(j);
+                if (film_job<P>(m, j, n_in, sid, ctx)) ctx.store(j);
                 ctx.event = own_event;
                 ctx.rec = own_rec;
                 for (int i = 0; i < 4; ++i) ctx.sc[i] = own_sc[i];
@@ -382,7 +385,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
         V3<float> n_film;
         int sid_film = 0;
         if (alive) {
-            code = phonon_step_fp32(m, t, ctx, n_film, sid_film);
+            code = phonon_step_fp32<P>(m, t, ctx, n_film, sid_film);
             alive = (code == kStepAlive);
             if (!alive) ctx.rec = false;
         }
@@ -465,6 +468,7 @@ __global__ void tape_finish_kernel(const double* partials, int n_blocks, const d
 }
 
 // ------------------------------------------------------------------ batched charges
+template <class P>
 __device__ inline void bcharge_to_ctrack(const BCharge& b, uint64_t seed, CTrack<P>& c, uint64_t& event) {
     c.pos = {b.pos[0], b.pos[1], b.pos[2]};
     c.p = {b.p[0], b.p[1], b.p[2]};
@@ -491,7 +495,7 @@ template <class T> __device__ inline void ctrack_to_bcharge(const CTrack<T>& c, 
     b._pad = 0;
 }
 
-struct ChargeCtx {
+template <class P> struct ChargeCtx {
     BlockStats* bs;
     BCharge* ls;
     uint32_t sp, depth, lanes, lane;
@@ -537,14 +541,14 @@ struct ChargeCtx {
     }
 };
 
-template <int W>
+template <int W, class P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 transport_charges_kernel(const DevModel m, const ChargeArgs a) {
     __shared__ BlockStats s_stats;
     bs_init(s_stats, a.ledger);
     __syncthreads();
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
-    ChargeCtx ctx;
+    ChargeCtx<P> ctx;
     ctx.bs = &s_stats;
     ctx.ls = a.lstack;
     ctx.sp = 0;
@@ -630,6 +634,8 @@ __device__ inline uint64_t find_event(const uint64_t* offsets, uint64_t n_events
     return lo;
 }
 
+// Q: Fp64 or Fp64X, the kinematics of the model.
+template <class Q>
 __global__ void workload_fill_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
                                      uint64_t n_events, const uint64_t* offsets, uint64_t n_tracks, BTrack* out,
                                      unsigned long long* ledger) {
@@ -645,9 +651,9 @@ __global__ void workload_fill_kernel(const DevModel m, const cmpmc_workload* wl,
         // POINT draws its direction from the event stream (as the host generator does);
         // LOGN / UNIF draw track j's initial state from its generator stream.
         Rng r = (wl->kind == CMPMC_WL_PHONON_POINT) ? rng_for_stream(seed, ev) : rng_for_track(base, j | (1ull << 62));
-        PTrack<Fp64> t;
+        PTrack<Q> t;
         double E;
-        workload_phonon<Fp64>(m, *wl, r, t, E);
+        workload_phonon<Q>(m, *wl, r, t, E);
         t.key = j;
         ptrack_to_btrack(t, ev, out[g]);
         atomicAdd(&s_inj, static_cast<unsigned long long>(__double2ll_rn(E * kLedgerScale)));
@@ -714,42 +720,66 @@ __global__ void charges_to_queue_kernel(const cmpmc_charge_track* in, const uint
     b._pad = 0;
 }
 
+// map_k_to_vg queries in FP32 (the batched form's arithmetic).
+template <class P>
+__global__ void map_query32_kernel(const DevModel m, int mode, uint64_t n, const double* k, double* out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3<float> vg;
+    float mag;
+    map_k_to_vg<P>(m, mode,
+                   V3<float>{static_cast<float>(k[3 * i]), static_cast<float>(k[3 * i + 1]), static_cast<float>(k[3 * i + 2])},
+                   vg, mag);
+    out[4 * i] = vg.x; out[4 * i + 1] = vg.y; out[4 * i + 2] = vg.z; out[4 * i + 3] = mag;
+}
+
 }  // namespace cmpmc_dev
 
 using namespace cmpmc_dev;
 
+hipError_t cmpmc_launch_map_query32(const DevModel* m, int mode, uint64_t n, const double* k, double* out,
+                                    hipStream_t s) {
+    if (m->kin_mode == 1)
+        hipLaunchKernelGGL(map_query32_kernel<Fp32X>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, *m,
+                           mode, n, k, out);
+    else
+        hipLaunchKernelGGL(map_query32_kernel<Fp32>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, *m,
+                           mode, n, k, out);
+    return hipGetLastError();
+}
+
 static inline unsigned blocks_for(uint64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 template <class K> static K pick(int waves, K k2, K k3, K k4) { return waves == 3 ? k3 : (waves == 4 ? k4 : k2); }
-static const void* batched_fn(int waves, bool diff) {
-    if (diff)
-        return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_batched_kernel<2, true>),
-                                 reinterpret_cast<const void*>(&transport_batched_kernel<3, true>),
-                                 reinterpret_cast<const void*>(&transport_batched_kernel<4, true>));
-    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_batched_kernel<2, false>),
-                             reinterpret_cast<const void*>(&transport_batched_kernel<3, false>),
-                             reinterpret_cast<const void*>(&transport_batched_kernel<4, false>));
+template <bool Diff, class P> static const void* batched_fn_p(int waves) {
+    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_batched_kernel<2, Diff, P>),
+                             reinterpret_cast<const void*>(&transport_batched_kernel<3, Diff, P>),
+                             reinterpret_cast<const void*>(&transport_batched_kernel<4, Diff, P>));
 }
-static const void* charges_fn(int waves) {
-    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_charges_kernel<2>),
-                             reinterpret_cast<const void*>(&transport_charges_kernel<3>),
-                             reinterpret_cast<const void*>(&transport_charges_kernel<4>));
+static const void* batched_fn(int waves, bool diff, bool exact) {
+    if (exact) return diff ? batched_fn_p<true, Fp32X>(waves) : batched_fn_p<false, Fp32X>(waves);
+    return diff ? batched_fn_p<true, Fp32>(waves) : batched_fn_p<false, Fp32>(waves);
 }
+template <class P> static const void* charges_fn_p(int waves) {
+    return pick<const void*>(waves, reinterpret_cast<const void*>(&transport_charges_kernel<2, P>),
+                             reinterpret_cast<const void*>(&transport_charges_kernel<3, P>),
+                             reinterpret_cast<const void*>(&transport_charges_kernel<4, P>));
+}
+static const void* charges_fn(int waves, bool exact) { return exact ? charges_fn_p<Fp32X>(waves) : charges_fn_p<Fp32>(waves); }
 
 // Resident blocks per CU of the persistent kernels (sizes their grids).
-extern "C" hipError_t cmpmc_occupancy(int which, int waves, int block, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,
-                                                        which == 0 ? batched_fn(waves, false) : charges_fn(waves),
-                                                        block, 0);
+extern "C" hipError_t cmpmc_occupancy(int which, int waves, int exact, int block, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, which == 0 ? batched_fn(waves, false, exact != 0) : charges_fn(waves, exact != 0), block, 0);
 }
-extern "C" hipError_t cmpmc_occupancy_diff(int waves, int block, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_fn(waves, true), block, 0);
+extern "C" hipError_t cmpmc_occupancy_diff(int waves, int exact, int block, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_fn(waves, true, exact != 0), block, 0);
 }
 
 extern "C" hipError_t cmpmc_launch_batched(const DevModel* m, const BatchedArgs* a, int block, int grid, int waves,
                                            int diff, hipStream_t s) {
     void* args[] = {const_cast<DevModel*>(m), const_cast<BatchedArgs*>(a)};
-    return hipLaunchKernel(batched_fn(waves, diff != 0), dim3(grid), dim3(block), args, 0, s);
+    return hipLaunchKernel(batched_fn(waves, diff != 0, m->kin_mode == 1), dim3(grid), dim3(block), args, 0, s);
 }
 extern "C" hipError_t cmpmc_launch_tape_forward(const TapeRec* tape, uint64_t n, unsigned long long* y_fixed,
                                                 hipStream_t s) {
@@ -770,7 +800,7 @@ extern "C" hipError_t cmpmc_launch_tape_finish(const double* partials, int n_blo
 extern "C" hipError_t cmpmc_launch_charges(const DevModel* m, const ChargeArgs* a, int block, int grid, int waves,
                                            hipStream_t s) {
     void* args[] = {const_cast<DevModel*>(m), const_cast<ChargeArgs*>(a)};
-    return hipLaunchKernel(charges_fn(waves), dim3(grid), dim3(block), args, 0, s);
+    return hipLaunchKernel(charges_fn(waves, m->kin_mode == 1), dim3(grid), dim3(block), args, 0, s);
 }
 extern "C" hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
                                                   uint64_t n, uint32_t* counts, hipStream_t s) {
@@ -783,8 +813,12 @@ extern "C" hipError_t cmpmc_launch_workload_fill(const DevModel* m, const cmpmc_
                                                  uint64_t n_tracks, BTrack* out, unsigned long long* ledger,
                                                  hipStream_t s) {
     if (n_tracks == 0) return hipSuccess;
-    hipLaunchKernelGGL(workload_fill_kernel, dim3(blocks_for(n_tracks, 256)), dim3(256), 0, s, *m, wl, seed,
-                       first_event, n_events, offsets, n_tracks, out, ledger);
+    if (m->kin_mode == 1)
+        hipLaunchKernelGGL(workload_fill_kernel<Fp64X>, dim3(blocks_for(n_tracks, 256)), dim3(256), 0, s, *m, wl, seed,
+                           first_event, n_events, offsets, n_tracks, out, ledger);
+    else
+        hipLaunchKernelGGL(workload_fill_kernel<Fp64>, dim3(blocks_for(n_tracks, 256)), dim3(256), 0, s, *m, wl, seed,
+                           first_event, n_events, offsets, n_tracks, out, ledger);
     return hipGetLastError();
 }
 extern "C" hipError_t cmpmc_launch_pair_fill(const DevModel* m, const cmpmc_workload* wl, uint64_t seed,

@@ -169,12 +169,13 @@ struct ChargeArgs {
 }  // namespace cmpmc_dev
 
 extern "C" {
-// which: 0 phonon kernel, 1 charge kernel; waves: register-budget variant (2, 3 or 4).
-hipError_t cmpmc_occupancy(int which, int waves, int block, int* blocks_per_cu);
+// which: 0 phonon kernel, 1 charge kernel; waves: register-budget variant (2, 3 or 4);
+// exact: the exact-kinematics instantiation.
+hipError_t cmpmc_occupancy(int which, int waves, int exact, int block, int* blocks_per_cu);
 hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block, hipStream_t s);
 hipError_t cmpmc_launch_batched(const cmpmc_dev::DevModel* m, const cmpmc_dev::BatchedArgs* a, int block, int grid,
                                 int waves, int diff, hipStream_t s);
-hipError_t cmpmc_occupancy_diff(int waves, int block, int* blocks_per_cu);
+hipError_t cmpmc_occupancy_diff(int waves, int exact, int block, int* blocks_per_cu);
 // Differentiable form: y[kDiffOut] from the tape (fixed-point sums), and the backward
 // reduction grad[4] = sum_h (g[chan] + g[2 + bin]) e_h s_h / theta.
 hipError_t cmpmc_launch_tape_forward(const cmpmc_dev::TapeRec* tape, uint64_t n, unsigned long long* y_fixed,
@@ -194,6 +195,10 @@ hipError_t cmpmc_launch_workload_fill(const cmpmc_dev::DevModel* m, const cmpmc_
                                       hipStream_t s);
 hipError_t cmpmc_launch_pair_fill(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
                                   uint64_t first_event, uint64_t n_events, cmpmc_dev::BCharge* out, hipStream_t s);
+// map_k_to_vg of n directions k[3n] with the model's kinematics, FP64 (fp32 = 0) or FP32
+// arithmetic: out[4n] = (vg_dir, vg_mag).
+hipError_t cmpmc_launch_map_query(const cmpmc_dev::DevModel* m, int fp32, int mode, uint64_t n, const double* k,
+                                  double* out, hipStream_t s);
 // Host-provided initial stacks -> per-event stack regions of the plain form.
 hipError_t cmpmc_launch_scatter_tracks(const cmpmc_phonon_track* in, const uint64_t* in_off, uint64_t n_events,
                                        uint64_t n_tracks, cmpmc_phonon_track* regions, const uint64_t* region_off,

@@ -9,9 +9,8 @@
 
 namespace cmpmc_dev {
 
-using P = Fp64;
-
-struct PlainCtx {
+// P: Fp64 (octahedral LUT kinematics) or Fp64X (exact kinematics).
+template <class P> struct PlainCtx {
     BlockStats* bs;
     // stacks of this event
     cmpmc_phonon_track* ps;
@@ -95,14 +94,14 @@ struct PlainCtx {
     }
 };
 
-__device__ inline void load_phonon(const cmpmc_phonon_track& o, PTrack<P>& t) {
+template <class P> __device__ inline void load_phonon(const cmpmc_phonon_track& o, PTrack<P>& t) {
     t.pos = {o.pos[0], o.pos[1], o.pos[2]};
     t.k = {o.k_dir[0], o.k_dir[1], o.k_dir[2]};
     t.vg = {o.vg_dir[0], o.vg_dir[1], o.vg_dir[2]};
     t.energy = o.energy; t.vg_mag = o.vg_mag; t.time = o.time; t.weight = o.weight;
     t.bounces = o.bounces; t.mode = o.mode;
 }
-__device__ inline void load_charge(const cmpmc_charge_track& o, CTrack<P>& c) {
+template <class P> __device__ inline void load_charge(const cmpmc_charge_track& o, CTrack<P>& c) {
     c.pos = {o.pos[0], o.pos[1], o.pos[2]};
     c.p = {o.p_eV[0], o.p_eV[1], o.p_eV[2]};
     c.ekin = o.ekin; c.time = o.time; c.weight = o.weight;
@@ -110,7 +109,8 @@ __device__ inline void load_charge(const cmpmc_charge_track& o, CTrack<P>& c) {
 }
 
 // Runs one event (generation, charges, phonons) on this lane.
-__device__ void plain_event(const DevModel& m, const PlainArgs& a, PlainCtx& ctx, uint64_t lane) {
+template <class P>
+__device__ void plain_event(const DevModel& m, const PlainArgs& a, PlainCtx<P>& ctx, uint64_t lane) {
     Rng rng;
     if (a.wl) {
         // device generator: the event's stream, documented draw order
@@ -143,14 +143,14 @@ __device__ void plain_event(const DevModel& m, const PlainArgs& a, PlainCtx& ctx
     // transport_charges: engine.hpp:204-379
     while (ctx.csp > 0 && !ctx.overflow) {
         CTrack<P> c;
-        load_charge(ctx.cs[--ctx.csp], c);
+        load_charge<P>(ctx.cs[--ctx.csp], c);
         for (uint32_t step = 0; step < 20000000u; ++step)
             if (!charge_step<P>(m, c, rng, ctx)) break;
     }
     // transport: engine.hpp:412-581
     while (!a.skip_phonons && ctx.psp > 0 && !ctx.overflow) {
         PTrack<P> t;
-        load_phonon(ctx.ps[--ctx.psp], t);
+        load_phonon<P>(ctx.ps[--ctx.psp], t);
         if (ctx.first_pending) { ctx.rec = true; ctx.first_pending = false; }
         while (phonon_step<P>(m, t, rng, ctx)) {
         }
@@ -162,6 +162,7 @@ __device__ void plain_event(const DevModel& m, const PlainArgs& a, PlainCtx& ctx
     }
 }
 
+template <class P>
 __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, const PlainArgs a) {
     __shared__ double s_cdf[2 * CMPMC_N_CDF];
     __shared__ BlockStats s_stats;
@@ -174,7 +175,7 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
     const uint64_t lane = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = lane < a.n_events && !(a.cancel && *a.cancel);
     if (active) {
-        PlainCtx ctx;
+        PlainCtx<P> ctx;
         ctx.bs = &s_stats;
         ctx.ps = a.pstack + a.pstack_off[lane];
         ctx.pcap = static_cast<uint32_t>(a.pstack_off[lane + 1] - a.pstack_off[lane]);
@@ -197,7 +198,7 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
         ctx.kstride = 1;
         ctx.lt = s_cdf;
         ctx.tt = s_cdf + CMPMC_N_CDF;
-        plain_event(m, a, ctx, lane);
+        plain_event<P>(m, a, ctx, lane);
         if (a.first_branch) a.first_branch[lane] = ctx.first_branch;
         if (a.hits_per_event) a.hits_per_event[lane] = a.skip_phonons ? ctx.psp : ctx.seq;
         if (ctx.overflow) {
@@ -234,7 +235,34 @@ __global__ void scatter_charges_kernel(const cmpmc_charge_track* in, const uint6
     regions[region_off[e] + (g - in_off[e])] = in[g];
 }
 
+// map_k_to_vg queries in FP64 (the plain form's arithmetic).
+template <class P>
+__global__ void map_query64_kernel(const DevModel m, int mode, uint64_t n, const double* k, double* out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    V3<double> vg;
+    double mag;
+    map_k_to_vg<P>(m, mode, V3<double>{k[3 * i], k[3 * i + 1], k[3 * i + 2]}, vg, mag);
+    out[4 * i] = vg.x; out[4 * i + 1] = vg.y; out[4 * i + 2] = vg.z; out[4 * i + 3] = mag;
+}
+
 }  // namespace cmpmc_dev
+
+hipError_t cmpmc_launch_map_query32(const cmpmc_dev::DevModel* m, int mode, uint64_t n, const double* k, double* out,
+                                    hipStream_t s);
+
+extern "C" hipError_t cmpmc_launch_map_query(const cmpmc_dev::DevModel* m, int fp32, int mode, uint64_t n,
+                                             const double* k, double* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (fp32) return cmpmc_launch_map_query32(m, mode, n, k, out, s);
+    if (m->kin_mode == 1)
+        hipLaunchKernelGGL(cmpmc_dev::map_query64_kernel<cmpmc_dev::Fp64X>, dim3(static_cast<unsigned>((n + 255) / 256)),
+                           dim3(256), 0, s, *m, mode, n, k, out);
+    else
+        hipLaunchKernelGGL(cmpmc_dev::map_query64_kernel<cmpmc_dev::Fp64>, dim3(static_cast<unsigned>((n + 255) / 256)),
+                           dim3(256), 0, s, *m, mode, n, k, out);
+    return hipGetLastError();
+}
 
 extern "C" hipError_t cmpmc_launch_scatter_tracks(const cmpmc_phonon_track* in, const uint64_t* in_off,
                                                   uint64_t n_events, uint64_t n, cmpmc_phonon_track* regions,
@@ -260,6 +288,9 @@ extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmp
     if (n == 0) return hipSuccess;
     const int bs = (block > 0 && block <= cmpmc_dev::kMaxBlock) ? block : 256;
     const unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
-    hipLaunchKernelGGL(cmpmc_dev::transport_plain_kernel, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    if (m->kin_mode == 1)
+        hipLaunchKernelGGL(cmpmc_dev::transport_plain_kernel<cmpmc_dev::Fp64X>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    else
+        hipLaunchKernelGGL(cmpmc_dev::transport_plain_kernel<cmpmc_dev::Fp64>, dim3(grid), dim3(bs), 0, stream, *m, *a);
     return hipGetLastError();
 }

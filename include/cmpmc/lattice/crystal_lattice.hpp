@@ -1,7 +1,7 @@
 // Drop-in CrystalLattice (lattice/crystal_lattice.hpp of the reference). The tables are
 // built on the host by cmpmc_lattice_build (bit-identical to the reference's) and
-// uploaded to the GPU by SimulationEngine. Only the default octahedral kinematics is
-// available on the GPU path.
+// uploaded to the GPU by SimulationEngine. Octahedral (default) and Exact kinematics are
+// available; the Bilinear table is not.
 #pragma once
 
 #include <memory>
@@ -38,19 +38,38 @@ public:
     static Config germanium_config() { return materials::Germanium(); }
     const Config& config() const noexcept { return cfg_; }
     double debye_energy() const noexcept { return cfg_.debye_freq * units::h_Planck; }
-    void set_kinematics_mode(KinematicsMode m) noexcept { mode_ = m; }
+    // crystal_lattice.hpp:73-75. Engines built on this lattice follow the mode on each call.
+    void set_kinematics_mode(KinematicsMode m) noexcept {
+        mode_ = m;
+        lat_.kinematics_mode = (m == KinematicsMode::Exact) ? 1 : 0;
+    }
     KinematicsMode kinematics_mode() const noexcept { return mode_; }
 
-    // crystal_lattice.hpp:87-176 (octahedral table; the GPU path supports no other mode)
+    // crystal_lattice.hpp:77-176: the octahedral table or the exact Christoffel solve
     Kinematics map_k_to_vg(PhononMode mode, const Vec3& k_det) const {
-        if (mode_ != KinematicsMode::Octahedral)
-            throw std::invalid_argument("cmpmc_amd: only KinematicsMode::Octahedral is available");
+        if (mode_ == KinematicsMode::Bilinear)
+            throw std::invalid_argument("cmpmc_amd: KinematicsMode::Bilinear is not available (Octahedral, Exact are)");
         const double k[3] = {k_det.x, k_det.y, k_det.z};
         double d[3], mag = 0.0;
         check_status(cmpmc_map_k_to_vg(&lat_, static_cast<int>(mode), 1, k, d, &mag));
         return {Vec3{d[0], d[1], d[2]}, mag};
     }
-    Kinematics map_k_to_vg_oct(PhononMode mode, const Vec3& k) const { return map_k_to_vg(mode, k); }
+    Kinematics map_k_to_vg_oct(PhononMode mode, const Vec3& k) const {
+        cmpmc_lattice oct = lat_;
+        oct.kinematics_mode = 0;
+        const double kk[3] = {k.x, k.y, k.z};
+        double d[3], mag = 0.0;
+        check_status(cmpmc_map_k_to_vg(&oct, static_cast<int>(mode), 1, kk, d, &mag));
+        return {Vec3{d[0], d[1], d[2]}, mag};
+    }
+    Kinematics map_k_to_vg_exact(PhononMode mode, const Vec3& k) const {
+        cmpmc_lattice ex = lat_;
+        ex.kinematics_mode = 1;
+        const double kk[3] = {k.x, k.y, k.z};
+        double d[3], mag = 0.0;
+        check_status(cmpmc_map_k_to_vg(&ex, static_cast<int>(mode), 1, kk, d, &mag));
+        return {Vec3{d[0], d[1], d[2]}, mag};
+    }
 
     // rates: crystal_lattice.hpp:179-189 (phonon_kernels.hpp:29-49)
     double isotope_rate(double E) const noexcept {

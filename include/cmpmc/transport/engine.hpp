@@ -145,7 +145,14 @@ public:
 
     const CrystalLattice& lattice() const noexcept { return lattice_; }
     const DetectorModel& detector() const noexcept { return detector_; }
-    cmpmc_model* model() const noexcept { return model_.get(); }
+    // The device model, following the lattice's current kinematics mode (the reference
+    // reads the mode at every map_k_to_vg call, crystal_lattice.hpp:87-88).
+    cmpmc_model* model() const {
+        if (lattice_.kinematics_mode() == CrystalLattice::KinematicsMode::Bilinear)
+            throw std::invalid_argument("cmpmc_amd: KinematicsMode::Bilinear is not available (Octahedral, Exact are)");
+        check_status(cmpmc_model_set_kinematics(model_.get(), lattice_.pod().kinematics_mode));
+        return model_.get();
+    }
 
     // engine.hpp:171-187
     PhononTrack make_track(const Vec3& pos, const Vec3& k_dir, PhononMode mode, double energy_eV, double time_s = 0.0,
@@ -173,7 +180,7 @@ public:
             uint64_t n = 0;
             uint64_t state[4] = {rng.s[0], rng.s[1], rng.s[2], rng.s[3]};
             cmpmc_sim_stats st{};
-            const int rc = cmpmc_transport_serial(model_.get(), reinterpret_cast<const cmpmc_phonon_track*>(stack.data()),
+            const int rc = cmpmc_transport_serial(model(), reinterpret_cast<const cmpmc_phonon_track*>(stack.data()),
                                                   stack.size(), state, reinterpret_cast<cmpmc_sensor_hit*>(out.data()),
                                                   out.size(), &n, &st, nullptr);
             if (rc == CMPMC_ERR_CAPACITY && out.size() < (size_t{1} << 28)) {   // same inputs: rerun with room
@@ -205,7 +212,7 @@ public:
             uint64_t state[4] = {rng.s[0], rng.s[1], rng.s[2], rng.s[3]};
             cmpmc_sim_stats st{};
             const int rc = cmpmc_transport_charges_serial(
-                model_.get(), reinterpret_cast<const cmpmc_charge_track*>(charges.data()), charges.size(), state,
+                model(), reinterpret_cast<const cmpmc_charge_track*>(charges.data()), charges.size(), state,
                 reinterpret_cast<cmpmc_phonon_track*>(ph.data()), ph.size(), &nph,
                 reinterpret_cast<cmpmc_sensor_hit*>(out.data()), out.size(), &nh, &st, nullptr);
             if (rc == CMPMC_ERR_CAPACITY && ph.size() < (size_t{1} << 26)) {

@@ -122,6 +122,8 @@ typedef struct {
     double mass_ratio[3], hv_sqrt[3], hv_sqrt_inv[3];
     double valley_rot[8][3][3];
     double l0_e, l0_h, k_sound_e, k_sound_h;
+    int32_t kinematics_mode;      /* CrystalLattice::KinematicsMode: 0 Octahedral (default), 1 Exact */
+    int32_t _pad;
 } cmpmc_lattice;
 
 /* superconductor/kaplan_film.hpp:29 KaplanFilm (crystal_config.hpp:240-257 fields) */
@@ -260,8 +262,10 @@ int cmpmc_detector_default(cmpmc_detector* out);
 int cmpmc_lattice_build(const cmpmc_crystal* cfg, float* oct_lut, double* cdf_lt, double* cdf_tt,
                         cmpmc_lattice* out);
 
-/* CrystalLattice::map_k_to_vg (octahedral, crystal_lattice.hpp:144-176) on the host, for
- * n directions k[3*n] -> vg_dir[3*n], vg_mag[n]. Used by make_track in generators. */
+/* CrystalLattice::map_k_to_vg on the host (crystal_lattice.hpp:87-88): the octahedral
+ * table (:144-176) or, with lat->kinematics_mode == 1, the exact Christoffel solve
+ * (:77-85), for n directions k[3*n] -> vg_dir[3*n], vg_mag[n]. Used by make_track in
+ * generators. */
 int cmpmc_map_k_to_vg(const cmpmc_lattice* lat, int mode, uint64_t n, const double* k,
                       double* vg_dir, double* vg_mag);
 
@@ -278,6 +282,19 @@ int cmpmc_model_cancel(cmpmc_model* m, int value);
 
 /* Releases the model's cached device buffers (they are grown on demand by runs). */
 int cmpmc_model_trim(cmpmc_model* m);
+
+/* CrystalLattice::set_kinematics_mode (crystal_lattice.hpp:73-75) for an existing model:
+ * 0 Octahedral (the table, default), 1 Exact (per-call Christoffel solve: FP64 Cardano
+ * with the Jacobi fallback in the plain form; FP32 Cardano with that FP64 solve near
+ * degenerate directions in the batched form). The model takes the lattice's mode at
+ * creation. */
+int cmpmc_model_set_kinematics(cmpmc_model* m, int kinematics_mode);
+
+/* map_k_to_vg evaluated on the device with the model's kinematics, in the arithmetic of
+ * a form (CMPMC_FORM_PLAIN: FP64; CMPMC_FORM_BATCHED: FP32), for n host directions.
+ * No counterpart in the reference: it exposes what the kernels compute, for checks. */
+int cmpmc_model_map_k_to_vg(cmpmc_model* m, int mode, int form, uint64_t n, const double* k, double* vg_dir,
+                            double* vg_mag, void* stream);
 
 /* SimulationEngine::transport(stack, hits, stats, rng): engine.hpp:403-582, exact
  * reference semantics (one FP64 lane, LIFO, one stream). Consumes the n tracks, appends

@@ -99,14 +99,19 @@ typedef struct {
     double mass_ratio[3], hv_sqrt[3], hv_sqrt_inv[3];
     double valley_rot[8][3][3];
     double l0_e, l0_h, k_sound_e, k_sound_h;
+    int kin_mode;                    /* KinematicsMode: 0 Octahedral (default), 1 Exact */
 } orc_lattice;
 
 int  orc_lattice_init(orc_lattice* L, const orc_crystal* cfg);
 void orc_lattice_free(orc_lattice* L);
 void orc_solve_christoffel(const double C[3][3][3][3], double rho, const double k[3],
                            double vg[3][3], double vp[3]);        /* phonon_kernels.hpp:182-247 */
+void orc_solve_christoffel_cardano(const double C[3][3][3][3], double rho, const double k[3],
+                                   double vg[3][3], double vp[3]); /* phonon_kernels.hpp:255-298 */
 void orc_map_k_to_vg(const orc_lattice* L, int mode, const double k[3],
-                     double vg_dir[3], double* vg_mag);          /* crystal_lattice.hpp:144-176 */
+                     double vg_dir[3], double* vg_mag);          /* crystal_lattice.hpp:87-88, 144-176 */
+void orc_map_k_to_vg_exact(const orc_lattice* L, int mode, const double k[3],
+                           double vg_dir[3], double* vg_mag);    /* crystal_lattice.hpp:77-85 */
 double orc_tamura_lt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:98-106 */
 double orc_tamura_tt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:115-129 */
 double orc_theta_L_to_LT(const orc_crystal* c, double x);
@@ -291,6 +296,7 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
 orc_engine* orc_engine_new(const orc_crystal* cfg, const orc_detector* det);
 void orc_engine_free(orc_engine* e);
 const orc_lattice* orc_engine_lattice(const orc_engine* e);
+void orc_engine_set_kinematics(orc_engine* e, int kin_mode);   /* CrystalLattice::set_kinematics_mode */
 const float* orc_engine_oct(const orc_engine* e);
 size_t orc_sizeof(int which);   /* struct sizes, for binding checks */
 

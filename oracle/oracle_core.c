@@ -209,6 +209,60 @@ void orc_solve_christoffel(const double C[3][3][3][3], double rho, const double 
     modes_from_eigen(C, rho, k, ev, evec, vg, vp);
 }
 
+/* Closed-form solve: trigonometric roots of the characteristic cubic of D, eigenvectors
+ * as the largest cross product of two rows of D - lambda I; a relative eigenvalue gap
+ * below 1e-4 (FT/ST degeneracies) falls back to the Jacobi solve. phonon_kernels.hpp:255-298 */
+void orc_solve_christoffel_cardano(const double C[3][3][3][3], double rho, const double k[3],
+                                   double vg[3][3], double vp[3]) {
+    const double gap_tol = 1e-4;
+    double D[3][3];
+    const double inv_rho = 1.0 / rho;
+    for (int i = 0; i < 3; ++i)
+        for (int l = i; l < 3; ++l) {
+            double sum = 0.0;
+            for (int j = 0; j < 3; ++j)
+                for (int m = 0; m < 3; ++m) sum += C[i][j][l][m] * k[j] * k[m];
+            D[i][l] = D[l][i] = sum * inv_rho;
+        }
+    const double q = (D[0][0] + D[1][1] + D[2][2]) / 3.0;
+    const double off2 = D[0][1] * D[0][1] + D[0][2] * D[0][2] + D[1][2] * D[1][2];
+    const double dev2 = (D[0][0] - q) * (D[0][0] - q) + (D[1][1] - q) * (D[1][1] - q) + (D[2][2] - q) * (D[2][2] - q)
+                        + 2.0 * off2;
+    const double p = sqrt(dev2 / 6.0);
+    if (!(p > 1e-12 * fabs(q))) { orc_solve_christoffel(C, rho, k, vg, vp); return; }
+    double Bm[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Bm[i][j] = (D[i][j] - (i == j ? q : 0.0)) / p;
+    const double det = Bm[0][0] * (Bm[1][1] * Bm[2][2] - Bm[1][2] * Bm[2][1])
+                     - Bm[0][1] * (Bm[1][0] * Bm[2][2] - Bm[1][2] * Bm[2][0])
+                     + Bm[0][2] * (Bm[1][0] * Bm[2][1] - Bm[1][1] * Bm[2][0]);
+    const double r = dclamp(0.5 * det, -1.0, 1.0);
+    const double phi = acos(r) / 3.0;
+    double ev[3];
+    ev[0] = q + 2.0 * p * cos(phi);
+    ev[2] = q + 2.0 * p * cos(phi + 2.0943951023931957);
+    ev[1] = 3.0 * q - ev[0] - ev[2];
+    const double scale = fabs(ev[0]);
+    if (fabs(ev[0] - ev[1]) < gap_tol * scale || fabs(ev[1] - ev[2]) < gap_tol * scale) {
+        orc_solve_christoffel(C, rho, k, vg, vp);
+        return;
+    }
+    double evec[3][3];
+    for (int n = 0; n < 3; ++n) {
+        const double r0[3] = {D[0][0] - ev[n], D[0][1], D[0][2]};
+        const double r1[3] = {D[1][0], D[1][1] - ev[n], D[1][2]};
+        const double r2[3] = {D[2][0], D[2][1], D[2][2] - ev[n]};
+        double c01[3], c02[3], c12[3];
+        v3_cross(r0, r1, c01);
+        v3_cross(r0, r2, c02);
+        v3_cross(r1, r2, c12);
+        const double n01 = v3_norm2(c01), n02 = v3_norm2(c02), n12 = v3_norm2(c12);
+        const double* best = (n01 >= n02 && n01 >= n12) ? c01 : ((n02 >= n12) ? c02 : c12);
+        v3_unit(best, evec[n]);
+    }
+    modes_from_eigen(C, rho, k, ev, evec, vg, vp);
+}
+
 /* ---------------- decay kinematics: phonon_kernels.hpp:70-129 ---------------- */
 double orc_theta_L_to_LT(const orc_crystal* c, double x) {
     const double m = c->v_sound / c->v_trans;
@@ -408,8 +462,22 @@ static void oct_encode(const double k[3], double* uo, double* vo) {
     *vo = dclamp(py, -1.0, 1.0);
 }
 
+void orc_map_k_to_vg_exact(const orc_lattice* L, int mode, const double kdet[3],
+                           double vg_dir[3], double* vg_mag) {   /* crystal_lattice.hpp:77-85 */
+    double kc[3], k[3], vg[3][3], vp[3], t[3], vd[3];
+    m3_mul_t(L->cfg.orient, kdet, kc);
+    v3_unit(kc, k);
+    orc_solve_christoffel_cardano(L->C, L->cfg.density, k, vg, vp);
+    const double mag = v3_norm(vg[mode]);
+    v3_div(vg[mode], dmax(mag, 1e-12), t);
+    m3_mul(L->cfg.orient, t, vd);
+    v3_unit(vd, vg_dir);
+    *vg_mag = mag;
+}
+
 void orc_map_k_to_vg(const orc_lattice* L, int mode, const double kdet[3],
-                     double vg_dir[3], double* vg_mag) {   /* crystal_lattice.hpp:144-176 */
+                     double vg_dir[3], double* vg_mag) {   /* crystal_lattice.hpp:87-88, 144-176 */
+    if (L->kin_mode == 1) { orc_map_k_to_vg_exact(L, mode, kdet, vg_dir, vg_mag); return; }
     double k[3];
     m3_mul_t(L->cfg.orient, kdet, k);
     double uo, vo;

@@ -212,6 +212,9 @@ def lib():
         L.orc_child_key.argtypes = [u64, u32]
         L.orc_rng_for_track.argtypes = [C.POINTER(Rng), u64, u64]
         L.orc_map_k_to_vg.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_map_k_to_vg_exact.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_solve_christoffel_cardano.argtypes = [C.c_void_p, dbl, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_engine_set_kinematics.argtypes = [C.c_void_p, C.c_int]
         L.orc_solve_christoffel.argtypes = [C.c_void_p, dbl, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_sample_lt_fraction.restype = dbl
         L.orc_sample_lt_fraction.argtypes = [C.c_void_p, C.POINTER(Rng)]
@@ -265,6 +268,8 @@ def ref():
         R.ref_lattice_info.argtypes = [C.POINTER(dbl)]
         R.ref_decay_fractions.argtypes = [u64, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
         R.ref_map_k_to_vg.argtypes = [C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_map_k_to_vg_exact.argtypes = [C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_set_kinematics_mode.argtypes = [C.c_int]
         R.ref_christoffel.argtypes = [C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
         R.ref_absorb_phonon.argtypes = [dbl, u64, C.c_int, C.POINTER(dbl), C.POINTER(C.c_int),
                                         C.POINTER(dbl), C.POINTER(u64)]
@@ -367,16 +372,21 @@ class Engine:
     def lattice_ptr(self):
         return lib().orc_engine_lattice(self.ptr)
 
-    def map_k_to_vg(self, mode: int, k) -> np.ndarray:
+    def set_kinematics(self, kin_mode: int) -> None:
+        """CrystalLattice::set_kinematics_mode: 0 Octahedral (default), 1 Exact."""
+        lib().orc_engine_set_kinematics(self.ptr, kin_mode)
+
+    def map_k_to_vg(self, mode: int, k, exact: bool = False) -> np.ndarray:
         k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
         out = np.empty((k.shape[0], 4))
         vg = np.empty(3)
         mag = C.c_double()
         L = lib()
         lp = self.lattice_ptr()
+        fn = L.orc_map_k_to_vg_exact if exact else L.orc_map_k_to_vg
         for i in range(k.shape[0]):
             row = np.ascontiguousarray(k[i])
-            L.orc_map_k_to_vg(lp, mode, dptr(row), dptr(vg), C.byref(mag))
+            fn(lp, mode, dptr(row), dptr(vg), C.byref(mag))
             out[i, :3] = vg
             out[i, 3] = mag.value
         return out

@@ -896,6 +896,7 @@ void orc_engine_free(orc_engine* e) {
 }
 
 const orc_lattice* orc_engine_lattice(const orc_engine* e) { return e->L; }
+void orc_engine_set_kinematics(orc_engine* e, int kin_mode) { ((orc_lattice*)e->L)->kin_mode = kin_mode; }
 const float* orc_engine_oct(const orc_engine* e) { return e->L->oct; }
 size_t orc_sizeof(int which) {
     switch (which) {

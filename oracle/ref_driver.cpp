@@ -170,6 +170,21 @@ void ref_map_k_to_vg(int mode, int n, const double* k, double* out) {
     }
 }
 
+// map_k_to_vg with KinematicsMode::Exact (Cardano, Jacobi near degeneracies) for n
+// directions: out = n*4 (vx, vy, vz, |v|).
+void ref_map_k_to_vg_exact(int mode, int n, const double* k, double* out) {
+    static CrystalLattice lat(materials::Germanium());
+    lat.set_kinematics_mode(CrystalLattice::KinematicsMode::Exact);
+    for (int i = 0; i < n; ++i) {
+        const auto kin = lat.map_k_to_vg(static_cast<PhononMode>(mode), Vec3{k[3 * i], k[3 * i + 1], k[3 * i + 2]});
+        out[4 * i] = kin.vg_dir.x; out[4 * i + 1] = kin.vg_dir.y; out[4 * i + 2] = kin.vg_dir.z; out[4 * i + 3] = kin.vg_mag;
+    }
+}
+
+// Kinematics mode of the lattices that ref_run_events builds (0 Octahedral, 1 Exact).
+static int g_kin_mode = 0;
+void ref_set_kinematics_mode(int mode) { g_kin_mode = mode; }
+
 // physics::solve_christoffel_exact for n directions: out = n*12 (3 x (vgx, vgy, vgz, vp)).
 void ref_christoffel(int n, const double* k, double* out) {
     const CrystalConfig cfg = materials::Germanium();
@@ -224,7 +239,8 @@ long ref_run_events(const orc_crystal* crystal, const orc_detector* det, const o
                     void* stats_out, orc_hit* hits, long hit_cap, uint64_t* hit_offsets,
                     double* wall_seconds, char* err, int err_len) {
     try {
-        const CrystalLattice lattice(to_crystal(crystal));
+        CrystalLattice lattice(to_crystal(crystal));
+        lattice.set_kinematics_mode(static_cast<CrystalLattice::KinematicsMode>(g_kin_mode));
         const DetectorModel detector = to_detector(det);
         const SimulationEngine engine(lattice, detector);
         const ChargePhysics charges(lattice);

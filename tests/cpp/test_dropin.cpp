@@ -125,6 +125,30 @@ int main(int argc, char** argv) {
         check("transport(): deposited + escaped/killed = injected", S.total_deposited_eV + S.escaped_or_killed_eV, Ein, 1e-9);
         check("transport(): stack consumed", static_cast<double>(st.size()), 0.0, 0.0);
     }
+    {   // KinematicsMode::Exact: per-call Christoffel solve on the host and in the kernels
+        CrystalLattice lx(materials::Germanium());
+        lx.set_kinematics_mode(CrystalLattice::KinematicsMode::Exact);
+        DetectorModel dx;
+        const SimulationEngine ex(lx, dx);
+        const auto a = lx.map_k_to_vg(PhononMode::L, Vec3{0.3, -0.5, 0.8}.unit());
+        const auto b = lx.map_k_to_vg_oct(PhononMode::L, Vec3{0.3, -0.5, 0.8}.unit());
+        check("exact vs octahedral v_g magnitude (LUT error < 1e-3)", a.vg_mag, b.vg_mag, 1e-3);
+        auto gen = [&](EventInput& in) {
+            in.phonons.push_back(ex.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::L, 7.5e-3));
+            in.injected_eV += 7.5e-3;
+        };
+        RunOptions o;
+        o.seed = 9;
+        const RunResult r = run_events(ex, 300, gen, o);
+        check("exact kinematics: deposited + escaped = injected", r.stats.total_deposited_eV + r.stats.escaped_or_killed_eV,
+              r.stats.total_injected_eV, 1e-9);
+        o.form = CMPMC_FORM_BATCHED;
+        const RunResult rb = run_events(ex, 300, gen, o);
+        check("exact kinematics, batched: deposited + escaped = injected",
+              rb.stats.total_deposited_eV + rb.stats.escaped_or_killed_eV, rb.stats.total_injected_eV, 1e-9);
+        check("exact kinematics: batched steps vs plain (5%)", static_cast<double>(rb.stats.total_steps),
+              static_cast<double>(r.stats.total_steps), 0.05);
+    }
     {   // a worker exception reaches the caller (guarded Matrix IV model)
         CrystalConfig cfg = materials::Germanium();
         cfg.iv_model = IVRateModel::Matrix;

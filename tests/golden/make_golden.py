@@ -48,6 +48,18 @@ def main():
         R.ref_map_k_to_vg(m, 512, O.dptr(np.ascontiguousarray(k)), O.dptr(tmp))
         vg[m] = tmp
     out["map_vg"] = vg
+    # 3b. KinematicsMode::Exact (Cardano with the Jacobi fallback): the random directions
+    # plus crystal axes and directions next to the FT/ST degeneracy lines
+    sp = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, 1, 1], [1, -1, 1], [1e-7, 0, 1],
+                   [1, 1e-6, 1e-6], [0.6, 0.8, 0], [1, 1, 1.0001], [1, 1e-3, 0], [1, 1, 1e-3]], dtype=float)
+    ke = np.vstack([sp / np.linalg.norm(sp, axis=1)[:, None], k])
+    out["map_exact_k"] = ke
+    vge = np.zeros((3, len(ke), 4))
+    for m in range(3):
+        tmp = np.zeros((len(ke), 4))
+        R.ref_map_k_to_vg_exact(m, len(ke), O.dptr(np.ascontiguousarray(ke)), O.dptr(tmp))
+        vge[m] = tmp
+    out["map_exact_vg"] = vge
     # 4. decay energy-fraction draws from Rng(99)
     lt, tt = np.zeros(1000), np.zeros(1000)
     R.ref_decay_fractions(99, 1000, O.dptr(lt), O.dptr(tt))
@@ -83,8 +95,11 @@ def main():
     dd = O.default_detector()
     O.demo_polynomial_surfaces(dd)
     runs["bench_demo"] = (O.workload(O.WL_PHONON_POINT, mode=2, energy_eV=7.5e-3), 100, 12345, dd)
+    runs["c1_exact"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector())
     for tag, (wl, n, seed, d) in runs.items():
+        R.ref_set_kinematics_mode(1 if tag.endswith("_exact") else 0)
         r = O.ref_run(wl, n, seed, detector=d, threads=4)
+        R.ref_set_kinematics_mode(0)
         out[f"run_{tag}_stats"] = np.frombuffer(r["stats_raw"], dtype=np.uint8)
         out[f"run_{tag}_offsets"] = r["hit_offsets"]
         h = r["hits"][:600]

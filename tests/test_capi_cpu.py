@@ -111,3 +111,17 @@ def test_model_without_device_reports_it(lib_cmpmc):
     with pytest.raises(cm.CmpmcError) as e:
         cm.Engine()
     assert e.value.code == cm.capi.ERR_NO_DEVICE
+
+
+def test_host_exact_kinematics_bit_exact(lib_cmpmc):
+    """cmpmc_map_k_to_vg with KinematicsMode::Exact (crystal_lattice.hpp:77-85) on the
+    host equals the reference build's values (golden, bitwise), degenerate axes included."""
+    import os
+    cm = lib_cmpmc
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz"))
+    lat = cm.Lattice()
+    lat.set_kinematics_mode(lat.KINEMATICS_EXACT)
+    k = gold["map_exact_k"]
+    for m in range(3):
+        d, mag = lat.map_k_to_vg(m, k)
+        assert np.array_equal(np.c_[d, mag], gold["map_exact_vg"][m])

@@ -68,6 +68,45 @@ def test_map_k_to_vg_golden(oracle, gold):
         assert np.array_equal(eng.map_k_to_vg(m, gold["map_k"]), gold["map_vg"][m])
 
 
+def test_map_k_to_vg_exact_golden(oracle, gold):
+    """KinematicsMode::Exact (crystal_lattice.hpp:77-85): Cardano roots with the Jacobi
+    fallback near FT/ST degeneracies, bit-equal to the reference on axes, near-degenerate
+    and random directions."""
+    eng = oracle.Engine()
+    for m in range(3):
+        assert np.array_equal(eng.map_k_to_vg(m, gold["map_exact_k"], exact=True), gold["map_exact_vg"][m])
+
+
+def test_cardano_equals_jacobi(oracle):
+    """tests/test_christoffel_cardano.cpp:52-53: the closed-form solve agrees with the
+    Jacobi sweeps (group velocities and phase speeds) to 1e-9 relative."""
+    O = oracle
+    L = O.lib()
+    c = O.germanium()
+    vm = np.zeros((6, 6))                       # cubic Voigt matrix
+    vm[:3, :3] = c.C12
+    vm[[0, 1, 2], [0, 1, 2]] = c.C11
+    vm[[3, 4, 5], [3, 4, 5]] = c.C44
+    voigt = [[0, 5, 4], [5, 1, 3], [4, 3, 2]]
+    Ct = np.zeros((3, 3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for l in range(3):
+                for m in range(3):
+                    Ct[i, j, l, m] = vm[voigt[i][j], voigt[l][m]]
+    Ct = np.ascontiguousarray(Ct)
+    Cp = Ct.ctypes.data
+    rng = np.random.default_rng(5)
+    for _ in range(500):
+        k = rng.normal(size=3)
+        k /= np.linalg.norm(k)
+        a_vg, a_vp, b_vg, b_vp = np.zeros(9), np.zeros(3), np.zeros(9), np.zeros(3)
+        L.orc_solve_christoffel_cardano(Cp, c.density, O.dptr(k), O.dptr(a_vg), O.dptr(a_vp))
+        L.orc_solve_christoffel(Cp, c.density, O.dptr(k), O.dptr(b_vg), O.dptr(b_vp))
+        assert np.allclose(a_vp, b_vp, rtol=1e-9, atol=0)
+        assert np.allclose(a_vg, b_vg, rtol=1e-7, atol=1e-6 * np.abs(b_vg).max())
+
+
 def test_decay_fractions_golden(oracle, gold):
     O = oracle
     L = O.lib()
@@ -130,15 +169,20 @@ def _gold_runs(O):
         "mixed": (O.workload(O.WL_MIXED), 41, 77, det),
         "pairs": (O.workload(O.WL_PAIR), 6, 42, d3),
         "bench_demo": (O.workload(O.WL_PHONON_POINT, mode=2, energy_eV=7.5e-3), 100, 12345, dd),
+        "c1_exact": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector()),
     }
 
 
-@pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo"])
+@pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo", "c1_exact"])
 def test_run_events_golden(oracle, gold, tag):
-    """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's."""
+    """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's
+    (c1_exact: with KinematicsMode::Exact)."""
     O = oracle
     wl, n, seed, det = _gold_runs(O)[tag]
-    r = O.Engine(detector=det).run(wl, n, seed, threads=4)
+    eng = O.Engine(detector=det)
+    if tag.endswith("_exact"):
+        eng.set_kinematics(1)
+    r = eng.run(wl, n, seed, threads=4)
     assert r["stats_raw"] == gold[f"run_{tag}_stats"].tobytes()
     assert np.array_equal(r["hit_offsets"], gold[f"run_{tag}_offsets"])
     h = r["hits"][:600]
