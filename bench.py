@@ -7,8 +7,9 @@ flat faces, default Ge material and surfaces; each event is one L-mode 7.5 meV p
 started at the origin with an isotropic direction, tracked to absorption.  Inputs are
 generated on the device from the per-event seeds before the timed region.
 
-value        events/s of the batched FP32 form over the timed steps (wall clock around
-             the library call, inputs already in HBM)
+value        events/s of the batched FP32 form over the timed steps, with the launch options
+             the autotuner (cmpmc_autotune) picked on a 2e5-event sample of the workload
+             (wall clock around the library call, inputs generated on the device)
 plain_*      the plain per-event FP64 kernel on the same 1e6 events (one run)
 grad_*       the differentiated workload: 1e5 events of U{1..64} L-mode 7.5 meV phonons
              from uniform positions, theta = (Ge B, Ge A, f_diffuse 0.9, p_film 0.2);
@@ -93,7 +94,10 @@ def main():
     ap.add_argument("--cpu-events", type=int, default=100_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--waves", type=int, default=0)
+    ap.add_argument("--waves", type=int, default=0,
+                    help="register-budget variant when --no-tune is given (0: library default)")
+    ap.add_argument("--no-tune", action="store_true", help="library default launch options instead of the autotuner")
+    ap.add_argument("--tune-events", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lognormal", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -118,16 +122,29 @@ def main():
     info = cm.device_info()
     eng = cm.Engine()
     wl = cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
-    kw = dict(keep_hits=False, form=cm.FORM_BATCHED, waves=args.waves)
+    tune = None
+    if args.no_tune:
+        opts = {"waves": args.waves}
+    else:
+        t = time.perf_counter()
+        tune = eng.autotune(wl, args.tune_events, seed=args.seed + 99991)
+        tune["tuning_s"] = time.perf_counter() - t
+        opts = {k: tune[k] for k in ("block", "waves", "refill_period", "lane_depth")}
+        _log(f"autotune ({args.tune_events} events, {tune['trials']} timed runs, {tune['tuning_s']:.1f} s): {opts}, "
+             f"{tune['best_ms']:.2f} ms vs default {tune['default_ms']:.2f} ms")
+    kw = dict(keep_hits=False, form=cm.FORM_BATCHED, **opts)
+
+    def step(seed):
+        return eng.run_workload(wl, args.events, seed=seed, **kw)
 
     for w in range(args.warmup):
-        eng.run_workload(wl, args.events, seed=args.seed + 7919 * (w + 1), **kw)
+        step(args.seed + 7919 * (w + 1))
         _log(f"warmup {w} done")
 
     times, kms, steps = [], [], 0
     for k in range(args.steps):
         t = time.perf_counter()
-        r = eng.run_workload(wl, args.events, seed=args.seed + 1000 * k, **kw)
+        r = step(args.seed + 1000 * k)
         times.append(time.perf_counter() - t)
         kms.append(r["kernel_ms"])
         s = r["stats"]
@@ -152,7 +169,8 @@ def main():
                                "films on both faces, default surfaces; one L-mode 7.5 meV phonon per event from "
                                "the origin, isotropic, tracked to absorption",
                    "events_per_step": args.events, "form": "batched FP32 persistent (FP64 time and ledger)",
-                   "device": info},
+                   "options": opts, "device": info},
+        "autotune": tune,
         "phonon_steps_per_s": steps / wall,
         "plain_events_per_s": plain_value,
         "plain_phonon_steps_per_s": rp["stats"]["total_steps"] / plain_wall,
@@ -168,6 +186,7 @@ def main():
     out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                        "frac": achieved / HBM_PEAK, "traffic": traffic,
                        "kernel": "transport_batched_kernel", "kernel_ms_avg": sum(kms) / args.steps,
+
                        "algorithmic_bytes_per_launch": steps / args.steps * BYTES_PER_STEP_FP32}
     out["plain_roofline"] = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK / 1e9,
                              "achieved": rp["stats"]["total_steps"] * BYTES_PER_STEP_FP64 / (rp["kernel_ms"] / 1e3) / 1e9}

@@ -291,6 +291,30 @@ class Engine:
                                                           stream),
                          n_events, keep_hits, hit_capacity, first_branch)
 
+    def autotune(self, wl: capi.Workload, n_events: int, seed: int = 1, cache: str | None = None,
+                 force: bool = False, stream=None) -> dict:
+        """Launch options of the batched form for `wl` on this device (cmpmc_autotune):
+        coordinate descent over timed runs of n_events events, or the entry of the cache
+        file. Returns run_workload / plan keyword arguments (block, waves, refill_period,
+        lane_depth) and the timings (best_ms, default_ms, trials, from_cache)."""
+        o = self._opts(FORM_BATCHED, seed, False, False, 0, 0, 0, 0, 0, False, 0)
+        r = capi.TuneResult()
+        check(capi.load().cmpmc_autotune(self.ptr, C.byref(wl), n_events, seed,
+                                         cache.encode() if cache else None, capi.TUNE_FORCE if force else 0,
+                                         C.byref(o), C.byref(r), stream))
+        return {"block": o.block_size, "waves": o.waves_per_simd, "refill_period": o.refill_period,
+                "lane_depth": o.lane_stack_depth, "best_ms": r.best_ms, "default_ms": r.default_ms,
+                "trials": r.trials, "from_cache": bool(r.from_cache)}
+
+    def plan(self, wl: capi.Workload, n_events: int, keep_hits: bool = True, hit_capacity: int | None = None,
+             block: int = 0, lane_depth: int = 0, grid_blocks: int = 0, refill_period: int = 0,
+             poll_cancel: bool = False, waves: int = 0, stream=None) -> "Plan":
+        """A graph-captured batched run of n_events events of `wl` (cmpmc_plan_create);
+        Plan.run(seed, first_event) replays it."""
+        return Plan(self, wl, n_events, keep_hits, hit_capacity,
+                    self._opts(FORM_BATCHED, 1, keep_hits, False, block, 0, lane_depth, grid_blocks, refill_period,
+                               poll_cancel, waves), stream)
+
     def run_events(self, tracks: np.ndarray, track_offsets: np.ndarray, rng_state: np.ndarray,
                    injected: np.ndarray | None = None, charges: np.ndarray | None = None,
                    charge_offsets: np.ndarray | None = None, seed: int = 1, form: int = FORM_PLAIN,
@@ -415,6 +439,44 @@ class DiffTape:
     def close(self) -> None:
         if getattr(self, "ptr", None):
             capi.load().cmpmc_diff_tape_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    """A batched run of a fixed device workload recorded as a HIP graph (cmpmc_plan_*).
+    run(seed, first_event) replays it and returns what Engine.run_workload returns for
+    those events and seed; buffers that prove too small are enlarged and the graph
+    recorded again (`recordings`)."""
+
+    def __init__(self, engine: Engine, wl: capi.Workload, n_events: int, keep_hits: bool,
+                 hit_capacity: int | None, opts: capi.RunOpts, stream=None):
+        self.engine = engine          # the model outlives its plans
+        self.n_events = n_events
+        self.keep_hits = keep_hits
+        self.hit_capacity = hit_capacity if hit_capacity is not None else (max(1024, 64 * n_events) if keep_hits else 0)
+        p = C.c_void_p()
+        check(capi.load().cmpmc_plan_create(engine.ptr, C.byref(wl), n_events, C.byref(opts),
+                                            self.hit_capacity if keep_hits else 0, stream, C.byref(p)))
+        self.ptr = p
+
+    @property
+    def recordings(self) -> int:
+        return int(capi.load().cmpmc_plan_recordings(self.ptr))
+
+    def run(self, seed: int, first_event: int = 0, stream=None) -> dict:
+        L = capi.load()
+        return self.engine._run(lambda out: L.cmpmc_plan_run(self.ptr, seed, first_event, C.byref(out), stream),
+                                self.n_events, self.keep_hits, self.hit_capacity, False)
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            capi.load().cmpmc_plan_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

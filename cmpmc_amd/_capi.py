@@ -128,8 +128,15 @@ class DiffParams(C.Structure):
 
 DIFF_OUT = 66
 
+
+class TuneResult(C.Structure):
+    _fields_ = [("best_ms", dbl), ("default_ms", dbl), ("trials", C.c_int32), ("from_cache", C.c_int32)]
+
+
+TUNE_FORCE = 1
+
 ABI_STRUCTS = [Crystal, Film, Surface, Detector, Lattice, PHONON_DTYPE, CHARGE_DTYPE, HIT_DTYPE, Stats,
-               EventBatch, Workload, RunOpts, RunOut, DiffParams]
+               EventBatch, Workload, RunOpts, RunOut, DiffParams, TuneResult]
 
 _lib = None
 
@@ -180,6 +187,14 @@ def load():
     L.cmpmc_diff_tape_size.argtypes = [C.c_void_p]
     L.cmpmc_diff_tape_size.restype = u64
     L.cmpmc_diff_tape_free.argtypes = [C.c_void_p]
+    L.cmpmc_plan_create.argtypes = [C.c_void_p, C.POINTER(Workload), u64, C.POINTER(RunOpts), u64, C.c_void_p,
+                                    C.POINTER(C.c_void_p)]
+    L.cmpmc_plan_run.argtypes = [C.c_void_p, u64, u64, C.POINTER(RunOut), C.c_void_p]
+    L.cmpmc_plan_recordings.argtypes = [C.c_void_p]
+    L.cmpmc_plan_recordings.restype = u64
+    L.cmpmc_plan_destroy.argtypes = [C.c_void_p]
+    L.cmpmc_autotune.argtypes = [C.c_void_p, C.POINTER(Workload), u64, u64, C.c_char_p, C.c_int, C.POINTER(RunOpts),
+                                 C.POINTER(TuneResult), C.c_void_p]
     check_abi(L)
     _lib = L
     return L
@@ -204,5 +219,6 @@ EXPORTED_SYMBOLS = [
     "cmpmc_model_destroy", "cmpmc_model_cancel", "cmpmc_model_trim", "cmpmc_model_set_kinematics",
     "cmpmc_model_map_k_to_vg", "cmpmc_transport_serial",
     "cmpmc_transport_charges_serial", "cmpmc_run_events", "cmpmc_run_workload", "cmpmc_diff_forward",
-    "cmpmc_diff_backward", "cmpmc_diff_tape_size", "cmpmc_diff_tape_free",
+    "cmpmc_diff_backward", "cmpmc_diff_tape_size", "cmpmc_diff_tape_free", "cmpmc_plan_create", "cmpmc_plan_run",
+    "cmpmc_plan_recordings", "cmpmc_plan_destroy", "cmpmc_autotune",
 ]

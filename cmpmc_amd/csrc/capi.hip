@@ -12,12 +12,13 @@
 #include <string>
 #include <vector>
 
+#include "capi_host.hpp"
 #include "capi_internal.h"
 #include "cmpmc_amd.h"
 #include "kernels_common.hpp"
 #include "transport_kernels.h"
 
-using namespace cmpmc_dev;
+using namespace cmpmc_host;
 
 // ------------------------------------------------------------------ errors
 namespace cmpmc_internal {
@@ -37,17 +38,6 @@ void clear_error() { g_err.clear(); }
 
 using cmpmc_internal::fail;
 
-#define HIPCHK(expr)                                                                                  \
-    do {                                                                                              \
-        const hipError_t e_ = (expr);                                                                 \
-        if (e_ != hipSuccess) return fail(CMPMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
-    } while (0)
-
-#define CHK(expr)                         \
-    do {                                  \
-        const int rc_ = (expr);           \
-        if (rc_ != CMPMC_OK) return rc_;  \
-    } while (0)
 
 extern "C" const char* cmpmc_last_error(void) { return cmpmc_internal::g_err.c_str(); }
 
@@ -79,6 +69,7 @@ extern "C" size_t cmpmc_abi_sizeof(int which) {
         case 11: return sizeof(cmpmc_run_opts);
         case 12: return sizeof(cmpmc_run_out);
         case 13: return sizeof(cmpmc_diff_params);
+        case 14: return sizeof(cmpmc_tune_result);
         default: return 0;
     }
 }
@@ -144,51 +135,7 @@ extern "C" int cmpmc_detector_default(cmpmc_detector* d) {
 }
 
 // ------------------------------------------------------------------ model
-namespace {
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-};
-
-int ensure(DevBuf& b, size_t bytes) {
-    if (bytes == 0) bytes = 16;
-    if (b.bytes >= bytes) return CMPMC_OK;
-    b.release();
-    const hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) {
-        b.p = nullptr;
-        return fail(CMPMC_ERR_HIP, "hipMalloc(%zu B): %s", bytes, hipGetErrorString(e));
-    }
-    b.bytes = bytes;
-    return CMPMC_OK;
-}
-
-template <class T> T* as(DevBuf& b) { return static_cast<T*>(b.p); }
-
-}  // namespace
-
-struct cmpmc_model {
-    int device = 0;
-    int n_cu = 0;
-    DevModel dm{};
-    cmpmc_detector det{};
-    cmpmc_crystal cfg{};
-    float4* d_oct = nullptr;
-    double* d_cdf = nullptr;
-    double* d_stiff = nullptr;              // C_ijlm: 81 doubles, then 81 floats
-    volatile int32_t* h_cancel = nullptr;   // host-mapped
-    int32_t* d_cancel = nullptr;
-    std::mutex mu;
-    // cached run buffers
-    DevBuf ws, pst, cst, poff, coff, pinit, cinit, rng, inj, tin, cin, tin_off, cin_off, hits, hits_tmp, keys_a, keys_b,
-        idx_a, idx_b, sorted, counts, fb, acc, queue, cqueue, lstack, clstack, qoff, wl, cub_tmp, fjobs;
-};
 
 static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc_detector* det) {
     DevModel& m = M->dm;
@@ -308,7 +255,7 @@ static void trim(cmpmc_model* M) {
     for (DevBuf* b : {&M->ws, &M->pst, &M->cst, &M->poff, &M->coff, &M->pinit, &M->cinit, &M->rng, &M->inj, &M->tin,
                       &M->cin, &M->tin_off, &M->cin_off, &M->hits, &M->hits_tmp, &M->keys_a, &M->keys_b, &M->idx_a,
                       &M->idx_b, &M->sorted, &M->counts, &M->fb, &M->acc, &M->queue, &M->cqueue, &M->lstack,
-                      &M->clstack, &M->qoff, &M->wl, &M->cub_tmp, &M->fjobs})
+                      &M->clstack, &M->qoff, &M->wl, &M->cub_tmp, &M->fjobs, &M->scan_part})
         b->release();
 }
 
@@ -378,62 +325,9 @@ extern "C" int cmpmc_model_map_k_to_vg(cmpmc_model* M, int mode, int form, uint6
     return rc;
 }
 
-// ------------------------------------------------------------------ accumulators
-namespace {
+// ------------------------------------------------------------------ runs
+namespace cmpmc_host {
 
-// Device accumulator block: counters, ledger (hi, lo) pairs, status words, hit counter,
-// Luke counter, queue head.
-struct Acc {
-    unsigned long long counters[kNumCounters];
-    unsigned long long ledger[2 * L_N];
-    unsigned long long status[4];
-    unsigned long long hit_count;
-    unsigned long long luke_count;
-    unsigned long long head;
-    unsigned long long chead;
-    unsigned long long tape_count;
-};
-
-struct Totals {
-    unsigned long long counters[kNumCounters] = {};
-    unsigned __int128 ledger[L_N] = {};
-};
-
-void add_acc(Totals& t, const Acc& a) {
-    for (int i = 0; i < kNumCounters; ++i) t.counters[i] += a.counters[i];
-    for (int i = 0; i < L_N; ++i)
-        t.ledger[i] += (static_cast<unsigned __int128>(a.ledger[2 * i]) << 20) + a.ledger[2 * i + 1];
-}
-
-double ledger_eV(unsigned __int128 v) { return static_cast<double>(static_cast<long double>(v) / 17592186044416.0L); }
-
-void apply_totals(const Totals& t, cmpmc_sim_stats& s) {
-    const unsigned long long* c = t.counters;
-    s.total_steps += c[C_STEPS];
-    s.isotope_scatters += c[C_ISO];
-    s.polycrystal_scatters += c[C_POLY];
-    s.decays_L_to_TT += c[C_TT];
-    s.decays_L_to_LT += c[C_LT];
-    s.surface_refls += c[C_REFL];
-    s.surface_downconversions += c[C_SURF_ANH];
-    s.sensor_hits += c[C_HITS];
-    s.killed_tracks += c[C_KILLED];
-    const unsigned long long* q = c + C_NCOUNTERS;
-    s.charge_steps += q[CC_STEPS];
-    s.luke_phonons_emitted += q[CC_LUKE];
-    s.intervalley_scatters += q[CC_IV];
-    s.charges_trapped += q[CC_TRAPPED];
-    s.trap_ionizations += q[CC_TRAP_ION];
-    s.charge_surface_refls += q[CC_SURF_REFL];
-    s.charges_collected += q[CC_COLLECTED];
-    s.total_luke_phonon_eV += ledger_eV(t.ledger[L_LUKE]);
-    s.charge_recomb_edep_eV += ledger_eV(t.ledger[L_RECOMB]);
-    s.total_injected_eV += ledger_eV(t.ledger[L_INJ]);
-    s.total_deposited_eV += ledger_eV(t.ledger[L_DEP]);
-    s.escaped_or_killed_eV += ledger_eV(t.ledger[L_ESC]);
-}
-
-unsigned long long fixed_point(double eV) { return static_cast<unsigned long long>(std::llrint(eV * 17592186044416.0)); }
 
 // ------------------------------------------------------------------ hit ordering kernels
 __global__ void iota_kernel(uint32_t* v, uint64_t n) {
@@ -461,11 +355,10 @@ inline unsigned nblk(uint64_t n) { return static_cast<unsigned>((n + 255) / 256)
 
 // Sorts the first n device hits by (event, track key, sequence) and writes them to the
 // host buffer; per-event counts become hit_offsets.
-int order_hits(cmpmc_model* M, uint64_t n, bool plain, uint64_t first_event, uint64_t n_events, cmpmc_sensor_hit* host,
-               uint64_t* offsets, hipStream_t s) {
+int order_hits(cmpmc_model* M, const DevHit* hits, uint64_t n, bool plain, uint64_t first_event, uint64_t n_events,
+               cmpmc_sensor_hit* host, uint64_t* offsets, hipStream_t s) {
     CHK(ensure(M->counts, (n_events + 1) * sizeof(uint32_t)));
     HIPCHK(hipMemsetAsync(M->counts.p, 0, (n_events + 1) * sizeof(uint32_t), s));
-    const DevHit* hits = as<DevHit>(M->hits);
     uint32_t* idx_final = nullptr;
     if (n > 1) {
         CHK(ensure(M->keys_a, n * 8));
@@ -508,51 +401,6 @@ int order_hits(cmpmc_model* M, uint64_t n, bool plain, uint64_t first_event, uin
     return CMPMC_OK;
 }
 
-// Everything a run needs, whatever its source (host stacks or device generator).
-struct Job {
-    uint64_t n_events = 0;
-    uint64_t first_event = 0;
-    const cmpmc_phonon_track* tracks = nullptr;
-    const uint64_t* toff = nullptr;
-    const cmpmc_charge_track* charges = nullptr;
-    const uint64_t* coff = nullptr;
-    const uint64_t* rng_in = nullptr;
-    uint64_t* rng_out = nullptr;
-    const double* injected = nullptr;
-    const cmpmc_workload* wl = nullptr;
-    bool skip_phonons = false;
-    // transport_charges: the event's remaining phonon stack goes here
-    cmpmc_phonon_track* ph_out = nullptr;
-    uint64_t ph_cap = 0;
-    uint64_t* n_ph = nullptr;
-};
-
-struct RunCfg {
-    int form = CMPMC_FORM_PLAIN;
-    bool keep_hits = true;
-    bool want_first_branch = false;
-    int block = 0;
-    uint64_t seed = 1;
-    uint64_t headroom = 0;
-    uint64_t lane_depth = 0;
-    uint32_t grid_blocks = 0;
-    uint32_t refill_period = 0;
-    int waves = 0;
-    bool poll_cancel = false;
-};
-
-struct Outputs {
-    cmpmc_sim_stats* stats;
-    cmpmc_sensor_hit* hits;       // host
-    uint64_t hit_cap;
-    uint64_t* n_hits;
-    uint64_t* hits_needed;
-    uint64_t* hit_offsets;
-    int32_t* first_branch;
-    double* kernel_ms;
-    uint64_t* stack_needed;
-    uint64_t* passes = nullptr;   // transport passes run (capacity retries add passes)
-};
 
 // ------------------------------------------------------------------ plain form
 int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s) {
@@ -576,7 +424,7 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
         } else {
             CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * sizeof(uint32_t)));
             HIPCHK(cmpmc_launch_workload_count(as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
-                                               as<uint32_t>(M->counts), s));
+                                               as<uint32_t>(M->counts), nullptr, s));
             HIPCHK(hipMemcpyAsync(nph.data(), M->counts.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
             std::fill(nch.begin(), nch.end(), 0u);
@@ -763,16 +611,12 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
         return fail(CMPMC_ERR_CAPACITY, "hit buffer too small: %llu hits produced, capacity %llu", produced,
                     static_cast<unsigned long long>(dev_hit_cap));
     const uint64_t nh = keep ? produced : 0;
-    CHK(order_hits(M, nh, true, J.first_event, N, O.hits, O.hit_offsets, s));
+    CHK(order_hits(M, as<DevHit>(M->hits), nh, true, J.first_event, N, O.hits, O.hit_offsets, s));
     if (O.n_hits) *O.n_hits = nh;
     return CMPMC_OK;
 }
 
 // ------------------------------------------------------------------ batched form
-// Register-budget variant of the persistent kernels when the options leave it open.
-constexpr int kDefaultWaves = 3;
-// Spawn-pool entries per wave of the batched phonon kernel when the options leave it open.
-constexpr uint64_t kDefaultPool = 2048;
 // which: 0 phonon kernel, 1 charge kernel, 2 phonon kernel of the differentiable form
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
@@ -784,17 +628,9 @@ int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     return per_cu * M->n_cu;
 }
 
-// Differentiable form of a batched run: the parameter-modified model, the likelihood
-// constants and the tape (owned by the caller's cmpmc_diff_tape).
-struct DiffCfg {
-    DevModel dm;
-    double k_f = 0.0, k_p = 0.0, bins_per_s = 0.0;
-    DevBuf* tape = nullptr;
-    uint64_t n_tape = 0;            // out: tape records written
-};
 
 int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s,
-                DiffCfg* D = nullptr) {
+                DiffCfg* D) {
     const DevModel* dmp = D ? &D->dm : &M->dm;
     const uint64_t N = J.n_events;
     const bool keep = C.keep_hits && O.hits != nullptr;
@@ -821,17 +657,17 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     const uint64_t clanes = static_cast<uint64_t>(cgrid) * block;
     // initial queues
     uint64_t n_init = 0, n_charges = 0;
-    std::vector<uint64_t> off;
     const bool pair = J.wl && J.wl->kind == CMPMC_WL_PAIR;
-    if (J.wl && !pair) {
+    if (J.wl && !pair) {                          // per-event counts and their offsets, on the device
         CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * 4));
-        HIPCHK(cmpmc_launch_workload_count(as<cmpmc_workload>(M->wl), C.seed, J.first_event, N, as<uint32_t>(M->counts), s));
-        std::vector<uint32_t> cnt(N);
-        HIPCHK(hipMemcpyAsync(cnt.data(), M->counts.p, N * 4, hipMemcpyDeviceToHost, s));
+        CHK(ensure(M->qoff, (N + 1) * 8));
+        CHK(ensure(M->scan_part, cmpmc_scan_partials(N) * 8));
+        HIPCHK(cmpmc_launch_workload_count(as<cmpmc_workload>(M->wl), C.seed, J.first_event, N, as<uint32_t>(M->counts),
+                                           nullptr, s));
+        HIPCHK(cmpmc_launch_scan_counts(as<uint32_t>(M->counts), N, as<uint64_t>(M->qoff), as<uint64_t>(M->scan_part),
+                                        ~0ull, nullptr, s));
+        HIPCHK(hipMemcpyAsync(&n_init, as<uint64_t>(M->qoff) + N, 8, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        off.assign(N + 1, 0);
-        for (uint64_t e = 0; e < N; ++e) off[e + 1] = off[e] + cnt[e];
-        n_init = off[N];
     } else if (!J.wl) {
         n_init = J.toff ? J.toff[N] : 0;
         n_charges = J.coff ? J.coff[N] : 0;
@@ -850,10 +686,8 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         BTrack* q = as<BTrack>(M->queue);
         unsigned long long inj_fixed = 0;
         if (J.wl && !pair) {
-            CHK(ensure(M->qoff, (N + 1) * 8));
-            HIPCHK(hipMemcpyAsync(M->qoff.p, off.data(), (N + 1) * 8, hipMemcpyHostToDevice, s));
             HIPCHK(cmpmc_launch_workload_fill(dmp, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
-                                              as<uint64_t>(M->qoff), n_init, q, d_acc->ledger, s));
+                                              as<uint64_t>(M->qoff), n_init, q, d_acc->ledger, nullptr, s));
         } else if (!J.wl && n_init) {
             CHK(ensure(M->tin, n_init * sizeof(cmpmc_phonon_track)));
             CHK(ensure(M->tin_off, (N + 1) * 8));
@@ -868,7 +702,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             CHK(ensure(M->cqueue, n_charges * sizeof(BCharge)));
             if (pair) {
                 HIPCHK(cmpmc_launch_pair_fill(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
-                                              as<BCharge>(M->cqueue), s));
+                                              as<BCharge>(M->cqueue), nullptr, s));
             } else {
                 CHK(ensure(M->cin, n_charges * sizeof(cmpmc_charge_track)));
                 CHK(ensure(M->cin_off, (N + 1) * 8));
@@ -885,7 +719,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         HIPCHK(hipEventRecord(e0, s));
         uint64_t n_luke = 0;
         if (n_charges) {
-            CHK(ensure(M->clstack, static_cast<size_t>(64) * clanes * sizeof(BCharge)));
+            CHK(ensure(M->clstack, static_cast<size_t>(kChargeDepth) * clanes * sizeof(BCharge)));
             ChargeArgs ca{};
             ca.seed = C.seed;
             ca.first_event = J.first_event;
@@ -893,7 +727,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ca.n_init = n_charges;
             ca.init_head = &d_acc->chead;
             ca.lstack = as<BCharge>(M->clstack);
-            ca.depth = 64;
+            ca.depth = kChargeDepth;
             ca.lanes = static_cast<uint32_t>(clanes);
             ca.luke_out = q + n_init;
             ca.luke_count = &d_acc->luke_count;
@@ -997,7 +831,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         return fail(CMPMC_ERR_CAPACITY, "hit buffer too small: %llu hits produced, capacity %llu", produced,
                     static_cast<unsigned long long>(dev_hit_cap));
     const uint64_t nh = keep ? produced : 0;
-    CHK(order_hits(M, nh, false, J.first_event, N, O.hits, O.hit_offsets, s));
+    CHK(order_hits(M, as<DevHit>(M->hits), nh, false, J.first_event, N, O.hits, O.hit_offsets, s));
     if (O.n_hits) *O.n_hits = nh;
     return CMPMC_OK;
 }
@@ -1031,6 +865,19 @@ int check_cfg(const RunCfg& C) {
     return CMPMC_OK;
 }
 
+int check_workload(const cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events) {
+    if (wl->kind != CMPMC_WL_PHONON_POINT && wl->kind != CMPMC_WL_PHONON_LOGN && wl->kind != CMPMC_WL_PAIR &&
+        wl->kind != CMPMC_WL_PHONON_UNIF)
+        return fail(CMPMC_ERR_INVALID, "workload: unknown kind %d", wl->kind);
+    if (wl->mode < 0 || wl->mode > 2 || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
+        (wl->kind == CMPMC_WL_PHONON_LOGN && !(wl->logn_median > 0.0)))
+        return fail(CMPMC_ERR_INVALID, "workload: invalid parameters");
+    if (wl->kind == CMPMC_WL_PAIR && M->dm.iv_model == 2)
+        return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
+    if (n_events >= (1ull << 32)) return fail(CMPMC_ERR_INVALID, "workload: at most 2^32 - 1 events per call");
+    return CMPMC_OK;
+}
+
 int validate_tracks(const cmpmc_phonon_track* t, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) {
         const cmpmc_phonon_track& x = t[i];
@@ -1052,7 +899,7 @@ int validate_charges(const cmpmc_dev::DevModel& m, const cmpmc_charge_track* t, 
     return CMPMC_OK;
 }
 
-}  // namespace
+}  // namespace cmpmc_host
 
 // ------------------------------------------------------------------ entry points
 extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, const cmpmc_run_opts* opts,
@@ -1091,15 +938,7 @@ extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, cons
 extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events, const cmpmc_run_opts* opts,
                                   cmpmc_run_out* out, void* stream) {
     if (!M || !wl || !out) return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: null argument");
-    if (wl->kind != CMPMC_WL_PHONON_POINT && wl->kind != CMPMC_WL_PHONON_LOGN && wl->kind != CMPMC_WL_PAIR &&
-        wl->kind != CMPMC_WL_PHONON_UNIF)
-        return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: unknown workload kind %d", wl->kind);
-    if (wl->mode < 0 || wl->mode > 2 || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
-        (wl->kind == CMPMC_WL_PHONON_LOGN && !(wl->logn_median > 0.0)))
-        return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: invalid workload parameters");
-    if (wl->kind == CMPMC_WL_PAIR && M->dm.iv_model == 2)
-        return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
-    if (n_events >= (1ull << 32)) return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload: at most 2^32 - 1 events per call");
+    CHK(check_workload(M, wl, n_events));
     std::lock_guard<std::mutex> lk(M->mu);
     cmpmc_internal::clear_error();
     Job J;

@@ -327,6 +327,7 @@ __device__ inline void map_k_to_vg_oct(const DevModel& m, int mode, V3<typename 
 // Every entry summed on its own (not mirrored), as the Jacobi path of the reference does:
 // D[l][i] then rounds like the reference's.
 __device__ inline void christoffel_D64(const double* C, double inv_rho, const double k[3], double D[3][3]) {
+#pragma clang fp contract(off)
     for (int i = 0; i < 3; ++i)
         for (int l = 0; l < 3; ++l) {
             double sum = 0.0;
@@ -340,6 +341,7 @@ __device__ inline void christoffel_D64(const double* C, double inv_rho, const do
 template <class T>
 __device__ inline void vg_from_eigen(const T* C, T inv_rho, const T k[3], const T ev[3], const T evec[3][3], int mode,
                                      T vg[3]) {
+#pragma clang fp contract(off)
     int long_idx = 0;
     T max_dot = T(-1);
     for (int i = 0; i < 3; ++i) {
@@ -365,6 +367,7 @@ __device__ inline void vg_from_eigen(const T* C, T inv_rho, const T k[3], const 
 
 // Cyclic Jacobi sweeps (phonon_kernels.hpp:182-247), FP64.
 __device__ inline void christoffel_jacobi64(const double* C, double rho, const double k[3], int mode, double vg[3]) {
+#pragma clang fp contract(off)
     double D[3][3];
     const double inv_rho = 1.0 / rho;
     christoffel_D64(C, inv_rho, k, D);
@@ -409,6 +412,7 @@ __device__ inline void christoffel_jacobi64(const double* C, double rho, const d
 // the eigenvectors are ill-conditioned. phonon_kernels.hpp:255-298.
 template <class T>
 __device__ inline bool christoffel_cardano(const T* C, T rho, const T k[3], int mode, T gap_tol, T vg[3]) {
+#pragma clang fp contract(off)
     T D[3][3];
     const T inv_rho = T(1) / rho;
     for (int i = 0; i < 3; ++i)
@@ -443,10 +447,15 @@ __device__ inline bool christoffel_cardano(const T* C, T rho, const T k[3], int 
         const V3<T> r0{D[0][0] - ev[n], D[0][1], D[0][2]};
         const V3<T> r1{D[1][0], D[1][1] - ev[n], D[1][2]};
         const V3<T> r2{D[2][0], D[2][1], D[2][2] - ev[n]};
-        const V3<T> c01 = cross(r0, r1), c02 = cross(r0, r2), c12 = cross(r1, r2);
-        const T n01 = norm2(c01), n02 = norm2(c02), n12 = norm2(c12);
+        // cross products and squared norms written out, so that the pragma above covers them
+        const V3<T> c01{r0.y * r1.z - r0.z * r1.y, r0.z * r1.x - r0.x * r1.z, r0.x * r1.y - r0.y * r1.x};
+        const V3<T> c02{r0.y * r2.z - r0.z * r2.y, r0.z * r2.x - r0.x * r2.z, r0.x * r2.y - r0.y * r2.x};
+        const V3<T> c12{r1.y * r2.z - r1.z * r2.y, r1.z * r2.x - r1.x * r2.z, r1.x * r2.y - r1.y * r2.x};
+        const T n01 = c01.x * c01.x + c01.y * c01.y + c01.z * c01.z;
+        const T n02 = c02.x * c02.x + c02.y * c02.y + c02.z * c02.z;
+        const T n12 = c12.x * c12.x + c12.y * c12.y + c12.z * c12.z;
         const V3<T> best = (n01 >= n02 && n01 >= n12) ? c01 : ((n02 >= n12) ? c02 : c12);
-        const T b2 = norm2(best);
+        const T b2 = best.x * best.x + best.y * best.y + best.z * best.z;
         const T inv = (b2 <= T(0)) ? T(0) : T(1) / sqrt(b2);
         evec[n][0] = (b2 <= T(0)) ? T(0) : best.x * inv;
         evec[n][1] = (b2 <= T(0)) ? T(0) : best.y * inv;
@@ -458,19 +467,38 @@ __device__ inline bool christoffel_cardano(const T* C, T rho, const T k[3], int 
 
 // CrystalLattice::map_k_to_vg_exact (crystal_lattice.hpp:77-85) in FP64, in the
 // reference's operation order: Cardano with the Jacobi fallback (gap 1e-4).
+// The solve and the rotations are evaluated without FMA contraction (the pragmas here and
+// in the helpers above), also in translation units that contract elsewhere: along exactly
+// degenerate directions ([100], [111]) the transverse polarisation is set by rounding
+// alone, so the FP32 form's FP64 fallback must round like the plain form and the
+// reference.
 __device__ __noinline__ void map_k_to_vg_exact64(const DevModel& m, int mode, const double kdet[3], double vg_dir[3],
                                                  double* vg_mag) {
-    const V3<double> kc = mat_mul_t(m.orient, V3<double>{kdet[0], kdet[1], kdet[2]});
-    const V3<double> ku = unit<Fp64>(kc);
-    const double k[3] = {ku.x, ku.y, ku.z};
+#pragma clang fp contract(off)
+    const double (*O)[3] = m.orient;
+    const double kc[3] = {O[0][0] * kdet[0] + O[1][0] * kdet[1] + O[2][0] * kdet[2],
+                          O[0][1] * kdet[0] + O[1][1] * kdet[1] + O[2][1] * kdet[2],
+                          O[0][2] * kdet[0] + O[1][2] * kdet[1] + O[2][2] * kdet[2]};
+    const double n2 = kc[0] * kc[0] + kc[1] * kc[1] + kc[2] * kc[2];
+    double k[3] = {0.0, 0.0, 1.0};
+    if (!(n2 <= 0.0)) {
+        const double inv = 1.0 / sqrt(n2);
+        k[0] = kc[0] * inv; k[1] = kc[1] * inv; k[2] = kc[2] * inv;
+    }
     double vg[3];
     if (!christoffel_cardano<double>(m.stiff, m.density, k, mode, 1e-4, vg))
         christoffel_jacobi64(m.stiff, m.density, k, mode, vg);
-    const V3<double> v{vg[0], vg[1], vg[2]};
-    const double mag = norm<Fp64>(v);
-    const V3<double> t = vdiv<Fp64>(v, tmax(mag, 1e-12));
-    const V3<double> o = unit<Fp64>(mat_mul(m.orient, t));
-    vg_dir[0] = o.x; vg_dir[1] = o.y; vg_dir[2] = o.z;
+    const double mag = sqrt(vg[0] * vg[0] + vg[1] * vg[1] + vg[2] * vg[2]);
+    const double inv_mag = 1.0 / tmax(mag, 1e-12);
+    const double t[3] = {vg[0] * inv_mag, vg[1] * inv_mag, vg[2] * inv_mag};
+    const double o[3] = {O[0][0] * t[0] + O[0][1] * t[1] + O[0][2] * t[2], O[1][0] * t[0] + O[1][1] * t[1] + O[1][2] * t[2],
+                         O[2][0] * t[0] + O[2][1] * t[1] + O[2][2] * t[2]};
+    const double o2 = o[0] * o[0] + o[1] * o[1] + o[2] * o[2];
+    vg_dir[0] = 0.0; vg_dir[1] = 0.0; vg_dir[2] = 1.0;
+    if (!(o2 <= 0.0)) {
+        const double inv = 1.0 / sqrt(o2);
+        vg_dir[0] = o[0] * inv; vg_dir[1] = o[1] * inv; vg_dir[2] = o[2] * inv;
+    }
     *vg_mag = mag;
 }
 

@@ -1,10 +1,10 @@
 // Batched form: persistent FP32 kernels over SoA-free register-resident tracks.
 //
 // Each lane owns one track at a time and steps it with FP32 kinematics (FP64 energy,
-// weight, time and ledger).  A lane whose track ends pops its own spawn stack (decay,
-// film re-emission and surface down-conversion daughters, in HBM); when that is empty
-// it takes new tracks from the global initial queue.  Refills are wave-aggregated: one
-// ballot counts the idle lanes of the wave and a single atomic claims that many queue
+// weight, time and ledger).  A lane whose track ends takes from its wave's spawn pool
+// (decay, film re-emission and surface down-conversion daughters, in HBM); when that is
+// empty it takes new tracks from the global initial queue.  Refills are wave-aggregated:
+// one ballot counts the idle lanes of the wave and a single atomic claims that many queue
 // entries, every `refill_period` steps (or at once when the whole wave is idle).
 // Every track draws from its own stream rng_for_track(event_base(seed, event), key), so
 // results do not depend on which lane runs which track, or when.
@@ -112,8 +112,9 @@ template <bool Diff, class P> struct BatchCtx {
     volatile uint32_t* top;              // its fill level (LDS, one word per wave; written by
                                          // whichever lane of the wave updates it, so every
                                          // access goes to memory)
+    volatile uint32_t* lost;             // tracks the wave's full pool turned away so far (LDS)
     uint32_t cap;
-    uint32_t need;
+    uint32_t need;                       // on overflow: pool size to ask for (cap + lost)
     bool overflow;
     uint64_t event;
     uint64_t seed;
@@ -172,21 +173,26 @@ template <bool Diff, class P> struct BatchCtx {
     }
     // Appends a complete track to the wave's pool. Callable from divergent code: the
     // lanes executing the call take consecutive slots (one LDS update by the first of
-    // them). A full pool loses the track (status reports the overflow and the host reruns
-    // with a larger pool).
+    // them). A full pool stays at cap (pops never leave the wave's region) and loses the
+    // track; status reports the overflow with cap plus the wave's lost tracks as the size
+    // to ask for, and the host reruns with a larger pool.
     __device__ void store(const PTrack<P>& t) {
         const uint64_t active = __ballot(1);
         const uint32_t rank = static_cast<uint32_t>(__popcll(active & ((1ull << __lane_id()) - 1ull)));
-        uint32_t base = 0;
+        uint32_t base = 0, lost_now = 0;
         if (rank == 0) {
             base = *top;
-            *top = base + static_cast<uint32_t>(__popcll(active));
+            const uint32_t end = base + static_cast<uint32_t>(__popcll(active));
+            *top = end < cap ? end : cap;
+            lost_now = *lost + (end > cap ? end - cap : 0u);
+            *lost = lost_now;
         }
         base = __builtin_amdgcn_readfirstlane(base);
+        lost_now = __builtin_amdgcn_readfirstlane(lost_now);
         const uint32_t slot = base + rank;
         if (slot >= cap) {
             overflow = true;
-            need = tmax(need, slot + 1);
+            need = tmax(need, cap + lost_now);
             bs_ledger(*bs, L_ESC, t.energy * t.weight);
             return;
         }
@@ -255,15 +261,21 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     __shared__ BlockStats s_stats;
     __shared__ DevModel::SurfF s_surf[3];
     __shared__ uint32_t s_top[kMaxBlock / 64];
+    __shared__ uint32_t s_lost[kMaxBlock / 64];
     for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
         s_cdf[i] = m.cdf_lt[i];
         s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
     }
     if (threadIdx.x < 3) s_surf[threadIdx.x] = m.surf_f[threadIdx.x];
-    if (threadIdx.x < kMaxBlock / 64) s_top[threadIdx.x] = 0;
+    if (threadIdx.x < kMaxBlock / 64) {
+        s_top[threadIdx.x] = 0;
+        s_lost[threadIdx.x] = 0;
+    }
     bs_init(s_stats, a.ledger);
     __syncthreads();
 
+    const uint64_t seed = a.dyn ? a.dyn->seed : a.seed;
+    const uint64_t n_init = a.n_init_dev ? tmin<uint64_t>(*a.n_init_dev, a.n_init) : a.n_init;
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t lid = __lane_id();
     const uint32_t wave = lane / 64;
@@ -273,17 +285,18 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.surfs = s_surf;
     ctx.pool = static_cast<Rec*>(a.pool) + static_cast<size_t>(wave) * a.pool_cap;
     ctx.top = &s_top[threadIdx.x / 64];
+    ctx.lost = &s_lost[threadIdx.x / 64];
     ctx.cap = a.pool_cap;
     ctx.need = 0;
     ctx.overflow = false;
     ctx.event = 0;
-    ctx.seed = a.seed;
+    ctx.seed = seed;
     ctx.hits = a.hits;
     ctx.hit_count = a.hit_count;
     ctx.hit_cap = a.hit_cap;
     ctx.keep_hits = a.keep_hits != 0;
     ctx.first_branch = a.first_branch;
-    ctx.first_event = a.first_event;
+    ctx.first_event = a.dyn ? a.dyn->first_event : a.first_event;
     ctx.rec = false;
     ctx.kws = a.kaplan_ws + static_cast<size_t>(lane) * 5 * CMPMC_KCAP;   // this lane's lists, contiguous
     ctx.lt = s_cdf;
@@ -332,8 +345,8 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
             base = __shfl(base, leader);
             if (want) {
                 const unsigned long long idx = base + __popcll(want_mask & lt_mask);
-                if (idx < a.n_init) {
-                    btrack_to_ptrack(a.init[idx], a.seed, t, ctx.event);
+                if (idx < n_init) {
+                    btrack_to_ptrack(a.init[idx], seed, t, ctx.event);
                     for (int i = 0; i < 4; ++i) ctx.sc[i] = 0.0;
                     alive = true;
                     ctx.rec = (t.key == 0);
@@ -545,6 +558,7 @@ template <int W, class P>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W)))
 transport_charges_kernel(const DevModel m, const ChargeArgs a) {
     __shared__ BlockStats s_stats;
+    const uint64_t seed = a.dyn ? a.dyn->seed : a.seed;
     bs_init(s_stats, a.ledger);
     __syncthreads();
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
@@ -572,7 +586,7 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
     for (uint32_t iter = 0;; ++iter) {
         if (!alive && ctx.sp > 0) {
             --ctx.sp;
-            bcharge_to_ctrack(ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane], a.seed, c, ctx.event);
+            bcharge_to_ctrack(ctx.ls[static_cast<size_t>(ctx.sp) * a.lanes + lane], seed, c, ctx.event);
             alive = true;
             steps = 0;
         }
@@ -586,7 +600,7 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
             if (want) {
                 const unsigned long long idx = base + __popcll(want_mask & lt_mask);
                 if (idx < a.n_init) {
-                    bcharge_to_ctrack(a.init[idx], a.seed, c, ctx.event);
+                    bcharge_to_ctrack(a.init[idx], seed, c, ctx.event);
                     alive = true;
                     steps = 0;
                 } else {
@@ -617,10 +631,12 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
 }
 
 // ------------------------------------------------------------------ generators
+// dyn (nullable) holds the seed and first event id of a graph replay.
 __global__ void workload_count_kernel(const cmpmc_workload* wl, uint64_t seed, uint64_t first_event, uint64_t n,
-                                      uint32_t* counts) {
+                                      uint32_t* counts, const DynParams* dyn) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (dyn) { seed = dyn->seed; first_event = dyn->first_event; }
     Rng r = rng_for_stream(seed, first_event + i);
     counts[i] = workload_count(*wl, r);
 }
@@ -634,16 +650,23 @@ __device__ inline uint64_t find_event(const uint64_t* offsets, uint64_t n_events
     return lo;
 }
 
-// Q: Fp64 or Fp64X, the kinematics of the model.
+// Q: Fp64 or Fp64X, the kinematics of the model. Grid-stride over the queue; with dyn the
+// number of tracks is dyn->n_init (bounded by n_tracks, the queue capacity).
 template <class Q>
 __global__ void workload_fill_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
                                      uint64_t n_events, const uint64_t* offsets, uint64_t n_tracks, BTrack* out,
-                                     unsigned long long* ledger) {
+                                     unsigned long long* ledger, const DynParams* dyn) {
     __shared__ unsigned long long s_inj;
     if (threadIdx.x == 0) s_inj = 0;
     __syncthreads();
-    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (g < n_tracks) {
+    if (dyn) {
+        seed = dyn->seed;
+        first_event = dyn->first_event;
+        n_tracks = tmin<uint64_t>(n_tracks, dyn->n_init);
+    }
+    unsigned long long inj = 0;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < n_tracks; g += stride) {
         const uint64_t e = find_event(offsets, n_events, g);
         const uint64_t j = g - offsets[e];
         const uint64_t ev = first_event + e;
@@ -656,8 +679,9 @@ __global__ void workload_fill_kernel(const DevModel m, const cmpmc_workload* wl,
         workload_phonon<Q>(m, *wl, r, t, E);
         t.key = j;
         ptrack_to_btrack(t, ev, out[g]);
-        atomicAdd(&s_inj, static_cast<unsigned long long>(__double2ll_rn(E * kLedgerScale)));
+        inj += static_cast<unsigned long long>(__double2ll_rn(E * kLedgerScale));
     }
+    if (inj) atomicAdd(&s_inj, inj);
     __syncthreads();
     if (threadIdx.x == 0 && s_inj) {
         atomicAdd(&ledger[2 * L_INJ], s_inj >> 20);
@@ -666,9 +690,10 @@ __global__ void workload_fill_kernel(const DevModel m, const cmpmc_workload* wl,
 }
 
 __global__ void pair_fill_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
-                                 uint64_t n_events, BCharge* out) {
+                                 uint64_t n_events, BCharge* out, const DynParams* dyn) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_events) return;
+    if (dyn) { seed = dyn->seed; first_event = dyn->first_event; }
     const uint64_t ev = first_event + i;
     Rng r = rng_for_stream(seed, ev);
     CTrack<Fp64> e, h;
@@ -677,6 +702,86 @@ __global__ void pair_fill_kernel(const DevModel m, const cmpmc_workload* wl, uin
     h.key = 0x8000000000000001ull;
     ctrack_to_bcharge(e, ev, out[2 * i]);
     ctrack_to_bcharge(h, ev, out[2 * i + 1]);
+}
+
+// ------------------------------------------------------------------ queue offsets
+// Exclusive scan of per-event counts in three launches: tile sums, a one-block scan of
+// the tile sums, then each tile's own scan. A tile is kScanTile counts (256 threads x 8).
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 8;
+constexpr uint64_t kScanTile = kScanThreads * kScanPer;
+
+// Inclusive block-wide scan of one value per thread (Hillis-Steele in LDS); returns the
+// block total through *total.
+__device__ inline uint64_t block_scan_incl(uint64_t v, uint64_t* buf, uint64_t* total) {
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < kScanThreads; d <<= 1) {
+        const uint64_t add = (static_cast<int>(threadIdx.x) >= d) ? buf[threadIdx.x - d] : 0ull;
+        __syncthreads();
+        buf[threadIdx.x] += add;
+        __syncthreads();
+    }
+    const uint64_t r = buf[threadIdx.x];
+    *total = buf[kScanThreads - 1];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_tiles_kernel(const uint32_t* counts, uint64_t n, uint64_t* partials) {
+    __shared__ uint64_t buf[kScanThreads];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + static_cast<uint64_t>(threadIdx.x) * kScanPer;
+    uint64_t v = 0;
+    for (int k = 0; k < kScanPer; ++k)
+        if (base + k < n) v += counts[base + k];
+    uint64_t total;
+    (void)block_scan_incl(v, buf, &total);
+    if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+// One block: partials[b] becomes the exclusive offset of tile b; offsets[n] the total.
+__global__ void __launch_bounds__(kScanThreads) scan_top_kernel(uint64_t* partials, uint64_t n_tiles, uint64_t n,
+                                                               uint64_t* offsets, uint64_t cap, DynParams* dyn) {
+    __shared__ uint64_t buf[kScanThreads];
+    uint64_t carry = 0;
+    for (uint64_t c = 0; c < n_tiles; c += kScanThreads) {
+        const uint64_t i = c + threadIdx.x;
+        const uint64_t v = (i < n_tiles) ? partials[i] : 0ull;
+        uint64_t total;
+        const uint64_t incl = block_scan_incl(v, buf, &total);
+        if (i < n_tiles) partials[i] = carry + incl - v;
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        offsets[n] = carry;
+        if (dyn) {
+            dyn->needed = carry;
+            dyn->n_init = (carry <= cap) ? carry : 0ull;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads) scan_apply_kernel(const uint32_t* counts, uint64_t n,
+                                                                 const uint64_t* partials, uint64_t* offsets) {
+    __shared__ uint64_t buf[kScanThreads];
+    const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kScanTile + static_cast<uint64_t>(threadIdx.x) * kScanPer;
+    uint32_t c[kScanPer];
+    uint64_t v = 0;
+    for (int k = 0; k < kScanPer; ++k) {
+        c[k] = (base + k < n) ? counts[base + k] : 0u;
+        v += c[k];
+    }
+    uint64_t total;
+    uint64_t run = partials[blockIdx.x] + block_scan_incl(v, buf, &total) - v;
+    for (int k = 0; k < kScanPer; ++k) {
+        if (base + k < n) offsets[base + k] = run;
+        run += c[k];
+    }
+}
+
+__global__ void zero_words_kernel(unsigned long long* p, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
 }
 
 __global__ void tracks_to_queue_kernel(const cmpmc_phonon_track* in, const uint64_t* offsets, uint64_t n_events,
@@ -803,29 +908,49 @@ extern "C" hipError_t cmpmc_launch_charges(const DevModel* m, const ChargeArgs* 
     return hipLaunchKernel(charges_fn(waves, m->kin_mode == 1), dim3(grid), dim3(block), args, 0, s);
 }
 extern "C" hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
-                                                  uint64_t n, uint32_t* counts, hipStream_t s) {
+                                                  uint64_t n, uint32_t* counts, const DynParams* dyn, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(workload_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, wl, seed, first_event, n, counts);
+    hipLaunchKernelGGL(workload_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, wl, seed, first_event, n, counts,
+                       dyn);
     return hipGetLastError();
 }
 extern "C" hipError_t cmpmc_launch_workload_fill(const DevModel* m, const cmpmc_workload* wl, uint64_t seed,
                                                  uint64_t first_event, uint64_t n_events, const uint64_t* offsets,
                                                  uint64_t n_tracks, BTrack* out, unsigned long long* ledger,
-                                                 hipStream_t s) {
+                                                 const DynParams* dyn, hipStream_t s) {
     if (n_tracks == 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>(std::min<uint64_t>(blocks_for(n_tracks, 256), 1u << 16)));
     if (m->kin_mode == 1)
-        hipLaunchKernelGGL(workload_fill_kernel<Fp64X>, dim3(blocks_for(n_tracks, 256)), dim3(256), 0, s, *m, wl, seed,
-                           first_event, n_events, offsets, n_tracks, out, ledger);
+        hipLaunchKernelGGL(workload_fill_kernel<Fp64X>, grid, dim3(256), 0, s, *m, wl, seed, first_event, n_events,
+                           offsets, n_tracks, out, ledger, dyn);
     else
-        hipLaunchKernelGGL(workload_fill_kernel<Fp64>, dim3(blocks_for(n_tracks, 256)), dim3(256), 0, s, *m, wl, seed,
-                           first_event, n_events, offsets, n_tracks, out, ledger);
+        hipLaunchKernelGGL(workload_fill_kernel<Fp64>, grid, dim3(256), 0, s, *m, wl, seed, first_event, n_events,
+                           offsets, n_tracks, out, ledger, dyn);
     return hipGetLastError();
 }
 extern "C" hipError_t cmpmc_launch_pair_fill(const DevModel* m, const cmpmc_workload* wl, uint64_t seed,
-                                             uint64_t first_event, uint64_t n_events, BCharge* out, hipStream_t s) {
+                                             uint64_t first_event, uint64_t n_events, BCharge* out,
+                                             const DynParams* dyn, hipStream_t s) {
     if (n_events == 0) return hipSuccess;
     hipLaunchKernelGGL(pair_fill_kernel, dim3(blocks_for(n_events, 256)), dim3(256), 0, s, *m, wl, seed, first_event,
-                       n_events, out);
+                       n_events, out, dyn);
+    return hipGetLastError();
+}
+extern "C" uint64_t cmpmc_scan_partials(uint64_t n) { return std::max<uint64_t>((n + kScanTile - 1) / kScanTile, 1); }
+extern "C" hipError_t cmpmc_launch_scan_counts(const uint32_t* counts, uint64_t n, uint64_t* offsets, uint64_t* partials,
+                                               uint64_t cap, DynParams* dyn, hipStream_t s) {
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles) hipLaunchKernelGGL(scan_tiles_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0, s,
+                                  counts, n, partials);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kScanThreads), 0, s, partials, tiles, n, offsets, cap, dyn);
+    if (tiles) hipLaunchKernelGGL(scan_apply_kernel, dim3(static_cast<unsigned>(tiles)), dim3(kScanThreads), 0, s,
+                                  counts, n, partials, offsets);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_zero_words(unsigned long long* p, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_words_kernel, dim3(static_cast<unsigned>(std::min<uint64_t>(blocks_for(n, 256), 4096))),
+                       dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 extern "C" hipError_t cmpmc_launch_tracks_to_queue(const cmpmc_phonon_track* in, const uint64_t* offsets,

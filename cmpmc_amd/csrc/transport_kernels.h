@@ -113,9 +113,21 @@ constexpr uint32_t kNoBin = 0xFFFF;
 constexpr uint32_t kFilmCap = 128;
 constexpr uint32_t kFilmFlush = 64;
 
+// Launch parameters that graph replays change without re-recording: the kernels of a
+// captured run read them from device memory (written before each replay) instead of
+// their by-value arguments. n_init and needed are written by the queue-offset scan.
+struct DynParams {
+    uint64_t seed;
+    uint64_t first_event;
+    unsigned long long n_init;         // initial tracks to run (0 when they overflowed the queue)
+    unsigned long long needed;         // initial tracks the events asked for
+};
+
 struct BatchedArgs {
     uint64_t seed;
     uint64_t first_event;
+    const DynParams* dyn;              // nullable: seed and first_event from device memory
+    const unsigned long long* n_init_dev;   // nullable: n_init = min(*n_init_dev, n_init)
     uint64_t n_events;
     const BTrack* init;                // initial queue
     uint64_t n_init;
@@ -147,6 +159,7 @@ struct BatchedArgs {
 struct ChargeArgs {
     uint64_t seed;
     uint64_t first_event;
+    const DynParams* dyn;              // nullable: seed and first_event from device memory
     const BCharge* init;
     uint64_t n_init;
     unsigned long long* init_head;
@@ -186,15 +199,27 @@ hipError_t cmpmc_launch_tape_finish(const double* partials, int n_blocks, const 
                                     hipStream_t s);
 hipError_t cmpmc_launch_charges(const cmpmc_dev::DevModel* m, const cmpmc_dev::ChargeArgs* a, int block, int grid,
                                 int waves, hipStream_t s);
-// Device generators of the batched form: per-event counts, then the initial queue.
+// Device generators of the batched form: per-event counts, their offsets, then the
+// initial queue. dyn (nullable) overrides seed and first_event (graph replays); the
+// fill then runs the n_init tracks that the scan wrote to dyn (at most n_tracks).
 hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl_dev, uint64_t seed, uint64_t first_event,
-                                       uint64_t n_events, uint32_t* counts, hipStream_t s);
+                                       uint64_t n_events, uint32_t* counts, const cmpmc_dev::DynParams* dyn,
+                                       hipStream_t s);
 hipError_t cmpmc_launch_workload_fill(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
                                       uint64_t first_event, uint64_t n_events, const uint64_t* offsets,
                                       uint64_t n_tracks, cmpmc_dev::BTrack* out, unsigned long long* ledger,
-                                      hipStream_t s);
+                                      const cmpmc_dev::DynParams* dyn, hipStream_t s);
 hipError_t cmpmc_launch_pair_fill(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
-                                  uint64_t first_event, uint64_t n_events, cmpmc_dev::BCharge* out, hipStream_t s);
+                                  uint64_t first_event, uint64_t n_events, cmpmc_dev::BCharge* out,
+                                  const cmpmc_dev::DynParams* dyn, hipStream_t s);
+// Exclusive scan of n per-event counts: offsets[0..n] (offsets[n] = total), in three
+// launches over blocks of kScanTile counts; partials holds scan_partials(n) words.
+// dyn->needed = total; dyn->n_init = total when it fits in cap, else 0.
+uint64_t cmpmc_scan_partials(uint64_t n);
+hipError_t cmpmc_launch_scan_counts(const uint32_t* counts, uint64_t n, uint64_t* offsets, uint64_t* partials,
+                                    uint64_t cap, cmpmc_dev::DynParams* dyn, hipStream_t s);
+// Sets n 64-bit words to zero with a kernel (no memset node when the stream is captured).
+hipError_t cmpmc_launch_zero_words(unsigned long long* p, uint64_t n, hipStream_t s);
 // map_k_to_vg of n directions k[3n] with the model's kinematics, FP64 (fp32 = 0) or FP32
 // arithmetic: out[4n] = (vg_dir, vg_mag).
 hipError_t cmpmc_launch_map_query(const cmpmc_dev::DevModel* m, int fp32, int mode, uint64_t n, const double* k,

@@ -243,7 +243,7 @@ int cmpmc_device_info(char* buf, size_t len);
 
 /* sizeof of the ABI structs, for binding checks: 0 crystal, 1 film, 2 surface,
  * 3 detector, 4 lattice, 5 phonon track, 6 charge track, 7 hit, 8 stats,
- * 9 event batch, 10 workload, 11 run opts, 12 run out, 13 diff params. Returns 0 for
+ * 9 event batch, 10 workload, 11 run opts, 12 run out, 13 diff params, 14 tune result. Returns 0 for
  * other values. */
 size_t cmpmc_abi_sizeof(int which);
 
@@ -325,6 +325,56 @@ int cmpmc_run_events(cmpmc_model* m, const cmpmc_event_batch* batch, const cmpmc
  * event i drawing from Rng::for_stream(opts->seed, i) (rng.hpp:34-36). */
 int cmpmc_run_workload(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events,
                        const cmpmc_run_opts* opts, cmpmc_run_out* out, void* stream);
+
+/* ---------------------------------------------------------------- graph-captured plans
+ * No counterpart in the reference (its run_events re-plans every call, parallel.hpp:76-148):
+ * a batched-form cmpmc_run_workload of a fixed workload and event count, recorded once
+ * as a HIP graph (zeroing of the accumulators, the device generator, the queue-offset
+ * scan, the queue fill and the persistent transport kernels) and replayed by each
+ * cmpmc_plan_run with a new seed and first event id, which the kernels read from device
+ * memory. A replay gives exactly what cmpmc_run_workload gives for the same seed and
+ * events (with first event id f: events f .. f + n_events - 1 of that seed, as in one
+ * larger run), whatever the options. Buffers belong to the plan (sized at creation: the initial queue
+ * from a first count with seed 1 plus 1/16, Luke queue, spawn pools, hits); a replay
+ * that outgrows one of them enlarges it, records the graph again and runs again (counted
+ * in out->passes). The model must outlive its plans; a plan keeps the model's
+ * kinematics mode of its creation. Replays of one plan are serialised. */
+typedef struct cmpmc_plan cmpmc_plan;
+
+/* opts->form is ignored (batched); hit_capacity: device hit buffer (0 with keep_hits
+ * unset). `stream`: where the first count runs (NULL = default stream). */
+int cmpmc_plan_create(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events, const cmpmc_run_opts* opts,
+                      uint64_t hit_capacity, void* stream, cmpmc_plan** out);
+/* Replays the plan on `stream` (NULL = default stream) and waits for it. out as for
+ * cmpmc_run_workload (stats accumulated; hits ordered by event when kept). */
+int cmpmc_plan_run(cmpmc_plan* plan, uint64_t seed, uint64_t first_event_id, cmpmc_run_out* out, void* stream);
+/* Graph recordings of the plan so far (1 after creation; +1 per enlargement). */
+uint64_t cmpmc_plan_recordings(const cmpmc_plan* plan);
+int cmpmc_plan_destroy(cmpmc_plan* plan);
+
+/* ---------------------------------------------------------------- autotuner
+ * No counterpart in the reference. Picks the batched form's launch options for a
+ * workload on this device: register-budget variant (waves_per_simd 2, 3, 4), block size
+ * (64, 128, 256), refill (compaction) period of dead lanes (1, 2, 4, 8) and spawn-pool
+ * depth (1024, 2048, 4096 entries per wavefront), by coordinate descent from the
+ * defaults over timed runs of n_events events with `seed` (a change is kept when it is
+ * at least 1 % faster). Every candidate runs the same trajectories (per-track random
+ * sub-streams make results independent of the launch shape), so the options change
+ * speed, never results. With cache_path set, the choice is stored there as one JSON
+ * line per key (device, CU count, workload, kinematics mode, Luke fraction, event count
+ * rounded to a power of two) and read back by later calls instead of tuning again,
+ * unless CMPMC_TUNE_FORCE is set. opts (in/out): the base options; the tuned fields are
+ * overwritten and form is set to CMPMC_FORM_BATCHED. */
+enum { CMPMC_TUNE_FORCE = 1 };
+typedef struct {
+    double best_ms;               /* kernel time of the chosen options */
+    double default_ms;            /* kernel time of the defaults */
+    int32_t trials;               /* timed runs (0 when read from the cache) */
+    int32_t from_cache;
+} cmpmc_tune_result;
+int cmpmc_autotune(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events, uint64_t seed,
+                   const char* cache_path, int flags, cmpmc_run_opts* opts, cmpmc_tune_result* result,
+                   void* stream);
 
 /* ---------------------------------------------------------------- differentiable form
  * No counterpart in the reference (it has no gradients): the batched FP32 transport of a
