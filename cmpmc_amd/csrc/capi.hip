@@ -676,6 +676,15 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     }
     const double lf = std::min(std::max(M->dm.luke_fraction, 0.0), 1.0);
     uint64_t luke_cap = (n_charges && lf > 0.0) ? n_charges * 4096 : 0;
+    if (D && J.wl && !pair) {
+        // room for the hits of the initial tracks before the first pass, so that a tape
+        // overflow (and a second transport pass) stays the exception: a phonon of energy E
+        // ends in about E / 0.3 meV film hits (mean hit energy at the default films), with
+        // half again as margin
+        const double e_max = (J.wl->kind == CMPMC_WL_PHONON_LOGN) ? J.wl->e_hi_eV : J.wl->energy_eV;
+        const uint64_t per_track = static_cast<uint64_t>(std::ceil(1.5 * std::max(e_max, 0.0) / 3.0e-4)) + 8;
+        tape_cap = std::max<uint64_t>(tape_cap, n_init * per_track);
+    }
     float ms_total = 0.0f;
     int rc = CMPMC_OK;
     for (int attempt = 0;; ++attempt) {
