@@ -15,6 +15,7 @@ Every compute call goes to the HIP library; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import time
 
 import numpy as np
@@ -254,6 +255,26 @@ class Engine:
         o.waves_per_simd = waves
         return o
 
+    def hit_estimate(self, wl: capi.Workload, n_events: int) -> int:
+        """A generous number of sensor hits for n_events of `wl` (the default hit
+        capacity, so that a run rarely has to be repeated for room): a phonon of energy
+        E ends in about E / 0.3 meV film hits at the default films; a pair yields its two
+        collected carriers and, when Luke phonons are tracked, about 4 eV of them. Capped
+        at 2^27 hits (7.5 GB of host records): a larger run that keeps its hits sizes the
+        buffer exactly on a second pass."""
+        e = 0.5 * (wl.e_lo_eV + wl.e_hi_eV) if wl.kind == WL_PHONON_LOGN else wl.energy_eV
+        per_phonon = 1.5 * max(e, 0.0) / 3.0e-4 + 8
+        if wl.kind == WL_PHONON_POINT:
+            n = per_phonon
+        elif wl.kind == WL_PHONON_UNIF:
+            n = per_phonon * 0.5 * (wl.n_min + wl.n_max)
+        elif wl.kind == WL_PHONON_LOGN:
+            mean = wl.logn_median * math.exp(0.5 * wl.logn_sigma ** 2)
+            n = per_phonon * min(mean, float(wl.n_max))
+        else:
+            n = 4 + (1.5 * 4.0 / 3.0e-4 * min(max(self.detector.luke_sample_fraction, 0.0), 1.0))
+        return int(min(max(1024, n * n_events), 1 << 27))
+
     def _run(self, call, n_events: int, keep_hits: bool, hit_capacity: int | None, first_branch: bool) -> dict:
         cap = hit_capacity if hit_capacity is not None else (max(1024, 64 * n_events) if keep_hits else 0)
         for _ in range(3):
@@ -287,6 +308,8 @@ class Engine:
         o = self._opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
                        poll_cancel, waves)
         L = capi.load()
+        if keep_hits and hit_capacity is None:
+            hit_capacity = self.hit_estimate(wl, n_events)
         return self._run(lambda out: L.cmpmc_run_workload(self.ptr, C.byref(wl), n_events, C.byref(o), C.byref(out),
                                                           stream),
                          n_events, keep_hits, hit_capacity, first_branch)
