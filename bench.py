@@ -7,9 +7,9 @@ flat faces, default Ge material and surfaces; each event is one L-mode 7.5 meV p
 started at the origin with an isotropic direction, tracked to absorption.  Inputs are
 generated on the device from the per-event seeds before the timed region.
 
-value        events/s of the batched FP32 form over the timed steps, with the launch options
-             the autotuner (cmpmc_autotune) picked on a 2e5-event sample of the workload
-             (wall clock around the library call, inputs generated on the device)
+value        events/s of the batched FP32 form over the timed steps, with the library's
+             default launch options (which the autotuner, --tune, also picked for this
+             workload) (wall clock around the library call, inputs generated on the device)
 plain_*      the plain per-event FP64 kernel on the same 1e6 events (one run)
 grad_*       the differentiated workload: 1e5 events of U{1..64} L-mode 7.5 meV phonons
              from uniform positions, theta = (Ge B, Ge A, f_diffuse 0.9, p_film 0.2);
@@ -94,9 +94,8 @@ def main():
     ap.add_argument("--cpu-events", type=int, default=100_000)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=42)
-    ap.add_argument("--waves", type=int, default=0,
-                    help="register-budget variant when --no-tune is given (0: library default)")
-    ap.add_argument("--no-tune", action="store_true", help="library default launch options instead of the autotuner")
+    ap.add_argument("--waves", type=int, default=0, help="register-budget variant (0: library default)")
+    ap.add_argument("--tune", action="store_true", help="launch options from the autotuner instead of the defaults")
     ap.add_argument("--tune-events", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lognormal", action="store_true")
@@ -123,7 +122,7 @@ def main():
     eng = cm.Engine()
     wl = cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
     tune = None
-    if args.no_tune:
+    if not args.tune:
         opts = {"waves": args.waves}
     else:
         t = time.perf_counter()
