@@ -171,6 +171,9 @@ struct Fp64 {
     __device__ static double hypot_(double x, double y) { return hypot(x, y); }
     __device__ static double sin_half_pi(double u) { return sin(kHalfPi * u); }
     __device__ static double inf() { return __builtin_huge_val(); }
+    __device__ static double div(double a, double b) { return a / b; }
+    __device__ static double rcp(double x) { return 1.0 / x; }
+    __device__ static double rsqrt(double x) { return 1.0 / sqrt(x); }
 };
 
 struct Fp32 {
@@ -185,16 +188,20 @@ struct Fp32 {
         s = __builtin_amdgcn_sinf(u);
         c = __builtin_amdgcn_cosf(u);
     }
-    __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+    __device__ static float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }   // v_sqrt_f32
     __device__ static float cos_(float x) { return __builtin_amdgcn_cosf(x * 0.15915494f); }
     __device__ static float sin_(float x) { return __builtin_amdgcn_sinf(x * 0.15915494f); }
     __device__ static float acos_(float x) { return acosf(x); }
     __device__ static float exp_(float x) { return __expf(x); }
     __device__ static float log_(float x) { return 0.69314718f * __builtin_amdgcn_logf(x); }
     __device__ static float cbrt_(float x) { return cbrtf(x); }
-    __device__ static float hypot_(float x, float y) { return __builtin_sqrtf(x * x + y * y); }
+    __device__ static float hypot_(float x, float y) { return __builtin_amdgcn_sqrtf(x * x + y * y); }
     __device__ static float sin_half_pi(float u) { return __builtin_amdgcn_sinf(0.25f * u); }
     __device__ static float inf() { return __builtin_huge_valf(); }
+    // hardware reciprocal (1 ulp) instead of the IEEE division sequence
+    __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 };
 
 // The same arithmetic with KinematicsMode::Exact (per-call Christoffel solve instead of
@@ -227,14 +234,14 @@ template <class R> __device__ inline V3<R> cross(V3<R> a, V3<R> r) {
 template <class P> __device__ inline typename P::R norm(V3<typename P::R> a) { return P::sqrt_(norm2(a)); }
 template <class P> __device__ inline V3<typename P::R> vdiv(V3<typename P::R> a, typename P::R s) {
     using R = typename P::R;
-    const R inv = R(1) / s;
+    const R inv = P::rcp(s);
     return {a.x * inv, a.y * inv, a.z * inv};
 }
 template <class P> __device__ inline V3<typename P::R> unit(V3<typename P::R> a) {   // math.hpp:146-151
     using R = typename P::R;
     const R n2 = norm2(a);
     if (n2 <= R(0)) return {R(0), R(0), R(1)};
-    const R inv = R(1) / P::sqrt_(n2);
+    const R inv = P::rsqrt(n2);
     return {a.x * inv, a.y * inv, a.z * inv};
 }
 template <class R> __device__ inline V3<R> rotate_around(V3<R> v, V3<R> k, R c, R s) {   // math.hpp:154-156
@@ -289,7 +296,7 @@ __device__ inline void map_k_to_vg_oct(const DevModel& m, int mode, V3<typename 
     using R = typename P::R;
     const V3<R> k = mat_mul_t(m.orient, kdet);
     const R l1 = fabs(k.x) + fabs(k.y) + fabs(k.z);
-    const R inv_l1 = (l1 > R(1e-18)) ? (R(1) / l1) : R(1);
+    const R inv_l1 = (l1 > R(1e-18)) ? P::rcp(l1) : R(1);
     R px = k.x * inv_l1, py = k.y * inv_l1;
     if (k.z < R(0)) {
         const R ox = (R(1) - fabs(py)) * (px >= R(0) ? R(1) : R(-1));
@@ -574,19 +581,19 @@ template <class P> __device__ inline double sample_fraction(const double* cdf, R
 template <class P> __device__ inline typename P::R theta_L_to_LT(const DevModel& m, typename P::R x) {
     using R = typename P::R;
     const R mm = R(m.v_sound / m.v_trans);
-    const R cth = (R(1) + (x * x) * (R(1) - mm * mm) + R(2) * x * mm * mm - mm * mm) / (R(2) * x);
+    const R cth = P::div(R(1) + (x * x) * (R(1) - mm * mm) + R(2) * x * mm * mm - mm * mm, R(2) * x);
     return P::acos_(tclamp(cth, R(-1), R(1)));
 }
 template <class P> __device__ inline typename P::R theta_T_from_LT(const DevModel& m, typename P::R x) {
     using R = typename P::R;
     const R mm = R(m.v_sound / m.v_trans);
-    const R cth = (R(1) - x * x + mm * mm * (R(1) - x) * (R(1) - x)) / (R(2) * mm * (R(1) - x));
+    const R cth = P::div(R(1) - x * x + mm * mm * (R(1) - x) * (R(1) - x), R(2) * mm * (R(1) - x));
     return P::acos_(tclamp(cth, R(-1), R(1)));
 }
 template <class P> __device__ inline typename P::R theta_TT(const DevModel& m, typename P::R x) {
     using R = typename P::R;
     const R mm = R(m.v_sound / m.v_trans);
-    const R cth = (R(1) - mm * mm * (R(1) - x) * (R(1) - x) + mm * mm * x * x) / (R(2) * mm * x);
+    const R cth = P::div(R(1) - mm * mm * (R(1) - x) * (R(1) - x) + mm * mm * x * x, R(2) * mm * x);
     return P::acos_(tclamp(cth, R(-1), R(1)));
 }
 
@@ -599,10 +606,10 @@ __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P:
     int surf = 2;
     const R hz = R(m.half_z);
     if (v.z > R(1e-15)) {
-        const R tz = (hz - p.z) / v.z;
+        const R tz = P::div(hz - p.z, v.z);
         if (tz >= R(0) && tz < t_min) { t_min = tz; surf = 0; }
     } else if (v.z < R(-1e-15)) {
-        const R tz = (-hz - p.z) / v.z;
+        const R tz = P::div(-hz - p.z, v.z);
         if (tz >= R(0) && tz < t_min) { t_min = tz; surf = 1; }
     }
     const R a = v.x * v.x + v.y * v.y;
@@ -611,7 +618,7 @@ __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P:
         const R rad = R(m.radius);
         const R c = p.x * p.x + p.y * p.y - rad * rad;
         const R disc = tmax(R(0), b * b - a * c);
-        const R tr = (-b + P::sqrt_(disc)) / a;
+        const R tr = P::div(-b + P::sqrt_(disc), a);
         if (tr >= R(0) && tr < t_min) { t_min = tr; surf = 2; }
     }
     t_min = tmax(R(0), t_min);
@@ -620,7 +627,7 @@ __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P:
     else {
         const R hx = p.x + v.x * t_min, hy = p.y + v.y * t_min;
         const R r = P::hypot_(hx, hy);
-        if (r > R(1e-15)) n_in = {-hx / r, -hy / r, R(0)};
+        if (r > R(1e-15)) n_in = {P::div(-hx, r), P::div(-hy, r), R(0)};
         else n_in = {R(-1), R(0), R(0)};
     }
     surface = surf;

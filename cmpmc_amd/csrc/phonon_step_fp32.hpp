@@ -55,7 +55,7 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
     const float rate_poly = trk.vg_mag * m.poly_rate_f;
     const float rate_down = (trk.mode != 0) ? 0.0f : m.c_anh_f * (e2 * e2 * e);
     const float rate_tot = rate_iso + rate_poly + rate_down;
-    const float d_phys = (rate_tot > 1e-12f) ? (P::neg_log(P::uniform(r)) * (trk.vg_mag / rate_tot)) : P::inf();
+    const float d_phys = (rate_tot > 1e-12f) ? (P::neg_log(P::uniform(r)) * P::div(trk.vg_mag, rate_tot)) : P::inf();
     V3<float> n_in;
     int sid;
     const float dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
@@ -64,7 +64,7 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
     V3<float> knew;
     bool reflect = false, specular = false;
     if (d_phys < dist) {                          // bulk event
-        const float tau = d_phys / trk.vg_mag;
+        const float tau = P::div(d_phys, trk.vg_mag);
         trk.pos = trk.pos + trk.vg * d_phys;
         trk.time += static_cast<double>(tau);
         ctx.flight(rate_iso, rate_down, tau);
@@ -80,7 +80,7 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
         mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
         knew = isotropic_dir<P>(r);
     } else {                                      // boundary
-        const float tau = dist / trk.vg_mag;
+        const float tau = P::div(dist, trk.vg_mag);
         trk.pos = trk.pos + trk.vg * dist;
         trk.time += static_cast<double>(tau);
         ctx.flight(rate_iso, rate_down, tau);
