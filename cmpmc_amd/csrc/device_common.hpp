@@ -37,6 +37,10 @@ struct DevFilm {
         high_qp_limit, subgap_absorption, absorber_gap, absorber_eff, absorber_eff_slope,
         temperature;
     int32_t keep_all_phonons, _pad;
+    // FP32 constants of the batched form's cascade (fill_fp32_constants): 1 / gap, the
+    // escape exponent per unit path fraction thickness / (v_sound tau0), the absorber
+    // efficiency and its slope
+    float inv_gap_f, esc_c_f, eff_f, eff_slope_f, slope_f, _pad_f;
 };
 
 struct DevSurface {
@@ -675,7 +679,14 @@ inline void fill_fp32_constants(DevModel& m) {
     m.half_z_f = static_cast<float>(m.half_z);
     m._pad_f = 0.0f;
     for (int k = 0; k < 3; ++k) {
-        const DevSurface& s = m.surf[k];
+        DevSurface& s = m.surf[k];
+        DevFilm& fl = s.film;
+        fl.inv_gap_f = (fl.gap > 0.0) ? static_cast<float>(1.0 / fl.gap) : 0.0f;
+        fl.esc_c_f = (fl.v_sound * fl.tau0 > 0.0) ? static_cast<float>(fl.film_thickness / (fl.v_sound * fl.tau0)) : 0.0f;
+        fl.eff_f = static_cast<float>(fl.absorber_eff);
+        fl.eff_slope_f = static_cast<float>(fl.absorber_eff_slope);
+        fl.slope_f = static_cast<float>(fl.slope);
+        fl._pad_f = 0.0f;
         DevModel::SurfF& f = m.surf_f[k];
         f.prob_abs = static_cast<float>(s.prob_abs);
         f.prob_refl = static_cast<float>(s.prob_refl);
@@ -718,12 +729,18 @@ struct KList {
 
 template <class P> __device__ inline double film_escape_probability(const DevFilm& f, double E, double frac) {
     if (f.gap <= 0.0) return 1.0;
-    const double denom = 1.0 + f.slope * (E / f.gap - 2.0);
-    if (denom <= 0.0) return 1.0;
-    const double mfp = (f.v_sound * f.tau0) / denom;
-    const double path = frac * f.film_thickness;
-    if constexpr (P::exact) return exp(-path / mfp);
-    else return static_cast<double>(__expf(static_cast<float>(-path / mfp)));
+    if constexpr (P::exact) {
+        const double denom = 1.0 + f.slope * (E / f.gap - 2.0);
+        if (denom <= 0.0) return 1.0;
+        const double mfp = (f.v_sound * f.tau0) / denom;
+        const double path = frac * f.film_thickness;
+        return exp(-path / mfp);
+    } else {
+        // path / mfp = frac * denom * thickness / (v_sound tau0), in FP32
+        const float denom = 1.0f + f.slope_f * (static_cast<float>(E) * f.inv_gap_f - 2.0f);
+        if (denom <= 0.0f) return 1.0;
+        return static_cast<double>(__expf(-static_cast<float>(frac) * f.esc_c_f * denom));
+    }
 }
 
 template <class P> __device__ inline bool film_direct(const DevFilm& f, double E, Rng& r) {
@@ -734,9 +751,11 @@ template <class P> __device__ inline bool film_direct(const DevFilm& f, double E
 
 template <class P> __device__ inline double film_qp_absorption(const DevFilm& f, double qpE, KList& phon, KList& qp, Rng& r) {
     using R = typename P::R;
-    const double eff0 = (f.gap > 0.0) ? (f.absorber_eff + f.absorber_eff_slope * (qpE / f.gap)) : 1.0;
-    const double eff = tclamp(eff0, 0.0, 1.0);
-    if (P::uniform(r) > R(eff)) return 0.0;
+    R eff0;
+    if constexpr (P::exact) eff0 = (f.gap > 0.0) ? (f.absorber_eff + f.absorber_eff_slope * (qpE / f.gap)) : 1.0;
+    else eff0 = (f.gap > 0.0) ? (f.eff_f + f.eff_slope_f * (static_cast<float>(qpE) * f.inv_gap_f)) : 1.0f;
+    const R eff = tclamp(eff0, R(0), R(1));
+    if (P::uniform(r) > eff) return 0.0;
     if (qpE >= f.low_qp_limit * f.gap) { qp.push(qpE); return 0.0; }
     else if (qpE > f.gap) {
         const double e = qpE - f.gap;
@@ -764,7 +783,7 @@ template <class P> __device__ inline double film_sample_qp(const DevFilm& f, dou
         }
     } else {
         // FP32 proposal and acceptance in units of the gap (x/d), FP64 return value
-        const float Ed = static_cast<float>(E / d), sp = static_cast<float>(span / d);
+        const float Ed = static_cast<float>(E) * f.inv_gap_f, sp = static_cast<float>(span) * f.inv_gap_f;
         auto w = [Ed](float x) { return (x * (Ed - x) + 1.0f) * __builtin_amdgcn_rsqf((x + 1.0f) * (Ed - x + 1.0f)); };
         const float w_max = tmax(w(1.0f), w(0.5f * Ed));
         for (;;) {
@@ -794,8 +813,8 @@ template <class P> __device__ inline double film_sample_phonon(const DevFilm& f,
             if (P::uniform(r) * w_max <= w(x)) return E - x;
         }
     } else {
-        const float Ed = static_cast<float>(E / d), sp = static_cast<float>(span / d);
-        const float d2E = 1.0f / Ed;
+        const float Ed = static_cast<float>(E) * f.inv_gap_f, sp = static_cast<float>(span) * f.inv_gap_f;
+        const float d2E = P::rcp(Ed);
         auto w = [Ed, d2E](float x) {
             const float diff = Ed - x;
             return (diff * diff) * (x - d2E) * __builtin_amdgcn_rsqf(x + 1.0f);
@@ -867,7 +886,7 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
                 }
                 const R u_side = P::uniform(r);
                 const R u_ang = P::uniform(r);
-                const double frac = ((u_side < R(0.5)) ? 0.5 : 1.5) / static_cast<double>(P::cos_(u_ang * R(1.47)));
+                const double frac = static_cast<double>(P::div(R((u_side < R(0.5)) ? 0.5 : 1.5), P::cos_(u_ang * R(1.47))));
                 if (P::uniform(r) < R(film_escape_probability<P>(f, E, frac))) out.push(E);
                 else if (f.keep_all_phonons || !(E < 2.0 * f.gap)) nph.push(E);
             }
