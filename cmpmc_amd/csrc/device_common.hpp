@@ -88,6 +88,11 @@ struct DevModel {
         float t_anh[4], t_spec[4];
         float prob_abs, prob_refl, film_abs;
         int32_t const_split, has_film, _pad;
+        // the film's pair-breaking threshold 2 gap and direct-absorption threshold
+        // 2 absorber_gap (FP64, compared with FP64 energies) and subgap_absorption
+        double film_gap2, film_abs_gap2;
+        float subgap_abs;
+        int32_t _pad2;
     } surf_f[3];
     float orient_f[9];
     float fL_f, fLST_f, fTT_f, pST_f, poly_rate_f, radius_f, half_z_f, _pad_f;
@@ -688,6 +693,10 @@ inline void fill_fp32_constants(DevModel& m) {
         fl.slope_f = static_cast<float>(fl.slope);
         fl._pad_f = 0.0f;
         DevModel::SurfF& f = m.surf_f[k];
+        f.film_gap2 = 2.0 * fl.gap;
+        f.film_abs_gap2 = 2.0 * fl.absorber_gap;
+        f.subgap_abs = static_cast<float>(fl.subgap_absorption);
+        f._pad2 = 0;
         f.prob_abs = static_cast<float>(s.prob_abs);
         f.prob_refl = static_cast<float>(s.prob_refl);
         f.film_abs = static_cast<float>(s.film.film_absorption);
@@ -799,7 +808,7 @@ template <class P> __device__ inline double film_sample_phonon(const DevFilm& f,
     const double d = f.gap;
     const double span = E - d;
     if (span <= 1e-16) return 0.0;
-    if constexpr (P::exact) {
+    auto sample64 = [&]() {
         const double d2E = (d * d) / E;
         auto w = [E, d, d2E](double x) {
             const double diff = E - x;
@@ -810,8 +819,11 @@ template <class P> __device__ inline double film_sample_phonon(const DevFilm& f,
         for (;;) {
             const double u = P::uniform(r);
             const double x = d + span * (u * u);
-            if (P::uniform(r) * w_max <= w(x)) return E - x;
+            if (static_cast<double>(P::uniform(r)) * w_max <= w(x)) return E - x;
         }
+    };
+    if constexpr (P::exact) {
+        return sample64();
     } else {
         const float Ed = static_cast<float>(E) * f.inv_gap_f, sp = static_cast<float>(span) * f.inv_gap_f;
         const float d2E = P::rcp(Ed);
@@ -821,6 +833,9 @@ template <class P> __device__ inline double film_sample_phonon(const DevFilm& f,
         };
         const float x_peak = tclamp(0.25f * (Ed + 3.0f * d2E), 1.0f, Ed);
         const float w_max = tmax(w(1.0f), w(x_peak)) * 1.02f;
+        // E within FP32 rounding of the gap (only reachable with low_qp_limit near 1):
+        // the FP32 envelope can vanish, so such a quasiparticle is sampled in FP64
+        if (!(w_max > 0.0f)) return sample64();
         for (;;) {
             const float u = P::uniform(r);
             const float xd = 1.0f + sp * (u * u);

@@ -11,9 +11,12 @@
 //  * surface constants come from the model's FP32 block (no per-step conversions, read
 //    through ctx.surf(sid), which the kernel keeps in LDS), and a surface whose
 //    reflection polynomials are constants uses its tabulated split;
-//  * a phonon entering a Kaplan film is not cascaded inline: the step returns
-//    kStepFilm with the entry normal and surface, and the kernel batches such tracks so
-//    that a whole wave runs cascades together (film_job below).
+//  * a phonon absorbed by a Kaplan film at or above the pair-breaking threshold (2 gap)
+//    is not cascaded inline: the step returns kStepFilm with the entry normal and
+//    surface, and the kernel batches such tracks so that a whole wave runs cascades
+//    together (film_job below). Below the threshold the cascade ends at once (direct
+//    absorption, or re-emission with a DOS mode and an inward direction); the step does
+//    that itself, re-emission through the converged tail like a diffuse reflection.
 // Rare branches (anharmonic decay, surface down-conversion) keep their own lookups.
 //
 // The context also receives the likelihood terms of every random choice that depends on
@@ -32,7 +35,9 @@ enum StepCode : int { kStepDead = 0, kStepAlive = 1, kStepFilm = 2 };
 enum BulkProcess : int { kProcIso = 0, kProcDecay = 1, kProcPoly = 2 };
 
 template <class P, class Ctx>
-__device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film) {
+__device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film,
+                                       bool alive) {
+    if (!alive) return kStepDead;
     Rng& r = trk.rng;
     if (trk.energy < m.min_phonon_eV || trk.bounces >= m.max_bounces) {
         ctx.count(C_KILLED);
@@ -86,63 +91,87 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
         ctx.flight(rate_iso, rate_down, tau);
         trk.bounces++;
         const DevModel::SurfF& sf = ctx.surf(sid);
+        bool reemit = false;
         if (sf.has_film && P::uniform(r) < sf.prob_abs) {
             const bool absorbed = P::uniform(r) < sf.film_abs;
             ctx.film_gate(absorbed);
             if (absorbed) {
                 ctx.branch(CMPMC_BR_FILM);
-                n_film = n_in;
-                sid_film = sid;
-                return kStepFilm;
+                if (!(trk.energy < sf.film_gap2)) {
+                    n_film = n_in;
+                    sid_film = sid;
+                    return kStepFilm;
+                }
+                // Below the pair-breaking threshold the film's cascade ends at once
+                // (film_absorb_phonon): direct absorption, or the phonon goes back into
+                // the crystal with a DOS mode and an inward direction (film_job). Both
+                // are done here, with the same draws in the same order, instead of in
+                // the wave's film phase.
+                if (!(trk.energy < sf.film_abs_gap2) && P::uniform(r) < sf.subgap_abs) {
+                    const double wed = trk.energy * trk.weight;
+                    ctx.count(C_HITS);
+                    ctx.deposited(wed);
+                    ctx.hit(trk, wed, sid);
+                    return kStepDead;
+                }
+                reemit = true;
             }
         }
-        if (P::uniform(r) > sf.prob_refl) {
-            ctx.count(C_KILLED);
-            ctx.escaped(trk.energy * trk.weight);
-            ctx.branch(CMPMC_BR_ESCAPE);
-            return kStepDead;
-        }
-        ctx.count(C_REFL);
-        trk.pos = trk.pos + n_in * 1e-11f;
-        float t_anh, t_spec;
-        if (sf.const_split) {
-            const int reg = (trk.energy > sf.e_refl ? 1 : 0) + (trk.energy > sf.e_anh ? 2 : 0);
-            t_anh = sf.t_anh[reg];
-            t_spec = sf.t_spec[reg];
-        } else {
-            double p_spec, p_diff, p_anh;
-            reflection_split(m.surf[sid], trk.energy, p_spec, p_diff, p_anh);
-            t_anh = static_cast<float>(p_anh);
-            t_spec = static_cast<float>(p_anh + p_spec);
-        }
-        const float u_refl = P::uniform(r);
-        if (u_refl < t_anh) {                     // surface-mediated down-conversion
-            ctx.count(C_SURF_ANH);
-            ctx.branch(CMPMC_BR_SURF_ANH);
-            const bool tt = P::uniform(r) < m.fTT_f;
-            const double x = tt ? sample_fraction<P>(ctx.cdf_tt(), r) : sample_fraction<P>(ctx.cdf_lt(), r);
-            const int m1 = tt ? sample_transverse_mode<P>(m, r) : 0;
-            const int m2 = sample_transverse_mode<P>(m, r);
-            const V3<float> k2 = sample_inward_wavevector<P>(m, m2, n_in, r);
-            PTrack<P> d;
-            make_track<P>(m, d, trk.pos, k2, m2, (1.0 - x) * trk.energy, trk.time, trk.weight);
-            ctx.push(d, trk);
-            trk.energy = x * trk.energy;
-            trk.mode = m1;
-            trk.k = sample_inward_wavevector<P>(m, m1, n_in, r);
-            map_k_to_vg<P>(m, m1, trk.k, trk.vg, trk.vg_mag);
-            return kStepAlive;
-        }
-        reflect = true;
-        if (u_refl < t_spec) {
-            ctx.branch(CMPMC_BR_SPECULAR);
-            ctx.reflect(false);
-            specular = true;
-            knew = trk.k - n_in * (2.0f * dot(trk.k, n_in));
-        } else {
-            ctx.branch(CMPMC_BR_DIFFUSE);
-            ctx.reflect(true);
+        if (reemit) {
+            trk.pos = trk.pos + n_in * 1e-11f;
+            const float um = P::uniform(r);         // sample_dos_mode
+            mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
+            reflect = true;                       // sample_inward_wavevector: first trial below
             knew = lambertian_dir<P>(r, n_in);
+        } else {
+            if (P::uniform(r) > sf.prob_refl) {
+                ctx.count(C_KILLED);
+                ctx.escaped(trk.energy * trk.weight);
+                ctx.branch(CMPMC_BR_ESCAPE);
+                return kStepDead;
+            }
+            ctx.count(C_REFL);
+            trk.pos = trk.pos + n_in * 1e-11f;
+            float t_anh, t_spec;
+            if (sf.const_split) {
+                const int reg = (trk.energy > sf.e_refl ? 1 : 0) + (trk.energy > sf.e_anh ? 2 : 0);
+                t_anh = sf.t_anh[reg];
+                t_spec = sf.t_spec[reg];
+            } else {
+                double p_spec, p_diff, p_anh;
+                reflection_split(m.surf[sid], trk.energy, p_spec, p_diff, p_anh);
+                t_anh = static_cast<float>(p_anh);
+                t_spec = static_cast<float>(p_anh + p_spec);
+            }
+            const float u_refl = P::uniform(r);
+            if (u_refl < t_anh) {                     // surface-mediated down-conversion
+                ctx.count(C_SURF_ANH);
+                ctx.branch(CMPMC_BR_SURF_ANH);
+                const bool tt = P::uniform(r) < m.fTT_f;
+                const double x = tt ? sample_fraction<P>(ctx.cdf_tt(), r) : sample_fraction<P>(ctx.cdf_lt(), r);
+                const int m1 = tt ? sample_transverse_mode<P>(m, r) : 0;
+                const int m2 = sample_transverse_mode<P>(m, r);
+                const V3<float> k2 = sample_inward_wavevector<P>(m, m2, n_in, r);
+                PTrack<P> d;
+                make_track<P>(m, d, trk.pos, k2, m2, (1.0 - x) * trk.energy, trk.time, trk.weight);
+                ctx.push(d, trk);
+                trk.energy = x * trk.energy;
+                trk.mode = m1;
+                trk.k = sample_inward_wavevector<P>(m, m1, n_in, r);
+                map_k_to_vg<P>(m, m1, trk.k, trk.vg, trk.vg_mag);
+                return kStepAlive;
+            }
+            reflect = true;
+            if (u_refl < t_spec) {
+                ctx.branch(CMPMC_BR_SPECULAR);
+                ctx.reflect(false);
+                specular = true;
+                knew = trk.k - n_in * (2.0f * dot(trk.k, n_in));
+            } else {
+                ctx.branch(CMPMC_BR_DIFFUSE);
+                ctx.reflect(true);
+                knew = lambertian_dir<P>(r, n_in);
+            }
         }
     }
     // converged tail: one LUT lookup for the new direction
@@ -150,20 +179,27 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
     float mag;
     map_k_to_vg<P>(m, mnew, knew, vg, mag);
     if (reflect && !(dot(vg, n_in) > 1e-6f)) {
-        if (specular) {                           // engine.hpp: resample an inward direction
-            knew = sample_inward_wavevector<P>(m, mnew, n_in, r);
-        } else {                                  // diffuse: trials 2..16 of the inward sampler
-            bool ok = false;
-            for (int trial = 1; trial < 16 && !ok; ++trial) {
-                const V3<float> kd = lambertian_dir<P>(r, n_in);
-                V3<float> v2;
-                float m2;
-                map_k_to_vg<P>(m, mnew, kd, v2, m2);
-                if (dot(v2, n_in) > 1e-6f) { knew = kd; ok = true; }
+        // an outward group velocity: sample_inward_wavevector (engine.hpp:927-936), whose
+        // 16 trials a specular reflection runs in full and a diffuse one from the second
+        // on (its first was the proposal above). The accepted proposal's lookup is the
+        // new velocity; only when every trial fails is the normal looked up.
+        bool ok = false;
+        for (int trial = specular ? 0 : 1; trial < 16 && !ok; ++trial) {
+            const V3<float> kd = lambertian_dir<P>(r, n_in);
+            V3<float> v2;
+            float m2;
+            map_k_to_vg<P>(m, mnew, kd, v2, m2);
+            if (dot(v2, n_in) > 1e-6f) {
+                knew = kd;
+                vg = v2;
+                mag = m2;
+                ok = true;
             }
-            if (!ok) knew = n_in;
         }
-        map_k_to_vg<P>(m, mnew, knew, vg, mag);
+        if (!ok) {
+            knew = n_in;
+            map_k_to_vg<P>(m, mnew, knew, vg, mag);
+        }
     }
     trk.k = knew;
     trk.mode = mnew;

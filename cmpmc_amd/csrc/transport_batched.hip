@@ -397,11 +397,9 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
         int code = kStepDead;
         V3<float> n_film;
         int sid_film = 0;
-        if (alive) {
-            code = phonon_step_fp32<P>(m, t, ctx, n_film, sid_film);
-            alive = (code == kStepAlive);
-            if (!alive) ctx.rec = false;
-        }
+        code = phonon_step_fp32<P>(m, t, ctx, n_film, sid_film, alive);
+        alive = (code == kStepAlive);
+        if (!alive) ctx.rec = false;
         const uint64_t film_mask = __ballot(code == kStepFilm);
         if (film_mask) {
             if (code == kStepFilm) {
