@@ -117,12 +117,15 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
                 reemit = true;
             }
         }
-        if (reemit) {
-            trk.pos = trk.pos + n_in * 1e-11f;
+        // re-emission and reflection both leave from 1e-11 m inside the surface (an
+        // escaping track ends below, whatever its position); both may end with a
+        // Lambertian proposal, drawn once for either
+        trk.pos = trk.pos + n_in * 1e-11f;
+        reflect = true;
+        bool lamb = reemit;
+        if (reemit) {                             // then sample_inward_wavevector, first trial below
             const float um = P::uniform(r);         // sample_dos_mode
             mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
-            reflect = true;                       // sample_inward_wavevector: first trial below
-            knew = lambertian_dir<P>(r, n_in);
         } else {
             if (P::uniform(r) > sf.prob_refl) {
                 ctx.count(C_KILLED);
@@ -131,7 +134,6 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
                 return kStepDead;
             }
             ctx.count(C_REFL);
-            trk.pos = trk.pos + n_in * 1e-11f;
             float t_anh, t_spec;
             if (sf.const_split) {
                 const int reg = (trk.energy > sf.e_refl ? 1 : 0) + (trk.energy > sf.e_anh ? 2 : 0);
@@ -161,7 +163,6 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
                 map_k_to_vg<P>(m, m1, trk.k, trk.vg, trk.vg_mag);
                 return kStepAlive;
             }
-            reflect = true;
             if (u_refl < t_spec) {
                 ctx.branch(CMPMC_BR_SPECULAR);
                 ctx.reflect(false);
@@ -170,9 +171,10 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
             } else {
                 ctx.branch(CMPMC_BR_DIFFUSE);
                 ctx.reflect(true);
-                knew = lambertian_dir<P>(r, n_in);
+                lamb = true;
             }
         }
+        if (lamb) knew = lambertian_dir<P>(r, n_in);
     }
     // converged tail: one LUT lookup for the new direction
     V3<float> vg;
