@@ -62,9 +62,11 @@ __device__ inline V3<typename P::R> sample_inward_wavevector(const DevModel& m, 
     return n;
 }
 
-// engine.hpp:938-988
+// engine.hpp:938-988, up to the kept phonon's lookup: pushes the daughter, scales the
+// track's energy and returns the kept phonon's mode and wavevector.
 template <class P, class Ctx>
-__device__ inline void anharmonic_decay(const DevModel& m, PTrack<P>& trk, Rng& r, Ctx& ctx) {
+__device__ inline void anharmonic_decay_split(const DevModel& m, PTrack<P>& trk, Rng& r, Ctx& ctx, int& m_keep,
+                                              V3<typename P::R>& k_keep) {
     using R = typename P::R;
     const V3<R> dir1 = trk.k;
     const V3<R> perp = orthogonal_unit<P>(dir1);
@@ -84,9 +86,8 @@ __device__ inline void anharmonic_decay(const DevModel& m, PTrack<P>& trk, Rng& 
         make_track<P>(m, d, trk.pos, k2, m2, (1.0 - x) * trk.energy, trk.time, trk.weight);
         ctx.push(d, trk);
         trk.energy = x * trk.energy;
-        trk.mode = m1;
-        trk.k = k1;
-        map_k_to_vg<P>(m, m1, k1, trk.vg, trk.vg_mag);
+        m_keep = m1;
+        k_keep = k1;
     } else {
         ctx.count(C_LT);
         ctx.branch(CMPMC_BR_DECAY_LT);
@@ -99,10 +100,20 @@ __device__ inline void anharmonic_decay(const DevModel& m, PTrack<P>& trk, Rng& 
         make_track<P>(m, d, trk.pos, kT, mT, (1.0 - x) * trk.energy, trk.time, trk.weight);
         ctx.push(d, trk);
         trk.energy = x * trk.energy;
-        trk.mode = 0;
-        trk.k = kL;
-        map_k_to_vg<P>(m, 0, kL, trk.vg, trk.vg_mag);
+        m_keep = 0;
+        k_keep = kL;
     }
+}
+
+// engine.hpp:938-988
+template <class P, class Ctx>
+__device__ inline void anharmonic_decay(const DevModel& m, PTrack<P>& trk, Rng& r, Ctx& ctx) {
+    int m_keep;
+    V3<typename P::R> k_keep;
+    anharmonic_decay_split<P>(m, trk, r, ctx, m_keep, k_keep);
+    trk.mode = m_keep;
+    trk.k = k_keep;
+    map_k_to_vg<P>(m, m_keep, k_keep, trk.vg, trk.vg_mag);
 }
 
 // Returns false when the track ends (killed, absorbed without re-emission, escaped).

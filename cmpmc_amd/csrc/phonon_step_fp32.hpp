@@ -37,154 +37,158 @@ enum BulkProcess : int { kProcIso = 0, kProcDecay = 1, kProcPoly = 2 };
 template <class P, class Ctx>
 __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film,
                                        bool alive) {
-    if (!alive) return kStepDead;
     Rng& r = trk.rng;
-    if (trk.energy < m.min_phonon_eV || trk.bounces >= m.max_bounces) {
+    int code = alive ? kStepAlive : kStepDead;
+    // the direction and mode the closing lookup uses; a lane without a track looks up a
+    // fixed direction, a lane whose track ends its unchanged one
+    int mnew = alive ? trk.mode : 0;
+    V3<float> knew = alive ? trk.k : V3<float>{0.0f, 0.0f, 1.0f};
+    V3<float> n_in{0.0f, 0.0f, 1.0f};
+    bool reflect = false, specular = false;
+    if (alive && (trk.energy < m.min_phonon_eV || trk.bounces >= m.max_bounces)) {
         ctx.count(C_KILLED);
         ctx.escaped(trk.energy * trk.weight);
         ctx.branch(CMPMC_BR_KILLED);
-        return kStepDead;
+        code = kStepDead;
     }
-    if (m.rr_bounces != 0 && trk.bounces > m.rr_bounces && (trk.bounces - m.rr_bounces) % 32 == 0) {
+    if (code == kStepAlive && m.rr_bounces != 0 && trk.bounces > m.rr_bounces &&
+        (trk.bounces - m.rr_bounces) % 32 == 0) {
         if (P::uniform(r) < static_cast<float>(m.rr_survival)) trk.weight /= m.rr_survival;
         else {
             ctx.count(C_KILLED);
             ctx.branch(CMPMC_BR_ROULETTE);
-            return kStepDead;
+            code = kStepDead;
         }
     }
-    ctx.count(C_STEPS);
-    const float e = static_cast<float>(trk.energy);
-    const float e2 = e * e;
-    const float rate_iso = m.c_iso_f * (e2 * e2);
-    const float rate_poly = trk.vg_mag * m.poly_rate_f;
-    const float rate_down = (trk.mode != 0) ? 0.0f : m.c_anh_f * (e2 * e2 * e);
-    const float rate_tot = rate_iso + rate_poly + rate_down;
-    const float d_phys = (rate_tot > 1e-12f) ? (P::neg_log(P::uniform(r)) * P::div(trk.vg_mag, rate_tot)) : P::inf();
-    V3<float> n_in;
-    int sid;
-    const float dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
-
-    int mnew = trk.mode;
-    V3<float> knew;
-    bool reflect = false, specular = false;
-    if (d_phys < dist) {                          // bulk event
-        const float tau = P::div(d_phys, trk.vg_mag);
-        trk.pos = trk.pos + trk.vg * d_phys;
-        trk.time += static_cast<double>(tau);
-        ctx.flight(rate_iso, rate_down, tau);
-        const float u_proc = P::uniform(r) * rate_tot;
-        if (rate_down > 0.0f && u_proc < rate_down) {
-            ctx.process(kProcDecay);
-            anharmonic_decay<P>(m, trk, r, ctx);
-            return kStepAlive;
-        }
-        if (u_proc < rate_down + rate_iso) { ctx.count(C_ISO); ctx.branch(CMPMC_BR_ISOTOPE); ctx.process(kProcIso); }
-        else { ctx.count(C_POLY); ctx.branch(CMPMC_BR_POLY); ctx.process(kProcPoly); }
-        const float um = P::uniform(r);             // sample_dos_mode
-        mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
-        knew = isotropic_dir<P>(r);
-    } else {                                      // boundary
-        const float tau = P::div(dist, trk.vg_mag);
-        trk.pos = trk.pos + trk.vg * dist;
-        trk.time += static_cast<double>(tau);
-        ctx.flight(rate_iso, rate_down, tau);
-        trk.bounces++;
-        const DevModel::SurfF& sf = ctx.surf(sid);
-        bool reemit = false;
-        if (sf.has_film && P::uniform(r) < sf.prob_abs) {
-            const bool absorbed = P::uniform(r) < sf.film_abs;
-            ctx.film_gate(absorbed);
-            if (absorbed) {
-                ctx.branch(CMPMC_BR_FILM);
-                if (!(trk.energy < sf.film_gap2)) {
-                    n_film = n_in;
-                    sid_film = sid;
-                    return kStepFilm;
-                }
-                // Below the pair-breaking threshold the film's cascade ends at once
-                // (film_absorb_phonon): direct absorption, or the phonon goes back into
-                // the crystal with a DOS mode and an inward direction (film_job). Both
-                // are done here, with the same draws in the same order, instead of in
-                // the wave's film phase.
-                if (!(trk.energy < sf.film_abs_gap2) && P::uniform(r) < sf.subgap_abs) {
-                    const double wed = trk.energy * trk.weight;
-                    ctx.count(C_HITS);
-                    ctx.deposited(wed);
-                    ctx.hit(trk, wed, sid);
-                    return kStepDead;
-                }
-                reemit = true;
-            }
-        }
-        // re-emission and reflection both leave from 1e-11 m inside the surface (an
-        // escaping track ends below, whatever its position); both may end with a
-        // Lambertian proposal, drawn once for either
-        trk.pos = trk.pos + n_in * 1e-11f;
-        reflect = true;
-        bool lamb = reemit;
-        if (reemit) {                             // then sample_inward_wavevector, first trial below
-            const float um = P::uniform(r);         // sample_dos_mode
-            mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
-        } else {
-            if (P::uniform(r) > sf.prob_refl) {
-                ctx.count(C_KILLED);
-                ctx.escaped(trk.energy * trk.weight);
-                ctx.branch(CMPMC_BR_ESCAPE);
-                return kStepDead;
-            }
-            ctx.count(C_REFL);
-            float t_anh, t_spec;
-            if (sf.const_split) {
-                const int reg = (trk.energy > sf.e_refl ? 1 : 0) + (trk.energy > sf.e_anh ? 2 : 0);
-                t_anh = sf.t_anh[reg];
-                t_spec = sf.t_spec[reg];
+    if (code == kStepAlive) {
+        ctx.count(C_STEPS);
+        const float e = static_cast<float>(trk.energy);
+        const float e2 = e * e;
+        const float rate_iso = m.c_iso_f * (e2 * e2);
+        const float rate_poly = trk.vg_mag * m.poly_rate_f;
+        const float rate_down = (trk.mode != 0) ? 0.0f : m.c_anh_f * (e2 * e2 * e);
+        const float rate_tot = rate_iso + rate_poly + rate_down;
+        const float d_phys = (rate_tot > 1e-12f) ? (P::neg_log(P::uniform(r)) * P::div(trk.vg_mag, rate_tot)) : P::inf();
+        int sid;
+        const float dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
+        if (d_phys < dist) {                      // bulk event
+            const float tau = P::div(d_phys, trk.vg_mag);
+            trk.pos = trk.pos + trk.vg * d_phys;
+            trk.time += static_cast<double>(tau);
+            ctx.flight(rate_iso, rate_down, tau);
+            const float u_proc = P::uniform(r) * rate_tot;
+            if (rate_down > 0.0f && u_proc < rate_down) {
+                ctx.process(kProcDecay);
+                anharmonic_decay_split<P>(m, trk, r, ctx, mnew, knew);
             } else {
-                double p_spec, p_diff, p_anh;
-                reflection_split(m.surf[sid], trk.energy, p_spec, p_diff, p_anh);
-                t_anh = static_cast<float>(p_anh);
-                t_spec = static_cast<float>(p_anh + p_spec);
+                if (u_proc < rate_down + rate_iso) { ctx.count(C_ISO); ctx.branch(CMPMC_BR_ISOTOPE); ctx.process(kProcIso); }
+                else { ctx.count(C_POLY); ctx.branch(CMPMC_BR_POLY); ctx.process(kProcPoly); }
+                const float um = P::uniform(r);     // sample_dos_mode
+                mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
+                knew = isotropic_dir<P>(r);
             }
-            const float u_refl = P::uniform(r);
-            if (u_refl < t_anh) {                     // surface-mediated down-conversion
-                ctx.count(C_SURF_ANH);
-                ctx.branch(CMPMC_BR_SURF_ANH);
-                const bool tt = P::uniform(r) < m.fTT_f;
-                const double x = tt ? sample_fraction<P>(ctx.cdf_tt(), r) : sample_fraction<P>(ctx.cdf_lt(), r);
-                const int m1 = tt ? sample_transverse_mode<P>(m, r) : 0;
-                const int m2 = sample_transverse_mode<P>(m, r);
-                const V3<float> k2 = sample_inward_wavevector<P>(m, m2, n_in, r);
-                PTrack<P> d;
-                make_track<P>(m, d, trk.pos, k2, m2, (1.0 - x) * trk.energy, trk.time, trk.weight);
-                ctx.push(d, trk);
-                trk.energy = x * trk.energy;
-                trk.mode = m1;
-                trk.k = sample_inward_wavevector<P>(m, m1, n_in, r);
-                map_k_to_vg<P>(m, m1, trk.k, trk.vg, trk.vg_mag);
-                return kStepAlive;
+        } else {                                  // boundary
+            const float tau = P::div(dist, trk.vg_mag);
+            trk.pos = trk.pos + trk.vg * dist;
+            trk.time += static_cast<double>(tau);
+            ctx.flight(rate_iso, rate_down, tau);
+            trk.bounces++;
+            const DevModel::SurfF& sf = ctx.surf(sid);
+            bool reemit = false;
+            if (sf.has_film && P::uniform(r) < sf.prob_abs) {
+                const bool absorbed = P::uniform(r) < sf.film_abs;
+                ctx.film_gate(absorbed);
+                if (absorbed) {
+                    ctx.branch(CMPMC_BR_FILM);
+                    if (!(trk.energy < sf.film_gap2)) {
+                        n_film = n_in;
+                        sid_film = sid;
+                        code = kStepFilm;
+                    } else if (!(trk.energy < sf.film_abs_gap2) && P::uniform(r) < sf.subgap_abs) {
+                        // Below the pair-breaking threshold the film's cascade ends at
+                        // once (film_absorb_phonon): direct absorption, or the phonon goes
+                        // back into the crystal with a DOS mode and an inward direction
+                        // (film_job). Both are done here, with the same draws in the same
+                        // order, instead of in the wave's film phase.
+                        const double wed = trk.energy * trk.weight;
+                        ctx.count(C_HITS);
+                        ctx.deposited(wed);
+                        ctx.hit(trk, wed, sid);
+                        code = kStepDead;
+                    } else {
+                        reemit = true;
+                    }
+                }
             }
-            if (u_refl < t_spec) {
-                ctx.branch(CMPMC_BR_SPECULAR);
-                ctx.reflect(false);
-                specular = true;
-                knew = trk.k - n_in * (2.0f * dot(trk.k, n_in));
-            } else {
-                ctx.branch(CMPMC_BR_DIFFUSE);
-                ctx.reflect(true);
-                lamb = true;
+            if (code == kStepAlive) {
+                // re-emission and reflection both leave from 1e-11 m inside the surface;
+                // both may end with a Lambertian proposal, drawn once for either
+                trk.pos = trk.pos + n_in * 1e-11f;
+                bool lamb = reemit;
+                if (reemit) {                     // then sample_inward_wavevector, first trial below
+                    const float um = P::uniform(r);     // sample_dos_mode
+                    mnew = (um < m.fL_f) ? 0 : ((um < m.fLST_f) ? 2 : 1);
+                    reflect = true;
+                } else if (P::uniform(r) > sf.prob_refl) {
+                    ctx.count(C_KILLED);
+                    ctx.escaped(trk.energy * trk.weight);
+                    ctx.branch(CMPMC_BR_ESCAPE);
+                    code = kStepDead;
+                } else {
+                    ctx.count(C_REFL);
+                    float t_anh, t_spec;
+                    if (sf.const_split) {
+                        const int reg = (trk.energy > sf.e_refl ? 1 : 0) + (trk.energy > sf.e_anh ? 2 : 0);
+                        t_anh = sf.t_anh[reg];
+                        t_spec = sf.t_spec[reg];
+                    } else {
+                        double p_spec, p_diff, p_anh;
+                        reflection_split(m.surf[sid], trk.energy, p_spec, p_diff, p_anh);
+                        t_anh = static_cast<float>(p_anh);
+                        t_spec = static_cast<float>(p_anh + p_spec);
+                    }
+                    const float u_refl = P::uniform(r);
+                    reflect = true;
+                    if (u_refl < t_anh) {                 // surface-mediated down-conversion
+                        ctx.count(C_SURF_ANH);
+                        ctx.branch(CMPMC_BR_SURF_ANH);
+                        const bool tt = P::uniform(r) < m.fTT_f;
+                        const double x = tt ? sample_fraction<P>(ctx.cdf_tt(), r) : sample_fraction<P>(ctx.cdf_lt(), r);
+                        const int m1 = tt ? sample_transverse_mode<P>(m, r) : 0;
+                        const int m2 = sample_transverse_mode<P>(m, r);
+                        const V3<float> k2 = sample_inward_wavevector<P>(m, m2, n_in, r);
+                        PTrack<P> d;
+                        make_track<P>(m, d, trk.pos, k2, m2, (1.0 - x) * trk.energy, trk.time, trk.weight);
+                        ctx.push(d, trk);
+                        trk.energy = x * trk.energy;
+                        mnew = m1;                    // then sample_inward_wavevector for m1
+                        lamb = true;
+                    } else if (u_refl < t_spec) {
+                        ctx.branch(CMPMC_BR_SPECULAR);
+                        ctx.reflect(false);
+                        specular = true;
+                        knew = trk.k - n_in * (2.0f * dot(trk.k, n_in));
+                    } else {
+                        ctx.branch(CMPMC_BR_DIFFUSE);
+                        ctx.reflect(true);
+                        lamb = true;
+                    }
+                }
+                if (lamb) knew = lambertian_dir<P>(r, n_in);
             }
         }
-        if (lamb) knew = lambertian_dir<P>(r, n_in);
     }
-    // converged tail: one LUT lookup for the new direction
+    // closing lookup, run by every lane of the wave (one instruction stream; no path
+    // returns early, so the track's registers are updated in place)
     V3<float> vg;
     float mag;
     map_k_to_vg<P>(m, mnew, knew, vg, mag);
-    if (reflect && !(dot(vg, n_in) > 1e-6f)) {
+    if (reflect && code == kStepAlive && !(dot(vg, n_in) > 1e-6f)) {
         // an outward group velocity: sample_inward_wavevector (engine.hpp:927-936), whose
-        // 16 trials a specular reflection runs in full and a diffuse one from the second
-        // on (its first was the proposal above). The accepted proposal's lookup is the
-        // new velocity; only when every trial fails is the normal looked up.
+        // 16 trials a specular reflection runs in full and a diffuse one (or a re-emission,
+        // or a down-converted phonon) from the second on (its first was the proposal
+        // above). The accepted proposal's lookup is the new velocity; only when every
+        // trial fails is the normal looked up.
         bool ok = false;
         for (int trial = specular ? 0 : 1; trial < 16 && !ok; ++trial) {
             const V3<float> kd = lambertian_dir<P>(r, n_in);
@@ -207,7 +211,7 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
     trk.mode = mnew;
     trk.vg = vg;
     trk.vg_mag = mag;
-    return kStepAlive;
+    return code;
 }
 
 // Kaplan film interaction of a track that phonon_step_fp32 handed over (engine.hpp:

@@ -356,6 +356,10 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
             }
             alive_mask = __ballot(alive);
         }
+        // One of three wave-uniform cases per iteration, and a single uniform exit at the
+        // end of the body (no continue or break inside: a loop with one latch keeps the
+        // track's registers in place from one iteration to the next).
+        bool stop = false;
         if (nf > 0 && (nf >= kFilmFlush || alive_mask == 0)) {
             // film phase: lane i runs deferred interaction nf - take + i
             const uint32_t take = nf < 64u ? nf : 64u;
@@ -380,38 +384,38 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                 for (int i = 0; i < 4; ++i) ctx.sc[i] = own_sc[i];
             }
             nf -= take;
-            continue;
-        }
-        if (alive_mask == 0) {
-            if (__ballot(!exhausted) == 0 && __builtin_amdgcn_readfirstlane(*ctx.top) == 0) break;
-            continue;
-        }
-        if ((iter & 1023u) == 0 && a.cancel) {
-            int c = 0;
-            if (lid == 0) c = *a.cancel;
-            if (__shfl(c, 0)) {
-                if (lid == 0) atomicOr(&a.status[2], 1ull);
-                break;
+        } else if (alive_mask == 0) {             // nothing to step: done once nothing is left
+            stop = __ballot(!exhausted) == 0 && __builtin_amdgcn_readfirstlane(*ctx.top) == 0;
+        } else {
+            if ((iter & 1023u) == 0 && a.cancel) {
+                int c = 0;
+                if (lid == 0) c = *a.cancel;
+                if (__builtin_amdgcn_readfirstlane(c)) {
+                    if (lid == 0) atomicOr(&a.status[2], 1ull);
+                    stop = true;
+                }
+            }
+            if (!stop) {
+                V3<float> n_film;
+                int sid_film = 0;
+                const int code = phonon_step_fp32<P>(m, t, ctx, n_film, sid_film, alive);
+                alive = (code == kStepAlive);
+                if (!alive) ctx.rec = false;
+                const uint64_t film_mask = __ballot(code == kStepFilm);
+                if (film_mask) {
+                    if (code == kStepFilm) {
+                        Rec& jr = fbuf[nf + __popcll(film_mask & lt_mask)];
+                        SRec& jb = Ops::base(jr);
+                        ptrack_to_srec(t, ctx.event, jb);
+                        jb.n_in[0] = n_film.x; jb.n_in[1] = n_film.y; jb.n_in[2] = n_film.z;
+                        jb.sid = sid_film;
+                        Ops::store_sc(jr, ctx.sc);
+                    }
+                    nf += static_cast<uint32_t>(__popcll(film_mask));
+                }
             }
         }
-        int code = kStepDead;
-        V3<float> n_film;
-        int sid_film = 0;
-        code = phonon_step_fp32<P>(m, t, ctx, n_film, sid_film, alive);
-        alive = (code == kStepAlive);
-        if (!alive) ctx.rec = false;
-        const uint64_t film_mask = __ballot(code == kStepFilm);
-        if (film_mask) {
-            if (code == kStepFilm) {
-                Rec& jr = fbuf[nf + __popcll(film_mask & lt_mask)];
-                SRec& jb = Ops::base(jr);
-                ptrack_to_srec(t, ctx.event, jb);
-                jb.n_in[0] = n_film.x; jb.n_in[1] = n_film.y; jb.n_in[2] = n_film.z;
-                jb.sid = sid_film;
-                Ops::store_sc(jr, ctx.sc);
-            }
-            nf += static_cast<uint32_t>(__popcll(film_mask));
-        }
+        if (stop) break;
     }
     if (ctx.overflow) {
         atomicOr(&a.status[0], 1ull);
