@@ -22,7 +22,7 @@ template <class P> struct CTrack {
     double time, weight;
     int valley;
     int type;                    // 1 electron, 2 hole
-    Rng rng;
+    typename P::TRng rng;
     uint64_t key;
     uint32_t spawn_no, hit_seq;
 };
@@ -164,8 +164,8 @@ template <class P> __device__ inline CRates<P> charge_rates(const DevModel& m, c
 }
 
 // charge_transport.hpp:109-166. Returns true if a phonon was emitted.
-template <class P>
-__device__ inline bool emit_luke(const DevModel& m, CTrack<P>& t, Rng& r, V3<typename P::R>& q_dir, typename P::R& e_ph_out) {
+template <class P, class G>
+__device__ inline bool emit_luke(const DevModel& m, CTrack<P>& t, G& r, V3<typename P::R>& q_dir, typename P::R& e_ph_out) {
     using R = typename P::R;
     V3<R> k_trk;
     R k_sound;
@@ -209,7 +209,7 @@ __device__ inline bool emit_luke(const DevModel& m, CTrack<P>& t, Rng& r, V3<typ
 }
 
 // charge_transport.hpp:169-184
-template <class P> __device__ inline void scatter_iv(const DevModel& m, CTrack<P>& t, Rng& r) {
+template <class P, class G> __device__ inline void scatter_iv(const DevModel& m, CTrack<P>& t, G& r) {
     using R = typename P::R;
     const int nv = m.n_valleys;
     if (t.type != 1 || nv < 2) return;
@@ -222,7 +222,7 @@ template <class P> __device__ inline void scatter_iv(const DevModel& m, CTrack<P
 }
 
 // charge_transport.hpp:187-199
-template <class P> __device__ inline bool should_absorb(const CTrack<P>& t, V3<typename P::R> n, const DevSurface& s, Rng& r) {
+template <class P, class G> __device__ inline bool should_absorb(const CTrack<P>& t, V3<typename P::R> n, const DevSurface& s, G& r) {
     using R = typename P::R;
     const V3<R> out{n.x * R(-1), n.y * R(-1), n.z * R(-1)};
     const R kp = dot(vdiv<P>(t.p, R(kHbarc)), out);
@@ -232,7 +232,7 @@ template <class P> __device__ inline bool should_absorb(const CTrack<P>& t, V3<t
 }
 
 // charge_transport.hpp:203-241
-template <class P> __device__ inline void reflect_charge(const DevModel& m, CTrack<P>& t, V3<typename P::R> n, double spec, Rng& r) {
+template <class P, class G> __device__ inline void reflect_charge(const DevModel& m, CTrack<P>& t, V3<typename P::R> n, double spec, G& r) {
     using R = typename P::R;
     if (P::uniform(r) < R(spec)) {
         const V3<R> kd = unit<P>(t.p);
@@ -256,8 +256,8 @@ template <class P> __device__ inline void reflect_charge(const DevModel& m, CTra
 
 // One pass of the charge step loop. Returns false when the carrier ends (collected,
 // trapped) or exceeds the 20,000,000-step cap (the caller counts steps).
-template <class P, class Ctx>
-__device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, Rng& r, Ctx& ctx) {
+template <class P, class G, class Ctx>
+__device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx& ctx) {
     using R = typename P::R;
     ctx.ccount(CC_STEPS);
     const V3<R> E{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
@@ -320,7 +320,7 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, Rng& r, Ct
             ctx.ccount(CC_TRAP_ION);
             const bool lib_e = (u_proc < R0.trap + R0.ion_e);
             const uint64_t nval = (m.n_valleys > 1) ? static_cast<uint64_t>(m.n_valleys) : 1;
-            const int kv = lib_e ? static_cast<int>(static_cast<uint16_t>(next_u64(r) % nval)) : 0;
+            const int kv = lib_e ? static_cast<int>(static_cast<uint16_t>(next_bits(r) % nval)) : 0;
             const V3<R> d = isotropic_dir<P>(r);
             CTrack<P> k;
             make_charge<P>(m, k, lib_e ? 1 : 2, trk.pos, d, R(1.0e-6), kv, trk.time);

@@ -157,11 +157,54 @@ __device__ inline uint64_t next_u64(Rng& r) {            // rng.hpp:49-59 (xoshi
     return result;
 }
 
+// Transport streams of the batched form: xoshiro128** (Blackman and Vigna), the 32-bit
+// member of the reference's generator family. Its state and every operation of a draw
+// are 32-bit words, which is what the wavefront's VALU executes natively (the 64-bit
+// generator needs two instructions per shift, rotate and xor). One draw gives one FP32
+// uniform. Track `key` of an event with base b is seeded with the first two splitmix64
+// outputs of b ^ mix64(key + C), split into four words; the plain form and the event
+// generators keep the reference's 64-bit streams.
+struct Rng32 {
+    uint32_t s0, s1, s2, s3;
+};
+__host__ __device__ inline Rng32 rng32_for_track(uint64_t base, uint64_t key) {
+    uint64_t z = base ^ mix64(key + 0xd1b54a32d192ed03ULL);
+    uint64_t w[2];
+    for (int i = 0; i < 2; ++i) {
+        z += 0x9e3779b97f4a7c15ULL;
+        uint64_t x = z;
+        x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+        x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+        w[i] = x ^ (x >> 31);
+    }
+    Rng32 r{static_cast<uint32_t>(w[0]), static_cast<uint32_t>(w[0] >> 32), static_cast<uint32_t>(w[1]),
+            static_cast<uint32_t>(w[1] >> 32)};
+    if ((r.s0 | r.s1 | r.s2 | r.s3) == 0u) r.s0 = 1u;   // the all-zero state is a fixed point
+    return r;
+}
+__device__ inline uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+__device__ inline uint32_t next_u32(Rng32& r) {
+    const uint32_t result = rotl32(r.s1 * 5u, 7) * 9u;
+    const uint32_t t = r.s1 << 9;
+    r.s2 ^= r.s0;
+    r.s3 ^= r.s1;
+    r.s1 ^= r.s2;
+    r.s0 ^= r.s3;
+    r.s2 ^= t;
+    r.s3 = rotl32(r.s3, 11);
+    return result;
+}
+// Raw bits of one draw (the charge kernel's valley choice takes them modulo the count).
+__device__ inline uint64_t next_bits(Rng& r) { return next_u64(r); }
+__device__ inline uint32_t next_bits(Rng32& r) { return next_u32(r); }
+
 // ------------------------------------------------------------------ precision policies
 struct Fp64 {
     using R = double;
+    using TRng = Rng;                            // a track's transport stream
     static constexpr bool exact = true;
     static constexpr bool kin_exact = false;     // octahedral LUT kinematics
+    __device__ static Rng track_stream(uint64_t base, uint64_t key) { return rng_for_track(base, key); }
     __device__ static double uniform(Rng& r) { return static_cast<double>((next_u64(r) >> 11) | 1ULL) * 0x1.0p-53; }
     __device__ static double neg_log(double u) { return -log(u); }
     // cos/sin of phi = 2 pi u, evaluated as the reference does (units::two_pi * u)
@@ -187,10 +230,14 @@ struct Fp64 {
 
 struct Fp32 {
     using R = float;
+    using TRng = Rng32;                          // a track's transport stream
     static constexpr bool exact = false;
     static constexpr bool kin_exact = false;
-    // top 24 bits of the same 64-bit draw: (0, 1), never 0 or 1
+    __device__ static Rng32 track_stream(uint64_t base, uint64_t key) { return rng32_for_track(base, key); }
+    // the top 24 bits of a draw: (0, 1), never 0 or 1 (64-bit event streams and 32-bit
+    // transport streams alike)
     __device__ static float uniform(Rng& r) { return static_cast<float>(static_cast<uint32_t>(next_u64(r) >> 40) | 1u) * 0x1.0p-24f; }
+    __device__ static float uniform(Rng32& r) { return static_cast<float>((next_u32(r) >> 8) | 1u) * 0x1.0p-24f; }
     __device__ static float neg_log(float u) { return -0.69314718f * __builtin_amdgcn_logf(u); }
     // v_sin_f32 / v_cos_f32 take their argument in revolutions: sin(2 pi u) directly
     __device__ static void sincos_2pi(float u, float& s, float& c) {
@@ -275,7 +322,7 @@ template <class R> __device__ inline V3<R> mat_mul_t(const double m[3][3], V3<R>
 }
 
 // ------------------------------------------------------------------ samplers (rng.hpp:78-96)
-template <class P> __device__ inline V3<typename P::R> isotropic_dir(Rng& r) {
+template <class P, class G> __device__ inline V3<typename P::R> isotropic_dir(G& r) {
     using R = typename P::R;
     const R ct = R(2) * P::uniform(r) - R(1);
     const R st = P::sqrt_(tmax(R(0), R(1) - ct * ct));
@@ -284,7 +331,7 @@ template <class P> __device__ inline V3<typename P::R> isotropic_dir(Rng& r) {
     return {st * c, st * s, ct};
 }
 
-template <class P> __device__ inline V3<typename P::R> lambertian_dir(Rng& r, V3<typename P::R> n) {
+template <class P, class G> __device__ inline V3<typename P::R> lambertian_dir(G& r, V3<typename P::R> n) {
     using R = typename P::R;
     const R u_phi = P::uniform(r);
     const R st = P::sqrt_(P::uniform(r));
@@ -564,21 +611,21 @@ __device__ inline void map_k_to_vg(const DevModel& m, int mode, V3<typename P::R
 }
 
 // phonon_kernels.hpp:53-63
-template <class P> __device__ inline int sample_dos_mode(const DevModel& m, Rng& r) {
+template <class P, class G> __device__ inline int sample_dos_mode(const DevModel& m, G& r) {
     using R = typename P::R;
     const R u = P::uniform(r);
     if (u < R(m.fL)) return 0;
     if (u < R(m.fL + m.fST)) return 2;
     return 1;
 }
-template <class P> __device__ inline int sample_transverse_mode(const DevModel& m, Rng& r) {
+template <class P, class G> __device__ inline int sample_transverse_mode(const DevModel& m, G& r) {
     using R = typename P::R;
     const double p_st = m.fST / (m.fST + m.fFT);
     return (P::uniform(r) < R(p_st)) ? 2 : 1;
 }
 
 // crystal_lattice.hpp:200-212 (inverse-CDF tables; `cdf` may point to LDS)
-template <class P> __device__ inline double sample_fraction(const double* cdf, Rng& r) {
+template <class P, class G> __device__ inline double sample_fraction(const double* cdf, G& r) {
     using R = typename P::R;
     const R u = P::uniform(r) * R(CMPMC_N_CDF - 1);
     const int i = tmin(static_cast<int>(u), CMPMC_N_CDF - 2);
@@ -752,13 +799,14 @@ template <class P> __device__ inline double film_escape_probability(const DevFil
     }
 }
 
-template <class P> __device__ inline bool film_direct(const DevFilm& f, double E, Rng& r) {
+template <class P, class G> __device__ inline bool film_direct(const DevFilm& f, double E, G& r) {
     using R = typename P::R;
     if (E < 2.0 * f.absorber_gap) return false;
     return P::uniform(r) < R(f.subgap_absorption);
 }
 
-template <class P> __device__ inline double film_qp_absorption(const DevFilm& f, double qpE, KList& phon, KList& qp, Rng& r) {
+template <class P, class G>
+__device__ inline double film_qp_absorption(const DevFilm& f, double qpE, KList& phon, KList& qp, G& r) {
     using R = typename P::R;
     R eff0;
     if constexpr (P::exact) eff0 = (f.gap > 0.0) ? (f.absorber_eff + f.absorber_eff_slope * (qpE / f.gap)) : 1.0;
@@ -776,7 +824,7 @@ template <class P> __device__ inline double film_qp_absorption(const DevFilm& f,
     return qpE;
 }
 
-template <class P> __device__ inline double film_sample_qp(const DevFilm& f, double E, Rng& r) {   // :209-227
+template <class P, class G> __device__ inline double film_sample_qp(const DevFilm& f, double E, G& r) {   // :209-227
     using R = typename P::R;
     const double d = f.gap;
     const double span = E - 2.0 * d;
@@ -804,7 +852,7 @@ template <class P> __device__ inline double film_sample_qp(const DevFilm& f, dou
     }
 }
 
-template <class P> __device__ inline double film_sample_phonon(const DevFilm& f, double E, Rng& r) {   // :269-289
+template <class P, class G> __device__ inline double film_sample_phonon(const DevFilm& f, double E, G& r) {   // :269-289
     const double d = f.gap;
     const double span = E - d;
     if (span <= 1e-16) return 0.0;
@@ -847,8 +895,8 @@ template <class P> __device__ inline double film_sample_phonon(const DevFilm& f,
 // KaplanFilm::absorb_phonon (Seed::Phonon, MeanFreePath loss): returns edep, fills `out`.
 // lists[0..3] provide the storage of the qp / phon / next_phon / next_qp buffers; the
 // generation swaps are done on named copies so that no list is indexed at run time.
-template <class P>
-__device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KList& out, const KList* lists, Rng& r) {
+template <class P, class G>
+__device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KList& out, const KList* lists, G& r) {
     using R = typename P::R;
     out.n = 0;
     if (film_direct<P>(f, energy, r)) return energy;

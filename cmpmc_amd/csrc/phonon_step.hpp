@@ -27,7 +27,7 @@ template <class P> struct PTrack {
     double energy, time, weight;
     uint32_t bounces;
     int mode;
-    Rng rng;
+    typename P::TRng rng;
     uint64_t key;
     uint32_t spawn_no, hit_seq;
 };
@@ -49,8 +49,8 @@ __device__ inline void make_track(const DevModel& m, PTrack<P>& t, V3<typename P
 }
 
 // engine.hpp:927-936
-template <class P>
-__device__ inline V3<typename P::R> sample_inward_wavevector(const DevModel& m, int mode, V3<typename P::R> n, Rng& r) {
+template <class P, class G>
+__device__ inline V3<typename P::R> sample_inward_wavevector(const DevModel& m, int mode, V3<typename P::R> n, G& r) {
     using R = typename P::R;
     for (int trial = 0; trial < 16; ++trial) {
         const V3<R> kd = lambertian_dir<P>(r, n);
@@ -64,8 +64,8 @@ __device__ inline V3<typename P::R> sample_inward_wavevector(const DevModel& m, 
 
 // engine.hpp:938-988, up to the kept phonon's lookup: pushes the daughter, scales the
 // track's energy and returns the kept phonon's mode and wavevector.
-template <class P, class Ctx>
-__device__ inline void anharmonic_decay_split(const DevModel& m, PTrack<P>& trk, Rng& r, Ctx& ctx, int& m_keep,
+template <class P, class G, class Ctx>
+__device__ inline void anharmonic_decay_split(const DevModel& m, PTrack<P>& trk, G& r, Ctx& ctx, int& m_keep,
                                               V3<typename P::R>& k_keep) {
     using R = typename P::R;
     const V3<R> dir1 = trk.k;
@@ -106,8 +106,8 @@ __device__ inline void anharmonic_decay_split(const DevModel& m, PTrack<P>& trk,
 }
 
 // engine.hpp:938-988
-template <class P, class Ctx>
-__device__ inline void anharmonic_decay(const DevModel& m, PTrack<P>& trk, Rng& r, Ctx& ctx) {
+template <class P, class G, class Ctx>
+__device__ inline void anharmonic_decay(const DevModel& m, PTrack<P>& trk, G& r, Ctx& ctx) {
     int m_keep;
     V3<typename P::R> k_keep;
     anharmonic_decay_split<P>(m, trk, r, ctx, m_keep, k_keep);

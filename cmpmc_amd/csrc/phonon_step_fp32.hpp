@@ -37,7 +37,7 @@ enum BulkProcess : int { kProcIso = 0, kProcDecay = 1, kProcPoly = 2 };
 template <class P, class Ctx>
 __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film,
                                        bool alive) {
-    Rng& r = trk.rng;
+    auto& r = trk.rng;
     int code = alive ? kStepAlive : kStepDead;
     // the direction and mode the closing lookup uses; a lane without a track looks up a
     // fixed direction, a lane whose track ends its unchanged one
@@ -219,7 +219,7 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
 // the continuing track, which the caller re-queues. Returns false if nothing re-emits.
 template <class P, class Ctx>
 __device__ inline bool film_job(const DevModel& m, PTrack<P>& trk, V3<float> n_in, int sid, Ctx& ctx) {
-    Rng& r = trk.rng;
+    auto& r = trk.rng;
     const DevFilm& film = m.surf[sid].film;
     double* wb = ctx.kaplan_base();
     const uint32_t ws = ctx.kaplan_stride();

@@ -6,7 +6,7 @@
 // empty it takes new tracks from the global initial queue.  Refills are wave-aggregated:
 // one ballot counts the idle lanes of the wave and a single atomic claims that many queue
 // entries, every `refill_period` steps (or at once when the whole wave is idle).
-// Every track draws from its own stream rng_for_track(event_base(seed, event), key), so
+// Every track draws from its own stream P::track_stream(event_base(seed, event), key), so
 // results do not depend on which lane runs which track, or when.
 #include <algorithm>
 
@@ -33,7 +33,7 @@ __device__ inline void btrack_to_ptrack(const BTrack& b, uint64_t seed, PTrack<P
     t.spawn_no = 0;
     t.hit_seq = 0;
     event = b.event;
-    t.rng = rng_for_track(event_base(seed, event), b.key);
+    t.rng = P::track_stream(event_base(seed, event), b.key);
 }
 
 template <class T> __device__ inline void ptrack_to_btrack(const PTrack<T>& t, uint64_t event, BTrack& b) {
@@ -49,8 +49,8 @@ template <class T> __device__ inline void ptrack_to_btrack(const PTrack<T>& t, u
     b.bounces_mode = (t.bounces << 8) | static_cast<uint32_t>(t.mode);
 }
 
-static_assert(sizeof(SRec) == 144, "SRec layout");
-static_assert(sizeof(SRecD) == 176, "SRecD layout");
+static_assert(sizeof(SRec) == 128, "SRec layout");
+static_assert(sizeof(SRecD) == 160, "SRecD layout");
 static_assert(sizeof(TapeRec) == 32, "TapeRec layout");
 
 template <class P>
@@ -67,7 +67,7 @@ __device__ inline void srec_to_ptrack(const SRec& b, PTrack<P>& t, uint64_t& eve
     t.key = b.key;
     t.spawn_no = b.spawn_no;
     t.hit_seq = b.hit_seq;
-    t.rng = Rng{b.rng[0], b.rng[1], b.rng[2], b.rng[3]};
+    t.rng = typename P::TRng{b.rng[0], b.rng[1], b.rng[2], b.rng[3]};
     event = b.event;
 }
 
@@ -200,11 +200,11 @@ template <bool Diff, class P> struct BatchCtx {
         ptrack_to_srec(t, event, Ops::base(r));
         Ops::store_sc(r, sc);
     }
-    // A daughter: key n of its parent, stream rng_for_track(event_base, key); it inherits
+    // A daughter: key n of its parent, stream P::track_stream(event_base, key); it inherits
     // the parent's scores at the spawn.
     __device__ void push(PTrack<P>& child, PTrack<P>& parent) {
         child.key = child_key(parent.key, ++parent.spawn_no);
-        child.rng = rng_for_track(event_base(seed, event), child.key);
+        child.rng = P::track_stream(event_base(seed, event), child.key);
         child.spawn_no = 0;
         child.hit_seq = 0;
         store(child);
@@ -496,7 +496,7 @@ __device__ inline void bcharge_to_ctrack(const BCharge& b, uint64_t seed, CTrack
     c.spawn_no = 0;
     c.hit_seq = 0;
     event = b.event;
-    c.rng = rng_for_track(event_base(seed, event), b.key);
+    c.rng = P::track_stream(event_base(seed, event), b.key);
 }
 template <class T> __device__ inline void ctrack_to_bcharge(const CTrack<T>& c, uint64_t event, BCharge& b) {
     b.pos[0] = static_cast<float>(c.pos.x); b.pos[1] = static_cast<float>(c.pos.y); b.pos[2] = static_cast<float>(c.pos.z);

@@ -71,8 +71,9 @@ struct alignas(16) BCharge {
 };
 
 // Spawn-pool entry and deferred film interaction of the batched phonon kernel
-// (144 B): the complete track state including its random stream, so that a track
-// continues exactly where it stopped. n_in / sid are the film entry normal and surface.
+// (128 B): the complete track state including its random stream (xoshiro128** words), so
+// that a track continues exactly where it stopped. n_in / sid are the film entry normal
+// and surface.
 struct alignas(16) SRec {
     float pos[3];
     float k[3];
@@ -80,7 +81,7 @@ struct alignas(16) SRec {
     float vg_mag;
     double energy, time, weight;
     uint64_t key;
-    uint64_t rng[4];
+    uint32_t rng[4];
     uint32_t event;
     uint32_t bounces_mode;             // bounces << 8 | mode
     uint32_t spawn_no, hit_seq;

@@ -32,27 +32,34 @@ extern "C" {
 #endif
 
 /* ---- RNG: core/rng.hpp:17-97 ---------------------------------------------------- */
-typedef struct { uint64_t s[4]; } orc_rng;
+/* s: xoshiro256** state; w32 != 0: a transport stream of the batched form, xoshiro128**
+ * with its four 32-bit words in s[0] (low, high) and s[1] (low, high) */
+typedef struct { uint64_t s[4]; uint64_t w32; } orc_rng;
 
 uint64_t orc_mix64(uint64_t x);
 void     orc_rng_seed(orc_rng* r, uint64_t seed);                  /* rng.hpp:20-29 */
 void     orc_rng_for_stream(orc_rng* r, uint64_t seed, uint64_t stream); /* rng.hpp:34-36 */
 uint64_t orc_next_u64(orc_rng* r);                                 /* rng.hpp:49-59 */
-double   orc_uniform(orc_rng* r);                                  /* rng.hpp:62-64 */
+uint32_t orc_next_u32(orc_rng* r);                                 /* xoshiro128** (w32 streams) */
+uint64_t orc_next_bits(orc_rng* r);                                /* one draw: u64, or u32 if w32 */
+double   orc_uniform(orc_rng* r);          /* rng.hpp:62-64; w32: ((u32 >> 8) | 1) * 2^-24 */
 double   orc_normal(orc_rng* r);                                   /* rng.hpp:71-75 */
 void     orc_isotropic_dir(orc_rng* r, double out[3]);             /* rng.hpp:78-83 */
 void     orc_lambertian_dir(orc_rng* r, const double n[3], double out[3]); /* rng.hpp:86-96 */
 
 /* Sub-stream scheme of the batched form (no reference counterpart: the reference draws
- * every track of an event from one sequential stream). Track `key` of event `event`
- * gets Rng(event_base ^ mix64(key + 0xd1b54a32d192ed03)), with
+ * every track of an event from one sequential stream). Track `key` of event `event` is
+ * transported with the xoshiro128** stream whose four words are the first two splitmix64
+ * outputs of event_base ^ mix64(key + 0xd1b54a32d192ed03) (orc_rng32_for_track), with
  * event_base = mix64(mix64(seed) ^ mix64(event + 0x632be59bd9b4e019)).  A daughter
  * spawned as the n-th child (n = 1, 2, ...) of a track with key K gets key
  * mix64(K ^ (n * 0x9e3779b97f4a7c15)).  Device-generated initial phonons of the
- * lognormal / uniform-count workloads draw their initial state from the generator
- * stream of key j | 2^62 and are then transported with the stream of key j. */
+ * lognormal / uniform-count workloads draw their initial state from the 64-bit generator
+ * stream Rng(event_base ^ mix64((j | 2^62) + 0xd1b54a32d192ed03)) (orc_rng_for_track)
+ * and are then transported with the stream of key j. */
 uint64_t orc_event_base(uint64_t seed, uint64_t event);
 void     orc_rng_for_track(orc_rng* r, uint64_t event_base, uint64_t key);
+void     orc_rng32_for_track(orc_rng* r, uint64_t event_base, uint64_t key);
 uint64_t orc_child_key(uint64_t parent_key, uint32_t spawn_no);
 
 /* ---- Materials: materials/crystal_config.hpp:43-257, presets.hpp:18-25, 360-362 ---- */

@@ -27,6 +27,7 @@ void orc_rng_seed(orc_rng* r, uint64_t seed) {   /* rng.hpp:20-29: four splitmix
         x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
         r->s[i] = x ^ (x >> 31);
     }
+    r->w32 = 0;
 }
 
 void orc_rng_for_stream(orc_rng* r, uint64_t seed, uint64_t stream) {  /* rng.hpp:34-36 */
@@ -48,7 +49,27 @@ uint64_t orc_next_u64(orc_rng* r) {             /* rng.hpp:49-59 (xoshiro256**) 
     return result;
 }
 
+static inline uint32_t rotl32(uint32_t x, int k) { return (x << k) | (x >> (32 - k)); }
+
+uint32_t orc_next_u32(orc_rng* r) {             /* xoshiro128** (Blackman & Vigna) */
+    uint32_t s[4] = {(uint32_t)r->s[0], (uint32_t)(r->s[0] >> 32), (uint32_t)r->s[1], (uint32_t)(r->s[1] >> 32)};
+    const uint32_t result = rotl32(s[1] * 5u, 7) * 9u;
+    const uint32_t t = s[1] << 9;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl32(s[3], 11);
+    r->s[0] = (uint64_t)s[0] | ((uint64_t)s[1] << 32);
+    r->s[1] = (uint64_t)s[2] | ((uint64_t)s[3] << 32);
+    return result;
+}
+
+uint64_t orc_next_bits(orc_rng* r) { return r->w32 ? (uint64_t)orc_next_u32(r) : orc_next_u64(r); }
+
 double orc_uniform(orc_rng* r) {                 /* rng.hpp:62-64: (0,1), never 0 */
+    if (r->w32) return (double)((orc_next_u32(r) >> 8) | 1u) * 0x1.0p-24;
     return (double)((orc_next_u64(r) >> 11) | 1ULL) * 0x1.0p-53;
 }
 
@@ -85,6 +106,13 @@ uint64_t orc_event_base(uint64_t seed, uint64_t event) {
 
 void orc_rng_for_track(orc_rng* r, uint64_t event_base, uint64_t key) {
     orc_rng_seed(r, event_base ^ orc_mix64(key + 0xd1b54a32d192ed03ULL));
+}
+
+void orc_rng32_for_track(orc_rng* r, uint64_t event_base, uint64_t key) {
+    orc_rng_seed(r, event_base ^ orc_mix64(key + 0xd1b54a32d192ed03ULL));   /* words from s[0], s[1] */
+    r->s[2] = r->s[3] = 0;
+    r->w32 = 1;
+    if (r->s[0] == 0 && r->s[1] == 0) r->s[0] = 1;   /* the all-zero state is a fixed point */
 }
 
 uint64_t orc_child_key(uint64_t parent_key, uint32_t spawn_no) {

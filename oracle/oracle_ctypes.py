@@ -25,7 +25,7 @@ dbl = C.c_double
 
 
 class Rng(C.Structure):
-    _fields_ = [("s", u64 * 4)]
+    _fields_ = [("s", u64 * 4), ("w32", u64)]
 
 
 class Crystal(C.Structure):
@@ -211,6 +211,11 @@ def lib():
         L.orc_child_key.restype = u64
         L.orc_child_key.argtypes = [u64, u32]
         L.orc_rng_for_track.argtypes = [C.POINTER(Rng), u64, u64]
+        L.orc_rng32_for_track.argtypes = [C.POINTER(Rng), u64, u64]
+        L.orc_next_u32.restype = u32
+        L.orc_next_u32.argtypes = [C.POINTER(Rng)]
+        L.orc_next_bits.restype = u64
+        L.orc_next_bits.argtypes = [C.POINTER(Rng)]
         L.orc_map_k_to_vg.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_map_k_to_vg_exact.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_solve_christoffel_cardano.argtypes = [C.c_void_p, dbl, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]

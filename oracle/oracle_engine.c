@@ -76,7 +76,7 @@ static void push_daughter(orc_track_vec* stack, orc_track* child, orc_track* par
                           int per_track, uint64_t ev_base) {
     if (per_track) {
         child->key = orc_child_key(parent->key, ++parent->spawn_no);
-        orc_rng_for_track(&child->rng, ev_base, child->key);
+        orc_rng32_for_track(&child->rng, ev_base, child->key);
         child->spawn_no = 0;
         child->hit_seq = 0;
     }
@@ -617,7 +617,7 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
                     st->trap_ionizations++;
                     const int lib_e = (u_proc < r_trap + r_ion_e);
                     const uint64_t nval = (L->cfg.n_valleys > 1) ? (uint64_t)L->cfg.n_valleys : 1;
-                    const int kv = lib_e ? (int)(uint16_t)(orc_next_u64(R) % nval) : 0;
+                    const int kv = lib_e ? (int)(uint16_t)(orc_next_bits(R) % nval) : 0;
                     double d[3];
                     orc_isotropic_dir(R, d);
                     orc_charge k;
@@ -625,7 +625,7 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
                     k.weight = trk.weight;
                     if (per_track) {
                         k.key = orc_child_key(trk.key, ++trk.spawn_no);
-                        orc_rng_for_track(&k.rng, ev_base, k.key);
+                        orc_rng32_for_track(&k.rng, ev_base, k.key);
                     }
                     orc_charge_push(charges, &k);
                 } else if (u_proc < r_trap + r_ion_e + r_ion_h + r_iv) {
@@ -642,7 +642,7 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
                             orc_make_track(e, trk.pos, qd, m, eph, trk.time, trk.weight * luke_wt_mult, &ph);
                             if (per_track) {
                                 ph.key = orc_child_key(trk.key, ++trk.spawn_no);
-                                orc_rng_for_track(&ph.rng, ev_base, ph.key);
+                                orc_rng32_for_track(&ph.rng, ev_base, ph.key);
                             }
                             orc_track_push(phonons, &ph);
                         }
@@ -694,7 +694,7 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
         orc_isotropic_dir(ev_rng, k);
         orc_track t;
         orc_make_track(e, origin, k, mode, energy, 0.0, 1.0, &t);
-        if (batched) { t.key = 0; orc_rng_for_track(&t.rng, base, 0); }
+        if (batched) { t.key = 0; orc_rng32_for_track(&t.rng, base, 0); }
         t.first_flag = 1;
         orc_track_push(phonons, &t);
         *injected += energy;
@@ -731,7 +731,7 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
             orc_isotropic_dir(r, k);
             orc_track t;
             orc_make_track(e, pos, k, m, E, 0.0, 1.0, &t);
-            if (batched) { t.key = j; orc_rng_for_track(&t.rng, base, j); }
+            if (batched) { t.key = j; orc_rng32_for_track(&t.rng, base, j); }
             /* plain: the first popped track is the last pushed; batched: key 0 */
             t.first_flag = batched ? (j == 0) : (j + 1 == n);
             orc_track_push(phonons, &t);
@@ -745,7 +745,7 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
         if (batched)
             for (size_t i = n0; i < charges->n; ++i) {
                 charges->v[i].key = 0x8000000000000000ULL | (uint64_t)(i - n0);
-                orc_rng_for_track(&charges->v[i].rng, base, charges->v[i].key);
+                orc_rng32_for_track(&charges->v[i].rng, base, charges->v[i].key);
             }
     }
     return phonons->n;

@@ -44,6 +44,29 @@ def test_uniform_open_interval(oracle):
     assert abs(u.mean() - 0.5) < 0.01
 
 
+def test_transport_stream_xoshiro128(oracle):
+    """The batched form's transport streams: xoshiro128** (Blackman & Vigna). From the
+    state (1, 2, 3, 4) its first outputs are rotl(2 * 5, 7) * 9 = 11520, then 0 (s1 is
+    2 ^ (3 ^ 1) = 0 after one step), then rotl(1029 * 5, 7) * 9 = 5927040, worked by hand
+    from the published update; uniforms are ((u >> 8) | 1) * 2^-24 in (0, 1)."""
+    O = oracle
+    L = O.lib()
+    r = O.Rng()
+    r.s[0] = 1 | (2 << 32)
+    r.s[1] = 3 | (4 << 32)
+    r.w32 = 1
+    assert [L.orc_next_u32(C.byref(r)) for _ in range(3)] == [11520, 0, 5927040]
+    assert L.orc_next_bits(C.byref(r)) < 2 ** 32
+    # a track stream: the first two splitmix64 outputs of its seed, as four 32-bit words
+    a, b = O.Rng(), O.Rng()
+    L.orc_rng_for_track(C.byref(a), 0x1234, 77)
+    L.orc_rng32_for_track(C.byref(b), 0x1234, 77)
+    assert b.w32 == 1 and (b.s[0], b.s[1]) == (a.s[0], a.s[1])
+    u = np.array([L.orc_uniform(C.byref(b)) for _ in range(20000)])
+    assert (u > 0).all() and (u < 1).all() and abs(u.mean() - 0.5) < 0.01
+    assert np.all(np.round(u * 2 ** 24) == u * 2 ** 24)       # 24-bit uniforms
+
+
 def test_samplers_golden(oracle, gold):
     O = oracle
     L = O.lib()
