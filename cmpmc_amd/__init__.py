@@ -290,7 +290,7 @@ class Engine:
             rc = call(out)
             wall = time.perf_counter() - t0
             if rc == capi.ERR_CAPACITY and keep_hits and out.hits_needed > cap:
-                cap = int(out.hits_needed)      # deterministic: rerun with room for every hit
+                cap = int(out.hits_needed)      # rerun with the room the library asks for
                 continue
             check(rc)
             res = {"stats": out.stats.as_dict(), "stats_raw": bytes(out.stats), "hits": hits[:out.n_hits].copy(),

@@ -353,10 +353,11 @@ __global__ void gather_hits_kernel(const DevHit* h, const uint32_t* idx, uint64_
 
 inline unsigned nblk(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
-// Sorts the first n device hits by (event, track key, sequence) and writes them to the
-// host buffer; per-event counts become hit_offsets.
-int order_hits(cmpmc_model* M, const DevHit* hits, uint64_t n, bool plain, uint64_t first_event, uint64_t n_events,
-               cmpmc_sensor_hit* host, uint64_t* offsets, hipStream_t s) {
+// Sorts the first n device hit slots by (event, track key, sequence) and writes the
+// first n_out of them (the hits; unused slots carry kNoHitKey and sort last) to the host
+// buffer; per-event counts become hit_offsets.
+int order_hits(cmpmc_model* M, const DevHit* hits, uint64_t n, uint64_t n_out, bool plain, uint64_t first_event,
+               uint64_t n_events, cmpmc_sensor_hit* host, uint64_t* offsets, hipStream_t s) {
     CHK(ensure(M->counts, (n_events + 1) * sizeof(uint32_t)));
     HIPCHK(hipMemsetAsync(M->counts.p, 0, (n_events + 1) * sizeof(uint32_t), s));
     uint32_t* idx_final = nullptr;
@@ -391,7 +392,7 @@ int order_hits(cmpmc_model* M, const DevHit* hits, uint64_t n, bool plain, uint6
                               first_event, n_events, as<cmpmc_sensor_hit>(M->sorted), as<uint32_t>(M->counts));
     HIPCHK(hipGetLastError());
     std::vector<uint32_t> counts(n_events);
-    if (n && host) HIPCHK(hipMemcpyAsync(host, M->sorted.p, n * sizeof(cmpmc_sensor_hit), hipMemcpyDeviceToHost, s));
+    if (n_out && host) HIPCHK(hipMemcpyAsync(host, M->sorted.p, n_out * sizeof(cmpmc_sensor_hit), hipMemcpyDeviceToHost, s));
     if (n_events) HIPCHK(hipMemcpyAsync(counts.data(), M->counts.p, n_events * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (offsets) {
@@ -611,7 +612,7 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
         return fail(CMPMC_ERR_CAPACITY, "hit buffer too small: %llu hits produced, capacity %llu", produced,
                     static_cast<unsigned long long>(dev_hit_cap));
     const uint64_t nh = keep ? produced : 0;
-    CHK(order_hits(M, as<DevHit>(M->hits), nh, true, J.first_event, N, O.hits, O.hit_offsets, s));
+    CHK(order_hits(M, as<DevHit>(M->hits), nh, nh, true, J.first_event, N, O.hits, O.hit_offsets, s));
     if (O.n_hits) *O.n_hits = nh;
     return CMPMC_OK;
 }
@@ -691,6 +692,8 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         if (O.passes) ++*O.passes;
         Acc zero{};
         HIPCHK(hipMemcpyAsync(d_acc, &zero, sizeof(Acc), hipMemcpyHostToDevice, s));
+        // hit slots are reserved a chunk per wavefront: the slots no hit fills keep kNoHitKey
+        if (keep) HIPCHK(cmpmc_launch_clear_hit_keys(as<DevHit>(M->hits), dev_hit_cap, s));
         CHK(ensure(M->queue, std::max<uint64_t>(n_init + luke_cap, 1) * sizeof(BTrack)));
         BTrack* q = as<BTrack>(M->queue);
         unsigned long long inj_fixed = 0;
@@ -744,6 +747,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ca.keep_hits = keep ? 1 : 0;
             ca.hits = as<DevHit>(M->hits);
             ca.hit_count = &d_acc->hit_count;
+            ca.hit_valid = &d_acc->hit_valid;
             ca.hit_cap = dev_hit_cap;
             ca.counters = d_acc->counters;
             ca.ledger = d_acc->ledger;
@@ -780,6 +784,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.keep_hits = keep ? 1 : 0;
             ba.hits = as<DevHit>(M->hits);
             ba.hit_count = &d_acc->hit_count;
+            ba.hit_valid = &d_acc->hit_valid;
             ba.hit_cap = dev_hit_cap;
             ba.first_branch = C.want_first_branch ? as<int32_t>(M->fb) : nullptr;
             ba.counters = d_acc->counters;
@@ -789,8 +794,11 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.cancel = C.poll_cancel ? M->d_cancel : nullptr;
             if (D) {
                 CHK(ensure(*D->tape, tape_cap * sizeof(TapeRec)));
+                // tape slots are reserved a chunk per wavefront: unused ones stay all-zero
+                HIPCHK(cmpmc_launch_zero_words(as<unsigned long long>(*D->tape), tape_cap * (sizeof(TapeRec) / 8), s));
                 ba.tape = as<TapeRec>(*D->tape);
                 ba.tape_count = &d_acc->tape_count;
+                ba.tape_valid = &d_acc->tape_valid;
                 ba.tape_cap = tape_cap;
                 ba.bins_per_s = D->bins_per_s;
                 ba.k_f = D->k_f;
@@ -820,7 +828,10 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             tape_cap = res.tape_count + res.tape_count / 8 + 1024;
             continue;
         }
-        if (D) D->n_tape = res.tape_count;
+        if (D) {
+            D->n_tape = res.tape_count;
+            D->n_tape_hits = res.tape_valid;
+        }
         res.ledger[2 * L_INJ] += inj_fixed >> 20;
         res.ledger[2 * L_INJ + 1] += inj_fixed & 0xFFFFFull;
         add_acc(tot, res);
@@ -832,15 +843,22 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     (void)hipEventDestroy(e2);
     if (rc != CMPMC_OK) return rc;
     if (O.kernel_ms) *O.kernel_ms += ms_total;
-    unsigned long long produced = 0;
-    HIPCHK(hipMemcpy(&produced, &d_acc->hit_count, 8, hipMemcpyDeviceToHost));
-    if (O.hits_needed) *O.hits_needed = keep ? produced : 0;
+    unsigned long long slots[2] = {0, 0};    // reserved hit slots, hits written
+    HIPCHK(hipMemcpy(&slots[0], &d_acc->hit_count, 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&slots[1], &d_acc->hit_valid, 8, hipMemcpyDeviceToHost));
+    const unsigned long long reserved = slots[0], produced = slots[1];
+    // room to ask for when the slots overflowed: the hits, plus the chunk rests the
+    // wavefronts leave unused (at most one partly filled chunk per reservation, and the
+    // reservations are at least as many as reserved / kHitChunk)
+    const unsigned long long room = reserved + reserved / 4 + static_cast<unsigned long long>(kHitChunk) * (lanes / 64);
+    if (O.hits_needed) *O.hits_needed = keep ? (reserved > dev_hit_cap ? room : produced) : 0;
     apply_totals(tot, *O.stats);
-    if (keep && produced > dev_hit_cap)
-        return fail(CMPMC_ERR_CAPACITY, "hit buffer too small: %llu hits produced, capacity %llu", produced,
-                    static_cast<unsigned long long>(dev_hit_cap));
+    if (keep && reserved > dev_hit_cap)
+        return fail(CMPMC_ERR_CAPACITY, "hit buffer too small: %llu hit slots reserved (%llu hits), capacity %llu",
+                    reserved, produced, static_cast<unsigned long long>(dev_hit_cap));
+    const uint64_t n_slots = keep ? reserved : 0;
     const uint64_t nh = keep ? produced : 0;
-    CHK(order_hits(M, as<DevHit>(M->hits), nh, false, J.first_event, N, O.hits, O.hit_offsets, s));
+    CHK(order_hits(M, as<DevHit>(M->hits), n_slots, nh, false, J.first_event, N, O.hits, O.hit_offsets, s));
     if (O.n_hits) *O.n_hits = nh;
     return CMPMC_OK;
 }
@@ -969,7 +987,8 @@ extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint
 struct cmpmc_diff_tape {
     cmpmc_model* model = nullptr;
     DevBuf tape, gbuf, partials;
-    uint64_t n = 0;
+    uint64_t n = 0;          // tape slots the reductions run over (unused ones are all-zero)
+    uint64_t n_hits = 0;     // hit records among them
     double inv_theta[4] = {0, 0, 0, 0};
 };
 
@@ -1039,6 +1058,7 @@ extern "C" int cmpmc_diff_forward(cmpmc_model* M, const cmpmc_workload* wl, uint
     const int rc = (rc0 == CMPMC_OK) ? run_batched(M, J, C, O, s, &D) : rc0;
     if (rc != CMPMC_OK) { T->tape.release(); delete T; return rc; }
     T->n = D.n_tape;
+    T->n_hits = D.n_tape_hits;
     CHK(ensure(T->gbuf, (2 * kDiffOut + 8) * sizeof(double)));
     unsigned long long* d_y = as<unsigned long long>(T->gbuf) + kDiffOut + 8;
     hipEvent_t e0, e1;
@@ -1090,7 +1110,7 @@ extern "C" int cmpmc_diff_backward(cmpmc_diff_tape* T, const double* g, double* 
     return CMPMC_OK;
 }
 
-extern "C" uint64_t cmpmc_diff_tape_size(const cmpmc_diff_tape* T) { return T ? T->n : 0; }
+extern "C" uint64_t cmpmc_diff_tape_size(const cmpmc_diff_tape* T) { return T ? T->n_hits : 0; }
 
 extern "C" int cmpmc_diff_tape_free(cmpmc_diff_tape* T) {
     if (!T) return CMPMC_OK;

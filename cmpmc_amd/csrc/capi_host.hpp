@@ -89,6 +89,8 @@ struct Acc {
     unsigned long long head;
     unsigned long long chead;
     unsigned long long tape_count;
+    unsigned long long hit_valid;      // batched form: hits written (hit_count counts reserved slots)
+    unsigned long long tape_valid;     // tape records written (tape_count counts reserved slots)
 };
 
 struct Totals {
@@ -188,7 +190,8 @@ struct DiffCfg {
     DevModel dm;
     double k_f = 0.0, k_p = 0.0, bins_per_s = 0.0;
     DevBuf* tape = nullptr;
-    uint64_t n_tape = 0;            // out: tape records written
+    uint64_t n_tape = 0;            // out: tape slots reserved (unused ones are all-zero records)
+    uint64_t n_tape_hits = 0;       // out: tape records written
 };
 
 // Register-budget variant of the persistent kernels when the options leave it open.
@@ -200,8 +203,8 @@ constexpr uint32_t kChargeDepth = 64;
 
 // Sorts the first n device hits by (event, track key, sequence) into the host buffer;
 // per-event counts become hit_offsets (n_events + 1 entries).
-int order_hits(cmpmc_model* M, const DevHit* hits, uint64_t n, bool plain, uint64_t first_event, uint64_t n_events,
-               cmpmc_sensor_hit* host, uint64_t* offsets, hipStream_t s);
+int order_hits(cmpmc_model* M, const DevHit* hits, uint64_t n, uint64_t n_out, bool plain, uint64_t first_event,
+               uint64_t n_events, cmpmc_sensor_hit* host, uint64_t* offsets, hipStream_t s);
 // Persistent grid of a kernel (which: 0 phonon, 1 charge, 2 differentiable phonon).
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves);
 int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s,

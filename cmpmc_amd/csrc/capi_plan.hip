@@ -57,6 +57,7 @@ int enqueue(cmpmc_plan* P, hipStream_t s) {
     const cmpmc_workload* wl = as<cmpmc_workload>(P->d_wl);
     BTrack* q = as<BTrack>(P->queue);
     HIPCHK(cmpmc_launch_zero_words(reinterpret_cast<unsigned long long*>(acc), sizeof(Acc) / 8, s));
+    if (P->keep) HIPCHK(cmpmc_launch_clear_hit_keys(as<DevHit>(P->hits), P->hit_cap, s));
     const unsigned long long* n_init_dev = nullptr;
     if (!P->pair) {
         HIPCHK(cmpmc_launch_workload_count(wl, 0, 0, N, as<uint32_t>(P->counts), dyn, s));
@@ -81,6 +82,7 @@ int enqueue(cmpmc_plan* P, hipStream_t s) {
         ca.keep_hits = P->keep ? 1 : 0;
         ca.hits = as<DevHit>(P->hits);
         ca.hit_count = &acc->hit_count;
+        ca.hit_valid = &acc->hit_valid;
         ca.hit_cap = P->hit_cap;
         ca.counters = acc->counters;
         ca.ledger = acc->ledger;
@@ -106,6 +108,7 @@ int enqueue(cmpmc_plan* P, hipStream_t s) {
         ba.keep_hits = P->keep ? 1 : 0;
         ba.hits = as<DevHit>(P->hits);
         ba.hit_count = &acc->hit_count;
+        ba.hit_valid = &acc->hit_valid;
         ba.hit_cap = P->hit_cap;
         ba.counters = acc->counters;
         ba.ledger = acc->ledger;
@@ -248,8 +251,8 @@ int plan_run(cmpmc_plan* P, uint64_t seed, uint64_t first_event, cmpmc_run_out* 
             P->depth = std::max<uint64_t>(P->depth * 2, res.status[1] + 64);
             again = true;
         }
-        if (!again && P->keep && res.hit_count > P->hit_cap) { // device hit buffer too small
-            P->hit_cap = res.hit_count + res.hit_count / 8 + 1024;
+        if (!again && P->keep && res.hit_count > P->hit_cap) { // device hit slots too few
+            P->hit_cap = res.hit_count + res.hit_count / 4 + static_cast<uint64_t>(kHitChunk) * (P->grid * P->block / 64);
             again = true;
         }
         if (!again) break;
@@ -259,7 +262,7 @@ int plan_run(cmpmc_plan* P, uint64_t seed, uint64_t first_event, cmpmc_run_out* 
     add_acc(tot, res);
     apply_totals(tot, out->stats);
     out->kernel_ms += ms_total;
-    const uint64_t produced = P->keep ? res.hit_count : 0;
+    const uint64_t produced = P->keep ? res.hit_valid : 0;   // hits written (hit_count: slots)
     out->hits_needed = produced;
     if (!P->keep || !out->hits) {
         out->n_hits = 0;
@@ -270,7 +273,7 @@ int plan_run(cmpmc_plan* P, uint64_t seed, uint64_t first_event, cmpmc_run_out* 
                     static_cast<unsigned long long>(produced), static_cast<unsigned long long>(out->hit_capacity));
     {
         std::lock_guard<std::mutex> lk(P->M->mu);   // the ordering uses the model's sort buffers
-        CHK(order_hits(P->M, as<DevHit>(P->hits), produced, false, first_event, P->n_events, out->hits,
+        CHK(order_hits(P->M, as<DevHit>(P->hits), res.hit_count, produced, false, first_event, P->n_events, out->hits,
                        out->hit_offsets, s));
     }
     out->n_hits = produced;

@@ -122,6 +122,14 @@ template <bool Diff, class P> struct BatchCtx {
     unsigned long long* hit_count;
     uint64_t hit_cap;
     bool keep_hits;
+    // this wave's current chunks of hit and tape slots and its count of hits written (LDS;
+    // written by whichever lane leads a reservation, so every access goes to memory)
+    volatile unsigned long long* hbase;
+    volatile unsigned long long* tbase;
+    volatile uint32_t* hfill;
+    volatile uint32_t* tfill;
+    volatile uint32_t* hvalid;
+    volatile uint32_t* tvalid;
     int32_t* first_branch;
     uint64_t first_event;
     bool rec;
@@ -209,10 +217,35 @@ template <bool Diff, class P> struct BatchCtx {
         child.hit_seq = 0;
         store(child);
     }
+    // Slot of the calling lane in the wave's current chunk: the active lanes take
+    // consecutive slots; when the chunk cannot hold them all, the first of them reserves
+    // a fresh chunk of kHitChunk slots with one atomic (the old chunk's rest stays unused).
+    __device__ unsigned long long reserve(unsigned long long* counter, volatile unsigned long long* base,
+                                          volatile uint32_t* fill, volatile uint32_t* written) {
+        const uint64_t active = __ballot(1);
+        const uint32_t n = static_cast<uint32_t>(__popcll(active));
+        const uint32_t rank = static_cast<uint32_t>(__popcll(active & ((1ull << __lane_id()) - 1ull)));
+        const int leader = __ffsll(static_cast<unsigned long long>(active)) - 1;
+        unsigned long long first = 0;
+        if (static_cast<int>(__lane_id()) == leader) {
+            unsigned long long b = *base;
+            uint32_t f = *fill;
+            if (f + n > kHitChunk) {
+                b = atomicAdd(counter, static_cast<unsigned long long>(kHitChunk));
+                f = 0;
+            }
+            first = b + f;
+            *base = b;
+            *fill = f + n;
+            if (written) *written += n;
+        }
+        first = __shfl(first, leader);
+        return first + rank;
+    }
     __device__ void hit(PTrack<P>& t, double wed, int sid) {
         const uint32_t seq = t.hit_seq++;
         if constexpr (Diff) {
-            const unsigned long long i = atomicAdd(tape_count, 1ull);
+            const unsigned long long i = reserve(tape_count, tbase, tfill, tvalid);
             if (i < tape_cap) {
                 TapeRec& q = tape[i];
                 for (int j = 0; j < 4; ++j) q.s[j] = static_cast<float>(sc[j]);
@@ -224,7 +257,7 @@ template <bool Diff, class P> struct BatchCtx {
             }
         }
         if (!keep_hits) return;
-        const unsigned long long i = atomicAdd(hit_count, 1ull);
+        const unsigned long long i = reserve(hit_count, hbase, hfill, hvalid);
         if (i < hit_cap) {
             DevHit& d = hits[i];
             d.k0 = event;
@@ -262,6 +295,9 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     __shared__ DevModel::SurfF s_surf[3];
     __shared__ uint32_t s_top[kMaxBlock / 64];
     __shared__ uint32_t s_lost[kMaxBlock / 64];
+    __shared__ unsigned long long s_hbase[kMaxBlock / 64], s_tbase[kMaxBlock / 64];
+    __shared__ uint32_t s_hfill[kMaxBlock / 64], s_tfill[kMaxBlock / 64], s_hvalid[kMaxBlock / 64],
+        s_tvalid[kMaxBlock / 64];
     for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
         s_cdf[i] = m.cdf_lt[i];
         s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
@@ -270,6 +306,12 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     if (threadIdx.x < kMaxBlock / 64) {
         s_top[threadIdx.x] = 0;
         s_lost[threadIdx.x] = 0;
+        s_hbase[threadIdx.x] = 0;
+        s_tbase[threadIdx.x] = 0;
+        s_hfill[threadIdx.x] = kHitChunk;        // no chunk yet: the first hit reserves one
+        s_tfill[threadIdx.x] = kHitChunk;
+        s_hvalid[threadIdx.x] = 0;
+        s_tvalid[threadIdx.x] = 0;
     }
     bs_init(s_stats, a.ledger);
     __syncthreads();
@@ -295,6 +337,12 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.hit_count = a.hit_count;
     ctx.hit_cap = a.hit_cap;
     ctx.keep_hits = a.keep_hits != 0;
+    ctx.hbase = &s_hbase[threadIdx.x / 64];
+    ctx.tbase = &s_tbase[threadIdx.x / 64];
+    ctx.hfill = &s_hfill[threadIdx.x / 64];
+    ctx.tfill = &s_tfill[threadIdx.x / 64];
+    ctx.hvalid = &s_hvalid[threadIdx.x / 64];
+    ctx.tvalid = &s_tvalid[threadIdx.x / 64];
     ctx.first_branch = a.first_branch;
     ctx.first_event = a.dyn ? a.dyn->first_event : a.first_event;
     ctx.rec = false;
@@ -421,6 +469,8 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
         atomicOr(&a.status[0], 1ull);
         atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
     }
+    if (lid == 0 && *ctx.hvalid) atomicAdd(a.hit_valid, static_cast<unsigned long long>(*ctx.hvalid));
+    if (Diff && lid == 0 && *ctx.tvalid) atomicAdd(a.tape_valid, static_cast<unsigned long long>(*ctx.tvalid));
     __syncthreads();
     bs_flush(s_stats, a.counters, a.ledger);
 }
@@ -521,6 +571,7 @@ template <class P> struct ChargeCtx {
     uint64_t luke_cap;
     DevHit* hits;
     unsigned long long* hit_count;
+    unsigned long long* hit_valid;
     uint64_t hit_cap;
     bool keep_hits;
 
@@ -545,6 +596,7 @@ template <class P> struct ChargeCtx {
     }
     __device__ void charge_hit(const CTrack<P>& c, double recomb_w, double e_rel, int sid) {
         if (!keep_hits) return;
+        atomicAdd(hit_valid, 1ull);
         const unsigned long long i = atomicAdd(hit_count, 1ull);
         if (i < hit_cap) {
             DevHit& d = hits[i];
@@ -578,6 +630,7 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
     ctx.luke_cap = a.luke_cap;
     ctx.hits = a.hits;
     ctx.hit_count = a.hit_count;
+    ctx.hit_valid = a.hit_valid;
     ctx.hit_cap = a.hit_cap;
     ctx.keep_hits = a.keep_hits != 0;
 
@@ -786,6 +839,12 @@ __global__ void zero_words_kernel(unsigned long long* p, uint64_t n) {
     for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0ull;
 }
 
+// Marks hit slots [0, n) unused (event key kNoHitKey) before a batched launch.
+__global__ void clear_hit_keys_kernel(DevHit* h, uint64_t n) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) h[i].k0 = kNoHitKey;
+}
+
 __global__ void tracks_to_queue_kernel(const cmpmc_phonon_track* in, const uint64_t* offsets, uint64_t n_events,
                                        uint64_t n_tracks, uint64_t first_event, BTrack* out) {
     const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -949,6 +1008,13 @@ extern "C" hipError_t cmpmc_launch_scan_counts(const uint32_t* counts, uint64_t 
                                   counts, n, partials, offsets);
     return hipGetLastError();
 }
+extern "C" hipError_t cmpmc_launch_clear_hit_keys(DevHit* h, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(clear_hit_keys_kernel, dim3(static_cast<unsigned>(std::min<uint64_t>(blocks_for(n, 256), 4096))),
+                       dim3(256), 0, s, h, n);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t cmpmc_launch_zero_words(unsigned long long* p, uint64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(zero_words_kernel, dim3(static_cast<unsigned>(std::min<uint64_t>(blocks_for(n, 256), 4096))),

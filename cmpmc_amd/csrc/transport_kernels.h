@@ -104,6 +104,13 @@ struct alignas(16) TapeRec {
     uint32_t chan_bin;                 // surface id | time bin << 8 (bin 0xFFFF: t >= t_max)
     uint32_t _pad;
 };
+
+// Hit and tape slots of the batched phonon kernel are reserved a chunk at a time per
+// wavefront (one global atomic per kHitChunk records instead of one per record). Slots a
+// wave leaves unused keep what the host wrote before the launch: a hit whose event key is
+// kNoHitKey (it sorts after every real hit) or an all-zero tape record (it adds nothing).
+constexpr uint32_t kHitChunk = 64;
+constexpr uint64_t kNoHitKey = ~0ull;
 constexpr int kDiffChannels = 2;       // top and bottom faces (surface ids 0, 1)
 constexpr int kDiffBins = 64;
 constexpr int kDiffOut = kDiffChannels + kDiffBins;
@@ -141,7 +148,8 @@ struct BatchedArgs {
     uint32_t refill_period;
     int32_t keep_hits;
     DevHit* hits;
-    unsigned long long* hit_count;
+    unsigned long long* hit_count;       // slots reserved (chunks; holes keep kNoHitKey)
+    unsigned long long* hit_valid;       // hits written
     uint64_t hit_cap;
     int32_t* first_branch;             // per event (relative to first_event)
     unsigned long long* counters;
@@ -151,7 +159,8 @@ struct BatchedArgs {
     const volatile int32_t* cancel;
     // differentiable form
     TapeRec* tape;
-    unsigned long long* tape_count;
+    unsigned long long* tape_count;    // tape slots reserved (chunks; holes stay all-zero)
+    unsigned long long* tape_valid;    // tape records written
     uint64_t tape_cap;
     double bins_per_s;                 // time bins per second of the arrival histogram
     double k_f, k_p;                   // f / (1 - f), p / (1 - p)
@@ -172,7 +181,8 @@ struct ChargeArgs {
     uint64_t luke_cap;
     int32_t keep_hits;
     DevHit* hits;
-    unsigned long long* hit_count;
+    unsigned long long* hit_count;     // slots reserved (shared with the phonon kernel)
+    unsigned long long* hit_valid;     // hits written
     uint64_t hit_cap;
     unsigned long long* counters;
     unsigned long long* ledger;
@@ -221,6 +231,7 @@ hipError_t cmpmc_launch_scan_counts(const uint32_t* counts, uint64_t n, uint64_t
                                     uint64_t cap, cmpmc_dev::DynParams* dyn, hipStream_t s);
 // Sets n 64-bit words to zero with a kernel (no memset node when the stream is captured).
 hipError_t cmpmc_launch_zero_words(unsigned long long* p, uint64_t n, hipStream_t s);
+hipError_t cmpmc_launch_clear_hit_keys(cmpmc_dev::DevHit* h, uint64_t n, hipStream_t s);
 // map_k_to_vg of n directions k[3n] with the model's kinematics, FP64 (fp32 = 0) or FP32
 // arithmetic: out[4n] = (vg_dir, vg_mag).
 hipError_t cmpmc_launch_map_query(const cmpmc_dev::DevModel* m, int fp32, int mode, uint64_t n, const double* k,

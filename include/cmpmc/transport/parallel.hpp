@@ -154,7 +154,7 @@ RunResult run_events(const SimulationEngine& engine, uint64_t n_events, Generato
         out.hit_offsets = result.hit_offsets.data();
         const int rc = cmpmc_run_events(engine.model(), &b, &o, &out, nullptr);
         if (rc == CMPMC_ERR_CAPACITY && options.keep_hits && out.hits_needed > cap) {
-            cap = out.hits_needed;          // deterministic: rerun with room for every hit
+            cap = out.hits_needed;          // rerun with the room the library asks for
             continue;
         }
         check_status(rc);

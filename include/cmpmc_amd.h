@@ -218,7 +218,9 @@ typedef struct {
     cmpmc_sensor_hit* hits;       /* host buffer of hit_capacity rows (may be NULL if !keep_hits) */
     uint64_t hit_capacity;
     uint64_t n_hits;              /* hits written */
-    uint64_t hits_needed;         /* total hits produced (> hit_capacity on CMPMC_ERR_CAPACITY) */
+    uint64_t hits_needed;         /* hits produced; on CMPMC_ERR_CAPACITY the capacity to ask for (the
+                                     batched form reserves hit slots a chunk per wavefront, so it
+                                     may need a little more room than it has hits) */
     uint64_t* hit_offsets;        /* n_events + 1 (may be NULL) */
     int32_t* first_branch;        /* n_events branch codes of the primary's first step (may be NULL) */
     double kernel_ms;             /* device time of the transport kernels (HIP events) */
