@@ -162,6 +162,21 @@ def test_batched_stack_overflow_retry(c1):
     assert a["stats_raw"] == b["stats_raw"]
 
 
+def test_batched_hit_capacity_retry(c1):
+    """Hit slots are reserved 64 at a time per wavefront and the unused ones sort last:
+    with a hit buffer far too small the library reports the room it needs, the binding
+    reruns, and hits, offsets and statistics equal those of a run with ample room."""
+    cm, eng, wl, O, oeng, owl = c1
+    print("case: batched C1 n=400 seed=9 hit_capacity=50", flush=True)
+    a = eng.run_workload(wl, 400, seed=9, form=cm.FORM_BATCHED, hit_capacity=50)
+    b = eng.run_workload(wl, 400, seed=9, form=cm.FORM_BATCHED, hit_capacity=400 * 200)
+    assert a["stats_raw"] == b["stats_raw"]
+    assert np.array_equal(a["hit_offsets"], b["hit_offsets"])
+    assert np.array_equal(a["hits"], b["hits"])
+    assert len(b["hits"]) == b["stats"]["sensor_hits"] == int(b["hit_offsets"][-1]) > 50
+    assert np.all(b["hits"]["edep"] > 0)      # no unused slot among the returned hits
+
+
 def test_cancel_flag(c1):
     cm, eng, wl, O, oeng, owl = c1
     eng.cancel(True)
