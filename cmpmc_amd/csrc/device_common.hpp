@@ -661,12 +661,18 @@ __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P:
     R t_min = P::inf();
     int surf = 2;
     const R hz = R(m.half_z);
-    if (v.z > R(1e-15)) {
-        const R tz = P::div(hz - p.z, v.z);
-        if (tz >= R(0) && tz < t_min) { t_min = tz; surf = 0; }
-    } else if (v.z < R(-1e-15)) {
-        const R tz = P::div(-hz - p.z, v.z);
-        if (tz >= R(0) && tz < t_min) { t_min = tz; surf = 1; }
+    if constexpr (P::exact) {
+        if (v.z > R(1e-15)) {
+            const R tz = P::div(hz - p.z, v.z);
+            if (tz >= R(0) && tz < t_min) { t_min = tz; surf = 0; }
+        } else if (v.z < R(-1e-15)) {
+            const R tz = P::div(-hz - p.z, v.z);
+            if (tz >= R(0) && tz < t_min) { t_min = tz; surf = 1; }
+        }
+    } else {                                      // the same tests without a branch per sign
+        const bool up = v.z > R(0);
+        const R tz = P::div((up ? hz : -hz) - p.z, v.z);
+        if (fabs(v.z) > R(1e-15) && tz >= R(0) && tz < t_min) { t_min = tz; surf = up ? 0 : 1; }
     }
     const R a = v.x * v.x + v.y * v.y;
     if (a > R(1e-18)) {
@@ -678,13 +684,21 @@ __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P:
         if (tr >= R(0) && tr < t_min) { t_min = tr; surf = 2; }
     }
     t_min = tmax(R(0), t_min);
-    if (surf == 0) n_in = {R(0), R(0), R(-1)};
-    else if (surf == 1) n_in = {R(0), R(0), R(1)};
-    else {
+    if constexpr (P::exact) {
+        if (surf == 0) n_in = {R(0), R(0), R(-1)};
+        else if (surf == 1) n_in = {R(0), R(0), R(1)};
+        else {
+            const R hx = p.x + v.x * t_min, hy = p.y + v.y * t_min;
+            const R r = P::hypot_(hx, hy);
+            if (r > R(1e-15)) n_in = {P::div(-hx, r), P::div(-hy, r), R(0)};
+            else n_in = {R(-1), R(0), R(0)};
+        }
+    } else {                                      // every lane computes the side normal; selects
         const R hx = p.x + v.x * t_min, hy = p.y + v.y * t_min;
         const R r = P::hypot_(hx, hy);
-        if (r > R(1e-15)) n_in = {P::div(-hx, r), P::div(-hy, r), R(0)};
-        else n_in = {R(-1), R(0), R(0)};
+        const bool side_ok = r > R(1e-15);
+        const V3<R> side = side_ok ? V3<R>{P::div(-hx, r), P::div(-hy, r), R(0)} : V3<R>{R(-1), R(0), R(0)};
+        n_in = (surf == 2) ? side : V3<R>{R(0), R(0), (surf == 0) ? R(-1) : R(1)};
     }
     surface = surf;
     return t_min;
