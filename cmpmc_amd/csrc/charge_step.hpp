@@ -27,21 +27,46 @@ template <class P> struct CTrack {
     uint32_t spawn_no, hit_seq;
 };
 
+// The FP32 form reads the model's FP32 charge block (charge_f) instead of converting
+// FP64 fields at every step, and divides with the policy's hardware reciprocal; the FP64
+// form keeps the reference's operations.
 template <class P> __device__ inline V3<typename P::R> to_valley(const DevModel& m, int iv, V3<typename P::R> v) {
-    return mat_mul(m.valley_rot[iv], v);
+    if constexpr (P::exact) return mat_mul(m.valley_rot[iv], v);
+    else {
+        const float* q = m.charge_f.valley_rot[iv];
+        return {q[0] * v.x + q[1] * v.y + q[2] * v.z, q[3] * v.x + q[4] * v.y + q[5] * v.z,
+                q[6] * v.x + q[7] * v.y + q[8] * v.z};
+    }
 }
 template <class P> __device__ inline V3<typename P::R> from_valley(const DevModel& m, int iv, V3<typename P::R> v) {
-    return mat_mul_t(m.valley_rot[iv], v);
+    if constexpr (P::exact) return mat_mul_t(m.valley_rot[iv], v);
+    else {
+        const float* q = m.charge_f.valley_rot[iv];
+        return {q[0] * v.x + q[3] * v.y + q[6] * v.z, q[1] * v.x + q[4] * v.y + q[7] * v.z,
+                q[2] * v.x + q[5] * v.y + q[8] * v.z};
+    }
 }
 template <class P> __device__ inline V3<typename P::R> ell_to_sph(const DevModel& m, int iv, V3<typename P::R> v) {
     using R = typename P::R;
     const V3<R> t = to_valley<P>(m, iv, v);
-    return {t.x * R(m.hv_sqrt_inv[0]), t.y * R(m.hv_sqrt_inv[1]), t.z * R(m.hv_sqrt_inv[2])};
+    if constexpr (P::exact) return {t.x * R(m.hv_sqrt_inv[0]), t.y * R(m.hv_sqrt_inv[1]), t.z * R(m.hv_sqrt_inv[2])};
+    else return {t.x * m.charge_f.hv_sqrt_inv[0], t.y * m.charge_f.hv_sqrt_inv[1], t.z * m.charge_f.hv_sqrt_inv[2]};
 }
 template <class P> __device__ inline V3<typename P::R> sph_to_ell(const DevModel& m, int iv, V3<typename P::R> v) {
     using R = typename P::R;
-    const V3<R> s{v.x * R(m.hv_sqrt[0]), v.y * R(m.hv_sqrt[1]), v.z * R(m.hv_sqrt[2])};
-    return from_valley<P>(m, iv, s);
+    if constexpr (P::exact) {
+        const V3<R> s{v.x * R(m.hv_sqrt[0]), v.y * R(m.hv_sqrt[1]), v.z * R(m.hv_sqrt[2])};
+        return from_valley<P>(m, iv, s);
+    } else {
+        const V3<R> s{v.x * m.charge_f.hv_sqrt[0], v.y * m.charge_f.hv_sqrt[1], v.z * m.charge_f.hv_sqrt[2]};
+        return from_valley<P>(m, iv, s);
+    }
+}
+// k = p / (hbar c) in the FP32 form: a multiplication by the constant's reciprocal
+template <class P> __device__ inline V3<typename P::R> p_to_k(V3<typename P::R> p) {
+    using R = typename P::R;
+    if constexpr (P::exact) return vdiv<P>(p, R(kHbarc));
+    else return p * static_cast<float>(1.0 / kHbarc);
 }
 // E_kin = sqrt(p^2 + m^2) - m; the FP32 form p^2 / (sqrt(p^2 + m^2) + m) avoids the
 // cancellation (E_kin / m ~ 1e-10 here).
@@ -50,26 +75,38 @@ template <class P> __device__ inline typename P::R kin_energy(typename P::R p2, 
     if constexpr (P::exact) return sqrt(p2 + mass * mass) - mass;
     else {
         const R mm = R(mass);
-        return p2 / (P::sqrt_(p2 + mm * mm) + mm);
+        return P::div(p2, P::sqrt_(p2 + mm * mm) + mm);
     }
 }
+template <class P> __device__ inline typename P::R kin_energy_f(typename P::R p2, float mm) {   // FP32 form
+    return P::div(p2, P::sqrt_(p2 + mm * mm) + mm);
+}
 template <class P> __device__ inline typename P::R ekin_e(const DevModel& m, int iv, V3<typename P::R> p) {
-    return kin_energy<P>(norm2(ell_to_sph<P>(m, iv, p)), m.e_mass_eV);
+    if constexpr (P::exact) return kin_energy<P>(norm2(ell_to_sph<P>(m, iv, p)), m.e_mass_eV);
+    else return kin_energy_f<P>(norm2(ell_to_sph<P>(m, iv, p)), m.charge_f.mass[0]);
 }
 template <class P> __device__ inline typename P::R ekin_h(const DevModel& m, V3<typename P::R> p) {
-    return kin_energy<P>(norm2(p), m.h_mass_eV);
+    if constexpr (P::exact) return kin_energy<P>(norm2(p), m.h_mass_eV);
+    else return kin_energy_f<P>(norm2(p), m.charge_f.mass[1]);
 }
 template <class P> __device__ inline V3<typename P::R> vel_e(const DevModel& m, int iv, V3<typename P::R> p) {
     using R = typename P::R;
     const R ek = ekin_e<P>(m, iv, p);
     const V3<R> pv = to_valley<P>(m, iv, p);
-    const V3<R> vv{pv.x / R(m.mass_ratio[0]), pv.y / R(m.mass_ratio[1]), pv.z / R(m.mass_ratio[2])};
-    return from_valley<P>(m, iv, vv) * (R(kCLight) / (ek + R(m.e_mass_eV)));
+    if constexpr (P::exact) {
+        const V3<R> vv{pv.x / R(m.mass_ratio[0]), pv.y / R(m.mass_ratio[1]), pv.z / R(m.mass_ratio[2])};
+        return from_valley<P>(m, iv, vv) * (R(kCLight) / (ek + R(m.e_mass_eV)));
+    } else {
+        const float* im = m.charge_f.inv_mass_ratio;
+        const V3<R> vv{pv.x * im[0], pv.y * im[1], pv.z * im[2]};
+        return from_valley<P>(m, iv, vv) * (R(kCLight) * P::rcp(ek + m.charge_f.mass[0]));
+    }
 }
 template <class P> __device__ inline V3<typename P::R> vel_h(const DevModel& m, V3<typename P::R> p) {
     using R = typename P::R;
     const R ek = ekin_h<P>(m, p);
-    return p * (R(kCLight) / (ek + R(m.h_mass_eV)));
+    if constexpr (P::exact) return p * (R(kCLight) / (ek + R(m.h_mass_eV)));
+    else return p * (R(kCLight) * P::rcp(ek + m.charge_f.mass[1]));
 }
 template <class P> __device__ inline V3<typename P::R> ekin_to_p_e(const DevModel& m, int iv, V3<typename P::R> dir,
                                                                    typename P::R ekin) {
@@ -78,7 +115,10 @@ template <class P> __device__ inline V3<typename P::R> ekin_to_p_e(const DevMode
     const V3<R> uh = ell_to_sph<P>(m, iv, u);
     const R band_u = norm2(uh);
     if (band_u <= R(0) || ekin <= R(0)) return {R(0), R(0), R(0)};
-    const R pm = P::sqrt_((ekin * ekin + R(2) * ekin * R(m.e_mass_eV)) / band_u);
+    R me;
+    if constexpr (P::exact) me = R(m.e_mass_eV);
+    else me = m.charge_f.mass[0];
+    const R pm = P::sqrt_(P::div(ekin * ekin + R(2) * ekin * me, band_u));
     return u * pm;
 }
 template <class P> __device__ inline V3<typename P::R> charge_velocity(const DevModel& m, const CTrack<P>& t) {
@@ -109,6 +149,16 @@ __device__ inline void make_charge(const DevModel& m, CTrack<P>& c, int type, V3
 // charge_transport.hpp:82-106
 template <class P> __device__ inline typename P::R luke_rate(const DevModel& m, const CTrack<P>& t) {
     using R = typename P::R;
+    if constexpr (!P::exact) {
+        const int h = (t.type == 1) ? 0 : 1;
+        const R k_mag = norm<P>((t.type == 1) ? ell_to_sph<P>(m, t.valley, p_to_k<P>(t.p)) : p_to_k<P>(t.p));
+        const R k_sound = m.charge_f.k_sound[h];
+        if (k_mag <= k_sound) return R(0);
+        const R mach = P::div(k_mag, k_sound);
+        const R dm = mach - R(1);
+        const R tau = P::div(m.charge_f.tstep[h] * mach, dm * dm * dm);
+        return (tau > R(0)) ? P::rcp(tau) : R(0);
+    }
     R k_mag, k_sound, l0;
     if (t.type == 1) {
         k_mag = norm<P>(ell_to_sph<P>(m, t.valley, vdiv<P>(t.p, R(kHbarc))));
@@ -130,6 +180,7 @@ template <class P> __device__ inline typename P::R luke_rate(const DevModel& m, 
 // charge_kernels.hpp:82-99
 template <class P> __device__ inline typename P::R iv_rate(const DevModel& m, typename P::R ekin) {
     using R = typename P::R;
+    if constexpr (!P::exact) return (ekin < m.charge_f.iv_min_energy) ? R(0) : m.charge_f.iv_rate;   // 0: one valley
     if (m.n_valleys < 2) return R(0);
     if (ekin < R(m.iv_min_energy)) return R(0);
     const double E2 = m.efield[0] * m.efield[0] + m.efield[1] * m.efield[1] + m.efield[2] * m.efield[2];
@@ -156,9 +207,15 @@ template <class P> __device__ inline CRates<P> charge_rates(const DevModel& m, c
     CRates<P> r;
     r.luke = luke_rate<P>(m, t);
     r.iv = (t.type == 1) ? iv_rate<P>(m, t.ekin) : R(0);
-    r.trap = mfp_rate<P>(m.trap_mfp[h], v);
-    r.ion_e = mfp_rate<P>(m.ion_e_mfp[h], v);
-    r.ion_h = mfp_rate<P>(m.ion_h_mfp[h], v);
+    if constexpr (P::exact) {
+        r.trap = mfp_rate<P>(m.trap_mfp[h], v);
+        r.ion_e = mfp_rate<P>(m.ion_e_mfp[h], v);
+        r.ion_h = mfp_rate<P>(m.ion_h_mfp[h], v);
+    } else {
+        r.trap = v * m.charge_f.inv_mfp[0][h];
+        r.ion_e = v * m.charge_f.inv_mfp[1][h];
+        r.ion_h = v * m.charge_f.inv_mfp[2][h];
+    }
     r.tot = r.luke + r.iv + r.trap + r.ion_e + r.ion_h;
     return r;
 }
@@ -170,16 +227,18 @@ __device__ inline bool emit_luke(const DevModel& m, CTrack<P>& t, G& r, V3<typen
     V3<R> k_trk;
     R k_sound;
     if (t.type == 1) {
-        k_trk = ell_to_sph<P>(m, t.valley, vdiv<P>(t.p, R(kHbarc)));
-        k_sound = R(m.k_sound_e);
+        k_trk = ell_to_sph<P>(m, t.valley, p_to_k<P>(t.p));
+        if constexpr (P::exact) k_sound = R(m.k_sound_e);
+        else k_sound = m.charge_f.k_sound[0];
     } else {
-        k_trk = vdiv<P>(t.p, R(kHbarc));
-        k_sound = R(m.k_sound_h);
+        k_trk = p_to_k<P>(t.p);
+        if constexpr (P::exact) k_sound = R(m.k_sound_h);
+        else k_sound = m.charge_f.k_sound[1];
     }
     const R k_mag = norm<P>(k_trk);
     if (k_mag <= k_sound * R(1.0001)) return false;
     const V3<R> k_dir = vdiv<P>(k_trk, k_mag);
-    const R v = k_sound / k_mag;
+    const R v = P::div(k_sound, k_mag);
     const R u = P::uniform(r);
     const R cos_th = tclamp(v + (R(1) - v) * P::cbrt_(R(1) - u), v, R(1));
     const R sin_th = P::sqrt_(tmax(R(0), R(1) - cos_th * cos_th));
@@ -191,7 +250,9 @@ __device__ inline bool emit_luke(const DevModel& m, CTrack<P>& t, G& r, V3<typen
     const V3<R> qv = rotate_around(rotate_around(k_dir * q, perp, cos_th, sin_th), k_dir, c_phi, s_phi);
     const V3<R> k_rec = k_trk - qv;
     const R q_mag = norm<P>(qv);
-    const R e_ph = q_mag * R(m.v_sound) * R(kHbar);
+    R e_ph;
+    if constexpr (P::exact) e_ph = q_mag * R(m.v_sound) * R(kHbar);
+    else e_ph = q_mag * m.charge_f.v_sound_hbar;
     if (e_ph >= t.ekin || e_ph <= R(0)) return false;
     if (t.type == 2) {
         t.p = k_rec * R(kHbarc);
@@ -260,18 +321,31 @@ template <class P, class G, class Ctx>
 __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx& ctx) {
     using R = typename P::R;
     ctx.ccount(CC_STEPS);
-    const V3<R> E{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
+    V3<R> E;
+    if constexpr (P::exact) E = V3<R>{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
+    else E = V3<R>{m.charge_f.efield[0], m.charge_f.efield[1], m.charge_f.efield[2]};
     const R q_sign = (trk.type == 1) ? R(-1) : R(1);
     const CRates<P> R0 = charge_rates<P>(m, trk);
-    R dt_max = R(m.dt_cap);
-    if (m.dk_frac > 0.0) {
-        const bool is_e = (trk.type == 1);
-        const R ks = R(is_e ? m.k_sound_e : m.k_sound_h);
-        const R k_now = is_e ? norm<P>(ell_to_sph<P>(m, trk.valley, vdiv<P>(trk.p, R(kHbarc)))) : norm<P>(trk.p) / R(kHbarc);
-        const R dk_rate = norm<P>(E) * R(kCLight) / R(kHbarc) * (is_e ? R(m.hv_max_scale) : R(1));
-        if (dk_rate > R(0)) dt_max = tmin(dt_max, R(m.dk_frac) * tmax(ks, k_now) / dk_rate);
+    R dt_max;
+    if constexpr (P::exact) {
+        dt_max = R(m.dt_cap);
+        if (m.dk_frac > 0.0) {
+            const bool is_e = (trk.type == 1);
+            const R ks = R(is_e ? m.k_sound_e : m.k_sound_h);
+            const R k_now = is_e ? norm<P>(ell_to_sph<P>(m, trk.valley, vdiv<P>(trk.p, R(kHbarc)))) : norm<P>(trk.p) / R(kHbarc);
+            const R dk_rate = norm<P>(E) * R(kCLight) / R(kHbarc) * (is_e ? R(m.hv_max_scale) : R(1));
+            if (dk_rate > R(0)) dt_max = tmin(dt_max, R(m.dk_frac) * tmax(ks, k_now) / dk_rate);
+        }
+    } else {
+        dt_max = m.charge_f.dt_cap;
+        if (m.dk_frac > 0.0) {
+            const int h = (trk.type == 1) ? 0 : 1;
+            const R k_now = norm<P>((trk.type == 1) ? ell_to_sph<P>(m, trk.valley, p_to_k<P>(trk.p)) : p_to_k<P>(trk.p));
+            const R dk_rate = m.charge_f.dk_rate[h];
+            if (dk_rate > R(0)) dt_max = tmin(dt_max, P::div(m.charge_f.dk_frac * tmax(m.charge_f.k_sound[h], k_now), dk_rate));
+        }
     }
-    const R dt_phys = (R0.tot > R(1)) ? (P::neg_log(P::uniform(r)) / R0.tot) : P::inf();
+    const R dt_phys = (R0.tot > R(1)) ? P::div(P::neg_log(P::uniform(r)), R0.tot) : P::inf();
     const R dt = tmin(dt_phys, dt_max);
     const V3<R> dp = E * (q_sign * R(kCLight) * dt);
     const V3<R> pm = trk.p + dp * R(0.5);
@@ -284,7 +358,7 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
         int sid;
         const R dist = intersect_exit<P>(m, trk.pos, vdir, n_in, sid);
         if (dist <= d_step) {
-            const R dtb = dist / vmag;
+            const R dtb = P::div(dist, vmag);
             trk.pos = trk.pos + vdir * dist;
             trk.time += static_cast<double>(dtb);
             trk.p = trk.p + E * (q_sign * R(kCLight) * dtb);

@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include "cmpmc_amd.h"
 
 // Capacity of each Kaplan cascade list of one lane (core/math.hpp:230, KaplanBuffer).
@@ -96,6 +98,21 @@ struct DevModel {
     } surf_f[3];
     float orient_f[9];
     float fL_f, fLST_f, fTT_f, pST_f, poly_rate_f, radius_f, half_z_f, _pad_f;
+    // FP32 charge constants of the batched charge step (index 0 electrons, 1 holes)
+    struct ChargeF {
+        float valley_rot[8][9];     // row-major copies of valley_rot
+        float hv_sqrt[3], hv_sqrt_inv[3], inv_mass_ratio[3];
+        float efield[3];
+        float dt_cap, dk_frac;
+        float dk_rate[2];           // |E| c / hbar c (times hv_max_scale for electrons)
+        float k_sound[2];
+        float tstep[2];             // 3 l0 / v_sound
+        float mass[2];              // e_mass_eV, h_mass_eV
+        float iv_rate, iv_min_energy;   // intervalley rate at this field (0 with one valley)
+        float inv_mfp[3][2];        // 1 / mean free path: trapping, e and h ionisation (0: none)
+        float v_sound_hbar;         // v_sound hbar: Luke phonon energy per unit |q|
+        float _pad;
+    } charge_f;
 };
 
 // ------------------------------------------------------------------ RNG (core/rng.hpp)
@@ -742,6 +759,45 @@ inline void fill_fp32_constants(DevModel& m) {
     m.pST_f = static_cast<float>(m.fST / (m.fST + m.fFT));
     m.poly_rate_f = (m.poly_mfp <= 0.0) ? 0.0f : static_cast<float>(1.0 / m.poly_mfp);
     m.radius_f = static_cast<float>(m.radius);
+    {
+        DevModel::ChargeF& c = m.charge_f;
+        for (int v = 0; v < 8; ++v)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) c.valley_rot[v][3 * i + j] = static_cast<float>(m.valley_rot[v][i][j]);
+        for (int i = 0; i < 3; ++i) {
+            c.hv_sqrt[i] = static_cast<float>(m.hv_sqrt[i]);
+            c.hv_sqrt_inv[i] = static_cast<float>(m.hv_sqrt_inv[i]);
+            c.inv_mass_ratio[i] = static_cast<float>(1.0 / m.mass_ratio[i]);
+            c.efield[i] = static_cast<float>(m.efield[i]);
+        }
+        c.dt_cap = static_cast<float>(m.dt_cap);
+        c.dk_frac = static_cast<float>(m.dk_frac);
+        const double E2 = m.efield[0] * m.efield[0] + m.efield[1] * m.efield[1] + m.efield[2] * m.efield[2];
+        const double dkr = std::sqrt(E2) * kCLight / kHbarc;
+        c.dk_rate[0] = static_cast<float>(dkr * m.hv_max_scale);
+        c.dk_rate[1] = static_cast<float>(dkr);
+        c.k_sound[0] = static_cast<float>(m.k_sound_e);
+        c.k_sound[1] = static_cast<float>(m.k_sound_h);
+        c.tstep[0] = static_cast<float>(3.0 * m.l0_e / m.v_sound);
+        c.tstep[1] = static_cast<float>(3.0 * m.l0_h / m.v_sound);
+        c.mass[0] = static_cast<float>(m.e_mass_eV);
+        c.mass[1] = static_cast<float>(m.h_mass_eV);
+        double ivr = 0.0;
+        if (m.n_valleys >= 2) {
+            if (m.iv_model == 0) ivr = m.iv_quad_rate * std::pow(m.iv_quad_field * m.iv_quad_field + E2, 0.5 * m.iv_quad_power);
+            else ivr = m.iv_lin_rate0 + m.iv_lin_rate1 * std::pow(std::sqrt(E2) * 0.01, m.iv_lin_power);
+        }
+        c.iv_rate = static_cast<float>(ivr);
+        c.iv_min_energy = static_cast<float>(m.iv_min_energy);
+        const double* mfps[3] = {m.trap_mfp, m.ion_e_mfp, m.ion_h_mfp};
+        for (int k = 0; k < 3; ++k)
+            for (int h = 0; h < 2; ++h) {
+                const double mfp = mfps[k][h];
+                c.inv_mfp[k][h] = (mfp > 0.0 && mfp < 1e20) ? static_cast<float>(1.0 / mfp) : 0.0f;
+            }
+        c.v_sound_hbar = static_cast<float>(m.v_sound * kHbar);
+        c._pad = 0.0f;
+    }
     m.half_z_f = static_cast<float>(m.half_z);
     m._pad_f = 0.0f;
     for (int k = 0; k < 3; ++k) {
