@@ -143,7 +143,14 @@ template <bool Diff, class P> struct BatchCtx {
     unsigned long long* tape_count;
     uint64_t tape_cap;
 
-    __device__ void count(Counter c) { bs_count(*bs, c); }
+    // the three counters every step touches are per-lane registers, summed at the end
+    uint32_t n_steps, n_iso, n_refl;
+    __device__ void count(Counter c) {
+        if (c == C_STEPS) ++n_steps;
+        else if (c == C_ISO) ++n_iso;
+        else if (c == C_REFL) ++n_refl;
+        else bs_count(*bs, c);
+    }
     __device__ void ccount(CCounter c) { bs_count(*bs, C_NCOUNTERS + c); }
     __device__ void escaped(double v) { bs_ledger(*bs, L_ESC, v); }
     __device__ void deposited(double v) { bs_ledger(*bs, L_DEP, v); }
@@ -357,6 +364,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.tape_count = a.tape_count;
     ctx.tape_cap = a.tape_cap;
 
+    ctx.n_steps = ctx.n_iso = ctx.n_refl = 0;
     PTrack<P> t;
     bool alive = false;
     bool exhausted = false;
@@ -470,6 +478,15 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
         atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
     }
     if (lid == 0 && *ctx.hvalid) atomicAdd(a.hit_valid, static_cast<unsigned long long>(*ctx.hvalid));
+    {
+        const uint32_t own[3] = {ctx.n_steps, ctx.n_iso, ctx.n_refl};
+        const Counter which[3] = {C_STEPS, C_ISO, C_REFL};
+        for (int k = 0; k < 3; ++k) {
+            unsigned long long v = own[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lid == 0) s_stats.cnt[threadIdx.x >> 6][which[k]] += v;
+        }
+    }
     if (Diff && lid == 0 && *ctx.tvalid) atomicAdd(a.tape_valid, static_cast<unsigned long long>(*ctx.tvalid));
     __syncthreads();
     bs_flush(s_stats, a.counters, a.ledger);
