@@ -741,6 +741,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ca.lstack = as<BCharge>(M->clstack);
             ca.depth = kChargeDepth;
             ca.lanes = static_cast<uint32_t>(clanes);
+            ca.claim_cap = charge_claim_cap(n_charges, clanes);
             ca.luke_out = q + n_init;
             ca.luke_count = &d_acc->luke_count;
             ca.luke_cap = luke_cap;

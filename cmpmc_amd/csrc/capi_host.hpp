@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <mutex>
@@ -132,6 +133,16 @@ inline void apply_totals(const Totals& t, cmpmc_sim_stats& s) {
     s.total_injected_eV += ledger_eV(t.ledger[L_INJ]);
     s.total_deposited_eV += ledger_eV(t.ledger[L_DEP]);
     s.escaped_or_killed_eV += ledger_eV(t.ledger[L_ESC]);
+}
+
+// Carriers a charge-kernel wavefront claims at once: enough for the grid to take the
+// whole batch in one round (at most 64), so that a small batch runs a few carriers per
+// wavefront instead of filling a few wavefronts (a wavefront's step costs the sum of the
+// branches its carriers take, and its carriers' chains are serial).
+inline uint32_t charge_claim_cap(uint64_t n_carriers, uint64_t lanes) {
+    const uint64_t waves = std::max<uint64_t>(lanes / 64, 1);
+    const uint64_t per = (n_carriers + waves - 1) / waves;
+    return static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(per, 1), 64));
 }
 
 inline unsigned long long fixed_point(double eV) {

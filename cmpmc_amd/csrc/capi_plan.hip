@@ -76,6 +76,7 @@ int enqueue(cmpmc_plan* P, hipStream_t s) {
         ca.lstack = as<BCharge>(P->clstack);
         ca.depth = kChargeDepth;
         ca.lanes = static_cast<uint32_t>(static_cast<uint64_t>(P->cgrid) * P->block);
+        ca.claim_cap = charge_claim_cap(2 * N, ca.lanes);
         ca.luke_out = q;
         ca.luke_count = &acc->luke_count;
         ca.luke_cap = P->luke ? P->queue_cap : 0;

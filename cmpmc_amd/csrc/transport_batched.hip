@@ -662,8 +662,12 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
             alive = true;
             steps = 0;
         }
-        const bool want = !alive && !exhausted;
-        const uint64_t want_mask = __ballot(want);
+        bool want = !alive && !exhausted;
+        uint64_t want_mask = __ballot(want);
+        if (static_cast<uint32_t>(__popcll(want_mask)) > a.claim_cap) {   // the first claim_cap of them
+            want = want && static_cast<uint32_t>(__popcll(want_mask & lt_mask)) < a.claim_cap;
+            want_mask = __ballot(want);
+        }
         if (want_mask) {
             const int leader = __ffsll(static_cast<unsigned long long>(want_mask)) - 1;
             unsigned long long base = 0;

@@ -176,6 +176,10 @@ struct ChargeArgs {
     BCharge* lstack;                   // knock-on carriers: entry d of lane L at [d * lanes + L]
     uint32_t depth;
     uint32_t lanes;
+    uint32_t claim_cap;                // carriers a wavefront claims at once (1..64): a
+                                       // wave's step runs every branch its carriers take,
+                                       // so a batch smaller than the grid spreads thinly
+    uint32_t _pad;
     BTrack* luke_out;                  // enqueued Luke phonons (append)
     unsigned long long* luke_count;
     uint64_t luke_cap;
