@@ -118,3 +118,19 @@ def test_luke_cascade_batched_vs_oracle_batched(lib_cmpmc, oracle):
     for f in ("time", "edep"):
         pv = sst.ks_2samp(ph_g[f], ph_o[f]).pvalue
         assert pv > 0.01, f"Luke phonon hit {f}: KS p = {pv:.3g}"
+
+
+def test_batched_charges_independent_of_carriers_per_wavefront(lib_cmpmc, oracle):
+    """A small batch runs one carrier per wavefront on the full grid, and 38 per
+    wavefront on a 4-block grid (16 wavefronts for 600 carriers); every carrier draws from
+    its own stream, so statistics and hits must be identical."""
+    cm, O = lib_cmpmc, oracle
+    eng, _ = _engines(cm, O, 0.0, TRAP)
+    n = 300
+    print(f"case: charge drift, carriers per wavefront 1 vs 38, n={n}", flush=True)
+    wl = cm.workload(cm.WL_PAIR)
+    wide = eng.run_workload(wl, n, seed=11, form=cm.FORM_BATCHED, hit_capacity=4 * n)
+    packed = eng.run_workload(wl, n, seed=11, form=cm.FORM_BATCHED, hit_capacity=4 * n, grid_blocks=4)
+    assert wide["stats"]["charges_collected"] + wide["stats"]["charges_trapped"] == 2 * n
+    assert wide["stats_raw"] == packed["stats_raw"], "charge statistics depend on the carriers per wavefront"
+    assert np.array_equal(wide["hits"], packed["hits"])
