@@ -930,6 +930,19 @@ int validate_charges(const cmpmc_dev::DevModel& m, const cmpmc_charge_track* t, 
 }  // namespace cmpmc_host
 
 // ------------------------------------------------------------------ entry points
+// A batch without events: zero statistics and no hits, as run_events over an empty
+// event list returns (parallel.hpp:76-148 loops over no events); nothing is launched.
+static int empty_run(cmpmc_run_out* out) {
+    std::memset(&out->stats, 0, sizeof(out->stats));
+    out->n_hits = 0;
+    out->hits_needed = 0;
+    out->stack_needed = 0;
+    if (out->hit_offsets) out->hit_offsets[0] = 0;
+    out->kernel_ms = 0.0;
+    out->passes = 0;
+    return CMPMC_OK;
+}
+
 extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, const cmpmc_run_opts* opts,
                                 cmpmc_run_out* out, void* stream) {
     if (!M || !b || !out) return fail(CMPMC_ERR_INVALID, "cmpmc_run_events: null argument");
@@ -954,6 +967,7 @@ extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, cons
     J.injected = b->injected_eV;
     const RunCfg C = cfg_from(opts);
     CHK(check_cfg(C));
+    if (J.n_events == 0) return empty_run(out);
     Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
               out->first_branch, &out->kernel_ms, &out->stack_needed, &out->passes};
     out->kernel_ms = 0.0;
@@ -975,6 +989,7 @@ extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint
     J.wl = wl;
     const RunCfg C = cfg_from(opts);
     CHK(check_cfg(C));
+    if (J.n_events == 0) return empty_run(out);
     Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
               out->first_branch, &out->kernel_ms, &out->stack_needed, &out->passes};
     out->kernel_ms = 0.0;

@@ -238,3 +238,35 @@ def test_charge_pairs_plain_and_batched(lib_cmpmc, oracle):
     assert gb["stats"]["charges_collected"] == 2 * n
     assert gb["stats"]["luke_phonons_emitted"] == pytest.approx(ob["stats"]["luke_phonons_emitted"], rel=0.05)
     assert gb["stats"]["total_luke_phonon_eV"] == pytest.approx(ob["stats"]["total_luke_phonon_eV"], rel=0.05)
+
+
+def test_empty_and_ragged_batches(c1):
+    """run_events over ragged stacks (events with no, one and several tracks), both
+    forms: events without tracks get no hits, and the batch gives the same per-event
+    results as its two halves run separately (each event draws from its own stream). An
+    empty batch returns zero statistics and no hits."""
+    cm, eng, wl, O, oeng, owl = c1
+    counts = [0, 1, 3, 0, 0, 5, 2, 0, 1, 4, 0, 6] * 10
+    n = len(counts)
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.uint64)
+    rng = cm.Rng(2024)
+    ks = np.array([rng.isotropic_dir() for _ in range(int(offs[-1]))])
+    tracks = cm.make_tracks(eng.lattice, np.zeros((len(ks), 3)), ks, 0, 7.5e-3)
+    states = np.array([cm.Rng.for_stream(SEED, ev).state() for ev in range(n)])
+    k = n // 2
+    for form in (cm.FORM_PLAIN, cm.FORM_BATCHED):
+        print(f"case: ragged batch of {n} events, form {form}", flush=True)
+        whole = eng.run_events(tracks, offs, states, seed=SEED, form=form)
+        ho = whole["hit_offsets"]
+        assert all(ho[i] == ho[i + 1] for i, c in enumerate(counts) if c == 0)
+        assert whole["stats"]["sensor_hits"] > 0
+        a = eng.run_events(tracks[:offs[k]], offs[:k + 1], states[:k], seed=SEED, form=form)
+        b = eng.run_events(tracks[offs[k]:], offs[k:] - offs[k], states[k:], seed=SEED, form=form, first_event_id=k)
+        assert np.array_equal(whole["hits"], np.concatenate([a["hits"], b["hits"]]))
+        assert np.array_equal(ho[k:] - ho[k], b["hit_offsets"])
+        for key in ("total_steps", "sensor_hits", "surface_refls", "decays_L_to_TT", "decays_L_to_LT"):
+            assert whole["stats"][key] == a["stats"][key] + b["stats"][key], key
+        e = eng.run_events(np.zeros(0, dtype=cm.PHONON_DTYPE), np.zeros(1, dtype=np.uint64),
+                           np.zeros((0, 4), dtype=np.uint64), seed=SEED, form=form)
+        assert len(e["hits"]) == 0 and e["hit_offsets"][0] == 0
+        assert all(v == 0 for v in e["stats"].values())
