@@ -8,6 +8,8 @@ Batched form (FP32, per-track sub-streams): compared with the oracle run in the 
 sub-stream scheme: identical first-step branch counts, ledger closure 1e-9, and hit
 time / energy / position distributions (two-sample KS p > 0.01, means within 3 sigma).
 """
+import ctypes
+
 import numpy as np
 import pytest
 from scipy import stats as sst
@@ -270,3 +272,32 @@ def test_empty_and_ragged_batches(c1):
                            np.zeros((0, 4), dtype=np.uint64), seed=SEED, form=form)
         assert len(e["hits"]) == 0 and e["hit_offsets"][0] == 0
         assert all(v == 0 for v in e["stats"].values())
+
+
+def test_invalid_inputs_rejected(c1):
+    """Invalid inputs fail on the host with CMPMC_ERR_INVALID before anything is launched
+    (negative energy, an unknown mode, 2^32 events), and the model runs normally after."""
+    cm, eng, wl, O, oeng, owl = c1
+    print("case: invalid tracks and event counts", flush=True)
+    k = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0]])
+    good = cm.make_tracks(eng.lattice, np.zeros((2, 3)), k, 0, 7.5e-3)
+    states = np.array([cm.Rng.for_stream(SEED, 0).state()])
+    offs = np.array([0, 2], dtype=np.uint64)
+    bad_energy = good.copy()
+    bad_energy["energy"][1] = -1.0
+    bad_mode = good.copy()
+    bad_mode["mode"][0] = 3
+    for bad, what in ((bad_energy, "negative energy"), (bad_mode, "mode 3")):
+        for form in (cm.FORM_PLAIN, cm.FORM_BATCHED):
+            with pytest.raises(cm.CmpmcError) as ei:
+                eng.run_events(bad, offs, states, seed=SEED, form=form)
+            assert ei.value.code == cm.capi.ERR_INVALID, what
+            assert "phonon track" in str(ei.value), what
+    L = cm.capi.load()
+    out = cm.capi.RunOut()
+    o = eng._opts(cm.FORM_BATCHED, SEED, False, False, 0, 0, 0, 0, 0, False, 0)
+    rc = L.cmpmc_run_workload(eng.ptr, ctypes.byref(wl), 1 << 32, ctypes.byref(o), ctypes.byref(out), None)
+    assert rc == cm.capi.ERR_INVALID
+    assert "2^32" in L.cmpmc_last_error().decode()
+    r = eng.run_events(good, offs, states, seed=SEED, form=cm.FORM_BATCHED)
+    assert r["stats"]["total_steps"] > 0
