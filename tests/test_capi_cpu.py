@@ -30,7 +30,9 @@ def test_exports_every_declared_symbol(lib_cmpmc):
 def test_struct_sizes(lib_cmpmc):
     L = lib_cmpmc.capi.load()
     lib_cmpmc.capi.check_abi(L)
-    # byte layouts of the reference's records (static_assert'ed in the survey)
+    # byte layouts of the reference's records on x86-64, which the C ABI mirrors:
+    # PhononTrack and ChargeTrack (alignas(64), core/track.hpp:62,75), SensorHit
+    # (core/track.hpp:99), SimStats (transport/engine.hpp:29)
     assert L.cmpmc_abi_sizeof(5) == 128 and L.cmpmc_abi_sizeof(6) == 128
     assert L.cmpmc_abi_sizeof(7) == 56 and L.cmpmc_abi_sizeof(8) == 208
 
