@@ -86,7 +86,6 @@ def main():
     ap.add_argument("--events", type=int, default=1_000_000)
     ap.add_argument("--plain-events", type=int, default=1_000_000)
     ap.add_argument("--lognormal-events", type=int, default=100_000)
-    ap.add_argument("--lognormal-plain-events", type=int, default=0)
     ap.add_argument("--grad-events", type=int, default=100_000)
     ap.add_argument("--charge-events", type=int, default=100_000)
     ap.add_argument("--luke-events", type=int, default=2_000)
@@ -95,8 +94,6 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--waves", type=int, default=0, help="register-budget variant (0: library default)")
-    ap.add_argument("--tune", action="store_true", help="launch options from the autotuner instead of the defaults")
-    ap.add_argument("--tune-events", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lognormal", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -122,15 +119,7 @@ def main():
     eng = cm.Engine()
     wl = cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
     tune = None
-    if not args.tune:
-        opts = {"waves": args.waves}
-    else:
-        t = time.perf_counter()
-        tune = eng.autotune(wl, args.tune_events, seed=args.seed + 99991)
-        tune["tuning_s"] = time.perf_counter() - t
-        opts = {k: tune[k] for k in ("block", "waves", "refill_period", "lane_depth")}
-        _log(f"autotune ({args.tune_events} events, {tune['trials']} timed runs, {tune['tuning_s']:.1f} s): {opts}, "
-             f"{tune['best_ms']:.2f} ms vs default {tune['default_ms']:.2f} ms")
+    opts = {"waves": args.waves}
     kw = dict(keep_hits=False, form=cm.FORM_BATCHED, **opts)
 
     def step(seed):
@@ -276,18 +265,6 @@ def main():
         out["lognormal_kernel_s"] = rl["kernel_ms"] / 1e3
         _log(f"lognormal: {args.lognormal_events} events in {lwall:.2f} s, {rl['stats']['total_steps']:.3e} steps, "
              f"closure {_closure(rl['stats']):.1e}")
-        if args.lognormal_plain_events > 0:
-            n = args.lognormal_plain_events
-            t = time.perf_counter()
-            rb = eng.run_workload(lw, n, seed=args.seed + 1, **kw)
-            bw = time.perf_counter() - t
-            t = time.perf_counter()
-            rpl = eng.run_workload(lw, n, seed=args.seed + 1, keep_hits=False, form=cm.FORM_PLAIN)
-            pw = time.perf_counter() - t
-            out["lognormal_slice"] = {"events": n, "batched_s": bw, "plain_s": pw, "batched_vs_plain": pw / bw,
-                                      "batched_steps": rb["stats"]["total_steps"],
-                                      "plain_steps": rpl["stats"]["total_steps"]}
-            _log(f"lognormal slice of {n}: batched {bw:.2f} s, plain {pw:.2f} s")
 
     if not args.no_cpu:
         try:

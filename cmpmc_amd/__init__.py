@@ -21,12 +21,12 @@ import time
 import numpy as np
 
 from . import _capi as capi
-from ._capi import (CHARGE_DTYPE, FORM_BATCHED, FORM_PLAIN, HIT_DTYPE, PHONON_DTYPE, WL_PAIR, WL_PHONON_LOGN,
-                    WL_PHONON_POINT, WL_PHONON_UNIF, CmpmcError, check)
+from ._capi import (CHARGE_DTYPE, FORM_BATCHED, FORM_PLAIN, HIT_DTYPE, MODE_BENCH_DOS, PHONON_DTYPE, WL_PAIR,
+                    WL_PHONON_LOGN, WL_PHONON_POINT, WL_PHONON_UNIF, CmpmcError, check)
 
 __all__ = ["Lattice", "Engine", "DiffTape", "apply_diff_params", "Rng", "germanium", "default_detector", "set_bias", "demo_polynomial_surfaces",
            "workload", "make_tracks", "CmpmcError", "FORM_PLAIN", "FORM_BATCHED", "PHONON_DTYPE", "CHARGE_DTYPE",
-           "HIT_DTYPE", "WL_PHONON_POINT", "WL_PHONON_LOGN", "WL_PAIR", "WL_PHONON_UNIF", "device_info"]
+           "HIT_DTYPE", "WL_PHONON_POINT", "WL_PHONON_LOGN", "WL_PAIR", "WL_PHONON_UNIF", "MODE_BENCH_DOS", "device_info"]
 
 MASK64 = (1 << 64) - 1
 
@@ -112,6 +112,15 @@ def apply_diff_params(crystal: capi.Crystal, det: capi.Detector, params) -> None
         s.spec[0], s.diff[0], s.spec_prob_const = 1.0 - f, f, 1.0 - f
         if s.has_film:
             s.film.film_absorption = p
+
+
+def constant_split_surfaces(det: capi.Detector, f_diffuse: float) -> None:
+    """Every surface reflects diffusely with probability f_diffuse and specularly
+    otherwise, at every energy (SurfaceProperty with constant spec/diff coefficients and
+    no down-conversion, surface_property.hpp:23-60)."""
+    for s in det.surf:
+        s.n_spec, s.n_diff, s.n_anh = 1, 1, 0
+        s.spec[0], s.diff[0], s.spec_prob_const = 1.0 - f_diffuse, f_diffuse, 1.0 - f_diffuse
 
 
 def set_bias(det: capi.Detector, volts: float, thickness_m: float = 2.54e-2) -> None:
@@ -247,12 +256,13 @@ class Engine:
 
     @staticmethod
     def _opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
-              poll_cancel, waves=0) -> capi.RunOpts:
+              poll_cancel, waves=0, time_budget_s=0.0) -> capi.RunOpts:
         o = capi.RunOpts()
         o.form, o.keep_hits, o.want_first_branch, o.block_size = form, int(keep_hits), int(first_branch), block
         o.seed, o.stack_headroom, o.lane_stack_depth = seed, headroom, lane_depth
         o.grid_blocks, o.refill_period, o.poll_cancel = grid_blocks, refill_period, int(poll_cancel)
         o.waves_per_simd = waves
+        o.time_budget_s = time_budget_s
         return o
 
     def hit_estimate(self, wl: capi.Workload, n_events: int) -> int:
@@ -303,10 +313,12 @@ class Engine:
     def run_workload(self, wl: capi.Workload, n_events: int, seed: int = 1, form: int = FORM_PLAIN,
                      keep_hits: bool = True, hit_capacity: int | None = None, first_branch: bool = False,
                      block: int = 0, headroom: int = 0, lane_depth: int = 0, grid_blocks: int = 0,
-                     refill_period: int = 0, poll_cancel: bool = False, waves: int = 0, stream=None) -> dict:
-        """run_events with the generator on the device (parallel.hpp:76-148)."""
+                     refill_period: int = 0, poll_cancel: bool = False, waves: int = 0, time_budget_s: float = 0.0,
+                     stream=None) -> dict:
+        """run_events with the generator on the device (parallel.hpp:76-148). time_budget_s > 0
+        stops the run with CMPMC_ERR_CANCELLED once it has run that long."""
         o = self._opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
-                       poll_cancel, waves)
+                       poll_cancel, waves, time_budget_s)
         L = capi.load()
         if keep_hits and hit_capacity is None:
             hit_capacity = self.hit_estimate(wl, n_events)

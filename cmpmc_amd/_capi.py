@@ -20,6 +20,8 @@ dbl = C.c_double
 OK, ERR_INVALID, ERR_UNVALIDATED, ERR_CAPACITY, ERR_HIP, ERR_NO_DEVICE, ERR_CANCELLED = range(7)
 FORM_PLAIN, FORM_BATCHED = 0, 1
 WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_PHONON_UNIF = 1, 2, 3, 5
+# workload mode 3: the reference benchmark generator's mode draw (cmpmc_bench.cpp:63-65)
+MODE_BENCH_DOS = 3
 BRANCH_NAMES = {0: "none", 1: "killed", 2: "isotope", 3: "poly", 4: "decay_TT", 5: "decay_LT", 6: "film",
                 7: "escape", 8: "surf_anh", 9: "specular", 10: "diffuse", 11: "roulette"}
 N_OCT, N_CDF, MAX_POLY = 256, 1024, 8
@@ -113,7 +115,7 @@ class Workload(C.Structure):
 class RunOpts(C.Structure):
     _fields_ = [("form", i32), ("keep_hits", i32), ("want_first_branch", i32), ("block_size", i32),
                 ("seed", u64), ("stack_headroom", u64), ("lane_stack_depth", u64), ("grid_blocks", u32),
-                ("refill_period", u32), ("poll_cancel", i32), ("waves_per_simd", i32)]
+                ("refill_period", u32), ("poll_cancel", i32), ("waves_per_simd", i32), ("time_budget_s", dbl)]
 
 
 class RunOut(C.Structure):

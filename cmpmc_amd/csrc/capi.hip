@@ -5,11 +5,14 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "capi_host.hpp"
@@ -272,9 +275,61 @@ extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
 
 extern "C" int cmpmc_model_cancel(cmpmc_model* M, int value) {
     if (!M) return fail(CMPMC_ERR_INVALID, "null model");
-    *M->h_cancel = value ? 1 : 0;
+    int32_t* w = const_cast<int32_t*>(M->h_cancel);
+    if (value) __atomic_fetch_or(w, kCancelUser, __ATOMIC_SEQ_CST);
+    else __atomic_fetch_and(w, ~kCancelUser, __ATOMIC_SEQ_CST);
     return CMPMC_OK;
 }
+
+// ------------------------------------------------------------------ time budget
+namespace cmpmc_host {
+struct BudgetWatch::State {
+    std::mutex mu;
+    std::condition_variable cv;
+    bool done = false;
+    bool fired = false;
+    int32_t* word = nullptr;
+    std::thread th;
+};
+
+BudgetWatch::BudgetWatch(cmpmc_model* M, double budget_s) {
+    if (!(budget_s > 0.0)) return;
+    st_ = new State;
+    State* st = st_;
+    st->word = const_cast<int32_t*>(M->h_cancel);
+    st->th = std::thread([st, budget_s]() {
+        std::unique_lock<std::mutex> lk(st->mu);
+        if (!st->cv.wait_for(lk, std::chrono::duration<double>(budget_s), [st] { return st->done; })) {
+            st->fired = true;
+            __atomic_fetch_or(st->word, kCancelBudget, __ATOMIC_SEQ_CST);
+        }
+    });
+}
+
+BudgetWatch::~BudgetWatch() {
+    if (!st_) return;
+    {
+        std::lock_guard<std::mutex> lk(st_->mu);
+        st_->done = true;
+    }
+    st_->cv.notify_all();
+    st_->th.join();
+    __atomic_fetch_and(st_->word, ~kCancelBudget, __ATOMIC_SEQ_CST);
+    delete st_;
+}
+
+bool BudgetWatch::fired() const {
+    if (!st_) return false;
+    std::lock_guard<std::mutex> lk(st_->mu);
+    return st_->fired;
+}
+
+// The cancel word the kernels of a run poll: the model's, when the caller asked for
+// polling or set a time budget; none otherwise.
+inline const volatile int32_t* run_cancel(cmpmc_model* M, const RunCfg& C) {
+    return (C.poll_cancel || C.time_budget_s > 0.0) ? M->d_cancel : nullptr;
+}
+}  // namespace cmpmc_host
 
 extern "C" int cmpmc_model_trim(cmpmc_model* M) {
     if (!M) return fail(CMPMC_ERR_INVALID, "null model");
@@ -436,9 +491,19 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
             nch[e] = J.coff ? static_cast<uint32_t>(J.coff[e + 1] - J.coff[e]) : 0;
         }
     }
+    if (C.limit_plain_events && !(C.time_budget_s > 0.0)) {
+        // one lane runs a whole event: an event of C2 size (thousands of phonons at up to
+        // 31 meV, ~1e6 steps each) would keep its lane busy for hours
+        const uint32_t heaviest = N ? *std::max_element(nph.begin(), nph.end()) : 0u;
+        if (heaviest > kPlainMaxPhonons)
+            return fail(CMPMC_ERR_INVALID,
+                        "plain form: an event starts with %u phonons (more than %u); one GPU lane runs a whole event, "
+                        "which at this size takes hours; use the batched form or set time_budget_s",
+                        heaviest, kPlainMaxPhonons);
+    }
     const bool any_charges = std::any_of(nch.begin(), nch.end(), [](uint32_t v) { return v > 0; });
     uint64_t headroom = C.headroom ? C.headroom : (any_charges && M->dm.luke_fraction > 0.0 ? 8192 : 128);
-    const uint64_t c_headroom = 16;
+    uint64_t c_headroom = 16;   // knock-on carriers per event beyond the initial ones
     const uint64_t max_lanes = 262144;
     const uint64_t region_budget = 24ull << 30;
     Totals tot;
@@ -548,7 +613,7 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
             a.ledger = d_acc->ledger;
             a.status = d_acc->status;
             a.kaplan_ws = as<double>(M->ws);
-            a.cancel = C.poll_cancel ? M->d_cancel : nullptr;
+            a.cancel = run_cancel(M, C);
             HIPCHK(hipEventRecord(ev0, s));
             HIPCHK(cmpmc_launch_plain(&M->dm, &a, C.block, s));
             HIPCHK(hipEventRecord(ev1, s));
@@ -558,11 +623,27 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
             float ms = 0.0f;
             HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
             ms_total += ms;
-            if (res.status[0]) {
+            if (res.status[kStCancelled]) {
+                rc = fail(CMPMC_ERR_CANCELLED, "run cancelled by the host");
+                break;
+            }
+            if (res.status[kStChargeNeed]) {
+                // an event's charge stack region was too small: rerun the chunk with more room
+                if (attempt >= 6 || c_headroom > (1ull << 20)) {
+                    if (O.stack_needed) *O.stack_needed = res.status[kStChargeNeed];
+                    rc = fail(CMPMC_ERR_CAPACITY, "event charge stack overflow: %llu entries needed",
+                              static_cast<unsigned long long>(res.status[kStChargeNeed]));
+                    break;
+                }
+                c_headroom = std::max<uint64_t>(2 * c_headroom, res.status[kStChargeNeed]);
+                HIPCHK(hipMemcpyAsync(&d_acc->hit_count, &hit_before, 8, hipMemcpyHostToDevice, s));
+                continue;
+            }
+            if (res.status[kStOverflow]) {
                 // a stack region was too small: rerun the chunk (same inputs, same result)
-                deepest = std::max<uint64_t>(deepest, res.status[1]);
+                deepest = std::max<uint64_t>(deepest, res.status[kStNeed]);
                 const uint64_t max_init = *std::max_element(nph.begin() + c0, nph.begin() + c1);
-                const uint64_t need = res.status[1] > max_init ? res.status[1] - max_init : headroom;
+                const uint64_t need = res.status[kStNeed] > max_init ? res.status[kStNeed] - max_init : headroom;
                 if (attempt >= 6 || headroom > (1ull << 22)) {
                     if (O.stack_needed) *O.stack_needed = deepest;
                     rc = fail(CMPMC_ERR_CAPACITY, "event stack overflow: %llu entries needed",
@@ -572,10 +653,6 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
                 headroom = std::max(headroom * 2, need * 2 + 64);
                 HIPCHK(hipMemcpyAsync(&d_acc->hit_count, &hit_before, 8, hipMemcpyHostToDevice, s));
                 continue;
-            }
-            if (C.poll_cancel && *M->h_cancel) {
-                rc = fail(CMPMC_ERR_CANCELLED, "run cancelled by the host");
-                break;
             }
             add_acc(tot, res);
             if (C.want_first_branch && O.first_branch)
@@ -676,7 +753,11 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         n_charges = 2 * N;
     }
     const double lf = std::min(std::max(M->dm.luke_fraction, 0.0), 1.0);
-    uint64_t luke_cap = (n_charges && lf > 0.0) ? n_charges * 4096 : 0;
+    uint64_t luke_cap =
+        (n_charges && lf > 0.0) ? n_charges * static_cast<uint64_t>(std::ceil(lf * kLukePerCarrier)) + 1024 : 0;
+    // knock-on carriers per lane of the charge kernel: the options' stack_headroom, or the
+    // default; grown on a rerun when a lane's stack overflowed
+    uint32_t cdepth = C.headroom ? static_cast<uint32_t>(std::min<uint64_t>(C.headroom, 1u << 16)) : kChargeDepth;
     if (D && J.wl && !pair) {
         // room for the hits of the initial tracks before the first pass, so that a tape
         // overflow (and a second transport pass) stays the exception: a phonon of energy E
@@ -731,7 +812,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         HIPCHK(hipEventRecord(e0, s));
         uint64_t n_luke = 0;
         if (n_charges) {
-            CHK(ensure(M->clstack, static_cast<size_t>(kChargeDepth) * clanes * sizeof(BCharge)));
+            CHK(ensure(M->clstack, static_cast<size_t>(cdepth) * clanes * sizeof(BCharge)));
             ChargeArgs ca{};
             ca.seed = C.seed;
             ca.first_event = J.first_event;
@@ -739,7 +820,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ca.n_init = n_charges;
             ca.init_head = &d_acc->chead;
             ca.lstack = as<BCharge>(M->clstack);
-            ca.depth = kChargeDepth;
+            ca.depth = cdepth;
             ca.lanes = static_cast<uint32_t>(clanes);
             ca.claim_cap = charge_claim_cap(n_charges, clanes);
             ca.luke_out = q + n_init;
@@ -753,7 +834,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ca.counters = d_acc->counters;
             ca.ledger = d_acc->ledger;
             ca.status = d_acc->status;
-            ca.cancel = C.poll_cancel ? M->d_cancel : nullptr;
+            ca.cancel = run_cancel(M, C);
             HIPCHK(cmpmc_launch_charges(&M->dm, &ca, block, cgrid, waves, s));
             HIPCHK(hipMemcpyAsync(&n_luke, &d_acc->luke_count, 8, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -792,7 +873,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ba.ledger = d_acc->ledger;
             ba.status = d_acc->status;
             ba.kaplan_ws = as<double>(M->ws);
-            ba.cancel = C.poll_cancel ? M->d_cancel : nullptr;
+            ba.cancel = run_cancel(M, C);
             if (D) {
                 CHK(ensure(*D->tape, tape_cap * sizeof(TapeRec)));
                 // tape slots are reserved a chunk per wavefront: unused ones stay all-zero
@@ -814,14 +895,24 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, e0, e2));
         ms_total += ms;
-        if (res.status[2]) { rc = fail(CMPMC_ERR_CANCELLED, "run cancelled by the host"); break; }
-        if (res.status[0]) {                      // a lane spawn stack overflowed: rerun deeper
+        if (res.status[kStCancelled]) { rc = fail(CMPMC_ERR_CANCELLED, "run cancelled by the host"); break; }
+        if (res.status[kStChargeNeed]) {          // a lane's knock-on carrier stack overflowed: rerun deeper
+            if (attempt >= 6 || cdepth >= (1u << 16)) {
+                if (O.stack_needed) *O.stack_needed = res.status[kStChargeNeed];
+                rc = fail(CMPMC_ERR_CAPACITY, "charge stack overflow (%llu carriers per lane needed)",
+                          static_cast<unsigned long long>(res.status[kStChargeNeed]));
+                break;
+            }
+            cdepth = static_cast<uint32_t>(std::max<uint64_t>(2ull * cdepth, res.status[kStChargeNeed] + 16));
+            continue;
+        }
+        if (res.status[kStOverflow]) {            // a lane spawn stack overflowed: rerun deeper
             if (attempt >= 4 || depth >= (1u << 22)) {
-                if (O.stack_needed) *O.stack_needed = res.status[1];
+                if (O.stack_needed) *O.stack_needed = res.status[kStNeed];
                 rc = fail(CMPMC_ERR_CAPACITY, "spawn pool overflow (%llu entries per wave)", static_cast<unsigned long long>(depth));
                 break;
             }
-            depth = std::max<uint64_t>(depth * 2, res.status[1] + 64);
+            depth = std::max<uint64_t>(depth * 2, res.status[kStNeed] + 64);
             continue;
         }
         if (D && res.tape_count > tape_cap) {     // tape too small: rerun with room for every hit
@@ -878,6 +969,8 @@ RunCfg cfg_from(const cmpmc_run_opts* o) {
     c.refill_period = o->refill_period;
     c.waves = o->waves_per_simd;
     c.poll_cancel = o->poll_cancel != 0;
+    c.time_budget_s = o->time_budget_s;
+    c.limit_plain_events = true;
     return c;
 }
 
@@ -890,6 +983,8 @@ int check_cfg(const RunCfg& C) {
                     kMaxBlock);
     if (C.waves != 0 && (C.waves < 2 || C.waves > 4))
         return fail(CMPMC_ERR_INVALID, "run options: waves_per_simd %d is not 0, 2, 3 or 4", C.waves);
+    if (!(C.time_budget_s >= 0.0) || !std::isfinite(C.time_budget_s))
+        return fail(CMPMC_ERR_INVALID, "run options: time_budget_s must be finite and >= 0");
     return CMPMC_OK;
 }
 
@@ -897,7 +992,9 @@ int check_workload(const cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_ev
     if (wl->kind != CMPMC_WL_PHONON_POINT && wl->kind != CMPMC_WL_PHONON_LOGN && wl->kind != CMPMC_WL_PAIR &&
         wl->kind != CMPMC_WL_PHONON_UNIF)
         return fail(CMPMC_ERR_INVALID, "workload: unknown kind %d", wl->kind);
-    if (wl->mode < 0 || wl->mode > 2 || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
+    const bool mode_ok = (wl->mode >= 0 && wl->mode <= 2) ||
+                         (wl->mode == CMPMC_MODE_BENCH_DOS && wl->kind == CMPMC_WL_PHONON_POINT);
+    if (!mode_ok || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
         (wl->kind == CMPMC_WL_PHONON_LOGN && !(wl->logn_median > 0.0)))
         return fail(CMPMC_ERR_INVALID, "workload: invalid parameters");
     if (wl->kind == CMPMC_WL_PAIR && M->dm.iv_model == 2)
@@ -930,6 +1027,17 @@ int validate_charges(const cmpmc_dev::DevModel& m, const cmpmc_charge_track* t, 
 }  // namespace cmpmc_host
 
 // ------------------------------------------------------------------ entry points
+// Runs a job in the form the options name, under the options' time budget: when the
+// budget runs out first, the kernels stop at their next poll of the cancel word and the
+// run fails with CMPMC_ERR_CANCELLED and a message that names the budget.
+static int run_job(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, hipStream_t s) {
+    BudgetWatch watch(M, C.time_budget_s);
+    const int rc = (C.form == CMPMC_FORM_BATCHED) ? run_batched(M, J, C, O, s) : run_plain(M, J, C, O, s);
+    if (rc == CMPMC_ERR_CANCELLED && watch.fired())
+        return fail(CMPMC_ERR_CANCELLED, "run stopped: its time budget of %.3g s ran out", C.time_budget_s);
+    return rc;
+}
+
 // A batch without events: zero statistics and no hits, as run_events over an empty
 // event list returns (parallel.hpp:76-148 loops over no events); nothing is launched.
 static int empty_run(cmpmc_run_out* out) {
@@ -948,6 +1056,9 @@ extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, cons
     if (!M || !b || !out) return fail(CMPMC_ERR_INVALID, "cmpmc_run_events: null argument");
     if (b->n_events && (!b->rng_state || (b->tracks && !b->track_offsets) || (b->charges && !b->charge_offsets)))
         return fail(CMPMC_ERR_INVALID, "cmpmc_run_events: offsets and rng_state are required");
+    // event ids are 32-bit in the device records (queues, hit keys, offsets)
+    if (b->n_events > (1ull << 32) || b->first_event_id > (1ull << 32) - b->n_events)
+        return fail(CMPMC_ERR_INVALID, "cmpmc_run_events: event ids first_event_id + n_events must stay within 2^32");
     std::lock_guard<std::mutex> lk(M->mu);
     cmpmc_internal::clear_error();
     const uint64_t nt = b->tracks ? b->track_offsets[b->n_events] : 0;
@@ -972,9 +1083,7 @@ extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, cons
               out->first_branch, &out->kernel_ms, &out->stack_needed, &out->passes};
     out->kernel_ms = 0.0;
     out->passes = 0;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (C.form == CMPMC_FORM_BATCHED) return run_batched(M, J, C, O, s);
-    return run_plain(M, J, C, O, s);
+    return run_job(M, J, C, O, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events, const cmpmc_run_opts* opts,
@@ -994,9 +1103,7 @@ extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint
               out->first_branch, &out->kernel_ms, &out->stack_needed, &out->passes};
     out->kernel_ms = 0.0;
     out->passes = 0;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (C.form == CMPMC_FORM_BATCHED) return run_batched(M, J, C, O, s);
-    return run_plain(M, J, C, O, s);
+    return run_job(M, J, C, O, static_cast<hipStream_t>(stream));
 }
 
 // ------------------------------------------------------------------ differentiable form

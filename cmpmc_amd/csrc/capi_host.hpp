@@ -68,7 +68,9 @@ struct cmpmc_model {
     float4* d_oct = nullptr;
     double* d_cdf = nullptr;
     double* d_stiff = nullptr;              // C_ijlm: 81 doubles, then 81 floats
-    volatile int32_t* h_cancel = nullptr;   // host-mapped
+    // host-mapped cancel word the kernels poll: bit 0 is the caller's cancel flag
+    // (cmpmc_model_cancel), bit 1 the run's time budget (set by the watchdog thread)
+    volatile int32_t* h_cancel = nullptr;
     int32_t* d_cancel = nullptr;
     std::mutex mu;
     // cached run buffers
@@ -180,7 +182,16 @@ struct RunCfg {
     uint32_t refill_period = 0;
     int waves = 0;
     bool poll_cancel = false;
+    double time_budget_s = 0.0;   // > 0: the run is cancelled once it has taken this long
+    // the plain form refuses events heavier than kPlainMaxPhonons initial phonons unless a
+    // time budget bounds the run (set for the run_events / run_workload entry points; the
+    // serial transport calls of a single stack keep the reference's unbounded semantics)
+    bool limit_plain_events = false;
 };
+
+// Largest initial phonon count of one event that the plain form runs without a time
+// budget.
+constexpr uint32_t kPlainMaxPhonons = 1024;
 
 struct Outputs {
     cmpmc_sim_stats* stats;
@@ -209,8 +220,33 @@ struct DiffCfg {
 constexpr int kDefaultWaves = 3;
 // Spawn-pool entries per wave of the batched phonon kernel when the options leave it open.
 constexpr uint64_t kDefaultPool = 2048;
-// Knock-on carriers per lane of the charge kernel.
+// Knock-on carriers per lane of the charge kernel (the first pass; a pass whose stacks
+// overflow is rerun with the depth the kernel reports).
 constexpr uint32_t kChargeDepth = 64;
+// Luke phonons enqueued per carrier that the first pass of a run reserves room for (about
+// 1,800 are emitted per carrier at 4 V across 2.54 cm of Ge); a run that emits more is
+// rerun with a queue of the measured size.
+constexpr uint64_t kLukePerCarrier = 2048;
+
+// Cancel-word bits (cmpmc_model::h_cancel).
+constexpr int32_t kCancelUser = 1;
+constexpr int32_t kCancelBudget = 2;
+
+// Arms the run's time budget: a watchdog thread sets kCancelBudget in the model's cancel
+// word once budget_s have passed, unless the run ends first; the destructor stops the
+// thread and clears the bit. fired() tells whether the budget ran out.
+class BudgetWatch {
+public:
+    BudgetWatch(cmpmc_model* M, double budget_s);
+    ~BudgetWatch();
+    bool fired() const;
+    BudgetWatch(const BudgetWatch&) = delete;
+    BudgetWatch& operator=(const BudgetWatch&) = delete;
+
+private:
+    struct State;
+    State* st_ = nullptr;
+};
 
 // Sorts the first n device hits by (event, track key, sequence) into the host buffer;
 // per-event counts become hit_offsets (n_events + 1 entries).

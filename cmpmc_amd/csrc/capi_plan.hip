@@ -435,10 +435,18 @@ int time_config(cmpmc_model* M, const cmpmc_workload* wl, uint64_t N, uint64_t s
 
 }  // namespace
 
+// The autotuner is switched off: its GPU test went silent twice right after a completed
+// tuning (the second tuning or the eager runs after it never reported), and no cause has
+// been found in this code or the kernels it runs. cmpmc_autotune fails until one is.
+constexpr bool kAutotuneEnabled = false;
+
 extern "C" int cmpmc_autotune(cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events, uint64_t seed,
                               const char* cache_path, int flags, cmpmc_run_opts* opts, cmpmc_tune_result* result,
                               void* stream) {
     if (!M || !wl || !opts || !result) return fail(CMPMC_ERR_INVALID, "cmpmc_autotune: null argument");
+    if (!kAutotuneEnabled)
+        return fail(CMPMC_ERR_INVALID, "cmpmc_autotune: the autotuner is switched off in this build (its GPU test "
+                                       "went silent after a tuning; cause not found)");
     if (n_events == 0) return fail(CMPMC_ERR_INVALID, "cmpmc_autotune: no events to time");
     CHK(check_workload(M, wl, n_events));
     std::memset(result, 0, sizeof(*result));

@@ -853,6 +853,12 @@ struct KList {
     }
 };
 
+// Rejection samplers below (the reference's are unbounded loops) stop after kMaxTrials
+// proposals and keep the last one, so that every loop of a kernel ends. Their acceptance
+// probability is above 0.3 for every valid input, so the cap is never reached in practice
+// (0.7^(2^20) ~ 0) and results are those of the unbounded loop.
+constexpr uint32_t kMaxTrials = 1u << 20;
+
 template <class P> __device__ inline double film_escape_probability(const DevFilm& f, double E, double frac) {
     if (f.gap <= 0.0) return 1.0;
     if constexpr (P::exact) {
@@ -902,22 +908,22 @@ template <class P, class G> __device__ inline double film_sample_qp(const DevFil
     if constexpr (P::exact) {
         auto w = [E, d](double x) { return (x * (E - x) + d * d) / sqrt((x + d) * (E - x + d)); };
         const double w_max = tmax(w(d), w(0.5 * E));
-        for (;;) {
+        for (uint32_t trial = 1;; ++trial) {
             const double u = P::uniform(r);
             const double s = sin(kHalfPi * u);
             const double x = d + span * (s * s);
-            if (P::uniform(r) * w_max <= w(x)) return x;
+            if (P::uniform(r) * w_max <= w(x) || trial >= kMaxTrials) return x;
         }
     } else {
         // FP32 proposal and acceptance in units of the gap (x/d), FP64 return value
         const float Ed = static_cast<float>(E) * f.inv_gap_f, sp = static_cast<float>(span) * f.inv_gap_f;
         auto w = [Ed](float x) { return (x * (Ed - x) + 1.0f) * __builtin_amdgcn_rsqf((x + 1.0f) * (Ed - x + 1.0f)); };
         const float w_max = tmax(w(1.0f), w(0.5f * Ed));
-        for (;;) {
+        for (uint32_t trial = 1;; ++trial) {
             const float u = P::uniform(r);
             const float s = P::sin_half_pi(u);
             const float xd = 1.0f + sp * (s * s);
-            if (P::uniform(r) * w_max <= w(xd)) return d + span * static_cast<double>(s * s);
+            if (P::uniform(r) * w_max <= w(xd) || trial >= kMaxTrials) return d + span * static_cast<double>(s * s);
         }
     }
 }
@@ -934,10 +940,10 @@ template <class P, class G> __device__ inline double film_sample_phonon(const De
         };
         const double x_peak = tclamp(0.25 * (E + 3.0 * d2E), d, E);
         const double w_max = tmax(w(d), w(x_peak)) * 1.02;
-        for (;;) {
+        for (uint32_t trial = 1;; ++trial) {
             const double u = P::uniform(r);
             const double x = d + span * (u * u);
-            if (static_cast<double>(P::uniform(r)) * w_max <= w(x)) return E - x;
+            if (static_cast<double>(P::uniform(r)) * w_max <= w(x) || trial >= kMaxTrials) return E - x;
         }
     };
     if constexpr (P::exact) {
@@ -954,10 +960,11 @@ template <class P, class G> __device__ inline double film_sample_phonon(const De
         // E within FP32 rounding of the gap (only reachable with low_qp_limit near 1):
         // the FP32 envelope can vanish, so such a quasiparticle is sampled in FP64
         if (!(w_max > 0.0f)) return sample64();
-        for (;;) {
+        for (uint32_t trial = 1;; ++trial) {
             const float u = P::uniform(r);
             const float xd = 1.0f + sp * (u * u);
-            if (P::uniform(r) * w_max <= w(xd)) return E - (d + span * static_cast<double>(u * u));
+            if (P::uniform(r) * w_max <= w(xd) || trial >= kMaxTrials)
+                return E - (d + span * static_cast<double>(u * u));
         }
     }
 }
@@ -981,7 +988,9 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
     const double init_e = energy / n_pairs;
     for (int i = 0; i < n_pairs; ++i) ph.push(init_e);
     double edep = 0.0;
-    while (qp.n > 0 || ph.n > 0) {
+    // a generation loop of the cascade (the reference's is unbounded); past kMaxTrials
+    // generations the phonons still in the film escape, which keeps the ledger closed
+    for (uint32_t gen = 1; qp.n > 0 || ph.n > 0; ++gen) {
         if (ph.n > 0) {                               // A: pair breaking
             nph.n = 0;
             for (int i = 0; i < ph.n; ++i) {
@@ -1020,7 +1029,7 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
                 const R u_side = P::uniform(r);
                 const R u_ang = P::uniform(r);
                 const double frac = static_cast<double>(P::div(R((u_side < R(0.5)) ? 0.5 : 1.5), P::cos_(u_ang * R(1.47))));
-                if (P::uniform(r) < R(film_escape_probability<P>(f, E, frac))) out.push(E);
+                if (P::uniform(r) < R(film_escape_probability<P>(f, E, frac)) || gen >= kMaxTrials) out.push(E);
                 else if (f.keep_all_phonons || !(E < 2.0 * f.gap)) nph.push(E);
             }
             const KList t = ph; ph = nph; nph = t;

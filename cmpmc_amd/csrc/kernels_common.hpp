@@ -98,7 +98,9 @@ template <class P> __device__ inline void fill_hit(cmpmc_sensor_hit& h, V3<typen
 
 // ------------------------------------------------------------------ device generators
 // Documented draw orders (the reference leaves the generator to the user):
-//   POINT: k = isotropic_dir(event stream).
+//   POINT: k = isotropic_dir(event stream); with mode 3 the mode is first drawn as the
+//          reference's benchmark generator does (cmpmc_bench.cpp:63-65): u < 0.53539 ST,
+//          u < 0.90217 FT, else L.
 //   LOGN:  n = clip(llround(exp(ln median + sigma * normal())), n_min, n_max) from the event
 //          stream; then per phonon j: r = R sqrt(u), phi = 2 pi u, z = H (2u - 1), mode =
 //          sample_dos_mode, E = e_lo + (e_hi - e_lo) u, k = isotropic_dir.
@@ -141,10 +143,15 @@ template <class P>
 __device__ inline void workload_phonon(const DevModel& m, const cmpmc_workload& w, Rng& r, PTrack<P>& t, double& E) {
     using R = typename P::R;
     if (w.kind == CMPMC_WL_PHONON_POINT) {
+        int md = w.mode;
+        if (md == CMPMC_MODE_BENCH_DOS) {
+            const R u = P::uniform(r);
+            md = (u < R(0.53539)) ? 2 : ((u < R(0.90217)) ? 1 : 0);
+        }
         const V3<R> k = isotropic_dir<P>(r);
         const V3<R> o{R(w.origin[0]), R(w.origin[1]), R(w.origin[2])};
         E = w.energy_eV;
-        make_track<P>(m, t, o, k, w.mode, E, 0.0, 1.0);
+        make_track<P>(m, t, o, k, md, E, 0.0, 1.0);
         return;
     }
     const V3<R> pos = uniform_volume<P>(m, r);

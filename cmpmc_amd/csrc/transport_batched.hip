@@ -447,7 +447,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                 int c = 0;
                 if (lid == 0) c = *a.cancel;
                 if (__builtin_amdgcn_readfirstlane(c)) {
-                    if (lid == 0) atomicOr(&a.status[2], 1ull);
+                    if (lid == 0) atomicOr(&a.status[kStCancelled], 1ull);
                     stop = true;
                 }
             }
@@ -474,8 +474,8 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
         if (stop) break;
     }
     if (ctx.overflow) {
-        atomicOr(&a.status[0], 1ull);
-        atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
+        atomicOr(&a.status[kStOverflow], 1ull);
+        atomicMax(&a.status[kStNeed], static_cast<unsigned long long>(ctx.need));
     }
     if (lid == 0 && *ctx.hvalid) atomicAdd(a.hit_valid, static_cast<unsigned long long>(*ctx.hvalid));
     {
@@ -581,7 +581,8 @@ template <class P> struct ChargeCtx {
     BlockStats* bs;
     BCharge* ls;
     uint32_t sp, depth, lanes, lane;
-    bool overflow;
+    uint32_t lost;                       // knock-on carriers this lane's full stack turned away
+    bool overflow;                       // the Luke queue was full
     uint64_t event;
     BTrack* luke_out;
     unsigned long long* luke_count;
@@ -599,9 +600,11 @@ template <class P> struct ChargeCtx {
     __device__ void luke_eV(double v) { bs_ledger(*bs, L_LUKE, v); }
     __device__ void recomb(double v) { bs_ledger(*bs, L_RECOMB, v); }
     __device__ void branch(int) {}
+    // A knock-on carrier (trap ionisation) goes on the lane's stack; a full stack turns it
+    // away, and the host reruns the batch with a stack deep enough for depth + lost.
     __device__ void push_charge(CTrack<P>& c, CTrack<P>& parent) {
         c.key = child_key(parent.key, ++parent.spawn_no);
-        if (sp >= depth) { overflow = true; bs_ledger(*bs, L_ESC, static_cast<double>(c.ekin) * c.weight); return; }
+        if (sp >= depth) { ++lost; bs_ledger(*bs, L_ESC, static_cast<double>(c.ekin) * c.weight); return; }
         ctrack_to_bcharge(c, event, ls[static_cast<size_t>(sp) * lanes + lane]);
         ++sp;
     }
@@ -640,6 +643,7 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
     ctx.depth = a.depth;
     ctx.lanes = a.lanes;
     ctx.lane = lane;
+    ctx.lost = 0;
     ctx.overflow = false;
     ctx.event = 0;
     ctx.luke_out = a.luke_out;
@@ -692,7 +696,7 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
             int cc = 0;
             if (__lane_id() == 0) cc = *a.cancel;
             if (__shfl(cc, 0)) {
-                if (__lane_id() == 0) atomicOr(&a.status[2], 1ull);
+                if (__lane_id() == 0) atomicOr(&a.status[kStCancelled], 1ull);
                 break;
             }
         }
@@ -701,7 +705,8 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
             if (alive && ++steps >= 20000000u) alive = false;   // engine.hpp:210 step cap
         }
     }
-    if (ctx.overflow) atomicOr(&a.status[0], 1ull);
+    if (ctx.overflow) atomicOr(&a.status[kStOverflow], 1ull);
+    if (ctx.lost) atomicMax(&a.status[kStChargeNeed], static_cast<unsigned long long>(a.depth + ctx.lost));
     __syncthreads();
     bs_flush(s_stats, a.counters, a.ledger);
 }

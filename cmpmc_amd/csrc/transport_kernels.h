@@ -17,6 +17,16 @@ namespace cmpmc_dev {
 struct DevModel;
 struct DevHit;
 
+// Status words of a launch (the accumulator block's status[4]):
+//   kStOverflow    a stack, pool or queue of the launch was too small;
+//   kStNeed        the size to ask for on the rerun (plain stacks, spawn pools);
+//   kStCancelled   the launch saw the host's cancel word raised and stopped early;
+//   kStChargeNeed  batched charge kernel: knock-on stack depth to ask for (0 = fits).
+enum StatusWord : int { kStOverflow = 0, kStNeed = 1, kStCancelled = 2, kStChargeNeed = 3 };
+
+// Steps between two reads of the cancel word by a lane of the plain kernel.
+constexpr uint32_t kPollSteps = 4096;
+
 // Plain per-event form: one lane per event, stacks in per-event regions.
 struct PlainArgs {
     uint64_t n_events;
@@ -40,9 +50,9 @@ struct PlainArgs {
     int32_t* first_branch;
     unsigned long long* counters;      // kNumCounters
     unsigned long long* ledger;        // 2 * L_N
-    unsigned long long* status;        // [0] overflow flag, [1] deepest stack needed
+    unsigned long long* status;        // StatusWord: overflow, deepest stack needed, cancelled
     double* kaplan_ws;                 // 5 * CMPMC_KCAP doubles per lane, lane-contiguous
-    const volatile int32_t* cancel;
+    const volatile int32_t* cancel;    // host-mapped cancel word (NULL: not polled)
 };
 
 // Compact record of a queued track of the batched form (80 B). Its random stream is
@@ -154,7 +164,7 @@ struct BatchedArgs {
     int32_t* first_branch;             // per event (relative to first_event)
     unsigned long long* counters;
     unsigned long long* ledger;
-    unsigned long long* status;        // [0] overflow flag, [1] largest pool needed, [2] cancelled
+    unsigned long long* status;        // StatusWord: overflow, largest pool needed, cancelled
     double* kaplan_ws;                 // 5 * CMPMC_KCAP doubles per lane, lane-contiguous
     const volatile int32_t* cancel;
     // differentiable form

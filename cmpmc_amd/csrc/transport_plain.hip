@@ -18,6 +18,7 @@ template <class P> struct PlainCtx {
     cmpmc_charge_track* cs;
     uint32_t csp, ccap;
     uint32_t need;           // deepest stack requirement seen (for capacity errors)
+    uint32_t clost;          // knock-on carriers a full charge stack turned away
     bool overflow;
     // hits
     DevHit* hits;
@@ -34,6 +35,12 @@ template <class P> struct PlainCtx {
     uint32_t kstride;
     const double* lt;
     const double* tt;
+    bool cancelled;          // the host raised the cancel word: the lane abandons its event
+
+    __device__ bool poll(const volatile int32_t* cancel) {
+        if (cancel && *cancel) cancelled = true;
+        return cancelled;
+    }
 
     __device__ void count(Counter c) { bs_count(*bs, c); }
     __device__ void ccount(CCounter c) { bs_count(*bs, C_NCOUNTERS + c); }
@@ -62,7 +69,7 @@ template <class P> struct PlainCtx {
     __device__ void push(PTrack<P>& child, PTrack<P>&) { store_phonon(child); }
     __device__ void push_luke(PTrack<P>& ph, CTrack<P>&) { store_phonon(ph); }
     __device__ void push_charge(CTrack<P>& c, CTrack<P>&) {
-        if (csp >= ccap) { overflow = true; return; }
+        if (csp >= ccap) { overflow = true; ++clost; return; }
         cmpmc_charge_track& o = cs[csp++];
         o.pos[0] = c.pos.x; o.pos[1] = c.pos.y; o.pos[2] = c.pos.z;
         o.p_eV[0] = c.p.x; o.p_eV[1] = c.p.y; o.p_eV[2] = c.p.z;
@@ -140,19 +147,25 @@ __device__ void plain_event(const DevModel& m, const PlainArgs& a, PlainCtx<P>& 
         if (a.injected) bs_ledger(*ctx.bs, L_INJ, a.injected[lane]);
     }
 
-    // transport_charges: engine.hpp:204-379
-    while (ctx.csp > 0 && !ctx.overflow) {
+    // transport_charges: engine.hpp:204-379. Every kPollSteps steps the lane reads the
+    // cancel word, so a run the host cancels (or whose time budget runs out) ends within
+    // that many steps of any lane.
+    while (ctx.csp > 0 && !ctx.overflow && !ctx.cancelled) {
         CTrack<P> c;
         load_charge<P>(ctx.cs[--ctx.csp], c);
-        for (uint32_t step = 0; step < 20000000u; ++step)
+        for (uint32_t step = 0; step < 20000000u; ++step) {
             if (!charge_step<P>(m, c, rng, ctx)) break;
+            if ((step & (kPollSteps - 1)) == kPollSteps - 1 && ctx.poll(a.cancel)) break;
+        }
     }
     // transport: engine.hpp:412-581
-    while (!a.skip_phonons && ctx.psp > 0 && !ctx.overflow) {
+    uint32_t steps = 0;
+    while (!a.skip_phonons && ctx.psp > 0 && !ctx.overflow && !ctx.cancelled) {
         PTrack<P> t;
         load_phonon<P>(ctx.ps[--ctx.psp], t);
         if (ctx.first_pending) { ctx.rec = true; ctx.first_pending = false; }
         while (phonon_step<P>(m, t, rng, ctx)) {
+            if ((++steps & (kPollSteps - 1)) == 0 && ctx.poll(a.cancel)) break;
         }
         ctx.rec = false;
     }
@@ -173,7 +186,7 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
     bs_init(s_stats, a.ledger);
     __syncthreads();
     const uint64_t lane = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool active = lane < a.n_events && !(a.cancel && *a.cancel);
+    const bool active = lane < a.n_events;
     if (active) {
         PlainCtx<P> ctx;
         ctx.bs = &s_stats;
@@ -184,6 +197,7 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
         ctx.ccap = a.cstack ? static_cast<uint32_t>(a.cstack_off[lane + 1] - a.cstack_off[lane]) : 0;
         ctx.csp = a.n_init_ch ? a.n_init_ch[lane] : 0;
         ctx.need = ctx.psp;
+        ctx.clost = 0;
         ctx.overflow = false;
         ctx.hits = a.hits;
         ctx.hit_count = a.hit_count;
@@ -198,13 +212,17 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
         ctx.kstride = 1;
         ctx.lt = s_cdf;
         ctx.tt = s_cdf + CMPMC_N_CDF;
-        plain_event<P>(m, a, ctx, lane);
+        ctx.cancelled = false;
+        if (!ctx.poll(a.cancel)) plain_event<P>(m, a, ctx, lane);
         if (a.first_branch) a.first_branch[lane] = ctx.first_branch;
         if (a.hits_per_event) a.hits_per_event[lane] = a.skip_phonons ? ctx.psp : ctx.seq;
-        if (ctx.overflow) {
-            atomicOr(&a.status[0], 1ull);
-            atomicMax(&a.status[1], static_cast<unsigned long long>(ctx.need));
+        if (ctx.clost) {           // the charge stack region was too small
+            atomicMax(&a.status[kStChargeNeed], static_cast<unsigned long long>(ctx.ccap + ctx.clost));
+        } else if (ctx.overflow) {
+            atomicOr(&a.status[kStOverflow], 1ull);
+            atomicMax(&a.status[kStNeed], static_cast<unsigned long long>(ctx.need));
         }
+        if (ctx.cancelled) atomicOr(&a.status[kStCancelled], 1ull);
     }
     __syncthreads();
     bs_flush(s_stats, a.counters, a.ledger);

@@ -187,9 +187,13 @@ enum {
     CMPMC_WL_PAIR = 3,           /* one e-/h+ pair (uniform volume unless pair_at_origin) */
     CMPMC_WL_PHONON_UNIF = 5     /* n ~ U{n_min..n_max} phonons (mode, energy), uniform volume */
 };
+/* Workload mode 3 (POINT only): the initial mode is drawn from the event stream before
+ * the direction, as the reference's benchmark generator does (benchmarks/cmpmc_bench.cpp:
+ * 63-65): ST if u < 0.53539, FT if u < 0.90217, else L. */
+#define CMPMC_MODE_BENCH_DOS 3
 typedef struct {
     int32_t kind;
-    int32_t mode;
+    int32_t mode;                 /* 0 L, 1 FT, 2 ST; 3 CMPMC_MODE_BENCH_DOS (POINT only) */
     double energy_eV;
     double origin[3];
     double logn_median, logn_sigma;
@@ -205,12 +209,19 @@ typedef struct {
     int32_t want_first_branch;
     int32_t block_size;           /* 0 = library default / tuned */
     uint64_t seed;                /* run seed (sub-streams of the batched form, device workloads) */
-    uint64_t stack_headroom;      /* plain form: per-event stack entries beyond the initial ones (0 = 512) */
+    uint64_t stack_headroom;      /* plain form: per-event stack entries beyond the initial ones (0 = 128, 8192
+                                     with Luke phonons); batched form: knock-on carrier stack entries per
+                                     lane of the charge kernel (0 = 64). Stacks that prove too small are
+                                     enlarged and the work rerun with the same result. */
     uint64_t lane_stack_depth;    /* batched form: spawn-pool entries per wavefront (0 = 2048) */
     uint32_t grid_blocks;         /* batched form: persistent grid size (0 = tuned) */
     uint32_t refill_period;       /* batched form: steps between refills of dead lanes (0 = tuned) */
     int32_t poll_cancel;          /* kernels poll the model's cancel flag (cmpmc_model_cancel) */
     int32_t waves_per_simd;       /* batched form: register-budget variant 2, 3 or 4 (0 = tuned) */
+    double time_budget_s;         /* > 0: the run stops with CMPMC_ERR_CANCELLED once it has run this long
+                                     (a host watchdog raises the cancel word the kernels poll; the plain
+                                     form's lanes poll it every 4096 steps). The plain form refuses events
+                                     of more than 1024 initial phonons unless this is set. */
 } cmpmc_run_opts;
 
 typedef struct {

@@ -152,6 +152,7 @@ class HitVec(C.Structure):
 
 
 WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_MIXED, WL_PHONON_UNIF = 1, 2, 3, 4, 5
+MODE_BENCH_DOS = 3     # POINT workload: mode drawn as benchmarks/cmpmc_bench.cpp:63-65 does
 BRANCH_NAMES = {0: "none", 1: "killed", 2: "isotope", 3: "poly", 4: "decay_TT", 5: "decay_LT",
                 6: "film", 7: "escape", 8: "surf_anh", 9: "specular", 10: "diffuse",
                 11: "roulette", 12: "film_miss"}
@@ -302,6 +303,15 @@ def germanium() -> Crystal:
     return c
 
 
+def trap_ion_crystal() -> Crystal:
+    """Germanium with charge trapping (0.5 m for electrons and holes) and the four trap
+    impact-ionisation channels on (0.2 m each; charge_kernels.hpp:101-126)."""
+    c = germanium()
+    c.e_trapping_mfp_m = c.h_trapping_mfp_m = 0.5
+    c.e_trap_ion_d0_mfp_m = c.e_trap_ion_a_mfp_m = c.h_trap_ion_d_mfp_m = c.h_trap_ion_a0_mfp_m = 0.2
+    return c
+
+
 def default_detector() -> Detector:
     d = Detector()
     lib().orc_detector_default(C.byref(d))
@@ -332,6 +342,15 @@ def demo_polynomial_surfaces(det: Detector) -> None:
             s.anh[j] = v
         s.anh_cutoff, s.refl_cutoff = 520.0, 350.0
     det.max_bounces = 100
+
+
+def constant_split_surfaces(det: Detector, f_diffuse: float) -> None:
+    """Every surface reflects diffusely with probability f_diffuse and specularly
+    otherwise, at every energy (constant spec/diff polynomials, no down-conversion)."""
+    for i in range(3):
+        s = det.surf[i]
+        s.n_spec, s.n_diff, s.n_anh = 1, 1, 0
+        s.spec[0], s.diff[0], s.spec_prob_const = 1.0 - f_diffuse, f_diffuse, 1.0 - f_diffuse
 
 
 def workload(kind: int, **kw) -> Workload:

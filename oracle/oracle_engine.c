@@ -691,6 +691,10 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
     }
     if (kind == ORC_WL_PHONON_POINT) {
         double k[3];
+        if (mode == 3) {     /* benchmarks/cmpmc_bench.cpp:63-65: ST below 0.53539, FT below 0.90217, else L */
+            const double u = orc_uniform(ev_rng);
+            mode = u < 0.53539 ? 2 : (u < 0.90217 ? 1 : 0);
+        }
         orc_isotropic_dir(ev_rng, k);
         orc_track t;
         orc_make_track(e, origin, k, mode, energy, 0.0, 1.0, &t);

@@ -253,11 +253,17 @@ long ref_run_events(const orc_crystal* crystal, const orc_detector* det, const o
         };
         auto gen = [&](EventInput& in) {
             switch (w.kind) {
-                case ORC_WL_PHONON_POINT:
+                case ORC_WL_PHONON_POINT: {
+                    PhononMode m = static_cast<PhononMode>(w.mode);
+                    if (w.mode == 3) {         // the mode draw of benchmarks/cmpmc_bench.cpp:63-65
+                        const double u = in.rng.uniform();
+                        m = u < 0.53539 ? PhononMode::ST : u < 0.90217 ? PhononMode::FT : PhononMode::L;
+                    }
                     in.phonons.push_back(engine.make_track({w.origin[0], w.origin[1], w.origin[2]}, in.rng.isotropic_dir(),
-                                                           static_cast<PhononMode>(w.mode), w.energy_eV));
+                                                           m, w.energy_eV));
                     in.injected_eV += w.energy_eV;
                     break;
+                }
                 case ORC_WL_MIXED:
                     if (in.event_id % 2 == 0) {
                         in.phonons.push_back(engine.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::FT, 7.5e-3));

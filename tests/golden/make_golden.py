@@ -95,10 +95,29 @@ def main():
     dd = O.default_detector()
     O.demo_polynomial_surfaces(dd)
     runs["bench_demo"] = (O.workload(O.WL_PHONON_POINT, mode=2, energy_eV=7.5e-3), 100, 12345, dd)
+    # the reference benchmark's phonon workload itself: demo surfaces, 100 bounces, mode
+    # drawn by the benchmark's generator (benchmarks/cmpmc_bench.cpp:46-66)
+    runs["bench_dos"] = (O.workload(O.WL_PHONON_POINT, mode=O.MODE_BENCH_DOS, energy_eV=7.5e-3), 300, 12345, dd)
+    # Russian roulette every 32 bounces past 32, survival 0.5 (variance_reduction.hpp:17-29)
+    dr = O.default_detector()
+    dr.russian_roulette_bounces, dr.russian_roulette_survival = 32, 0.5
+    runs["roulette"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, dr)
+    # half-specular surfaces (constant split 0.5 / 0.5)
+    ds = O.default_detector()
+    O.constant_split_surfaces(ds, 0.5)
+    runs["specular"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, ds)
+    # charge drift at 4 V with trapping and trap ionisation on (all four ionisation
+    # lengths 0.2 m, trapping 0.5 m)
+    dt = O.default_detector()
+    O.set_bias(dt, 4.0)
+    dt.luke_sample_fraction = 0.0
+    runs["trap_ion"] = (O.workload(O.WL_PAIR), 6, 42, dt, O.trap_ion_crystal())
     runs["c1_exact"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector())
-    for tag, (wl, n, seed, d) in runs.items():
+    for tag, spec in runs.items():
+        wl, n, seed, d = spec[:4]
+        cr = spec[4] if len(spec) > 4 else None
         R.ref_set_kinematics_mode(1 if tag.endswith("_exact") else 0)
-        r = O.ref_run(wl, n, seed, detector=d, threads=4)
+        r = O.ref_run(wl, n, seed, detector=d, crystal=cr, threads=4)
         R.ref_set_kinematics_mode(0)
         out[f"run_{tag}_stats"] = np.frombuffer(r["stats_raw"], dtype=np.uint8)
         out[f"run_{tag}_offsets"] = r["hit_offsets"]

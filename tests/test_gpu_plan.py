@@ -173,3 +173,11 @@ def test_plan_create_reports_switched_off(lib_cmpmc):
     with pytest.raises(cm.CmpmcError) as ei:
         eng.plan(cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 100)
     assert ei.value.code == 1 and "switched off" in str(ei.value)
+
+
+def test_autotune_reports_switched_off(lib_cmpmc):
+    cm = lib_cmpmc
+    eng = cm.Engine()
+    with pytest.raises(cm.CmpmcError) as ei:
+        eng.autotune(cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 100)
+    assert ei.value.code == 1 and "switched off" in str(ei.value)

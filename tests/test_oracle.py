@@ -186,23 +186,38 @@ def _gold_runs(O):
     d3.luke_sample_fraction = 0.0
     dd = O.default_detector()
     O.demo_polynomial_surfaces(dd)
+    dr = O.default_detector()
+    dr.russian_roulette_bounces, dr.russian_roulette_survival = 32, 0.5
+    ds = O.default_detector()
+    O.constant_split_surfaces(ds, 0.5)
+    dt = O.default_detector()
+    O.set_bias(dt, 4.0)
+    dt.luke_sample_fraction = 0.0
     return {
         "c1": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 400, 42, O.default_detector()),
         "lognormal": (O.workload(O.WL_PHONON_LOGN, logn_median=4.0), 40, 42, O.default_detector()),
         "mixed": (O.workload(O.WL_MIXED), 41, 77, det),
         "pairs": (O.workload(O.WL_PAIR), 6, 42, d3),
         "bench_demo": (O.workload(O.WL_PHONON_POINT, mode=2, energy_eV=7.5e-3), 100, 12345, dd),
+        "bench_dos": (O.workload(O.WL_PHONON_POINT, mode=O.MODE_BENCH_DOS, energy_eV=7.5e-3), 300, 12345, dd),
+        "roulette": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, dr),
+        "specular": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, ds),
+        "trap_ion": (O.workload(O.WL_PAIR), 6, 42, dt, O.trap_ion_crystal()),
         "c1_exact": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector()),
     }
 
 
-@pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo", "c1_exact"])
+@pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo", "bench_dos", "roulette",
+                                 "specular", "trap_ion", "c1_exact"])
 def test_run_events_golden(oracle, gold, tag):
     """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's
-    (c1_exact: with KinematicsMode::Exact)."""
+    (c1_exact: with KinematicsMode::Exact; bench_dos: the reference benchmark's own phonon
+    workload; roulette, specular, trap_ion: Russian roulette, half-specular surfaces, trap
+    impact ionisation)."""
     O = oracle
-    wl, n, seed, det = _gold_runs(O)[tag]
-    eng = O.Engine(detector=det)
+    spec = _gold_runs(O)[tag]
+    wl, n, seed, det = spec[:4]
+    eng = O.Engine(crystal=spec[4] if len(spec) > 4 else None, detector=det)
     if tag.endswith("_exact"):
         eng.set_kinematics(1)
     r = eng.run(wl, n, seed, threads=4)
