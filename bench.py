@@ -26,12 +26,18 @@ lognormal_*  the lognormal workload (phonon count per event lognormal(200, 1) cl
              [1, 20000], uniform positions, DOS modes, E ~ U[1, 31] meV) on a bounded
              slice of events, batched FP32 (and the plain kernel on a smaller slice when
              --lognormal-plain-events is given)
-roofline     200 B per phonon step (read + write of a 100 B FP32 track record) over the
-             batched kernel's average device time per launch (HIP events on the launch
-             stream), against 8 TB/s HBM; traffic = rocprof FETCH_SIZE + WRITE_SIZE bytes
-             per launch when a committed measurement is present
+roofline     the binding resource of the batched phonon kernel, VALU issue: achieved =
+             VALU wave-instructions per phonon step (SQ_INSTS_VALU over steps, from the
+             committed rocprofv3 counters of the same kernel and workload) x this
+             launch's steps / its device time (HIP events on the launch stream), against
+             the issue peak (one wave64 VALU instruction per 4 cycles per SIMD, 4 SIMDs
+             per CU, at the clock the counter run measured); beside it the measured HBM
+             traffic per launch (rocprof FETCH_SIZE + WRITE_SIZE) against 8 TB/s, and the
+             200 B/step record model of the path's bytes
 cpu_baseline the reference implementation compiled in place (oracle/_ref) or else the
-             CPU oracle, on the host cores, over a bounded sample of the same workload
+             CPU oracle, timed on the host: all cores the process may use (the box's CPU
+             share) and one thread, on bounded samples of the headline workload; the
+             same beside the lognormal, charge-drift and Luke-cascade lines
 """
 from __future__ import annotations
 
@@ -48,6 +54,8 @@ METRIC = "phonon events/s (7.5 meV Ge), batched vs plain GPU; grad/fwd cost rati
 BYTES_PER_STEP_FP32 = 200.0
 BYTES_PER_STEP_FP64 = 320.0
 HBM_PEAK = 8.0e12
+# VALU issue of a CDNA compute unit: 4 SIMDs, one wave64 VALU instruction per 4 cycles each
+VALU_WAVE_INSTS_PER_CU_CYCLE = 1.0
 
 
 def _log(msg):
@@ -58,24 +66,96 @@ def _closure(s):
     return abs(s["total_injected_eV"] - s["total_deposited_eV"] - s["escaped_or_killed_eV"]) / s["total_injected_eV"]
 
 
-def cpu_baseline(n_events: int, seed: int, threads: int) -> dict:
-    """Times the reference (compiled in place) or else the oracle on `threads` host cores
-    over the first n_events of the 7.5 meV workload (run_events, FP64)."""
+def host_info() -> dict:
+    """The host the CPU legs run on: logical CPUs, those this process may use, the
+    threads the legs use (the CPU share the environment sets, else all usable CPUs) and
+    the CPU model."""
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "threads": min(usable, share) if share > 0 else usable,
+            "cpu_model": model}
+
+
+def cpu_leg(wl_kind: str, n_events: int, seed: int, threads: int, crystal=None, detector=None) -> dict:
+    """Wall time of the reference's run_events (compiled in place) or else the oracle, on
+    `threads` host threads, generator and transport timed, lattice build excluded
+    (parallel.hpp:79, 146), without hit storage."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ctypes as O
-    wl = O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
+    wl = {"c1": lambda: O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3),
+          "lognormal": lambda: O.workload(O.WL_PHONON_LOGN),
+          "pair": lambda: O.workload(O.WL_PAIR)}[wl_kind]()
     kind = "reference" if O.ref_available() else "port"
+    chunk = 16 if wl_kind == "c1" else 1
     t = time.perf_counter()
     if kind == "reference":
-        r = O.ref_run(wl, n_events, seed, threads=threads, keep_hits=False, hit_cap=1)
+        r = O.ref_run(wl, n_events, seed, detector=detector, crystal=crystal, threads=threads, chunk=chunk,
+                      keep_hits=False, hit_cap=1)
     else:
-        r = O.Engine().run(wl, n_events, seed, threads=threads, keep_hits=False)
+        r = O.Engine(crystal=crystal, detector=detector).run(wl, n_events, seed, threads=threads, chunk=chunk,
+                                                             keep_hits=False)
     wall = time.perf_counter() - t
-    steps = r["stats"]["total_steps"]
-    return {"value": n_events / wall, "unit": "events/s", "cores": threads, "kind": kind,
-            "sample": f"first {n_events} events of the 7.5 meV single-phonon workload, seed {seed}, FP64 "
-                      f"run_events without hit storage: {steps} phonon steps in {wall:.2f} s",
-            "phonon_steps_per_s": steps / wall}
+    s = r["stats"]
+    return {"events": n_events, "threads": threads, "kind": kind, "wall_s": wall, "events_per_s": n_events / wall,
+            "phonon_steps_per_s": s["total_steps"] / wall, "charge_steps_per_s": s["charge_steps"] / wall}
+
+
+def cpu_pair(wl_kind: str, n_all: int, n_one: int, seed: int, host: dict, crystal=None, detector=None) -> dict:
+    """The same workload on all the host threads and on one thread (the first events of
+    the seed in each)."""
+    a = cpu_leg(wl_kind, n_all, seed, host["threads"], crystal, detector)
+    o = cpu_leg(wl_kind, n_one, seed, 1, crystal, detector)
+    _log(f"cpu {wl_kind}: {a['events_per_s']:.4g} events/s on {a['threads']} threads, "
+         f"{o['events_per_s']:.4g} on 1 thread")
+    return {"all_cores": a, "one_thread": o}
+
+
+def oracle_configs():
+    """Crystal and detector of the charge-drift and Luke-cascade lines for the CPU legs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes as O
+    c3 = O.germanium()
+    c3.e_trapping_mfp_m = c3.h_trapping_mfp_m = 0.5
+    d3 = O.default_detector()
+    O.set_bias(d3, 4.0)
+    d3.luke_sample_fraction = 0.0
+    d4 = O.default_detector()
+    O.set_bias(d4, 4.0)
+    d4.luke_sample_fraction = 1.0
+    return (c3, d3), (O.germanium(), d4)
+
+
+def roofline(steps_per_launch: float, launch_s: float, device: str) -> dict:
+    """The batched phonon kernel against its binding resource, VALU issue, from the
+    committed counter summary of the same kernel and workload (VALU wave-instructions per
+    phonon step, the clock of that run); HBM traffic per launch as the secondary ratio."""
+    prof = os.path.join(ROOT, "profiles", "valu_batched_phonon_7p5meV.json")
+    p = json.load(open(prof))
+    per_step = p["valu_wave_insts_per_step"]
+    n_cu = int(device.split(",")[1].split()[0])
+    peak = VALU_WAVE_INSTS_PER_CU_CYCLE * n_cu * p["clock_hz"]
+    achieved = per_step * steps_per_launch / launch_s
+    hbm = p.get("hbm_bytes_per_launch")
+    r = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "wave-instr/s", "frac": achieved / peak,
+         "traffic": hbm, "kernel": "transport_batched_kernel", "kernel_ms_avg": launch_s * 1e3,
+         "phonon_steps_per_launch": steps_per_launch, "valu_wave_insts_per_step": per_step,
+         "clock_hz": p["clock_hz"], "source": "profiles/" + p["source"]}
+    if hbm:
+        r["hbm"] = {"achieved_GBps": hbm / launch_s / 1e9, "peak_GBps": HBM_PEAK / 1e9,
+                    "frac": hbm / launch_s / HBM_PEAK}
+    # the path's per-step record model (a 100 B FP32 track read and written every step);
+    # tracks stay in registers here, so this is a byte count of the work, not traffic
+    r["record_model"] = {"bytes_per_step": BYTES_PER_STEP_FP32,
+                         "GBps": steps_per_launch * BYTES_PER_STEP_FP32 / launch_s / 1e9}
+    return r
 
 
 def main():
@@ -85,13 +165,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--events", type=int, default=1_000_000)
     ap.add_argument("--plain-events", type=int, default=1_000_000)
-    ap.add_argument("--lognormal-events", type=int, default=100_000)
+    ap.add_argument("--lognormal-events", type=int, default=1_000_000)
     ap.add_argument("--grad-events", type=int, default=100_000)
     ap.add_argument("--charge-events", type=int, default=100_000)
     ap.add_argument("--luke-events", type=int, default=2_000)
     ap.add_argument("--luke-plain-events", type=int, default=0)
-    ap.add_argument("--cpu-events", type=int, default=100_000)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-events", type=int, default=100_000, help="headline CPU leg, all cores")
+    ap.add_argument("--cpu-events-1t", type=int, default=5_000, help="headline CPU leg, one thread")
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--waves", type=int, default=0, help="register-budget variant (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -166,18 +246,7 @@ def main():
         "batched_vs_plain": value / plain_value,
         "grad_fwd_ratio": None,
     }
-    achieved = steps * BYTES_PER_STEP_FP32 / kernel_s
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_batched_phonon_7p5meV.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("hbm_bytes_per_launch")
-    out["roofline"] = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK, "traffic": traffic,
-                       "kernel": "transport_batched_kernel", "kernel_ms_avg": sum(kms) / args.steps,
-                       "algorithmic_bytes_per_launch": steps / args.steps * BYTES_PER_STEP_FP32,
-                       "note": "algorithmic bytes model a track record read and written through HBM every "
-                               "step; tracks stay in registers here (measured traffic is `traffic`), so frac "
-                               "can pass 1; the binding resource is VALU issue (profiles/)"}
+    out["roofline"] = roofline(steps / args.steps, kernel_s / args.steps, info)
     out["plain_roofline"] = {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK / 1e9,
                              "achieved": rp["stats"]["total_steps"] * BYTES_PER_STEP_FP64 / (rp["kernel_ms"] / 1e3) / 1e9}
     out["plain_roofline"]["frac"] = out["plain_roofline"]["achieved"] * 1e9 / HBM_PEAK
@@ -268,10 +337,24 @@ def main():
 
     if not args.no_cpu:
         try:
-            out["cpu_baseline"] = cpu_baseline(args.cpu_events, args.seed, args.cpu_threads)
-            _log(f"cpu baseline: {out['cpu_baseline']['value']:.1f} events/s on {args.cpu_threads} threads")
+            host = host_info()
+            c1 = cpu_pair("c1", args.cpu_events, args.cpu_events_1t, args.seed, host)
+            a, o = c1["all_cores"], c1["one_thread"]
+            out["cpu_baseline"] = {
+                "value": a["events_per_s"], "unit": "events/s", "cores": a["threads"], "kind": a["kind"],
+                "sample": f"first {a['events']} events (all cores) and first {o['events']} events (one thread) of "
+                          f"the headline 7.5 meV workload, seed {args.seed}, FP64 run_events without hit storage, "
+                          f"generator + transport timed",
+                "phonon_steps_per_s": a["phonon_steps_per_s"], "one_thread": o, "host": host}
+            (c3, d3), (c4, d4) = oracle_configs()
+            if "lognormal_events" in out:
+                out["lognormal_cpu"] = cpu_pair("lognormal", 64, 4, args.seed, host)
+            if "charge" in out:
+                out["charge"]["cpu"] = cpu_pair("pair", 4000, 400, args.seed, host, crystal=c3, detector=d3)
+            if "luke" in out:
+                out["luke"]["cpu"] = cpu_pair("pair", 64, 4, args.seed, host, crystal=c4, detector=d4)
         except Exception as e:  # the CPU leg must not hide the GPU result
-            out["cpu_baseline"] = {"error": str(e)}
+            out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -150,6 +150,8 @@ typedef struct {
     double fermi_level_above_Ev_eV;
     uint32_t russian_roulette_bounces;
     double russian_roulette_survival;
+    int32_t charge_stepping;             /* DetectorModel::ChargeStepping: 0 FirstOrderDkCap, 1 LemanSecondOrder */
+    double charge_sampling_dk_frac;      /* LemanSecondOrder: sampling-step cap on |dk| / max(k_s, |k|) */
 } orc_detector;
 
 void orc_detector_default(orc_detector* d);

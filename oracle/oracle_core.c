@@ -575,6 +575,8 @@ void orc_detector_default(orc_detector* d) {
     d->fermi_level_above_Ev_eV = -1.0;
     d->russian_roulette_bounces = 0;
     d->russian_roulette_survival = 0.5;
+    d->charge_stepping = 0;
+    d->charge_sampling_dk_frac = 0.25;
 }
 
 static double eval_poly(const double* c, int n, double f) {   /* surface_property.hpp:182-189 */

@@ -78,6 +78,7 @@ class Detector(C.Structure):
         ("max_bounces", u32), ("min_phonon_eV", dbl), ("luke_sample_fraction", dbl),
         ("charge_max_dk_frac", dbl), ("charge_dt_cap_s", dbl), ("fermi_level_above_Ev_eV", dbl),
         ("russian_roulette_bounces", u32), ("russian_roulette_survival", dbl),
+        ("charge_stepping", C.c_int), ("charge_sampling_dk_frac", dbl),
     ]
 
 
@@ -149,6 +150,10 @@ class TrackVec(C.Structure):
 
 class HitVec(C.Structure):
     _fields_ = [("v", C.POINTER(Hit)), ("n", C.c_size_t), ("cap", C.c_size_t)]
+
+
+class ChargeVec(C.Structure):
+    _fields_ = [("v", C.POINTER(Charge)), ("n", C.c_size_t), ("cap", C.c_size_t)]
 
 
 WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_MIXED, WL_PHONON_UNIF = 1, 2, 3, 4, 5
@@ -245,6 +250,11 @@ def lib():
         L.orc_make_track.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(dbl), C.c_int, dbl, dbl, dbl,
                                      C.POINTER(Track)]
         L.orc_track_push.argtypes = [C.POINTER(TrackVec), C.POINTER(Track)]
+        L.orc_charge_push.argtypes = [C.POINTER(ChargeVec), C.POINTER(Charge)]
+        L.orc_make_charge.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), dbl, C.c_int, dbl,
+                                      C.POINTER(Charge)]
+        L.orc_transport_charges.argtypes = [C.c_void_p, C.POINTER(ChargeVec), C.POINTER(TrackVec), C.POINTER(HitVec),
+                                            C.POINTER(Stats), C.POINTER(Rng), C.c_int, u64]
         L.orc_crystal_germanium.argtypes = [C.POINTER(Crystal)]
         L.orc_detector_default.argtypes = [C.POINTER(Detector)]
         L.orc_film_aluminum.argtypes = [C.POINTER(Film)]
@@ -441,6 +451,30 @@ class Engine:
         libc.free(C.cast(vec.v, C.c_void_p))
         libc.free(C.cast(hv.v, C.c_void_p))
         return {"stats": st.as_dict(), "stats_raw": bytes(st), "hits": hits, "rng": [r.s[i] for i in range(4)]}
+
+    def transport_charges(self, carriers, rng: "Rng") -> dict:
+        """SimulationEngine::transport_charges of one charge stack (reference semantics, one
+        stream `rng`, advanced in place): carriers are (type, pos, dir, ekin_eV, valley) made
+        with ChargePhysics::make_charge. Returns the statistics, hits and the pushed Luke
+        phonons' energies."""
+        L = lib()
+        cv = ChargeVec()
+        for typ, pos, d, ek, valley in carriers:
+            c = Charge()
+            p = np.ascontiguousarray(pos, dtype=np.float64)
+            dd = np.ascontiguousarray(d, dtype=np.float64)
+            L.orc_make_charge(self.ptr, int(typ), dptr(p), dptr(dd), float(ek), int(valley), 0.0, C.byref(c))
+            L.orc_charge_push(C.byref(cv), C.byref(c))
+        pv, hv, st = TrackVec(), HitVec(), Stats()
+        L.orc_transport_charges(self.ptr, C.byref(cv), C.byref(pv), C.byref(hv), C.byref(st), C.byref(rng), 0, 0)
+        hits = np.frombuffer(bytes((Hit * hv.n).from_address(C.addressof(hv.v.contents))), dtype=HIT_DTYPE).copy() \
+            if hv.n else np.zeros(0, dtype=HIT_DTYPE)
+        ph_e = np.array([pv.v[i].energy for i in range(pv.n)])
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        for v in (cv.v, pv.v, hv.v):
+            libc.free(C.cast(v, C.c_void_p))
+        return {"stats": st.as_dict(), "hits": hits, "phonon_energies": ph_e}
 
     def run(self, wl: Workload, n_events: int, seed: int, threads: int = 0, chunk: int = 16,
             keep_hits: bool = True, batched: bool = False, first_branch: bool = False) -> dict:

@@ -538,10 +538,11 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
         for (uint32_t step = 0; step < 20000000u; ++step) {
             st->charge_steps++;
             const double* E = D->efield;
-            const double dk_frac = D->charge_max_dk_frac;
+            const int second_order = (D->charge_stepping == 1);
+            const double dk_frac = second_order ? D->charge_sampling_dk_frac : D->charge_max_dk_frac;
             const rates_t R0 = charge_rates(e, &trk, E);
-            const double r_luke = R0.luke, r_iv = R0.iv, r_trap = R0.trap, r_ion_e = R0.ion_e, r_ion_h = R0.ion_h;
-            const double r_tot = R0.tot;
+            double r_luke = R0.luke, r_iv = R0.iv, r_trap = R0.trap, r_ion_e = R0.ion_e, r_ion_h = R0.ion_h;
+            double r_tot = R0.tot;
             double dt_max = D->charge_dt_cap_s;
             if (dk_frac > 0.0) {
                 const int is_e = (trk.type == 1);
@@ -558,7 +559,29 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
                 const double dk_rate = v3_norm(E) * ORC_C_LIGHT / ORC_HBARC * (is_e ? hv_max_scale(L) : 1.0);
                 if (dk_rate > 0.0) dt_max = dmin(dt_max, dk_frac * dmax(ks, k_now) / dk_rate);
             }
-            const double dt_phys = (r_tot > 1.0) ? (-log(orc_uniform(R)) / r_tot) : INFINITY;
+            double dt_phys;
+            if (second_order) {
+                /* engine.hpp:234-254: the total rate taken linear over the sampling step,
+                 * Gamma(t) = a0 + a1 t with a0 the rate now and a1 from the rate at the end of
+                 * a full step of free acceleration; the event time solves
+                 * a0 t + a1 t^2 / 2 = -ln u (Leman, arXiv:1109.1193, Sec. V.E, Eqs. (28)-(30)) */
+                orc_charge trial = trk;
+                const double sm = q_sign * ORC_C_LIGHT * dt_max;
+                for (int i = 0; i < 3; ++i) trial.p[i] += E[i] * sm;
+                trial.ekin = (trial.type == 1) ? ekin_e(L, trial.valley, trial.p) : ekin_h(L, trial.p);
+                const double a0 = R0.tot;
+                const double a1 = (charge_rates(e, &trial, E).tot - a0) / dt_max;
+                const double lnu = log(orc_uniform(R));
+                if (fabs(a1) * dt_max <= 1e-9 * dmax(a0, 1e-300)) {
+                    dt_phys = (a0 > 0.0) ? (-lnu / a0) : INFINITY;
+                } else {
+                    const double disc = a0 * a0 - 2.0 * a1 * lnu;
+                    dt_phys = (disc < 0.0) ? INFINITY : (sqrt(disc) - a0) / a1;
+                    if (!(dt_phys >= 0.0)) dt_phys = INFINITY;
+                }
+            } else {
+                dt_phys = (r_tot > 1.0) ? (-log(orc_uniform(R)) / r_tot) : INFINITY;
+            }
             const double dt = dmin(dt_phys, dt_max);
             const double sdp = q_sign * ORC_C_LIGHT * dt;
             double dp[3], pm[3], vm[3];
@@ -608,6 +631,12 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
             for (int i = 0; i < 3; ++i) trk.p[i] += dp[i];
             trk.ekin = (trk.type == 1) ? ekin_e(L, trk.valley, trk.p) : ekin_h(L, trk.p);
             if (dt_phys <= dt_max) {
+                if (second_order) {   /* process chosen with the rates at the event (engine.hpp:337-342) */
+                    const rates_t Rv = charge_rates(e, &trk, E);
+                    r_luke = Rv.luke; r_iv = Rv.iv; r_trap = Rv.trap; r_ion_e = Rv.ion_e; r_ion_h = Rv.ion_h;
+                    r_tot = Rv.tot;
+                    if (r_tot <= 0.0) continue;
+                }
                 const double u_proc = orc_uniform(R) * r_tot;
                 if (u_proc < r_trap) {
                     st->charges_trapped++;

@@ -65,6 +65,9 @@ DetectorModel to_detector(const orc_detector* d) {
     m.fermi_level_above_Ev_eV = d->fermi_level_above_Ev_eV;
     m.russian_roulette_bounces = d->russian_roulette_bounces;
     m.russian_roulette_survival = d->russian_roulette_survival;
+    m.charge_stepping = d->charge_stepping ? DetectorModel::ChargeStepping::LemanSecondOrder
+                                           : DetectorModel::ChargeStepping::FirstOrderDkCap;
+    m.charge_sampling_dk_frac = d->charge_sampling_dk_frac;
     return m;
 }
 

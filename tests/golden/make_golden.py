@@ -112,6 +112,18 @@ def main():
     O.set_bias(dt, 4.0)
     dt.luke_sample_fraction = 0.0
     runs["trap_ion"] = (O.workload(O.WL_PAIR), 6, 42, dt, O.trap_ion_crystal())
+    # Leman second-order charge stepping (engine.hpp:234-254, 337-342): pairs at 4 V, Luke
+    # phonons counted; and with trapping / trap ionisation and 2 % of the Luke phonons tracked
+    dl = O.default_detector()
+    O.set_bias(dl, 4.0)
+    dl.luke_sample_fraction = 0.0
+    dl.charge_stepping = 1
+    runs["leman"] = (O.workload(O.WL_PAIR), 6, 42, dl)
+    dl2 = O.default_detector()
+    O.set_bias(dl2, 4.0)
+    dl2.luke_sample_fraction = 0.02
+    dl2.charge_stepping = 1
+    runs["leman_ion"] = (O.workload(O.WL_PAIR), 6, 43, dl2, O.trap_ion_crystal())
     runs["c1_exact"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector())
     for tag, spec in runs.items():
         wl, n, seed, d = spec[:4]

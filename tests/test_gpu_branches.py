@@ -141,14 +141,19 @@ def test_phonon_branches_both_forms(lib_cmpmc, oracle, which):
         assert np.any(gb["hits"]["edep"] > gb["hits"]["incident_energy"] * 1.5)
 
 
+# Trap impact-ionisation lengths of the four channels (e- on D0, e- on A-, h+ on D+,
+# h+ on A0), all different so that a channel mapped to the wrong projectile or liberated
+# carrier changes the mix of collected electrons and holes. Each carrier liberates about
+# 0.2 more (a sub-critical cascade: ~530 ionisations per 1,000 pairs).
+ION_MFP = (0.3, 0.5, 0.4, 0.8)
+
+
 def _charge_engines(cm, O, trap_ion):
-    cr, ocr = cm.germanium(), O.trap_ion_crystal() if trap_ion else O.germanium()
-    cr.e_trapping_mfp_m = cr.h_trapping_mfp_m = 0.5
-    if trap_ion:
-        cr.e_trap_ion_d0_mfp_m = cr.e_trap_ion_a_mfp_m = 0.2
-        cr.h_trap_ion_d_mfp_m = cr.h_trap_ion_a0_mfp_m = 0.2
-    else:
-        ocr.e_trapping_mfp_m = ocr.h_trapping_mfp_m = 0.5
+    cr, ocr = cm.germanium(), O.germanium()
+    for c in (cr, ocr):
+        c.e_trapping_mfp_m = c.h_trapping_mfp_m = 0.5
+        if trap_ion:
+            c.e_trap_ion_d0_mfp_m, c.e_trap_ion_a_mfp_m, c.h_trap_ion_d_mfp_m, c.h_trap_ion_a0_mfp_m = ION_MFP
     d, od = cm.default_detector(), O.default_detector()
     cm.set_bias(d, 4.0)
     O.set_bias(od, 4.0)
@@ -159,37 +164,47 @@ def _charge_engines(cm, O, trap_ion):
 @pytest.mark.parametrize("trap_ion", [False, True], ids=["trapping", "trap_ionisation"])
 def test_charge_drift_trapping_both_forms(lib_cmpmc, oracle, trap_ion):
     """C3 shape (e-/h+ pairs at 4 V, uniform positions, trapping 0.5 m), optionally with
-    trap impact ionisation (0.2 m for all four channels): both forms against the oracle
-    (plain mode and batched mode)."""
+    trap impact ionisation (four channels of different lengths): both forms against the
+    oracle (plain mode and batched mode). Knock-on carriers make the totals a branching
+    process (a sub-critical cascade has a large spread of total progeny), so quantities
+    are compared per carrier: Luke emissions, trapping and ionisation per carrier, and
+    the electron share of the collected carriers."""
     cm, O = lib_cmpmc, oracle
     eng, oeng = _charge_engines(cm, O, trap_ion)
     wl, owl = cm.workload(cm.WL_PAIR), O.workload(O.WL_PAIR)
-    n, seed = 400, 42
+    n, seed = 1000, 42
     print(f"case: charge drift trap_ion={trap_ion}, n={n} seed={seed}", flush=True)
-    keys = ("charges_trapped", "trap_ionizations", "charges_collected", "luke_phonons_emitted", "charge_steps")
     for form, batched in ((cm.FORM_PLAIN, False), (cm.FORM_BATCHED, True)):
         g = eng.run_workload(wl, n, seed=seed, form=form)
         o = oeng.run(owl, n, seed, batched=batched)
         gs, os_ = g["stats"], o["stats"]
         tag = f"{'trap-ion' if trap_ion else 'trapping'} {'batched' if batched else 'plain'}"
-        # every carrier ends collected or trapped
+        carriers = []
         for s in (gs, os_):
+            # every carrier ends collected or trapped
             assert s["charges_collected"] + s["charges_trapped"] == 2 * n + s["trap_ionizations"]
-        for k in keys[:3]:
-            _count_ok(gs[k], os_[k], f"{tag} {k}", nsig=4.0, rel_floor=0.0)
-        for k in keys[3:]:
-            assert gs[k] == pytest.approx(os_[k], rel=0.05), f"{tag} {k}"
+            carriers.append(2 * n + s["trap_ionizations"])
+        print(f"{tag}: gpu {[gs[k] for k in ('trap_ionizations', 'charges_trapped', 'luke_phonons_emitted')]}, "
+              f"oracle {[os_[k] for k in ('trap_ionizations', 'charges_trapped', 'luke_phonons_emitted')]}",
+              flush=True)
+        ng, no = carriers
+        assert gs["luke_phonons_emitted"] / ng == pytest.approx(os_["luke_phonons_emitted"] / no, rel=0.05), \
+            f"{tag} Luke emissions per carrier"
+        for k in ("charges_trapped", "trap_ionizations"):
+            pg, po = gs[k] / ng, os_[k] / no
+            sig = np.sqrt(max(pg, 1e-3) * (1 - pg) / ng + max(po, 1e-3) * (1 - po) / no)
+            assert abs(pg - po) < 4 * sig, f"{tag} {k} per carrier: {pg:.4f} vs {po:.4f}"
         if trap_ion:
-            assert gs["trap_ionizations"] > 50
+            assert gs["trap_ionizations"] > 300
         else:
             assert gs["trap_ionizations"] == 0 and gs["charges_trapped"] > 20
         h, q = g["hits"], o["hits"]
         assert len(h) == gs["charges_collected"]
+        eg, eo = np.mean(h["excitation"] == 1), np.mean(q["excitation"] == 1)
+        assert abs(eg - eo) < 4 * np.sqrt(0.25 / len(h) + 0.25 / len(q)), f"{tag} electron share {eg:.3f} vs {eo:.3f}"
         _ks_ok(h["time"], q["time"], f"{tag} collection time")
         _mean_ok(h["time"], q["time"], f"{tag} collection time")
-        _ks_ok(h["edep"], q["edep"], f"{tag} recombination energy")
-        for s in (gs,):
-            assert abs(s["total_deposited_eV"] - s["charge_recomb_edep_eV"]) <= 1e-9 * s["total_deposited_eV"]
+        assert abs(gs["total_deposited_eV"] - gs["charge_recomb_edep_eV"]) <= 1e-9 * gs["total_deposited_eV"]
 
 
 def test_charge_stack_overflow_reruns(lib_cmpmc, oracle):

@@ -193,6 +193,14 @@ def _gold_runs(O):
     dt = O.default_detector()
     O.set_bias(dt, 4.0)
     dt.luke_sample_fraction = 0.0
+    dl = O.default_detector()
+    O.set_bias(dl, 4.0)
+    dl.luke_sample_fraction = 0.0
+    dl.charge_stepping = 1
+    dl2 = O.default_detector()
+    O.set_bias(dl2, 4.0)
+    dl2.luke_sample_fraction = 0.02
+    dl2.charge_stepping = 1
     return {
         "c1": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 400, 42, O.default_detector()),
         "lognormal": (O.workload(O.WL_PHONON_LOGN, logn_median=4.0), 40, 42, O.default_detector()),
@@ -203,17 +211,19 @@ def _gold_runs(O):
         "roulette": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, dr),
         "specular": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, ds),
         "trap_ion": (O.workload(O.WL_PAIR), 6, 42, dt, O.trap_ion_crystal()),
+        "leman": (O.workload(O.WL_PAIR), 6, 42, dl),
+        "leman_ion": (O.workload(O.WL_PAIR), 6, 43, dl2, O.trap_ion_crystal()),
         "c1_exact": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector()),
     }
 
 
 @pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo", "bench_dos", "roulette",
-                                 "specular", "trap_ion", "c1_exact"])
+                                 "specular", "trap_ion", "leman", "leman_ion", "c1_exact"])
 def test_run_events_golden(oracle, gold, tag):
     """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's
     (c1_exact: with KinematicsMode::Exact; bench_dos: the reference benchmark's own phonon
     workload; roulette, specular, trap_ion: Russian roulette, half-specular surfaces, trap
-    impact ionisation)."""
+    impact ionisation; leman, leman_ion: Leman second-order charge stepping)."""
     O = oracle
     spec = _gold_runs(O)[tag]
     wl, n, seed, det = spec[:4]
