@@ -18,6 +18,7 @@ grad_*       the differentiated workload: 1e5 events of U{1..64} L-mode 7.5 meV 
 charge_*     e-/h+ pairs at 4 V across the 2.54 cm height, uniform in the volume, trapping
              on (electron and hole trapping lengths 0.5 m), Luke phonons counted: 1e5 pairs,
              batched FP32 and plain FP64 (pairs/s, charge steps/s)
+             (charge.leman: the same pairs with Leman second-order stepping)
 luke_*       the same pairs (trapping off) with every Luke phonon enqueued on the device
              and tracked to absorption: 2,000 events batched (the plain form on a slice
              when --luke-plain-events is given: its one-event-per-lane time is set by the
@@ -284,12 +285,13 @@ def main():
     if args.charge_events > 0 or args.luke_events > 0:
         pair = cm.workload(cm.WL_PAIR)
 
-        def pair_engine(luke_fraction, trap_mfp):
+        def pair_engine(luke_fraction, trap_mfp, leman=False):
             cr = cm.germanium()
             cr.e_trapping_mfp_m = cr.h_trapping_mfp_m = trap_mfp
             det = cm.default_detector()
             cm.set_bias(det, 4.0)
             det.luke_sample_fraction = luke_fraction
+            det.charge_stepping = 1 if leman else 0
             return cm.Engine(cm.Lattice(cr), det)
 
         def timed(e, n, form):
@@ -309,6 +311,18 @@ def main():
                              "luke_per_pair": sb["luke_phonons_emitted"] / args.charge_events,
                              "trap_mfp_m": 0.5}
             _log(f"charge drift: batched {tb:.3f} s, plain {tp:.3f} s for {args.charge_events} pairs")
+            # the same pairs with Leman second-order stepping (engine.hpp:234-254)
+            e3l = pair_engine(0.0, 0.5, leman=True)
+            timed(e3l, 1000, cm.FORM_BATCHED)
+            tbl, sbl = timed(e3l, args.charge_events, cm.FORM_BATCHED)
+            tpl, spl = timed(e3l, args.charge_events, cm.FORM_PLAIN)
+            out["charge"]["leman"] = {
+                "batched_pairs_per_s": args.charge_events / tbl, "plain_pairs_per_s": args.charge_events / tpl,
+                "batched_vs_plain": tpl / tbl, "speedup_vs_first_order_batched": tb / tbl,
+                "charge_steps_per_pair": sbl["charge_steps"] / args.charge_events,
+                "first_order_charge_steps_per_pair": sb["charge_steps"] / args.charge_events,
+                "luke_per_pair": sbl["luke_phonons_emitted"] / args.charge_events}
+            _log(f"charge drift, Leman: batched {tbl:.3f} s, plain {tpl:.3f} s")
         if args.luke_events > 0:
             e4 = pair_engine(1.0, 0.0)
             timed(e4, 100, cm.FORM_BATCHED)
@@ -322,6 +336,14 @@ def main():
                 out["luke"]["plain_events"] = args.luke_plain_events
                 out["luke"]["plain_events_per_s"] = args.luke_plain_events / tp
             _log(f"Luke cascade: batched {tb:.3f} s for {args.luke_events} events")
+            e4l = pair_engine(1.0, 0.0, leman=True)
+            timed(e4l, 100, cm.FORM_BATCHED)
+            tbl, sbl = timed(e4l, args.luke_events, cm.FORM_BATCHED)
+            out["luke"]["leman"] = {"batched_events_per_s": args.luke_events / tbl,
+                                    "speedup_vs_first_order": tb / tbl,
+                                    "luke_phonons_per_pair": sbl["luke_phonons_emitted"] / args.luke_events,
+                                    "deposited_eV_per_event": sbl["total_deposited_eV"] / args.luke_events}
+            _log(f"Luke cascade, Leman: batched {tbl:.3f} s")
 
     if not args.no_lognormal and args.lognormal_events > 0:
         lw = cm.workload(cm.WL_PHONON_LOGN)

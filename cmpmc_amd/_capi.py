@@ -60,7 +60,8 @@ class Detector(C.Structure):
     _fields_ = [("radius", dbl), ("half_z", dbl), ("efield", dbl * 3), ("surf", Surface * 3),
                 ("max_bounces", u32), ("russian_roulette_bounces", u32)] + [(n, dbl) for n in (
                     "min_phonon_eV", "luke_sample_fraction", "charge_max_dk_frac", "charge_dt_cap_s",
-                    "fermi_level_above_Ev_eV", "russian_roulette_survival")]
+                    "fermi_level_above_Ev_eV", "russian_roulette_survival")] + [
+                ("charge_stepping", i32), ("_pad_cs", i32), ("charge_sampling_dk_frac", dbl)]
 
 
 class Lattice(C.Structure):

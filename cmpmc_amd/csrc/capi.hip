@@ -134,6 +134,7 @@ extern "C" int cmpmc_detector_default(cmpmc_detector* d) {
     d->min_phonon_eV = 10.0 * 1e-6; d->luke_sample_fraction = 1.0;
     d->charge_max_dk_frac = 0.05; d->charge_dt_cap_s = 100.0 * 1e-12;
     d->fermi_level_above_Ev_eV = -1.0; d->russian_roulette_survival = 0.5;
+    d->charge_stepping = 0; d->charge_sampling_dk_frac = 0.25;
     return CMPMC_OK;
 }
 
@@ -192,13 +193,20 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     }
     m.max_bounces = det->max_bounces; m.rr_bounces = det->russian_roulette_bounces;
     m.min_phonon_eV = det->min_phonon_eV; m.luke_fraction = det->luke_sample_fraction;
-    m.dk_frac = det->charge_max_dk_frac; m.dt_cap = det->charge_dt_cap_s;
+    m.charge_stepping = det->charge_stepping;
+    m.dk_frac = det->charge_stepping == 1 ? det->charge_sampling_dk_frac : det->charge_max_dk_frac;
+    m.dt_cap = det->charge_dt_cap_s;
     m.fermi = det->fermi_level_above_Ev_eV; m.rr_survival = det->russian_roulette_survival;
     fill_fp32_constants(m);
 }
 
 static int check_detector(const cmpmc_detector* d) {
     if (!(d->radius > 0.0) || !(d->half_z > 0.0)) return fail(CMPMC_ERR_INVALID, "detector: radius and half_z must be positive");
+    if (d->charge_stepping != 0 && d->charge_stepping != 1)
+        return fail(CMPMC_ERR_INVALID, "detector: charge_stepping %d is not 0 (first order) or 1 (Leman second order)",
+                    d->charge_stepping);
+    if (d->charge_stepping == 1 && !(d->charge_sampling_dk_frac > 0.0))
+        return fail(CMPMC_ERR_INVALID, "detector: Leman stepping needs charge_sampling_dk_frac > 0");
     for (int s = 0; s < 3; ++s) {
         const cmpmc_surface& a = d->surf[s];
         if (a.n_spec < 0 || a.n_spec > CMPMC_MAX_POLY || a.n_diff < 0 || a.n_diff > CMPMC_MAX_POLY || a.n_anh < 0 ||

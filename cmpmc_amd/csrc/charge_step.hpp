@@ -1,5 +1,6 @@
 // One step of SimulationEngine::transport_charges (engine.hpp:210-377), first-order
-// (FirstOrderDkCap) stepping, uniform field, RecordOnly release at collection, with the
+// (FirstOrderDkCap) or Leman second-order (LemanSecondOrder) stepping, uniform field,
+// RecordOnly release at collection, with the
 // ChargePhysics leaves of charge/charge_transport.hpp:49-241 and the Herring-Vogt
 // kinematics of lattice/crystal_lattice.hpp:239-293.
 //
@@ -315,8 +316,46 @@ template <class P, class G> __device__ inline void reflect_charge(const DevModel
     }
 }
 
+// Time to the next bulk event under Leman second-order stepping (engine.hpp:234-254;
+// Leman, arXiv:1109.1193, Sec. V.E, Eqs. (28)-(30)): the total rate is taken linear over
+// the sampling step, Gamma(t) = a0 + a1 t, with a0 the rate now and a1 from the rate after
+// a full step of free acceleration; the event time solves a0 t + a1 t^2 / 2 = -ln u, and
+// is infinite when that has no positive root (a1 < 0 and too few events left). One draw,
+// as the first-order step's. The FP64 form evaluates the reference's expression; the FP32
+// form the same root as -2 ln u / (sqrt(disc) + a0), which does not cancel when
+// |a1| t << a0.
+template <class P, class G>
+__device__ inline typename P::R leman_event_time(const DevModel& m, const CTrack<P>& trk, V3<typename P::R> E,
+                                                 typename P::R q_sign, typename P::R dt_max, typename P::R a0, G& r) {
+    using R = typename P::R;
+    CTrack<P> trial;
+    trial.pos = trk.pos;
+    trial.valley = trk.valley;
+    trial.type = trk.type;
+    trial.p = trk.p + E * (q_sign * R(kCLight) * dt_max);
+    trial.ekin = (trk.type == 1) ? ekin_e<P>(m, trial.valley, trial.p) : ekin_h<P>(m, trial.p);
+    const R a1 = P::div(charge_rates<P>(m, trial).tot - a0, dt_max);
+    if constexpr (P::exact) {
+        const R lnu = log(P::uniform(r));
+        if (fabs(a1) * dt_max <= R(1e-9) * tmax(a0, R(1e-300))) return (a0 > R(0)) ? (-lnu / a0) : P::inf();
+        const R disc = a0 * a0 - R(2) * a1 * lnu;
+        R t = (disc < R(0)) ? P::inf() : (sqrt(disc) - a0) / a1;
+        if (!(t >= R(0))) t = P::inf();
+        return t;
+    } else {
+        const R lnu = P::log_(P::uniform(r));
+        const R disc = a0 * a0 - R(2) * a1 * lnu;
+        if (disc < R(0)) return P::inf();
+        R t = P::div(R(-2) * lnu, P::sqrt_(disc) + a0);
+        if (!(t >= R(0))) t = P::inf();
+        return t;
+    }
+}
+
 // One pass of the charge step loop. Returns false when the carrier ends (collected,
-// trapped) or exceeds the 20,000,000-step cap (the caller counts steps).
+// trapped) or exceeds the 20,000,000-step cap (the caller counts steps). The stepping is
+// the detector's: first order with the wavevector-change cap (the default), or Leman's
+// second order with the sampling cap (m.dk_frac holds the cap of the chosen stepping).
 template <class P, class G, class Ctx>
 __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx& ctx) {
     using R = typename P::R;
@@ -345,7 +384,10 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             if (dk_rate > R(0)) dt_max = tmin(dt_max, P::div(m.charge_f.dk_frac * tmax(m.charge_f.k_sound[h], k_now), dk_rate));
         }
     }
-    const R dt_phys = (R0.tot > R(1)) ? P::div(P::neg_log(P::uniform(r)), R0.tot) : P::inf();
+    const bool second_order = (m.charge_stepping == 1);
+    R dt_phys;
+    if (second_order) dt_phys = leman_event_time<P>(m, trk, E, q_sign, dt_max, R0.tot, r);
+    else dt_phys = (R0.tot > R(1)) ? P::div(P::neg_log(P::uniform(r)), R0.tot) : P::inf();
     const R dt = tmin(dt_phys, dt_max);
     const V3<R> dp = E * (q_sign * R(kCLight) * dt);
     const V3<R> pm = trk.p + dp * R(0.5);
@@ -385,14 +427,17 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
     trk.p = trk.p + dp;
     trk.ekin = (trk.type == 1) ? ekin_e<P>(m, trk.valley, trk.p) : ekin_h<P>(m, trk.p);
     if (dt_phys <= dt_max) {
-        const R u_proc = P::uniform(r) * R0.tot;
-        if (u_proc < R0.trap) {
+        // second order: the process is chosen with the rates at the event (engine.hpp:337-342)
+        const CRates<P> Rv = second_order ? charge_rates<P>(m, trk) : R0;
+        if (second_order && !(Rv.tot > R(0))) return true;
+        const R u_proc = P::uniform(r) * Rv.tot;
+        if (u_proc < Rv.trap) {
             ctx.ccount(CC_TRAPPED);
             ctx.escaped((static_cast<double>(trk.ekin) + 0.5 * m.bandgap) * trk.weight);
             return false;
-        } else if (u_proc < R0.trap + R0.ion_e + R0.ion_h) {
+        } else if (u_proc < Rv.trap + Rv.ion_e + Rv.ion_h) {
             ctx.ccount(CC_TRAP_ION);
-            const bool lib_e = (u_proc < R0.trap + R0.ion_e);
+            const bool lib_e = (u_proc < Rv.trap + Rv.ion_e);
             const uint64_t nval = (m.n_valleys > 1) ? static_cast<uint64_t>(m.n_valleys) : 1;
             const int kv = lib_e ? static_cast<int>(static_cast<uint16_t>(next_bits(r) % nval)) : 0;
             const V3<R> d = isotropic_dir<P>(r);
@@ -400,10 +445,10 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             make_charge<P>(m, k, lib_e ? 1 : 2, trk.pos, d, R(1.0e-6), kv, trk.time);
             k.weight = trk.weight;
             ctx.push_charge(k, trk);
-        } else if (u_proc < R0.trap + R0.ion_e + R0.ion_h + R0.iv) {
+        } else if (u_proc < Rv.trap + Rv.ion_e + Rv.ion_h + Rv.iv) {
             scatter_iv<P>(m, trk, r);
             ctx.ccount(CC_IV);
-        } else if (R0.luke > R(0)) {
+        } else if (Rv.luke > R(0)) {
             V3<R> qd;
             R eph;
             if (emit_luke<P>(m, trk, r, qd, eph)) {

@@ -79,6 +79,9 @@ struct DevModel {
     double radius, half_z, efield[3];
     DevSurface surf[3];
     uint32_t max_bounces, rr_bounces;
+    // DetectorModel::ChargeStepping: 0 first order (dk_frac = charge_max_dk_frac), 1 Leman
+    // second order (dk_frac = charge_sampling_dk_frac)
+    int32_t charge_stepping, _pad_cs;
     double min_phonon_eV, luke_fraction, dk_frac, dt_cap, fermi, rr_survival;
     // FP32 copies used by the batched kernels (uniform scalar operands, no per-step
     // conversions), filled on the host by fill_fp32_constants.

@@ -82,8 +82,6 @@ struct DetectorModel {
     Vec3 electric_field_at(const Vec3& p) const noexcept { return e_field.at(p); }
 
     cmpmc_detector to_pod() const {
-        if (charge_stepping != ChargeStepping::FirstOrderDkCap)
-            throw std::invalid_argument("cmpmc_amd: only FirstOrderDkCap charge stepping runs on the GPU path");
         if (charge_release != ChargeRelease::RecordOnly)
             throw std::invalid_argument("cmpmc_amd: only ChargeRelease::RecordOnly runs on the GPU path");
         cmpmc_detector d{};
@@ -99,6 +97,8 @@ struct DetectorModel {
         d.charge_dt_cap_s = charge_dt_cap_s;
         d.fermi_level_above_Ev_eV = fermi_level_above_Ev_eV;
         d.russian_roulette_survival = russian_roulette_survival;
+        d.charge_stepping = charge_stepping == ChargeStepping::LemanSecondOrder ? 1 : 0;
+        d.charge_sampling_dk_frac = charge_sampling_dk_frac;
         return d;
     }
 };
@@ -230,6 +230,18 @@ public:
             charges.clear();
             return;
         }
+    }
+
+    // engine.hpp:388-399: transport_charges with Leman second-order stepping (max_chord_m
+    // is ignored there too); runs on a model of this engine's detector switched to Leman.
+    void transport_charges_fast_chord(std::vector<ChargeTrack>& charge_stack, std::vector<PhononTrack>& phonon_stack,
+                                      std::vector<SensorHit>& hits, SimStats& stats, Rng& rng,
+                                      double max_chord_m = 250.0 * units::um) const {
+        (void)max_chord_m;
+        DetectorModel d = detector_;
+        d.charge_stepping = DetectorModel::ChargeStepping::LemanSecondOrder;
+        const SimulationEngine leman(lattice_, d);
+        leman.transport_charges(charge_stack, phonon_stack, hits, stats, rng);
     }
 
 private:

@@ -159,6 +159,10 @@ typedef struct {
     double charge_max_dk_frac, charge_dt_cap_s;
     double fermi_level_above_Ev_eV;
     double russian_roulette_survival;
+    int32_t charge_stepping;      /* DetectorModel::ChargeStepping (engine.hpp:131-133): 0 FirstOrderDkCap,
+                                     1 LemanSecondOrder (engine.hpp:234-254, 337-342) */
+    int32_t _pad_cs;
+    double charge_sampling_dk_frac; /* LemanSecondOrder sampling-step cap (default 0.25) */
 } cmpmc_detector;
 
 typedef struct cmpmc_model cmpmc_model;
