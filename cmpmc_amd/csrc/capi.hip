@@ -152,6 +152,10 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     m.kin_mode = lat->kinematics_mode;
     m.density = lat->cfg.density;
     std::memcpy(m.orient, c.orient, sizeof(m.orient));
+    m.orient_identity = 1;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (c.orient[i][j] != (i == j ? 1.0 : 0.0)) m.orient_identity = 0;
     m.B_iso = c.B_iso; m.A_anh = c.A_anh; m.fTT = c.fTT_frac; m.poly_mfp = c.polycrystal_mfp_m;
     m.fL = c.fLDOS; m.fST = c.fSTDOS; m.fFT = c.fFTDOS; m.v_sound = c.v_sound; m.v_trans = c.v_trans;
     const double h = kHPlanck;

@@ -60,7 +60,7 @@ struct DevModel {
     const double* stiff;        // C_ijlm [3][3][3][3] (exact kinematics)
     const float* stiff_f;       // the same in FP32
     int32_t kin_mode;           // CrystalLattice::KinematicsMode (host side selects the kernel policy)
-    int32_t _pad_k;
+    int32_t orient_identity;    // orient is the identity (Ge's default): the table lookup skips both rotations
     double density;
     const double* cdf_lt;       // [1024]
     const double* cdf_tt;       // [1024]
@@ -370,7 +370,8 @@ template <class P>
 __device__ inline void map_k_to_vg_oct(const DevModel& m, int mode, V3<typename P::R> kdet, V3<typename P::R>& vg_dir,
                                        typename P::R& vg_mag) {
     using R = typename P::R;
-    const V3<R> k = mat_mul_t(m.orient, kdet);
+    // with the identity orientation both rotations return their argument exactly
+    const V3<R> k = m.orient_identity ? kdet : mat_mul_t(m.orient, kdet);
     const R l1 = fabs(k.x) + fabs(k.y) + fabs(k.z);
     const R inv_l1 = (l1 > R(1e-18)) ? P::rcp(l1) : R(1);
     R px = k.x * inv_l1, py = k.y * inv_l1;
@@ -398,7 +399,7 @@ __device__ inline void map_k_to_vg_oct(const DevModel& m, int mode, V3<typename 
                    static_cast<R>(e00.z * w00 + e10.z * w10 + e01.z * w01 + e11.z * w11)};
     const R mag = norm<P>(vc);
     const V3<R> t = vdiv<P>(vc, tmax(mag, R(1e-12)));
-    vg_dir = unit<P>(mat_mul(m.orient, t));
+    vg_dir = unit<P>(m.orient_identity ? t : mat_mul(m.orient, t));
     vg_mag = mag;
 }
 
@@ -674,6 +675,23 @@ template <class P> __device__ inline typename P::R theta_TT(const DevModel& m, t
 }
 
 // ------------------------------------------------------------------ geometry (solids.hpp:46-93)
+// A distance that every ray from p (inside the cylinder) travels before it meets a
+// surface: the distance to the nearest surface, less margins that cover the rounding of
+// intersect_exit (a few nm for the FP32 side-wall root near the wall, ~1e-17 m in FP64)
+// and of this estimate. A free flight shorter than it is a bulk event whatever the
+// direction, with the same decision d_phys < dist that the exit distance would give, so
+// the step need not compute the exit. Negative at or outside a surface.
+template <class P> __device__ inline typename P::R inner_safety(const DevModel& m, V3<typename P::R> p) {
+    using R = typename P::R;
+    if constexpr (P::exact) {
+        const R s = tmin(R(m.radius) - sqrt(p.x * p.x + p.y * p.y), R(m.half_z) - fabs(p.z));
+        return s * R(0.999999) - R(1e-12);
+    } else {
+        const R s = tmin(m.radius_f - P::sqrt_(p.x * p.x + p.y * p.y), m.half_z_f - fabs(p.z));
+        return s * 0.999f - 1e-7f;
+    }
+}
+
 template <class P>
 __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P::R> p, V3<typename P::R> v,
                                                V3<typename P::R>& n_in, int& surface) {

@@ -6,6 +6,13 @@
 
 namespace cmpmc_dev {
 
+// Data in the block's LDS reached through a pointer or reference of this address space
+// compiles to ds_read / ds_write; through a generic pointer (a pointer stored in a context
+// struct loses the address space) it compiles to flat accesses that wait for all
+// outstanding memory operations. lds() converts the address of a __shared__ object.
+#define CMPMC_LDS __attribute__((address_space(3)))
+template <class T> __device__ inline CMPMC_LDS T* lds(T* p) { return (CMPMC_LDS T*)p; }
+
 // Global counters (uint64): phonon counters C_* then charge counters CC_* .
 constexpr int kNumCounters = C_NCOUNTERS + CC_N;
 
@@ -28,20 +35,20 @@ struct BlockStats {
     unsigned long long* gledger;     // global (hi, lo) pairs, for lanes whose slot fills up
 };
 
-__device__ inline void bs_init(BlockStats& s, unsigned long long* gledger) {
+__device__ inline void bs_init(CMPMC_LDS BlockStats& s, unsigned long long* gledger) {
     for (int i = threadIdx.x; i < (kMaxBlock / 64) * kNumCounters; i += blockDim.x) (&s.cnt[0][0])[i] = 0;
     for (int i = threadIdx.x; i < L_N * kMaxBlock; i += blockDim.x) (&s.led[0][0])[i] = 0;
     if (threadIdx.x == 0) s.gledger = gledger;
 }
 // Counts every active lane of the wave that executes this call.
-__device__ inline void bs_count(BlockStats& s, int c) {
+__device__ inline void bs_count(CMPMC_LDS BlockStats& s, int c) {
     const uint64_t active = __builtin_amdgcn_read_exec();
     if (__lane_id() == __ffsll(static_cast<unsigned long long>(active)) - 1)
         s.cnt[threadIdx.x >> 6][c] += static_cast<unsigned long long>(__popcll(active));
 }
 // Adds a non-negative energy to the lane's slot; a slot that reaches 2^62 units
 // (2.6e5 eV) is moved to the global pair at once, so long-running lanes cannot wrap.
-__device__ inline void bs_ledger(BlockStats& s, int slot, double eV) {
+__device__ inline void bs_ledger(CMPMC_LDS BlockStats& s, int slot, double eV) {
     const unsigned long long v =
         s.led[slot][threadIdx.x] + static_cast<unsigned long long>(__double2ll_rn(eV * kLedgerScale));
     if (v >> 62) {

@@ -152,10 +152,14 @@ __device__ inline bool phonon_step(const DevModel& m, PTrack<P>& trk, Rng& r, Ct
     }
     const R rate_tot = rate_iso + rate_poly + rate_down;
     const R d_phys = (rate_tot > R(1e-12)) ? (P::neg_log(P::uniform(r)) * (trk.vg_mag / rate_tot)) : P::inf();
-    V3<R> n_in;
-    int sid;
-    const R dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
-    if (d_phys < dist) {                       // bulk event
+    V3<R> n_in{R(0), R(0), R(1)};
+    int sid = 0;
+    R dist = P::inf();
+    // a flight shorter than the distance to every surface ends in the bulk (the exit
+    // distance is not needed: the decision d_phys < dist is the same)
+    const bool inside = d_phys < inner_safety<P>(m, trk.pos);
+    if (!inside) dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
+    if (inside || d_phys < dist) {             // bulk event
         trk.pos = trk.pos + trk.vg * d_phys;
         trk.time += static_cast<double>(d_phys / trk.vg_mag);
         const R u_proc = P::uniform(r) * rate_tot;

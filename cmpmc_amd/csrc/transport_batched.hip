@@ -106,13 +106,13 @@ template <> struct RecOps<true> {
 template <bool Diff, class P> struct BatchCtx {
     using Ops = RecOps<Diff>;
     using Rec = typename Ops::Rec;
-    BlockStats* bs;
+    CMPMC_LDS BlockStats* bs;
     const DevModel::SurfF* surfs;        // LDS copy of the model's FP32 surface block
     Rec* pool;                           // this wave's spawn pool
-    volatile uint32_t* top;              // its fill level (LDS, one word per wave; written by
+    CMPMC_LDS volatile uint32_t* top;    // its fill level (LDS, one word per wave; written by
                                          // whichever lane of the wave updates it, so every
                                          // access goes to memory)
-    volatile uint32_t* lost;             // tracks the wave's full pool turned away so far (LDS)
+    CMPMC_LDS volatile uint32_t* lost;   // tracks the wave's full pool turned away so far (LDS)
     uint32_t cap;
     uint32_t need;                       // on overflow: pool size to ask for (cap + lost)
     bool overflow;
@@ -124,12 +124,12 @@ template <bool Diff, class P> struct BatchCtx {
     bool keep_hits;
     // this wave's current chunks of hit and tape slots and its count of hits written (LDS;
     // written by whichever lane leads a reservation, so every access goes to memory)
-    volatile unsigned long long* hbase;
-    volatile unsigned long long* tbase;
-    volatile uint32_t* hfill;
-    volatile uint32_t* tfill;
-    volatile uint32_t* hvalid;
-    volatile uint32_t* tvalid;
+    CMPMC_LDS volatile unsigned long long* hbase;
+    CMPMC_LDS volatile unsigned long long* tbase;
+    CMPMC_LDS volatile uint32_t* hfill;
+    CMPMC_LDS volatile uint32_t* tfill;
+    CMPMC_LDS volatile uint32_t* hvalid;
+    CMPMC_LDS volatile uint32_t* tvalid;
     int32_t* first_branch;
     uint64_t first_event;
     bool rec;
@@ -227,8 +227,8 @@ template <bool Diff, class P> struct BatchCtx {
     // Slot of the calling lane in the wave's current chunk: the active lanes take
     // consecutive slots; when the chunk cannot hold them all, the first of them reserves
     // a fresh chunk of kHitChunk slots with one atomic (the old chunk's rest stays unused).
-    __device__ unsigned long long reserve(unsigned long long* counter, volatile unsigned long long* base,
-                                          volatile uint32_t* fill, volatile uint32_t* written) {
+    __device__ unsigned long long reserve(unsigned long long* counter, CMPMC_LDS volatile unsigned long long* base,
+                                          CMPMC_LDS volatile uint32_t* fill, CMPMC_LDS volatile uint32_t* written) {
         const uint64_t active = __ballot(1);
         const uint32_t n = static_cast<uint32_t>(__popcll(active));
         const uint32_t rank = static_cast<uint32_t>(__popcll(active & ((1ull << __lane_id()) - 1ull)));
@@ -320,7 +320,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
         s_hvalid[threadIdx.x] = 0;
         s_tvalid[threadIdx.x] = 0;
     }
-    bs_init(s_stats, a.ledger);
+    bs_init(*lds(&s_stats), a.ledger);
     __syncthreads();
 
     const uint64_t seed = a.dyn ? a.dyn->seed : a.seed;
@@ -330,11 +330,11 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     const uint32_t wave = lane / 64;
     Rec* const fbuf = static_cast<Rec*>(a.film_jobs) + static_cast<size_t>(wave) * kFilmCap;
     Ctx ctx;
-    ctx.bs = &s_stats;
+    ctx.bs = lds(&s_stats);
     ctx.surfs = s_surf;
     ctx.pool = static_cast<Rec*>(a.pool) + static_cast<size_t>(wave) * a.pool_cap;
-    ctx.top = &s_top[threadIdx.x / 64];
-    ctx.lost = &s_lost[threadIdx.x / 64];
+    ctx.top = lds(&s_top[threadIdx.x / 64]);
+    ctx.lost = lds(&s_lost[threadIdx.x / 64]);
     ctx.cap = a.pool_cap;
     ctx.need = 0;
     ctx.overflow = false;
@@ -344,12 +344,12 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.hit_count = a.hit_count;
     ctx.hit_cap = a.hit_cap;
     ctx.keep_hits = a.keep_hits != 0;
-    ctx.hbase = &s_hbase[threadIdx.x / 64];
-    ctx.tbase = &s_tbase[threadIdx.x / 64];
-    ctx.hfill = &s_hfill[threadIdx.x / 64];
-    ctx.tfill = &s_tfill[threadIdx.x / 64];
-    ctx.hvalid = &s_hvalid[threadIdx.x / 64];
-    ctx.tvalid = &s_tvalid[threadIdx.x / 64];
+    ctx.hbase = lds(&s_hbase[threadIdx.x / 64]);
+    ctx.tbase = lds(&s_tbase[threadIdx.x / 64]);
+    ctx.hfill = lds(&s_hfill[threadIdx.x / 64]);
+    ctx.tfill = lds(&s_tfill[threadIdx.x / 64]);
+    ctx.hvalid = lds(&s_hvalid[threadIdx.x / 64]);
+    ctx.tvalid = lds(&s_tvalid[threadIdx.x / 64]);
     ctx.first_branch = a.first_branch;
     ctx.first_event = a.dyn ? a.dyn->first_event : a.first_event;
     ctx.rec = false;
@@ -578,7 +578,7 @@ template <class T> __device__ inline void ctrack_to_bcharge(const CTrack<T>& c, 
 }
 
 template <class P> struct ChargeCtx {
-    BlockStats* bs;
+    CMPMC_LDS BlockStats* bs;
     BCharge* ls;
     uint32_t sp, depth, lanes, lane;
     uint32_t lost;                       // knock-on carriers this lane's full stack turned away
@@ -633,11 +633,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, W))
 transport_charges_kernel(const DevModel m, const ChargeArgs a) {
     __shared__ BlockStats s_stats;
     const uint64_t seed = a.dyn ? a.dyn->seed : a.seed;
-    bs_init(s_stats, a.ledger);
+    bs_init(*lds(&s_stats), a.ledger);
     __syncthreads();
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     ChargeCtx<P> ctx;
-    ctx.bs = &s_stats;
+    ctx.bs = lds(&s_stats);
     ctx.ls = a.lstack;
     ctx.sp = 0;
     ctx.depth = a.depth;

@@ -11,7 +11,7 @@ namespace cmpmc_dev {
 
 // P: Fp64 (octahedral LUT kinematics) or Fp64X (exact kinematics).
 template <class P> struct PlainCtx {
-    BlockStats* bs;
+    CMPMC_LDS BlockStats* bs;
     // stacks of this event
     cmpmc_phonon_track* ps;
     uint32_t psp, pcap;
@@ -183,13 +183,13 @@ __global__ void __launch_bounds__(256) transport_plain_kernel(const DevModel m, 
         s_cdf[i] = m.cdf_lt[i];
         s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
     }
-    bs_init(s_stats, a.ledger);
+    bs_init(*lds(&s_stats), a.ledger);
     __syncthreads();
     const uint64_t lane = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool active = lane < a.n_events;
     if (active) {
         PlainCtx<P> ctx;
-        ctx.bs = &s_stats;
+        ctx.bs = lds(&s_stats);
         ctx.ps = a.pstack + a.pstack_off[lane];
         ctx.pcap = static_cast<uint32_t>(a.pstack_off[lane + 1] - a.pstack_off[lane]);
         ctx.psp = a.n_init_ph ? a.n_init_ph[lane] : 0;
