@@ -129,6 +129,13 @@ double orc_sample_tt_fraction(const orc_lattice* L, orc_rng* r);
 
 /* ---- Detector: transport/engine.hpp:96-156, geometry/solids.hpp, surface_property.hpp------------------------- */
 #define ORC_MAX_POLY 8
+/* geometry/electrode_pattern.hpp:23-35 ElectrodePad: shape 0 Disk, 1 AnnularSector, 2 Rectangle */
+typedef struct {
+    int32_t shape, channel_id;
+    double cx, cy, r_min, r_max, phi_min, phi_max, fill_fraction;
+} orc_pad;
+/* phonon/interface_transmission.hpp:35-48 IsotropicMedium */
+typedef struct { double rho, vL, vT; } orc_medium;
 typedef struct {
     double prob_abs, prob_refl, refl_cutoff, anh_cutoff;
     double prob_charge_abs, prob_charge_spec, min_k_elec, min_k_hole;
@@ -137,7 +144,24 @@ typedef struct {
     double spec_prob_const;
     int    has_film;
     orc_film film;
+    int    n_pads;                 /* electrode mask: n_pads > 0 pads (pads owned by the caller) */
+    int    has_amm;                /* film entry by AMM transmission (substrate -> film) */
+    const orc_pad* pads;
+    orc_medium amm_substrate, amm_film;
 } orc_surface;
+
+/* ElectrodePatternMask::find_channel (electrode_pattern.hpp:82-92): the channel of the
+ * first pad containing (x, y), -1 between pads; a pad with fill_fraction < 1 draws one
+ * uniform and is active with that probability. */
+int32_t orc_find_channel(const orc_surface* s, const double pos[3], orc_rng* r);
+/* phonon/interface_transmission.hpp:56-141: normal-incidence, SH and P-SV (energy-flux
+ * coefficients R_P, R_S, T_P, T_S into out[4]) AMM transmission, DMM, and the engine's
+ * mode- and angle-resolved entry probability. */
+double orc_amm_normal(double Z1, double Z2);
+double orc_amm_sh(const orc_medium* m1, const orc_medium* m2, double theta1);
+double orc_dmm(const orc_medium* m1, const orc_medium* m2);
+void orc_amm_psv(const orc_medium* m1, const orc_medium* m2, int incident_P, double theta1, double out[4]);
+double orc_amm_transmission(const orc_medium* substrate, const orc_medium* film, int longitudinal, double cos_theta);
 
 typedef struct {
     double radius, half_z;

@@ -7,6 +7,8 @@
  */
 #include "oracle_internal.h"
 
+#include <complex.h>
+
 #include <stdlib.h>
 #include <string.h>
 
@@ -806,4 +808,130 @@ void orc_stats_add(orc_stats* a, const orc_stats* b) {
     a->total_injected_eV += b->total_injected_eV;
     a->total_deposited_eV += b->total_deposited_eV;
     a->escaped_or_killed_eV += b->escaped_or_killed_eV;
+}
+
+/* ---------------- electrode mask: geometry/electrode_pattern.hpp:37-92 ---------------- */
+static int pad_contains(const orc_pad* p, double x, double y) {
+    if (p->shape == 0) {                                   /* disk about (cx, cy), radius r_max */
+        const double dx = x - p->cx, dy = y - p->cy;
+        return (dx * dx + dy * dy) <= (p->r_max * p->r_max);
+    }
+    if (p->shape == 1) {                                   /* annular sector */
+        const double dx = x - p->cx, dy = y - p->cy;
+        const double r2 = dx * dx + dy * dy;
+        if (r2 < p->r_min * p->r_min || r2 > p->r_max * p->r_max) return 0;
+        double phi = atan2(dy, dx);
+        if (phi < 0.0) phi += ORC_TWO_PI;
+        if (p->phi_min <= p->phi_max) return phi >= p->phi_min && phi <= p->phi_max;
+        return phi >= p->phi_min || phi <= p->phi_max;     /* wraps across phi = 0 */
+    }
+    if (p->shape == 2)                                     /* rectangle, half-widths r_min, r_max */
+        return fabs(x - p->cx) <= p->r_min && fabs(y - p->cy) <= p->r_max;
+    return 0;
+}
+
+int32_t orc_find_channel(const orc_surface* s, const double pos[3], orc_rng* r) {
+    for (int i = 0; i < s->n_pads; ++i) {
+        const orc_pad* p = &s->pads[i];
+        if (!pad_contains(p, pos[0], pos[1])) continue;
+        if (p->fill_fraction >= 1.0 || orc_uniform(r) < p->fill_fraction) return p->channel_id;
+        return -1;
+    }
+    return -1;
+}
+
+/* ---------------- AMM / DMM interface transmission: phonon/interface_transmission.hpp:56-141 ------- */
+double orc_amm_normal(double Z1, double Z2) { return 4.0 * Z1 * Z2 / ((Z1 + Z2) * (Z1 + Z2)); }
+
+double orc_amm_sh(const orc_medium* m1, const orc_medium* m2, double theta1) {
+    const double s2 = m2->vT / m1->vT * sin(theta1);       /* Snell: sin theta2 */
+    if (s2 >= 1.0) return 0.0;                              /* beyond the critical angle */
+    const double c1 = cos(theta1), c2 = sqrt(1.0 - s2 * s2);
+    const double a = m1->rho * m1->vT * c1, b = m2->rho * m2->vT * c2;
+    return (a + b > 0.0) ? 4.0 * a * b / ((a + b) * (a + b)) : 0.0;
+}
+
+double orc_dmm(const orc_medium* m1, const orc_medium* m2) {
+    const double s1 = 1.0 / (m1->vL * m1->vL) + 2.0 / (m1->vT * m1->vT);
+    const double s2 = 1.0 / (m2->vL * m2->vL) + 2.0 / (m2->vT * m2->vT);
+    return s2 / (s1 + s2);
+}
+
+/* A plane P or SV wave of horizontal slowness p and vertical slowness s in medium m:
+ * displacement (dx, dz) and tractions (tau_xz, tau_zz), common factor i omega dropped. */
+typedef struct { double complex dx, dz, txz, tzz; } psv_wave;
+
+static double complex vertical_slowness(double p, double v) {   /* Im >= 0 branch */
+    return csqrt(CMPLX(1.0 / (v * v) - p * p, 0.0));
+}
+
+static psv_wave psv_plane_wave(const orc_medium* m, int P, double complex s, double p) {
+    const double v = P ? m->vL : m->vT;
+    psv_wave w;
+    w.dx = P ? CMPLX(v * p, 0.0) : v * s;
+    w.dz = P ? v * s : CMPLX(-v * p, 0.0);
+    const double mu = m->rho * m->vT * m->vT;
+    const double la = m->rho * (m->vL * m->vL - 2.0 * m->vT * m->vT);
+    w.txz = mu * (s * w.dx + p * w.dz);
+    w.tzz = la * (p * w.dx + s * w.dz) + 2.0 * mu * s * w.dz;
+    return w;
+}
+
+/* |z|^2 = re^2 + im^2 (the reference's std::norm for double) */
+static double cnorm2(double complex z) { const double x = creal(z), y = cimag(z); return x * x + y * y; }
+
+void orc_amm_psv(const orc_medium* m1, const orc_medium* m2, int incident_P, double theta1, double out[4]) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    const double v_inc = incident_P ? m1->vL : m1->vT;
+    const double p = sin(theta1) / v_inc;
+    const double complex a1 = vertical_slowness(p, m1->vL), b1 = vertical_slowness(p, m1->vT);
+    const double complex a2 = vertical_slowness(p, m2->vL), b2 = vertical_slowness(p, m2->vT);
+    const psv_wave inc = psv_plane_wave(m1, incident_P, incident_P ? a1 : b1, p);
+    /* unknowns (R_P, R_S, T_P, T_S): reflected waves travel back (-s), transmitted on (+s);
+     * rows: continuity of u_x, u_z, tau_xz, tau_zz at the interface */
+    const psv_wave waves[4] = {psv_plane_wave(m1, 1, -a1, p), psv_plane_wave(m1, 0, -b1, p),
+                               psv_plane_wave(m2, 1, a2, p), psv_plane_wave(m2, 0, b2, p)};
+    const double sgn[4] = {1.0, 1.0, -1.0, -1.0};
+    double complex M[4][5];
+    for (int j = 0; j < 4; ++j) {
+        M[0][j] = sgn[j] * waves[j].dx;
+        M[1][j] = sgn[j] * waves[j].dz;
+        M[2][j] = sgn[j] * waves[j].txz;
+        M[3][j] = sgn[j] * waves[j].tzz;
+    }
+    M[0][4] = -inc.dx; M[1][4] = -inc.dz; M[2][4] = -inc.txz; M[3][4] = -inc.tzz;
+    for (int c = 0; c < 4; ++c) {                          /* Gauss-Jordan with partial pivoting */
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (cabs(M[r][c]) > cabs(M[piv][c])) piv = r;
+        for (int k = 0; k < 5; ++k) { const double complex t = M[c][k]; M[c][k] = M[piv][k]; M[piv][k] = t; }
+        const double complex d = M[c][c];
+        if (cabs(d) == 0.0) return;
+        for (int k = c; k < 5; ++k) M[c][k] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double complex f = M[r][c];
+            for (int k = c; k < 5; ++k) M[r][k] -= f * M[c][k];
+        }
+    }
+    /* energy-flux coefficients rho v^2 Re(s) |A|^2 relative to the incident wave's */
+    const double flux_inc = m1->rho * v_inc * v_inc * creal(incident_P ? a1 : b1);
+    const orc_medium* med[4] = {m1, m1, m2, m2};
+    const double vel[4] = {m1->vL, m1->vT, m2->vL, m2->vT};
+    const double complex sl[4] = {a1, b1, a2, b2};
+    for (int j = 0; j < 4; ++j)
+        out[j] = med[j]->rho * vel[j] * vel[j] * fabs(creal(sl[j])) * cnorm2(M[j][4]) / flux_inc;
+}
+
+/* IsotropicAMMInterface::transmission (interface_transmission.hpp:123-129): L phonons as P
+ * waves; a transverse phonon's polarisation is not tracked, t_T = (t_SV + t_SH) / 2. */
+double orc_amm_transmission(const orc_medium* substrate, const orc_medium* film, int longitudinal, double cos_theta) {
+    const double th = acos(dclamp(fabs(cos_theta), 0.0, 1.0));
+    double c[4];
+    if (longitudinal) {
+        orc_amm_psv(substrate, film, 1, th, c);
+        return c[2] + c[3];
+    }
+    orc_amm_psv(substrate, film, 0, th, c);
+    return 0.5 * ((c[2] + c[3]) + orc_amm_sh(substrate, film, th));
 }

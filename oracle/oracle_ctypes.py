@@ -59,6 +59,15 @@ class Film(C.Structure):
         ("keep_all_phonons", C.c_int)]
 
 
+class Pad(C.Structure):
+    _fields_ = [("shape", i32), ("channel_id", i32)] + [(n, dbl) for n in (
+        "cx", "cy", "r_min", "r_max", "phi_min", "phi_max", "fill_fraction")]
+
+
+class Medium(C.Structure):
+    _fields_ = [("rho", dbl), ("vL", dbl), ("vT", dbl)]
+
+
 class Surface(C.Structure):
     _fields_ = [
         ("prob_abs", dbl), ("prob_refl", dbl), ("refl_cutoff", dbl), ("anh_cutoff", dbl),
@@ -68,6 +77,8 @@ class Surface(C.Structure):
         ("spec_prob_const", dbl),
         ("has_film", C.c_int),
         ("film", Film),
+        ("n_pads", C.c_int), ("has_amm", C.c_int), ("pads", C.POINTER(Pad)),
+        ("amm_substrate", Medium), ("amm_film", Medium),
     ]
 
 
@@ -245,6 +256,17 @@ def lib():
         L.orc_intersect_exit.argtypes = [C.POINTER(Detector), C.POINTER(dbl), C.POINTER(dbl),
                                          C.POINTER(dbl), C.POINTER(C.c_int)]
         L.orc_reflection_split.argtypes = [C.POINTER(Surface), dbl, C.POINTER(dbl)]
+        L.orc_find_channel.argtypes = [C.POINTER(Surface), C.POINTER(dbl), C.POINTER(Rng)]
+        L.orc_find_channel.restype = i32
+        L.orc_amm_normal.argtypes = [dbl, dbl]
+        L.orc_amm_normal.restype = dbl
+        L.orc_amm_sh.argtypes = [C.POINTER(Medium), C.POINTER(Medium), dbl]
+        L.orc_amm_sh.restype = dbl
+        L.orc_dmm.argtypes = [C.POINTER(Medium), C.POINTER(Medium)]
+        L.orc_dmm.restype = dbl
+        L.orc_amm_psv.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, dbl, C.POINTER(dbl)]
+        L.orc_amm_transmission.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, dbl]
+        L.orc_amm_transmission.restype = dbl
         L.orc_transport.argtypes = [C.c_void_p, C.POINTER(TrackVec), C.POINTER(HitVec), C.POINTER(Stats),
                                     C.POINTER(Rng), C.c_int, u64, C.POINTER(C.c_int)]
         L.orc_make_track.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(dbl), C.c_int, dbl, dbl, dbl,
@@ -298,6 +320,9 @@ def ref():
         R.ref_transport_stack.argtypes = [C.POINTER(Detector), C.c_int, C.POINTER(dbl), u64, u64,
                                           C.c_void_p, C.POINTER(Hit), C.c_long, C.POINTER(u64)]
         R.ref_sizeof_simstats.restype = C.c_int
+        R.ref_amm.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, C.POINTER(dbl), C.POINTER(dbl),
+                              C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_qet_channels.argtypes = [dbl, dbl, u64, C.c_int, C.POINTER(dbl), C.POINTER(i32)]
         _ref = R
     return _ref
 
@@ -361,6 +386,68 @@ def constant_split_surfaces(det: Detector, f_diffuse: float) -> None:
         s = det.surf[i]
         s.n_spec, s.n_diff, s.n_anh = 1, 1, 0
         s.spec[0], s.diff[0], s.spec_prob_const = 1.0 - f_diffuse, f_diffuse, 1.0 - f_diffuse
+
+
+_PAD_ARRAYS = []     # pad arrays referenced by surfaces (kept alive for the process)
+
+
+def set_pads(surface: Surface, pads) -> None:
+    """Electrode mask of a surface: pads as (shape, channel_id, cx, cy, r_min, r_max,
+    phi_min, phi_max, fill_fraction) rows (geometry/electrode_pattern.hpp:23-35)."""
+    arr = (Pad * len(pads))()
+    for i, row in enumerate(pads):
+        arr[i].shape, arr[i].channel_id = int(row[0]), int(row[1])
+        for j, n in enumerate(("cx", "cy", "r_min", "r_max", "phi_min", "phi_max", "fill_fraction")):
+            setattr(arr[i], n, float(row[2 + j]))
+    _PAD_ARRAYS.append(arr)
+    surface.n_pads = len(pads)
+    surface.pads = C.cast(arr, C.POINTER(Pad))
+
+
+def qet_pads(radius: float, fill: float = 1.0):
+    """The five-channel SuperCDMS-style layout of ElectrodePatternMask::supercdms_qet_mask
+    (electrode_pattern.hpp:99-147): core disk, three 120-degree sectors with 2-degree gaps,
+    guard ring."""
+    import math
+    pads = [(0, 0, 0.0, 0.0, 0.0, 0.55 * radius, 0.0, 2 * math.pi, fill)]
+    gap, span = 2.0 * (math.pi / 180.0), 2 * math.pi / 3.0
+    for k in range(3):
+        pads.append((1, k + 1, 0.0, 0.0, 0.57 * radius, 0.90 * radius, k * span + 0.5 * gap,
+                     (k + 1) * span - 0.5 * gap, fill))
+    pads.append((1, 4, 0.0, 0.0, 0.92 * radius, 0.98 * radius, 0.0, 2 * math.pi, fill))
+    return pads
+
+
+def set_amm(surface: Surface, substrate, film) -> None:
+    """AMM film entry (IsotropicAMMInterface) with (rho, vL, vT) media."""
+    surface.has_amm = 1
+    surface.amm_substrate = Medium(*substrate)
+    surface.amm_film = Medium(*film)
+
+
+def medium_from_bulk_shear(rho: float, K: float, G: float):
+    """IsotropicMedium::from_bulk_shear (interface_transmission.hpp:41-43)."""
+    import math
+    return (rho, math.sqrt((K + 4.0 * G / 3.0) / rho), math.sqrt(G / rho))
+
+
+def medium_from_lame(rho: float, lam: float, mu: float):
+    import math
+    return (rho, math.sqrt((lam + 2.0 * mu) / rho), math.sqrt(mu / rho))
+
+
+GE_MEDIUM = (5320., 5310., 3250.)          # Ge, isotropic (Leman, arXiv:1109.1193, Table I)
+AL_MEDIUM = medium_from_bulk_shear(2720., 76e9, 26e9)
+
+
+def mask_amm_detector(fill: float = 0.9) -> Detector:
+    """The default detector with the QET electrode mask (fill fraction `fill`) and AMM
+    film entry (Ge -> Al) on both flat faces."""
+    d = default_detector()
+    for i in (0, 1):
+        set_pads(d.surf[i], qet_pads(d.radius, fill))
+        set_amm(d.surf[i], GE_MEDIUM, AL_MEDIUM)
+    return d
 
 
 def workload(kind: int, **kw) -> Workload:

@@ -202,10 +202,19 @@ void orc_transport(const orc_engine* e, orc_track_vec* stack, orc_hit_vec* hits,
             trk.time += dist / trk.vg_mag;
             trk.bounces++;
             const orc_surface* s = &D->surf[sid];
-            const int32_t ch_id = -1;
-            if (s->has_film && orc_uniform(R) < s->prob_abs) {
+            /* engine.hpp:483-495: an electrode mask picks the channel (or bare crystal between
+             * pads); AMM transmission replaces the film-entry probability */
+            int32_t ch_id = -1;
+            int active_pad = 1;
+            if (s->n_pads > 0) {
+                ch_id = orc_find_channel(s, trk.pos, R);
+                active_pad = (ch_id >= 0);
+            }
+            if (active_pad && s->has_film && orc_uniform(R) < s->prob_abs) {
                 const orc_film* film = &s->film;
-                const double p_enter = film->film_absorption;
+                const double p_enter = s->has_amm
+                    ? orc_amm_transmission(&s->amm_substrate, &s->amm_film, trk.mode == 0, v3_dot(trk.k, n_in))
+                    : film->film_absorption;
                 if (orc_uniform(R) < p_enter) {
                     const double edep = orc_absorb_phonon(film, trk.energy, refl, &nrefl, R);
                     if (edep > 0.0) {
@@ -603,8 +612,13 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
                     for (int i = 0; i < 3; ++i) trk.p[i] += E[i] * sb;
                     trk.ekin = (trk.type == 1) ? ekin_e(L, trk.valley, trk.p) : ekin_h(L, trk.p);
                     const orc_surface* s = &D->surf[sid];
-                    const int32_t ch = -1;
-                    if (!should_absorb(&trk, n_in, s, R)) {
+                    int32_t ch = -1;
+                    int active_pad = 1;
+                    if (s->n_pads > 0) {                        /* engine.hpp:287-293 */
+                        ch = orc_find_channel(s, trk.pos, R);
+                        active_pad = (ch >= 0);
+                    }
+                    if (!active_pad || !should_absorb(&trk, n_in, s, R)) {
                         st->charge_surface_refls++;
                         for (int i = 0; i < 3; ++i) trk.pos[i] += n_in[i] * 1e-11;
                         reflect_charge(e, &trk, n_in, s->prob_charge_spec, R);

@@ -48,6 +48,22 @@ SurfaceProperty to_surface(const orc_surface& s) {
         f.keep_all_phonons = s.film.keep_all_phonons != 0;
         p.kaplan_sensor = KaplanFilm(f);
     }
+    if (s.n_pads > 0) {
+        std::vector<geometry::ElectrodePad> pads;
+        for (int i = 0; i < s.n_pads; ++i) {
+            const orc_pad& q = s.pads[i];
+            geometry::ElectrodePad d;
+            d.shape = static_cast<geometry::ElectrodePad::Shape>(q.shape);
+            d.channel_id = q.channel_id;
+            d.cx = q.cx; d.cy = q.cy; d.r_min = q.r_min; d.r_max = q.r_max;
+            d.phi_min = q.phi_min; d.phi_max = q.phi_max; d.fill_fraction = q.fill_fraction;
+            pads.push_back(d);
+        }
+        p.electrode_mask = geometry::ElectrodePatternMask(std::move(pads));
+    }
+    if (s.has_amm)
+        p.amm_interface = physics::IsotropicAMMInterface{{s.amm_substrate.rho, s.amm_substrate.vL, s.amm_substrate.vT},
+                                                         {s.amm_film.rho, s.amm_film.vL, s.amm_film.vT}};
     return p;
 }
 
@@ -330,6 +346,32 @@ long ref_run_events(const orc_crystal* crystal, const orc_detector* det, const o
         if (err && err_len > 0) { std::strncpy(err, ex.what(), static_cast<size_t>(err_len - 1)); err[err_len - 1] = 0; }
         return -1;
     }
+}
+
+// IsotropicAMMInterface / amm_psv / amm_sh of the reference for n angles:
+// out[8 * i ..] = R_P, R_S, T_P, T_S (P incident), R_P, R_S, T_P, T_S (SV incident);
+// sh[i] = amm_sh; tl[i], tt[i] = transmission(L / T, cos theta_i).
+void ref_amm(const orc_medium* m1, const orc_medium* m2, int n, const double* theta, double* out, double* sh,
+             double* tl, double* tt) {
+    const physics::IsotropicMedium a{m1->rho, m1->vL, m1->vT}, b{m2->rho, m2->vL, m2->vT};
+    const physics::IsotropicAMMInterface I{a, b};
+    for (int i = 0; i < n; ++i) {
+        const auto p = physics::amm_psv(a, b, true, theta[i]);
+        const auto s = physics::amm_psv(a, b, false, theta[i]);
+        const double v[8] = {p.R_P, p.R_S, p.T_P, p.T_S, s.R_P, s.R_S, s.T_P, s.T_S};
+        for (int k = 0; k < 8; ++k) out[8 * i + k] = v[k];
+        sh[i] = physics::amm_sh(a, b, theta[i]);
+        tl[i] = I.transmission(true, std::cos(theta[i]));
+        tt[i] = I.transmission(false, std::cos(theta[i]));
+    }
+}
+
+// ElectrodePatternMask::supercdms_qet_mask(radius, fill).find_channel for n points drawn
+// with one Rng(seed) (the draws of fill_fraction < 1 come from it).
+void ref_qet_channels(double radius, double fill, uint64_t seed, int n, const double* xy, int32_t* ch) {
+    const auto mask = geometry::ElectrodePatternMask::supercdms_qet_mask(radius, fill);
+    Rng rng(seed);
+    for (int i = 0; i < n; ++i) ch[i] = mask.find_channel(Vec3{xy[2 * i], xy[2 * i + 1], 0.0}, rng);
 }
 
 // Direct SimulationEngine::transport of one stack (the drop-in's serial semantics):

@@ -79,6 +79,23 @@ def main():
     res = np.zeros((300, 5))
     R.ref_intersect(300, O.dptr(np.ascontiguousarray(p)), O.dptr(np.ascontiguousarray(v)), O.dptr(res))
     out["isect_p"], out["isect_v"], out["isect_out"] = p, v, res
+    # 6b. AMM interface transmission (phonon/interface_transmission.hpp): P-SV flux
+    # coefficients, SH and the engine's transmission for Ge/Al and Si/W over 300 angles
+    th = np.linspace(0.0, 0.5 * np.pi, 300)
+    for tag, m1, m2 in (("amm_geal", (5320., 5310., 3250.), O.medium_from_bulk_shear(2720., 76e9, 26e9)),
+                        ("amm_siw", (2330., 9000., 5400.), O.medium_from_bulk_shear(18560., 297.672e9, 150.536e9))):
+        a, b = O.Medium(*m1), O.Medium(*m2)
+        psv, sh, tl, tt = np.zeros(8 * 300), np.zeros(300), np.zeros(300), np.zeros(300)
+        R.ref_amm(O.C.byref(a), O.C.byref(b), 300, O.dptr(th), O.dptr(psv), O.dptr(sh), O.dptr(tl), O.dptr(tt))
+        out[f"{tag}_media"] = np.array([m1, m2])
+        out[f"{tag}_theta"] = th
+        out[f"{tag}_psv"], out[f"{tag}_sh"], out[f"{tag}_tl"], out[f"{tag}_tt"] = psv.reshape(300, 8), sh, tl, tt
+    # 6c. supercdms_qet_mask channels (electrode_pattern.hpp:99-147) at 2000 points of the
+    # face, fill fraction 0.8 (draws from Rng(31))
+    xy = np.random.default_rng(17).uniform(-0.0381, 0.0381, (2000, 2))
+    ch = np.zeros(2000, dtype=np.int32)
+    R.ref_qet_channels(0.0381, 0.8, 31, 2000, O.dptr(np.ascontiguousarray(xy)), ch.ctypes.data_as(O.C.POINTER(O.i32)))
+    out["qet_xy"], out["qet_channels"] = xy, ch
     # 7. run_events end to end
     runs = {
         "c1": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 400, 42, O.default_detector()),
@@ -124,6 +141,13 @@ def main():
     dl2.luke_sample_fraction = 0.02
     dl2.charge_stepping = 1
     runs["leman_ion"] = (O.workload(O.WL_PAIR), 6, 43, dl2, O.trap_ion_crystal())
+    # electrode masks (QET layout, fill 0.9) and AMM film entry (Ge -> Al) on both faces:
+    # phonons, and pairs at 4 V collected through the masks
+    runs["mask_amm"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.mask_amm_detector())
+    dm = O.mask_amm_detector()
+    O.set_bias(dm, 4.0)
+    dm.luke_sample_fraction = 0.0
+    runs["mask_pairs"] = (O.workload(O.WL_PAIR), 6, 42, dm)
     runs["c1_exact"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector())
     for tag, spec in runs.items():
         wl, n, seed, d = spec[:4]
@@ -135,6 +159,8 @@ def main():
         out[f"run_{tag}_offsets"] = r["hit_offsets"]
         h = r["hits"][:600]
         out[f"run_{tag}_hits"] = np.c_[h["pos"], h["time"], h["edep"], h["incident"], h["surface"], h["mode"]]
+        if tag.startswith("mask"):
+            out[f"run_{tag}_channels"] = h["channel"].astype(np.int32)
     path = os.path.join(HERE, "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")

@@ -461,3 +461,144 @@ def test_batched_scheme_is_order_independent(oracle):
     b = eng.run(wl, 30, 9, threads=4, chunk=7, batched=True)
     assert a["stats"]["total_steps"] == b["stats"]["total_steps"]
     assert np.array_equal(a["hits"], b["hits"])
+
+
+# ------------------------------------------------------------------ electrode masks and AMM film entry
+# §8(f) rank 3: ElectrodePatternMask::find_channel (electrode_pattern.hpp:82-92) and
+# IsotropicAMMInterface::transmission (interface_transmission.hpp:123-129), hooked into the
+# film branch (engine.hpp:483-495) and charge collection (engine.hpp:287-293).
+@pytest.mark.parametrize("tag", ["amm_geal", "amm_siw"])
+def test_amm_golden(oracle, gold, tag):
+    """P-SV flux coefficients, SH transmission and the engine's entry probability equal the
+    reference's bit for bit over 300 angles."""
+    O = oracle
+    L = O.lib()
+    m1, m2 = (O.Medium(*row) for row in gold[f"{tag}_media"])
+    c = np.zeros(4)
+    for i, th in enumerate(gold[f"{tag}_theta"]):
+        # (exactly grazing incidence, theta = pi/2, is 0/0 in both: NaN)
+        L.orc_amm_psv(C.byref(m1), C.byref(m2), 1, th, O.dptr(c))
+        assert np.array_equal(c, gold[f"{tag}_psv"][i, :4], equal_nan=True)
+        L.orc_amm_psv(C.byref(m1), C.byref(m2), 0, th, O.dptr(c))
+        assert np.array_equal(c, gold[f"{tag}_psv"][i, 4:], equal_nan=True)
+        got = [L.orc_amm_sh(C.byref(m1), C.byref(m2), th),
+               L.orc_amm_transmission(C.byref(m1), C.byref(m2), 1, np.cos(th)),
+               L.orc_amm_transmission(C.byref(m1), C.byref(m2), 0, np.cos(th))]
+        want = [gold[f"{tag}_{k}"][i] for k in ("sh", "tl", "tt")]
+        assert np.array_equal(got, want, equal_nan=True)
+
+
+def _media(O):
+    Si, Ge = (2330., 9000., 5400.), (5320., 5310., 3250.)
+    Al2O3 = O.medium_from_lame(3866., 139.8e9, 166.24e9)
+    Al = O.medium_from_bulk_shear(2720., 76e9, 26e9)
+    W = O.medium_from_bulk_shear(18560., 297.672e9, 150.536e9)
+    Ti = O.medium_from_bulk_shear(4265., 113.529e9, 63.5361e9)
+    Ir = O.medium_from_bulk_shear(21980., 320e9, 210e9)
+    # substrate, film, AMM L normal, T normal, SH flux average, DMM (tests/test_interface_amm.cpp:34-36)
+    return [("Si/Al", Si, Al, 0.991, 0.960, 0.960, 0.743), ("Si/W", Si, W, 0.588, 0.621, 0.482, 0.778),
+            ("Si/Ti", Si, Ti, 0.974, 0.982, 0.896, 0.658), ("Si/Ir", Si, Ir, 0.522, 0.527, 0.406, 0.752),
+            ("Ge/Al", Ge, Al, 0.943, 0.881, 0.905, 0.510), ("Al2O3/Al", Al2O3, Al, 0.822, 0.748, 0.858, 0.810)], Si, Al
+
+
+def test_amm_physics(oracle):
+    """The reference's own AMM checks (tests/test_interface_amm.cpp:17-90) on the oracle:
+    energy-flux conservation of the 4x4 P-SV solve at every angle (not imposed by the
+    solver), the normal-incidence closed form 4 Z1 Z2 / (Z1 + Z2)^2 with no mode
+    conversion, total transmission between identical media, SH and P-SV reciprocity, the
+    published normal-incidence / flux-averaged / DMM table, DMM detailed balance."""
+    O = oracle
+    L = O.lib()
+    pairs, Si, Al = _media(O)
+    c = np.zeros(4)
+
+    def psv(a, b, P, th):
+        L.orc_amm_psv(C.byref(a), C.byref(b), P, th, O.dptr(c))
+        return c.copy()
+
+    for name, s, f, tL, tT, tSH, tD in pairs:
+        ms, mf = O.Medium(*s), O.Medium(*f)
+        worst = 0.0
+        for th in (np.arange(400) + 0.5) / 400 * (np.pi / 2):
+            for a, b in ((ms, mf), (mf, ms)):
+                for P in (1, 0):
+                    worst = max(worst, abs(psv(a, b, P, th).sum() - 1.0))
+        assert worst <= 1e-10, f"{name}: flux conservation {worst:.2e}"
+        p0, s0 = psv(ms, mf, 1, 0.0), psv(ms, mf, 0, 0.0)
+        assert p0[2] == pytest.approx(L.orc_amm_normal(s[0] * s[1], f[0] * f[1]), rel=1e-12)
+        assert s0[3] == pytest.approx(L.orc_amm_normal(s[0] * s[2], f[0] * f[2]), rel=1e-12)
+        assert p0[1] + p0[3] + s0[0] + s0[2] <= 1e-14, f"{name}: mode conversion at normal incidence"
+        n = 20000
+        th = (np.arange(n) + 0.5) / n * (np.pi / 2)
+        sh = sum(L.orc_amm_sh(C.byref(ms), C.byref(mf), t) * 2 * np.sin(t) * np.cos(t) for t in th) * (np.pi / 2) / n
+        assert abs(p0[2] - tL) <= 6e-4 and abs(s0[3] - tT) <= 6e-4 and abs(sh - tSH) <= 6e-4, name
+        assert abs(L.orc_dmm(C.byref(ms), C.byref(mf)) - tD) <= 6e-4, name
+        assert L.orc_dmm(C.byref(ms), C.byref(mf)) + L.orc_dmm(C.byref(mf), C.byref(ms)) == pytest.approx(1.0, abs=1e-14)
+    mSi = O.Medium(*Si)
+    for th in (np.arange(100) + 0.5) / 100 * 1.5:
+        assert abs(psv(mSi, mSi, 1, th)[2] - 1.0) <= 1e-12 and abs(psv(mSi, mSi, 0, th)[3] - 1.0) <= 1e-12
+        assert abs(L.orc_amm_sh(C.byref(mSi), C.byref(mSi), th) - 1.0) <= 1e-12
+    mAl = O.Medium(*Al)
+    th1 = 0.6
+    th2 = np.arcsin(Al[2] / Si[2] * np.sin(th1))
+    assert L.orc_amm_sh(C.byref(mSi), C.byref(mAl), th1) == pytest.approx(L.orc_amm_sh(C.byref(mAl), C.byref(mSi), th2), rel=1e-12)
+    thp2 = np.arcsin(Al[1] / Si[1] * np.sin(th1))
+    assert psv(mSi, mAl, 1, th1)[2] == pytest.approx(psv(mAl, mSi, 1, thp2)[2], rel=1e-10)
+
+
+def test_qet_mask_golden_and_geometry(oracle, gold):
+    """find_channel of the QET layout (fill fraction 0.8) equals the reference's at 2000
+    points; with fill 1 the channel follows from the radius and azimuth alone."""
+    O = oracle
+    L = O.lib()
+    s = O.Surface()
+    O.set_pads(s, O.qet_pads(0.0381, 0.8))
+    rng = O.Rng()
+    L.orc_rng_seed(C.byref(rng), 31)
+    p = np.zeros(3)
+    got = []
+    for x, y in gold["qet_xy"]:
+        p[0], p[1] = x, y
+        got.append(L.orc_find_channel(C.byref(s), O.dptr(p), C.byref(rng)))
+    assert np.array_equal(np.array(got), gold["qet_channels"])
+    s1 = O.Surface()
+    O.set_pads(s1, O.qet_pads(0.0381, 1.0))
+    R = 0.0381
+    for x, y in gold["qet_xy"]:
+        p[0], p[1] = x, y
+        r, phi = np.hypot(x, y), np.arctan2(y, x) % (2 * np.pi)
+        sec = int(phi // (2 * np.pi / 3))
+        off = phi - sec * 2 * np.pi / 3
+        if r <= 0.55 * R:
+            want = 0
+        elif 0.57 * R <= r <= 0.90 * R and np.radians(1) <= off <= 2 * np.pi / 3 - np.radians(1):
+            want = sec + 1
+        elif 0.92 * R <= r <= 0.98 * R:
+            want = 4
+        else:
+            want = -1
+        if min(abs(r - k * R) for k in (0.55, 0.57, 0.90, 0.92, 0.98)) > 1e-9 and \
+                min(abs(off - np.radians(1)), abs(off - 2 * np.pi / 3 + np.radians(1))) > 1e-9:
+            assert L.orc_find_channel(C.byref(s1), O.dptr(p), C.byref(rng)) == want
+
+
+@pytest.mark.parametrize("tag", ["mask_amm", "mask_pairs"])
+def test_mask_runs_golden(oracle, gold, tag):
+    """Whole runs with QET masks and AMM film entry on both faces (phonons; pairs at 4 V
+    collected through the masks): statistics bytes, offsets, hits and channels equal the
+    reference's."""
+    O = oracle
+    det = O.mask_amm_detector()
+    if tag == "mask_pairs":
+        O.set_bias(det, 4.0)
+        det.luke_sample_fraction = 0.0
+        wl, n = O.workload(O.WL_PAIR), 6
+    else:
+        wl, n = O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200
+    r = O.Engine(detector=det).run(wl, n, 42, threads=4)
+    assert r["stats_raw"] == gold[f"run_{tag}_stats"].tobytes()
+    assert np.array_equal(r["hit_offsets"], gold[f"run_{tag}_offsets"])
+    h = r["hits"][:600]
+    got = np.c_[h["pos"], h["time"], h["edep"], h["incident"], h["surface"], h["mode"]]
+    assert np.array_equal(got, gold[f"run_{tag}_hits"])
+    assert np.array_equal(h["channel"], gold[f"run_{tag}_channels"])
