@@ -26,7 +26,9 @@ from ._capi import (CHARGE_DTYPE, FORM_BATCHED, FORM_PLAIN, HIT_DTYPE, MODE_BENC
 
 __all__ = ["Lattice", "Engine", "DiffTape", "apply_diff_params", "Rng", "germanium", "default_detector", "set_bias", "demo_polynomial_surfaces",
            "workload", "make_tracks", "CmpmcError", "FORM_PLAIN", "FORM_BATCHED", "PHONON_DTYPE", "CHARGE_DTYPE",
-           "HIT_DTYPE", "WL_PHONON_POINT", "WL_PHONON_LOGN", "WL_PAIR", "WL_PHONON_UNIF", "MODE_BENCH_DOS", "device_info"]
+           "HIT_DTYPE", "WL_PHONON_POINT", "WL_PHONON_LOGN", "WL_PAIR", "WL_PHONON_UNIF", "MODE_BENCH_DOS", "device_info",
+           "constant_split_surfaces", "set_pads", "clear_pads", "qet_mask", "grid_mask", "find_channel", "medium",
+           "medium_from_bulk_shear", "medium_from_lame", "set_amm", "amm_transmission", "amm_psv"]
 
 MASK64 = (1 << 64) - 1
 
@@ -121,6 +123,107 @@ def constant_split_surfaces(det: capi.Detector, f_diffuse: float) -> None:
     for s in det.surf:
         s.n_spec, s.n_diff, s.n_anh = 1, 1, 0
         s.spec[0], s.diff[0], s.spec_prob_const = 1.0 - f_diffuse, f_diffuse, 1.0 - f_diffuse
+
+
+# ------------------------------------------------------------------ electrode masks and AMM film entry
+_PAD_ARRAYS: list = []     # pad arrays a surface points at (the model copies them at creation)
+
+
+def _pad_array(pads) -> C.Array:
+    if isinstance(pads, C.Array):
+        return pads
+    arr = (capi.Pad * len(pads))()
+    for i, row in enumerate(pads):
+        if isinstance(row, capi.Pad):
+            arr[i] = row
+            continue
+        arr[i].shape, arr[i].channel_id = int(row[0]), int(row[1])
+        for j, n in enumerate(("cx", "cy", "r_min", "r_max", "phi_min", "phi_max", "fill_fraction")):
+            setattr(arr[i], n, float(row[2 + j]))
+    return arr
+
+
+def set_pads(surface: capi.Surface, pads) -> None:
+    """Electrode mask of a surface (SurfaceProperty::electrode_mask,
+    geometry/electrode_pattern.hpp:66-92): `pads` as Pad structures or (shape, channel_id,
+    cx, cy, r_min, r_max, phi_min, phi_max, fill_fraction) rows. A phonon or carrier
+    meeting the surface off every pad reflects as from bare crystal; hits carry the pad's
+    channel id."""
+    arr = _pad_array(pads)
+    _PAD_ARRAYS.append(arr)
+    surface.n_pads = len(arr)
+    surface.pads = C.cast(arr, C.POINTER(capi.Pad))
+
+
+def clear_pads(surface: capi.Surface) -> None:
+    surface.n_pads = 0
+    surface.pads = None
+
+
+def qet_mask(radius_m: float, fill_fraction: float = 1.0) -> C.Array:
+    """ElectrodePatternMask::supercdms_qet_mask (electrode_pattern.hpp:99-147)."""
+    out = (capi.Pad * 5)()
+    check(capi.load().cmpmc_qet_mask(radius_m, fill_fraction, out))
+    return out
+
+
+def grid_mask(nx: int, ny: int, pitch_x: float, pitch_y: float, pad_half_w: float, pad_half_h: float,
+              fill_fraction: float = 1.0) -> C.Array:
+    """ElectrodePatternMask::rectangular_grid_mask (electrode_pattern.hpp:150-176)."""
+    out = (capi.Pad * (nx * ny))()
+    check(capi.load().cmpmc_grid_mask(nx, ny, pitch_x, pitch_y, pad_half_w, pad_half_h, fill_fraction, out))
+    return out
+
+
+def find_channel(pads, x: float, y: float, rng: "Rng | None" = None) -> int:
+    """ElectrodePatternMask::find_channel (electrode_pattern.hpp:82-92) on the host; a pad
+    with fill_fraction < 1 draws one uniform from `rng` (advanced in place)."""
+    arr = _pad_array(pads)
+    st = (C.c_uint64 * 4)(*(rng.s if rng is not None else (1, 2, 3, 4)))
+    ch = capi.load().cmpmc_find_channel(arr, len(arr), x, y, st)
+    if rng is not None:
+        rng.s = [int(v) for v in st]
+    return int(ch)
+
+
+def medium(rho: float, vL: float, vT: float) -> capi.Medium:
+    """IsotropicMedium (interface_transmission.hpp:35-48)."""
+    return capi.Medium(rho, vL, vT)
+
+
+def medium_from_bulk_shear(rho: float, K: float, G: float) -> capi.Medium:
+    """IsotropicMedium::from_bulk_shear (interface_transmission.hpp:41-43)."""
+    return capi.Medium(rho, math.sqrt((K + 4.0 * G / 3.0) / rho), math.sqrt(G / rho))
+
+
+def medium_from_lame(rho: float, lam: float, mu: float) -> capi.Medium:
+    """IsotropicMedium::from_lame (interface_transmission.hpp:44-47)."""
+    return capi.Medium(rho, math.sqrt((lam + 2.0 * mu) / rho), math.sqrt(mu / rho))
+
+
+def set_amm(surface: capi.Surface, substrate, film) -> None:
+    """AMM film entry (SurfaceProperty::amm_interface = IsotropicAMMInterface{substrate,
+    film}): a phonon reaching an active part of the surface enters the film with the AMM
+    transmission at its incidence angle instead of film.film_absorption."""
+    surface.has_amm = 1
+    surface.amm_substrate = substrate if isinstance(substrate, capi.Medium) else capi.Medium(*substrate)
+    surface.amm_film = film if isinstance(film, capi.Medium) else capi.Medium(*film)
+
+
+def amm_transmission(substrate, film, longitudinal: bool, cos_theta: float) -> float:
+    """IsotropicAMMInterface::transmission (interface_transmission.hpp:123-129)."""
+    s = substrate if isinstance(substrate, capi.Medium) else capi.Medium(*substrate)
+    f = film if isinstance(film, capi.Medium) else capi.Medium(*film)
+    return float(capi.load().cmpmc_amm_transmission(C.byref(s), C.byref(f), int(longitudinal), cos_theta))
+
+
+def amm_psv(m1, m2, incident_P: bool, theta1: float) -> np.ndarray:
+    """P-SV energy-flux coefficients (R_P, R_S, T_P, T_S) (interface_transmission.hpp:70-117)."""
+    a = m1 if isinstance(m1, capi.Medium) else capi.Medium(*m1)
+    b = m2 if isinstance(m2, capi.Medium) else capi.Medium(*m2)
+    out = (C.c_double * 4)()
+    check(capi.load().cmpmc_amm_psv(C.byref(a), C.byref(b), int(incident_P), theta1, out))
+    return np.array(out[:])
 
 
 def set_bias(det: capi.Detector, volts: float, thickness_m: float = 2.54e-2) -> None:

@@ -47,13 +47,24 @@ class Film(C.Structure):
         "absorber_eff_slope", "temperature")] + [("keep_all_phonons", i32), ("loss_model", i32)]
 
 
+class Pad(C.Structure):
+    _fields_ = [("shape", i32), ("channel_id", i32)] + [(n, dbl) for n in (
+        "cx", "cy", "r_min", "r_max", "phi_min", "phi_max", "fill_fraction")]
+
+
+class Medium(C.Structure):
+    _fields_ = [("rho", dbl), ("vL", dbl), ("vT", dbl)]
+
+
 class Surface(C.Structure):
     _fields_ = [(n, dbl) for n in (
         "prob_abs", "prob_refl", "refl_cutoff", "anh_cutoff", "prob_charge_abs", "prob_charge_spec",
         "min_k_elec", "min_k_hole")] + [
         ("n_spec", i32), ("n_diff", i32), ("n_anh", i32), ("has_film", i32),
         ("spec", dbl * MAX_POLY), ("diff", dbl * MAX_POLY), ("anh", dbl * MAX_POLY),
-        ("spec_prob_const", dbl), ("film", Film)]
+        ("spec_prob_const", dbl), ("film", Film),
+        ("n_pads", i32), ("has_amm", i32), ("pads", C.POINTER(Pad)),
+        ("amm_substrate", Medium), ("amm_film", Medium)]
 
 
 class Detector(C.Structure):
@@ -139,7 +150,7 @@ class TuneResult(C.Structure):
 TUNE_FORCE = 1
 
 ABI_STRUCTS = [Crystal, Film, Surface, Detector, Lattice, PHONON_DTYPE, CHARGE_DTYPE, HIT_DTYPE, Stats,
-               EventBatch, Workload, RunOpts, RunOut, DiffParams, TuneResult]
+               EventBatch, Workload, RunOpts, RunOut, DiffParams, TuneResult, Pad, Medium]
 
 _lib = None
 
@@ -198,6 +209,19 @@ def load():
     L.cmpmc_plan_destroy.argtypes = [C.c_void_p]
     L.cmpmc_autotune.argtypes = [C.c_void_p, C.POINTER(Workload), u64, u64, C.c_char_p, C.c_int, C.POINTER(RunOpts),
                                  C.POINTER(TuneResult), C.c_void_p]
+    L.cmpmc_qet_mask.argtypes = [dbl, dbl, C.POINTER(Pad)]
+    L.cmpmc_grid_mask.argtypes = [C.c_int, C.c_int, dbl, dbl, dbl, dbl, dbl, C.POINTER(Pad)]
+    L.cmpmc_find_channel.argtypes = [C.POINTER(Pad), C.c_int, dbl, dbl, C.POINTER(u64)]
+    L.cmpmc_find_channel.restype = i32
+    L.cmpmc_amm_normal.argtypes = [dbl, dbl]
+    L.cmpmc_amm_normal.restype = dbl
+    L.cmpmc_amm_sh.argtypes = [C.POINTER(Medium), C.POINTER(Medium), dbl]
+    L.cmpmc_amm_sh.restype = dbl
+    L.cmpmc_dmm.argtypes = [C.POINTER(Medium), C.POINTER(Medium)]
+    L.cmpmc_dmm.restype = dbl
+    L.cmpmc_amm_psv.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, dbl, C.POINTER(dbl)]
+    L.cmpmc_amm_transmission.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, dbl]
+    L.cmpmc_amm_transmission.restype = dbl
     check_abi(L)
     _lib = L
     return L
@@ -223,5 +247,6 @@ EXPORTED_SYMBOLS = [
     "cmpmc_model_map_k_to_vg", "cmpmc_transport_serial",
     "cmpmc_transport_charges_serial", "cmpmc_run_events", "cmpmc_run_workload", "cmpmc_diff_forward",
     "cmpmc_diff_backward", "cmpmc_diff_tape_size", "cmpmc_diff_tape_free", "cmpmc_plan_create", "cmpmc_plan_run",
-    "cmpmc_plan_recordings", "cmpmc_plan_destroy", "cmpmc_autotune",
+    "cmpmc_plan_recordings", "cmpmc_plan_destroy", "cmpmc_autotune", "cmpmc_qet_mask", "cmpmc_grid_mask",
+    "cmpmc_find_channel", "cmpmc_amm_normal", "cmpmc_amm_sh", "cmpmc_dmm", "cmpmc_amm_psv", "cmpmc_amm_transmission",
 ]

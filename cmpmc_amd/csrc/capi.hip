@@ -73,6 +73,8 @@ extern "C" size_t cmpmc_abi_sizeof(int which) {
         case 12: return sizeof(cmpmc_run_out);
         case 13: return sizeof(cmpmc_diff_params);
         case 14: return sizeof(cmpmc_tune_result);
+        case 15: return sizeof(cmpmc_pad);
+        case 16: return sizeof(cmpmc_medium);
         default: return 0;
     }
 }
@@ -179,6 +181,7 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     m.ion_h_mfp[0] = c.e_trap_ion_a_mfp_m; m.ion_h_mfp[1] = c.h_trap_ion_a0_mfp_m;
     m.radius = det->radius; m.half_z = det->half_z;
     for (int i = 0; i < 3; ++i) m.efield[i] = det->efield[i];
+    int32_t pad_off = 0;
     for (int s = 0; s < 3; ++s) {
         const cmpmc_surface& a = det->surf[s];
         DevSurface& b = m.surf[s];
@@ -194,7 +197,15 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
         b.film = DevFilm{f.film_absorption, f.film_thickness, f.v_sound, f.gap_energy, f.phonon_lifetime,
                          f.phonon_lifetime_slope, f.low_qp_limit, f.high_qp_limit, f.subgap_absorption, f.absorber_gap,
                          f.absorber_eff, f.absorber_eff_slope, f.temperature, f.keep_all_phonons, 0};
+        b.n_pads = a.n_pads;
+        b.pad_off = pad_off;
+        pad_off += a.n_pads;
+        b.has_amm = a.has_amm;
+        b.amm_sub[0] = a.amm_substrate.rho; b.amm_sub[1] = a.amm_substrate.vL; b.amm_sub[2] = a.amm_substrate.vT;
+        b.amm_film[0] = a.amm_film.rho; b.amm_film[1] = a.amm_film.vL; b.amm_film[2] = a.amm_film.vT;
     }
+    m.pads = M->d_pads;
+    m.amm_tab = M->d_amm;
     m.max_bounces = det->max_bounces; m.rr_bounces = det->russian_roulette_bounces;
     m.min_phonon_eV = det->min_phonon_eV; m.luke_fraction = det->luke_sample_fraction;
     m.charge_stepping = det->charge_stepping;
@@ -218,6 +229,53 @@ static int check_detector(const cmpmc_detector* d) {
             return fail(CMPMC_ERR_INVALID, "surface %d: polynomial length must be in [0, %d]", s, CMPMC_MAX_POLY);
         if (a.has_film && a.film.loss_model != 0)
             return fail(CMPMC_ERR_INVALID, "surface %d: only the MeanFreePath film-loss model is supported", s);
+        if (a.n_pads < 0 || a.n_pads > CMPMC_MAX_PADS || (a.n_pads > 0 && !a.pads))
+            return fail(CMPMC_ERR_INVALID, "surface %d: n_pads must be in [0, %d] with a pad array", s, CMPMC_MAX_PADS);
+        for (int i = 0; i < a.n_pads; ++i) {
+            const cmpmc_pad& q = a.pads[i];
+            if (q.shape < CMPMC_PAD_DISK || q.shape > CMPMC_PAD_RECTANGLE)
+                return fail(CMPMC_ERR_INVALID, "surface %d pad %d: unknown shape %d", s, i, q.shape);
+            if (!std::isfinite(q.cx) || !std::isfinite(q.cy) || !std::isfinite(q.r_min) || !std::isfinite(q.r_max) ||
+                !std::isfinite(q.phi_min) || !std::isfinite(q.phi_max) || std::isnan(q.fill_fraction))
+                return fail(CMPMC_ERR_INVALID, "surface %d pad %d: non-finite geometry", s, i);
+        }
+        if (a.has_amm) {
+            const cmpmc_medium* md[2] = {&a.amm_substrate, &a.amm_film};
+            for (const cmpmc_medium* q : md)
+                if (!(q->rho > 0.0) || !(q->vL > 0.0) || !(q->vT > 0.0) || !std::isfinite(q->rho) ||
+                    !std::isfinite(q->vL) || !std::isfinite(q->vT))
+                    return fail(CMPMC_ERR_INVALID, "surface %d: AMM media need finite rho, vL, vT > 0", s);
+        }
+    }
+    return CMPMC_OK;
+}
+
+// Electrode pads of all surfaces in one device array, and the FP32 AMM film-entry tables
+// ([surface][L, T][kAmmTab + 1], uniform in |cos theta|) of the batched form.
+static int upload_surfaces(cmpmc_model* M, const cmpmc_detector* det) {
+    int n_pads = 0;
+    bool any_amm = false;
+    for (int s = 0; s < 3; ++s) {
+        n_pads += det->surf[s].n_pads;
+        any_amm = any_amm || det->surf[s].has_amm;
+    }
+    if (n_pads > 0) {
+        std::vector<cmpmc_pad> all;
+        all.reserve(n_pads);
+        for (int s = 0; s < 3; ++s)
+            all.insert(all.end(), det->surf[s].pads, det->surf[s].pads + det->surf[s].n_pads);
+        HIPCHK(hipMalloc(&M->d_pads, all.size() * sizeof(cmpmc_pad)));
+        HIPCHK(hipMemcpy(M->d_pads, all.data(), all.size() * sizeof(cmpmc_pad), hipMemcpyHostToDevice));
+    }
+    if (any_amm) {
+        const size_t per = kAmmTab + 1;
+        std::vector<float> tab(3 * 2 * per, 0.0f);
+        for (int s = 0; s < 3; ++s) {
+            const cmpmc_surface& a = det->surf[s];
+            if (a.has_amm) cmpmc_internal::amm_entry_table(&a.amm_substrate, &a.amm_film, kAmmTab, tab.data() + s * 2 * per);
+        }
+        HIPCHK(hipMalloc(&M->d_amm, tab.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(M->d_amm, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     return CMPMC_OK;
 }
@@ -261,6 +319,12 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
     HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&M->d_cancel), hc, 0));
     M->det = *det;
     M->cfg = lat->cfg;
+    if (upload_surfaces(M, det) != CMPMC_OK) {
+        cmpmc_model_destroy(M);
+        return CMPMC_ERR_HIP;
+    }
+    // the model keeps no pointer into the caller's pad arrays
+    for (int s = 0; s < 3; ++s) M->det.surf[s].pads = nullptr;
     fill_dev_model(M, lat, det);
     *out = M;
     return CMPMC_OK;
@@ -280,6 +344,8 @@ extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
     if (M->d_oct) (void)hipFree(M->d_oct);
     if (M->d_cdf) (void)hipFree(M->d_cdf);
     if (M->d_stiff) (void)hipFree(M->d_stiff);
+    if (M->d_pads) (void)hipFree(M->d_pads);
+    if (M->d_amm) (void)hipFree(M->d_amm);
     if (M->h_cancel) (void)hipHostFree(const_cast<int32_t*>(M->h_cancel));
     delete M;
     return CMPMC_OK;
@@ -1146,6 +1212,9 @@ extern "C" int cmpmc_diff_forward(cmpmc_model* M, const cmpmc_workload* wl, uint
         if (sf.n_spec > 1 || sf.n_diff > 1 || sf.n_anh > 0)
             return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: surface %d has energy-dependent reflection or "
                         "down-conversion; the diffuse fraction is defined for constant splits only", k);
+        if (sf.n_pads > 0 || sf.has_amm)
+            return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: surface %d has an electrode mask or AMM film entry; "
+                        "the film probability p is defined for uniform films only", k);
     }
     std::lock_guard<std::mutex> lk(M->mu);
     cmpmc_internal::clear_error();

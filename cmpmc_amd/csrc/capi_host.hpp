@@ -68,6 +68,8 @@ struct cmpmc_model {
     float4* d_oct = nullptr;
     double* d_cdf = nullptr;
     double* d_stiff = nullptr;              // C_ijlm: 81 doubles, then 81 floats
+    cmpmc_pad* d_pads = nullptr;            // electrode pads of all surfaces
+    float* d_amm = nullptr;                 // FP32 AMM film-entry tables [3][2][kAmmTab + 1]
     // host-mapped cancel word the kernels poll: bit 0 is the caller's cancel flag
     // (cmpmc_model_cancel), bit 1 the run's time budget (set by the watchdog thread)
     volatile int32_t* h_cancel = nullptr;

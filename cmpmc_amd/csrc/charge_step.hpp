@@ -7,7 +7,7 @@
 // Ctx (in addition to the phonon context):
 //   ctx.ccount(CCounter), ctx.luke_eV(double), ctx.recomb(double), ctx.escaped(double),
 //   ctx.deposited(double), ctx.push_charge(CTrack&, CTrack&), ctx.push_luke(PTrack&, CTrack&),
-//   ctx.charge_hit(CTrack&, edep_w, e_rel, surface)
+//   ctx.charge_hit(CTrack&, edep_w, e_rel, surface, channel)
 #pragma once
 
 #include "phonon_step.hpp"
@@ -406,7 +406,15 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             trk.p = trk.p + E * (q_sign * R(kCLight) * dtb);
             trk.ekin = (trk.type == 1) ? ekin_e<P>(m, trk.valley, trk.p) : ekin_h<P>(m, trk.p);
             const DevSurface& s = m.surf[sid];
-            if (!should_absorb<P>(trk, n_in, s, r)) {
+            // engine.hpp:283-293: an electrode mask picks the channel; bare crystal between
+            // pads reflects the carrier
+            int32_t ch = -1;
+            bool active_pad = true;
+            if (s.n_pads > 0) {
+                ch = surface_channel<P>(m, s.pad_off, s.n_pads, trk.pos, r);
+                active_pad = (ch >= 0);
+            }
+            if (!active_pad || !should_absorb<P>(trk, n_in, s, r)) {
                 ctx.ccount(CC_SURF_REFL);
                 trk.pos = trk.pos + n_in * R(1e-11);
                 reflect_charge<P>(m, trk, n_in, s.prob_charge_spec, r);
@@ -418,7 +426,7 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             const double recomb = e_rel * trk.weight;
             ctx.recomb(recomb);
             ctx.deposited(recomb);
-            ctx.charge_hit(trk, recomb, e_rel, sid);
+            ctx.charge_hit(trk, recomb, e_rel, sid, ch);
             return false;
         }
     }

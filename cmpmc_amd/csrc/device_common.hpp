@@ -15,6 +15,8 @@
 #include <cmath>
 
 #include "cmpmc_amd.h"
+#include "electrode_mask.hpp"
+#include "interface_amm.hpp"
 
 // Capacity of each Kaplan cascade list of one lane (core/math.hpp:230, KaplanBuffer).
 #ifndef CMPMC_KCAP
@@ -52,11 +54,20 @@ struct DevSurface {
     double spec[CMPMC_MAX_POLY], diff[CMPMC_MAX_POLY], anh[CMPMC_MAX_POLY];
     double spec_prob_const;
     DevFilm film;
+    // electrode mask: pads [pad_off, pad_off + n_pads) of DevModel::pads (n_pads = 0: none);
+    // AMM film entry (has_amm) between these isotropic media (rho, vL, vT)
+    int32_t n_pads, pad_off, has_amm, _pad_m;
+    double amm_sub[3], amm_film[3];
 };
+
+// Entries per mode of the batched form's AMM film-entry table, uniform in |cos theta|.
+constexpr int kAmmTab = 4096;
 
 // Passed by value as a kernel argument (read through the scalar cache).
 struct DevModel {
     const float4* oct;          // [3][256][256] (vx, vy, vz, |v|)
+    const cmpmc_pad* pads;      // electrode pads of all surfaces (DevSurface::pad_off, n_pads)
+    const float* amm_tab;       // [3 surfaces][L, T][kAmmTab + 1] AMM entry probabilities (FP32 form)
     const double* stiff;        // C_ijlm [3][3][3][3] (exact kinematics)
     const float* stiff_f;       // the same in FP32
     int32_t kin_mode;           // CrystalLattice::KinematicsMode (host side selects the kernel policy)
@@ -92,12 +103,12 @@ struct DevModel {
         double e_refl, e_anh;
         float t_anh[4], t_spec[4];
         float prob_abs, prob_refl, film_abs;
-        int32_t const_split, has_film, _pad;
+        int32_t const_split, has_film, has_amm;
         // the film's pair-breaking threshold 2 gap and direct-absorption threshold
         // 2 absorber_gap (FP64, compared with FP64 energies) and subgap_absorption
         double film_gap2, film_abs_gap2;
         float subgap_abs;
-        int32_t _pad2;
+        int32_t n_pads, pad_off, _pad3;
     } surf_f[3];
     float orient_f[9];
     float fL_f, fLST_f, fTT_f, pST_f, poly_rate_f, radius_f, half_z_f, _pad_f;
@@ -834,7 +845,10 @@ inline void fill_fp32_constants(DevModel& m) {
         f.film_gap2 = 2.0 * fl.gap;
         f.film_abs_gap2 = 2.0 * fl.absorber_gap;
         f.subgap_abs = static_cast<float>(fl.subgap_absorption);
-        f._pad2 = 0;
+        f.n_pads = s.n_pads;
+        f.pad_off = s.pad_off;
+        f.has_amm = s.has_amm;
+        f._pad3 = 0;
         f.prob_abs = static_cast<float>(s.prob_abs);
         f.prob_refl = static_cast<float>(s.prob_refl);
         f.film_abs = static_cast<float>(s.film.film_absorption);
@@ -854,8 +868,35 @@ inline void fill_fp32_constants(DevModel& m) {
             f.t_anh[r] = static_cast<float>(pa);
             f.t_spec[r] = static_cast<float>(pa + ps);
         }
-        f._pad = 0;
     }
+}
+
+// ------------------------------------------------------------------ electrode masks, AMM entry
+// ElectrodePatternMask::find_channel of a surface's pads (engine.hpp:483-488, 287-290):
+// the readout channel of a contact at pos, -1 on bare crystal between pads; draws one
+// uniform when the pad's fill fraction is below 1.
+template <class P, class G>
+__device__ inline int32_t surface_channel(const DevModel& m, int pad_off, int n_pads, V3<typename P::R> pos, G& r) {
+    using R = typename P::R;
+    return cmpmc_mask::find_channel<R>(m.pads + pad_off, n_pads, pos.x, pos.y, [&r]() { return P::uniform(r); });
+}
+
+// The plain form's AMM film-entry probability (IsotropicAMMInterface::transmission,
+// interface_transmission.hpp:123-129), solved per call in FP64; out of line so that the
+// 4x4 complex solve does not weigh on the kernel's registers.
+__device__ __noinline__ double amm_entry64(const DevSurface& s, int longitudinal, double cos_theta) {
+    const cmpmc_amm::Medium sub{s.amm_sub[0], s.amm_sub[1], s.amm_sub[2]};
+    const cmpmc_amm::Medium film{s.amm_film[0], s.amm_film[1], s.amm_film[2]};
+    return cmpmc_amm::transmission(sub, film, longitudinal != 0, cos_theta);
+}
+
+// The batched form's: linear interpolation in |cos theta| of the host-built table.
+__device__ inline float amm_entry32(const DevModel& m, int sid, int longitudinal, float cos_theta) {
+    const float* t = m.amm_tab + (sid * 2 + (longitudinal ? 0 : 1)) * (kAmmTab + 1);
+    const float x = tmin(fabsf(cos_theta), 1.0f) * static_cast<float>(kAmmTab);
+    const int i = tmin(static_cast<int>(x), kAmmTab - 1);
+    const float f = x - static_cast<float>(i);
+    return t[i] + (t[i + 1] - t[i]) * f;
 }
 
 // ------------------------------------------------------------------ Kaplan film cascade

@@ -89,7 +89,8 @@ struct DevHit {
 };
 
 template <class P> __device__ inline void fill_hit(cmpmc_sensor_hit& h, V3<typename P::R> pos, double time, double edep,
-                                                   double incident, int surface, int excitation, int mode) {
+                                                   double incident, int surface, int excitation, int mode,
+                                                   int32_t channel) {
     h.pos[0] = static_cast<double>(pos.x);
     h.pos[1] = static_cast<double>(pos.y);
     h.pos[2] = static_cast<double>(pos.z);
@@ -100,7 +101,7 @@ template <class P> __device__ inline void fill_hit(cmpmc_sensor_hit& h, V3<typen
     h.excitation = static_cast<uint8_t>(excitation);
     h.mode = static_cast<uint8_t>(mode);
     h._pad = 0;
-    h.channel_id = -1;
+    h.channel_id = channel;
 }
 
 // ------------------------------------------------------------------ device generators

@@ -5,7 +5,7 @@
 //   ctx.escaped(double)          escaped_or_killed_eV += value
 //   ctx.deposited(double)        total_deposited_eV += value
 //   ctx.push(PTrack&, PTrack&)   push a daughter (child, parent)
-//   ctx.hit(PTrack&, edep_w, surface)
+//   ctx.hit(PTrack&, edep_w, surface, channel)
 //   ctx.branch(int)              branch code of the step (first-step bookkeeping)
 //   ctx.kaplan_base/stride()     this lane's Kaplan workspace: 5 lists of CMPMC_KCAP
 //                                doubles, element j of list i at base[(i*KCAP + j)*stride]
@@ -179,9 +179,22 @@ __device__ inline bool phonon_step(const DevModel& m, PTrack<P>& trk, Rng& r, Ct
     trk.time += static_cast<double>(dist / trk.vg_mag);
     trk.bounces++;
     const DevSurface& s = m.surf[sid];
-    if (s.has_film && P::uniform(r) < R(s.prob_abs)) {
+    // engine.hpp:483-495: an electrode mask picks the readout channel (bare crystal between
+    // pads reflects); AMM transmission replaces the film-entry probability
+    int32_t ch = -1;
+    bool active_pad = true;
+    if (s.n_pads > 0) {
+        ch = surface_channel<P>(m, s.pad_off, s.n_pads, trk.pos, r);
+        active_pad = (ch >= 0);
+    }
+    if (active_pad && s.has_film && P::uniform(r) < R(s.prob_abs)) {
         const DevFilm& film = s.film;
-        if (P::uniform(r) < R(film.film_absorption)) {
+        R p_enter = R(film.film_absorption);
+        if (s.has_amm) {
+            if constexpr (P::exact) p_enter = amm_entry64(s, trk.mode == 0, dot(trk.k, n_in));
+            else p_enter = amm_entry32(m, sid, trk.mode == 0, dot(trk.k, n_in));
+        }
+        if (P::uniform(r) < p_enter) {
             double* wb = ctx.kaplan_base();
             const uint32_t ws = ctx.kaplan_stride();
             KList L[5];
@@ -192,7 +205,7 @@ __device__ inline bool phonon_step(const DevModel& m, PTrack<P>& trk, Rng& r, Ct
                 const double wed = edep * trk.weight;
                 ctx.count(C_HITS);
                 ctx.deposited(wed);
-                ctx.hit(trk, wed, sid);
+                ctx.hit(trk, wed, sid, ch);
             }
             double sum_refl = 0.0;
             for (int i = 0; i < out.n; ++i) sum_refl += out.get(i);

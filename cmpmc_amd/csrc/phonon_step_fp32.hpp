@@ -36,7 +36,7 @@ enum BulkProcess : int { kProcIso = 0, kProcDecay = 1, kProcPoly = 2 };
 
 template <class P, class Ctx>
 __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& ctx, V3<float>& n_film, int& sid_film,
-                                       bool alive) {
+                                       int& ch_film, bool alive) {
     auto& r = trk.rng;
     int code = alive ? kStepAlive : kStepDead;
     // the direction and mode the closing lookup uses; a lane without a track looks up a
@@ -95,14 +95,24 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
             trk.bounces++;
             const DevModel::SurfF& sf = ctx.surf(sid);
             bool reemit = false;
-            if (sf.has_film && P::uniform(r) < sf.prob_abs) {
-                const bool absorbed = P::uniform(r) < sf.film_abs;
+            // an electrode mask picks the readout channel (bare crystal between pads
+            // reflects); AMM transmission replaces the film-entry probability
+            int32_t ch = -1;
+            bool active_pad = true;
+            if (sf.n_pads > 0) {
+                ch = surface_channel<P>(m, sf.pad_off, sf.n_pads, trk.pos, r);
+                active_pad = (ch >= 0);
+            }
+            if (active_pad && sf.has_film && P::uniform(r) < sf.prob_abs) {
+                const float p_enter = sf.has_amm ? amm_entry32(m, sid, trk.mode == 0, dot(trk.k, n_in)) : sf.film_abs;
+                const bool absorbed = P::uniform(r) < p_enter;
                 ctx.film_gate(absorbed);
                 if (absorbed) {
                     ctx.branch(CMPMC_BR_FILM);
                     if (!(trk.energy < sf.film_gap2)) {
                         n_film = n_in;
                         sid_film = sid;
+                        ch_film = ch;
                         code = kStepFilm;
                     } else if (!(trk.energy < sf.film_abs_gap2) && P::uniform(r) < sf.subgap_abs) {
                         // Below the pair-breaking threshold the film's cascade ends at
@@ -113,7 +123,7 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
                         const double wed = trk.energy * trk.weight;
                         ctx.count(C_HITS);
                         ctx.deposited(wed);
-                        ctx.hit(trk, wed, sid);
+                        ctx.hit(trk, wed, sid, ch);
                         code = kStepDead;
                     } else {
                         reemit = true;
@@ -218,7 +228,7 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
 // 521-552): the cascade, the sensor hit, the re-emitted phonons (pushed as daughters) and
 // the continuing track, which the caller re-queues. Returns false if nothing re-emits.
 template <class P, class Ctx>
-__device__ inline bool film_job(const DevModel& m, PTrack<P>& trk, V3<float> n_in, int sid, Ctx& ctx) {
+__device__ inline bool film_job(const DevModel& m, PTrack<P>& trk, V3<float> n_in, int sid, int ch, Ctx& ctx) {
     auto& r = trk.rng;
     const DevFilm& film = m.surf[sid].film;
     double* wb = ctx.kaplan_base();
@@ -231,7 +241,7 @@ __device__ inline bool film_job(const DevModel& m, PTrack<P>& trk, V3<float> n_i
         const double wed = edep * trk.weight;
         ctx.count(C_HITS);
         ctx.deposited(wed);
-        ctx.hit(trk, wed, sid);
+        ctx.hit(trk, wed, sid, ch);
     }
     double sum_refl = 0.0;
     for (int i = 0; i < out.n; ++i) sum_refl += out.get(i);

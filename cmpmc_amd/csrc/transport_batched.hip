@@ -249,7 +249,7 @@ template <bool Diff, class P> struct BatchCtx {
         first = __shfl(first, leader);
         return first + rank;
     }
-    __device__ void hit(PTrack<P>& t, double wed, int sid) {
+    __device__ void hit(PTrack<P>& t, double wed, int sid, int32_t ch) {
         const uint32_t seq = t.hit_seq++;
         if constexpr (Diff) {
             const unsigned long long i = reserve(tape_count, tbase, tfill, tvalid);
@@ -270,7 +270,7 @@ template <bool Diff, class P> struct BatchCtx {
             d.k0 = event;
             d.k1 = t.key;
             d.k2 = seq;
-            fill_hit<P>(d.h, t.pos, t.time, wed, t.energy, sid, 0, t.mode);
+            fill_hit<P>(d.h, t.pos, t.time, wed, t.energy, sid, 0, t.mode, ch);
         }
     }
 };
@@ -427,6 +427,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                 srec_to_ptrack(jb, j, jev);
                 const V3<float> n_in{jb.n_in[0], jb.n_in[1], jb.n_in[2]};
                 const int sid = jb.sid;
+                const int ch = jb.channel;
                 const uint64_t own_event = ctx.event;
                 const bool own_rec = ctx.rec;
                 double own_sc[4];
@@ -434,7 +435,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                 Ops::load_sc(jr, ctx.sc);
                 ctx.event = jev;
                 ctx.rec = false;
-                if (film_job<P>(m, j, n_in, sid, ctx)) ctx.store(j);
+                if (film_job<P>(m, j, n_in, sid, ch, ctx)) ctx.store(j);
                 ctx.event = own_event;
                 ctx.rec = own_rec;
                 for (int i = 0; i < 4; ++i) ctx.sc[i] = own_sc[i];
@@ -453,8 +454,8 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
             }
             if (!stop) {
                 V3<float> n_film;
-                int sid_film = 0;
-                const int code = phonon_step_fp32<P>(m, t, ctx, n_film, sid_film, alive);
+                int sid_film = 0, ch_film = -1;
+                const int code = phonon_step_fp32<P>(m, t, ctx, n_film, sid_film, ch_film, alive);
                 alive = (code == kStepAlive);
                 if (!alive) ctx.rec = false;
                 const uint64_t film_mask = __ballot(code == kStepFilm);
@@ -465,6 +466,7 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
                         ptrack_to_srec(t, ctx.event, jb);
                         jb.n_in[0] = n_film.x; jb.n_in[1] = n_film.y; jb.n_in[2] = n_film.z;
                         jb.sid = sid_film;
+                        jb.channel = ch_film;
                         Ops::store_sc(jr, ctx.sc);
                     }
                     nf += static_cast<uint32_t>(__popcll(film_mask));
@@ -614,7 +616,7 @@ template <class P> struct ChargeCtx {
         if (i < luke_cap) ptrack_to_btrack(ph, event, luke_out[i]);
         else { overflow = true; bs_ledger(*bs, L_ESC, ph.energy * ph.weight); }
     }
-    __device__ void charge_hit(const CTrack<P>& c, double recomb_w, double e_rel, int sid) {
+    __device__ void charge_hit(const CTrack<P>& c, double recomb_w, double e_rel, int sid, int32_t ch) {
         if (!keep_hits) return;
         atomicAdd(hit_valid, 1ull);
         const unsigned long long i = atomicAdd(hit_count, 1ull);
@@ -623,7 +625,7 @@ template <class P> struct ChargeCtx {
             d.k0 = event;
             d.k1 = c.key;
             d.k2 = 0;
-            fill_hit<P>(d.h, c.pos, c.time, recomb_w, e_rel, sid, c.type, 0);
+            fill_hit<P>(d.h, c.pos, c.time, recomb_w, e_rel, sid, c.type, 0, ch);
         }
     }
 };

@@ -82,8 +82,8 @@ struct alignas(16) BCharge {
 
 // Spawn-pool entry and deferred film interaction of the batched phonon kernel
 // (128 B): the complete track state including its random stream (xoshiro128** words), so
-// that a track continues exactly where it stopped. n_in / sid are the film entry normal
-// and surface.
+// that a track continues exactly where it stopped. n_in / sid / channel are the film
+// entry normal, surface and readout channel.
 struct alignas(16) SRec {
     float pos[3];
     float k[3];
@@ -97,7 +97,8 @@ struct alignas(16) SRec {
     uint32_t spawn_no, hit_seq;
     float n_in[3];
     int32_t sid;
-    uint32_t _pad[2];
+    int32_t channel;                   // readout channel of the film entry (-1: no mask)
+    uint32_t _pad;
 };
 
 // Spawn record of the differentiable form: the track plus its accumulated relative

@@ -89,14 +89,14 @@ template <class P> struct PlainCtx {
         }
         ++seq;
     }
-    __device__ void hit(const PTrack<P>& t, double wed, int sid) {
+    __device__ void hit(const PTrack<P>& t, double wed, int sid, int32_t ch) {
         cmpmc_sensor_hit h;
-        fill_hit<P>(h, t.pos, t.time, wed, t.energy, sid, 0, t.mode);
+        fill_hit<P>(h, t.pos, t.time, wed, t.energy, sid, 0, t.mode, ch);
         emit(h);
     }
-    __device__ void charge_hit(const CTrack<P>& t, double recomb, double e_rel, int sid) {
+    __device__ void charge_hit(const CTrack<P>& t, double recomb, double e_rel, int sid, int32_t ch) {
         cmpmc_sensor_hit h;
-        fill_hit<P>(h, t.pos, t.time, recomb, e_rel, sid, t.type, 0);
+        fill_hit<P>(h, t.pos, t.time, recomb, e_rel, sid, t.type, 0, ch);
         emit(h);
     }
 };

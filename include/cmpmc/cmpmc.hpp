@@ -5,6 +5,8 @@
 #include "cmpmc/core/core.hpp"
 #include "cmpmc/materials/crystal_config.hpp"
 #include "cmpmc/lattice/crystal_lattice.hpp"
+#include "cmpmc/geometry/electrode_pattern.hpp"
+#include "cmpmc/phonon/interface_transmission.hpp"
 #include "cmpmc/geometry/surface_property.hpp"
 #include "cmpmc/transport/engine.hpp"
 #include "cmpmc/transport/parallel.hpp"

@@ -1,14 +1,17 @@
 // Drop-in geometry and sensor-film API (geometry/solids.hpp, geometry/surface_property.hpp,
 // superconductor/kaplan_film.hpp of the reference): the cylinder solid, per-surface
-// boundary models and the Kaplan film configuration. The film cascade itself runs on
-// the GPU inside the transport kernels.
+// boundary models (with the optional electrode mask and AMM film entry) and the Kaplan
+// film configuration. The film cascade itself runs on the GPU inside the transport
+// kernels.
 #pragma once
 
 #include <limits>
 #include <optional>
 #include <vector>
 
+#include "cmpmc/geometry/electrode_pattern.hpp"
 #include "cmpmc/materials/crystal_config.hpp"
+#include "cmpmc/phonon/interface_transmission.hpp"
 
 namespace cmpmc {
 
@@ -70,6 +73,11 @@ struct SurfaceProperty {
     std::vector<double> diff_coeffs = {0.0};
     std::vector<double> anh_coeffs = {0.0};
     std::optional<KaplanFilm> kaplan_sensor = std::nullopt;
+    // AMM film entry: a phonon reaching an active pad enters the film with the AMM
+    // transmission of its mode at its incidence angle instead of film_absorption
+    std::optional<physics::IsotropicAMMInterface> amm_interface = std::nullopt;
+    // electrode mask: hits carry the pad's channel; bare crystal between pads reflects
+    std::optional<geometry::ElectrodePatternMask> electrode_mask = std::nullopt;
     double spec_prob_const = 0.0;
     bool demo_values = false;
 
@@ -120,7 +128,10 @@ struct SurfaceProperty {
         return s;
     }
 
-    cmpmc_surface to_pod() const {
+    // The C description of this surface. With an electrode mask, the pads are written to
+    // *pads (which must outlive the cmpmc_surface: the library copies them when a model is
+    // created).
+    cmpmc_surface to_pod(std::vector<cmpmc_pad>* pads = nullptr) const {
         if (spec_coeffs.size() > CMPMC_MAX_POLY || diff_coeffs.size() > CMPMC_MAX_POLY || anh_coeffs.size() > CMPMC_MAX_POLY)
             throw std::invalid_argument("cmpmc_amd: at most 8 polynomial coefficients per surface model");
         cmpmc_surface s{};
@@ -136,6 +147,17 @@ struct SurfaceProperty {
         s.spec_prob_const = spec_prob_const;
         s.has_film = kaplan_sensor.has_value() ? 1 : 0;
         if (kaplan_sensor) s.film = kaplan_sensor->to_pod();
+        if (amm_interface) {
+            s.has_amm = 1;
+            s.amm_substrate = amm_interface->substrate.to_pod();
+            s.amm_film = amm_interface->film.to_pod();
+        }
+        if (electrode_mask && !electrode_mask->empty()) {
+            if (!pads) throw std::invalid_argument("cmpmc_amd: SurfaceProperty::to_pod needs pad storage for a mask");
+            *pads = electrode_mask->to_pod();
+            s.n_pads = static_cast<int32_t>(pads->size());
+            s.pads = pads->data();
+        }
         return s;
     }
 };

@@ -81,14 +81,16 @@ struct DetectorModel {
 
     Vec3 electric_field_at(const Vec3& p) const noexcept { return e_field.at(p); }
 
-    cmpmc_detector to_pod() const {
+    // The C description of the detector; the surfaces' electrode pads go to pads[i]
+    // (which must outlive the result until the model is created).
+    cmpmc_detector to_pod(std::array<std::vector<cmpmc_pad>, 3>* pads = nullptr) const {
         if (charge_release != ChargeRelease::RecordOnly)
             throw std::invalid_argument("cmpmc_amd: only ChargeRelease::RecordOnly runs on the GPU path");
         cmpmc_detector d{};
         d.radius = solid.radius;
         d.half_z = solid.half_z;
         d.efield[0] = e_field.field_Vm.x; d.efield[1] = e_field.field_Vm.y; d.efield[2] = e_field.field_Vm.z;
-        for (int i = 0; i < 3; ++i) d.surf[i] = surfaces[i].to_pod();
+        for (int i = 0; i < 3; ++i) d.surf[i] = surfaces[i].to_pod(pads ? &(*pads)[i] : nullptr);
         d.max_bounces = max_bounces;
         d.russian_roulette_bounces = russian_roulette_bounces;
         d.min_phonon_eV = min_phonon_eV;
@@ -137,7 +139,8 @@ public:
             if (sp.demo_values)
                 require_validated(detector_.allow_unvalidated, "SurfaceProperty::demo_polynomial_*",
                                   "demonstration reflection/down-conversion polynomials, not physically motivated");
-        const cmpmc_detector det = detector_.to_pod();
+        std::array<std::vector<cmpmc_pad>, 3> pads;
+        const cmpmc_detector det = detector_.to_pod(&pads);
         cmpmc_model* m = nullptr;
         check_status(cmpmc_model_create(&lattice_.pod(), &det, &m));
         model_ = std::shared_ptr<cmpmc_model>(m, [](cmpmc_model* p) { cmpmc_model_destroy(p); });

@@ -136,8 +136,30 @@ typedef struct {
 } cmpmc_film;
 
 #define CMPMC_MAX_POLY 8
+#define CMPMC_MAX_PADS 65536          /* electrode pads per surface */
+
+/* geometry/electrode_pattern.hpp:23-35 ElectrodePad: one pad of a patterned sensor /
+ * electrode layout on a face, in the detector's (x, y). Disk about (cx, cy) of radius
+ * r_max; AnnularSector r in [r_min, r_max], azimuth in [phi_min, phi_max] (wrapping across
+ * 0 when phi_min > phi_max); Rectangle [cx -+ r_min] x [cy -+ r_max]. A pad with
+ * fill_fraction < 1 is active metal with that probability (one uniform per contact). */
+enum { CMPMC_PAD_DISK = 0, CMPMC_PAD_ANNULAR_SECTOR = 1, CMPMC_PAD_RECTANGLE = 2 };
+typedef struct {
+    int32_t shape;
+    int32_t channel_id;
+    double cx, cy, r_min, r_max, phi_min, phi_max, fill_fraction;
+} cmpmc_pad;
+
+/* phonon/interface_transmission.hpp:35-48 IsotropicMedium */
+typedef struct {
+    double rho, vL, vT;           /* kg/m^3, m/s, m/s */
+} cmpmc_medium;
+
 /* geometry/surface_property.hpp:23-179 SurfaceProperty (polynomials as fixed arrays;
- * n_* = 0 is the reference's empty coefficient vector) */
+ * n_* = 0 is the reference's empty coefficient vector). n_pads > 0: the surface's
+ * electrode mask (ElectrodePatternMask; pads[n_pads], read at model creation). has_amm:
+ * a phonon reaching an active pad enters the film with the AMM probability of
+ * IsotropicAMMInterface{amm_substrate, amm_film} instead of film.film_absorption. */
 typedef struct {
     double prob_abs, prob_refl, refl_cutoff, anh_cutoff;
     double prob_charge_abs, prob_charge_spec, min_k_elec, min_k_hole;
@@ -145,6 +167,10 @@ typedef struct {
     double spec[CMPMC_MAX_POLY], diff[CMPMC_MAX_POLY], anh[CMPMC_MAX_POLY];
     double spec_prob_const;
     cmpmc_film film;
+    int32_t n_pads;
+    int32_t has_amm;
+    const cmpmc_pad* pads;
+    cmpmc_medium amm_substrate, amm_film;
 } cmpmc_surface;
 
 /* transport/engine.hpp:96-150 DetectorModel (cylinder solid, uniform field) */
@@ -272,6 +298,29 @@ int cmpmc_film_aluminum(cmpmc_film* out);
 /* DetectorModel{} defaults: engine.hpp:96-150 with diffuse_al_sensor_surface() on both
  * faces and diffuse_bare_surface() on the side wall (surface_property.hpp:232-248). */
 int cmpmc_detector_default(cmpmc_detector* out);
+
+/* ---------------------------------------------------------------- surfaces: masks and AMM */
+/* ElectrodePatternMask::supercdms_qet_mask (electrode_pattern.hpp:99-147): five pads (core
+ * disk 0.55 R, three 120-degree sectors 0.57-0.90 R with 2-degree gaps, guard ring
+ * 0.92-0.98 R) into out[5]. */
+int cmpmc_qet_mask(double crystal_radius_m, double fill_fraction, cmpmc_pad out[5]);
+/* ElectrodePatternMask::rectangular_grid_mask (electrode_pattern.hpp:150-176): nx * ny
+ * rectangular pads centred on the face, channel ids in x-major order, into out. */
+int cmpmc_grid_mask(int nx, int ny, double pitch_x, double pitch_y, double pad_half_w, double pad_half_h,
+                    double fill_fraction, cmpmc_pad* out);
+/* ElectrodePatternMask::find_channel (electrode_pattern.hpp:82-92) on the host: the channel
+ * of the first pad containing (x, y) or -1; a pad with fill_fraction < 1 draws one uniform
+ * from the xoshiro256** stream rng_state[4] (advanced in place). */
+int32_t cmpmc_find_channel(const cmpmc_pad* pads, int n_pads, double x, double y, uint64_t rng_state[4]);
+/* interface_transmission.hpp:56-141 on the host: normal-incidence, SH and DMM
+ * transmission; the P-SV energy-flux coefficients (R_P, R_S, T_P, T_S) into out[4]; and
+ * IsotropicAMMInterface::transmission (L: longitudinal; cos_theta = k . n). */
+double cmpmc_amm_normal(double Z1, double Z2);
+double cmpmc_amm_sh(const cmpmc_medium* m1, const cmpmc_medium* m2, double theta1);
+double cmpmc_dmm(const cmpmc_medium* m1, const cmpmc_medium* m2);
+int cmpmc_amm_psv(const cmpmc_medium* m1, const cmpmc_medium* m2, int incident_P, double theta1, double out[4]);
+double cmpmc_amm_transmission(const cmpmc_medium* substrate, const cmpmc_medium* film, int longitudinal,
+                              double cos_theta);
 
 /* CrystalLattice(const CrystalConfig&) table build: crystal_lattice.hpp:45-55, 340-477.
  * oct_lut: 3*256*256*4 floats; cdf_lt, cdf_tt: 1024 doubles each (caller-owned);

@@ -149,6 +149,49 @@ int main(int argc, char** argv) {
         check("exact kinematics: batched steps vs plain (5%)", static_cast<double>(rb.stats.total_steps),
               static_cast<double>(r.stats.total_steps), 0.05);
     }
+    {   // electrode mask and AMM film entry (SurfaceProperty::electrode_mask, amm_interface)
+        const double R = det.solid.radius;
+        const auto qet = geometry::ElectrodePatternMask::supercdms_qet_mask(R, 0.9);
+        check("QET mask: 5 pads", static_cast<double>(qet.num_pads()), 5, 0.0);
+        check("QET mask: centre is channel 0", qet.find_geometric_channel(0.0, 0.0), 0, 0.0);
+        check("QET mask: r = 0.7 R at 60 deg is channel 1",
+              qet.find_geometric_channel(0.7 * R * 0.5, 0.7 * R * std::sqrt(0.75)), 1, 0.0);
+        check("QET mask: r = 0.56 R is bare crystal", qet.find_geometric_channel(0.56 * R, 0.0), -1, 0.0);
+        check("QET mask: r = 0.95 R is the guard ring", qet.find_geometric_channel(0.0, -0.95 * R), 4, 0.0);
+        const physics::IsotropicMedium ge{5320.0, 5310.0, 3250.0};
+        const auto al = physics::IsotropicMedium::from_bulk_shear(2720.0, 76e9, 26e9);
+        const physics::IsotropicAMMInterface amm{ge, al};
+        check("AMM: normal-incidence L = 4 Z1 Z2 / (Z1 + Z2)^2", amm.transmission(true, 1.0),
+              physics::amm_normal(ge.rho * ge.vL, al.rho * al.vL), 1e-9);
+        const auto c = physics::amm_psv(ge, al, true, 0.4);
+        check("AMM: P-SV flux coefficients sum to 1", c.sum(), 1.0, 1e-9);
+        DetectorModel dm = det;
+        for (int f : {0, 1}) {
+            dm.surfaces[f].electrode_mask = qet;
+            dm.surfaces[f].amm_interface = amm;
+        }
+        const SimulationEngine em(lattice, dm);
+        auto gen = [&](EventInput& in) {
+            in.phonons.push_back(em.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::L, 4e-3));
+            in.injected_eV += 4e-3;
+        };
+        RunOptions o;
+        o.seed = 13;
+        for (int form : {CMPMC_FORM_PLAIN, CMPMC_FORM_BATCHED}) {
+            o.form = form;
+            const RunResult r = run_events(em, 300, gen, o);
+            const std::string tag = form == CMPMC_FORM_PLAIN ? "masked, plain" : "masked, batched";
+            check(tag + ": deposited + escaped = injected", r.stats.total_deposited_eV + r.stats.escaped_or_killed_eV,
+                  r.stats.total_injected_eV, 1e-9);
+            size_t bad = 0, seen[5] = {0, 0, 0, 0, 0};
+            for (const SensorHit& h : r.hits) {
+                if (h.channel_id < 0 || h.channel_id > 4) ++bad;
+                else ++seen[h.channel_id];
+            }
+            check(tag + ": every hit carries a QET channel", static_cast<double>(bad), 0.0, 0.0);
+            check(tag + ": core and sectors all hit", (seen[0] && seen[1] && seen[2] && seen[3]) ? 1 : 0, 1, 0.0);
+        }
+    }
     {   // a worker exception reaches the caller (guarded Matrix IV model)
         CrystalConfig cfg = materials::Germanium();
         cfg.iv_model = IVRateModel::Matrix;

@@ -127,3 +127,43 @@ def test_host_exact_kinematics_bit_exact(lib_cmpmc):
     for m in range(3):
         d, mag = lat.map_k_to_vg(m, k)
         assert np.array_equal(np.c_[d, mag], gold["map_exact_vg"][m])
+
+
+@pytest.mark.parametrize("tag", ["amm_geal", "amm_siw"])
+def test_host_amm_equals_reference(lib_cmpmc, tag):
+    """The library's AMM model (the code the plain kernel runs, compiled for the host)
+    equals the reference's P-SV coefficients, SH transmission and film-entry probability
+    bit for bit over 300 angles (golden vectors written by the reference)."""
+    cm = lib_cmpmc
+    g = np.load(GOLD)
+    m1, m2 = (tuple(row) for row in g[f"{tag}_media"])
+    for i, th in enumerate(g[f"{tag}_theta"]):
+        assert np.array_equal(cm.amm_psv(m1, m2, True, th), g[f"{tag}_psv"][i, :4], equal_nan=True)
+        assert np.array_equal(cm.amm_psv(m1, m2, False, th), g[f"{tag}_psv"][i, 4:], equal_nan=True)
+        L = cm.capi.load()
+        a, b = cm.capi.Medium(*m1), cm.capi.Medium(*m2)
+        got = [L.cmpmc_amm_sh(C.byref(a), C.byref(b), th), cm.amm_transmission(m1, m2, True, np.cos(th)),
+               cm.amm_transmission(m1, m2, False, np.cos(th))]
+        assert np.array_equal(got, [g[f"{tag}_{k}"][i] for k in ("sh", "tl", "tt")], equal_nan=True)
+
+
+def test_host_masks_equal_reference(lib_cmpmc, oracle):
+    """cmpmc_qet_mask equals ElectrodePatternMask::supercdms_qet_mask's pads (through the
+    oracle's restatement, pinned to the reference), and cmpmc_find_channel with fill 0.8
+    returns the reference's channels at 2000 points from the same stream; the grid preset
+    centres nx x ny pads on the face with x-major channel ids."""
+    cm = lib_cmpmc
+    g = np.load(GOLD)
+    pads = cm.qet_mask(0.0381, 0.8)
+    for got, want in zip(pads, oracle.qet_pads(0.0381, 0.8)):
+        assert (got.shape, got.channel_id) == want[:2]
+        assert [got.cx, got.cy, got.r_min, got.r_max, got.phi_min, got.phi_max, got.fill_fraction] == \
+            pytest.approx(list(want[2:]), rel=1e-15, abs=0.0)
+    rng = cm.Rng(31)
+    got = [cm.find_channel(pads, float(x), float(y), rng) for x, y in g["qet_xy"]]
+    assert np.array_equal(np.array(got), g["qet_channels"])
+    grid = cm.grid_mask(3, 2, 0.01, 0.02, 0.004, 0.005)
+    assert [p.channel_id for p in grid] == list(range(6))
+    assert [(p.cx, p.cy) for p in grid] == pytest.approx([(-0.01, -0.01), (-0.01, 0.01), (0.0, -0.01), (0.0, 0.01),
+                                                           (0.01, -0.01), (0.01, 0.01)])
+    assert cm.find_channel(grid, 0.0031, 0.0149) == 3 and cm.find_channel(grid, 0.0041, 0.01) == -1
