@@ -417,16 +417,18 @@ class Engine:
                      keep_hits: bool = True, hit_capacity: int | None = None, first_branch: bool = False,
                      block: int = 0, headroom: int = 0, lane_depth: int = 0, grid_blocks: int = 0,
                      refill_period: int = 0, poll_cancel: bool = False, waves: int = 0, time_budget_s: float = 0.0,
-                     stream=None) -> dict:
+                     stream=None, no_alloc: bool = False) -> dict:
         """run_events with the generator on the device (parallel.hpp:76-148). time_budget_s > 0
-        stops the run with CMPMC_ERR_CANCELLED once it has run that long."""
+        stops the run with CMPMC_ERR_CANCELLED once it has run that long. no_alloc: run
+        through cmpmc_run_workload_noalloc (every device buffer from the library's fixed
+        per-device scratch; no device allocation)."""
         o = self._opts(form, seed, keep_hits, first_branch, block, headroom, lane_depth, grid_blocks, refill_period,
                        poll_cancel, waves, time_budget_s)
         L = capi.load()
+        fn = L.cmpmc_run_workload_noalloc if no_alloc else L.cmpmc_run_workload
         if keep_hits and hit_capacity is None:
             hit_capacity = self.hit_estimate(wl, n_events)
-        return self._run(lambda out: L.cmpmc_run_workload(self.ptr, C.byref(wl), n_events, C.byref(o), C.byref(out),
-                                                          stream),
+        return self._run(lambda out: fn(self.ptr, C.byref(wl), n_events, C.byref(o), C.byref(out), stream),
                          n_events, keep_hits, hit_capacity, first_branch)
 
     def autotune(self, wl: capi.Workload, n_events: int, seed: int = 1, cache: str | None = None,

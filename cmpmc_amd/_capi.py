@@ -89,10 +89,12 @@ PHONON_DTYPE = np.dtype({"names": ["pos", "k_dir", "vg_dir", "energy", "vg_mag",
 CHARGE_DTYPE = np.dtype({"names": ["pos", "p_eV", "ekin", "time", "weight", "valley", "type", "alive"],
                          "formats": [("<f8", (3,)), ("<f8", (3,)), "<f8", "<f8", "<f8", "<u2", "u1", "u1"],
                          "offsets": [0, 24, 48, 56, 64, 72, 74, 75], "itemsize": 128})
-HIT_DTYPE = np.dtype({"names": ["pos", "time", "edep", "incident_energy", "surface_id", "excitation", "mode",
+# every byte is a field (the reserved byte too): numpy copies of a structured array leave
+# unnamed gap bytes uninitialised
+HIT_DTYPE = np.dtype({"names": ["pos", "time", "edep", "incident_energy", "surface_id", "excitation", "mode", "_pad",
                                 "channel_id"],
-                      "formats": [("<f8", (3,)), "<f8", "<f8", "<f8", "u1", "u1", "u1", "<i4"],
-                      "offsets": [0, 24, 32, 40, 48, 49, 50, 52], "itemsize": 56})
+                      "formats": [("<f8", (3,)), "<f8", "<f8", "<f8", "u1", "u1", "u1", "u1", "<i4"],
+                      "offsets": [0, 24, 32, 40, 48, 49, 50, 51, 52], "itemsize": 56})
 
 STATS_FIELDS = [
     ("total_steps", u64), ("isotope_scatters", u64), ("polycrystal_scatters", u64), ("decays_L_to_TT", u64),
@@ -194,6 +196,8 @@ def load():
     L.cmpmc_run_events.argtypes = [C.c_void_p, C.POINTER(EventBatch), C.POINTER(RunOpts), C.POINTER(RunOut), C.c_void_p]
     L.cmpmc_run_workload.argtypes = [C.c_void_p, C.POINTER(Workload), u64, C.POINTER(RunOpts), C.POINTER(RunOut),
                                      C.c_void_p]
+    L.cmpmc_run_workload_noalloc.argtypes = [C.c_void_p, C.POINTER(Workload), u64, C.POINTER(RunOpts),
+                                             C.POINTER(RunOut), C.c_void_p]
     L.cmpmc_diff_forward.argtypes = [C.c_void_p, C.POINTER(Workload), u64, C.POINTER(DiffParams), dbl,
                                      C.POINTER(RunOpts), C.POINTER(dbl), C.POINTER(Stats), C.POINTER(C.c_void_p),
                                      C.POINTER(dbl), C.c_void_p]
@@ -249,4 +253,5 @@ EXPORTED_SYMBOLS = [
     "cmpmc_diff_backward", "cmpmc_diff_tape_size", "cmpmc_diff_tape_free", "cmpmc_plan_create", "cmpmc_plan_run",
     "cmpmc_plan_recordings", "cmpmc_plan_destroy", "cmpmc_autotune", "cmpmc_qet_mask", "cmpmc_grid_mask",
     "cmpmc_find_channel", "cmpmc_amm_normal", "cmpmc_amm_sh", "cmpmc_dmm", "cmpmc_amm_psv", "cmpmc_amm_transmission",
+    "cmpmc_run_workload_noalloc",
 ]

@@ -5,6 +5,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <cstdarg>
@@ -139,6 +140,43 @@ extern "C" int cmpmc_detector_default(cmpmc_detector* d) {
     d->charge_stepping = 0; d->charge_sampling_dk_frac = 0.25;
     return CMPMC_OK;
 }
+
+// ------------------------------------------------------------------ scratch of the no-allocation entry point
+// cmpmc_run_workload_noalloc carves every run buffer from this array instead of calling
+// hipMalloc. Each GPU the library runs on holds its own copy (kScratchBytes) from module
+// load until the process ends. Host code never uses the array's name as a device address:
+// the address is resolved once per device with hipGetSymbolAddress (scratch_ptr) and
+// that pointer is what the run buffers point into.
+namespace {
+constexpr size_t kScratchBytes = static_cast<size_t>(CMPMC_NOALLOC_SCRATCH_MIB) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+struct ScratchSlot {
+    std::once_flag once;
+    void* ptr = nullptr;
+    hipError_t err = hipSuccess;
+    std::mutex mu;           // one no-allocation call at a time on this device (they share the array)
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+// Resolves the scratch address of `device`, which must be the current device; once per
+// device. Returns nullptr (and records why) when the address cannot be resolved.
+void* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) {
+        fail(CMPMC_ERR_INVALID, "scratch: device %d is beyond the %d devices this library serves", device, kMaxDevices);
+        return nullptr;
+    }
+    ScratchSlot& sl = g_scratch[device];
+    std::call_once(sl.once, [&sl] {
+        void* p = nullptr;
+        sl.err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        sl.ptr = (sl.err == hipSuccess) ? p : nullptr;
+    });
+    if (!sl.ptr) fail(CMPMC_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(sl.err));
+    return sl.ptr;
+}
+}  // namespace
 
 // ------------------------------------------------------------------ model
 
@@ -319,6 +357,10 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
     HIPCHK(hipHostGetDevicePointer(reinterpret_cast<void**>(&M->d_cancel), hc, 0));
     M->det = *det;
     M->cfg = lat->cfg;
+    // the scratch address of this device, resolved with the device current (a failure is
+    // reported by the no-allocation entry point, the only user)
+    (void)scratch_ptr(M->device);
+    cmpmc_internal::clear_error();
     if (upload_surfaces(M, det) != CMPMC_OK) {
         cmpmc_model_destroy(M);
         return CMPMC_ERR_HIP;
@@ -330,12 +372,16 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
     return CMPMC_OK;
 }
 
+// The model's cached run buffers.
+static std::array<DevBuf*, 32> run_buffers(cmpmc_model* M) {
+    return {&M->ws, &M->pst, &M->cst, &M->poff, &M->coff, &M->pinit, &M->cinit, &M->rng, &M->inj, &M->tin,
+            &M->cin, &M->tin_off, &M->cin_off, &M->hits, &M->hits_tmp, &M->keys_a, &M->keys_b, &M->idx_a,
+            &M->idx_b, &M->sorted, &M->counts, &M->fb, &M->acc, &M->queue, &M->cqueue, &M->lstack,
+            &M->clstack, &M->qoff, &M->wl, &M->cub_tmp, &M->fjobs, &M->scan_part};
+}
+
 static void trim(cmpmc_model* M) {
-    for (DevBuf* b : {&M->ws, &M->pst, &M->cst, &M->poff, &M->coff, &M->pinit, &M->cinit, &M->rng, &M->inj, &M->tin,
-                      &M->cin, &M->tin_off, &M->cin_off, &M->hits, &M->hits_tmp, &M->keys_a, &M->keys_b, &M->idx_a,
-                      &M->idx_b, &M->sorted, &M->counts, &M->fb, &M->acc, &M->queue, &M->cqueue, &M->lstack,
-                      &M->clstack, &M->qoff, &M->wl, &M->cub_tmp, &M->fjobs, &M->scan_part})
-        b->release();
+    for (DevBuf* b : run_buffers(M)) b->release();
 }
 
 extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
@@ -1182,6 +1228,70 @@ extern "C" int cmpmc_run_workload(cmpmc_model* M, const cmpmc_workload* wl, uint
     out->kernel_ms = 0.0;
     out->passes = 0;
     return run_job(M, J, C, O, static_cast<hipStream_t>(stream));
+}
+
+// cmpmc_run_workload with every run buffer (stacks, queues, Kaplan workspace, counters,
+// hit staging and sort space) carved from this device's scratch array: no hipMalloc,
+// hipFree or allocating library call. The model's own cached buffers are set aside for
+// the call and restored afterwards. A run whose buffers do not fit is refused before
+// anything is launched when the fixed-size parts already exceed the scratch, and fails
+// with CMPMC_ERR_CAPACITY when a later buffer does not fit.
+extern "C" int cmpmc_run_workload_noalloc(cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events,
+                                          const cmpmc_run_opts* opts, cmpmc_run_out* out, void* stream) {
+    if (!M || !wl || !out) return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload_noalloc: null argument");
+    CHK(check_workload(M, wl, n_events));
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != M->device)
+        return fail(CMPMC_ERR_INVALID, "cmpmc_run_workload_noalloc: the current device %d is not the model's (%d)", dev,
+                    M->device);
+    void* base = scratch_ptr(dev);
+    if (!base) return CMPMC_ERR_HIP;
+    std::lock_guard<std::mutex> lk(M->mu);
+    std::lock_guard<std::mutex> slk(g_scratch[dev].mu);
+    cmpmc_internal::clear_error();
+    Job J;
+    J.n_events = n_events;
+    J.first_event = 0;
+    J.wl = wl;
+    const RunCfg C = cfg_from(opts);
+    CHK(check_cfg(C));
+    if (J.n_events == 0) return empty_run(out);
+    // the parts whose size the options fix: the Kaplan workspace of every lane, the spawn
+    // pools and film-job buffers of the batched form, and the per-event stacks of the plain form
+    const size_t kws_lane = static_cast<size_t>(5) * CMPMC_KCAP * sizeof(double);
+    size_t fixed = 0;
+    if (C.form == CMPMC_FORM_BATCHED) {
+        const int block = C.block > 0 ? C.block : 256;
+        const int waves = C.waves ? C.waves : kDefaultWaves;
+        const uint64_t lanes = static_cast<uint64_t>(grid_for(M, 0, C, block, waves)) * block;
+        const uint64_t depth = C.lane_depth ? C.lane_depth : kDefaultPool;
+        fixed = lanes * kws_lane + (lanes / 64) * (depth + kFilmCap) * sizeof(SRec);
+    } else {
+        fixed = std::min<uint64_t>(n_events, 262144) * (kws_lane + 128 * sizeof(cmpmc_phonon_track));
+    }
+    if (fixed > kScratchBytes)
+        return fail(CMPMC_ERR_CAPACITY,
+                    "cmpmc_run_workload_noalloc: the run needs at least %zu MiB of buffers, more than the %zu MiB "
+                    "scratch; run fewer events or, in the batched form, fewer grid_blocks",
+                    (fixed + (1u << 20) - 1) >> 20, kScratchBytes >> 20);
+    Outputs O{&out->stats, out->hits, out->hit_capacity, &out->n_hits, &out->hits_needed, out->hit_offsets,
+              out->first_branch, &out->kernel_ms, &out->stack_needed, &out->passes};
+    out->kernel_ms = 0.0;
+    out->passes = 0;
+    // set the cached buffers aside, run on the arena, put them back
+    const std::array<DevBuf*, 32> bufs = run_buffers(M);
+    std::array<DevBuf, 32> saved;
+    for (size_t i = 0; i < bufs.size(); ++i) {
+        saved[i] = *bufs[i];
+        *bufs[i] = DevBuf{};
+    }
+    Arena arena{static_cast<char*>(base), kScratchBytes, 0};
+    t_arena = &arena;
+    const int rc = run_job(M, J, C, O, static_cast<hipStream_t>(stream));
+    t_arena = nullptr;
+    for (size_t i = 0; i < bufs.size(); ++i) *bufs[i] = saved[i];
+    return rc;
 }
 
 // ------------------------------------------------------------------ differentiable form

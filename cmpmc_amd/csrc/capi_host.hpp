@@ -31,20 +31,47 @@ namespace cmpmc_host {
 
 using namespace cmpmc_dev;
 
-// A device allocation that only grows (ensure) until released.
+// A device allocation that only grows (ensure) until released. Buffers carved from an
+// Arena are not owned and are never freed.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    bool owned = true;
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && owned) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
+        owned = true;
     }
 };
+
+// A bump allocator over a fixed device region. While one is installed on the calling
+// thread (t_arena), ensure() carves 256-byte aligned buffers from it instead of calling
+// hipMalloc, and fails with CMPMC_ERR_CAPACITY when the region is exhausted.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+};
+inline thread_local Arena* t_arena = nullptr;
 
 inline int ensure(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (b.bytes >= bytes) return CMPMC_OK;
+    if (t_arena) {
+        Arena& a = *t_arena;
+        const size_t off = (a.used + 255) & ~static_cast<size_t>(255);
+        if (off > a.cap || bytes > a.cap - off)
+            return cmpmc_internal::fail(CMPMC_ERR_CAPACITY,
+                                        "scratch exhausted: a buffer of %zu B does not fit in the %zu MiB scratch "
+                                        "(%zu B in use); run fewer events or, in the batched form, fewer grid_blocks",
+                                        bytes, a.cap >> 20, a.used);
+        b.p = a.base + off;
+        b.bytes = bytes;
+        b.owned = false;
+        a.used = off + bytes;
+        return CMPMC_OK;
+    }
     b.release();
     const hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {

@@ -392,6 +392,21 @@ int cmpmc_run_events(cmpmc_model* m, const cmpmc_event_batch* batch, const cmpmc
 int cmpmc_run_workload(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events,
                        const cmpmc_run_opts* opts, cmpmc_run_out* out, void* stream);
 
+/* No counterpart in the reference: cmpmc_run_workload for callers that cannot allocate
+ * device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results (bit-identical statistics and hits for the same
+ * workload, options and seed); every run buffer comes from a fixed per-device scratch of
+ * CMPMC_NOALLOC_SCRATCH_MIB MiB that the library holds from load until the process ends,
+ * so the call makes no hipMalloc, hipFree or allocating library call. A run that needs
+ * more (roughly 26 KiB per event in the plain form; in the batched form 10 KiB per lane
+ * plus the spawn pools, so pass a small grid_blocks) fails with CMPMC_ERR_CAPACITY and a
+ * message naming the limit. Calls on one device are serialised inside the library (they
+ * share the scratch, which keeps the last call's contents); the current device must be
+ * the model's. Hits and offsets are written to the caller's host buffers in `out`. */
+#define CMPMC_NOALLOC_SCRATCH_MIB 64
+int cmpmc_run_workload_noalloc(cmpmc_model* m, const cmpmc_workload* wl, uint64_t n_events,
+                               const cmpmc_run_opts* opts, cmpmc_run_out* out, void* stream);
+
 /* ---------------------------------------------------------------- graph-captured plans
  * No counterpart in the reference (its run_events re-plans every call, parallel.hpp:76-148):
  * a batched-form cmpmc_run_workload of a fixed workload and event count, recorded once
