@@ -21,14 +21,17 @@ import time
 import numpy as np
 
 from . import _capi as capi
-from ._capi import (CHARGE_DTYPE, FORM_BATCHED, FORM_PLAIN, HIT_DTYPE, MODE_BENCH_DOS, PHONON_DTYPE, WL_PAIR,
+from ._capi import (CHARGE_DTYPE, FORM_BATCHED, FORM_PLAIN, HIT_DTYPE, MODE_BENCH_DOS, NIEL_EMPIRICAL,
+                    NIEL_IMPACT_TUNL, NIEL_LEWIN_SMITH, NIEL_LINDHARD, NIEL_SARKIS, PHONON_DTYPE, WL_DEPOSIT, WL_PAIR,
                     WL_PHONON_LOGN, WL_PHONON_POINT, WL_PHONON_UNIF, CmpmcError, check)
 
 __all__ = ["Lattice", "Engine", "DiffTape", "apply_diff_params", "Rng", "germanium", "default_detector", "set_bias", "demo_polynomial_surfaces",
            "workload", "make_tracks", "CmpmcError", "FORM_PLAIN", "FORM_BATCHED", "PHONON_DTYPE", "CHARGE_DTYPE",
            "HIT_DTYPE", "WL_PHONON_POINT", "WL_PHONON_LOGN", "WL_PAIR", "WL_PHONON_UNIF", "MODE_BENCH_DOS", "device_info",
            "constant_split_surfaces", "set_pads", "clear_pads", "qet_mask", "grid_mask", "find_channel", "medium",
-           "medium_from_bulk_shear", "medium_from_lame", "set_amm", "amm_transmission", "amm_psv"]
+           "medium_from_bulk_shear", "medium_from_lame", "set_amm", "amm_transmission", "amm_psv", "WL_DEPOSIT",
+           "partition_opts", "ionization_yield", "lindhard_yield", "sample_fano_binomial", "partition_deposit",
+           "NIEL_LINDHARD", "NIEL_LEWIN_SMITH", "NIEL_SARKIS", "NIEL_IMPACT_TUNL", "NIEL_EMPIRICAL"]
 
 MASK64 = (1 << 64) - 1
 
@@ -251,14 +254,72 @@ def demo_polynomial_surfaces(det: capi.Detector, max_bounces: int = 100) -> None
 
 def workload(kind: int, mode: int = 0, energy_eV: float = 7.5e-3, origin=(0.0, 0.0, 0.0), logn_median: float = 200.0,
              logn_sigma: float = 1.0, n_min: int = 1, n_max: int = 20000, e_lo_eV: float = 1e-3, e_hi_eV: float = 31e-3,
-             pair_at_origin: bool = False) -> capi.Workload:
+             pair_at_origin: bool = False, partition: dict | None = None) -> capi.Workload:
+    """A device-generated workload. DEPOSIT: a deposit of energy_eV (mode 0 electron
+    recoil, 1 nuclear recoil) at `origin` (pair_at_origin set) or uniform in the volume,
+    partitioned with the PartitionOptions defaults changed by `partition`."""
     w = capi.Workload()
     w.kind, w.mode, w.energy_eV = kind, mode, energy_eV
     for i in range(3):
         w.origin[i] = origin[i]
     w.logn_median, w.logn_sigma, w.n_min, w.n_max = logn_median, logn_sigma, n_min, n_max
     w.e_lo_eV, w.e_hi_eV, w.pair_at_origin = e_lo_eV, e_hi_eV, int(pair_at_origin)
+    w.partition = partition_opts(**(partition or {}))
     return w
+
+
+# ------------------------------------------------------------------ energy partition
+def partition_opts(**changes) -> capi.PartitionOpts:
+    """PartitionOptions{} (energy_partition.hpp:22-29) with fields changed; nested
+    empirical fields as `empirical_<name>`."""
+    o = capi.PartitionOpts()
+    check(capi.load().cmpmc_partition_opts_default(C.byref(o)))
+    for k, v in changes.items():
+        if k.startswith("empirical_"):
+            setattr(o.empirical, k[len("empirical_"):], v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def ionization_yield(model: int, energy_eV: float, crystal: capi.Crystal | None = None,
+                     empirical: capi.EmpiricalNiel | None = None) -> float:
+    """compute_ionization_yield (niel_models.hpp:161-173) for the crystal's target."""
+    c = crystal if crystal is not None else germanium()
+    return float(capi.load().cmpmc_ionization_yield(model, energy_eV, C.byref(c), C.byref(empirical) if empirical else None))
+
+
+def lindhard_yield(energy_eV: float, crystal: capi.Crystal | None = None, z1: float = 0.0, a1: float = 0.0) -> float:
+    """EnergyPartition::lindhard_ionization_yield (energy_partition.hpp:55-74)."""
+    c = crystal if crystal is not None else germanium()
+    return float(capi.load().cmpmc_lindhard_yield(energy_eV, C.byref(c), z1, a1))
+
+
+def sample_fano_binomial(mean: float, fano: float, rng: "Rng") -> int:
+    """sample_fano_binomial (niel_models.hpp:203-261) from `rng` (advanced in place)."""
+    st = (C.c_uint64 * 4)(*rng.s)
+    n = capi.load().cmpmc_sample_fano_binomial(mean, fano, st)
+    rng.s = [int(v) for v in st]
+    return int(n)
+
+
+def partition_deposit(lattice: "Lattice", pos, edep_eV: float, nuclear: bool, rng: "Rng",
+                      opts: capi.PartitionOpts | None = None, time_s: float = 0.0) -> dict:
+    """EnergyPartition::partition_deposit (energy_partition.hpp:78-157) on the host with
+    `rng` (advanced in place): {"result", "charges" (CHARGE_DTYPE), "phonons" (PHONON_DTYPE)}."""
+    L = capi.load()
+    o = opts if opts is not None else partition_opts()
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    st = (C.c_uint64 * 4)(*rng.s)
+    nq, nph, res = C.c_uint64(), C.c_uint64(), capi.PartitionResult()
+    qcap, pcap = 2 * int(o.max_sampled_pairs), int(o.max_sampled_phonons)
+    q = np.zeros(max(qcap, 1), dtype=CHARGE_DTYPE)
+    ph = np.zeros(max(pcap, 1), dtype=PHONON_DTYPE)
+    check(L.cmpmc_partition_deposit(C.byref(lattice.c), p.ctypes.data_as(C.POINTER(C.c_double)), time_s, edep_eV,
+                                    int(nuclear), C.byref(o), st, q.ctypes.data, qcap, C.byref(nq), ph.ctypes.data,
+                                    pcap, C.byref(nph), C.byref(res)))
+    rng.s = [int(v) for v in st]
+    return {"result": res.as_dict(), "charges": q[:nq.value].copy(), "phonons": ph[:nph.value].copy()}
 
 
 class Lattice:

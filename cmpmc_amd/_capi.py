@@ -19,7 +19,8 @@ dbl = C.c_double
 
 OK, ERR_INVALID, ERR_UNVALIDATED, ERR_CAPACITY, ERR_HIP, ERR_NO_DEVICE, ERR_CANCELLED = range(7)
 FORM_PLAIN, FORM_BATCHED = 0, 1
-WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_PHONON_UNIF = 1, 2, 3, 5
+WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_PHONON_UNIF, WL_DEPOSIT = 1, 2, 3, 5, 6
+NIEL_LINDHARD, NIEL_LEWIN_SMITH, NIEL_SARKIS, NIEL_IMPACT_TUNL, NIEL_EMPIRICAL = 0, 1, 2, 3, 4
 # workload mode 3: the reference benchmark generator's mode draw (cmpmc_bench.cpp:63-65)
 MODE_BENCH_DOS = 3
 BRANCH_NAMES = {0: "none", 1: "killed", 2: "isotope", 3: "poly", 4: "decay_TT", 5: "decay_LT", 6: "film",
@@ -37,7 +38,7 @@ class Crystal(C.Structure):
         "ac_deform_e", "ac_deform_h", "iv_min_energy", "iv_quad_rate", "iv_quad_field", "iv_quad_power",
         "iv_lin_rate0", "iv_lin_rate1", "iv_lin_power", "e_trapping_mfp_m", "h_trapping_mfp_m",
         "e_trap_ion_d0_mfp_m", "e_trap_ion_a_mfp_m", "h_trap_ion_d_mfp_m", "h_trap_ion_a0_mfp_m")] + [
-        ("orient", (dbl * 3) * 3)]
+        ("orient", (dbl * 3) * 3)] + [(n, dbl) for n in ("atomic_Z", "atomic_A", "pair_energy", "fano_factor")]
 
 
 class Film(C.Structure):
@@ -120,10 +121,29 @@ class EventBatch(C.Structure):
                 ("injected_eV", C.POINTER(dbl)), ("first_event_id", u64)]
 
 
+class EmpiricalNiel(C.Structure):
+    _fields_ = [("energy_dependent_k", i32), ("_pad", i32)] + [(n, dbl) for n in (
+        "k_fixed", "k_low", "k_high", "e_low_eV", "e_high_eV")]
+
+
+class PartitionOpts(C.Structure):
+    _fields_ = [("niel_model", i32), ("use_fano_fluctuations", i32), ("use_charge_cloud", i32), ("_pad", i32),
+                ("max_sampled_pairs", u32), ("max_sampled_phonons", u32), ("empirical", EmpiricalNiel)]
+
+
+class PartitionResult(C.Structure):
+    _fields_ = [(n, dbl) for n in ("total_edep_eV", "ionizing_eV", "niel_eV", "lindhard_yield")] + [
+        (n, u64) for n in ("n_pairs_true", "n_pairs_sampled", "n_phonons_true", "n_phonons_sampled")] + [
+        (n, dbl) for n in ("charge_weight", "phonon_weight", "prompt_phonon_eV")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class Workload(C.Structure):
     _fields_ = [("kind", i32), ("mode", i32), ("energy_eV", dbl), ("origin", dbl * 3), ("logn_median", dbl),
                 ("logn_sigma", dbl), ("n_min", u32), ("n_max", u32), ("e_lo_eV", dbl), ("e_hi_eV", dbl),
-                ("pair_at_origin", i32), ("_pad", i32)]
+                ("pair_at_origin", i32), ("_pad", i32), ("partition", PartitionOpts)]
 
 
 class RunOpts(C.Structure):
@@ -152,7 +172,8 @@ class TuneResult(C.Structure):
 TUNE_FORCE = 1
 
 ABI_STRUCTS = [Crystal, Film, Surface, Detector, Lattice, PHONON_DTYPE, CHARGE_DTYPE, HIT_DTYPE, Stats,
-               EventBatch, Workload, RunOpts, RunOut, DiffParams, TuneResult, Pad, Medium]
+               EventBatch, Workload, RunOpts, RunOut, DiffParams, TuneResult, Pad, Medium, EmpiricalNiel,
+               PartitionOpts, PartitionResult]
 
 _lib = None
 
@@ -226,6 +247,16 @@ def load():
     L.cmpmc_amm_psv.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, dbl, C.POINTER(dbl)]
     L.cmpmc_amm_transmission.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, dbl]
     L.cmpmc_amm_transmission.restype = dbl
+    L.cmpmc_partition_opts_default.argtypes = [C.POINTER(PartitionOpts)]
+    L.cmpmc_ionization_yield.argtypes = [C.c_int, dbl, C.POINTER(Crystal), C.POINTER(EmpiricalNiel)]
+    L.cmpmc_ionization_yield.restype = dbl
+    L.cmpmc_lindhard_yield.argtypes = [dbl, C.POINTER(Crystal), dbl, dbl]
+    L.cmpmc_lindhard_yield.restype = dbl
+    L.cmpmc_sample_fano_binomial.argtypes = [dbl, dbl, C.POINTER(u64)]
+    L.cmpmc_sample_fano_binomial.restype = u64
+    L.cmpmc_partition_deposit.argtypes = [C.POINTER(Lattice), C.POINTER(dbl), dbl, dbl, C.c_int, C.POINTER(PartitionOpts),
+                                          C.POINTER(u64), C.c_void_p, u64, C.POINTER(u64), C.c_void_p, u64,
+                                          C.POINTER(u64), C.POINTER(PartitionResult)]
     check_abi(L)
     _lib = L
     return L
@@ -253,5 +284,6 @@ EXPORTED_SYMBOLS = [
     "cmpmc_diff_backward", "cmpmc_diff_tape_size", "cmpmc_diff_tape_free", "cmpmc_plan_create", "cmpmc_plan_run",
     "cmpmc_plan_recordings", "cmpmc_plan_destroy", "cmpmc_autotune", "cmpmc_qet_mask", "cmpmc_grid_mask",
     "cmpmc_find_channel", "cmpmc_amm_normal", "cmpmc_amm_sh", "cmpmc_dmm", "cmpmc_amm_psv", "cmpmc_amm_transmission",
-    "cmpmc_run_workload_noalloc",
+    "cmpmc_run_workload_noalloc", "cmpmc_partition_opts_default", "cmpmc_ionization_yield", "cmpmc_lindhard_yield",
+    "cmpmc_sample_fano_binomial", "cmpmc_partition_deposit",
 ]

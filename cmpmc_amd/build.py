@@ -37,7 +37,8 @@ UNITS = [
     ("capi.hip", []),
     ("capi_plan.hip", []),
 ]
-HOST_UNITS = [("lattice_build.cpp", ["-ffp-contract=off"]), ("surfaces.cpp", ["-ffp-contract=off"])]
+HOST_UNITS = [("lattice_build.cpp", ["-ffp-contract=off"]), ("surfaces.cpp", ["-ffp-contract=off"]),
+              ("partition.cpp", ["-ffp-contract=off"])]
 
 
 def _run(cmd):

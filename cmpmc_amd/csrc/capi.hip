@@ -76,6 +76,9 @@ extern "C" size_t cmpmc_abi_sizeof(int which) {
         case 14: return sizeof(cmpmc_tune_result);
         case 15: return sizeof(cmpmc_pad);
         case 16: return sizeof(cmpmc_medium);
+        case 17: return sizeof(cmpmc_empirical_niel);
+        case 18: return sizeof(cmpmc_partition_opts);
+        case 19: return sizeof(cmpmc_partition_result);
         default: return 0;
     }
 }
@@ -101,6 +104,7 @@ extern "C" int cmpmc_crystal_germanium(cmpmc_crystal* c) {
     c->iv_quad_rate = 6.72e-2; c->iv_quad_field = 217.0; c->iv_quad_power = 3.24;
     c->iv_lin_rate0 = 1.5e6; c->iv_lin_rate1 = 1.5; c->iv_lin_power = 4.0;
     for (int i = 0; i < 3; ++i) c->orient[i][i] = 1.0;
+    c->atomic_Z = 32.0; c->atomic_A = 72.63; c->pair_energy = 2.96; c->fano_factor = 0.1057;
     return CMPMC_OK;
 }
 

@@ -70,6 +70,7 @@ struct CrystalConfig {
         c.h_trap_ion_d_mfp_m = h_trap_ion_d_mfp_m; c.h_trap_ion_a0_mfp_m = h_trap_ion_a0_mfp_m;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) c.orient[i][j] = orient.m[i][j];
+        c.atomic_Z = atomic_Z; c.atomic_A = atomic_A; c.pair_energy = pair_energy; c.fano_factor = fano_factor;
         return c;
     }
 };

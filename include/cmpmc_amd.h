@@ -105,6 +105,9 @@ typedef struct {
     double e_trapping_mfp_m, h_trapping_mfp_m;
     double e_trap_ion_d0_mfp_m, e_trap_ion_a_mfp_m, h_trap_ion_d_mfp_m, h_trap_ion_a0_mfp_m;
     double orient[3][3];
+    /* energy partition (crystal_config.hpp:46-47, 87-88): target Z and A [g/mol], mean
+     * energy per e-/h+ pair [eV], Fano factor */
+    double atomic_Z, atomic_A, pair_energy, fano_factor;
 } cmpmc_crystal;
 
 #define CMPMC_N_OCT 256
@@ -215,22 +218,50 @@ enum {
     CMPMC_WL_PHONON_POINT = 1,   /* one phonon (mode, energy) at origin, isotropic k */
     CMPMC_WL_PHONON_LOGN = 2,    /* lognormal count, uniform volume, DOS modes, U[e_lo, e_hi] */
     CMPMC_WL_PAIR = 3,           /* one e-/h+ pair (uniform volume unless pair_at_origin) */
-    CMPMC_WL_PHONON_UNIF = 5     /* n ~ U{n_min..n_max} phonons (mode, energy), uniform volume */
+    CMPMC_WL_PHONON_UNIF = 5,    /* n ~ U{n_min..n_max} phonons (mode, energy), uniform volume */
+    CMPMC_WL_DEPOSIT = 6         /* a deposit of energy_eV partitioned by partition_deposit */
 };
 /* Workload mode 3 (POINT only): the initial mode is drawn from the event stream before
  * the direction, as the reference's benchmark generator does (benchmarks/cmpmc_bench.cpp:
  * 63-65): ST if u < 0.53539, FT if u < 0.90217, else L. */
 #define CMPMC_MODE_BENCH_DOS 3
+/* ---------------------------------------------------------------- energy partition
+ * charge/niel_models.hpp:28-43 NIELModel and EmpiricalNIELParams, energy_partition.hpp:
+ * 22-29 PartitionOptions, :37-49 EnergyPartition::Result. */
+enum { CMPMC_NIEL_LINDHARD = 0, CMPMC_NIEL_LEWIN_SMITH = 1, CMPMC_NIEL_SARKIS = 2, CMPMC_NIEL_IMPACT_TUNL = 3,
+       CMPMC_NIEL_EMPIRICAL = 4 };
+typedef struct {
+    int32_t energy_dependent_k;
+    int32_t _pad;
+    double k_fixed, k_low, k_high, e_low_eV, e_high_eV;
+} cmpmc_empirical_niel;
+typedef struct {
+    int32_t niel_model;
+    int32_t use_fano_fluctuations;
+    int32_t use_charge_cloud;
+    int32_t _pad;
+    uint32_t max_sampled_pairs;    /* pairs beyond this are represented by weight */
+    uint32_t max_sampled_phonons;  /* prompt phonons beyond this are represented by weight */
+    cmpmc_empirical_niel empirical;
+} cmpmc_partition_opts;
+typedef struct {
+    double total_edep_eV, ionizing_eV, niel_eV, lindhard_yield;
+    uint64_t n_pairs_true, n_pairs_sampled, n_phonons_true, n_phonons_sampled;
+    double charge_weight, phonon_weight, prompt_phonon_eV;
+} cmpmc_partition_result;
+
 typedef struct {
     int32_t kind;
-    int32_t mode;                 /* 0 L, 1 FT, 2 ST; 3 CMPMC_MODE_BENCH_DOS (POINT only) */
+    int32_t mode;                 /* 0 L, 1 FT, 2 ST; 3 CMPMC_MODE_BENCH_DOS (POINT only);
+                                     DEPOSIT: 0 electron recoil, 1 nuclear recoil */
     double energy_eV;
     double origin[3];
     double logn_median, logn_sigma;
     uint32_t n_min, n_max;
     double e_lo_eV, e_hi_eV;
-    int32_t pair_at_origin;
+    int32_t pair_at_origin;       /* PAIR: at (0,0,0); DEPOSIT: at `origin` (else uniform) */
     int32_t _pad;
+    cmpmc_partition_opts partition;   /* DEPOSIT */
 } cmpmc_workload;
 
 typedef struct {
@@ -321,6 +352,31 @@ double cmpmc_dmm(const cmpmc_medium* m1, const cmpmc_medium* m2);
 int cmpmc_amm_psv(const cmpmc_medium* m1, const cmpmc_medium* m2, int incident_P, double theta1, double out[4]);
 double cmpmc_amm_transmission(const cmpmc_medium* substrate, const cmpmc_medium* film, int longitudinal,
                               double cos_theta);
+
+/* ---------------------------------------------------------------- energy partition (host)
+ * PartitionOptions{} defaults (energy_partition.hpp:22-29): Lindhard, Fano fluctuations
+ * and charge cloud on, at most 1000 sampled pairs and 5000 sampled phonons, the
+ * EmpiricalNIELParams{} defaults (niel_models.hpp:36-43). */
+int cmpmc_partition_opts_default(cmpmc_partition_opts* out);
+/* compute_ionization_yield (niel_models.hpp:161-173) for the crystal's target; and
+ * EnergyPartition::lindhard_ionization_yield with a projectile (z1, a1 <= 0: self-recoil,
+ * energy_partition.hpp:55-74). */
+double cmpmc_ionization_yield(int model, double energy_eV, const cmpmc_crystal* crystal,
+                              const cmpmc_empirical_niel* empirical);
+double cmpmc_lindhard_yield(double energy_eV, const cmpmc_crystal* crystal, double z1, double a1);
+/* sample_fano_binomial (niel_models.hpp:203-261) from the xoshiro256** stream
+ * rng_state[4] (advanced in place). */
+uint64_t cmpmc_sample_fano_binomial(double mean, double fano, uint64_t rng_state[4]);
+/* EnergyPartition::partition_deposit (energy_partition.hpp:78-157) on the host: the
+ * deposit's sampled pairs (electron then hole, 2 tracks per pair) into charges and its
+ * sampled prompt phonons into phonons, drawing from rng_state[4] in the reference's
+ * order. Fails with CMPMC_ERR_CAPACITY (and *n_charges / *n_phonons set to the numbers
+ * needed, the stream left unchanged) when a buffer is too small. */
+int cmpmc_partition_deposit(const cmpmc_lattice* lattice, const double pos[3], double time_s, double edep_eV,
+                            int is_nuclear_recoil, const cmpmc_partition_opts* opts, uint64_t rng_state[4],
+                            cmpmc_charge_track* charges, uint64_t charge_cap, uint64_t* n_charges,
+                            cmpmc_phonon_track* phonons, uint64_t phonon_cap, uint64_t* n_phonons,
+                            cmpmc_partition_result* result);
 
 /* CrystalLattice(const CrystalConfig&) table build: crystal_lattice.hpp:45-55, 340-477.
  * oct_lut: 3*256*256*4 floats; cdf_lt, cdf_tt: 1024 doubles each (caller-owned);

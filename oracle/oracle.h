@@ -81,6 +81,7 @@ typedef struct {
     double e_trapping_mfp_m, h_trapping_mfp_m;
     double e_trap_ion_d0_mfp_m, e_trap_ion_a_mfp_m, h_trap_ion_d_mfp_m, h_trap_ion_a0_mfp_m;
     double orient[3][3];
+    double atomic_Z, atomic_A, pair_energy, fano_factor;   /* crystal_config.hpp:46-47, 87-88 */
 } orc_crystal;
 
 typedef struct {
@@ -288,8 +289,40 @@ enum {
     ORC_WL_PHONON_LOGN  = 2,   /* lognormal count, uniform volume, DOS modes, U[e_lo, e_hi] */
     ORC_WL_PAIR         = 3,   /* one e-/h+ pair at a uniform position in the volume */
     ORC_WL_MIXED        = 4,   /* tests/test_parallel.cpp:17-31 MixedGenerator */
-    ORC_WL_PHONON_UNIF  = 5    /* n ~ U{n_lo..n_hi} phonons of (mode, energy) at uniform positions */
+    ORC_WL_PHONON_UNIF  = 5,   /* n ~ U{n_lo..n_hi} phonons of (mode, energy) at uniform positions */
+    ORC_WL_DEPOSIT      = 6    /* a deposit of energy_eV (mode 0 ER, 1 NR) split by partition_deposit */
 };
+
+/* ---- Energy partition: charge/energy_partition.hpp:22-157, niel_models.hpp:28-261 ---- */
+enum { ORC_NIEL_LINDHARD = 0, ORC_NIEL_LEWIN_SMITH = 1, ORC_NIEL_SARKIS = 2, ORC_NIEL_IMPACT_TUNL = 3,
+       ORC_NIEL_EMPIRICAL = 4 };
+typedef struct {
+    int energy_dependent_k;
+    int _pad;
+    double k_fixed, k_low, k_high, e_low_eV, e_high_eV;
+} orc_empirical_niel;
+typedef struct {
+    int niel_model, use_fano_fluctuations, use_charge_cloud, _pad;
+    uint32_t max_sampled_pairs, max_sampled_phonons;
+    orc_empirical_niel empirical;
+} orc_partition_opts;
+typedef struct {
+    double total_edep_eV, ionizing_eV, niel_eV, lindhard_yield;
+    uint64_t n_pairs_true, n_pairs_sampled, n_phonons_true, n_phonons_sampled;
+    double charge_weight, phonon_weight, prompt_phonon_eV;
+} orc_partition_result;
+
+void     orc_partition_opts_default(orc_partition_opts* o);               /* PartitionOptions{} */
+double   orc_ionization_yield(int model, double E, const orc_crystal* c, const orc_empirical_niel* emp);
+double   orc_lindhard_yield(double E, const orc_crystal* c, double z1, double a1);
+uint64_t orc_sample_fano_binomial(double mean, double fano, orc_rng* r);
+double   orc_cloud_radius(uint64_t n_pairs, const orc_crystal* c);
+void     orc_fano_batch(double mean, double fano, orc_rng* r, uint64_t n, uint64_t* out);
+/* EnergyPartition::partition_deposit: charges (e then h per pair) and prompt phonons are
+ * appended to the vectors. */
+void     orc_partition_deposit(const orc_engine* e, const double pos[3], double time, double edep,
+                               int nuclear, const orc_partition_opts* o, orc_rng* r,
+                               orc_charge_vec* charges, orc_track_vec* phonons, orc_partition_result* out);
 
 typedef struct {
     int kind;
@@ -299,7 +332,9 @@ typedef struct {
     double logn_median, logn_sigma;
     uint32_t n_min, n_max;     /* LOGN clip, UNIF range */
     double e_lo_eV, e_hi_eV;   /* LOGN energy range */
-    int pair_at_origin;        /* PAIR: 1 = origin (the reference bench), 0 = uniform volume */
+    int pair_at_origin;        /* PAIR: 1 = origin (the reference bench), 0 = uniform volume;
+                                  DEPOSIT: 1 = at `origin`, 0 = uniform volume */
+    orc_partition_opts partition;  /* DEPOSIT */
 } orc_workload;
 
 typedef struct {

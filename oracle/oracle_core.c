@@ -124,6 +124,7 @@ uint64_t orc_child_key(uint64_t parent_key, uint32_t spawn_no) {
 /* ---------------- materials: crystal_config.hpp:43-257, presets.hpp ---------------- */
 void orc_crystal_germanium(orc_crystal* c) {
     memset(c, 0, sizeof(*c));
+    c->atomic_Z = 32.0; c->atomic_A = 72.63; c->pair_energy = 2.96; c->fano_factor = 0.1057;
     c->density = 5323.0;
     c->C11 = 126.0 * 1e9; c->C12 = 44.0 * 1e9; c->C44 = 67.0 * 1e9;
     c->beta = -73.2 * 1e9; c->gamma = -70.8 * 1e9; c->lambda_ = 37.6 * 1e9; c->mu = 56.1 * 1e9;

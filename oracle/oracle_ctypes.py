@@ -48,6 +48,7 @@ class Crystal(C.Structure):
         ("e_trap_ion_d0_mfp_m", dbl), ("e_trap_ion_a_mfp_m", dbl),
         ("h_trap_ion_d_mfp_m", dbl), ("h_trap_ion_a0_mfp_m", dbl),
         ("orient", (dbl * 3) * 3),
+        ("atomic_Z", dbl), ("atomic_A", dbl), ("pair_energy", dbl), ("fano_factor", dbl),
     ]
 
 
@@ -93,11 +94,31 @@ class Detector(C.Structure):
     ]
 
 
+class EmpiricalNiel(C.Structure):
+    _fields_ = [("energy_dependent_k", C.c_int), ("_pad", C.c_int)] + [(n, dbl) for n in (
+        "k_fixed", "k_low", "k_high", "e_low_eV", "e_high_eV")]
+
+
+class PartitionOpts(C.Structure):
+    _fields_ = [("niel_model", C.c_int), ("use_fano_fluctuations", C.c_int), ("use_charge_cloud", C.c_int),
+                ("_pad", C.c_int), ("max_sampled_pairs", u32), ("max_sampled_phonons", u32),
+                ("empirical", EmpiricalNiel)]
+
+
+class PartitionResult(C.Structure):
+    _fields_ = [(n, dbl) for n in ("total_edep_eV", "ionizing_eV", "niel_eV", "lindhard_yield")] + [
+        (n, u64) for n in ("n_pairs_true", "n_pairs_sampled", "n_phonons_true", "n_phonons_sampled")] + [
+        (n, dbl) for n in ("charge_weight", "phonon_weight", "prompt_phonon_eV")]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class Workload(C.Structure):
     _fields_ = [
         ("kind", C.c_int), ("mode", C.c_int), ("energy_eV", dbl), ("origin", dbl * 3),
         ("logn_median", dbl), ("logn_sigma", dbl), ("n_min", u32), ("n_max", u32),
-        ("e_lo_eV", dbl), ("e_hi_eV", dbl), ("pair_at_origin", C.c_int),
+        ("e_lo_eV", dbl), ("e_hi_eV", dbl), ("pair_at_origin", C.c_int), ("partition", PartitionOpts),
     ]
 
 
@@ -167,7 +188,8 @@ class ChargeVec(C.Structure):
     _fields_ = [("v", C.POINTER(Charge)), ("n", C.c_size_t), ("cap", C.c_size_t)]
 
 
-WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_MIXED, WL_PHONON_UNIF = 1, 2, 3, 4, 5
+WL_PHONON_POINT, WL_PHONON_LOGN, WL_PAIR, WL_MIXED, WL_PHONON_UNIF, WL_DEPOSIT = 1, 2, 3, 4, 5, 6
+NIEL_LINDHARD, NIEL_LEWIN_SMITH, NIEL_SARKIS, NIEL_IMPACT_TUNL, NIEL_EMPIRICAL = 0, 1, 2, 3, 4
 MODE_BENCH_DOS = 3     # POINT workload: mode drawn as benchmarks/cmpmc_bench.cpp:63-65 does
 BRANCH_NAMES = {0: "none", 1: "killed", 2: "isotope", 3: "poly", 4: "decay_TT", 5: "decay_LT",
                 6: "film", 7: "escape", 8: "surf_anh", 9: "specular", 10: "diffuse",
@@ -278,6 +300,19 @@ def lib():
         L.orc_transport_charges.argtypes = [C.c_void_p, C.POINTER(ChargeVec), C.POINTER(TrackVec), C.POINTER(HitVec),
                                             C.POINTER(Stats), C.POINTER(Rng), C.c_int, u64]
         L.orc_crystal_germanium.argtypes = [C.POINTER(Crystal)]
+        L.orc_partition_opts_default.argtypes = [C.POINTER(PartitionOpts)]
+        L.orc_ionization_yield.argtypes = [C.c_int, dbl, C.POINTER(Crystal), C.POINTER(EmpiricalNiel)]
+        L.orc_ionization_yield.restype = dbl
+        L.orc_lindhard_yield.argtypes = [dbl, C.POINTER(Crystal), dbl, dbl]
+        L.orc_lindhard_yield.restype = dbl
+        L.orc_sample_fano_binomial.argtypes = [dbl, dbl, C.c_void_p]
+        L.orc_sample_fano_binomial.restype = u64
+        L.orc_cloud_radius.argtypes = [u64, C.POINTER(Crystal)]
+        L.orc_fano_batch.argtypes = [dbl, dbl, C.c_void_p, u64, C.POINTER(u64)]
+        L.orc_cloud_radius.restype = dbl
+        L.orc_partition_deposit.argtypes = [C.c_void_p, C.POINTER(dbl), dbl, dbl, C.c_int, C.POINTER(PartitionOpts),
+                                            C.c_void_p, C.POINTER(ChargeVec), C.POINTER(TrackVec),
+                                            C.POINTER(PartitionResult)]
         L.orc_detector_default.argtypes = [C.POINTER(Detector)]
         L.orc_film_aluminum.argtypes = [C.POINTER(Film)]
         _lib = L
@@ -286,7 +321,8 @@ def lib():
 
 
 def check_layout(L) -> None:
-    for idx, cls in enumerate([Crystal, Detector, Workload, Hit, Stats, Track, Surface, Film, RunResult, Charge]):
+    for idx, cls in enumerate([Crystal, Detector, Workload, Hit, Stats, Track, Surface, Film, RunResult, Charge,
+                               PartitionOpts, PartitionResult]):
         n = L.orc_sizeof(idx)
         if n != C.sizeof(cls):
             raise RuntimeError(f"oracle struct {cls.__name__}: C {n} B vs ctypes {C.sizeof(cls)} B")
@@ -323,6 +359,13 @@ def ref():
         R.ref_amm.argtypes = [C.POINTER(Medium), C.POINTER(Medium), C.c_int, C.POINTER(dbl), C.POINTER(dbl),
                               C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         R.ref_qet_channels.argtypes = [dbl, dbl, u64, C.c_int, C.POINTER(dbl), C.POINTER(i32)]
+        R.ref_yields.argtypes = [C.c_int, C.c_int, C.POINTER(EmpiricalNiel), C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_lindhard_projectile.argtypes = [C.c_int, C.POINTER(dbl), dbl, dbl, C.POINTER(dbl)]
+        R.ref_fano.argtypes = [dbl, dbl, u64, C.c_int, C.POINTER(u64)]
+        R.ref_partition.argtypes = [dbl, C.c_int, C.POINTER(PartitionOpts), u64, C.POINTER(dbl), C.POINTER(dbl),
+                                    C.c_int, C.POINTER(C.c_int), C.POINTER(dbl), C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(dbl), C.POINTER(u64)]
+        R.ref_partition.restype = C.c_int
         _ref = R
     return _ref
 
@@ -465,7 +508,19 @@ def workload(kind: int, **kw) -> Workload:
     w.e_lo_eV = kw.get("e_lo_eV", 1e-3)
     w.e_hi_eV = kw.get("e_hi_eV", 31e-3)
     w.pair_at_origin = kw.get("pair_at_origin", 0)
+    lib().orc_partition_opts_default(C.byref(w.partition))
+    for k, v in kw.get("partition", {}).items():
+        setattr(w.partition, k, v)
     return w
+
+
+def partition_opts(**kw) -> PartitionOpts:
+    """PartitionOptions{} with the given fields changed."""
+    o = PartitionOpts()
+    lib().orc_partition_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
 
 
 class Engine:
@@ -562,6 +617,38 @@ class Engine:
         for v in (cv.v, pv.v, hv.v):
             libc.free(C.cast(v, C.c_void_p))
         return {"stats": st.as_dict(), "hits": hits, "phonon_energies": ph_e}
+
+    def partition(self, pos, edep_eV: float, nuclear: bool, opts: PartitionOpts | None, rng: "Rng",
+                  time_s: float = 0.0) -> dict:
+        """EnergyPartition::partition_deposit with `rng` (advanced in place): the result
+        record, the charges (type, pos, p, ekin, weight, valley) and the prompt phonons
+        (mode, pos, k, vg, vg_mag, energy, weight) as arrays."""
+        L = lib()
+        o = opts if opts is not None else partition_opts()
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        cv, pv, res = ChargeVec(), TrackVec(), PartitionResult()
+        L.orc_partition_deposit(self.ptr, dptr(p), time_s, edep_eV, int(nuclear), C.byref(o), C.byref(rng),
+                                C.byref(cv), C.byref(pv), C.byref(res))
+        q = [cv.v[i] for i in range(cv.n)]
+        t = [pv.v[i] for i in range(pv.n)]
+        out = {"result": res.as_dict(),
+               "charge_type": np.array([c.type for c in q], dtype=np.int32),
+               "charge_pos": np.array([list(c.pos) for c in q]).reshape(-1, 3),
+               "charge_p": np.array([list(c.p) for c in q]).reshape(-1, 3),
+               "charge_ekin": np.array([c.ekin for c in q]),
+               "charge_weight": np.array([c.weight for c in q]),
+               "charge_valley": np.array([c.valley for c in q], dtype=np.int32),
+               "phonon_mode": np.array([x.mode for x in t], dtype=np.int32),
+               "phonon_k": np.array([list(x.k) for x in t]).reshape(-1, 3),
+               "phonon_vg": np.array([list(x.vg) for x in t]).reshape(-1, 3),
+               "phonon_vg_mag": np.array([x.vg_mag for x in t]),
+               "phonon_energy": np.array([x.energy for x in t]),
+               "phonon_weight": np.array([x.weight for x in t])}
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        for v in (cv.v, pv.v):
+            libc.free(C.cast(v, C.c_void_p))
+        return out
 
     def run(self, wl: Workload, n_events: int, seed: int, threads: int = 0, chunk: int = 16,
             keep_hits: bool = True, batched: bool = False, first_branch: bool = False) -> dict:

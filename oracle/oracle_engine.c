@@ -794,6 +794,25 @@ size_t orc_generate_event(const orc_engine* e, const orc_workload* wl, uint64_t 
                 charges->v[i].key = 0x8000000000000000ULL | (uint64_t)(i - n0);
                 orc_rng32_for_track(&charges->v[i].rng, base, charges->v[i].key);
             }
+    } else if (kind == ORC_WL_DEPOSIT) {
+        /* a generator calling EnergyPartition::partition_deposit with the event stream:
+         * position (origin, or uniform in the volume), then the partition's draws */
+        double pos[3] = {origin[0], origin[1], origin[2]};
+        if (!wl->pair_at_origin) uniform_volume(D, ev_rng, pos);
+        const size_t q0 = charges->n, p0 = phonons->n;
+        orc_partition_deposit(e, pos, 0.0, energy, mode == 1, &wl->partition, ev_rng, charges, phonons, NULL);
+        *injected += energy;
+        const size_t np = phonons->n - p0;
+        for (size_t j = 0; j < np; ++j) {
+            orc_track* t = &phonons->v[p0 + j];
+            if (batched) { t->key = j; orc_rng32_for_track(&t->rng, base, j); }
+            t->first_flag = batched ? (j == 0) : (j + 1 == np);
+        }
+        if (batched)
+            for (size_t i = q0; i < charges->n; ++i) {
+                charges->v[i].key = 0x8000000000000000ULL | (uint64_t)(i - q0);
+                orc_rng32_for_track(&charges->v[i].rng, base, charges->v[i].key);
+            }
     }
     return phonons->n;
 }
@@ -957,6 +976,8 @@ size_t orc_sizeof(int which) {
         case 7: return sizeof(orc_film);
         case 8: return sizeof(orc_run_result);
         case 9: return sizeof(orc_charge);
+        case 10: return sizeof(orc_partition_opts);
+        case 11: return sizeof(orc_partition_result);
         default: return 0;
     }
 }

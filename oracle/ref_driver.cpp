@@ -7,6 +7,7 @@
 // oracle/oracle.h describe the workload and the detector; this file converts them into
 // the reference's DetectorModel / generator lambdas.
 #include "cmpmc/cmpmc.hpp"
+#include "cmpmc/charge/energy_partition.hpp"
 #include "oracle.h"
 
 #include <cstring>
@@ -85,6 +86,25 @@ DetectorModel to_detector(const orc_detector* d) {
                                            : DetectorModel::ChargeStepping::FirstOrderDkCap;
     m.charge_sampling_dk_frac = d->charge_sampling_dk_frac;
     return m;
+}
+
+physics::EmpiricalNIELParams to_empirical(const orc_empirical_niel& e) {
+    physics::EmpiricalNIELParams p;
+    p.energy_dependent_k = e.energy_dependent_k != 0;
+    p.k_fixed = e.k_fixed; p.k_low = e.k_low; p.k_high = e.k_high;
+    p.e_low_eV = e.e_low_eV; p.e_high_eV = e.e_high_eV;
+    return p;
+}
+
+PartitionOptions to_partition(const orc_partition_opts& o) {
+    PartitionOptions p;
+    p.niel_model = static_cast<physics::NIELModel>(o.niel_model);
+    p.empirical_niel = to_empirical(o.empirical);
+    p.use_fano_fluctuations = o.use_fano_fluctuations != 0;
+    p.use_charge_cloud = o.use_charge_cloud != 0;
+    p.max_sampled_pairs = o.max_sampled_pairs;
+    p.max_sampled_phonons = o.max_sampled_phonons;
+    return p;
 }
 
 CrystalConfig to_crystal(const orc_crystal* c) {
@@ -265,6 +285,8 @@ long ref_run_events(const orc_crystal* crystal, const orc_detector* det, const o
         const ChargePhysics charges(lattice);
         const size_t n_valleys = lattice.config().valley_dirs.size();
         const orc_workload w = *wl;
+        const EnergyPartition partitioner(lattice);
+        const PartitionOptions popts = to_partition(w.partition);
         auto pair = [&](EventInput& in, const Vec3& pos) {
             const auto v = static_cast<uint16_t>(in.rng.uniform() * n_valleys) % n_valleys;
             in.charges.push_back(charges.make_charge(ExcitationType::Electron, pos, in.rng.isotropic_dir(), 1e-6, v));
@@ -320,6 +342,15 @@ long ref_run_events(const orc_crystal* crystal, const orc_detector* det, const o
                         in.phonons.push_back(engine.make_track(pos, in.rng.isotropic_dir(), m, E));
                         in.injected_eV += E;
                     }
+                    break;
+                }
+                case ORC_WL_DEPOSIT: {
+                    // a generator that partitions the deposit with the event stream
+                    Vec3 pos{w.origin[0], w.origin[1], w.origin[2]};
+                    if (!w.pair_at_origin) pos = uniform_volume(detector, in.rng);
+                    partitioner.partition_deposit(pos, 0.0, w.energy_eV, w.mode == 1, in.charges, in.phonons, in.rng,
+                                                  popts);
+                    in.injected_eV += w.energy_eV;
                     break;
                 }
                 default:
@@ -396,6 +427,65 @@ long ref_transport_stack(const orc_detector* det, int n, const double* rows /* n
     for (long i = 0; i < m; ++i) to_hit(out[static_cast<size_t>(i)], &hits[i]);
     for (int i = 0; i < 4; ++i) rng_state_out[i] = rng.s[i];
     return static_cast<long>(out.size());
+}
+
+// compute_ionization_yield of the reference for n energies: crystal 0 Ge, 1 Si presets.
+void ref_yields(int crystal, int model, const orc_empirical_niel* emp, int n, const double* E, double* out) {
+    const CrystalConfig cfg = crystal == 1 ? materials::Silicon() : materials::Germanium();
+    const physics::EmpiricalNIELParams p = to_empirical(*emp);
+    for (int i = 0; i < n; ++i) out[i] = physics::compute_ionization_yield(static_cast<physics::NIELModel>(model), E[i], cfg, p);
+}
+
+// EnergyPartition::lindhard_ionization_yield with a projectile (z1, a1) in Ge.
+void ref_lindhard_projectile(int n, const double* E, double z1, double a1, double* out) {
+    const CrystalLattice lattice(materials::Germanium());
+    const EnergyPartition part(lattice);
+    for (int i = 0; i < n; ++i) out[i] = part.lindhard_ionization_yield(E[i], z1, a1);
+}
+
+// n draws of sample_fano_binomial(mean, fano) from Rng(seed).
+void ref_fano(double mean, double fano, uint64_t seed, int n, uint64_t* out) {
+    Rng rng(seed);
+    for (int i = 0; i < n; ++i) out[i] = physics::sample_fano_binomial(mean, fano, rng);
+}
+
+// EnergyPartition::partition_deposit in Ge at pos with Rng(seed). result[11] = the
+// Result fields in declaration order (counts as doubles); charges rows of 11 (type,
+// pos3, p3, ekin, weight, valley, time), phonons rows of 14 (mode, pos3, k3, vg3, vg_mag,
+// energy, weight, time). Returns -1 when a capacity is too small.
+int ref_partition(double E, int nuclear, const orc_partition_opts* o, uint64_t seed, const double* pos,
+                  double* result, int q_cap, int* n_q, double* q, int p_cap, int* n_p, double* ph,
+                  uint64_t* rng_state_out) {
+    const CrystalLattice lattice(materials::Germanium());
+    const EnergyPartition part(lattice);
+    std::vector<ChargeTrack> qs;
+    std::vector<PhononTrack> ps;
+    Rng rng(seed);
+    const auto r = part.partition_deposit(Vec3{pos[0], pos[1], pos[2]}, 0.0, E, nuclear != 0, qs, ps, rng,
+                                          to_partition(*o));
+    const double rv[11] = {r.total_edep_eV, r.ionizing_eV, r.niel_eV, r.lindhard_yield,
+                           static_cast<double>(r.n_pairs_true), static_cast<double>(r.n_pairs_sampled),
+                           static_cast<double>(r.n_phonons_true), static_cast<double>(r.n_phonons_sampled),
+                           r.charge_weight, r.phonon_weight, r.prompt_phonon_eV};
+    std::memcpy(result, rv, sizeof(rv));
+    *n_q = static_cast<int>(qs.size());
+    *n_p = static_cast<int>(ps.size());
+    for (int i = 0; i < 4; ++i) rng_state_out[i] = rng.s[i];
+    if (*n_q > q_cap || *n_p > p_cap) return -1;
+    for (int i = 0; i < *n_q; ++i) {
+        const ChargeTrack& c = qs[static_cast<size_t>(i)];
+        const double row[11] = {static_cast<double>(c.type), c.pos.x, c.pos.y, c.pos.z, c.p_eV.x, c.p_eV.y, c.p_eV.z,
+                                c.ekin, c.weight, static_cast<double>(c.valley), c.time};
+        std::memcpy(q + 11 * i, row, sizeof(row));
+    }
+    for (int i = 0; i < *n_p; ++i) {
+        const PhononTrack& t = ps[static_cast<size_t>(i)];
+        const double row[14] = {static_cast<double>(static_cast<int>(t.mode)), t.pos.x, t.pos.y, t.pos.z,
+                                t.k_dir.x, t.k_dir.y, t.k_dir.z, t.vg_dir.x, t.vg_dir.y, t.vg_dir.z,
+                                t.vg_mag, t.energy, t.weight, t.time};
+        std::memcpy(ph + 14 * i, row, sizeof(row));
+    }
+    return 0;
 }
 
 }  // extern "C"

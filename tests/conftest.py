@@ -25,3 +25,10 @@ def lib_cmpmc():
     import cmpmc_amd
     cmpmc_amd.capi.load()
     return cmpmc_amd
+
+
+@pytest.fixture(scope="session")
+def gold():
+    """The golden vectors written by the reference's own build (tests/golden/make_golden.py)."""
+    import numpy as np
+    return np.load(os.path.join(ROOT, "tests", "golden", "reference_vectors.npz"))

@@ -5,6 +5,7 @@ Run in a container that has the reference tree:  python tests/golden/make_golden
 The fixtures are data only (inputs and the reference's outputs); they pin the CPU
 oracle where the reference build is absent (for example on a GPU box).
 """
+import ctypes as C
 import os
 import sys
 
@@ -149,6 +150,16 @@ def main():
     dm.luke_sample_fraction = 0.0
     runs["mask_pairs"] = (O.workload(O.WL_PAIR), 6, 42, dm)
     runs["c1_exact"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector())
+    # energy partition (charge/energy_partition.hpp, niel_models.hpp): deposits partitioned
+    # into pairs and prompt phonons, transported at 4 V with 5 % of the Luke phonons
+    dp = O.default_detector()
+    O.set_bias(dp, 4.0)
+    dp.luke_sample_fraction = 0.05
+    runs["deposit_nr"] = (O.workload(O.WL_DEPOSIT, mode=1, energy_eV=100.0,
+                                     partition=dict(max_sampled_phonons=12, max_sampled_pairs=8)), 4, 42, dp)
+    runs["deposit_er"] = (O.workload(O.WL_DEPOSIT, mode=0, energy_eV=30.0, origin=(1e-3, -2e-3, 4e-3), pair_at_origin=1,
+                                     partition=dict(max_sampled_phonons=10, max_sampled_pairs=4)), 3, 7, dp)
+    partition_goldens(R, out)
     for tag, spec in runs.items():
         wl, n, seed, d = spec[:4]
         cr = spec[4] if len(spec) > 4 else None
@@ -164,6 +175,77 @@ def main():
     path = os.path.join(HERE, "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")
+
+
+PARTITION_CASES = [
+    # (deposit eV, nuclear, option changes, seed, position)
+    (50.0, 0, {}, 1, (0.0, 0.0, 0.0)),
+    (1000.0, 0, {}, 2, (1e-3, -2e-3, 5e-3)),
+    (100.0, 1, {}, 3, (0.0, 0.0, 0.0)),
+    (1000.0, 1, {"niel_model": 1}, 4, (0.0, 0.0, 0.0)),
+    (500.0, 1, {"niel_model": 4}, 5, (0.0, 0.0, 0.0)),
+    (150.0, 1, {"niel_model": 4, "empirical.energy_dependent_k": 0}, 6, (0.0, 0.0, 0.0)),
+    (200.0, 0, {"use_fano_fluctuations": 0}, 7, (0.0, 0.0, 0.0)),
+    (200.0, 0, {"use_charge_cloud": 0}, 8, (2e-3, 0.0, 0.0)),
+    (5000.0, 0, {"max_sampled_pairs": 7, "max_sampled_phonons": 11}, 9, (0.0, 0.0, 0.0)),
+    (2.0, 0, {}, 10, (0.0, 0.0, 0.0)),
+    (0.5, 1, {}, 11, (0.0, 0.0, 0.0)),
+]
+FANO_CASES = [(0.5, 0.1057), (3.5, 0.119), (7.3, 0.15), (20.0, 0.30), (100.0, 0.119), (1e4, 0.119),
+              (5.0, 1.0), (50.0, 1.0), (4.0, 0.5), (2.0, 0.1057), (12.0, 0.1057)]
+
+
+def partition_opts(changes):
+    o = O.partition_opts()
+    for k, v in changes.items():
+        if k.startswith("empirical."):
+            setattr(o.empirical, k.split(".", 1)[1], v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def partition_goldens(R, out):
+    E = np.concatenate([np.geomspace(0.3, 3e6, 60), [47.8327, 47.9, 9999.0, 10000.0, 10001.0, 2000.0, 2001.0, 100.0]])
+    out["yield_E"] = E
+    emp = O.partition_opts().empirical
+    ys = np.zeros((2, 5, len(E)))
+    for c in range(2):
+        for m in range(5):
+            y = np.zeros(len(E))
+            R.ref_yields(c, m, emp, len(E), O.dptr(E), O.dptr(y))
+            ys[c, m] = y
+    out["yield_values"] = ys                     # [Ge, Si][model][E]
+    emp2 = O.partition_opts().empirical
+    emp2.energy_dependent_k = 0
+    y = np.zeros(len(E))
+    R.ref_yields(0, 4, emp2, len(E), O.dptr(E), O.dptr(y))
+    out["yield_empirical_fixed_ge"] = y
+    lp = np.zeros(len(E))
+    R.ref_lindhard_projectile(len(E), O.dptr(E), 14.0, 28.085, O.dptr(lp))
+    out["yield_lindhard_si_in_ge"] = lp
+    fz = np.zeros((len(FANO_CASES), 200), dtype=np.uint64)
+    for i, (mean, fano) in enumerate(FANO_CASES):
+        a = (O.u64 * 200)()
+        R.ref_fano(mean, fano, 100 + i, 200, a)
+        fz[i] = list(a)
+    out["fano_cases"] = np.array(FANO_CASES)
+    out["fano_values"] = fz
+    for i, (e, nr, ch, seed, pos) in enumerate(PARTITION_CASES):
+        o = partition_opts(ch)
+        res = np.zeros(11)
+        nq, nph = C.c_int(), C.c_int()
+        q = np.zeros(11 * 2100)
+        ph = np.zeros(14 * 5100)
+        st = (O.u64 * 4)()
+        p = np.array(pos, dtype=float)
+        rc = R.ref_partition(e, nr, o, seed, O.dptr(p), O.dptr(res), 2100, C.byref(nq), O.dptr(q), 5100,
+                             C.byref(nph), O.dptr(ph), st)
+        assert rc == 0
+        out[f"part{i}_result"] = res
+        out[f"part{i}_charges"] = q[:11 * nq.value].reshape(-1, 11)
+        out[f"part{i}_phonons"] = ph[:14 * nph.value].reshape(-1, 14)
+        out[f"part{i}_rng"] = np.array(list(st), dtype=np.uint64)
 
 
 if __name__ == "__main__":

@@ -167,3 +167,56 @@ def test_host_masks_equal_reference(lib_cmpmc, oracle):
     assert [(p.cx, p.cy) for p in grid] == pytest.approx([(-0.01, -0.01), (-0.01, 0.01), (0.0, -0.01), (0.0, 0.01),
                                                            (0.01, -0.01), (0.01, 0.01)])
     assert cm.find_channel(grid, 0.0031, 0.0149) == 3 and cm.find_channel(grid, 0.0041, 0.01) == -1
+
+
+def test_host_partition_equals_reference(lib_cmpmc, gold):
+    """The library's energy partition on the host (cmpmc_ionization_yield,
+    cmpmc_sample_fano_binomial, cmpmc_partition_deposit: the shared core the device
+    generator runs) equals the reference's yields, Fano draws and partitioned tracks
+    bit for bit (golden vectors written by the reference)."""
+    cm = lib_cmpmc
+    E = gold["yield_E"]
+    si = cm.germanium()
+    si.atomic_Z, si.atomic_A = 14.0, 28.085
+    for ci, cr in enumerate((cm.germanium(), si)):
+        for m in range(5):
+            got = np.array([cm.ionization_yield(m, e, cr) for e in E])
+            assert np.array_equal(got, gold["yield_values"][ci, m]), f"crystal {ci} model {m}"
+    fixed = cm.partition_opts(empirical_energy_dependent_k=0).empirical
+    assert np.array_equal(np.array([cm.ionization_yield(4, e, None, fixed) for e in E]), gold["yield_empirical_fixed_ge"])
+    assert np.array_equal(np.array([cm.lindhard_yield(e, None, 14.0, 28.085) for e in E]), gold["yield_lindhard_si_in_ge"])
+    for i, (mean, fano) in enumerate(gold["fano_cases"]):
+        r = cm.Rng(100 + i)
+        got = [cm.sample_fano_binomial(mean, fano, r) for _ in range(200)]
+        assert np.array_equal(np.array(got, dtype=np.uint64), gold["fano_values"][i]), (mean, fano)
+    lat = cm.Lattice()
+    from test_oracle_partition import PARTITION_CASES, RESULT_KEYS
+    for i, (e, nr, ch, seed, pos) in enumerate(PARTITION_CASES):
+        o = cm.partition_opts(**{k.replace("empirical.", "empirical_"): v for k, v in ch.items()})
+        r = cm.Rng(seed)
+        out = cm.partition_deposit(lat, pos, e, bool(nr), r, o)
+        res = out["result"]
+        assert np.array_equal(np.array([float(res[k]) for k in RESULT_KEYS]), gold[f"part{i}_result"]), i
+        q = out["charges"]
+        got_q = np.c_[q["type"], q["pos"], q["p_eV"], q["ekin"], q["weight"], q["valley"], q["time"]] \
+            if len(q) else np.zeros((0, 11))
+        assert np.array_equal(got_q, gold[f"part{i}_charges"]), i
+        p = out["phonons"]
+        got_p = np.c_[p["mode"], p["pos"], p["k_dir"], p["vg_dir"], p["vg_mag"], p["energy"], p["weight"], p["time"]] \
+            if len(p) else np.zeros((0, 14))
+        assert np.array_equal(got_p, gold[f"part{i}_phonons"]), i
+        assert np.array_equal(np.array(r.s, dtype=np.uint64), gold[f"part{i}_rng"]), i
+
+
+def test_host_partition_capacity_error_keeps_stream(lib_cmpmc):
+    cm = lib_cmpmc
+    L = cm.capi.load()
+    lat = cm.Lattice()
+    o = cm.partition_opts()
+    st = (C.c_uint64 * 4)(1, 2, 3, 4)
+    nq, nph = C.c_uint64(), C.c_uint64()
+    pos = (C.c_double * 3)(0.0, 0.0, 0.0)
+    rc = L.cmpmc_partition_deposit(C.byref(lat.c), pos, 0.0, 1000.0, 0, C.byref(o), st, None, 0, C.byref(nq), None, 0,
+                                   C.byref(nph), None)
+    assert rc == cm.capi.ERR_CAPACITY and nq.value > 0 and nph.value > 0
+    assert list(st) == [1, 2, 3, 4]
