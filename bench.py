@@ -23,6 +23,10 @@ luke_*       the same pairs (trapping off) with every Luke phonon enqueued on th
              and tracked to absorption: 2,000 events batched (the plain form on a slice
              when --luke-plain-events is given: its one-event-per-lane time is set by the
              slowest events, ~50-80 s)
+deposit      the reference's end-to-end recoil demo: 100 eV nuclear recoils at the centre,
+             partitioned on the device (EnergyPartition: Lindhard yield, Fano pair number,
+             prompt phonons down-sampled to 1,000 tracks), drifted at 4 V with 5 % of the
+             Luke phonons tracked: 2,000 events batched (events/s)
 lognormal_*  the lognormal workload (phonon count per event lognormal(200, 1) clipped to
              [1, 20000], uniform positions, DOS modes, E ~ U[1, 31] meV) on a bounded
              slice of events, batched FP32 (and the plain kernel on a smaller slice when
@@ -38,7 +42,7 @@ roofline     the binding resource of the batched phonon kernel, VALU issue: achi
 cpu_baseline the reference implementation compiled in place (oracle/_ref) or else the
              CPU oracle, timed on the host: all cores the process may use (the box's CPU
              share) and one thread, on bounded samples of the headline workload; the
-             same beside the lognormal, charge-drift and Luke-cascade lines
+             same beside the lognormal, charge-drift, Luke-cascade and deposit lines
 """
 from __future__ import annotations
 
@@ -55,6 +59,10 @@ METRIC = "phonon events/s (7.5 meV Ge), batched vs plain GPU; grad/fwd cost rati
 BYTES_PER_STEP_FP32 = 200.0
 BYTES_PER_STEP_FP64 = 320.0
 HBM_PEAK = 8.0e12
+# the reference's end-to-end recoil demo (examples/phonon_benchmark.cpp:170-203): a 100 eV
+# nuclear recoil partitioned by EnergyPartition (prompt phonons down-sampled to 1,000
+# tracks), drifted at 4 V with 5 % of the Luke phonons tracked; here at the detector's centre
+DEPOSIT_WL = dict(mode=1, energy_eV=100.0, pair_at_origin=1, partition=dict(max_sampled_phonons=1000))
 # VALU issue of a CDNA compute unit: 4 SIMDs, one wave64 VALU instruction per 4 cycles each
 VALU_WAVE_INSTS_PER_CU_CYCLE = 1.0
 
@@ -93,7 +101,8 @@ def cpu_leg(wl_kind: str, n_events: int, seed: int, threads: int, crystal=None, 
     import oracle_ctypes as O
     wl = {"c1": lambda: O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3),
           "lognormal": lambda: O.workload(O.WL_PHONON_LOGN),
-          "pair": lambda: O.workload(O.WL_PAIR)}[wl_kind]()
+          "pair": lambda: O.workload(O.WL_PAIR),
+          "deposit": lambda: O.workload(O.WL_DEPOSIT, **DEPOSIT_WL)}[wl_kind]()
     kind = "reference" if O.ref_available() else "port"
     chunk = 16 if wl_kind == "c1" else 1
     t = time.perf_counter()
@@ -171,6 +180,7 @@ def main():
     ap.add_argument("--charge-events", type=int, default=100_000)
     ap.add_argument("--luke-events", type=int, default=2_000)
     ap.add_argument("--luke-plain-events", type=int, default=0)
+    ap.add_argument("--deposit-events", type=int, default=2_000)
     ap.add_argument("--cpu-events", type=int, default=100_000, help="headline CPU leg, all cores")
     ap.add_argument("--cpu-events-1t", type=int, default=5_000, help="headline CPU leg, one thread")
     ap.add_argument("--seed", type=int, default=42)
@@ -345,6 +355,25 @@ def main():
                                     "deposited_eV_per_event": sbl["total_deposited_eV"] / args.luke_events}
             _log(f"Luke cascade, Leman: batched {tbl:.3f} s")
 
+    if args.deposit_events > 0:
+        dd = cm.default_detector()
+        cm.set_bias(dd, 4.0)
+        dd.luke_sample_fraction = 0.05
+        ed = cm.Engine(detector=dd)
+        dw = cm.workload(cm.WL_DEPOSIT, **DEPOSIT_WL)
+        ed.run_workload(dw, 20, seed=1, keep_hits=False, form=cm.FORM_BATCHED)      # warm
+        t = time.perf_counter()
+        rd = ed.run_workload(dw, args.deposit_events, seed=args.seed, keep_hits=False, form=cm.FORM_BATCHED)
+        tdw = time.perf_counter() - t
+        sd = rd["stats"]
+        out["deposit"] = {"events": args.deposit_events, "batched_events_per_s": args.deposit_events / tdw,
+                          "phonon_steps_per_s": sd["total_steps"] / tdw, "charge_steps": sd["charge_steps"],
+                          "pairs_per_event": sd["charges_collected"] / (2.0 * args.deposit_events),
+                          "deposited_eV_per_event": sd["total_deposited_eV"] / args.deposit_events,
+                          "kernel_s": rd["kernel_ms"] / 1e3}
+        _log(f"deposit (100 eV NR at 4 V): {args.deposit_events} events in {tdw:.2f} s, "
+             f"{sd['total_steps']:.3e} phonon steps")
+
     if not args.no_lognormal and args.lognormal_events > 0:
         lw = cm.workload(cm.WL_PHONON_LOGN)
         t = time.perf_counter()
@@ -375,6 +404,13 @@ def main():
                 out["charge"]["cpu"] = cpu_pair("pair", 4000, 400, args.seed, host, crystal=c3, detector=d3)
             if "luke" in out:
                 out["luke"]["cpu"] = cpu_pair("pair", 64, 4, args.seed, host, crystal=c4, detector=d4)
+            if "deposit" in out:
+                sys.path.insert(0, os.path.join(ROOT, "oracle"))
+                import oracle_ctypes as O
+                od = O.default_detector()
+                O.set_bias(od, 4.0)
+                od.luke_sample_fraction = 0.05
+                out["deposit"]["cpu"] = cpu_pair("deposit", 32, 2, args.seed, host, detector=od)
         except Exception as e:  # the CPU leg must not hide the GPU result
             out["cpu_baseline"] = {"error": repr(e)}
     print(json.dumps(out), flush=True)

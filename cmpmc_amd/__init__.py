@@ -312,12 +312,17 @@ def partition_deposit(lattice: "Lattice", pos, edep_eV: float, nuclear: bool, rn
     p = np.ascontiguousarray(pos, dtype=np.float64)
     st = (C.c_uint64 * 4)(*rng.s)
     nq, nph, res = C.c_uint64(), C.c_uint64(), capi.PartitionResult()
-    qcap, pcap = 2 * int(o.max_sampled_pairs), int(o.max_sampled_phonons)
-    q = np.zeros(max(qcap, 1), dtype=CHARGE_DTYPE)
-    ph = np.zeros(max(pcap, 1), dtype=PHONON_DTYPE)
-    check(L.cmpmc_partition_deposit(C.byref(lattice.c), p.ctypes.data_as(C.POINTER(C.c_double)), time_s, edep_eV,
-                                    int(nuclear), C.byref(o), st, q.ctypes.data, qcap, C.byref(nq), ph.ctypes.data,
-                                    pcap, C.byref(nph), C.byref(res)))
+    pp = p.ctypes.data_as(C.POINTER(C.c_double))
+    # the first call sizes the buffers (a capacity failure leaves the stream untouched)
+    rc = L.cmpmc_partition_deposit(C.byref(lattice.c), pp, time_s, edep_eV, int(nuclear), C.byref(o), st, None, 0,
+                                   C.byref(nq), None, 0, C.byref(nph), C.byref(res))
+    q = np.zeros(max(nq.value, 1), dtype=CHARGE_DTYPE)
+    ph = np.zeros(max(nph.value, 1), dtype=PHONON_DTYPE)
+    if rc == capi.ERR_CAPACITY:
+        rc = L.cmpmc_partition_deposit(C.byref(lattice.c), pp, time_s, edep_eV, int(nuclear), C.byref(o), st,
+                                       q.ctypes.data, nq.value, C.byref(nq), ph.ctypes.data, nph.value, C.byref(nph),
+                                       C.byref(res))
+    check(rc)
     rng.s = [int(v) for v in st]
     return {"result": res.as_dict(), "charges": q[:nq.value].copy(), "phonons": ph[:nph.value].copy()}
 
@@ -445,6 +450,13 @@ class Engine:
         elif wl.kind == WL_PHONON_LOGN:
             mean = wl.logn_median * math.exp(0.5 * wl.logn_sigma ** 2)
             n = per_phonon * min(mean, float(wl.n_max))
+        elif wl.kind == WL_DEPOSIT:
+            # prompt phonons near the Debye energy (~36 meV in Ge), both carriers of each pair
+            # and, when Luke phonons are tracked, about 4 eV of them per pair
+            pairs = min(float(wl.partition.max_sampled_pairs), wl.energy_eV / 2.96 + 1.0)
+            phon = min(float(wl.partition.max_sampled_phonons), wl.energy_eV / 0.036 + 1.0)
+            luke = 1.5 * 4.0 / 3.0e-4 * min(max(self.detector.luke_sample_fraction, 0.0), 1.0)
+            n = 2 * pairs * (1.0 + luke) + phon * (1.5 * 0.036 / 3.0e-4 + 8)
         else:
             n = 4 + (1.5 * 4.0 / 3.0e-4 * min(max(self.detector.luke_sample_fraction, 0.0), 1.0))
         return int(min(max(1024, n * n_events), 1 << 27))

@@ -239,21 +239,28 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
         b.film = DevFilm{f.film_absorption, f.film_thickness, f.v_sound, f.gap_energy, f.phonon_lifetime,
                          f.phonon_lifetime_slope, f.low_qp_limit, f.high_qp_limit, f.subgap_absorption, f.absorber_gap,
                          f.absorber_eff, f.absorber_eff_slope, f.temperature, f.keep_all_phonons, 0};
-        b.n_pads = a.n_pads;
-        b.pad_off = pad_off;
+        DevSurfX& x = m.sx[s];
+        x.n_pads = a.n_pads;
+        x.pad_off = pad_off;
         pad_off += a.n_pads;
-        b.has_amm = a.has_amm;
-        b.amm_sub[0] = a.amm_substrate.rho; b.amm_sub[1] = a.amm_substrate.vL; b.amm_sub[2] = a.amm_substrate.vT;
-        b.amm_film[0] = a.amm_film.rho; b.amm_film[1] = a.amm_film.vL; b.amm_film[2] = a.amm_film.vT;
+        x.has_amm = a.has_amm;
+        x._pad = 0;
+        x.amm_sub[0] = a.amm_substrate.rho; x.amm_sub[1] = a.amm_substrate.vL; x.amm_sub[2] = a.amm_substrate.vT;
+        x.amm_film[0] = a.amm_film.rho; x.amm_film[1] = a.amm_film.vL; x.amm_film[2] = a.amm_film.vT;
     }
     m.pads = M->d_pads;
     m.amm_tab = M->d_amm;
+    m.surf_x = 0;
+    for (int s = 0; s < 3; ++s)
+        if (m.sx[s].n_pads > 0 || m.sx[s].has_amm) m.surf_x = 1;
     m.max_bounces = det->max_bounces; m.rr_bounces = det->russian_roulette_bounces;
     m.min_phonon_eV = det->min_phonon_eV; m.luke_fraction = det->luke_sample_fraction;
     m.charge_stepping = det->charge_stepping;
     m.dk_frac = det->charge_stepping == 1 ? det->charge_sampling_dk_frac : det->charge_max_dk_frac;
     m.dt_cap = det->charge_dt_cap_s;
     m.fermi = det->fermi_level_above_Ev_eV; m.rr_survival = det->russian_roulette_survival;
+    m.atomic_Z = c.atomic_Z; m.atomic_A = c.atomic_A; m.pair_energy = c.pair_energy; m.fano = c.fano_factor;
+    m.debye_eV = c.debye_freq * kHPlanck;
     fill_fp32_constants(m);
 }
 
@@ -377,11 +384,12 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
 }
 
 // The model's cached run buffers.
-static std::array<DevBuf*, 32> run_buffers(cmpmc_model* M) {
+constexpr size_t kRunBuffers = 34;
+static std::array<DevBuf*, kRunBuffers> run_buffers(cmpmc_model* M) {
     return {&M->ws, &M->pst, &M->cst, &M->poff, &M->coff, &M->pinit, &M->cinit, &M->rng, &M->inj, &M->tin,
             &M->cin, &M->tin_off, &M->cin_off, &M->hits, &M->hits_tmp, &M->keys_a, &M->keys_b, &M->idx_a,
             &M->idx_b, &M->sorted, &M->counts, &M->fb, &M->acc, &M->queue, &M->cqueue, &M->lstack,
-            &M->clstack, &M->qoff, &M->wl, &M->cub_tmp, &M->fjobs, &M->scan_part};
+            &M->clstack, &M->qoff, &M->wl, &M->cub_tmp, &M->fjobs, &M->scan_part, &M->ccounts, &M->cqoff};
 }
 
 static void trim(cmpmc_model* M) {
@@ -605,6 +613,14 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
         if (J.wl->kind == CMPMC_WL_PAIR) {
             std::fill(nph.begin(), nph.end(), 0u);
             std::fill(nch.begin(), nch.end(), 2u);
+        } else if (J.wl->kind == CMPMC_WL_DEPOSIT) {
+            CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * sizeof(uint32_t)));
+            CHK(ensure(M->ccounts, std::max<uint64_t>(N, 1) * sizeof(uint32_t)));
+            HIPCHK(cmpmc_launch_deposit_count(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
+                                              as<uint32_t>(M->counts), as<uint32_t>(M->ccounts), s));
+            HIPCHK(hipMemcpyAsync(nph.data(), M->counts.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(nch.data(), M->ccounts.p, N * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
         } else {
             CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * sizeof(uint32_t)));
             HIPCHK(cmpmc_launch_workload_count(as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
@@ -630,6 +646,9 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
                         heaviest, kPlainMaxPhonons);
     }
     const bool any_charges = std::any_of(nch.begin(), nch.end(), [](uint32_t v) { return v > 0; });
+    // deposits are generated by their own pass; the other workloads on the event's lane
+    const bool dep = J.wl && J.wl->kind == CMPMC_WL_DEPOSIT;
+    const bool lane_gen = J.wl && !dep;
     uint64_t headroom = C.headroom ? C.headroom : (any_charges && M->dm.luke_fraction > 0.0 ? 8192 : 128);
     uint64_t c_headroom = 16;   // knock-on carriers per event beyond the initial ones
     const uint64_t max_lanes = 262144;
@@ -707,6 +726,16 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
                     HIPCHK(hipMemcpyAsync(M->inj.p, J.injected + c0, n * 8, hipMemcpyHostToDevice, s));
                 }
             }
+            if (dep) {
+                // the chunk's partitions into the stack regions (then run as host-built stacks)
+                CHK(ensure(M->rng, n * 32));
+                CHK(ensure(M->inj, n * 8));
+                HIPCHK(cmpmc_launch_deposit_stacks(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event + c0, n,
+                                                   as<cmpmc_phonon_track>(M->pst), as<uint64_t>(M->poff),
+                                                   ch ? as<cmpmc_charge_track>(M->cst) : nullptr,
+                                                   ch ? as<uint64_t>(M->coff) : nullptr, as<uint64_t>(M->rng),
+                                                   as<double>(M->inj), s));
+            }
             CHK(ensure(M->ws, static_cast<size_t>(5) * CMPMC_KCAP * n * sizeof(double)));
             if (C.want_first_branch || J.skip_phonons) CHK(ensure(M->fb, n * 4 + 16));
             if (J.skip_phonons) CHK(ensure(M->counts, n * 4 + 16));
@@ -722,13 +751,13 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
             a.first_event = J.first_event + c0;
             a.pstack = as<cmpmc_phonon_track>(M->pst);
             a.pstack_off = as<uint64_t>(M->poff);
-            a.n_init_ph = J.wl ? nullptr : as<uint32_t>(M->pinit);
+            a.n_init_ph = lane_gen ? nullptr : as<uint32_t>(M->pinit);
             a.cstack = ch ? as<cmpmc_charge_track>(M->cst) : nullptr;
             a.cstack_off = ch ? as<uint64_t>(M->coff) : nullptr;
-            a.n_init_ch = (ch && !J.wl) ? as<uint32_t>(M->cinit) : nullptr;
-            a.rng_state = J.wl ? nullptr : as<uint64_t>(M->rng);
-            a.injected = (J.injected && !J.wl) ? as<double>(M->inj) : nullptr;
-            a.wl = J.wl ? as<cmpmc_workload>(M->wl) : nullptr;
+            a.n_init_ch = (ch && !lane_gen) ? as<uint32_t>(M->cinit) : nullptr;
+            a.rng_state = lane_gen ? nullptr : as<uint64_t>(M->rng);
+            a.injected = ((J.injected && !J.wl) || dep) ? as<double>(M->inj) : nullptr;
+            a.wl = lane_gen ? as<cmpmc_workload>(M->wl) : nullptr;
             a.seed = C.seed;
             a.hits = as<DevHit>(M->hits);
             a.hit_count = &d_acc->hit_count;
@@ -827,9 +856,9 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
-    const int exact = M->dm.kin_mode == 1 ? 1 : 0;
-    const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, exact, block, &per_cu)
-                                      : cmpmc_occupancy(which, waves, exact, block, &per_cu);
+    const int flags = (M->dm.kin_mode == 1 ? 1 : 0) | (M->dm.surf_x ? 2 : 0);
+    const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, flags, block, &per_cu)
+                                      : cmpmc_occupancy(which, waves, flags, block, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
     return per_cu * M->n_cu;
 }
@@ -864,7 +893,23 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     // initial queues
     uint64_t n_init = 0, n_charges = 0;
     const bool pair = J.wl && J.wl->kind == CMPMC_WL_PAIR;
-    if (J.wl && !pair) {                          // per-event counts and their offsets, on the device
+    const bool deposit = J.wl && J.wl->kind == CMPMC_WL_DEPOSIT;
+    if (deposit) {                                // per-event phonon and carrier counts, offsets
+        CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * 4));
+        CHK(ensure(M->ccounts, std::max<uint64_t>(N, 1) * 4));
+        CHK(ensure(M->qoff, (N + 1) * 8));
+        CHK(ensure(M->cqoff, (N + 1) * 8));
+        CHK(ensure(M->scan_part, cmpmc_scan_partials(N) * 8));
+        HIPCHK(cmpmc_launch_deposit_count(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
+                                          as<uint32_t>(M->counts), as<uint32_t>(M->ccounts), s));
+        HIPCHK(cmpmc_launch_scan_counts(as<uint32_t>(M->counts), N, as<uint64_t>(M->qoff), as<uint64_t>(M->scan_part),
+                                        ~0ull, nullptr, s));
+        HIPCHK(cmpmc_launch_scan_counts(as<uint32_t>(M->ccounts), N, as<uint64_t>(M->cqoff),
+                                        as<uint64_t>(M->scan_part), ~0ull, nullptr, s));
+        HIPCHK(hipMemcpyAsync(&n_init, as<uint64_t>(M->qoff) + N, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(&n_charges, as<uint64_t>(M->cqoff) + N, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } else if (J.wl && !pair) {                   // per-event counts and their offsets, on the device
         CHK(ensure(M->counts, std::max<uint64_t>(N, 1) * 4));
         CHK(ensure(M->qoff, (N + 1) * 8));
         CHK(ensure(M->scan_part, cmpmc_scan_partials(N) * 8));
@@ -906,7 +951,12 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         CHK(ensure(M->queue, std::max<uint64_t>(n_init + luke_cap, 1) * sizeof(BTrack)));
         BTrack* q = as<BTrack>(M->queue);
         unsigned long long inj_fixed = 0;
-        if (J.wl && !pair) {
+        if (deposit) {
+            CHK(ensure(M->cqueue, std::max<uint64_t>(n_charges, 1) * sizeof(BCharge)));
+            HIPCHK(cmpmc_launch_deposit_fill(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
+                                             as<uint64_t>(M->qoff), as<uint64_t>(M->cqoff), q, as<BCharge>(M->cqueue),
+                                             d_acc->ledger, s));
+        } else if (J.wl && !pair) {
             HIPCHK(cmpmc_launch_workload_fill(dmp, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
                                               as<uint64_t>(M->qoff), n_init, q, d_acc->ledger, nullptr, s));
         } else if (!J.wl && n_init) {
@@ -919,7 +969,7 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
         }
         if (!J.wl && J.injected)
             for (uint64_t e = 0; e < N; ++e) inj_fixed += fixed_point(J.injected[e]);
-        if (n_charges) {
+        if (n_charges && !deposit) {
             CHK(ensure(M->cqueue, n_charges * sizeof(BCharge)));
             if (pair) {
                 HIPCHK(cmpmc_launch_pair_fill(&M->dm, as<cmpmc_workload>(M->wl), C.seed, J.first_event, N,
@@ -1118,8 +1168,25 @@ int check_cfg(const RunCfg& C) {
 
 int check_workload(const cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_events) {
     if (wl->kind != CMPMC_WL_PHONON_POINT && wl->kind != CMPMC_WL_PHONON_LOGN && wl->kind != CMPMC_WL_PAIR &&
-        wl->kind != CMPMC_WL_PHONON_UNIF)
+        wl->kind != CMPMC_WL_PHONON_UNIF && wl->kind != CMPMC_WL_DEPOSIT)
         return fail(CMPMC_ERR_INVALID, "workload: unknown kind %d", wl->kind);
+    if (wl->kind == CMPMC_WL_DEPOSIT) {
+        const cmpmc_partition_opts& o = wl->partition;
+        if ((wl->mode != 0 && wl->mode != 1) || !(wl->energy_eV >= 0.0) || !std::isfinite(wl->energy_eV))
+            return fail(CMPMC_ERR_INVALID, "deposit workload: mode must be 0 (electron) or 1 (nuclear recoil) and the "
+                        "energy finite and >= 0");
+        if (o.niel_model < CMPMC_NIEL_LINDHARD || o.niel_model > CMPMC_NIEL_EMPIRICAL)
+            return fail(CMPMC_ERR_INVALID, "deposit workload: unknown NIEL model %d", o.niel_model);
+        if (o.max_sampled_pairs == 0 || o.max_sampled_phonons == 0 || o.max_sampled_pairs > (1u << 20) ||
+            o.max_sampled_phonons > (1u << 20))
+            return fail(CMPMC_ERR_INVALID, "deposit workload: max_sampled_pairs and max_sampled_phonons must be in "
+                        "[1, 2^20]");
+        if (M->dm.iv_model == 2)
+            return fail(CMPMC_ERR_UNVALIDATED,
+                        "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
+        if (n_events >= (1ull << 32)) return fail(CMPMC_ERR_INVALID, "workload: at most 2^32 - 1 events per call");
+        return CMPMC_OK;
+    }
     const bool mode_ok = (wl->mode >= 0 && wl->mode <= 2) ||
                          (wl->mode == CMPMC_MODE_BENCH_DOS && wl->kind == CMPMC_WL_PHONON_POINT);
     if (!mode_ok || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
@@ -1284,8 +1351,8 @@ extern "C" int cmpmc_run_workload_noalloc(cmpmc_model* M, const cmpmc_workload* 
     out->kernel_ms = 0.0;
     out->passes = 0;
     // set the cached buffers aside, run on the arena, put them back
-    const std::array<DevBuf*, 32> bufs = run_buffers(M);
-    std::array<DevBuf, 32> saved;
+    const std::array<DevBuf*, kRunBuffers> bufs = run_buffers(M);
+    std::array<DevBuf, kRunBuffers> saved;
     for (size_t i = 0; i < bufs.size(); ++i) {
         saved[i] = *bufs[i];
         *bufs[i] = DevBuf{};
@@ -1326,7 +1393,7 @@ extern "C" int cmpmc_diff_forward(cmpmc_model* M, const cmpmc_workload* wl, uint
         if (sf.n_spec > 1 || sf.n_diff > 1 || sf.n_anh > 0)
             return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: surface %d has energy-dependent reflection or "
                         "down-conversion; the diffuse fraction is defined for constant splits only", k);
-        if (sf.n_pads > 0 || sf.has_amm)
+        if (M->dm.sx[k].n_pads > 0 || M->dm.sx[k].has_amm)
             return fail(CMPMC_ERR_INVALID, "cmpmc_diff_forward: surface %d has an electrode mask or AMM film entry; "
                         "the film probability p is defined for uniform films only", k);
     }

@@ -105,7 +105,7 @@ struct cmpmc_model {
     // cached run buffers
     cmpmc_host::DevBuf ws, pst, cst, poff, coff, pinit, cinit, rng, inj, tin, cin, tin_off, cin_off, hits, hits_tmp,
         keys_a, keys_b, idx_a, idx_b, sorted, counts, fb, acc, queue, cqueue, lstack, clstack, qoff, wl, cub_tmp, fjobs,
-        scan_part;
+        scan_part, ccounts, cqoff;
 };
 
 namespace cmpmc_host {

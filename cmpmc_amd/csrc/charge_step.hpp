@@ -410,9 +410,12 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             // pads reflects the carrier
             int32_t ch = -1;
             bool active_pad = true;
-            if (s.n_pads > 0) {
-                ch = surface_channel<P>(m, s.pad_off, s.n_pads, trk.pos, r);
-                active_pad = (ch >= 0);
+            if constexpr (P::surf_x) {
+                const DevSurfX& x = m.sx[sid];
+                if (x.n_pads > 0) {
+                    ch = surface_channel<P>(m, x.pad_off, x.n_pads, trk.pos, r);
+                    active_pad = (ch >= 0);
+                }
             }
             if (!active_pad || !should_absorb<P>(trk, n_in, s, r)) {
                 ctx.ccount(CC_SURF_REFL);

@@ -54,9 +54,13 @@ struct DevSurface {
     double spec[CMPMC_MAX_POLY], diff[CMPMC_MAX_POLY], anh[CMPMC_MAX_POLY];
     double spec_prob_const;
     DevFilm film;
-    // electrode mask: pads [pad_off, pad_off + n_pads) of DevModel::pads (n_pads = 0: none);
-    // AMM film entry (has_amm) between these isotropic media (rho, vL, vT)
-    int32_t n_pads, pad_off, has_amm, _pad_m;
+};
+
+// Surface extras, read only by the kernels compiled with them (SurfX policies): electrode
+// mask = pads [pad_off, pad_off + n_pads) of DevModel::pads (n_pads = 0: none); AMM film
+// entry (has_amm) between these isotropic media (rho, vL, vT).
+struct DevSurfX {
+    int32_t n_pads, pad_off, has_amm, _pad;
     double amm_sub[3], amm_film[3];
 };
 
@@ -89,11 +93,16 @@ struct DevModel {
     // detector
     double radius, half_z, efield[3];
     DevSurface surf[3];
+    DevSurfX sx[3];
     uint32_t max_bounces, rr_bounces;
     // DetectorModel::ChargeStepping: 0 first order (dk_frac = charge_max_dk_frac), 1 Leman
     // second order (dk_frac = charge_sampling_dk_frac)
-    int32_t charge_stepping, _pad_cs;
+    int32_t charge_stepping;
+    int32_t surf_x;             // some surface has an electrode mask or AMM film entry
     double min_phonon_eV, luke_fraction, dk_frac, dt_cap, fermi, rr_survival;
+    // energy partition of deposit workloads: target Z, A, mean pair energy, Fano factor,
+    // primary (Debye) phonon energy
+    double atomic_Z, atomic_A, pair_energy, fano, debye_eV;
     // FP32 copies used by the batched kernels (uniform scalar operands, no per-step
     // conversions), filled on the host by fill_fp32_constants.
     // Reflection split of a surface whose polynomials are constants (n <= 1): piecewise
@@ -235,6 +244,7 @@ struct Fp64 {
     using TRng = Rng;                            // a track's transport stream
     static constexpr bool exact = true;
     static constexpr bool kin_exact = false;     // octahedral LUT kinematics
+    static constexpr bool surf_x = false;        // no electrode masks or AMM film entry
     __device__ static Rng track_stream(uint64_t base, uint64_t key) { return rng_for_track(base, key); }
     __device__ static double uniform(Rng& r) { return static_cast<double>((next_u64(r) >> 11) | 1ULL) * 0x1.0p-53; }
     __device__ static double neg_log(double u) { return -log(u); }
@@ -264,6 +274,7 @@ struct Fp32 {
     using TRng = Rng32;                          // a track's transport stream
     static constexpr bool exact = false;
     static constexpr bool kin_exact = false;
+    static constexpr bool surf_x = false;
     __device__ static Rng32 track_stream(uint64_t base, uint64_t key) { return rng32_for_track(base, key); }
     // the top 24 bits of a draw: (0, 1), never 0 or 1 (64-bit event streams and 32-bit
     // transport streams alike)
@@ -299,6 +310,12 @@ struct Fp64X : Fp64 {
 };
 struct Fp32X : Fp32 {
     static constexpr bool kin_exact = true;
+};
+// The same with the surface extras (electrode masks, AMM film entry) compiled in: the
+// kernels of a model whose surfaces have neither are built without that code, so the
+// common case keeps its registers and instruction stream.
+template <class B> struct SurfX : B {
+    static constexpr bool surf_x = true;
 };
 
 template <class T> __host__ __device__ inline T tmax(T a, T b) { return (a < b) ? b : a; }   // std::max
@@ -845,9 +862,9 @@ inline void fill_fp32_constants(DevModel& m) {
         f.film_gap2 = 2.0 * fl.gap;
         f.film_abs_gap2 = 2.0 * fl.absorber_gap;
         f.subgap_abs = static_cast<float>(fl.subgap_absorption);
-        f.n_pads = s.n_pads;
-        f.pad_off = s.pad_off;
-        f.has_amm = s.has_amm;
+        f.n_pads = m.sx[k].n_pads;
+        f.pad_off = m.sx[k].pad_off;
+        f.has_amm = m.sx[k].has_amm;
         f._pad3 = 0;
         f.prob_abs = static_cast<float>(s.prob_abs);
         f.prob_refl = static_cast<float>(s.prob_refl);
@@ -884,7 +901,7 @@ __device__ inline int32_t surface_channel(const DevModel& m, int pad_off, int n_
 // The plain form's AMM film-entry probability (IsotropicAMMInterface::transmission,
 // interface_transmission.hpp:123-129), solved per call in FP64; out of line so that the
 // 4x4 complex solve does not weigh on the kernel's registers.
-__device__ __noinline__ double amm_entry64(const DevSurface& s, int longitudinal, double cos_theta) {
+__device__ __noinline__ double amm_entry64(const DevSurfX& s, int longitudinal, double cos_theta) {
     const cmpmc_amm::Medium sub{s.amm_sub[0], s.amm_sub[1], s.amm_sub[2]};
     const cmpmc_amm::Medium film{s.amm_film[0], s.amm_film[1], s.amm_film[2]};
     return cmpmc_amm::transmission(sub, film, longitudinal != 0, cos_theta);

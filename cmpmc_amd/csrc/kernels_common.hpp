@@ -3,6 +3,7 @@
 #pragma once
 
 #include "charge_step.hpp"
+#include "energy_partition.hpp"
 
 namespace cmpmc_dev {
 
@@ -171,6 +172,101 @@ __device__ inline void workload_phonon(const DevModel& m, const cmpmc_workload& 
     }
     const V3<R> k = isotropic_dir<P>(r);
     make_track<P>(m, t, pos, k, md, E, 0.0, 1.0);
+}
+
+// ------------------------------------------------------------------ deposit workloads
+// A deposit event: position (origin when pair_at_origin is set, else uniform in the
+// volume, three uniforms), then EnergyPartition::partition_deposit (energy_partition.hpp:
+// 78-157) with the event stream, in the reference's draw order, through the shared core of
+// energy_partition.hpp. Both forms draw the whole partition from the event's reference
+// stream Rng::for_stream(seed, event); the batched form then transports initial phonon j
+// with key j and initial carrier c with key 2^63 | c.
+__device__ inline cmpmc_part::Material part_material(const DevModel& m) {
+    return cmpmc_part::Material{m.atomic_Z, m.atomic_A, m.density, m.bandgap, m.pair_energy, m.fano,
+                                m.debye_eV, m.fL, m.fST, m.n_valleys};
+}
+
+// The reference Rng's samplers (rng.hpp:62-83) on a device event stream.
+struct PartRng {
+    Rng& r;
+    __device__ double uniform() { return Fp64::uniform(r); }
+    __device__ double normal() {
+        const double u1 = uniform();
+        const double u2 = uniform();
+        return sqrt(-2.0 * log(u1)) * cos(kTwoPi * u2);
+    }
+    __device__ uint64_t next_u64() { return cmpmc_dev::next_u64(r); }
+    __device__ void isotropic_dir(double d[3]) {
+        const V3<double> v = cmpmc_dev::isotropic_dir<Fp64>(r);
+        d[0] = v.x; d[1] = v.y; d[2] = v.z;
+    }
+};
+
+__device__ inline void deposit_position(const DevModel& m, const cmpmc_workload& w, Rng& r, double pos[3]) {
+    if (w.pair_at_origin) {
+        pos[0] = w.origin[0]; pos[1] = w.origin[1]; pos[2] = w.origin[2];
+        return;
+    }
+    const V3<double> p = uniform_volume<Fp64>(m, r);
+    pos[0] = p.x; pos[1] = p.y; pos[2] = p.z;
+}
+
+// Per-event counts: phonons = sampled prompt phonons, charges = 2 x sampled pairs.
+__device__ inline void deposit_counts(const DevModel& m, const cmpmc_workload& w, Rng& r, uint32_t& n_ph,
+                                      uint32_t& n_ch) {
+    double pos[3];
+    deposit_position(m, w, r, pos);
+    PartRng g{r};
+    const cmpmc_partition_result res = cmpmc_part::plan(part_material(m), w.energy_eV, w.mode == 1, w.partition, g);
+    n_ph = static_cast<uint32_t>(res.n_phonons_sampled);
+    n_ch = static_cast<uint32_t>(2 * res.n_pairs_sampled);
+}
+
+// The tracks of a deposit event: sink.charge(CTrack<Q>&, index) for every carrier (electron
+// then hole per pair) and sink.phonon(PTrack<Q>&, index) for every prompt phonon, made as
+// the reference makes them (make_charge; a PhononTrack with k_dir as drawn).
+template <class Q, class Sink> struct DepositEmit {
+    const DevModel& m;
+    Sink& sink;
+    const double* pos0;
+    uint32_t nq = 0, nph = 0;
+    __device__ void pair(const double p[3], int valley, const double de[3], const double dh[3], double w, double ekin) {
+        using R = typename Q::R;
+        const V3<R> pos{R(p[0]), R(p[1]), R(p[2])};
+        CTrack<Q> e, h;
+        make_charge<Q>(m, e, 1, pos, V3<R>{R(de[0]), R(de[1]), R(de[2])}, R(ekin), valley, 0.0);
+        e.weight = w;
+        sink.charge(e, nq++);
+        make_charge<Q>(m, h, 2, pos, V3<R>{R(dh[0]), R(dh[1]), R(dh[2])}, R(ekin), 0, 0.0);
+        h.weight = w;
+        sink.charge(h, nq++);
+    }
+    __device__ void phonon(int mode, const double k[3], double e, double w) {
+        using R = typename Q::R;
+        PTrack<Q> t;
+        t.pos = V3<R>{R(pos0[0]), R(pos0[1]), R(pos0[2])};
+        t.k = V3<R>{R(k[0]), R(k[1]), R(k[2])};
+        map_k_to_vg<Q>(m, mode, t.k, t.vg, t.vg_mag);
+        t.energy = e;
+        t.time = 0.0;
+        t.weight = w;
+        t.bounces = 0;
+        t.mode = mode;
+        t.spawn_no = 0;
+        t.hit_seq = 0;
+        sink.phonon(t, nph++);
+    }
+};
+
+template <class Q, class Sink>
+__device__ inline void deposit_event(const DevModel& m, const cmpmc_workload& w, Rng& r, Sink& sink) {
+    double pos[3];
+    deposit_position(m, w, r, pos);
+    PartRng g{r};
+    const cmpmc_part::Material mat = part_material(m);
+    const cmpmc_partition_result res = cmpmc_part::plan(mat, w.energy_eV, w.mode == 1, w.partition, g);
+    DepositEmit<Q, Sink> emit{m, sink, pos};
+    cmpmc_part::emit_tracks(mat, w.partition, res, pos, g, emit);
 }
 
 template <class P> __device__ inline void workload_pair(const DevModel& m, const cmpmc_workload& w, Rng& r,

@@ -183,16 +183,21 @@ __device__ inline bool phonon_step(const DevModel& m, PTrack<P>& trk, Rng& r, Ct
     // pads reflects); AMM transmission replaces the film-entry probability
     int32_t ch = -1;
     bool active_pad = true;
-    if (s.n_pads > 0) {
-        ch = surface_channel<P>(m, s.pad_off, s.n_pads, trk.pos, r);
-        active_pad = (ch >= 0);
+    if constexpr (P::surf_x) {
+        const DevSurfX& x = m.sx[sid];
+        if (x.n_pads > 0) {
+            ch = surface_channel<P>(m, x.pad_off, x.n_pads, trk.pos, r);
+            active_pad = (ch >= 0);
+        }
     }
     if (active_pad && s.has_film && P::uniform(r) < R(s.prob_abs)) {
         const DevFilm& film = s.film;
         R p_enter = R(film.film_absorption);
-        if (s.has_amm) {
-            if constexpr (P::exact) p_enter = amm_entry64(s, trk.mode == 0, dot(trk.k, n_in));
-            else p_enter = amm_entry32(m, sid, trk.mode == 0, dot(trk.k, n_in));
+        if constexpr (P::surf_x) {
+            if (m.sx[sid].has_amm) {
+                if constexpr (P::exact) p_enter = amm_entry64(m.sx[sid], trk.mode == 0, dot(trk.k, n_in));
+                else p_enter = amm_entry32(m, sid, trk.mode == 0, dot(trk.k, n_in));
+            }
         }
         if (P::uniform(r) < p_enter) {
             double* wb = ctx.kaplan_base();

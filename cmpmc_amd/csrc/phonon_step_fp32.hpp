@@ -99,12 +99,17 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
             // reflects); AMM transmission replaces the film-entry probability
             int32_t ch = -1;
             bool active_pad = true;
-            if (sf.n_pads > 0) {
-                ch = surface_channel<P>(m, sf.pad_off, sf.n_pads, trk.pos, r);
-                active_pad = (ch >= 0);
+            if constexpr (P::surf_x) {
+                if (sf.n_pads > 0) {
+                    ch = surface_channel<P>(m, sf.pad_off, sf.n_pads, trk.pos, r);
+                    active_pad = (ch >= 0);
+                }
             }
             if (active_pad && sf.has_film && P::uniform(r) < sf.prob_abs) {
-                const float p_enter = sf.has_amm ? amm_entry32(m, sid, trk.mode == 0, dot(trk.k, n_in)) : sf.film_abs;
+                float p_enter = sf.film_abs;
+                if constexpr (P::surf_x) {
+                    if (sf.has_amm) p_enter = amm_entry32(m, sid, trk.mode == 0, dot(trk.k, n_in));
+                }
                 const bool absorbed = P::uniform(r) < p_enter;
                 ctx.film_gate(absorbed);
                 if (absorbed) {

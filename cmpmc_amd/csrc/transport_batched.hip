@@ -787,6 +787,58 @@ __global__ void pair_fill_kernel(const DevModel m, const cmpmc_workload* wl, uin
     ctrack_to_bcharge(h, ev, out[2 * i + 1]);
 }
 
+// Deposit workloads: per-event counts of sampled prompt phonons and carriers (the
+// partition's draws up to the pair number, replayed by the fill).
+__global__ void deposit_count_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
+                                     uint64_t n, uint32_t* ph_counts, uint32_t* ch_counts) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Rng r = rng_for_stream(seed, first_event + i);
+    uint32_t nph = 0, nch = 0;
+    deposit_counts(m, *wl, r, nph, nch);
+    if (ph_counts) ph_counts[i] = nph;
+    if (ch_counts) ch_counts[i] = nch;
+}
+
+// Writes a deposit event's tracks into the initial queues: phonon j with key j, carrier c
+// with key 2^63 | c (the sub-stream keys of the batched form).
+template <class Q> struct QueueDepositSink {
+    uint64_t ev;
+    BTrack* q;
+    BCharge* cq;
+    __device__ void charge(CTrack<Q>& c, uint32_t i) {
+        c.key = 0x8000000000000000ull | i;
+        ctrack_to_bcharge(c, ev, cq[i]);
+    }
+    __device__ void phonon(PTrack<Q>& t, uint32_t j) {
+        t.key = j;
+        ptrack_to_btrack(t, ev, q[j]);
+    }
+};
+
+// One thread per event (the partition draws sequentially from the event stream).
+template <class Q>
+__global__ void deposit_fill_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
+                                    uint64_t n_events, const uint64_t* ph_off, const uint64_t* ch_off, BTrack* q,
+                                    BCharge* cq, unsigned long long* ledger) {
+    __shared__ unsigned long long s_inj;
+    if (threadIdx.x == 0) s_inj = 0;
+    __syncthreads();
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n_events) {
+        const uint64_t ev = first_event + i;
+        Rng r = rng_for_stream(seed, ev);
+        QueueDepositSink<Q> sink{ev, q + ph_off[i], cq + ch_off[i]};
+        deposit_event<Q>(m, *wl, r, sink);
+        atomicAdd(&s_inj, static_cast<unsigned long long>(__double2ll_rn(wl->energy_eV * kLedgerScale)));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_inj) {
+        atomicAdd(&ledger[2 * L_INJ], s_inj >> 20);
+        atomicAdd(&ledger[2 * L_INJ + 1], s_inj & 0xFFFFFull);
+    }
+}
+
 // ------------------------------------------------------------------ queue offsets
 // Exclusive scan of per-event counts in three launches: tile sums, a one-block scan of
 // the tile sums, then each tile's own scan. A tile is kScanTile counts (256 threads x 8).
@@ -950,7 +1002,10 @@ template <bool Diff, class P> static const void* batched_fn_p(int waves) {
                              reinterpret_cast<const void*>(&transport_batched_kernel<3, Diff, P>),
                              reinterpret_cast<const void*>(&transport_batched_kernel<4, Diff, P>));
 }
-static const void* batched_fn(int waves, bool diff, bool exact) {
+// flags: bit 0 exact kinematics, bit 1 surface extras (the differentiable form has none)
+static const void* batched_fn(int waves, bool diff, int flags) {
+    const bool exact = (flags & 1) != 0, surf = (flags & 2) != 0 && !diff;
+    if (surf) return exact ? batched_fn_p<false, SurfX<Fp32X>>(waves) : batched_fn_p<false, SurfX<Fp32>>(waves);
     if (exact) return diff ? batched_fn_p<true, Fp32X>(waves) : batched_fn_p<false, Fp32X>(waves);
     return diff ? batched_fn_p<true, Fp32>(waves) : batched_fn_p<false, Fp32>(waves);
 }
@@ -959,21 +1014,26 @@ template <class P> static const void* charges_fn_p(int waves) {
                              reinterpret_cast<const void*>(&transport_charges_kernel<3, P>),
                              reinterpret_cast<const void*>(&transport_charges_kernel<4, P>));
 }
-static const void* charges_fn(int waves, bool exact) { return exact ? charges_fn_p<Fp32X>(waves) : charges_fn_p<Fp32>(waves); }
+static const void* charges_fn(int waves, int flags) {
+    const bool exact = (flags & 1) != 0;
+    if (flags & 2) return exact ? charges_fn_p<SurfX<Fp32X>>(waves) : charges_fn_p<SurfX<Fp32>>(waves);
+    return exact ? charges_fn_p<Fp32X>(waves) : charges_fn_p<Fp32>(waves);
+}
+static int model_flags(const DevModel* m) { return (m->kin_mode == 1 ? 1 : 0) | (m->surf_x ? 2 : 0); }
 
 // Resident blocks per CU of the persistent kernels (sizes their grids).
-extern "C" hipError_t cmpmc_occupancy(int which, int waves, int exact, int block, int* blocks_per_cu) {
+extern "C" hipError_t cmpmc_occupancy(int which, int waves, int flags, int block, int* blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, which == 0 ? batched_fn(waves, false, exact != 0) : charges_fn(waves, exact != 0), block, 0);
+        blocks_per_cu, which == 0 ? batched_fn(waves, false, flags) : charges_fn(waves, flags), block, 0);
 }
-extern "C" hipError_t cmpmc_occupancy_diff(int waves, int exact, int block, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_fn(waves, true, exact != 0), block, 0);
+extern "C" hipError_t cmpmc_occupancy_diff(int waves, int flags, int block, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, batched_fn(waves, true, flags), block, 0);
 }
 
 extern "C" hipError_t cmpmc_launch_batched(const DevModel* m, const BatchedArgs* a, int block, int grid, int waves,
                                            int diff, hipStream_t s) {
     void* args[] = {const_cast<DevModel*>(m), const_cast<BatchedArgs*>(a)};
-    return hipLaunchKernel(batched_fn(waves, diff != 0, m->kin_mode == 1), dim3(grid), dim3(block), args, 0, s);
+    return hipLaunchKernel(batched_fn(waves, diff != 0, model_flags(m)), dim3(grid), dim3(block), args, 0, s);
 }
 extern "C" hipError_t cmpmc_launch_tape_forward(const TapeRec* tape, uint64_t n, unsigned long long* y_fixed,
                                                 hipStream_t s) {
@@ -994,7 +1054,7 @@ extern "C" hipError_t cmpmc_launch_tape_finish(const double* partials, int n_blo
 extern "C" hipError_t cmpmc_launch_charges(const DevModel* m, const ChargeArgs* a, int block, int grid, int waves,
                                            hipStream_t s) {
     void* args[] = {const_cast<DevModel*>(m), const_cast<ChargeArgs*>(a)};
-    return hipLaunchKernel(charges_fn(waves, m->kin_mode == 1), dim3(grid), dim3(block), args, 0, s);
+    return hipLaunchKernel(charges_fn(waves, model_flags(m)), dim3(grid), dim3(block), args, 0, s);
 }
 extern "C" hipError_t cmpmc_launch_workload_count(const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
                                                   uint64_t n, uint32_t* counts, const DynParams* dyn, hipStream_t s) {
@@ -1040,6 +1100,28 @@ extern "C" hipError_t cmpmc_launch_clear_hit_keys(DevHit* h, uint64_t n, hipStre
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(clear_hit_keys_kernel, dim3(static_cast<unsigned>(std::min<uint64_t>(blocks_for(n, 256), 4096))),
                        dim3(256), 0, s, h, n);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t cmpmc_launch_deposit_count(const DevModel* m, const cmpmc_workload* wl, uint64_t seed,
+                                                 uint64_t first_event, uint64_t n, uint32_t* ph_counts,
+                                                 uint32_t* ch_counts, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(deposit_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, *m, wl, seed, first_event, n,
+                       ph_counts, ch_counts);
+    return hipGetLastError();
+}
+extern "C" hipError_t cmpmc_launch_deposit_fill(const DevModel* m, const cmpmc_workload* wl, uint64_t seed,
+                                                uint64_t first_event, uint64_t n, const uint64_t* ph_off,
+                                                const uint64_t* ch_off, BTrack* q, BCharge* cq,
+                                                unsigned long long* ledger, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (m->kin_mode == 1)
+        hipLaunchKernelGGL(deposit_fill_kernel<Fp64X>, dim3(blocks_for(n, 64)), dim3(64), 0, s, *m, wl, seed,
+                           first_event, n, ph_off, ch_off, q, cq, ledger);
+    else
+        hipLaunchKernelGGL(deposit_fill_kernel<Fp64>, dim3(blocks_for(n, 64)), dim3(64), 0, s, *m, wl, seed,
+                           first_event, n, ph_off, ch_off, q, cq, ledger);
     return hipGetLastError();
 }
 

@@ -209,12 +209,18 @@ struct ChargeArgs {
 
 extern "C" {
 // which: 0 phonon kernel, 1 charge kernel; waves: register-budget variant (2, 3 or 4);
-// exact: the exact-kinematics instantiation.
-hipError_t cmpmc_occupancy(int which, int waves, int exact, int block, int* blocks_per_cu);
+// flags: bit 0 the exact-kinematics instantiation, bit 1 the one with surface extras.
+hipError_t cmpmc_occupancy(int which, int waves, int flags, int block, int* blocks_per_cu);
 hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block, hipStream_t s);
+// Plain form of deposit workloads: every event's partition into its stack regions, its
+// stream state after the partition and its injected energy (one thread per event).
+hipError_t cmpmc_launch_deposit_stacks(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
+                                       uint64_t first_event, uint64_t n_events, cmpmc_phonon_track* pstack,
+                                       const uint64_t* poff, cmpmc_charge_track* cstack, const uint64_t* coff,
+                                       uint64_t* rng_state, double* injected, hipStream_t s);
 hipError_t cmpmc_launch_batched(const cmpmc_dev::DevModel* m, const cmpmc_dev::BatchedArgs* a, int block, int grid,
                                 int waves, int diff, hipStream_t s);
-hipError_t cmpmc_occupancy_diff(int waves, int exact, int block, int* blocks_per_cu);
+hipError_t cmpmc_occupancy_diff(int waves, int flags, int block, int* blocks_per_cu);
 // Differentiable form: y[kDiffOut] from the tape (fixed-point sums), and the backward
 // reduction grad[4] = sum_h (g[chan] + g[2 + bin]) e_h s_h / theta.
 hipError_t cmpmc_launch_tape_forward(const cmpmc_dev::TapeRec* tape, uint64_t n, unsigned long long* y_fixed,
@@ -238,6 +244,16 @@ hipError_t cmpmc_launch_workload_fill(const cmpmc_dev::DevModel* m, const cmpmc_
 hipError_t cmpmc_launch_pair_fill(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
                                   uint64_t first_event, uint64_t n_events, cmpmc_dev::BCharge* out,
                                   const cmpmc_dev::DynParams* dyn, hipStream_t s);
+// Deposit workloads (both forms): per-event counts of sampled prompt phonons and carriers
+// (either pointer may be null), and the batched form's initial queues at the scanned
+// offsets, with the deposits' energy added to the ledger's injected total.
+hipError_t cmpmc_launch_deposit_count(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
+                                      uint64_t first_event, uint64_t n_events, uint32_t* ph_counts,
+                                      uint32_t* ch_counts, hipStream_t s);
+hipError_t cmpmc_launch_deposit_fill(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl_dev, uint64_t seed,
+                                     uint64_t first_event, uint64_t n_events, const uint64_t* ph_off,
+                                     const uint64_t* ch_off, cmpmc_dev::BTrack* q, cmpmc_dev::BCharge* cq,
+                                     unsigned long long* ledger, hipStream_t s);
 // Exclusive scan of n per-event counts: offsets[0..n] (offsets[n] = total), in three
 // launches over blocks of kScanTile counts; partials holds scan_partials(n) words.
 // dyn->needed = total; dyn->n_init = total when it fits in cap, else 0.

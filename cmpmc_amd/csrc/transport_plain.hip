@@ -115,6 +115,45 @@ template <class P> __device__ inline void load_charge(const cmpmc_charge_track& 
     c.valley = o.valley; c.type = o.type;
 }
 
+// Deposit workloads in the plain form: a generation pass writes each event's partition
+// (energy_partition.hpp, the reference's draw order on the event stream) into its stack
+// regions, with the stream state after the partition and the injected energy; the
+// transport kernel then runs the events as host-built stacks. Keeping the partition out of
+// the transport kernel keeps that kernel's registers for the transport loop.
+template <class P> struct StackDepositSink {
+    cmpmc_phonon_track* ps;
+    cmpmc_charge_track* cs;
+    __device__ void charge(CTrack<P>& c, uint32_t i) {
+        cmpmc_charge_track& o = cs[i];
+        o.pos[0] = c.pos.x; o.pos[1] = c.pos.y; o.pos[2] = c.pos.z;
+        o.p_eV[0] = c.p.x; o.p_eV[1] = c.p.y; o.p_eV[2] = c.p.z;
+        o.ekin = c.ekin; o.time = c.time; o.weight = c.weight;
+        o.valley = static_cast<uint16_t>(c.valley); o.type = static_cast<uint8_t>(c.type); o.alive = 1;
+    }
+    __device__ void phonon(const PTrack<P>& t, uint32_t j) {
+        cmpmc_phonon_track& o = ps[j];
+        o.pos[0] = t.pos.x; o.pos[1] = t.pos.y; o.pos[2] = t.pos.z;
+        o.k_dir[0] = t.k.x; o.k_dir[1] = t.k.y; o.k_dir[2] = t.k.z;
+        o.vg_dir[0] = t.vg.x; o.vg_dir[1] = t.vg.y; o.vg_dir[2] = t.vg.z;
+        o.energy = t.energy; o.vg_mag = t.vg_mag; o.time = t.time; o.weight = t.weight;
+        o.bounces = t.bounces; o.mode = static_cast<uint8_t>(t.mode);
+    }
+};
+
+template <class P>
+__global__ void deposit_stacks_kernel(const DevModel m, const cmpmc_workload* wl, uint64_t seed, uint64_t first_event,
+                                      uint64_t n, cmpmc_phonon_track* pstack, const uint64_t* poff,
+                                      cmpmc_charge_track* cstack, const uint64_t* coff, uint64_t* rng_state,
+                                      double* injected) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Rng r = rng_for_stream(seed, first_event + i);
+    StackDepositSink<P> sink{pstack + poff[i], cstack ? cstack + coff[i] : nullptr};
+    deposit_event<P>(m, *wl, r, sink);
+    rng_state[4 * i] = r.s0; rng_state[4 * i + 1] = r.s1; rng_state[4 * i + 2] = r.s2; rng_state[4 * i + 3] = r.s3;
+    injected[i] = wl->energy_eV;
+}
+
 // Runs one event (generation, charges, phonons) on this lane.
 template <class P>
 __device__ void plain_event(const DevModel& m, const PlainArgs& a, PlainCtx<P>& ctx, uint64_t lane) {
@@ -300,15 +339,39 @@ extern "C" hipError_t cmpmc_launch_scatter_charges(const cmpmc_charge_track* in,
     return hipGetLastError();
 }
 
+extern "C" hipError_t cmpmc_launch_deposit_stacks(const cmpmc_dev::DevModel* m, const cmpmc_workload* wl,
+                                                  uint64_t seed, uint64_t first_event, uint64_t n,
+                                                  cmpmc_phonon_track* pstack, const uint64_t* poff,
+                                                  cmpmc_charge_track* cstack, const uint64_t* coff,
+                                                  uint64_t* rng_state, double* injected, hipStream_t s) {
+    using namespace cmpmc_dev;
+    if (n == 0) return hipSuccess;
+    const dim3 grid(static_cast<unsigned>((n + 63) / 64));
+    if (m->kin_mode == 1)
+        hipLaunchKernelGGL(deposit_stacks_kernel<Fp64X>, grid, dim3(64), 0, s, *m, wl, seed, first_event, n, pstack,
+                           poff, cstack, coff, rng_state, injected);
+    else
+        hipLaunchKernelGGL(deposit_stacks_kernel<Fp64>, grid, dim3(64), 0, s, *m, wl, seed, first_event, n, pstack,
+                           poff, cstack, coff, rng_state, injected);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block,
                                          hipStream_t stream) {
     const uint64_t n = a->n_events;
     if (n == 0) return hipSuccess;
     const int bs = (block > 0 && block <= cmpmc_dev::kMaxBlock) ? block : 256;
     const unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
-    if (m->kin_mode == 1)
-        hipLaunchKernelGGL(cmpmc_dev::transport_plain_kernel<cmpmc_dev::Fp64X>, dim3(grid), dim3(bs), 0, stream, *m, *a);
-    else
-        hipLaunchKernelGGL(cmpmc_dev::transport_plain_kernel<cmpmc_dev::Fp64>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    using namespace cmpmc_dev;
+    if (m->surf_x) {
+        if (m->kin_mode == 1)
+            hipLaunchKernelGGL(transport_plain_kernel<SurfX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+        else
+            hipLaunchKernelGGL(transport_plain_kernel<SurfX<Fp64>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    } else if (m->kin_mode == 1) {
+        hipLaunchKernelGGL(transport_plain_kernel<Fp64X>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    } else {
+        hipLaunchKernelGGL(transport_plain_kernel<Fp64>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    }
     return hipGetLastError();
 }

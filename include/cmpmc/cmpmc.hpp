@@ -10,3 +10,5 @@
 #include "cmpmc/geometry/surface_property.hpp"
 #include "cmpmc/transport/engine.hpp"
 #include "cmpmc/transport/parallel.hpp"
+#include "cmpmc/charge/niel_models.hpp"
+#include "cmpmc/charge/energy_partition.hpp"

@@ -3,6 +3,7 @@
 // engine, generator lambda, run_events) compiles against include/cmpmc/ and runs on
 // libcmpmc_amd. Prints "name value reference rel_err PASS|FAIL" lines and returns
 // nonzero on any failure.
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -191,6 +192,47 @@ int main(int argc, char** argv) {
             check(tag + ": every hit carries a QET channel", static_cast<double>(bad), 0.0, 0.0);
             check(tag + ": core and sectors all hit", (seen[0] && seen[1] && seen[2] && seen[3]) ? 1 : 0, 1, 0.0);
         }
+    }
+    {   // energy partition (charge/energy_partition.hpp): bookkeeping, then the reference's
+        // end-to-end recoil demo (partition, drift at 4 V, NTL phonons) through run_events
+        const EnergyPartition part(lattice);
+        PartitionOptions all;
+        all.max_sampled_pairs = 1u << 30;
+        all.max_sampled_phonons = 1u << 30;
+        Rng r(77);
+        for (double E : {50.0, 1000.0}) {
+            std::vector<ChargeTrack> q;
+            std::vector<PhononTrack> ph;
+            const auto res = part.partition_deposit(Vec3{0, 0, 0}, 0.0, E, false, q, ph, r, all);
+            double eq = 0, ep = 0;
+            for (const auto& c : q) eq += c.ekin * c.weight;
+            for (const auto& t : ph) ep += t.energy * t.weight;
+            check("partition ER " + std::to_string(static_cast<int>(E)) + " eV: N E_g + E_kin + E_ph = omega",
+                  res.n_pairs_true * lattice.config().bandgap + eq + ep, E, 1e-9);
+        }
+        check("Fano sampler: n_bar <= 1 gives 1", static_cast<double>(physics::sample_fano_binomial(0.7, 0.1, r)), 1, 0.0);
+        DetectorModel dd;
+        dd.e_field = UniformElectricField::from_bias(4.0, 2.0 * dd.solid.half_z);
+        dd.luke_sample_fraction = 1.0;
+        const SimulationEngine er(lattice, dd);
+        PartitionOptions popt;
+        popt.max_sampled_phonons = 200;
+        std::atomic<uint64_t> pairs{0};
+        auto recoil = [&](EventInput& in) {
+            const auto res = part.partition_deposit(Vec3{0, 0, 0}, 0.0, 100.0, true, in.charges, in.phonons, in.rng, popt);
+            pairs += res.n_pairs_true;
+            in.injected_eV += 100.0;
+        };
+        RunOptions o;
+        o.seed = 1337;
+        o.form = CMPMC_FORM_BATCHED;
+        const uint64_t ne = 6;
+        const RunResult rr = run_events(er, ne, recoil, o);
+        const double expected = ne * 100.0 + static_cast<double>(pairs.load()) * 4.0;
+        check("recoil demo: calorimeter energy = E + N_pairs e V (2%)",
+              rr.stats.total_deposited_eV + rr.stats.escaped_or_killed_eV, expected, 0.02);
+        check("recoil demo: every carrier collected", static_cast<double>(rr.stats.charges_collected),
+              2.0 * static_cast<double>(pairs.load()), 0.0);
     }
     {   // a worker exception reaches the caller (guarded Matrix IV model)
         CrystalConfig cfg = materials::Germanium();
