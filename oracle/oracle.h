@@ -164,6 +164,38 @@ double orc_dmm(const orc_medium* m1, const orc_medium* m2);
 void orc_amm_psv(const orc_medium* m1, const orc_medium* m2, int incident_P, double theta1, double out[4]);
 double orc_amm_transmission(const orc_medium* substrate, const orc_medium* film, int longitudinal, double cos_theta);
 
+/* ---- FEM mesh fields: geometry/mesh_field.hpp (oracle_mesh.c) ----------------------- */
+typedef struct orc_mesh {
+    int dim;                  /* 3: TriLinearMesh3D (tetrahedra), 2: BiLinearMesh2D (triangles) */
+    int axisymmetric_rz;      /* 2D: (r = hypot(x, y), z) projection */
+    size_t n_vertices, n_cells;
+    double* x;                /* dim coordinates per vertex */
+    double* v;                /* potential per vertex (3D: clamped) */
+    int32_t* cells;           /* dim + 1 vertex indices per cell, sorted ascending */
+    int32_t* nbr;             /* neighbour across the facet opposite each vertex, -1 on the hull */
+    double* tinv;             /* inverse edge matrix per cell (9 or 4, row-major) */
+    double* grad;             /* cell field E = -grad V (3 or 2 components) */
+} orc_mesh;
+
+orc_mesh* orc_mesh3d_new(const double* xyz, size_t nv, const double* V, const int32_t* tets, size_t nt,
+                         double v_min_clamp, double v_max_clamp);
+orc_mesh* orc_mesh2d_new(const double* uv, size_t nv, const double* V, const int32_t* tris, size_t nt,
+                         int axisymmetric_rz);
+void orc_mesh3d_grid_vertices(double hx, double hy, double hz, int nx, int ny, int nz, double* xyz);
+orc_mesh* orc_mesh3d_grid(double hx, double hy, double hz, int nx, int ny, int nz, const double* V);
+void orc_mesh2d_grid_vertices(double u_min, double u_max, double v_min, double v_max, int nu, int nv,
+                              double* uv);
+orc_mesh* orc_mesh2d_grid(double u_min, double u_max, double v_min, double v_max, int nu, int nv,
+                          const double* V, int axisymmetric_rz);
+orc_mesh* orc_mesh3d_load(const char* coords_file, const char* tets_file, double coord_unit_m,
+                          double potential_unit_V);
+void orc_mesh_free(orc_mesh* m);
+/* p: mesh coordinates (3 or 2). Returns 1 inside (bary filled), 0 outside; hint updated. */
+int orc_mesh_locate(const orc_mesh* m, const double* p, int32_t* hint, double* bary);
+double orc_mesh_potential(const orc_mesh* m, const double p[3], int32_t* hint);
+void orc_mesh_field(const orc_mesh* m, const double p[3], int32_t* hint, double E[3]);
+void orc_mesh_at(const orc_mesh* m, const double p[3], double E[3]);   /* thread-local hint */
+
 typedef struct {
     double radius, half_z;
     double efield[3];
@@ -177,12 +209,14 @@ typedef struct {
     double russian_roulette_survival;
     int32_t charge_stepping;             /* DetectorModel::ChargeStepping: 0 FirstOrderDkCap, 1 LemanSecondOrder */
     double charge_sampling_dk_frac;      /* LemanSecondOrder: sampling-step cap on |dk| / max(k_s, |k|) */
+    const orc_mesh* mesh;                /* mesh_field_3d / mesh_field_2d; NULL: the uniform efield */
 } orc_detector;
 
 void orc_detector_default(orc_detector* d);
 void orc_reflection_split(const orc_surface* s, double energy_eV, double out[3]); /* spec, diff, anh */
 double orc_intersect_exit(const orc_detector* d, const double p[3], const double v[3],
                           double n_in[3], int* surface);
+void orc_field_at(const orc_detector* d, const double p[3], double E[3]);   /* engine.hpp:151-155 */
 
 /* ---- Records: core/track.hpp, transport/engine.hpp:29-93 ---------------------------- */
 typedef struct {

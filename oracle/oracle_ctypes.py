@@ -91,6 +91,7 @@ class Detector(C.Structure):
         ("charge_max_dk_frac", dbl), ("charge_dt_cap_s", dbl), ("fermi_level_above_Ev_eV", dbl),
         ("russian_roulette_bounces", u32), ("russian_roulette_survival", dbl),
         ("charge_stepping", C.c_int), ("charge_sampling_dk_frac", dbl),
+        ("mesh", C.c_void_p),
     ]
 
 
@@ -314,6 +315,26 @@ def lib():
                                             C.c_void_p, C.POINTER(ChargeVec), C.POINTER(TrackVec),
                                             C.POINTER(PartitionResult)]
         L.orc_detector_default.argtypes = [C.POINTER(Detector)]
+        L.orc_mesh3d_new.restype = C.c_void_p
+        L.orc_mesh3d_new.argtypes = [C.POINTER(dbl), C.c_size_t, C.POINTER(dbl), C.POINTER(i32), C.c_size_t, dbl, dbl]
+        L.orc_mesh2d_new.restype = C.c_void_p
+        L.orc_mesh2d_new.argtypes = [C.POINTER(dbl), C.c_size_t, C.POINTER(dbl), C.POINTER(i32), C.c_size_t, C.c_int]
+        L.orc_mesh3d_grid_vertices.argtypes = [dbl, dbl, dbl, C.c_int, C.c_int, C.c_int, C.POINTER(dbl)]
+        L.orc_mesh3d_grid.restype = C.c_void_p
+        L.orc_mesh3d_grid.argtypes = [dbl, dbl, dbl, C.c_int, C.c_int, C.c_int, C.POINTER(dbl)]
+        L.orc_mesh2d_grid_vertices.argtypes = [dbl, dbl, dbl, dbl, C.c_int, C.c_int, C.POINTER(dbl)]
+        L.orc_mesh2d_grid.restype = C.c_void_p
+        L.orc_mesh2d_grid.argtypes = [dbl, dbl, dbl, dbl, C.c_int, C.c_int, C.POINTER(dbl), C.c_int]
+        L.orc_mesh3d_load.restype = C.c_void_p
+        L.orc_mesh3d_load.argtypes = [C.c_char_p, C.c_char_p, dbl, dbl]
+        L.orc_mesh_free.argtypes = [C.c_void_p]
+        L.orc_mesh_locate.restype = C.c_int
+        L.orc_mesh_locate.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl)]
+        L.orc_mesh_potential.restype = dbl
+        L.orc_mesh_potential.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(i32)]
+        L.orc_mesh_field.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl)]
+        L.orc_mesh_at.argtypes = [C.c_void_p, C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_field_at.argtypes = [C.POINTER(Detector), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_film_aluminum.argtypes = [C.POINTER(Film)]
         _lib = L
         check_layout(L)
@@ -366,6 +387,16 @@ def ref():
                                     C.c_int, C.POINTER(C.c_int), C.POINTER(dbl), C.c_int, C.POINTER(C.c_int),
                                     C.POINTER(dbl), C.POINTER(u64)]
         R.ref_partition.restype = C.c_int
+        q = [C.c_int, C.POINTER(dbl), i32, C.POINTER(i32), C.POINTER(dbl), C.POINTER(i32), C.POINTER(dbl),
+             C.POINTER(dbl)]
+        R.ref_mesh3d_query.argtypes = [C.POINTER(dbl), C.c_long, C.POINTER(dbl), C.POINTER(i32), C.c_long, dbl, dbl] + q
+        R.ref_mesh3d_grid_query.argtypes = [dbl, dbl, dbl, C.c_int, C.c_int, C.c_int, C.POINTER(dbl)] + q + [
+            C.POINTER(C.c_long)]
+        R.ref_mesh3d_load_query.argtypes = [C.c_char_p, C.c_char_p, dbl, dbl] + q + [C.POINTER(C.c_long)]
+        R.ref_mesh2d_query.argtypes = [C.POINTER(dbl), C.c_long, C.POINTER(dbl), C.POINTER(i32), C.c_long,
+                                       C.c_int] + q
+        R.ref_mesh2d_grid_query.argtypes = [dbl, dbl, dbl, dbl, C.c_int, C.c_int, C.POINTER(dbl), C.c_int] + q + [
+            C.POINTER(C.c_long)]
         _ref = R
     return _ref
 
@@ -491,6 +522,286 @@ def mask_amm_detector(fill: float = 0.9) -> Detector:
         set_pads(d.surf[i], qet_pads(d.radius, fill))
         set_amm(d.surf[i], GE_MEDIUM, AL_MEDIUM)
     return d
+
+
+def poly3(c, x, y, z):
+    """Potential c0 + c1 x + c2 y + c3 z + c4 x x + c5 y z + c6 z z (the form the reference
+    driver's grid potential evaluates, term by term in this order)."""
+    return c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * x + c[5] * y * z + c[6] * z * z
+
+
+def poly2(c, u, v):
+    """Potential c0 + c1 u + c2 v + c3 u u + c4 u v + c5 v v."""
+    return c[0] + c[1] * u + c[2] * v + c[3] * u * u + c[4] * u * v + c[5] * v * v
+
+
+def _query_buffers(pts, dim):
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    n = len(pts)
+    return pts, n, {"found": np.zeros(n, np.int32), "bary": np.zeros((n, dim + 1)), "hints": np.zeros(n, np.int32),
+                    "E": np.zeros((n, 3)), "pot": np.zeros(n)}
+
+
+def _qargs(pts, n, hint0, o):
+    return [n, dptr(pts), hint0, o["found"].ctypes.data_as(C.POINTER(i32)), dptr(o["bary"]),
+            o["hints"].ctypes.data_as(C.POINTER(i32)), dptr(o["E"]), dptr(o["pot"])]
+
+
+class Mesh:
+    """An oracle FEM mesh (orc_mesh): TriLinearMesh3D (dim 3) or BiLinearMesh2D (dim 2)."""
+
+    def __init__(self, ptr: int, dim: int, axisymmetric_rz: bool = False):
+        if not ptr:
+            raise RuntimeError("oracle mesh construction failed")
+        self.ptr, self.dim, self.axisymmetric_rz = ptr, dim, bool(axisymmetric_rz)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().orc_mesh_free(self.ptr)
+            self.ptr = None
+
+    @classmethod
+    def tet(cls, xyz, V, tets, v_min=-1e30, v_max=1e30):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        V = np.ascontiguousarray(V, np.float64)
+        tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+        return cls(lib().orc_mesh3d_new(dptr(xyz), len(xyz), dptr(V), tets.ctypes.data_as(C.POINTER(i32)),
+                                        len(tets), v_min, v_max), 3)
+
+    @classmethod
+    def tri(cls, uv, V, tris, axisymmetric_rz=True):
+        uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+        V = np.ascontiguousarray(V, np.float64)
+        tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        return cls(lib().orc_mesh2d_new(dptr(uv), len(uv), dptr(V), tris.ctypes.data_as(C.POINTER(i32)),
+                                        len(tris), int(axisymmetric_rz)), 2, axisymmetric_rz)
+
+    @staticmethod
+    def grid3d_vertices(hx, hy, hz, nx, ny, nz):
+        n = (max(1, nx) + 1) * (max(1, ny) + 1) * (max(1, nz) + 1)
+        out = np.zeros((n, 3))
+        lib().orc_mesh3d_grid_vertices(hx, hy, hz, nx, ny, nz, dptr(out))
+        return out
+
+    @classmethod
+    def grid3d(cls, hx, hy, hz, nx, ny, nz, coeffs):
+        x = cls.grid3d_vertices(hx, hy, hz, nx, ny, nz)
+        V = np.ascontiguousarray(poly3(coeffs, x[:, 0], x[:, 1], x[:, 2]))
+        return cls(lib().orc_mesh3d_grid(hx, hy, hz, nx, ny, nz, dptr(V)), 3)
+
+    @staticmethod
+    def grid2d_vertices(umin, umax, vmin, vmax, nu, nv):
+        n = (max(1, nu) + 1) * (max(1, nv) + 1)
+        out = np.zeros((n, 2))
+        lib().orc_mesh2d_grid_vertices(umin, umax, vmin, vmax, nu, nv, dptr(out))
+        return out
+
+    @classmethod
+    def grid2d(cls, umin, umax, vmin, vmax, nu, nv, coeffs, axisymmetric_rz=True):
+        x = cls.grid2d_vertices(umin, umax, vmin, vmax, nu, nv)
+        V = np.ascontiguousarray(poly2(coeffs, x[:, 0], x[:, 1]))
+        return cls(lib().orc_mesh2d_grid(umin, umax, vmin, vmax, nu, nv, dptr(V), int(axisymmetric_rz)), 2,
+                   axisymmetric_rz)
+
+    @classmethod
+    def load3d(cls, coords_file, tets_file, coord_unit_m=1.0, potential_unit_V=1.0):
+        return cls(lib().orc_mesh3d_load(str(coords_file).encode(), str(tets_file).encode(), coord_unit_m,
+                                         potential_unit_V), 3)
+
+    def query(self, pts, hint0=0):
+        """The reference driver's query sequence: per point, the walk from the carried hint
+        (found, bary, hint left), field_at and potential_at from the same hint."""
+        pts, n, o = _query_buffers(pts, self.dim)
+        L = lib()
+        h = int(hint0)
+        proj = np.zeros(2)
+        for i in range(n):
+            p = np.ascontiguousarray(pts[i])
+            if self.dim == 2:
+                proj[:] = (np.hypot(p[0], p[1]), p[2]) if self.axisymmetric_rz else (p[0], p[1])
+                q = proj
+            else:
+                q = p
+            b = np.zeros(4)
+            hf = i32(h)
+            o["found"][i] = L.orc_mesh_locate(self.ptr, dptr(q), C.byref(hf), dptr(b))
+            o["bary"][i] = b[:self.dim + 1]
+            o["hints"][i] = hf.value
+            he = i32(h)
+            E = np.zeros(3)
+            L.orc_mesh_field(self.ptr, dptr(p), C.byref(he), dptr(E))
+            o["E"][i] = E
+            hp = i32(h)
+            o["pot"][i] = L.orc_mesh_potential(self.ptr, dptr(p), C.byref(hp))
+            h = hf.value
+        return o
+
+
+_MESHES = []    # meshes referenced by detectors (kept alive for the process)
+
+
+def set_mesh(det: Detector, mesh: Mesh | None) -> None:
+    """DetectorModel::mesh_field_3d / mesh_field_2d (None: the uniform efield)."""
+    if mesh is None:
+        det.mesh = None
+        return
+    _MESHES.append(mesh)
+    det.mesh = mesh.ptr
+
+
+# Mesh-field cases shared by the golden writer and the tests. The 3D grid covers the
+# default crystal (R 3.81 cm, half-height 1.27 cm) with a margin; its potential is a bias
+# of about 4 V across the thickness plus quadratic terms of the same order.
+MESH3_GRID = ((0.0382, 0.0382, 0.0128), (12, 12, 6), (0.0, 0.0, 0.0, 157.48, 2000.0, -800.0, 5000.0))
+MESH2_GRID_RZ = ((0.0, 0.0382), (-0.0128, 0.0128), (16, 8), (0.0, 0.0, 157.48, 1500.0, 3000.0, -4000.0))
+MESH2_GRID_XY = ((-0.0382, 0.0382), (-0.0382, 0.0382), (10, 9), (0.5, 40.0, -90.0, 700.0, 300.0, -200.0))
+
+
+def mesh3_grid() -> Mesh:
+    (hx, hy, hz), (nx, ny, nz), c = MESH3_GRID
+    return Mesh.grid3d(hx, hy, hz, nx, ny, nz, c)
+
+
+def mesh2_grid_rz() -> Mesh:
+    (u0, u1), (v0, v1), (nu, nv), c = MESH2_GRID_RZ
+    return Mesh.grid2d(u0, u1, v0, v1, nu, nv, c, True)
+
+
+def irregular_mesh3(seed: int = 5):
+    """A tetrahedral mesh that is not a grid: the Kuhn grid of a 5 x 4 x 3 box with its
+    interior vertices moved by up to 0.2 cells, random potentials (some beyond the clamp
+    range +-30 V) and the vertex order of every tetrahedron shuffled. Returns (xyz, V, tets)."""
+    rng = np.random.default_rng(seed)
+    h, n = (0.02, 0.015, 0.01), (5, 4, 3)
+    xyz = Mesh.grid3d_vertices(*h, *n)
+    cell = np.array([2 * h[i] / n[i] for i in range(3)])
+    inner = np.all(np.abs(np.abs(xyz) - np.array(h)) > 1e-12, axis=1)
+    xyz[inner] += rng.uniform(-0.2, 0.2, (inner.sum(), 3)) * cell
+    V = rng.uniform(-40.0, 40.0, len(xyz))
+    vy, vz = n[1] + 1, n[2] + 1
+    tets = []
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for k in range(n[2]):
+                v = lambda a, b, c: ((i + a) * vy + (j + b)) * vz + (k + c)  # noqa: E731
+                for t in ((v(0, 0, 0), v(1, 0, 0), v(1, 1, 0), v(1, 1, 1)), (v(0, 0, 0), v(1, 0, 0), v(1, 0, 1), v(1, 1, 1)),
+                          (v(0, 0, 0), v(0, 1, 0), v(1, 1, 0), v(1, 1, 1)), (v(0, 0, 0), v(0, 1, 0), v(0, 1, 1), v(1, 1, 1)),
+                          (v(0, 0, 0), v(0, 0, 1), v(1, 0, 1), v(1, 1, 1)), (v(0, 0, 0), v(0, 0, 1), v(0, 1, 1), v(1, 1, 1))):
+                    tets.append(rng.permutation(t))
+    return xyz, V, np.array(tets, dtype=np.int32)
+
+
+def irregular_mesh2(seed: int = 6):
+    """An irregular planar triangle mesh: a 6 x 5 grid on [-2, 2] x [-1.5, 1.5] cm with its
+    interior vertices moved and each triangle's vertex order shuffled. Returns (uv, V, tris)."""
+    rng = np.random.default_rng(seed)
+    nu, nv = 6, 5
+    uv = Mesh.grid2d_vertices(-0.02, 0.02, -0.015, 0.015, nu, nv)
+    cell = np.array([0.04 / nu, 0.03 / nv])
+    inner = (np.abs(np.abs(uv[:, 0]) - 0.02) > 1e-12) & (np.abs(np.abs(uv[:, 1]) - 0.015) > 1e-12)
+    uv[inner] += rng.uniform(-0.2, 0.2, (inner.sum(), 2)) * cell
+    V = rng.uniform(-5.0, 5.0, len(uv))
+    tris = []
+    for i in range(nu):
+        for j in range(nv):
+            v00, v10, v01, v11 = i * (nv + 1) + j, (i + 1) * (nv + 1) + j, i * (nv + 1) + j + 1, (i + 1) * (nv + 1) + j + 1
+            tris.append(rng.permutation((v00, v10, v11)))
+            tris.append(rng.permutation((v00, v01, v11)))
+    return uv, V, np.array(tris, dtype=np.int32)
+
+
+def write_mesh_files(directory: str):
+    """The irregular 3D mesh written as the two text files of load_from_files, in mm and
+    mV (so that the loader's unit factors 1e-3 apply), with '#' and '%' comment lines and a
+    blank line. Returns (coords_path, tets_path)."""
+    xyz, V, tets = irregular_mesh3()
+    cp, tp = os.path.join(directory, "mesh_coords.txt"), os.path.join(directory, "mesh_tets.txt")
+    with open(cp, "w") as f:
+        f.write("# x y z V (mm, mV)\n% exported\n\n")
+        for (x, y, z), v in zip(xyz * 1e3, V * 1e3):
+            f.write(f"{float(x)!r} {float(y)!r} {float(z)!r} {float(v)!r}\n")
+    with open(tp, "w") as f:
+        f.write("# i0 i1 i2 i3\n")
+        for t in tets:
+            f.write(" ".join(str(int(a)) for a in t) + "\n")
+    return cp, tp
+
+
+def mesh_query_points(n: int, half, seed: int):
+    """n points uniform in the box |x_i| <= 1.1 half_i (some outside the mesh), then every
+    10th replaced by a grid-like special point (the origin, a face or an edge)."""
+    rng = np.random.default_rng(seed)
+    h = np.array(half, dtype=float)
+    p = rng.uniform(-1.1, 1.1, (n, 3)) * h
+    p[::10] = np.round(p[::10] / h * 4) / 4 * h
+    return p
+
+
+def mesh_runs():
+    """Run cases in mesh fields: (workload, events, seed, detector[, crystal])."""
+    runs = {}
+    d3 = default_detector()
+    set_bias(d3, 4.0)
+    d3.luke_sample_fraction = 0.0
+    set_mesh(d3, mesh3_grid())
+    runs["mesh3_pairs"] = (workload(WL_PAIR), 8, 42, d3)
+    d2 = default_detector()
+    d2.luke_sample_fraction = 0.02
+    set_mesh(d2, mesh2_grid_rz())
+    runs["mesh2_pairs"] = (workload(WL_PAIR), 8, 43, d2)
+    dl = default_detector()
+    dl.luke_sample_fraction = 0.0
+    dl.charge_stepping = 1
+    set_mesh(dl, mesh3_grid())
+    runs["mesh3_leman_ion"] = (workload(WL_PAIR), 6, 44, dl, trap_ion_crystal())
+    return runs
+
+
+def ref_mesh3d_query(xyz, V, tets, pts, hint0=0, v_min=-1e30, v_max=1e30):
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    V = np.ascontiguousarray(V, np.float64)
+    tets = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    pts, n, o = _query_buffers(pts, 3)
+    ref().ref_mesh3d_query(dptr(xyz), len(xyz), dptr(V), tets.ctypes.data_as(C.POINTER(i32)), len(tets), v_min, v_max,
+                           *_qargs(pts, n, hint0, o))
+    return o
+
+
+def ref_mesh3d_grid_query(hx, hy, hz, nx, ny, nz, coeffs, pts, hint0=0):
+    c = np.ascontiguousarray(coeffs, np.float64)
+    pts, n, o = _query_buffers(pts, 3)
+    counts = (C.c_long * 2)()
+    ref().ref_mesh3d_grid_query(hx, hy, hz, nx, ny, nz, dptr(c), *_qargs(pts, n, hint0, o), counts)
+    o["counts"] = (counts[0], counts[1])
+    return o
+
+
+def ref_mesh3d_load_query(coords_file, tets_file, cu, pu, pts, hint0=0):
+    pts, n, o = _query_buffers(pts, 3)
+    counts = (C.c_long * 2)()
+    ref().ref_mesh3d_load_query(str(coords_file).encode(), str(tets_file).encode(), cu, pu,
+                                *_qargs(pts, n, hint0, o), counts)
+    o["counts"] = (counts[0], counts[1])
+    return o
+
+
+def ref_mesh2d_query(uv, V, tris, axisym, pts, hint0=0):
+    uv = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+    V = np.ascontiguousarray(V, np.float64)
+    tris = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    pts, n, o = _query_buffers(pts, 2)
+    ref().ref_mesh2d_query(dptr(uv), len(uv), dptr(V), tris.ctypes.data_as(C.POINTER(i32)), len(tris), int(axisym),
+                           *_qargs(pts, n, hint0, o))
+    return o
+
+
+def ref_mesh2d_grid_query(umin, umax, vmin, vmax, nu, nv, coeffs, axisym, pts, hint0=0):
+    c = np.ascontiguousarray(coeffs, np.float64)
+    pts, n, o = _query_buffers(pts, 2)
+    counts = (C.c_long * 2)()
+    ref().ref_mesh2d_grid_query(umin, umax, vmin, vmax, nu, nv, dptr(c), int(axisym), *_qargs(pts, n, hint0, o), counts)
+    o["counts"] = (counts[0], counts[1])
+    return o
 
 
 def workload(kind: int, **kw) -> Workload:

@@ -546,7 +546,8 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
         const double q_sign = (trk.type == 1) ? -1.0 : 1.0;
         for (uint32_t step = 0; step < 20000000u; ++step) {
             st->charge_steps++;
-            const double* E = D->efield;
+            double E[3];
+            orc_field_at(D, trk.pos, E);   /* engine.hpp:213: once per step, at the step's start */
             const int second_order = (D->charge_stepping == 1);
             const double dk_frac = second_order ? D->charge_sampling_dk_frac : D->charge_max_dk_frac;
             const rates_t R0 = charge_rates(e, &trk, E);

@@ -68,6 +68,62 @@ SurfaceProperty to_surface(const orc_surface& s) {
     return p;
 }
 
+geometry::TriLinearMesh3D mesh3_from(const double* xyz, long nv, const double* V, const int32_t* tets, long nt,
+                                     double vmin, double vmax) {
+    std::vector<Vec3> x;
+    for (long i = 0; i < nv; ++i) x.push_back(Vec3{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]});
+    std::vector<geometry::TriLinearMesh3D::Tet> t;
+    for (long i = 0; i < nt; ++i) t.push_back({tets[4 * i], tets[4 * i + 1], tets[4 * i + 2], tets[4 * i + 3]});
+    return geometry::TriLinearMesh3D(std::move(x), std::vector<double>(V, V + nv), std::move(t), vmin, vmax);
+}
+
+geometry::BiLinearMesh2D mesh2_from(const double* uv, long nv, const double* V, const int32_t* tris, long nt,
+                                    int axisym) {
+    std::vector<geometry::BiLinearMesh2D::Point2D> x;
+    for (long i = 0; i < nv; ++i) x.push_back({uv[2 * i], uv[2 * i + 1]});
+    std::vector<geometry::BiLinearMesh2D::Tri> t;
+    for (long i = 0; i < nt; ++i) t.push_back({tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]});
+    return geometry::BiLinearMesh2D(std::move(x), std::vector<double>(V, V + nv), std::move(t), axisym != 0);
+}
+
+// n points queried in order, the walk hint carried from each point to the next (starting
+// at hint0): find_tetrahedron (found, bary, the hint it leaves), field_at, potential_at.
+void query3(const geometry::TriLinearMesh3D& g, int n, const double* pts, int32_t hint0, int32_t* found,
+            double* bary, int32_t* hints, double* E, double* pot) {
+    int h = hint0;
+    for (int i = 0; i < n; ++i) {
+        const Vec3 p{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        std::array<double, 4> b{};
+        int hf = h, he = h, hp = h;
+        found[i] = g.find_tetrahedron(p, hf, b) ? 1 : 0;
+        for (int k = 0; k < 4; ++k) bary[4 * i + k] = b[k];
+        hints[i] = hf;
+        const Vec3 e = g.field_at(p, he);
+        E[3 * i] = e.x; E[3 * i + 1] = e.y; E[3 * i + 2] = e.z;
+        pot[i] = g.potential_at(p, hp);
+        h = hf;
+    }
+}
+
+void query2(const geometry::BiLinearMesh2D& g, int axisym, int n, const double* pts, int32_t hint0, int32_t* found,
+            double* bary, int32_t* hints, double* E, double* pot) {
+    int h = hint0;
+    for (int i = 0; i < n; ++i) {
+        const Vec3 p{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        const geometry::BiLinearMesh2D::Point2D q = axisym ? geometry::BiLinearMesh2D::Point2D{std::hypot(p.x, p.y), p.z}
+                                                           : geometry::BiLinearMesh2D::Point2D{p.x, p.y};
+        std::array<double, 3> b{};
+        int hf = h, he = h, hp = h;
+        found[i] = g.find_triangle(q, hf, b) ? 1 : 0;
+        for (int k = 0; k < 3; ++k) bary[3 * i + k] = b[k];
+        hints[i] = hf;
+        const Vec3 e = g.field_at(p, he);
+        E[3 * i] = e.x; E[3 * i + 1] = e.y; E[3 * i + 2] = e.z;
+        pot[i] = g.potential_at(p, hp);
+        h = hf;
+    }
+}
+
 DetectorModel to_detector(const orc_detector* d) {
     DetectorModel m;
     m.solid.radius = d->radius;
@@ -85,6 +141,13 @@ DetectorModel to_detector(const orc_detector* d) {
     m.charge_stepping = d->charge_stepping ? DetectorModel::ChargeStepping::LemanSecondOrder
                                            : DetectorModel::ChargeStepping::FirstOrderDkCap;
     m.charge_sampling_dk_frac = d->charge_sampling_dk_frac;
+    if (d->mesh) {
+        const orc_mesh* g = d->mesh;
+        if (g->dim == 3) m.mesh_field_3d = mesh3_from(g->x, static_cast<long>(g->n_vertices), g->v, g->cells,
+                                                      static_cast<long>(g->n_cells), -1e30, 1e30);
+        else m.mesh_field_2d = mesh2_from(g->x, static_cast<long>(g->n_vertices), g->v, g->cells,
+                                          static_cast<long>(g->n_cells), g->axisymmetric_rz);
+    }
     return m;
 }
 
@@ -486,6 +549,56 @@ int ref_partition(double E, int nuclear, const orc_partition_opts* o, uint64_t s
         std::memcpy(ph + 14 * i, row, sizeof(row));
     }
     return 0;
+}
+
+// TriLinearMesh3D(vertices, potentials, tets, v_min_clamp, v_max_clamp), queried at n points.
+void ref_mesh3d_query(const double* xyz, long nv, const double* V, const int32_t* tets, long nt, double vmin,
+                      double vmax, int n, const double* pts, int32_t hint0, int32_t* found, double* bary,
+                      int32_t* hints, double* E, double* pot) {
+    query3(mesh3_from(xyz, nv, V, tets, nt, vmin, vmax), n, pts, hint0, found, bary, hints, E, pot);
+}
+
+// TriLinearMesh3D::build_cartesian_grid with V(x, y, z) = c0 + c1 x + c2 y + c3 z + c4 x x
+// + c5 y z + c6 z z, queried at n points; counts = {vertices, tetrahedra}.
+void ref_mesh3d_grid_query(double hx, double hy, double hz, int nx, int ny, int nz, const double* c, int n,
+                           const double* pts, int32_t hint0, int32_t* found, double* bary, int32_t* hints,
+                           double* E, double* pot, long* counts) {
+    const auto g = geometry::TriLinearMesh3D::build_cartesian_grid(hx, hy, hz, nx, ny, nz, [c](const Vec3& p) {
+        return c[0] + c[1] * p.x + c[2] * p.y + c[3] * p.z + c[4] * p.x * p.x + c[5] * p.y * p.z + c[6] * p.z * p.z;
+    });
+    counts[0] = static_cast<long>(g.num_vertices());
+    counts[1] = static_cast<long>(g.num_tetrahedra());
+    query3(g, n, pts, hint0, found, bary, hints, E, pot);
+}
+
+// TriLinearMesh3D::load_from_files, queried at n points; counts = {vertices, tetrahedra}.
+void ref_mesh3d_load_query(const char* coords, const char* tets, double cu, double pu, int n, const double* pts,
+                           int32_t hint0, int32_t* found, double* bary, int32_t* hints, double* E, double* pot,
+                           long* counts) {
+    const auto g = geometry::TriLinearMesh3D::load_from_files(coords, tets, cu, pu);
+    counts[0] = static_cast<long>(g.num_vertices());
+    counts[1] = static_cast<long>(g.num_tetrahedra());
+    query3(g, n, pts, hint0, found, bary, hints, E, pot);
+}
+
+// BiLinearMesh2D(vertices, potentials, triangles, axisymmetric_rz), queried at n 3D points
+// (find_triangle at their (r, z) or (x, y) projection).
+void ref_mesh2d_query(const double* uv, long nv, const double* V, const int32_t* tris, long nt, int axisym, int n,
+                      const double* pts, int32_t hint0, int32_t* found, double* bary, int32_t* hints, double* E,
+                      double* pot) {
+    query2(mesh2_from(uv, nv, V, tris, nt, axisym), axisym, n, pts, hint0, found, bary, hints, E, pot);
+}
+
+// BiLinearMesh2D::build_grid with V(u, v) = c0 + c1 u + c2 v + c3 u u + c4 u v + c5 v v.
+void ref_mesh2d_grid_query(double umin, double umax, double vmin, double vmax, int nu, int nv, const double* c,
+                           int axisym, int n, const double* pts, int32_t hint0, int32_t* found, double* bary,
+                           int32_t* hints, double* E, double* pot, long* counts) {
+    const auto g = geometry::BiLinearMesh2D::build_grid(umin, umax, vmin, vmax, nu, nv, [c](double u, double v) {
+        return c[0] + c[1] * u + c[2] * v + c[3] * u * u + c[4] * u * v + c[5] * v * v;
+    }, axisym != 0);
+    counts[0] = static_cast<long>(g.num_vertices());
+    counts[1] = static_cast<long>(g.num_triangles());
+    query2(g, axisym, n, pts, hint0, found, bary, hints, E, pot);
 }
 
 }  // extern "C"

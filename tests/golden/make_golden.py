@@ -160,6 +160,8 @@ def main():
     runs["deposit_er"] = (O.workload(O.WL_DEPOSIT, mode=0, energy_eV=30.0, origin=(1e-3, -2e-3, 4e-3), pair_at_origin=1,
                                      partition=dict(max_sampled_phonons=10, max_sampled_pairs=4)), 3, 7, dp)
     partition_goldens(R, out)
+    mesh_goldens(out)
+    runs.update(O.mesh_runs())
     for tag, spec in runs.items():
         wl, n, seed, d = spec[:4]
         cr = spec[4] if len(spec) > 4 else None
@@ -175,6 +177,45 @@ def main():
     path = os.path.join(HERE, "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print(path, os.path.getsize(path), "bytes")
+
+
+def mesh_goldens(out):
+    """geometry/mesh_field.hpp: point location, field and potential of the reference's
+    meshes at query points (the walk hint carried from point to point)."""
+    import tempfile
+    keys = ("found", "bary", "hints", "E", "pot")
+    (hx, hy, hz), (nx, ny, nz), c3 = O.MESH3_GRID
+    pts3 = O.mesh_query_points(400, (hx, hy, hz), 11)
+    out["mesh_pts3"] = pts3
+    r = O.ref_mesh3d_grid_query(hx, hy, hz, nx, ny, nz, c3, pts3, 0)
+    for k in keys:
+        out[f"mesh3_grid_{k}"] = r[k]
+    out["mesh3_grid_counts"] = np.array(r["counts"])
+    xyz, V, tets = O.irregular_mesh3()
+    ptsi = O.mesh_query_points(300, (0.02, 0.015, 0.01), 12)
+    out["mesh_pts3i"] = ptsi
+    r = O.ref_mesh3d_query(xyz, V, tets, ptsi, -1, -30.0, 30.0)
+    for k in keys:
+        out[f"mesh3_irr_{k}"] = r[k]
+    with tempfile.TemporaryDirectory() as d:
+        cp, tp = O.write_mesh_files(d)
+        r = O.ref_mesh3d_load_query(cp, tp, 1e-3, 1e-3, ptsi, 7)
+    for k in keys:
+        out[f"mesh3_file_{k}"] = r[k]
+    out["mesh3_file_counts"] = np.array(r["counts"])
+    (u0, u1), (v0, v1), (nu, nv), c2 = O.MESH2_GRID_RZ
+    r = O.ref_mesh2d_grid_query(u0, u1, v0, v1, nu, nv, c2, True, pts3, 0)
+    for k in keys:
+        out[f"mesh2_rz_{k}"] = r[k]
+    out["mesh2_rz_counts"] = np.array(r["counts"])
+    (u0, u1), (v0, v1), (nu, nv), c2 = O.MESH2_GRID_XY
+    r = O.ref_mesh2d_grid_query(u0, u1, v0, v1, nu, nv, c2, False, pts3, 5)
+    for k in keys:
+        out[f"mesh2_xy_{k}"] = r[k]
+    uv, V2, tris = O.irregular_mesh2()
+    r = O.ref_mesh2d_query(uv, V2, tris, False, ptsi, 2)
+    for k in keys:
+        out[f"mesh2_irr_{k}"] = r[k]
 
 
 PARTITION_CASES = [

@@ -204,7 +204,7 @@ def _gold_runs(O):
     dp = O.default_detector()
     O.set_bias(dp, 4.0)
     dp.luke_sample_fraction = 0.05
-    return {
+    return O.mesh_runs() | {
         "deposit_nr": (O.workload(O.WL_DEPOSIT, mode=1, energy_eV=100.0,
                                   partition=dict(max_sampled_phonons=12, max_sampled_pairs=8)), 4, 42, dp),
         "deposit_er": (O.workload(O.WL_DEPOSIT, mode=0, energy_eV=30.0, origin=(1e-3, -2e-3, 4e-3), pair_at_origin=1,
@@ -226,14 +226,16 @@ def _gold_runs(O):
 
 @pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo", "bench_dos", "roulette",
                                  "specular", "trap_ion", "leman", "leman_ion", "c1_exact", "deposit_nr",
-                                 "deposit_er"])
+                                 "deposit_er", "mesh3_pairs", "mesh2_pairs", "mesh3_leman_ion"])
 def test_run_events_golden(oracle, gold, tag):
     """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's
     (c1_exact: with KinematicsMode::Exact; bench_dos: the reference benchmark's own phonon
     workload; roulette, specular, trap_ion: Russian roulette, half-specular surfaces, trap
     impact ionisation; leman, leman_ion: Leman second-order charge stepping; deposit_nr,
     deposit_er: generators that partition a nuclear / electron recoil with
-    EnergyPartition::partition_deposit, transported at 4 V)."""
+    EnergyPartition::partition_deposit, transported at 4 V; mesh3_pairs, mesh2_pairs,
+    mesh3_leman_ion: pairs drifting in FEM mesh fields, a tetrahedral grid and an
+    axisymmetric triangle grid)."""
     O = oracle
     spec = _gold_runs(O)[tag]
     wl, n, seed, det = spec[:4]
