@@ -31,7 +31,8 @@ __all__ = ["Lattice", "Engine", "DiffTape", "apply_diff_params", "Rng", "germani
            "constant_split_surfaces", "set_pads", "clear_pads", "qet_mask", "grid_mask", "find_channel", "medium",
            "medium_from_bulk_shear", "medium_from_lame", "set_amm", "amm_transmission", "amm_psv", "WL_DEPOSIT",
            "partition_opts", "ionization_yield", "lindhard_yield", "sample_fano_binomial", "partition_deposit",
-           "NIEL_LINDHARD", "NIEL_LEWIN_SMITH", "NIEL_SARKIS", "NIEL_IMPACT_TUNL", "NIEL_EMPIRICAL"]
+           "NIEL_LINDHARD", "NIEL_LEWIN_SMITH", "NIEL_SARKIS", "NIEL_IMPACT_TUNL", "NIEL_EMPIRICAL", "Mesh",
+           "set_mesh"]
 
 MASK64 = (1 << 64) - 1
 
@@ -229,6 +230,158 @@ def amm_psv(m1, m2, incident_P: bool, theta1: float) -> np.ndarray:
     return np.array(out[:])
 
 
+# ------------------------------------------------------------------ FEM mesh fields
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Mesh:
+    """An FEM mesh field (geometry/mesh_field.hpp TriLinearMesh3D, dim 3, or
+    BiLinearMesh2D, dim 2) held by the library (cmpmc_mesh). Give it to a detector with
+    set_mesh: the charge step then reads its field instead of the uniform one."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+        dim, ax, nv, nc = C.c_int(), C.c_int(), C.c_uint64(), C.c_uint64()
+        check(capi.load().cmpmc_mesh_info(self._h, C.byref(dim), C.byref(ax), C.byref(nv), C.byref(nc)))
+        self.dim, self.axisymmetric_rz = dim.value, bool(ax.value)
+        self.num_vertices, self.num_cells = nv.value, nc.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            capi.load().cmpmc_mesh_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @staticmethod
+    def _made(fn, *args) -> "Mesh":
+        out = C.c_void_p()
+        check(fn(*args, C.byref(out)))
+        return Mesh(out.value)
+
+    @classmethod
+    def tet(cls, xyz, potentials, tets, v_min_clamp: float = -1e30, v_max_clamp: float = 1e30) -> "Mesh":
+        """TriLinearMesh3D(vertices, potentials, tetrahedra, V_min_clamp, V_max_clamp)."""
+        x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(potentials, np.float64)
+        t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+        return cls._made(capi.load().cmpmc_mesh3d_create, _dp(x), len(x), _dp(v), _ip(t), len(t), v_min_clamp,
+                         v_max_clamp)
+
+    @classmethod
+    def tri(cls, uv, potentials, tris, axisymmetric_rz: bool = True) -> "Mesh":
+        """BiLinearMesh2D(vertices (u, v), potentials, triangles, axisymmetric_rz)."""
+        x = np.ascontiguousarray(uv, np.float64).reshape(-1, 2)
+        v = np.ascontiguousarray(potentials, np.float64)
+        t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+        return cls._made(capi.load().cmpmc_mesh2d_create, _dp(x), len(x), _dp(v), _ip(t), len(t),
+                         int(axisymmetric_rz))
+
+    @staticmethod
+    def grid3d_vertices(half_x, half_y, half_z, nx, ny, nz) -> np.ndarray:
+        n = (max(1, nx) + 1) * (max(1, ny) + 1) * (max(1, nz) + 1)
+        out = np.zeros((n, 3))
+        check(capi.load().cmpmc_mesh3d_grid_vertices(half_x, half_y, half_z, nx, ny, nz, _dp(out)))
+        return out
+
+    @classmethod
+    def grid3d(cls, half_x, half_y, half_z, nx, ny, nz, potential) -> "Mesh":
+        """TriLinearMesh3D::build_cartesian_grid: `potential` is a callable V(x, y, z)
+        (vectorised over the vertex arrays) or the vertex potentials themselves."""
+        x = cls.grid3d_vertices(half_x, half_y, half_z, nx, ny, nz)
+        v = potential(x[:, 0], x[:, 1], x[:, 2]) if callable(potential) else potential
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (len(x),)))
+        return cls._made(capi.load().cmpmc_mesh3d_grid, half_x, half_y, half_z, nx, ny, nz, _dp(v))
+
+    @staticmethod
+    def grid2d_vertices(u_min, u_max, v_min, v_max, nu, nv) -> np.ndarray:
+        n = (max(1, nu) + 1) * (max(1, nv) + 1)
+        out = np.zeros((n, 2))
+        check(capi.load().cmpmc_mesh2d_grid_vertices(u_min, u_max, v_min, v_max, nu, nv, _dp(out)))
+        return out
+
+    @classmethod
+    def grid2d(cls, u_min, u_max, v_min, v_max, nu, nv, potential, axisymmetric_rz: bool = True) -> "Mesh":
+        """BiLinearMesh2D::build_grid: `potential` a callable V(u, v) or the vertex potentials."""
+        x = cls.grid2d_vertices(u_min, u_max, v_min, v_max, nu, nv)
+        v = potential(x[:, 0], x[:, 1]) if callable(potential) else potential
+        v = np.ascontiguousarray(np.broadcast_to(np.asarray(v, np.float64), (len(x),)))
+        return cls._made(capi.load().cmpmc_mesh2d_grid, u_min, u_max, v_min, v_max, nu, nv, _dp(v),
+                         int(axisymmetric_rz))
+
+    @classmethod
+    def load3d(cls, coords_file, tets_file, coord_unit_m: float = 1.0, potential_unit_V: float = 1.0) -> "Mesh":
+        """TriLinearMesh3D::load_from_files."""
+        return cls._made(capi.load().cmpmc_mesh3d_load, str(coords_file).encode(), str(tets_file).encode(),
+                         coord_unit_m, potential_unit_V)
+
+    def locate(self, p, hint: int = 0):
+        """find_tetrahedron / find_triangle at mesh coordinates p: (inside, bary, hint)."""
+        q = np.ascontiguousarray(p, np.float64)
+        b = np.zeros(self.dim + 1)
+        h = C.c_int32(hint)
+        r = capi.load().cmpmc_mesh_locate(self._h, _dp(q), C.byref(h), _dp(b))
+        if r < 0:
+            check(r)
+        return bool(r), b, h.value
+
+    def field_at(self, p, hint: int = 0):
+        """field_at: (E [V/m], hint) at a 3D position."""
+        q = np.ascontiguousarray(p, np.float64)
+        E = np.zeros(3)
+        h = C.c_int32(hint)
+        check(capi.load().cmpmc_mesh_field(self._h, _dp(q), C.byref(h), _dp(E)))
+        return E, h.value
+
+    def potential_at(self, p, hint: int = 0):
+        """potential_at: (V, hint) at a 3D position."""
+        q = np.ascontiguousarray(p, np.float64)
+        h = C.c_int32(hint)
+        v = capi.load().cmpmc_mesh_potential(self._h, _dp(q), C.byref(h))
+        return float(v), h.value
+
+    def query(self, pts, hint0: int = 0) -> dict:
+        """Per point, in order, with the walk hint carried from point to point: the walk at
+        the point's mesh coordinates (found, bary, the hint it leaves), field_at and
+        potential_at from the same start."""
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = len(pts)
+        o = {"found": np.zeros(n, np.int32), "bary": np.zeros((n, self.dim + 1)), "hints": np.zeros(n, np.int32),
+             "E": np.zeros((n, 3)), "pot": np.zeros(n)}
+        h = int(hint0)
+        for i, p in enumerate(pts):
+            if self.dim == 2:
+                # the C library's hypot, as the field query's projection uses
+                q = (float(np.hypot(p[0], p[1])), p[2]) if self.axisymmetric_rz else (p[0], p[1])
+            else:
+                q = p
+            inside, b, hf = self.locate(q, h)
+            o["found"][i], o["bary"][i], o["hints"][i] = int(inside), b, hf
+            o["E"][i] = self.field_at(p, h)[0]
+            o["pot"][i] = self.potential_at(p, h)[0]
+            h = hf
+        return o
+
+
+def set_mesh(det: capi.Detector, mesh: "Mesh | None") -> None:
+    """DetectorModel::mesh_field_3d / mesh_field_2d (None: the uniform efield). A model
+    copies the mesh at creation; keep `mesh` alive until then."""
+    det.mesh = None if mesh is None else mesh.handle.value
+    if mesh is not None:
+        _MESHES.append(mesh)
+
+
+_MESHES: list = []     # meshes a detector points at (kept alive with the process)
+
+
 def set_bias(det: capi.Detector, volts: float, thickness_m: float = 2.54e-2) -> None:
     """UniformElectricField::from_bias(V, d) = (0, 0, -V/d) (charge_transport.hpp:22-24)."""
     det.efield[0], det.efield[1], det.efield[2] = 0.0, 0.0, -volts / thickness_m
@@ -419,6 +572,18 @@ class Engine:
         check(capi.load().cmpmc_model_map_k_to_vg(self.ptr, mode, form, k.shape[0], k.ctypes.data, d.ctypes.data,
                                                   m.ctypes.data, None))
         return d, m
+
+    def field_at(self, pts, form: int = FORM_PLAIN, start=None) -> dict:
+        """The electric field as the charge step evaluates it on the device, at 3D points:
+        {"inside", "cell", "E"}; with a mesh each walk starts at start[i] (None: the cell a
+        new carrier starts from)."""
+        p = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = len(p)
+        st = None if start is None else np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.int32), (n,)))
+        inside, cell, E = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros((n, 3))
+        check(capi.load().cmpmc_model_field_at(self.ptr, form, n, p.ctypes.data, None if st is None else st.ctypes.data,
+                                               inside.ctypes.data, cell.ctypes.data, E.ctypes.data, None))
+        return {"inside": inside, "cell": cell, "E": E}
 
     def trim(self) -> None:
         check(capi.load().cmpmc_model_trim(self.ptr))

@@ -73,7 +73,8 @@ class Detector(C.Structure):
                 ("max_bounces", u32), ("russian_roulette_bounces", u32)] + [(n, dbl) for n in (
                     "min_phonon_eV", "luke_sample_fraction", "charge_max_dk_frac", "charge_dt_cap_s",
                     "fermi_level_above_Ev_eV", "russian_roulette_survival")] + [
-                ("charge_stepping", i32), ("_pad_cs", i32), ("charge_sampling_dk_frac", dbl)]
+                ("charge_stepping", i32), ("_pad_cs", i32), ("charge_sampling_dk_frac", dbl),
+                ("mesh", C.c_void_p)]
 
 
 class Lattice(C.Structure):
@@ -257,6 +258,23 @@ def load():
     L.cmpmc_partition_deposit.argtypes = [C.POINTER(Lattice), C.POINTER(dbl), dbl, dbl, C.c_int, C.POINTER(PartitionOpts),
                                           C.POINTER(u64), C.c_void_p, u64, C.POINTER(u64), C.c_void_p, u64,
                                           C.POINTER(u64), C.POINTER(PartitionResult)]
+    ip = C.POINTER(i32)
+    L.cmpmc_mesh3d_create.argtypes = [C.POINTER(dbl), u64, C.POINTER(dbl), ip, u64, dbl, dbl, C.POINTER(C.c_void_p)]
+    L.cmpmc_mesh2d_create.argtypes = [C.POINTER(dbl), u64, C.POINTER(dbl), ip, u64, C.c_int, C.POINTER(C.c_void_p)]
+    L.cmpmc_mesh3d_grid_vertices.argtypes = [dbl, dbl, dbl, C.c_int, C.c_int, C.c_int, C.POINTER(dbl)]
+    L.cmpmc_mesh3d_grid.argtypes = [dbl, dbl, dbl, C.c_int, C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(C.c_void_p)]
+    L.cmpmc_mesh2d_grid_vertices.argtypes = [dbl, dbl, dbl, dbl, C.c_int, C.c_int, C.POINTER(dbl)]
+    L.cmpmc_mesh2d_grid.argtypes = [dbl, dbl, dbl, dbl, C.c_int, C.c_int, C.POINTER(dbl), C.c_int,
+                                    C.POINTER(C.c_void_p)]
+    L.cmpmc_mesh3d_load.argtypes = [C.c_char_p, C.c_char_p, dbl, dbl, C.POINTER(C.c_void_p)]
+    L.cmpmc_mesh_destroy.argtypes = [C.c_void_p]
+    L.cmpmc_mesh_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(u64), C.POINTER(u64)]
+    L.cmpmc_mesh_locate.argtypes = [C.c_void_p, C.POINTER(dbl), ip, C.POINTER(dbl)]
+    L.cmpmc_mesh_field.argtypes = [C.c_void_p, C.POINTER(dbl), ip, C.POINTER(dbl)]
+    L.cmpmc_mesh_potential.argtypes = [C.c_void_p, C.POINTER(dbl), ip]
+    L.cmpmc_mesh_potential.restype = dbl
+    L.cmpmc_model_field_at.argtypes = [C.c_void_p, C.c_int, u64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p]
     check_abi(L)
     _lib = L
     return L
@@ -285,5 +303,8 @@ EXPORTED_SYMBOLS = [
     "cmpmc_plan_recordings", "cmpmc_plan_destroy", "cmpmc_autotune", "cmpmc_qet_mask", "cmpmc_grid_mask",
     "cmpmc_find_channel", "cmpmc_amm_normal", "cmpmc_amm_sh", "cmpmc_dmm", "cmpmc_amm_psv", "cmpmc_amm_transmission",
     "cmpmc_run_workload_noalloc", "cmpmc_partition_opts_default", "cmpmc_ionization_yield", "cmpmc_lindhard_yield",
-    "cmpmc_sample_fano_binomial", "cmpmc_partition_deposit",
+    "cmpmc_sample_fano_binomial", "cmpmc_partition_deposit", "cmpmc_mesh3d_create", "cmpmc_mesh2d_create",
+    "cmpmc_mesh3d_grid_vertices", "cmpmc_mesh3d_grid", "cmpmc_mesh2d_grid_vertices", "cmpmc_mesh2d_grid",
+    "cmpmc_mesh3d_load", "cmpmc_mesh_destroy", "cmpmc_mesh_info", "cmpmc_mesh_locate", "cmpmc_mesh_field",
+    "cmpmc_mesh_potential", "cmpmc_model_field_at",
 ]

@@ -20,6 +20,7 @@
 #include "capi_internal.h"
 #include "cmpmc_amd.h"
 #include "kernels_common.hpp"
+#include "mesh_host.hpp"
 #include "transport_kernels.h"
 
 using namespace cmpmc_host;
@@ -264,6 +265,44 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     fill_fp32_constants(m);
 }
 
+// The detector's FEM mesh field on the device (one allocation: FP64 records, FP32 records,
+// neighbours, seed cells) and its DevMesh (engine.hpp:151-155: the mesh replaces the
+// uniform field in the charge step).
+static int upload_mesh(cmpmc_model* M, const cmpmc_crystal& c, const cmpmc_mesh& mesh) {
+    cmpmc_internal::MeshImage im;
+    const cmpmc_internal::IvParams iv{c.iv_model, c.iv_quad_rate, c.iv_quad_field, c.iv_quad_power,
+                                      c.iv_lin_rate0, c.iv_lin_rate1, c.iv_lin_power};
+    cmpmc_internal::mesh_image(mesh, iv, c.n_valleys < 2, im);
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    const size_t b64 = im.rec64.size() * sizeof(double), b32 = im.rec32.size() * sizeof(float),
+                 bn = im.nbr4.size() * sizeof(int32_t), bs = im.seed.size() * sizeof(int32_t);
+    const size_t o32 = up(b64), on = o32 + up(b32), os = on + up(bn), total = os + up(bs);
+    HIPCHK(hipMalloc(&M->d_mesh, total));
+    if (b64) HIPCHK(hipMemcpy(M->d_mesh, im.rec64.data(), b64, hipMemcpyHostToDevice));
+    if (b32) HIPCHK(hipMemcpy(M->d_mesh + o32, im.rec32.data(), b32, hipMemcpyHostToDevice));
+    if (bn) HIPCHK(hipMemcpy(M->d_mesh + on, im.nbr4.data(), bn, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(M->d_mesh + os, im.seed.data(), bs, hipMemcpyHostToDevice));
+    DevMesh& g = M->dm.mesh;
+    g.rec = reinterpret_cast<const double*>(M->d_mesh);
+    g.rec_f = reinterpret_cast<const float*>(M->d_mesh + o32);
+    g.nbr = reinterpret_cast<const int32_t*>(M->d_mesh + on);
+    g.seed = reinterpret_cast<const int32_t*>(M->d_mesh + os);
+    g.dim = mesh.dim;
+    g.axisym = mesh.axisymmetric_rz;
+    g.n_cells = static_cast<int32_t>(mesh.n_cells());
+    for (int i = 0; i < 3; ++i) {
+        g.seed_n[i] = im.seed_n[i];
+        g.lo[i] = im.lo[i];
+        g.inv_h[i] = im.inv_h[i];
+        g.lo_f[i] = static_cast<float>(im.lo[i]);
+        g.inv_h_f[i] = static_cast<float>(im.inv_h[i]);
+    }
+    g.iv_zero = im.iv_zero;
+    g.iv_zero_f = (c.n_valleys < 2) ? 0.0f : static_cast<float>(im.iv_zero);
+    g.tol_f = im.tol_f;
+    return CMPMC_OK;
+}
+
 static int check_detector(const cmpmc_detector* d) {
     if (!(d->radius > 0.0) || !(d->half_z > 0.0)) return fail(CMPMC_ERR_INVALID, "detector: radius and half_z must be positive");
     if (d->charge_stepping != 0 && d->charge_stepping != 1)
@@ -376,9 +415,14 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
         cmpmc_model_destroy(M);
         return CMPMC_ERR_HIP;
     }
-    // the model keeps no pointer into the caller's pad arrays
+    // the model keeps no pointer into the caller's pad arrays or mesh
     for (int s = 0; s < 3; ++s) M->det.surf[s].pads = nullptr;
+    M->det.mesh = nullptr;
     fill_dev_model(M, lat, det);
+    if (det->mesh && upload_mesh(M, lat->cfg, *det->mesh) != CMPMC_OK) {
+        cmpmc_model_destroy(M);
+        return CMPMC_ERR_HIP;
+    }
     *out = M;
     return CMPMC_OK;
 }
@@ -404,6 +448,7 @@ extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
     if (M->d_stiff) (void)hipFree(M->d_stiff);
     if (M->d_pads) (void)hipFree(M->d_pads);
     if (M->d_amm) (void)hipFree(M->d_amm);
+    if (M->d_mesh) (void)hipFree(M->d_mesh);
     if (M->h_cancel) (void)hipHostFree(const_cast<int32_t*>(M->h_cancel));
     delete M;
     return CMPMC_OK;
@@ -511,6 +556,37 @@ extern "C" int cmpmc_model_map_k_to_vg(cmpmc_model* M, int mode, int form, uint6
                 vg_mag[i] = out[4 * i + 3];
             }
         }
+    }
+    buf.release();
+    return rc;
+}
+
+extern "C" int cmpmc_model_field_at(cmpmc_model* M, int form, uint64_t n, const double* pts, const int32_t* start,
+                                    int32_t* inside, int32_t* cell, double* E, void* stream) {
+    if (!M || (n && (!pts || !inside || !cell || !E)) || (form != CMPMC_FORM_PLAIN && form != CMPMC_FORM_BATCHED))
+        return fail(CMPMC_ERR_INVALID, "cmpmc_model_field_at: bad argument");
+    if (n == 0) return CMPMC_OK;
+    std::lock_guard<std::mutex> lk(M->mu);
+    cmpmc_internal::clear_error();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    DevBuf buf;
+    CHK(ensure(buf, n * (6 * sizeof(double) + 3 * sizeof(int32_t))));
+    double* d_p = static_cast<double*>(buf.p);
+    double* d_E = d_p + 3 * n;
+    int32_t* d_start = reinterpret_cast<int32_t*>(d_E + 3 * n);
+    int32_t* d_in = d_start + n;
+    int32_t* d_cell = d_in + n;
+    int rc = CMPMC_OK;
+    const auto launch = (form == CMPMC_FORM_BATCHED) ? cmpmc_launch_mesh_query32 : cmpmc_launch_mesh_query64;
+    if (hipMemcpyAsync(d_p, pts, n * 3 * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        (start && hipMemcpyAsync(d_start, start, n * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess) ||
+        launch(&M->dm, n, d_p, start ? d_start : nullptr, d_in, d_cell, d_E, s) != hipSuccess) {
+        rc = fail(CMPMC_ERR_HIP, "cmpmc_model_field_at: launch failed");
+    } else if (hipMemcpyAsync(E, d_E, n * 3 * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+               hipMemcpyAsync(inside, d_in, n * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+               hipMemcpyAsync(cell, d_cell, n * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+               hipStreamSynchronize(s) != hipSuccess) {
+        rc = fail(CMPMC_ERR_HIP, "cmpmc_model_field_at: copy failed");
     }
     buf.release();
     return rc;
@@ -856,7 +932,7 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
-    const int flags = (M->dm.kin_mode == 1 ? 1 : 0) | (M->dm.surf_x ? 2 : 0);
+    const int flags = (M->dm.kin_mode == 1 ? 1 : 0) | (M->dm.surf_x ? 2 : 0) | (M->dm.mesh.dim != 0 ? 4 : 0);
     const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, flags, block, &per_cu)
                                       : cmpmc_occupancy(which, waves, flags, block, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;

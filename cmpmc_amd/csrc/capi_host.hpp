@@ -97,6 +97,7 @@ struct cmpmc_model {
     double* d_stiff = nullptr;              // C_ijlm: 81 doubles, then 81 floats
     cmpmc_pad* d_pads = nullptr;            // electrode pads of all surfaces
     float* d_amm = nullptr;                 // FP32 AMM film-entry tables [3][2][kAmmTab + 1]
+    char* d_mesh = nullptr;                 // FEM mesh field: FP64 and FP32 records, neighbours, seeds
     // host-mapped cancel word the kernels poll: bit 0 is the caller's cancel flag
     // (cmpmc_model_cancel), bit 1 the run's time budget (set by the watchdog thread)
     volatile int32_t* h_cancel = nullptr;

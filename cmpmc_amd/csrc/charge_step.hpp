@@ -1,6 +1,6 @@
 // One step of SimulationEngine::transport_charges (engine.hpp:210-377), first-order
-// (FirstOrderDkCap) or Leman second-order (LemanSecondOrder) stepping, uniform field,
-// RecordOnly release at collection, with the
+// (FirstOrderDkCap) or Leman second-order (LemanSecondOrder) stepping, in the uniform field
+// or the FEM mesh field (MeshX policies), RecordOnly release at collection, with the
 // ChargePhysics leaves of charge/charge_transport.hpp:49-241 and the Herring-Vogt
 // kinematics of lattice/crystal_lattice.hpp:239-293.
 //
@@ -10,6 +10,7 @@
 //   ctx.charge_hit(CTrack&, edep_w, e_rel, surface, channel)
 #pragma once
 
+#include "mesh_walk.hpp"
 #include "phonon_step.hpp"
 
 namespace cmpmc_dev {
@@ -26,7 +27,9 @@ template <class P> struct CTrack {
     typename P::TRng rng;
     uint64_t key;
     uint32_t spawn_no, hit_seq;
+    int32_t cell;                // MeshX: 1 + the mesh cell of the last field lookup (0: none)
 };
+
 
 // The FP32 form reads the model's FP32 charge block (charge_f) instead of converting
 // FP64 fields at every step, and divides with the policy's hardware reciprocal; the FP64
@@ -145,6 +148,7 @@ __device__ inline void make_charge(const DevModel& m, CTrack<P>& c, int type, V3
     c.type = type;
     c.spawn_no = 0;
     c.hit_seq = 0;
+    c.cell = 0;
 }
 
 // charge_transport.hpp:82-106
@@ -179,12 +183,21 @@ template <class P> __device__ inline typename P::R luke_rate(const DevModel& m, 
 }
 
 // charge_kernels.hpp:82-99
-template <class P> __device__ inline typename P::R iv_rate(const DevModel& m, typename P::R ekin) {
+template <class P> __device__ inline typename P::R iv_rate(const DevModel& m, typename P::R ekin, const CField<P>& f) {
     using R = typename P::R;
-    if constexpr (!P::exact) return (ekin < m.charge_f.iv_min_energy) ? R(0) : m.charge_f.iv_rate;   // 0: one valley
+    if constexpr (!P::exact) {   // the rate at the uniform field / the cell's field (0: one valley)
+        if constexpr (P::mesh) return (ekin < m.charge_f.iv_min_energy) ? R(0) : f.iv;
+        else return (ekin < m.charge_f.iv_min_energy) ? R(0) : m.charge_f.iv_rate;
+    }
     if (m.n_valleys < 2) return R(0);
     if (ekin < R(m.iv_min_energy)) return R(0);
-    const double E2 = m.efield[0] * m.efield[0] + m.efield[1] * m.efield[1] + m.efield[2] * m.efield[2];
+    double E2;
+    if constexpr (P::mesh) {
+        if (f.iv >= R(0)) return f.iv;   // tetrahedral cells: the rate at the cell's field
+        E2 = f.E.x * f.E.x + f.E.y * f.E.y + f.E.z * f.E.z;
+    } else {
+        E2 = m.efield[0] * m.efield[0] + m.efield[1] * m.efield[1] + m.efield[2] * m.efield[2];
+    }
     if (m.iv_model == 0) {
         const double e0 = m.iv_quad_field;
         return R(m.iv_quad_rate * pow(e0 * e0 + E2, 0.5 * m.iv_quad_power));
@@ -201,13 +214,13 @@ template <class P> __device__ inline typename P::R mfp_rate(double mfp, typename
     return (mfp > 0.0 && mfp < 1e20) ? (v / R(mfp)) : R(0);
 }
 // engine.hpp:876-886
-template <class P> __device__ inline CRates<P> charge_rates(const DevModel& m, const CTrack<P>& t) {
+template <class P> __device__ inline CRates<P> charge_rates(const DevModel& m, const CTrack<P>& t, const CField<P>& f) {
     using R = typename P::R;
     const R v = norm<P>(charge_velocity<P>(m, t));
     const int h = (t.type == 1) ? 0 : 1;
     CRates<P> r;
     r.luke = luke_rate<P>(m, t);
-    r.iv = (t.type == 1) ? iv_rate<P>(m, t.ekin) : R(0);
+    r.iv = (t.type == 1) ? iv_rate<P>(m, t.ekin, f) : R(0);
     if constexpr (P::exact) {
         r.trap = mfp_rate<P>(m.trap_mfp[h], v);
         r.ion_e = mfp_rate<P>(m.ion_e_mfp[h], v);
@@ -325,16 +338,16 @@ template <class P, class G> __device__ inline void reflect_charge(const DevModel
 // form the same root as -2 ln u / (sqrt(disc) + a0), which does not cancel when
 // |a1| t << a0.
 template <class P, class G>
-__device__ inline typename P::R leman_event_time(const DevModel& m, const CTrack<P>& trk, V3<typename P::R> E,
+__device__ inline typename P::R leman_event_time(const DevModel& m, const CTrack<P>& trk, const CField<P>& f,
                                                  typename P::R q_sign, typename P::R dt_max, typename P::R a0, G& r) {
     using R = typename P::R;
     CTrack<P> trial;
     trial.pos = trk.pos;
     trial.valley = trk.valley;
     trial.type = trk.type;
-    trial.p = trk.p + E * (q_sign * R(kCLight) * dt_max);
+    trial.p = trk.p + f.E * (q_sign * R(kCLight) * dt_max);
     trial.ekin = (trk.type == 1) ? ekin_e<P>(m, trial.valley, trial.p) : ekin_h<P>(m, trial.p);
-    const R a1 = P::div(charge_rates<P>(m, trial).tot - a0, dt_max);
+    const R a1 = P::div(charge_rates<P>(m, trial, f).tot - a0, dt_max);
     if constexpr (P::exact) {
         const R lnu = log(P::uniform(r));
         if (fabs(a1) * dt_max <= R(1e-9) * tmax(a0, R(1e-300))) return (a0 > R(0)) ? (-lnu / a0) : P::inf();
@@ -360,11 +373,13 @@ template <class P, class G, class Ctx>
 __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx& ctx) {
     using R = typename P::R;
     ctx.ccount(CC_STEPS);
-    V3<R> E;
-    if constexpr (P::exact) E = V3<R>{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
-    else E = V3<R>{m.charge_f.efield[0], m.charge_f.efield[1], m.charge_f.efield[2]};
+    CField<P> f;                  // engine.hpp:213: the field once per step, at the step's start
+    if constexpr (P::mesh) f = mesh_field<P>(m, trk.pos, trk.cell);
+    else if constexpr (P::exact) f.E = V3<R>{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
+    else f.E = V3<R>{m.charge_f.efield[0], m.charge_f.efield[1], m.charge_f.efield[2]};
+    const V3<R> E = f.E;
     const R q_sign = (trk.type == 1) ? R(-1) : R(1);
-    const CRates<P> R0 = charge_rates<P>(m, trk);
+    const CRates<P> R0 = charge_rates<P>(m, trk, f);
     R dt_max;
     if constexpr (P::exact) {
         dt_max = R(m.dt_cap);
@@ -380,13 +395,15 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
         if (m.dk_frac > 0.0) {
             const int h = (trk.type == 1) ? 0 : 1;
             const R k_now = norm<P>((trk.type == 1) ? ell_to_sph<P>(m, trk.valley, p_to_k<P>(trk.p)) : p_to_k<P>(trk.p));
-            const R dk_rate = m.charge_f.dk_rate[h];
+            R dk_rate;
+            if constexpr (P::mesh) dk_rate = f.emag * m.charge_f.dk_scale[h];
+            else dk_rate = m.charge_f.dk_rate[h];
             if (dk_rate > R(0)) dt_max = tmin(dt_max, P::div(m.charge_f.dk_frac * tmax(m.charge_f.k_sound[h], k_now), dk_rate));
         }
     }
     const bool second_order = (m.charge_stepping == 1);
     R dt_phys;
-    if (second_order) dt_phys = leman_event_time<P>(m, trk, E, q_sign, dt_max, R0.tot, r);
+    if (second_order) dt_phys = leman_event_time<P>(m, trk, f, q_sign, dt_max, R0.tot, r);
     else dt_phys = (R0.tot > R(1)) ? P::div(P::neg_log(P::uniform(r)), R0.tot) : P::inf();
     const R dt = tmin(dt_phys, dt_max);
     const V3<R> dp = E * (q_sign * R(kCLight) * dt);
@@ -439,7 +456,7 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
     trk.ekin = (trk.type == 1) ? ekin_e<P>(m, trk.valley, trk.p) : ekin_h<P>(m, trk.p);
     if (dt_phys <= dt_max) {
         // second order: the process is chosen with the rates at the event (engine.hpp:337-342)
-        const CRates<P> Rv = second_order ? charge_rates<P>(m, trk) : R0;
+        const CRates<P> Rv = second_order ? charge_rates<P>(m, trk, f) : R0;
         if (second_order && !(Rv.tot > R(0))) return true;
         const R u_proc = P::uniform(r) * Rv.tot;
         if (u_proc < Rv.trap) {
@@ -455,6 +472,7 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             CTrack<P> k;
             make_charge<P>(m, k, lib_e ? 1 : 2, trk.pos, d, R(1.0e-6), kv, trk.time);
             k.weight = trk.weight;
+            k.cell = trk.cell;        // created where the parent's last lookup was
             ctx.push_charge(k, trk);
         } else if (u_proc < Rv.trap + Rv.ion_e + Rv.ion_h + Rv.iv) {
             scatter_iv<P>(m, trk, r);

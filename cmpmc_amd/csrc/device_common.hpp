@@ -67,6 +67,30 @@ struct DevSurfX {
 // Entries per mode of the batched form's AMM film-entry table, uniform in |cos theta|.
 constexpr int kAmmTab = 4096;
 
+// FEM mesh field of the detector (geometry/mesh_field.hpp), read only by the kernels
+// compiled with it (MeshX policies). Per cell one record, FP64 for the plain form and FP32
+// for the batched form: dim 3 [tinv 9 | reference vertex 3 | E 3 | iv 1] (16 words),
+// dim 2 FP64 [tinv 4 | reference vertex 2 | E_u E_v] (8), dim 2 FP32 [the same 8 | iv |E|
+// 0 0] (12); iv is the field-dependent intervalley rate at the cell's field. nbr: the cell
+// across the facet opposite each vertex (-1: hull). seed: a start cell per box of a
+// regular grid over the mesh's bounding box (a carrier without a cell starts there).
+struct DevMesh {
+    const double* rec;
+    const float* rec_f;
+    const int32_t* nbr;         // 4 per cell
+    const int32_t* seed;
+    int32_t dim;                // 0: no mesh (uniform field), 2, 3
+    int32_t axisym;             // dim 2: (r, z) projection with E_r rotated back
+    int32_t n_cells;
+    int32_t seed_n[3];
+    double lo[3], inv_h[3];     // seed grid origin and boxes per metre
+    double iv_zero;             // field-dependent intervalley rate at E = 0 (outside the mesh)
+    float lo_f[3], inv_h_f[3];
+    float iv_zero_f;            // the same, FP32 (0 with one valley)
+    float tol_f;                // FP32 walk: a point is inside a cell when every barycentric
+                                // coordinate is >= -tol_f (bound of their FP32 rounding)
+};
+
 // Passed by value as a kernel argument (read through the scalar cache).
 struct DevModel {
     const float4* oct;          // [3][256][256] (vx, vy, vz, |v|)
@@ -94,6 +118,7 @@ struct DevModel {
     double radius, half_z, efield[3];
     DevSurface surf[3];
     DevSurfX sx[3];
+    DevMesh mesh;
     uint32_t max_bounces, rr_bounces;
     // DetectorModel::ChargeStepping: 0 first order (dk_frac = charge_max_dk_frac), 1 Leman
     // second order (dk_frac = charge_sampling_dk_frac)
@@ -134,7 +159,8 @@ struct DevModel {
         float iv_rate, iv_min_energy;   // intervalley rate at this field (0 with one valley)
         float inv_mfp[3][2];        // 1 / mean free path: trapping, e and h ionisation (0: none)
         float v_sound_hbar;         // v_sound hbar: Luke phonon energy per unit |q|
-        float _pad;
+        float dk_scale[2];          // c / hbar c (times hv_max_scale for electrons): dk_rate per |E|
+        float _pad[3];
     } charge_f;
 };
 
@@ -245,6 +271,7 @@ struct Fp64 {
     static constexpr bool exact = true;
     static constexpr bool kin_exact = false;     // octahedral LUT kinematics
     static constexpr bool surf_x = false;        // no electrode masks or AMM film entry
+    static constexpr bool mesh = false;          // uniform field (no FEM mesh)
     __device__ static Rng track_stream(uint64_t base, uint64_t key) { return rng_for_track(base, key); }
     __device__ static double uniform(Rng& r) { return static_cast<double>((next_u64(r) >> 11) | 1ULL) * 0x1.0p-53; }
     __device__ static double neg_log(double u) { return -log(u); }
@@ -261,7 +288,7 @@ struct Fp64 {
     __device__ static double exp_(double x) { return exp(x); }
     __device__ static double log_(double x) { return log(x); }
     __device__ static double cbrt_(double x) { return cbrt(x); }
-    __device__ static double hypot_(double x, double y) { return hypot(x, y); }
+    __device__ static double hypot_(double x, double y) { return hypot_corrected(x, y); }
     __device__ static double sin_half_pi(double u) { return sin(kHalfPi * u); }
     __device__ static double inf() { return __builtin_huge_val(); }
     __device__ static double div(double a, double b) { return a / b; }
@@ -275,6 +302,7 @@ struct Fp32 {
     static constexpr bool exact = false;
     static constexpr bool kin_exact = false;
     static constexpr bool surf_x = false;
+    static constexpr bool mesh = false;
     __device__ static Rng32 track_stream(uint64_t base, uint64_t key) { return rng32_for_track(base, key); }
     // the top 24 bits of a draw: (0, 1), never 0 or 1 (64-bit event streams and 32-bit
     // transport streams alike)
@@ -316,6 +344,13 @@ struct Fp32X : Fp32 {
 // common case keeps its registers and instruction stream.
 template <class B> struct SurfX : B {
     static constexpr bool surf_x = true;
+};
+// The same with the detector's FEM mesh field (looked up per charge step) instead of the
+// uniform field; these kernels carry the surface extras too, whose per-hit checks cost
+// nothing beside the per-step cell walk.
+template <class B> struct MeshX : B {
+    static constexpr bool surf_x = true;
+    static constexpr bool mesh = true;
 };
 
 template <class T> __host__ __device__ inline T tmax(T a, T b) { return (a < b) ? b : a; }   // std::max
@@ -825,6 +860,8 @@ inline void fill_fp32_constants(DevModel& m) {
         const double dkr = std::sqrt(E2) * kCLight / kHbarc;
         c.dk_rate[0] = static_cast<float>(dkr * m.hv_max_scale);
         c.dk_rate[1] = static_cast<float>(dkr);
+        c.dk_scale[0] = static_cast<float>(kCLight / kHbarc * m.hv_max_scale);
+        c.dk_scale[1] = static_cast<float>(kCLight / kHbarc);
         c.k_sound[0] = static_cast<float>(m.k_sound_e);
         c.k_sound[1] = static_cast<float>(m.k_sound_h);
         c.tstep[0] = static_cast<float>(3.0 * m.l0_e / m.v_sound);
@@ -845,7 +882,7 @@ inline void fill_fp32_constants(DevModel& m) {
                 c.inv_mfp[k][h] = (mfp > 0.0 && mfp < 1e20) ? static_cast<float>(1.0 / mfp) : 0.0f;
             }
         c.v_sound_hbar = static_cast<float>(m.v_sound * kHbar);
-        c._pad = 0.0f;
+        c._pad[0] = c._pad[1] = c._pad[2] = 0.0f;
     }
     m.half_z_f = static_cast<float>(m.half_z);
     m._pad_f = 0.0f;

@@ -18,6 +18,8 @@
 
 #include <cmath>
 
+#include "hypot_corrected.hpp"
+
 #if defined(__HIPCC__)
 #define CMPMC_HD __host__ __device__
 #else
@@ -37,7 +39,7 @@ CMPMC_HD inline Cx cadd(Cx a, Cx b) { return {a.re + b.re, a.im + b.im}; }
 CMPMC_HD inline Cx csub(Cx a, Cx b) { return {a.re - b.re, a.im - b.im}; }
 CMPMC_HD inline Cx cmul(Cx a, Cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 CMPMC_HD inline Cx cscale(double s, Cx a) { return {s * a.re, s * a.im}; }
-CMPMC_HD inline double cmod(Cx a) { return hypot(a.re, a.im); }
+CMPMC_HD inline double cmod(Cx a) { return hypot_corrected(a.re, a.im); }
 // a / b by Smith's method (no overflow of the intermediate |b|^2)
 CMPMC_HD inline Cx cdiv(Cx a, Cx b) {
     if (fabs(b.re) < fabs(b.im)) {

@@ -564,6 +564,7 @@ __device__ inline void bcharge_to_ctrack(const BCharge& b, uint64_t seed, CTrack
     c.key = b.key;
     c.spawn_no = 0;
     c.hit_seq = 0;
+    c.cell = P::mesh ? static_cast<int32_t>(b._pad) : 0;   // mesh cell (1 + index) of the last lookup
     event = b.event;
     c.rng = P::track_stream(event_base(seed, event), b.key);
 }
@@ -576,7 +577,7 @@ template <class T> __device__ inline void ctrack_to_bcharge(const CTrack<T>& c, 
     b.weight = c.weight;
     b.key = c.key;
     b.valley_type = (static_cast<uint32_t>(c.valley) << 8) | static_cast<uint32_t>(c.type);
-    b._pad = 0;
+    b._pad = T::mesh ? static_cast<uint32_t>(c.cell) : 0u;
 }
 
 template <class P> struct ChargeCtx {
@@ -1016,10 +1017,22 @@ template <class P> static const void* charges_fn_p(int waves) {
 }
 static const void* charges_fn(int waves, int flags) {
     const bool exact = (flags & 1) != 0;
+    if (flags & 4) return exact ? charges_fn_p<MeshX<Fp32X>>(waves) : charges_fn_p<MeshX<Fp32>>(waves);
     if (flags & 2) return exact ? charges_fn_p<SurfX<Fp32X>>(waves) : charges_fn_p<SurfX<Fp32>>(waves);
     return exact ? charges_fn_p<Fp32X>(waves) : charges_fn_p<Fp32>(waves);
 }
-static int model_flags(const DevModel* m) { return (m->kin_mode == 1 ? 1 : 0) | (m->surf_x ? 2 : 0); }
+// bit 2: the FEM mesh field (charge kernels only; the phonon kernels do not read the field)
+static int model_flags(const DevModel* m) {
+    return (m->kin_mode == 1 ? 1 : 0) | (m->surf_x ? 2 : 0) | (m->mesh.dim != 0 ? 4 : 0);
+}
+
+extern "C" hipError_t cmpmc_launch_mesh_query32(const DevModel* m, uint64_t n, const double* pts, const int32_t* start,
+                                                int32_t* inside, int32_t* cell, double* E, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mesh_query_kernel<MeshX<Fp32>>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                       *m, n, pts, start, inside, cell, E);
+    return hipGetLastError();
+}
 
 // Resident blocks per CU of the persistent kernels (sizes their grids).
 extern "C" hipError_t cmpmc_occupancy(int which, int waves, int flags, int block, int* blocks_per_cu) {

@@ -209,7 +209,8 @@ struct ChargeArgs {
 
 extern "C" {
 // which: 0 phonon kernel, 1 charge kernel; waves: register-budget variant (2, 3 or 4);
-// flags: bit 0 the exact-kinematics instantiation, bit 1 the one with surface extras.
+// flags: bit 0 the exact-kinematics instantiation, bit 1 the one with surface extras,
+// bit 2 the one with the FEM mesh field (charge kernel).
 hipError_t cmpmc_occupancy(int which, int waves, int flags, int block, int* blocks_per_cu);
 hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block, hipStream_t s);
 // Plain form of deposit workloads: every event's partition into its stack regions, its
@@ -218,6 +219,12 @@ hipError_t cmpmc_launch_deposit_stacks(const cmpmc_dev::DevModel* m, const cmpmc
                                        uint64_t first_event, uint64_t n_events, cmpmc_phonon_track* pstack,
                                        const uint64_t* poff, cmpmc_charge_track* cstack, const uint64_t* coff,
                                        uint64_t* rng_state, double* injected, hipStream_t s);
+// The charge step's field evaluation at n points (FP64 plain form / FP32 batched form):
+// the walk from start[i] (NULL: the seed box), inside, final cell and E per point.
+hipError_t cmpmc_launch_mesh_query64(const cmpmc_dev::DevModel* m, uint64_t n, const double* pts, const int32_t* start,
+                                     int32_t* inside, int32_t* cell, double* E, hipStream_t s);
+hipError_t cmpmc_launch_mesh_query32(const cmpmc_dev::DevModel* m, uint64_t n, const double* pts, const int32_t* start,
+                                     int32_t* inside, int32_t* cell, double* E, hipStream_t s);
 hipError_t cmpmc_launch_batched(const cmpmc_dev::DevModel* m, const cmpmc_dev::BatchedArgs* a, int block, int grid,
                                 int waves, int diff, hipStream_t s);
 hipError_t cmpmc_occupancy_diff(int waves, int flags, int block, int* blocks_per_cu);

@@ -75,6 +75,7 @@ template <class P> struct PlainCtx {
         o.p_eV[0] = c.p.x; o.p_eV[1] = c.p.y; o.p_eV[2] = c.p.z;
         o.ekin = c.ekin; o.time = c.time; o.weight = c.weight;
         o.valley = static_cast<uint16_t>(c.valley); o.type = static_cast<uint8_t>(c.type); o.alive = 1;
+        if constexpr (P::mesh) *reinterpret_cast<int32_t*>(o._pad) = c.cell;
     }
     __device__ void emit(const cmpmc_sensor_hit& h) {
         if (keep_hits) {
@@ -108,11 +109,15 @@ template <class P> __device__ inline void load_phonon(const cmpmc_phonon_track& 
     t.energy = o.energy; t.vg_mag = o.vg_mag; t.time = o.time; t.weight = o.weight;
     t.bounces = o.bounces; t.mode = o.mode;
 }
+// The first four padding bytes of a stacked charge carry its mesh cell (1 + index; the
+// walk validates it, so any value is a usable start).
 template <class P> __device__ inline void load_charge(const cmpmc_charge_track& o, CTrack<P>& c) {
     c.pos = {o.pos[0], o.pos[1], o.pos[2]};
     c.p = {o.p_eV[0], o.p_eV[1], o.p_eV[2]};
     c.ekin = o.ekin; c.time = o.time; c.weight = o.weight;
     c.valley = o.valley; c.type = o.type;
+    if constexpr (P::mesh) c.cell = *reinterpret_cast<const int32_t*>(o._pad);
+    else c.cell = 0;
 }
 
 // Deposit workloads in the plain form: a generation pass writes each event's partition
@@ -129,6 +134,7 @@ template <class P> struct StackDepositSink {
         o.p_eV[0] = c.p.x; o.p_eV[1] = c.p.y; o.p_eV[2] = c.p.z;
         o.ekin = c.ekin; o.time = c.time; o.weight = c.weight;
         o.valley = static_cast<uint16_t>(c.valley); o.type = static_cast<uint8_t>(c.type); o.alive = 1;
+        *reinterpret_cast<int32_t*>(o._pad) = 0;      // no mesh cell yet
     }
     __device__ void phonon(const PTrack<P>& t, uint32_t j) {
         cmpmc_phonon_track& o = ps[j];
@@ -356,6 +362,17 @@ extern "C" hipError_t cmpmc_launch_deposit_stacks(const cmpmc_dev::DevModel* m, 
     return hipGetLastError();
 }
 
+// FP64 form of the field query (the FP32 one is compiled with the batched kernels)
+extern "C" hipError_t cmpmc_launch_mesh_query64(const cmpmc_dev::DevModel* m, uint64_t n, const double* pts,
+                                                const int32_t* start, int32_t* inside, int32_t* cell, double* E,
+                                                hipStream_t s) {
+    using namespace cmpmc_dev;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(mesh_query_kernel<MeshX<Fp64>>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                       *m, n, pts, start, inside, cell, E);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmpmc_dev::PlainArgs* a, int block,
                                          hipStream_t stream) {
     const uint64_t n = a->n_events;
@@ -363,7 +380,12 @@ extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmp
     const int bs = (block > 0 && block <= cmpmc_dev::kMaxBlock) ? block : 256;
     const unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
     using namespace cmpmc_dev;
-    if (m->surf_x) {
+    if (m->mesh.dim != 0) {
+        if (m->kin_mode == 1)
+            hipLaunchKernelGGL(transport_plain_kernel<MeshX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+        else
+            hipLaunchKernelGGL(transport_plain_kernel<MeshX<Fp64>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    } else if (m->surf_x) {
         if (m->kin_mode == 1)
             hipLaunchKernelGGL(transport_plain_kernel<SurfX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
         else

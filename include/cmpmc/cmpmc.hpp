@@ -6,6 +6,7 @@
 #include "cmpmc/materials/crystal_config.hpp"
 #include "cmpmc/lattice/crystal_lattice.hpp"
 #include "cmpmc/geometry/electrode_pattern.hpp"
+#include "cmpmc/geometry/mesh_field.hpp"
 #include "cmpmc/phonon/interface_transmission.hpp"
 #include "cmpmc/geometry/surface_property.hpp"
 #include "cmpmc/transport/engine.hpp"

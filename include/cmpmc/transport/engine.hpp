@@ -10,6 +10,7 @@
 #include <optional>
 #include <vector>
 
+#include "cmpmc/geometry/mesh_field.hpp"
 #include "cmpmc/geometry/surface_property.hpp"
 #include "cmpmc/lattice/crystal_lattice.hpp"
 
@@ -60,6 +61,9 @@ struct UniformElectricField {
 struct DetectorModel {
     CylinderSolid solid{};
     UniformElectricField e_field = UniformElectricField::from_bias(0.0, 2.54 * units::cm);
+    // engine.hpp:99-100: an FEM mesh field replaces e_field (the 3D one first)
+    std::optional<geometry::TriLinearMesh3D> mesh_field_3d = std::nullopt;
+    std::optional<geometry::BiLinearMesh2D> mesh_field_2d = std::nullopt;
     std::array<SurfaceProperty, 3> surfaces{SurfaceProperty::diffuse_al_sensor_surface(),
                                            SurfaceProperty::diffuse_al_sensor_surface(),
                                            SurfaceProperty::diffuse_bare_surface()};
@@ -79,10 +83,16 @@ struct DetectorModel {
     uint32_t russian_roulette_bounces = 0;
     double russian_roulette_survival = 0.5;
 
-    Vec3 electric_field_at(const Vec3& p) const noexcept { return e_field.at(p); }
+    // engine.hpp:151-155
+    Vec3 electric_field_at(const Vec3& p) const noexcept {
+        if (mesh_field_3d.has_value()) return mesh_field_3d->at(p);
+        if (mesh_field_2d.has_value()) return mesh_field_2d->at(p);
+        return e_field.at(p);
+    }
 
     // The C description of the detector; the surfaces' electrode pads go to pads[i]
-    // (which must outlive the result until the model is created).
+    // (which must outlive the result until the model is created), and the mesh field is
+    // passed by its library handle (held by this DetectorModel).
     cmpmc_detector to_pod(std::array<std::vector<cmpmc_pad>, 3>* pads = nullptr) const {
         if (charge_release != ChargeRelease::RecordOnly)
             throw std::invalid_argument("cmpmc_amd: only ChargeRelease::RecordOnly runs on the GPU path");
@@ -101,6 +111,8 @@ struct DetectorModel {
         d.russian_roulette_survival = russian_roulette_survival;
         d.charge_stepping = charge_stepping == ChargeStepping::LemanSecondOrder ? 1 : 0;
         d.charge_sampling_dk_frac = charge_sampling_dk_frac;
+        d.mesh = mesh_field_3d.has_value() ? mesh_field_3d->handle()
+                 : mesh_field_2d.has_value() ? mesh_field_2d->handle() : nullptr;
         return d;
     }
 };

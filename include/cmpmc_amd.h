@@ -176,7 +176,13 @@ typedef struct {
     cmpmc_medium amm_substrate, amm_film;
 } cmpmc_surface;
 
-/* transport/engine.hpp:96-150 DetectorModel (cylinder solid, uniform field) */
+/* geometry/mesh_field.hpp TriLinearMesh3D / BiLinearMesh2D: an electric field from a
+ * piecewise-linear potential on a tetrahedral (3D) or triangle (2D, planar or axisymmetric
+ * (r, z)) mesh. Opaque; built by the cmpmc_mesh* functions below. */
+typedef struct cmpmc_mesh cmpmc_mesh;
+
+/* transport/engine.hpp:96-150 DetectorModel (cylinder solid, uniform field or an FEM
+ * mesh field: mesh_field_3d / mesh_field_2d, engine.hpp:99-100, 151-155) */
 typedef struct {
     double radius, half_z;
     double efield[3];
@@ -192,6 +198,9 @@ typedef struct {
                                      1 LemanSecondOrder (engine.hpp:234-254, 337-342) */
     int32_t _pad_cs;
     double charge_sampling_dk_frac; /* LemanSecondOrder sampling-step cap (default 0.25) */
+    const cmpmc_mesh* mesh;       /* NULL: the uniform efield. Else the charge step's field is the
+                                     mesh's (zero outside it), read at model creation (the model
+                                     keeps its own copy; the mesh may be destroyed afterwards) */
 } cmpmc_detector;
 
 typedef struct cmpmc_model cmpmc_model;
@@ -378,6 +387,42 @@ int cmpmc_partition_deposit(const cmpmc_lattice* lattice, const double pos[3], d
                             cmpmc_phonon_track* phonons, uint64_t phonon_cap, uint64_t* n_phonons,
                             cmpmc_partition_result* result);
 
+/* ---------------------------------------------------------------- FEM mesh fields
+ * geometry/mesh_field.hpp. Cells are given by vertex indices (any order: they are sorted,
+ * as the reference sorts them); an index outside [0, n_vertices) is CMPMC_ERR_INVALID
+ * (undefined behaviour in the reference). Host-side queries take an in/out walk hint (a
+ * cell index; out of range: the walk starts at cell n_cells / 2), as find_tetrahedron /
+ * field_at / potential_at do. */
+/* TriLinearMesh3D(vertices, potentials, tetrahedra, V_min_clamp, V_max_clamp) (mesh_field.hpp:48-59) */
+int cmpmc_mesh3d_create(const double* xyz, uint64_t n_vertices, const double* potentials, const int32_t* tets,
+                        uint64_t n_tets, double v_min_clamp, double v_max_clamp, cmpmc_mesh** out);
+/* BiLinearMesh2D(vertices (u, v), potentials, triangles, axisymmetric_rz) (mesh_field.hpp:348-357) */
+int cmpmc_mesh2d_create(const double* uv, uint64_t n_vertices, const double* potentials, const int32_t* tris,
+                        uint64_t n_tris, int axisymmetric_rz, cmpmc_mesh** out);
+/* TriLinearMesh3D::build_cartesian_grid (mesh_field.hpp:128-181) in two steps: the
+ * (nx+1)(ny+1)(nz+1) vertex positions (index (i (ny+1) + j)(nz+1) + k) for the caller to
+ * evaluate its potential at, then the mesh of 6 tetrahedra per voxel from those potentials. */
+int cmpmc_mesh3d_grid_vertices(double half_x, double half_y, double half_z, int nx, int ny, int nz, double* xyz);
+int cmpmc_mesh3d_grid(double half_x, double half_y, double half_z, int nx, int ny, int nz, const double* potentials,
+                      cmpmc_mesh** out);
+/* BiLinearMesh2D::build_grid (mesh_field.hpp:444-478), the same two steps (index i (nv+1) + j) */
+int cmpmc_mesh2d_grid_vertices(double u_min, double u_max, double v_min, double v_max, int nu, int nv, double* uv);
+int cmpmc_mesh2d_grid(double u_min, double u_max, double v_min, double v_max, int nu, int nv,
+                      const double* potentials, int axisymmetric_rz, cmpmc_mesh** out);
+/* TriLinearMesh3D::load_from_files (mesh_field.hpp:188-230) */
+int cmpmc_mesh3d_load(const char* coords_file, const char* tets_file, double coord_unit_m, double potential_unit_V,
+                      cmpmc_mesh** out);
+int cmpmc_mesh_destroy(cmpmc_mesh* mesh);
+/* dim 3 or 2; num_vertices / num_tetrahedra / num_triangles */
+int cmpmc_mesh_info(const cmpmc_mesh* mesh, int* dim, int* axisymmetric_rz, uint64_t* n_vertices, uint64_t* n_cells);
+/* find_tetrahedron / find_triangle (mesh_field.hpp:63-93, 358-387): p in mesh coordinates
+ * (3 or 2), bary[dim + 1]. Returns 1 inside, 0 outside (hint left at the last cell). */
+int cmpmc_mesh_locate(const cmpmc_mesh* mesh, const double* p, int32_t* hint, double* bary);
+/* field_at (mesh_field.hpp:107-111, 398-418): E [V/m] at a 3D position, zero outside */
+int cmpmc_mesh_field(const cmpmc_mesh* mesh, const double p[3], int32_t* hint, double E[3]);
+/* potential_at (mesh_field.hpp:96-104, 389-396): V at a 3D position, 0 outside */
+double cmpmc_mesh_potential(const cmpmc_mesh* mesh, const double p[3], int32_t* hint);
+
 /* CrystalLattice(const CrystalConfig&) table build: crystal_lattice.hpp:45-55, 340-477.
  * oct_lut: 3*256*256*4 floats; cdf_lt, cdf_tt: 1024 doubles each (caller-owned);
  * `out` receives cfg, the derived charge tensors and the three table pointers. */
@@ -417,6 +462,16 @@ int cmpmc_model_set_kinematics(cmpmc_model* m, int kinematics_mode);
  * No counterpart in the reference: it exposes what the kernels compute, for checks. */
 int cmpmc_model_map_k_to_vg(cmpmc_model* m, int mode, int form, uint64_t n, const double* k, double* vg_dir,
                             double* vg_mag, void* stream);
+/* The model's electric field evaluated on the device as the charge step evaluates it
+ * (DetectorModel::electric_field_at, engine.hpp:151-155), at n 3D points (host arrays):
+ * form CMPMC_FORM_PLAIN in FP64 (the reference's walk and arithmetic), CMPMC_FORM_BATCHED in
+ * FP32. With a mesh, each point's walk starts at cell start[i] (out of range: n_cells / 2,
+ * find_tetrahedron's rule; start NULL: the cell a new carrier starts from); inside[i],
+ * cell[i] (where the walk ended) and E[3i..] are written. Without a mesh: the uniform field,
+ * inside 1, cell -1. No counterpart in the reference: a device-side view of the field for
+ * tests and tools. */
+int cmpmc_model_field_at(cmpmc_model* m, int form, uint64_t n, const double* pts, const int32_t* start,
+                         int32_t* inside, int32_t* cell, double* E, void* stream);
 
 /* SimulationEngine::transport(stack, hits, stats, rng): engine.hpp:403-582, exact
  * reference semantics (one FP64 lane, LIFO, one stream). Consumes the n tracks, appends

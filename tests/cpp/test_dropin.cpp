@@ -234,6 +234,50 @@ int main(int argc, char** argv) {
         check("recoil demo: every carrier collected", static_cast<double>(rr.stats.charges_collected),
               2.0 * static_cast<double>(pairs.load()), 0.0);
     }
+    {   // FEM mesh fields (geometry/mesh_field.hpp): a dyadic grid with V = -128 z has the
+        // field (0, 0, 128) exactly; pairs drifting in it take the uniform field's steps
+        const auto mesh = geometry::TriLinearMesh3D::build_cartesian_grid(
+            0.0625, 0.0625, 0.015625, 8, 8, 8, [](const Vec3& p) { return -128.0 * p.z; });
+        check("mesh3d: 6 tetrahedra per voxel", static_cast<double>(mesh.num_tetrahedra()), 6.0 * 512, 0.0);
+        DetectorModel dm;
+        dm.luke_sample_fraction = 0.0;
+        dm.mesh_field_3d = mesh;
+        DetectorModel du = dm;
+        du.mesh_field_3d.reset();
+        du.e_field.field_Vm = Vec3{0.0, 0.0, 128.0};
+        const Vec3 E = dm.electric_field_at(Vec3{0.011, -0.007, 0.003});
+        check("mesh3d: electric_field_at E_z", E.z, 128.0, 0.0);
+        check("mesh3d: electric_field_at E_x", E.x, 0.0, 0.0);
+        const SimulationEngine em(lattice, dm), eu(lattice, du);
+        const ChargePhysics cp(lattice);
+        auto pairs = [&](EventInput& in) {
+            const Vec3 pos{0.01 * (in.rng.uniform() - 0.5), 0.01 * (in.rng.uniform() - 0.5), 0.01 * (in.rng.uniform() - 0.5)};
+            in.charges.push_back(cp.make_charge(ExcitationType::Electron, pos, in.rng.isotropic_dir(), 1e-6, 0));
+            in.charges.push_back(cp.make_charge(ExcitationType::Hole, pos, in.rng.isotropic_dir(), 1e-6, 0));
+        };
+        RunOptions o;
+        o.seed = 4242;
+        o.form = CMPMC_FORM_PLAIN;
+        const RunResult a = run_events(em, 16, pairs, o), b = run_events(eu, 16, pairs, o);
+        check("mesh3d: pairs collected", static_cast<double>(a.stats.charges_collected), 32.0, 0.0);
+        check("mesh3d: plain run equals the uniform-field run bit for bit",
+              std::memcmp(&a.stats, &b.stats, sizeof(SimStats)) == 0 ? 1 : 0, 1, 0.0);
+        // axisymmetric (r, z) mesh: V = 40 r + 150 z gives E_r = -40, E_z = -150
+        const auto m2 = geometry::BiLinearMesh2D::build_grid(
+            0.0, 0.04, -0.0128, 0.0128, 10, 6, [](double u, double v) { return 40.0 * u + 150.0 * v; }, true);
+        int hint = 0;
+        const Vec3 E2 = m2.field_at(Vec3{0.006, 0.008, 0.002}, hint);
+        check("mesh2d rz: E_x = -40 x / r", E2.x, -40.0 * 0.6, 1e-12);
+        check("mesh2d rz: E_y = -40 y / r", E2.y, -40.0 * 0.8, 1e-12);
+        check("mesh2d rz: E_z", E2.z, -150.0, 1e-12);
+        check("mesh2d rz: potential", m2.potential_at(Vec3{0.006, 0.008, 0.002}, hint), 40.0 * 0.01 + 150.0 * 0.002, 1e-12);
+        bool threw = false;
+        try {
+            geometry::TriLinearMesh3D bad({Vec3{0, 0, 0}, Vec3{1, 0, 0}, Vec3{0, 1, 0}, Vec3{0, 0, 1}}, {0, 0, 0, 0},
+                                          {geometry::TriLinearMesh3D::Tet{0, 1, 2, 7}});
+        } catch (const std::invalid_argument&) { threw = true; }
+        check("mesh3d: a cell naming a missing vertex is rejected", threw ? 1 : 0, 1, 0.0);
+    }
     {   // a worker exception reaches the caller (guarded Matrix IV model)
         CrystalConfig cfg = materials::Germanium();
         cfg.iv_model = IVRateModel::Matrix;

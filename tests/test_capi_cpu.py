@@ -220,3 +220,53 @@ def test_host_partition_capacity_error_keeps_stream(lib_cmpmc):
                                    C.byref(nph), None)
     assert rc == cm.capi.ERR_CAPACITY and nq.value > 0 and nph.value > 0
     assert list(st) == [1, 2, 3, 4]
+
+
+MESH_KEYS = ("found", "bary", "hints", "E", "pot")
+
+
+def test_host_mesh_fields_equal_reference(lib_cmpmc, oracle, gold, tmp_path):
+    """cmpmc_mesh*: construction (grids, explicit meshes with clamped potentials and
+    shuffled cells, the two-file loader) and the walk / field / potential queries equal the
+    reference's TriLinearMesh3D and BiLinearMesh2D bit for bit, walk hints included."""
+    cm, O = lib_cmpmc, oracle
+    (hx, hy, hz), (nx, ny, nz), c3 = O.MESH3_GRID
+    cases = [("mesh3_grid", cm.Mesh.grid3d(hx, hy, hz, nx, ny, nz, lambda x, y, z: O.poly3(c3, x, y, z)),
+              "mesh_pts3", 0)]
+    xyz, V, tets = O.irregular_mesh3()
+    cases.append(("mesh3_irr", cm.Mesh.tet(xyz, V, tets, -30.0, 30.0), "mesh_pts3i", -1))
+    cp, tp = O.write_mesh_files(str(tmp_path))
+    cases.append(("mesh3_file", cm.Mesh.load3d(cp, tp, 1e-3, 1e-3), "mesh_pts3i", 7))
+    (u0, u1), (v0, v1), (nu, nv), c2 = O.MESH2_GRID_RZ
+    cases.append(("mesh2_rz", cm.Mesh.grid2d(u0, u1, v0, v1, nu, nv, lambda u, v: O.poly2(c2, u, v), True),
+                  "mesh_pts3", 0))
+    (u0, u1), (v0, v1), (nu, nv), c2 = O.MESH2_GRID_XY
+    cases.append(("mesh2_xy", cm.Mesh.grid2d(u0, u1, v0, v1, nu, nv, lambda u, v: O.poly2(c2, u, v), False),
+                  "mesh_pts3", 5))
+    uv, V2, tris = O.irregular_mesh2()
+    cases.append(("mesh2_irr", cm.Mesh.tri(uv, V2, tris, False), "mesh_pts3i", 2))
+    for tag, mesh, pts, h0 in cases:
+        o = mesh.query(gold[pts], h0)
+        for k in MESH_KEYS:
+            assert np.array_equal(o[k], gold[f"{tag}_{k}"]), (tag, k)
+    assert (cases[0][1].num_vertices, cases[0][1].num_cells) == tuple(gold["mesh3_grid_counts"])
+    assert (cases[2][1].num_vertices, cases[2][1].num_cells) == tuple(gold["mesh3_file_counts"])
+    assert (cases[3][1].num_vertices, cases[3][1].num_cells) == tuple(gold["mesh2_rz_counts"])
+
+
+def test_host_mesh_argument_errors(lib_cmpmc, tmp_path):
+    """A cell naming a vertex that does not exist, a non-finite coordinate and a missing
+    file are errors (the reference reads out of bounds / builds an empty mesh)."""
+    cm = lib_cmpmc
+    xyz = np.eye(4, 3)
+    with pytest.raises(cm.CmpmcError, match="names vertex"):
+        cm.Mesh.tet(xyz, np.zeros(4), [[0, 1, 2, 4]])
+    bad = xyz.copy()
+    bad[2, 1] = np.nan
+    with pytest.raises(cm.CmpmcError, match="not finite"):
+        cm.Mesh.tet(bad, np.zeros(4), [[0, 1, 2, 3]])
+    with pytest.raises(cm.CmpmcError, match="cannot open"):
+        cm.Mesh.load3d(tmp_path / "none.txt", tmp_path / "none2.txt")
+    empty = cm.Mesh.tet(np.zeros((0, 3)), np.zeros(0), np.zeros((0, 4)))
+    E, _ = empty.field_at([0.0, 0.0, 0.0])
+    assert empty.num_cells == 0 and np.all(E == 0.0)
