@@ -67,6 +67,37 @@ DEPOSIT_WL = dict(mode=1, energy_eV=100.0, pair_at_origin=1, partition=dict(max_
 VALU_WAVE_INSTS_PER_CU_CYCLE = 1.0
 
 
+# FEM mesh fields of the mesh leg: a tetrahedral Kuhn grid over the crystal's bounding box
+# (48 x 48 x 16 voxels, 221,184 tetrahedra) and an axisymmetric (r, z) grid (64 x 32
+# cells, 4,096 triangles), potentials of about 4 V across the height with quadratic terms
+# (V = 157.48 z + 2000 x^2 - 800 y z + 5000 z^2; V = 157.48 z + 1500 r^2 + 3000 r z - 4000 z^2);
+# None: the uniform 4 V field on the same pairs.
+MESH_CASES = {
+    "uniform": None,
+    "tet_grid": ("3d", (0.0382, 0.0382, 0.0128), (48, 48, 16), (0.0, 0.0, 0.0, 157.48, 2000.0, -800.0, 5000.0)),
+    "rz_grid": ("rz", (0.0, 0.0382, -0.0128, 0.0128), (64, 32), (0.0, 0.0, 157.48, 1500.0, 3000.0, -4000.0)),
+}
+
+
+def mesh_from_spec(mesh_cls, spec):
+    """A mesh of MESH_CASES with the library's or the oracle's Mesh class (the same vertex
+    positions and potentials)."""
+    kind, box, n, c = spec
+    if kind == "3d":
+        x = mesh_cls.grid3d_vertices(*box, *n)
+        V = c[0] + c[1] * x[:, 0] + c[2] * x[:, 1] + c[3] * x[:, 2] + c[4] * x[:, 0] * x[:, 0] + \
+            c[5] * x[:, 1] * x[:, 2] + c[6] * x[:, 2] * x[:, 2]
+        if mesh_cls.__module__.startswith("oracle"):
+            return mesh_cls.grid3d(*box, *n, c)
+        return mesh_cls.grid3d(*box, *n, V)
+    u = mesh_cls.grid2d_vertices(*box, *n)
+    if mesh_cls.__module__.startswith("oracle"):
+        return mesh_cls.grid2d(*box, *n, c, True)
+    V = c[0] + c[1] * u[:, 0] + c[2] * u[:, 1] + c[3] * u[:, 0] * u[:, 0] + c[4] * u[:, 0] * u[:, 1] + \
+        c[5] * u[:, 1] * u[:, 1]
+    return mesh_cls.grid2d(*box, *n, V, True)
+
+
 def _log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -181,6 +212,8 @@ def main():
     ap.add_argument("--luke-events", type=int, default=2_000)
     ap.add_argument("--luke-plain-events", type=int, default=0)
     ap.add_argument("--deposit-events", type=int, default=2_000)
+    ap.add_argument("--mesh-events", type=int, default=100_000, help="pairs drifting in FEM mesh fields (batched)")
+    ap.add_argument("--mesh-plain-events", type=int, default=10_000, help="the same, plain form")
     ap.add_argument("--cpu-events", type=int, default=100_000, help="headline CPU leg, all cores")
     ap.add_argument("--cpu-events-1t", type=int, default=5_000, help="headline CPU leg, one thread")
     ap.add_argument("--seed", type=int, default=42)
@@ -374,6 +407,43 @@ def main():
         _log(f"deposit (100 eV NR at 4 V): {args.deposit_events} events in {tdw:.2f} s, "
              f"{sd['total_steps']:.3e} phonon steps")
 
+    if args.mesh_events > 0:
+        # pairs at 4 V-scale bias with trapping on, as the charge-drift leg, in FEM mesh
+        # fields: a tetrahedral grid over the crystal's bounding box and an axisymmetric
+        # (r, z) triangle grid (MESH_CASES), against the uniform 4 V field
+        pair = cm.workload(cm.WL_PAIR)
+        out["mesh"] = {"pairs": args.mesh_events, "plain_pairs": args.mesh_plain_events}
+        for name, spec in MESH_CASES.items():
+            det = cm.default_detector()
+            det.luke_sample_fraction = 0.0
+            if spec is None:
+                cm.set_bias(det, 4.0)
+                mesh = None
+            else:
+                mesh = mesh_from_spec(cm.Mesh, spec)
+                cm.set_mesh(det, mesh)
+            cr = cm.germanium()
+            cr.e_trapping_mfp_m = cr.h_trapping_mfp_m = 0.5
+            em = cm.Engine(cm.Lattice(cr), det)
+            em.run_workload(pair, 1000, seed=1, keep_hits=False, form=cm.FORM_BATCHED)      # warm
+            t = time.perf_counter()
+            rb = em.run_workload(pair, args.mesh_events, seed=args.seed, keep_hits=False, form=cm.FORM_BATCHED)
+            tb = time.perf_counter() - t
+            row = {"cells": 0 if mesh is None else mesh.num_cells, "batched_pairs_per_s": args.mesh_events / tb,
+                   "batched_charge_steps_per_s": rb["stats"]["charge_steps"] / tb,
+                   "batched_kernel_s": rb["kernel_ms"] / 1e3,
+                   "charge_steps_per_pair": rb["stats"]["charge_steps"] / args.mesh_events}
+            if args.mesh_plain_events > 0:
+                t = time.perf_counter()
+                rp = em.run_workload(pair, args.mesh_plain_events, seed=args.seed, keep_hits=False, form=cm.FORM_PLAIN)
+                tp = time.perf_counter() - t
+                row["plain_pairs_per_s"] = args.mesh_plain_events / tp
+                row["plain_charge_steps_per_s"] = rp["stats"]["charge_steps"] / tp
+            out["mesh"][name] = row
+            _log(f"mesh field {name}: batched {tb:.3f} s for {args.mesh_events} pairs"
+                 + (f", plain {tp:.3f} s for {args.mesh_plain_events}" if args.mesh_plain_events > 0 else ""))
+            em.close()
+
     if not args.no_lognormal and args.lognormal_events > 0:
         lw = cm.workload(cm.WL_PHONON_LOGN)
         t = time.perf_counter()
@@ -404,6 +474,18 @@ def main():
                 out["charge"]["cpu"] = cpu_pair("pair", 4000, 400, args.seed, host, crystal=c3, detector=d3)
             if "luke" in out:
                 out["luke"]["cpu"] = cpu_pair("pair", 64, 4, args.seed, host, crystal=c4, detector=d4)
+            if "mesh" in out:
+                sys.path.insert(0, os.path.join(ROOT, "oracle"))
+                import oracle_ctypes as O
+                for name, spec in MESH_CASES.items():
+                    if spec is None:
+                        continue
+                    od = O.default_detector()
+                    od.luke_sample_fraction = 0.0
+                    O.set_mesh(od, mesh_from_spec(O.Mesh, spec))
+                    ocr = O.germanium()
+                    ocr.e_trapping_mfp_m = ocr.h_trapping_mfp_m = 0.5
+                    out["mesh"][name]["cpu"] = cpu_pair("pair", 2000, 200, args.seed, host, crystal=ocr, detector=od)
             if "deposit" in out:
                 sys.path.insert(0, os.path.join(ROOT, "oracle"))
                 import oracle_ctypes as O

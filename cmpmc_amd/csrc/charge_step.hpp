@@ -28,6 +28,7 @@ template <class P> struct CTrack {
     uint64_t key;
     uint32_t spawn_no, hit_seq;
     int32_t cell;                // MeshX: 1 + the mesh cell of the last field lookup (0: none)
+    MeshCache<P> mc;             // MeshX: the cell's field between walks
 };
 
 
@@ -149,6 +150,7 @@ __device__ inline void make_charge(const DevModel& m, CTrack<P>& c, int type, V3
     c.spawn_no = 0;
     c.hit_seq = 0;
     c.cell = 0;
+    mesh_cache_reset(c.mc);
 }
 
 // charge_transport.hpp:82-106
@@ -374,8 +376,10 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
     using R = typename P::R;
     ctx.ccount(CC_STEPS);
     CField<P> f;                  // engine.hpp:213: the field once per step, at the step's start
-    if constexpr (P::mesh) f = mesh_field<P>(m, trk.pos, trk.cell);
-    else if constexpr (P::exact) f.E = V3<R>{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
+    if constexpr (P::mesh) {
+        const bool inside = (trk.mc.safe > R(0)) || mesh_lookup<P>(m, trk.pos, trk.cell, trk.mc);
+        f = mesh_cell_field<P>(m, trk.pos, trk.cell - 1, inside);
+    } else if constexpr (P::exact) f.E = V3<R>{R(m.efield[0]), R(m.efield[1]), R(m.efield[2])};
     else f.E = V3<R>{m.charge_f.efield[0], m.charge_f.efield[1], m.charge_f.efield[2]};
     const V3<R> E = f.E;
     const R q_sign = (trk.type == 1) ? R(-1) : R(1);
@@ -437,6 +441,7 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             if (!active_pad || !should_absorb<P>(trk, n_in, s, r)) {
                 ctx.ccount(CC_SURF_REFL);
                 trk.pos = trk.pos + n_in * R(1e-11);
+                if constexpr (P::mesh) trk.mc.safe -= dist + R(1e-11);
                 reflect_charge<P>(m, trk, n_in, s.prob_charge_spec, r);
                 return true;
             }
@@ -451,6 +456,7 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
         }
     }
     trk.pos = trk.pos + vm * dt;
+    if constexpr (P::mesh) trk.mc.safe -= d_step;
     trk.time += static_cast<double>(dt);
     trk.p = trk.p + dp;
     trk.ekin = (trk.type == 1) ? ekin_e<P>(m, trk.valley, trk.p) : ekin_h<P>(m, trk.p);

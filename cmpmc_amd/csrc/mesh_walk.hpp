@@ -45,8 +45,21 @@ template <class R> __device__ inline int32_t mesh_start(const DevMesh& g, int32_
     return g.seed[(static_cast<int64_t>(ix[0]) * g.seed_n[1] + ix[1]) * g.seed_n[2] + ix[2]];
 }
 
+// A walk's result besides the cell: the distance the point can move before one of its
+// barycentric coordinates may fall to `margin` (face i is (lambda_i - margin) / |grad
+// lambda_i| away in that sense; negative when some lambda_i <= margin).
+template <class R> struct WalkOut {
+    R safe;
+};
+template <class R> __device__ inline R face_room(R lam, R n2, R margin) {
+    if (!(lam > margin) || !(n2 > R(0))) return R(-1);
+    if constexpr (sizeof(R) == 8) return (lam - margin) / sqrt(n2);
+    else return (lam - margin) * __builtin_amdgcn_rsqf(n2);
+}
+
 // dim 3, FP64 records: mesh_field.hpp:240-247 (bary_coord) and 63-93 (walk)
-__device__ inline bool mesh_walk3(const DevMesh& g, double px, double py, double pz, int32_t& c) {
+__device__ inline bool mesh_walk3(const DevMesh& g, double px, double py, double pz, int32_t& c, WalkOut<double>& w,
+                                  double margin) {
     const int64_t max_steps = static_cast<int64_t>(g.n_cells) + 4;
     for (int64_t step = 0; step < max_steps; ++step) {
         const double2* r = reinterpret_cast<const double2*>(g.rec) + static_cast<int64_t>(c) * 8;
@@ -61,7 +74,16 @@ __device__ inline bool mesh_walk3(const DevMesh& g, double px, double py, double
         if (b1 < mb) { mb = b1; mi = 1; }
         if (b2 < mb) { mb = b2; mi = 2; }
         if (b3 < mb) { mb = b3; mi = 3; }
-        if (mb >= -1e-10) return true;
+        if (mb >= -1e-10) {
+            const double sx = t01.x + t23.y + t67.x, sy = t01.y + t45.x + t67.y, sz = t23.x + t45.y + t8x.x;
+            const double n0 = t01.x * t01.x + t01.y * t01.y + t23.x * t23.x;
+            const double n1 = t23.y * t23.y + t45.x * t45.x + t45.y * t45.y;
+            const double n2 = t67.x * t67.x + t67.y * t67.y + t8x.x * t8x.x;
+            const double n3 = sx * sx + sy * sy + sz * sz;
+            w.safe = tmin(tmin(face_room(b0, n0, margin), face_room(b1, n1, margin)),
+                          tmin(face_room(b2, n2, margin), face_room(b3, n3, margin)));
+            return true;
+        }
         const int32_t next = g.nbr[static_cast<int64_t>(c) * 4 + mi];
         if (next < 0) return false;
         c = next;
@@ -70,7 +92,8 @@ __device__ inline bool mesh_walk3(const DevMesh& g, double px, double py, double
 }
 
 // dim 2, FP64 records: mesh_field.hpp:500-508 and 358-387
-__device__ inline bool mesh_walk2(const DevMesh& g, double pu, double pv, int32_t& c) {
+__device__ inline bool mesh_walk2(const DevMesh& g, double pu, double pv, int32_t& c, WalkOut<double>& w,
+                                  double margin) {
     const int64_t max_steps = static_cast<int64_t>(g.n_cells) + 4;
     for (int64_t step = 0; step < max_steps; ++step) {
         const double2* r = reinterpret_cast<const double2*>(g.rec) + static_cast<int64_t>(c) * 4;
@@ -83,7 +106,13 @@ __device__ inline bool mesh_walk2(const DevMesh& g, double pu, double pv, int32_
         double mb = b0;
         if (b1 < mb) { mb = b1; mi = 1; }
         if (b2 < mb) { mb = b2; mi = 2; }
-        if (mb >= -1e-10) return true;
+        if (mb >= -1e-10) {
+            const double su = t01.x + t23.x, sv = t01.y + t23.y;
+            w.safe = tmin(tmin(face_room(b0, t01.x * t01.x + t01.y * t01.y, margin),
+                               face_room(b1, t23.x * t23.x + t23.y * t23.y, margin)),
+                          face_room(b2, su * su + sv * sv, margin));
+            return true;
+        }
         const int32_t next = g.nbr[static_cast<int64_t>(c) * 4 + mi];
         if (next < 0) return false;
         c = next;
@@ -92,7 +121,8 @@ __device__ inline bool mesh_walk2(const DevMesh& g, double pu, double pv, int32_
 }
 
 // dim 3, FP32 records [t0..t3 | t4..t7 | t8 x y z | Ex Ey Ez iv]
-__device__ inline bool mesh_walk3f(const DevMesh& g, float px, float py, float pz, int32_t& c) {
+__device__ inline bool mesh_walk3f(const DevMesh& g, float px, float py, float pz, int32_t& c, WalkOut<float>& w,
+                                   float margin) {
     const float tol = -g.tol_f;
     const int32_t cap = (g.n_cells < kMeshWalkCap32 - 4) ? g.n_cells + 4 : kMeshWalkCap32;
     for (int32_t step = 0; step < cap; ++step) {
@@ -108,7 +138,14 @@ __device__ inline bool mesh_walk3f(const DevMesh& g, float px, float py, float p
         if (b1 < mb) { mb = b1; mi = 1; }
         if (b2 < mb) { mb = b2; mi = 2; }
         if (b3 < mb) { mb = b3; mi = 3; }
-        if (mb >= tol) return true;
+        if (mb >= tol) {
+            const float sx = a.x + a.w + b.z, sy = a.y + b.x + b.w, sz = a.z + b.y + d.x;
+            const float n0 = a.x * a.x + a.y * a.y + a.z * a.z, n1 = a.w * a.w + b.x * b.x + b.y * b.y;
+            const float n2 = b.z * b.z + b.w * b.w + d.x * d.x, n3 = sx * sx + sy * sy + sz * sz;
+            w.safe = tmin(tmin(face_room(b0, n0, margin), face_room(b1, n1, margin)),
+                          tmin(face_room(b2, n2, margin), face_room(b3, n3, margin)));
+            return true;
+        }
         const int32_t next = g.nbr[static_cast<int64_t>(c) * 4 + mi];
         if (next < 0) return false;
         c = next;
@@ -117,7 +154,8 @@ __device__ inline bool mesh_walk3f(const DevMesh& g, float px, float py, float p
 }
 
 // dim 2, FP32 records [t0..t3 | x y Eu Ev | iv |E| 0 0]
-__device__ inline bool mesh_walk2f(const DevMesh& g, float pu, float pv, int32_t& c) {
+__device__ inline bool mesh_walk2f(const DevMesh& g, float pu, float pv, int32_t& c, WalkOut<float>& w,
+                                   float margin) {
     const float tol = -g.tol_f;
     const int32_t cap = (g.n_cells < kMeshWalkCap32 - 4) ? g.n_cells + 4 : kMeshWalkCap32;
     for (int32_t step = 0; step < cap; ++step) {
@@ -131,7 +169,12 @@ __device__ inline bool mesh_walk2f(const DevMesh& g, float pu, float pv, int32_t
         float mb = b0;
         if (b1 < mb) { mb = b1; mi = 1; }
         if (b2 < mb) { mb = b2; mi = 2; }
-        if (mb >= tol) return true;
+        if (mb >= tol) {
+            const float su = a.x + a.z, sv = a.y + a.w;
+            w.safe = tmin(tmin(face_room(b0, a.x * a.x + a.y * a.y, margin), face_room(b1, a.z * a.z + a.w * a.w, margin)),
+                          face_room(b2, su * su + sv * sv, margin));
+            return true;
+        }
         const int32_t next = g.nbr[static_cast<int64_t>(c) * 4 + mi];
         if (next < 0) return false;
         c = next;
@@ -139,29 +182,63 @@ __device__ inline bool mesh_walk2f(const DevMesh& g, float pu, float pv, int32_t
     return false;
 }
 
-// The field at pos and the carrier's cell (cell1 = 1 + index, updated; inside: the walk
-// ended inside the mesh). Outside the mesh the field is zero (mesh_field.hpp:109, 401).
+// A carrier's view of the mesh between walks: `safe`, the distance it can still move
+// without leaving its cell (the cell itself is CTrack::cell). The walk sets safe =
+// min_i (lambda_i - margin) / |grad lambda_i| (each barycentric coordinate falls by at most
+// |grad lambda_i| per metre moved) and every step subtracts the distance moved; while it
+// is positive the point is inside the same cell by more than the margin (1e-9 in the FP64
+// form, twice the walk tolerance in the FP32 form), where every walk (from any start)
+// ends in that cell, so the walk is skipped and the cell's field read directly. Outside
+// the mesh, or within the margin of a face, the next step walks again. (Only the distance
+// is kept in registers: the field is re-read from the cell's record, which stays in cache.)
+template <class P> struct MeshCache {
+    typename P::R safe;
+};
+
+template <class P> __device__ inline void mesh_cache_reset(MeshCache<P>& mc) { mc.safe = typename P::R(-1); }
+
+// The walk from the carrier's cell (cell1 = 1 + index, updated) and its new safe distance.
+// Returns whether the point is inside the mesh.
 template <class P>
-__device__ inline CField<P> mesh_field(const DevModel& m, const V3<typename P::R>& pos, int32_t& cell1, bool& inside) {
+__device__ inline bool mesh_lookup(const DevModel& m, const V3<typename P::R>& pos, int32_t& cell1, MeshCache<P>& mc) {
     using R = typename P::R;
     const DevMesh& g = m.mesh;
-    CField<P> f;
-    f.E = V3<R>{R(0), R(0), R(0)};
-    f.emag = R(0);
-    if constexpr (P::exact) f.iv = R(g.iv_zero);
-    else f.iv = g.iv_zero_f;
-    inside = false;
-    if (g.n_cells <= 0) return f;
+    mc.safe = R(-1);
+    if (g.n_cells <= 0) return false;
     R q0, q1, q2 = R(0);
     if (g.dim == 3) { q0 = pos.x; q1 = pos.y; q2 = pos.z; }
     else if (g.axisym) { q0 = P::hypot_(pos.x, pos.y); q1 = pos.z; }
     else { q0 = pos.x; q1 = pos.y; }
     int32_t c = mesh_start<R>(g, cell1, q0, q1, q2);
-    if constexpr (P::exact) inside = (g.dim == 3) ? mesh_walk3(g, q0, q1, q2, c) : mesh_walk2(g, q0, q1, c);
-    else inside = (g.dim == 3) ? mesh_walk3f(g, q0, q1, q2, c) : mesh_walk2f(g, q0, q1, c);
-    cell1 = c + 1;
-    if (!inside) return f;
+    WalkOut<R> w;
+    const R margin = P::exact ? R(1e-9) : R(2) * R(g.tol_f);
+    bool inside;
     if constexpr (P::exact) {
+        inside = (g.dim == 3) ? mesh_walk3(g, q0, q1, q2, c, w, margin) : mesh_walk2(g, q0, q1, c, w, margin);
+    } else {
+        inside = (g.dim == 3) ? mesh_walk3f(g, q0, q1, q2, c, w, margin) : mesh_walk2f(g, q0, q1, c, w, margin);
+    }
+    cell1 = c + 1;
+    if (inside) mc.safe = w.safe;
+    return inside;
+}
+
+// The field at pos in cell c (mesh_field.hpp:107-111, 398-418), or zero outside the mesh.
+template <class P> __device__ inline CField<P> mesh_cell_field(const DevModel& m, const V3<typename P::R>& pos,
+                                                               int32_t c, bool inside) {
+    using R = typename P::R;
+    const DevMesh& g = m.mesh;
+    CField<P> f;
+    if (!inside) {
+        f.E = V3<R>{R(0), R(0), R(0)};
+        f.emag = R(0);
+        if constexpr (P::exact) f.iv = R(g.iv_zero);
+        else f.iv = g.iv_zero_f;
+        return f;
+    }
+    R e0, e1;
+    if constexpr (P::exact) {
+        f.emag = R(0);
         if (g.dim == 3) {
             const double2* r = reinterpret_cast<const double2*>(g.rec) + static_cast<int64_t>(c) * 8;
             const double2 e01 = r[6], e2i = r[7];
@@ -170,11 +247,9 @@ __device__ inline CField<P> mesh_field(const DevModel& m, const V3<typename P::R
             return f;
         }
         const double2 gu = reinterpret_cast<const double2*>(g.rec)[static_cast<int64_t>(c) * 4 + 3];
+        e0 = gu.x;
+        e1 = gu.y;
         f.iv = R(-1);                      // from E at the position (it varies within the cell)
-        if (!g.axisym) f.E = V3<R>{gu.x, gu.y, R(0)};
-        else if (q0 > R(1e-15)) f.E = V3<R>{gu.x * (pos.x / q0), gu.x * (pos.y / q0), gu.y};
-        else f.E = V3<R>{R(0), R(0), gu.y};
-        return f;
     } else {
         if (g.dim == 3) {
             const float4 e = reinterpret_cast<const float4*>(g.rec_f)[static_cast<int64_t>(c) * 4 + 3];
@@ -185,20 +260,34 @@ __device__ inline CField<P> mesh_field(const DevModel& m, const V3<typename P::R
         }
         const float4* r = reinterpret_cast<const float4*>(g.rec_f) + static_cast<int64_t>(c) * 3;
         const float4 b = r[1], e = r[2];
+        e0 = b.z;
+        e1 = b.w;
         f.iv = e.x;
         f.emag = e.y;
-        if (!g.axisym) f.E = V3<R>{b.z, b.w, R(0)};
-        else if (q0 > R(1e-15)) {
-            const R s = P::rcp(q0);
-            f.E = V3<R>{b.z * (pos.x * s), b.z * (pos.y * s), b.w};
-        } else f.E = V3<R>{R(0), R(0), b.w};
+    }
+    if (!g.axisym) {
+        f.E = V3<R>{e0, e1, R(0)};
         return f;
     }
+    const R r = P::hypot_(pos.x, pos.y);
+    if (r > R(1e-15)) {
+        if constexpr (P::exact) f.E = V3<R>{e0 * (pos.x / r), e0 * (pos.y / r), e1};
+        else {
+            const R s = P::rcp(r);
+            f.E = V3<R>{e0 * (pos.x * s), e0 * (pos.y * s), e1};
+        }
+    } else {
+        f.E = V3<R>{R(0), R(0), e1};
+    }
+    return f;
 }
 
-template <class P> __device__ inline CField<P> mesh_field(const DevModel& m, const V3<typename P::R>& pos, int32_t& cell1) {
-    bool inside;
-    return mesh_field<P>(m, pos, cell1, inside);
+// One field evaluation without a carrier cache (a fresh walk).
+template <class P>
+__device__ inline CField<P> mesh_field(const DevModel& m, const V3<typename P::R>& pos, int32_t& cell1, bool& inside) {
+    MeshCache<P> mc;
+    inside = mesh_lookup<P>(m, pos, cell1, mc);
+    return mesh_cell_field<P>(m, pos, cell1 - 1, inside);
 }
 
 // The field evaluation of the charge step at n points, for tests and tools: the walk from

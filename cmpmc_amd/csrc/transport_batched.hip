@@ -565,6 +565,7 @@ __device__ inline void bcharge_to_ctrack(const BCharge& b, uint64_t seed, CTrack
     c.spawn_no = 0;
     c.hit_seq = 0;
     c.cell = P::mesh ? static_cast<int32_t>(b._pad) : 0;   // mesh cell (1 + index) of the last lookup
+    mesh_cache_reset(c.mc);
     event = b.event;
     c.rng = P::track_stream(event_base(seed, event), b.key);
 }

@@ -118,6 +118,7 @@ template <class P> __device__ inline void load_charge(const cmpmc_charge_track& 
     c.valley = o.valley; c.type = o.type;
     if constexpr (P::mesh) c.cell = *reinterpret_cast<const int32_t*>(o._pad);
     else c.cell = 0;
+    mesh_cache_reset(c.mc);
 }
 
 // Deposit workloads in the plain form: a generation pass writes each event's partition
