@@ -199,6 +199,29 @@ def roofline(steps_per_launch: float, launch_s: float, device: str) -> dict:
     return r
 
 
+def precision_study(path: str, seed: int) -> None:
+    """Plain FP64 against batched FP32 on the same events, per observable
+    (cmpmc_amd.precision): the headline shape (1e5 events of one 7.5 meV L phonon from the
+    origin) and the differentiated workload's shape (U{1..64} L phonons of 7.5 meV from
+    uniform positions), each with the octahedral table and with the exact Christoffel
+    solve (the batched table form beside the exact runs)."""
+    import cmpmc_amd as cm
+    from cmpmc_amd import precision as P
+    c1 = cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
+    c5 = cm.workload(cm.WL_PHONON_UNIF, mode=0, energy_eV=7.5e-3, n_min=1, n_max=64)
+    out = {"device": cm.device_info(), "seed": seed, "cases": {}}
+    for name, wl, n, kin in (("headline_table", c1, 100_000, 0), ("uniform_count_table", c5, 5_000, 0),
+                             ("headline_exact", c1, 100_000, 1), ("uniform_count_exact", c5, 2_000, 1)):
+        t = time.perf_counter()
+        out["cases"][name] = P.study(cm, wl, n, seed, kin_mode=kin, with_table_fp32=True)
+        c = out["cases"][name]["fp32_vs_fp64"]
+        _log(f"precision {name}: {n} events in {time.perf_counter() - t:.1f} s; z(hits/event) "
+             f"{c['z_hits_per_event']:.2f}, z(time) {c['z_hit_time_us']:.2f}, KS p time {c['ks_p_time']:.3f}")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps({"precision_study": path, "cases": list(out["cases"])}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +243,8 @@ def main():
     ap.add_argument("--waves", type=int, default=0, help="register-budget variant (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lognormal", action="store_true")
+    ap.add_argument("--precision", metavar="PATH", default="",
+                    help="run only the FP32-against-FP64 precision study and write its tables to PATH (JSON)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a 1 GiB device-to-device copy (a kernel of known traffic for HBM counters)")
     args = ap.parse_args()
@@ -227,6 +252,10 @@ def main():
         print(f"bench.py: this build runs on one GPU (--gpus {args.gpus} requested); the path's sharding over "
               "several GPUs is described, not implemented", file=sys.stderr)
         sys.exit(2)
+
+    if args.precision:
+        precision_study(args.precision, args.seed)
+        return
 
     if args.copy_calibration:
         import torch
