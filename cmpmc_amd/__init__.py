@@ -493,13 +493,23 @@ class Lattice:
         check(capi.load().cmpmc_lattice_build(C.byref(self.crystal), self.oct.ctypes.data, self.cdf_lt.ctypes.data,
                                                self.cdf_tt.ctypes.data, C.byref(self.c)))
 
-    KINEMATICS_OCTAHEDRAL, KINEMATICS_EXACT = 0, 1
+    KINEMATICS_OCTAHEDRAL, KINEMATICS_EXACT, KINEMATICS_BILINEAR = 0, 1, 2
+
+    def build_bilinear(self) -> None:
+        """The bilinear (cos theta, phi) v_g table (crystal_lattice.hpp:350-368), which
+        KinematicsMode::Bilinear reads; engines created afterwards carry it to the device."""
+        if getattr(self, "bil", None) is None:
+            self.bil = np.zeros((3, 181, 361, 3))
+            check(capi.load().cmpmc_lattice_build_bilinear(C.byref(self.c), self.bil.ctypes.data))
 
     def set_kinematics_mode(self, kin_mode: int) -> None:
-        """CrystalLattice::set_kinematics_mode: 0 Octahedral (LUT, default), 1 Exact.
-        Engines take the lattice's mode when they are created."""
-        if kin_mode not in (0, 1):
-            raise ValueError("kinematics mode: 0 (Octahedral) or 1 (Exact)")
+        """CrystalLattice::set_kinematics_mode: 0 Octahedral (LUT, default), 1 Exact,
+        2 Bilinear (builds the bilinear table). Engines take the lattice's mode when they
+        are created."""
+        if kin_mode not in (0, 1, 2):
+            raise ValueError("kinematics mode: 0 (Octahedral), 1 (Exact) or 2 (Bilinear)")
+        if kin_mode == 2:
+            self.build_bilinear()
         self.c.kinematics_mode = kin_mode
 
     def map_k_to_vg(self, mode: int, k) -> tuple[np.ndarray, np.ndarray]:
@@ -561,7 +571,8 @@ class Engine:
         check(capi.load().cmpmc_model_cancel(self.ptr, int(value)))
 
     def set_kinematics(self, kin_mode: int) -> None:
-        """Kinematics of this engine's device model: 0 Octahedral LUT, 1 Exact."""
+        """Kinematics of this engine's device model: 0 Octahedral LUT, 1 Exact, 2 Bilinear
+        (for an engine whose lattice had its bilinear table built)."""
         check(capi.load().cmpmc_model_set_kinematics(self.ptr, kin_mode))
 
     def device_map_k_to_vg(self, mode: int, k, form: int = FORM_PLAIN) -> tuple[np.ndarray, np.ndarray]:

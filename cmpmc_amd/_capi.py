@@ -38,14 +38,17 @@ class Crystal(C.Structure):
         "ac_deform_e", "ac_deform_h", "iv_min_energy", "iv_quad_rate", "iv_quad_field", "iv_quad_power",
         "iv_lin_rate0", "iv_lin_rate1", "iv_lin_power", "e_trapping_mfp_m", "h_trapping_mfp_m",
         "e_trap_ion_d0_mfp_m", "e_trap_ion_a_mfp_m", "h_trap_ion_d_mfp_m", "h_trap_ion_a0_mfp_m")] + [
-        ("orient", (dbl * 3) * 3)] + [(n, dbl) for n in ("atomic_Z", "atomic_A", "pair_energy", "fano_factor")]
+        ("orient", (dbl * 3) * 3)] + [(n, dbl) for n in ("atomic_Z", "atomic_A", "pair_energy", "fano_factor")] + [
+        ("n_iv_op", i32), ("_pad_iv", i32), ("iv_deform_eV_m", dbl * 8), ("iv_energy_eV", dbl * 8),
+        ("neutral_impurity_cm3", dbl), ("epsilon_rel", dbl), ("alpha_nonparabolic", dbl)]
 
 
 class Film(C.Structure):
     _fields_ = [(n, dbl) for n in (
         "film_absorption", "film_thickness", "v_sound", "gap_energy", "phonon_lifetime", "phonon_lifetime_slope",
         "low_qp_limit", "high_qp_limit", "subgap_absorption", "absorber_gap", "absorber_eff",
-        "absorber_eff_slope", "temperature")] + [("keep_all_phonons", i32), ("loss_model", i32)]
+        "absorber_eff_slope", "temperature")] + [("keep_all_phonons", i32), ("loss_model", i32),
+                                                 ("phonon_loss_time", dbl)]
 
 
 class Pad(C.Structure):
@@ -73,7 +76,7 @@ class Detector(C.Structure):
                 ("max_bounces", u32), ("russian_roulette_bounces", u32)] + [(n, dbl) for n in (
                     "min_phonon_eV", "luke_sample_fraction", "charge_max_dk_frac", "charge_dt_cap_s",
                     "fermi_level_above_Ev_eV", "russian_roulette_survival")] + [
-                ("charge_stepping", i32), ("_pad_cs", i32), ("charge_sampling_dk_frac", dbl),
+                ("charge_stepping", i32), ("allow_unvalidated", i32), ("charge_sampling_dk_frac", dbl),
                 ("mesh", C.c_void_p)]
 
 
@@ -82,7 +85,7 @@ class Lattice(C.Structure):
                 ("cdf_tt", C.POINTER(dbl)), ("e_cond_mass_eV", dbl), ("h_mass_eV", dbl),
                 ("mass_ratio", dbl * 3), ("hv_sqrt", dbl * 3), ("hv_sqrt_inv", dbl * 3),
                 ("valley_rot", ((dbl * 3) * 3) * 8), ("l0_e", dbl), ("l0_h", dbl), ("k_sound_e", dbl),
-                ("k_sound_h", dbl), ("kinematics_mode", i32), ("_pad", i32)]
+                ("k_sound_h", dbl), ("kinematics_mode", i32), ("_pad", i32), ("bil_lut", C.POINTER(dbl))]
 
 
 PHONON_DTYPE = np.dtype({"names": ["pos", "k_dir", "vg_dir", "energy", "vg_mag", "time", "weight", "bounces", "mode"],
@@ -203,6 +206,7 @@ def load():
     L.cmpmc_detector_default.argtypes = [C.POINTER(Detector)]
     L.cmpmc_lattice_build.argtypes = [C.POINTER(Crystal), C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Lattice)]
     L.cmpmc_map_k_to_vg.argtypes = [C.POINTER(Lattice), C.c_int, u64, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.cmpmc_lattice_build_bilinear.argtypes = [C.POINTER(Lattice), C.c_void_p]
     L.cmpmc_model_create.argtypes = [C.POINTER(Lattice), C.POINTER(Detector), C.POINTER(C.c_void_p)]
     L.cmpmc_model_destroy.argtypes = [C.c_void_p]
     L.cmpmc_model_cancel.argtypes = [C.c_void_p, C.c_int]
@@ -295,7 +299,8 @@ def check(code: int) -> None:
 
 EXPORTED_SYMBOLS = [
     "cmpmc_last_error", "cmpmc_device_info", "cmpmc_abi_sizeof", "cmpmc_crystal_germanium", "cmpmc_film_aluminum",
-    "cmpmc_detector_default", "cmpmc_lattice_build", "cmpmc_map_k_to_vg", "cmpmc_model_create",
+    "cmpmc_detector_default", "cmpmc_lattice_build", "cmpmc_lattice_build_bilinear", "cmpmc_map_k_to_vg",
+    "cmpmc_model_create",
     "cmpmc_model_destroy", "cmpmc_model_cancel", "cmpmc_model_trim", "cmpmc_model_set_kinematics",
     "cmpmc_model_map_k_to_vg", "cmpmc_transport_serial",
     "cmpmc_transport_charges_serial", "cmpmc_run_events", "cmpmc_run_workload", "cmpmc_diff_forward",

@@ -106,6 +106,11 @@ extern "C" int cmpmc_crystal_germanium(cmpmc_crystal* c) {
     c->iv_lin_rate0 = 1.5e6; c->iv_lin_rate1 = 1.5; c->iv_lin_power = 4.0;
     for (int i = 0; i < 3; ++i) c->orient[i][i] = 1.0;
     c->atomic_Z = 32.0; c->atomic_A = 72.63; c->pair_energy = 2.96; c->fano_factor = 0.1057;
+    // IVRateModel::Matrix inputs: the CrystalConfig defaults (crystal_config.hpp:89-91, 104-105)
+    c->n_iv_op = 2;
+    c->iv_deform_eV_m[0] = 3.0e10; c->iv_deform_eV_m[1] = 0.2e10;
+    c->iv_energy_eV[0] = 27.3 * 1e-3; c->iv_energy_eV[1] = 10.3 * 1e-3;
+    c->neutral_impurity_cm3 = 1.0e11; c->epsilon_rel = 16.2; c->alpha_nonparabolic = 0.3;
     return CMPMC_OK;
 }
 
@@ -186,6 +191,47 @@ void* scratch_ptr(int device) {
 // ------------------------------------------------------------------ model
 
 
+// The IVRateModel::Matrix constants of a crystal (charge_kernels.hpp:36-79): the
+// per-call invariants of intervalley_matrix_rate in the reference's operations (host libm,
+// as the reference's), and their FP32 counterparts with energies in eV.
+static void matrix_iv_constants(const cmpmc_crystal& c, double* out) {
+    std::memset(out, 0, kIvmDoubles * sizeof(double));
+    const double hbar_SI = kHbar * kESI;
+    const double hbar2_SI = hbar_SI * hbar_SI;
+    const double m_prod_ratio = c.e_mass[0] * c.e_mass[1] * c.e_mass[2];
+    const double m_dos_ratio = std::cbrt(m_prod_ratio);
+    const double m_dos32_SI = std::sqrt(m_prod_ratio) * std::pow(kMeKg, 1.5);
+    const double n_final_valleys = static_cast<double>(2 * c.n_valleys - 1);
+    const double opt_scale = (n_final_valleys * m_dos32_SI) / (std::sqrt(2.0) * kPi * hbar2_SI * c.density);
+    const int n = std::min(std::max(c.n_iv_op, 0), kIvmOps);
+    float* q = reinterpret_cast<float*>(out + kIvmF);
+    const double sq_e = std::sqrt(kESI);
+    out[kIvmN] = n;
+    for (int i = 0; i < n; ++i) {
+        const double d_op_SI = c.iv_deform_eV_m[i] * kESI;
+        const double e_min_SI = c.iv_energy_eV[i] * kESI;
+        out[kIvmC + i] = opt_scale * ((d_op_SI * d_op_SI) / e_min_SI);
+        out[kIvmE + i] = c.iv_energy_eV[i];
+        q[kIvmFC + i] = static_cast<float>(out[kIvmC + i] * sq_e);
+        q[kIvmFE + i] = static_cast<float>(c.iv_energy_eV[i]);
+    }
+    out[kIvmAlpha] = c.alpha_nonparabolic;
+    q[kIvmFAlpha] = static_cast<float>(c.alpha_nonparabolic);
+    if (c.neutral_impurity_cm3 > 0.0 && c.epsilon_rel > 0.0) {
+        const double n_I_m3 = c.neutral_impurity_cm3 * 1.0e6;
+        const double e_T_eV = 0.75 * m_dos_ratio / c.epsilon_rel;
+        out[kIvmImpA] = 4.0 * std::sqrt(2.0) * n_I_m3 * hbar2_SI;
+        out[kIvmM32] = m_dos32_SI;
+        out[kIvmET] = e_T_eV * kESI;
+        q[kIvmFImp] = static_cast<float>(out[kIvmImpA] / (m_dos32_SI * sq_e));
+        q[kIvmFET] = static_cast<float>(e_T_eV);
+    }
+}
+
+// Doubles of the bilinear table (crystal_lattice.hpp:28-29): 3 modes x 181 x 361 nodes x v_g.
+constexpr size_t kBilNodes = static_cast<size_t>(3) * CMPMC_N_CTHETA * CMPMC_N_PHI;
+constexpr size_t kBilDoubles = kBilNodes * 3;
+
 static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc_detector* det) {
     DevModel& m = M->dm;
     const cmpmc_crystal& c = lat->cfg;
@@ -194,6 +240,9 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     m.cdf_tt = M->d_cdf + CMPMC_N_CDF;
     m.stiff = M->d_stiff;
     m.stiff_f = reinterpret_cast<const float*>(M->d_stiff + 81);
+    m.ivm = M->d_ivm;
+    m.bil = M->d_bil;
+    m.bil_f = M->d_bil ? reinterpret_cast<const float4*>(M->d_bil + kBilDoubles) : nullptr;
     m.kin_mode = lat->kinematics_mode;
     m.density = lat->cfg.density;
     std::memcpy(m.orient, c.orient, sizeof(m.orient));
@@ -239,7 +288,8 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
         const cmpmc_film& f = a.film;
         b.film = DevFilm{f.film_absorption, f.film_thickness, f.v_sound, f.gap_energy, f.phonon_lifetime,
                          f.phonon_lifetime_slope, f.low_qp_limit, f.high_qp_limit, f.subgap_absorption, f.absorber_gap,
-                         f.absorber_eff, f.absorber_eff_slope, f.temperature, f.keep_all_phonons, 0};
+                         f.absorber_eff, f.absorber_eff_slope, f.temperature, f.phonon_loss_time,
+                         f.keep_all_phonons, f.loss_model};
         DevSurfX& x = m.sx[s];
         x.n_pads = a.n_pads;
         x.pad_off = pad_off;
@@ -315,8 +365,9 @@ static int check_detector(const cmpmc_detector* d) {
         if (a.n_spec < 0 || a.n_spec > CMPMC_MAX_POLY || a.n_diff < 0 || a.n_diff > CMPMC_MAX_POLY || a.n_anh < 0 ||
             a.n_anh > CMPMC_MAX_POLY)
             return fail(CMPMC_ERR_INVALID, "surface %d: polynomial length must be in [0, %d]", s, CMPMC_MAX_POLY);
-        if (a.has_film && a.film.loss_model != 0)
-            return fail(CMPMC_ERR_INVALID, "surface %d: only the MeanFreePath film-loss model is supported", s);
+        if (a.has_film && (a.film.loss_model < 0 || a.film.loss_model > 1 || std::isnan(a.film.phonon_loss_time)))
+            return fail(CMPMC_ERR_INVALID, "surface %d: film loss model %d (0 MeanFreePath, 1 KaplanRateCompetition) "
+                        "with a loss time that is a number", s, a.film.loss_model);
         if (a.n_pads < 0 || a.n_pads > CMPMC_MAX_PADS || (a.n_pads > 0 && !a.pads))
             return fail(CMPMC_ERR_INVALID, "surface %d: n_pads must be in [0, %d] with a pad array", s, CMPMC_MAX_PADS);
         for (int i = 0; i < a.n_pads; ++i) {
@@ -373,9 +424,12 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
         return fail(CMPMC_ERR_INVALID, "cmpmc_model_create: null argument");
     *out = nullptr;
     CHK(check_detector(det));
-    if (lat->kinematics_mode != 0 && lat->kinematics_mode != 1)
-        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0) and Exact (1) are available",
+    if (lat->kinematics_mode < 0 || lat->kinematics_mode > 2)
+        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0), Exact (1) and Bilinear (2) are available",
                     lat->kinematics_mode);
+    if (lat->kinematics_mode == 2 && !lat->bil_lut)
+        return fail(CMPMC_ERR_INVALID, "KinematicsMode::Bilinear needs the lattice's bilinear table "
+                                       "(cmpmc_lattice_build_bilinear)");
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CMPMC_ERR_NO_DEVICE, "cmpmc_model_create: no GPU visible");
     cmpmc_model* M = new cmpmc_model();
@@ -399,6 +453,31 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
         for (int i = 0; i < 81; ++i) Cf[i] = static_cast<float>(Cd[i]);
         HIPCHK(hipMemcpy(M->d_stiff, Cd, sizeof(Cd), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(M->d_stiff + 81, Cf, sizeof(Cf), hipMemcpyHostToDevice));
+    }
+    if (lat->cfg.iv_model == 2) {
+        if (lat->cfg.n_iv_op < 0 || lat->cfg.n_iv_op > kIvmOps) {
+            cmpmc_model_destroy(M);
+            return fail(CMPMC_ERR_INVALID, "IVRateModel::Matrix: n_iv_op must be in [0, %d]", kIvmOps);
+        }
+        double ivm[kIvmDoubles];
+        matrix_iv_constants(lat->cfg, ivm);
+        if (hipMalloc(&M->d_ivm, sizeof(ivm)) != hipSuccess) {
+            cmpmc_model_destroy(M);
+            return fail(CMPMC_ERR_HIP, "cmpmc_model_create: allocation failed");
+        }
+        HIPCHK(hipMemcpy(M->d_ivm, ivm, sizeof(ivm), hipMemcpyHostToDevice));
+    }
+    if (lat->bil_lut) {
+        // the FP64 table for the plain form and an FP32 (vx, vy, vz, 0) copy for the batched form
+        std::vector<float> f(kBilNodes * 4, 0.0f);
+        for (size_t i = 0; i < kBilNodes; ++i)
+            for (int c = 0; c < 3; ++c) f[4 * i + c] = static_cast<float>(lat->bil_lut[3 * i + c]);
+        if (hipMalloc(&M->d_bil, kBilDoubles * sizeof(double) + f.size() * sizeof(float)) != hipSuccess) {
+            cmpmc_model_destroy(M);
+            return fail(CMPMC_ERR_HIP, "cmpmc_model_create: bilinear table allocation failed");
+        }
+        HIPCHK(hipMemcpy(M->d_bil, lat->bil_lut, kBilDoubles * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(M->d_bil + kBilDoubles, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     void* hc = nullptr;
     HIPCHK(hipHostMalloc(&hc, sizeof(int32_t), hipHostMallocMapped));
@@ -446,6 +525,8 @@ extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
     if (M->d_oct) (void)hipFree(M->d_oct);
     if (M->d_cdf) (void)hipFree(M->d_cdf);
     if (M->d_stiff) (void)hipFree(M->d_stiff);
+    if (M->d_bil) (void)hipFree(M->d_bil);
+    if (M->d_ivm) (void)hipFree(M->d_ivm);
     if (M->d_pads) (void)hipFree(M->d_pads);
     if (M->d_amm) (void)hipFree(M->d_amm);
     if (M->d_mesh) (void)hipFree(M->d_mesh);
@@ -521,8 +602,12 @@ extern "C" int cmpmc_model_trim(cmpmc_model* M) {
 
 extern "C" int cmpmc_model_set_kinematics(cmpmc_model* M, int mode) {
     if (!M) return fail(CMPMC_ERR_INVALID, "null model");
-    if (mode != 0 && mode != 1)
-        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0) and Exact (1) are available", mode);
+    if (mode < 0 || mode > 2)
+        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0), Exact (1) and Bilinear (2) are available",
+                    mode);
+    if (mode == 2 && !M->d_bil)
+        return fail(CMPMC_ERR_INVALID, "KinematicsMode::Bilinear: the model was created from a lattice without the "
+                                       "bilinear table (cmpmc_lattice_build_bilinear)");
     std::lock_guard<std::mutex> lk(M->mu);
     M->dm.kin_mode = mode;
     return CMPMC_OK;
@@ -932,7 +1017,7 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
-    const int flags = (M->dm.kin_mode == 1 ? 1 : 0) | (M->dm.surf_x ? 2 : 0) | (M->dm.mesh.dim != 0 ? 4 : 0);
+    const int flags = (M->dm.kin_mode != 0 ? 1 : 0) | (M->dm.surf_x ? 2 : 0) | (M->dm.mesh.dim != 0 ? 4 : 0);
     const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, flags, block, &per_cu)
                                       : cmpmc_occupancy(which, waves, flags, block, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;
@@ -1257,7 +1342,7 @@ int check_workload(const cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_ev
             o.max_sampled_phonons > (1u << 20))
             return fail(CMPMC_ERR_INVALID, "deposit workload: max_sampled_pairs and max_sampled_phonons must be in "
                         "[1, 2^20]");
-        if (M->dm.iv_model == 2)
+        if (M->dm.iv_model == 2 && !M->det.allow_unvalidated)
             return fail(CMPMC_ERR_UNVALIDATED,
                         "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
         if (n_events >= (1ull << 32)) return fail(CMPMC_ERR_INVALID, "workload: at most 2^32 - 1 events per call");
@@ -1268,7 +1353,7 @@ int check_workload(const cmpmc_model* M, const cmpmc_workload* wl, uint64_t n_ev
     if (!mode_ok || !(wl->energy_eV >= 0.0) || wl->n_min > wl->n_max ||
         (wl->kind == CMPMC_WL_PHONON_LOGN && !(wl->logn_median > 0.0)))
         return fail(CMPMC_ERR_INVALID, "workload: invalid parameters");
-    if (wl->kind == CMPMC_WL_PAIR && M->dm.iv_model == 2)
+    if (wl->kind == CMPMC_WL_PAIR && M->dm.iv_model == 2 && !M->det.allow_unvalidated)
         return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
     if (n_events >= (1ull << 32)) return fail(CMPMC_ERR_INVALID, "workload: at most 2^32 - 1 events per call");
     return CMPMC_OK;
@@ -1336,7 +1421,7 @@ extern "C" int cmpmc_run_events(cmpmc_model* M, const cmpmc_event_batch* b, cons
     const uint64_t nc = b->charges ? b->charge_offsets[b->n_events] : 0;
     CHK(validate_tracks(b->tracks, nt));
     CHK(validate_charges(M->dm, b->charges, nc));
-    if (nc && M->dm.iv_model == 2)
+    if (nc && M->dm.iv_model == 2 && !M->det.allow_unvalidated)
         return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
     Job J;
     J.n_events = b->n_events;
@@ -1632,7 +1717,7 @@ extern "C" int cmpmc_transport_charges_serial(cmpmc_model* M, const cmpmc_charge
                                               uint64_t hit_cap, uint64_t* n_hits, cmpmc_sim_stats* stats, void* stream) {
     if (!M || (n && !charges) || !rng_state || !n_hits || !stats || !n_phonons)
         return fail(CMPMC_ERR_INVALID, "cmpmc_transport_charges_serial: null argument");
-    if (M->dm.iv_model == 2)
+    if (M->dm.iv_model == 2 && !M->det.allow_unvalidated)
         return fail(CMPMC_ERR_UNVALIDATED, "CMPMC: 'IVRateModel::Matrix' is not validated; set allow_unvalidated = true to run it anyway");
     if (n == 0) return CMPMC_OK;
     std::lock_guard<std::mutex> lk(M->mu);

@@ -95,6 +95,8 @@ struct cmpmc_model {
     float4* d_oct = nullptr;
     double* d_cdf = nullptr;
     double* d_stiff = nullptr;              // C_ijlm: 81 doubles, then 81 floats
+    double* d_bil = nullptr;                // bilinear v_g table (FP64), then its FP32 float4 copy
+    double* d_ivm = nullptr;                // IVRateModel::Matrix constants (kIvmDoubles)
     cmpmc_pad* d_pads = nullptr;            // electrode pads of all surfaces
     float* d_amm = nullptr;                 // FP32 AMM film-entry tables [3][2][kAmmTab + 1]
     char* d_mesh = nullptr;                 // FEM mesh field: FP64 and FP32 records, neighbours, seeds

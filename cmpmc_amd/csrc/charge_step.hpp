@@ -184,9 +184,46 @@ template <class P> __device__ inline typename P::R luke_rate(const DevModel& m, 
     return (tau > R(0)) ? (R(1) / tau) : R(0);
 }
 
+// IVRateModel::Matrix: intervalley_matrix_rate (charge_kernels.hpp:36-79) from the host's
+// constants; FP64 in the reference's operations, FP32 with energies in eV.
+template <class P> __device__ __noinline__ typename P::R iv_matrix_rate(const double* c, typename P::R ekin) {
+    using R = typename P::R;
+    if (!(ekin > R(0))) return R(0);
+    const int n = static_cast<int>(c[kIvmN]);
+    if constexpr (P::exact) {
+        double opt = 0.0;
+        for (int i = 0; i < n; ++i) {
+            const double e_min = c[kIvmE + i];
+            if (ekin <= e_min || e_min <= 0.0) continue;
+            const double e = ekin - e_min;            // nonparabolic_energy_func_SI (charge_kernels.hpp:29-34)
+            const double e_J = e * kESI;
+            const double aE = c[kIvmAlpha] * e;
+            opt += c[kIvmC + i] * (sqrt(e_J * (1.0 + aE)) * (1.0 + 2.0 * aE));
+        }
+        double imp = 0.0;
+        if (c[kIvmImpA] > 0.0) {
+            const double e_J = ekin * kESI;
+            imp = (c[kIvmImpA] * sqrt(e_J)) / (c[kIvmM32] * (e_J + c[kIvmET]));
+        }
+        return opt + imp;
+    } else {
+        const float* q = reinterpret_cast<const float*>(c + kIvmF);
+        float opt = 0.0f;
+        for (int i = 0; i < n; ++i) {
+            const float e_min = q[kIvmFE + i];
+            if (ekin <= e_min || e_min <= 0.0f) continue;
+            const float e = ekin - e_min, aE = q[kIvmFAlpha] * e;
+            opt += q[kIvmFC + i] * (P::sqrt_(e * (1.0f + aE)) * (1.0f + 2.0f * aE));
+        }
+        const float imp = q[kIvmFImp] * P::div(P::sqrt_(ekin), ekin + q[kIvmFET]);
+        return opt + imp;
+    }
+}
+
 // charge_kernels.hpp:82-99
 template <class P> __device__ inline typename P::R iv_rate(const DevModel& m, typename P::R ekin, const CField<P>& f) {
     using R = typename P::R;
+    if (m.iv_model == 2) return (m.n_valleys < 2) ? R(0) : iv_matrix_rate<P>(m.ivm, ekin);
     if constexpr (!P::exact) {   // the rate at the uniform field / the cell's field (0: one valley)
         if constexpr (P::mesh) return (ekin < m.charge_f.iv_min_energy) ? R(0) : f.iv;
         else return (ekin < m.charge_f.iv_min_energy) ? R(0) : m.charge_f.iv_rate;

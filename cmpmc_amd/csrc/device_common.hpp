@@ -34,17 +34,22 @@ constexpr double kHbar = 6.582119569e-16;
 constexpr double kCLight = 299792458.0;
 constexpr double kHbarc = kHbar * kCLight;
 constexpr double kBoltzmann = 8.617333262e-5;
+constexpr double kESI = 1.602176634e-19;
+constexpr double kMeKg = 9.1093837015e-31;
 
 // ------------------------------------------------------------------ device model
 struct DevFilm {
     double film_absorption, film_thickness, v_sound, gap, tau0, slope, low_qp_limit,
         high_qp_limit, subgap_absorption, absorber_gap, absorber_eff, absorber_eff_slope,
         temperature;
-    int32_t keep_all_phonons, _pad;
+    double loss_time;           // FilmLossModel::KaplanRateCompetition: tau_l [s] (<= 0: no loss)
+    int32_t keep_all_phonons;
+    int32_t loss_model;         // FilmLossModel: 0 MeanFreePath, 1 KaplanRateCompetition
     // FP32 constants of the batched form's cascade (fill_fp32_constants): 1 / gap, the
     // escape exponent per unit path fraction thickness / (v_sound tau0), the absorber
-    // efficiency and its slope
-    float inv_gap_f, esc_c_f, eff_f, eff_slope_f, slope_f, _pad_f;
+    // efficiency and its slope, and tau_l / (pi tau0) (rate competition: g tau_l is the
+    // pair-breaking integral in units of the gap times this)
+    float inv_gap_f, esc_c_f, eff_f, eff_slope_f, slope_f, loss_c_f;
 };
 
 struct DevSurface {
@@ -91,6 +96,19 @@ struct DevMesh {
                                 // coordinate is >= -tol_f (bound of their FP32 rounding)
 };
 
+// IVRateModel::Matrix constants (charge_kernels.hpp:36-79), computed on the host from the
+// crystal (in the reference's operations, so the FP64 rate is the reference's) and read by
+// the charge step. FP64 part (doubles): the phonon count, per phonon c_i = opt_scale
+// (D_i^2 / E_i) [SI] and the threshold E_i [eV], the non-parabolicity, the impurity
+// numerator 4 sqrt(2) n_I hbar^2, m_dos^(3/2) and E_T [J] (numerator 0: no impurity term).
+// FP32 part (floats from double kIvmF): per phonon c_i sqrt(e) and E_i, the
+// non-parabolicity, the impurity term's constant and E_T [eV]: rates with energies in eV.
+constexpr int kIvmOps = 8;
+constexpr int kIvmN = 0, kIvmC = 1, kIvmE = 1 + kIvmOps, kIvmAlpha = 1 + 2 * kIvmOps, kIvmImpA = kIvmAlpha + 1,
+              kIvmM32 = kIvmImpA + 1, kIvmET = kIvmM32 + 1, kIvmF = 24;
+constexpr int kIvmFC = 0, kIvmFE = kIvmOps, kIvmFAlpha = 2 * kIvmOps, kIvmFImp = kIvmFAlpha + 1, kIvmFET = kIvmFImp + 1;
+constexpr int kIvmDoubles = kIvmF + 12;
+
 // Passed by value as a kernel argument (read through the scalar cache).
 struct DevModel {
     const float4* oct;          // [3][256][256] (vx, vy, vz, |v|)
@@ -98,7 +116,11 @@ struct DevModel {
     const float* amm_tab;       // [3 surfaces][L, T][kAmmTab + 1] AMM entry probabilities (FP32 form)
     const double* stiff;        // C_ijlm [3][3][3][3] (exact kinematics)
     const float* stiff_f;       // the same in FP32
-    int32_t kin_mode;           // CrystalLattice::KinematicsMode (host side selects the kernel policy)
+    const double* ivm;          // IVRateModel::Matrix constants (kIvm* layout; NULL: other models)
+    const double* bil;          // bilinear (cos theta, phi) v_g table [3][181][361][3] (NULL: none)
+    const float4* bil_f;        // the same in FP32, one (vx, vy, vz, 0) per node
+    int32_t kin_mode;           // CrystalLattice::KinematicsMode: 0 table (the default kernels), 1 exact,
+                                // 2 bilinear (both in the kin_exact kernel instantiations)
     int32_t orient_identity;    // orient is the identity (Ge's default): the table lookup skips both rotations
     double density;
     const double* cdf_lt;       // [1024]
@@ -677,14 +699,57 @@ __device__ __noinline__ void map_k_to_vg_exact32(const DevModel& m, int mode, co
     *vg_mag = mag;
 }
 
-// CrystalLattice::map_k_to_vg: the octahedral LUT (default) or, for the Exact policies,
-// the per-call solve.
+// KinematicsMode::Bilinear (crystal_lattice.hpp:89-114): the direction in the crystal frame
+// as (cos theta, phi), bilinear weights over the 181 x 361 table and the normalised
+// interpolated v_g rotated back. The FP64 form follows the reference's operations; the FP32
+// form reads the FP32 copy of the table.
+template <class P>
+__device__ inline void map_k_to_vg_bilinear(const DevModel& m, int mode, V3<typename P::R> kdet,
+                                            V3<typename P::R>& vg_dir, typename P::R& vg_mag) {
+    using R = typename P::R;
+    const V3<R> k = unit<P>(mat_mul_t(m.orient, kdet));
+    const R ct = tclamp(k.z, R(-1), R(1));
+    R phi = atan2(k.y, k.x);
+    if (phi < R(0)) phi += R(kTwoPi);
+    const R u = (ct + R(1)) * R(0.5) * R(CMPMC_N_CTHETA - 1);
+    const R v = (phi / R(kTwoPi)) * R(CMPMC_N_PHI - 1);
+    const int i0 = tclamp(static_cast<int>(u), 0, CMPMC_N_CTHETA - 2);
+    const int j0 = tclamp(static_cast<int>(v), 0, CMPMC_N_PHI - 2);
+    const R du = u - R(i0), dv = v - R(j0);
+    const R w00 = (R(1) - du) * (R(1) - dv), w10 = du * (R(1) - dv), w01 = (R(1) - du) * dv, w11 = du * dv;
+    const size_t base = (static_cast<size_t>(mode) * CMPMC_N_CTHETA + i0) * CMPMC_N_PHI + j0;
+    V3<R> vc;
+    if constexpr (sizeof(R) == 8) {
+        const double* e00 = m.bil + base * 3;
+        const double* e10 = e00 + 3 * CMPMC_N_PHI;
+        const double* e01 = e00 + 3;
+        const double* e11 = e10 + 3;
+        vc = V3<R>{e00[0] * w00 + e10[0] * w10 + e01[0] * w01 + e11[0] * w11,
+                   e00[1] * w00 + e10[1] * w10 + e01[1] * w01 + e11[1] * w11,
+                   e00[2] * w00 + e10[2] * w10 + e01[2] * w01 + e11[2] * w11};
+    } else {
+        const float4 e00 = m.bil_f[base], e01 = m.bil_f[base + 1], e10 = m.bil_f[base + CMPMC_N_PHI],
+                     e11 = m.bil_f[base + CMPMC_N_PHI + 1];
+        vc = V3<R>{e00.x * w00 + e10.x * w10 + e01.x * w01 + e11.x * w11,
+                   e00.y * w00 + e10.y * w10 + e01.y * w01 + e11.y * w11,
+                   e00.z * w00 + e10.z * w10 + e01.z * w01 + e11.z * w11};
+    }
+    const R mag = norm<P>(vc);
+    const V3<R> t = vdiv<P>(vc, tmax(mag, R(1e-12)));
+    vg_dir = unit<P>(mat_mul(m.orient, t));
+    vg_mag = mag;
+}
+
+// CrystalLattice::map_k_to_vg: the octahedral LUT (default) or, for the kin_exact
+// policies, the per-call solve or the bilinear table (the model's kinematics mode).
 template <class P>
 __device__ inline void map_k_to_vg(const DevModel& m, int mode, V3<typename P::R> kdet, V3<typename P::R>& vg_dir,
                                    typename P::R& vg_mag) {
     using R = typename P::R;
     if constexpr (!P::kin_exact) {
         map_k_to_vg_oct<P>(m, mode, kdet, vg_dir, vg_mag);
+    } else if (m.kin_mode == 2) {
+        map_k_to_vg_bilinear<P>(m, mode, kdet, vg_dir, vg_mag);
     } else {
         const R kk[3] = {kdet.x, kdet.y, kdet.z};
         R d[3];
@@ -894,7 +959,7 @@ inline void fill_fp32_constants(DevModel& m) {
         fl.eff_f = static_cast<float>(fl.absorber_eff);
         fl.eff_slope_f = static_cast<float>(fl.absorber_eff_slope);
         fl.slope_f = static_cast<float>(fl.slope);
-        fl._pad_f = 0.0f;
+        fl.loss_c_f = (fl.tau0 > 0.0) ? static_cast<float>(fl.loss_time / (kPi * fl.tau0)) : 0.0f;
         DevModel::SurfF& f = m.surf_f[k];
         f.film_gap2 = 2.0 * fl.gap;
         f.film_abs_gap2 = 2.0 * fl.absorber_gap;
@@ -991,6 +1056,48 @@ template <class P> __device__ inline double film_escape_probability(const DevFil
     }
 }
 
+// KaplanFilm::kaplan_pair_breaking_rate (kaplan_film.hpp:69-84) in FP64, the reference's
+// 48-point quadrature in its operation order.
+__device__ inline double film_pair_breaking_rate64(const DevFilm& f, double omega) {
+    const double d = f.gap;
+    if (d <= 0.0 || f.tau0 <= 0.0 || omega < 2.0 * d) return 0.0;
+    const double s = omega - 2.0 * d;
+    double acc = 0.0;
+    for (int i = 0; i < 48; ++i) {
+        const double t = kPi * (i + 0.5) / 48;
+        const double E = d + 0.5 * s * (1.0 - cos(t));
+        acc += (E * (omega - E) + d * d) / sqrt((E + d) * (omega - E + d));
+    }
+    return acc * kPi / 48 / (kPi * d * f.tau0);
+}
+
+// KaplanFilm::loss_probability (kaplan_film.hpp:91-99): rate competition
+// tau_pb / (tau_pb + tau_l), or the mean-free-path escape probability.
+template <class P> __device__ inline double film_loss_probability(const DevFilm& f, double E, double frac) {
+    if (f.loss_model == 1) {
+        if (f.loss_time <= 0.0) return 0.0;
+        if constexpr (P::exact) {
+            const double g = film_pair_breaking_rate64(f, E);
+            if (g <= 0.0) return 1.0;
+            return 1.0 / (1.0 + g * f.loss_time);
+        } else {
+            // the same quadrature in FP32 in units of the gap: g tau_l = I(x) tau_l / (pi tau0)
+            const float x = static_cast<float>(E) * f.inv_gap_f;
+            if (!(f.gap > 0.0) || !(f.tau0 > 0.0) || x < 2.0f) return 1.0;
+            const float sx = x - 2.0f;
+            float acc = 0.0f;
+            for (int i = 0; i < 48; ++i) {
+                const float e = 1.0f + 0.5f * sx * (1.0f - __builtin_amdgcn_cosf((i + 0.5f) * (0.5f / 48.0f)));
+                acc += (e * (x - e) + 1.0f) * __builtin_amdgcn_rsqf((e + 1.0f) * (x - e + 1.0f));
+            }
+            const float gt = acc * (3.14159265f / 48.0f) * f.loss_c_f;
+            if (!(gt > 0.0f)) return 1.0;
+            return static_cast<double>(P::rcp(1.0f + gt));
+        }
+    }
+    return film_escape_probability<P>(f, E, frac);
+}
+
 template <class P, class G> __device__ inline bool film_direct(const DevFilm& f, double E, G& r) {
     using R = typename P::R;
     if (E < 2.0 * f.absorber_gap) return false;
@@ -1085,7 +1192,8 @@ template <class P, class G> __device__ inline double film_sample_phonon(const De
     }
 }
 
-// KaplanFilm::absorb_phonon (Seed::Phonon, MeanFreePath loss): returns edep, fills `out`.
+// KaplanFilm::absorb_phonon (Seed::Phonon; MeanFreePath or KaplanRateCompetition loss):
+// returns edep, fills `out`.
 // lists[0..3] provide the storage of the qp / phon / next_phon / next_qp buffers; the
 // generation swaps are done on named copies so that no list is indexed at run time.
 template <class P, class G>
@@ -1093,7 +1201,7 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
     using R = typename P::R;
     out.n = 0;
     if (film_direct<P>(f, energy, r)) return energy;
-    if (energy < 2.0 * f.gap || P::uniform(r) <= R(film_escape_probability<P>(f, energy, 4.0))) {
+    if (energy < 2.0 * f.gap || P::uniform(r) <= R(film_loss_probability<P>(f, energy, 4.0))) {
         out.push(energy);
         return 0.0;
     }
@@ -1141,6 +1249,14 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
                 if (f.temperature > 0.0) {            // is_thermalized
                     const double kT = kBoltzmann * f.temperature;
                     if (E < kT * static_cast<double>(P::neg_log(P::uniform(r)))) continue;
+                }
+                if (f.loss_model == 1) {              // rate competition (kaplan_film.hpp:183-191)
+                    if (E < 2.0 * f.gap || P::uniform(r) < R(film_loss_probability<P>(f, E, 1.0)) ||
+                        gen >= kMaxTrials)
+                        out.push(E);
+                    else
+                        nph.push(E);
+                    continue;
                 }
                 const R u_side = P::uniform(r);
                 const R u_ang = P::uniform(r);

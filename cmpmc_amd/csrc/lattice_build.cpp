@@ -318,11 +318,78 @@ extern "C" int cmpmc_lattice_build(const cmpmc_crystal* cfg, float* oct, double*
     return CMPMC_OK;
 }
 
+// CrystalLattice::build_group_velocity_lut (crystal_lattice.hpp:350-368): a uniform grid in
+// cos(theta) and phi, the Jacobi solve at every node (a node at a pole moved off it).
+extern "C" int cmpmc_lattice_build_bilinear(cmpmc_lattice* lat, double* bil) {
+    if (!lat || !bil) return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_lattice_build_bilinear: null argument");
+    Stiff C;
+    stiffness(lat->cfg, C);
+    constexpr double two_pi = 2.0 * 3.141592653589793;
+    for (int i = 0; i < CMPMC_N_CTHETA; ++i) {
+        const double ct = -1.0 + 2.0 * (static_cast<double>(i) / (CMPMC_N_CTHETA - 1));
+        const double st = std::sqrt(dmax(0.0, 1.0 - ct * ct));
+        for (int j = 0; j < CMPMC_N_PHI; ++j) {
+            const double phi = two_pi * (static_cast<double>(j) / (CMPMC_N_PHI - 1));
+            V k{st * std::cos(phi), st * std::sin(phi), ct};
+            if (st < 1e-7) k = unit(V{1e-6, 0.0, ct});
+            V vg[3];
+            christoffel(C, lat->cfg.density, k, vg);
+            for (int m = 0; m < 3; ++m) {
+                double* e = bil + ((static_cast<size_t>(m) * CMPMC_N_CTHETA + i) * CMPMC_N_PHI + j) * 3;
+                e[0] = vg[m].x;
+                e[1] = vg[m].y;
+                e[2] = vg[m].z;
+            }
+        }
+    }
+    lat->bil_lut = bil;
+    return CMPMC_OK;
+}
+
 extern "C" int cmpmc_map_k_to_vg(const cmpmc_lattice* lat, int mode, uint64_t n, const double* k, double* vg_dir,
                                  double* vg_mag) {
     if (!lat || !lat->oct_lut || (n && (!k || !vg_dir || !vg_mag)) || mode < 0 || mode > 2)
         return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_map_k_to_vg: bad argument");
     const double (*O)[3] = lat->cfg.orient;
+    if (lat->kinematics_mode == 2) {              // KinematicsMode::Bilinear (crystal_lattice.hpp:89-114)
+        if (!lat->bil_lut)
+            return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_map_k_to_vg: KinematicsMode::Bilinear needs the "
+                                                           "bilinear table (cmpmc_lattice_build_bilinear)");
+        constexpr double two_pi = 2.0 * 3.141592653589793;
+        for (uint64_t q = 0; q < n; ++q) {
+            const V kd{k[3 * q], k[3 * q + 1], k[3 * q + 2]};
+            const V kc = unit(V{O[0][0] * kd.x + O[1][0] * kd.y + O[2][0] * kd.z,
+                                O[0][1] * kd.x + O[1][1] * kd.y + O[2][1] * kd.z,
+                                O[0][2] * kd.x + O[1][2] * kd.y + O[2][2] * kd.z});
+            const double ct = (kc.z < -1.0) ? -1.0 : (1.0 < kc.z ? 1.0 : kc.z);
+            double phi = std::atan2(kc.y, kc.x);
+            if (phi < 0.0) phi += two_pi;
+            const double u = (ct + 1.0) * 0.5 * (CMPMC_N_CTHETA - 1);
+            const double v = (phi / two_pi) * (CMPMC_N_PHI - 1);
+            int i0 = static_cast<int>(u), j0 = static_cast<int>(v);
+            i0 = i0 < 0 ? 0 : (i0 > CMPMC_N_CTHETA - 2 ? CMPMC_N_CTHETA - 2 : i0);
+            j0 = j0 < 0 ? 0 : (j0 > CMPMC_N_PHI - 2 ? CMPMC_N_PHI - 2 : j0);
+            const double du = u - i0, dv = v - j0;
+            const double w00 = (1.0 - du) * (1.0 - dv), w10 = du * (1.0 - dv), w01 = (1.0 - du) * dv, w11 = du * dv;
+            const double* e00 = lat->bil_lut + ((static_cast<size_t>(mode) * CMPMC_N_CTHETA + i0) * CMPMC_N_PHI + j0) * 3;
+            const double* e10 = e00 + 3 * CMPMC_N_PHI;
+            const double* e01 = e00 + 3;
+            const double* e11 = e10 + 3;
+            const V vc{e00[0] * w00 + e10[0] * w10 + e01[0] * w01 + e11[0] * w11,
+                       e00[1] * w00 + e10[1] * w10 + e01[1] * w01 + e11[1] * w11,
+                       e00[2] * w00 + e10[2] * w10 + e01[2] * w01 + e11[2] * w11};
+            const double mag = std::sqrt(nrm2(vc));
+            const double inv = 1.0 / dmax(mag, 1e-12);
+            const V t{vc.x * inv, vc.y * inv, vc.z * inv};
+            const V d = unit(V{O[0][0] * t.x + O[0][1] * t.y + O[0][2] * t.z, O[1][0] * t.x + O[1][1] * t.y + O[1][2] * t.z,
+                               O[2][0] * t.x + O[2][1] * t.y + O[2][2] * t.z});
+            vg_dir[3 * q] = d.x;
+            vg_dir[3 * q + 1] = d.y;
+            vg_dir[3 * q + 2] = d.z;
+            vg_mag[q] = mag;
+        }
+        return CMPMC_OK;
+    }
     if (lat->kinematics_mode == 1) {              // KinematicsMode::Exact (crystal_lattice.hpp:77-85)
         Stiff C;
         stiffness(lat->cfg, C);

@@ -987,7 +987,7 @@ using namespace cmpmc_dev;
 
 hipError_t cmpmc_launch_map_query32(const DevModel* m, int mode, uint64_t n, const double* k, double* out,
                                     hipStream_t s) {
-    if (m->kin_mode == 1)
+    if (m->kin_mode != 0)
         hipLaunchKernelGGL(map_query32_kernel<Fp32X>, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, *m,
                            mode, n, k, out);
     else
@@ -1024,7 +1024,7 @@ static const void* charges_fn(int waves, int flags) {
 }
 // bit 2: the FEM mesh field (charge kernels only; the phonon kernels do not read the field)
 static int model_flags(const DevModel* m) {
-    return (m->kin_mode == 1 ? 1 : 0) | (m->surf_x ? 2 : 0) | (m->mesh.dim != 0 ? 4 : 0);
+    return (m->kin_mode != 0 ? 1 : 0) | (m->surf_x ? 2 : 0) | (m->mesh.dim != 0 ? 4 : 0);
 }
 
 extern "C" hipError_t cmpmc_launch_mesh_query32(const DevModel* m, uint64_t n, const double* pts, const int32_t* start,
@@ -1083,7 +1083,7 @@ extern "C" hipError_t cmpmc_launch_workload_fill(const DevModel* m, const cmpmc_
                                                  const DynParams* dyn, hipStream_t s) {
     if (n_tracks == 0) return hipSuccess;
     const dim3 grid(static_cast<unsigned>(std::min<uint64_t>(blocks_for(n_tracks, 256), 1u << 16)));
-    if (m->kin_mode == 1)
+    if (m->kin_mode != 0)
         hipLaunchKernelGGL(workload_fill_kernel<Fp64X>, grid, dim3(256), 0, s, *m, wl, seed, first_event, n_events,
                            offsets, n_tracks, out, ledger, dyn);
     else
@@ -1130,7 +1130,7 @@ extern "C" hipError_t cmpmc_launch_deposit_fill(const DevModel* m, const cmpmc_w
                                                 const uint64_t* ch_off, BTrack* q, BCharge* cq,
                                                 unsigned long long* ledger, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    if (m->kin_mode == 1)
+    if (m->kin_mode != 0)
         hipLaunchKernelGGL(deposit_fill_kernel<Fp64X>, dim3(blocks_for(n, 64)), dim3(64), 0, s, *m, wl, seed,
                            first_event, n, ph_off, ch_off, q, cq, ledger);
     else

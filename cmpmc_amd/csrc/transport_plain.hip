@@ -319,7 +319,7 @@ extern "C" hipError_t cmpmc_launch_map_query(const cmpmc_dev::DevModel* m, int f
                                              const double* k, double* out, hipStream_t s) {
     if (n == 0) return hipSuccess;
     if (fp32) return cmpmc_launch_map_query32(m, mode, n, k, out, s);
-    if (m->kin_mode == 1)
+    if (m->kin_mode != 0)
         hipLaunchKernelGGL(cmpmc_dev::map_query64_kernel<cmpmc_dev::Fp64X>, dim3(static_cast<unsigned>((n + 255) / 256)),
                            dim3(256), 0, s, *m, mode, n, k, out);
     else
@@ -354,7 +354,7 @@ extern "C" hipError_t cmpmc_launch_deposit_stacks(const cmpmc_dev::DevModel* m, 
     using namespace cmpmc_dev;
     if (n == 0) return hipSuccess;
     const dim3 grid(static_cast<unsigned>((n + 63) / 64));
-    if (m->kin_mode == 1)
+    if (m->kin_mode != 0)
         hipLaunchKernelGGL(deposit_stacks_kernel<Fp64X>, grid, dim3(64), 0, s, *m, wl, seed, first_event, n, pstack,
                            poff, cstack, coff, rng_state, injected);
     else
@@ -382,16 +382,16 @@ extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmp
     const unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
     using namespace cmpmc_dev;
     if (m->mesh.dim != 0) {
-        if (m->kin_mode == 1)
+        if (m->kin_mode != 0)
             hipLaunchKernelGGL(transport_plain_kernel<MeshX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
         else
             hipLaunchKernelGGL(transport_plain_kernel<MeshX<Fp64>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
     } else if (m->surf_x) {
-        if (m->kin_mode == 1)
+        if (m->kin_mode != 0)
             hipLaunchKernelGGL(transport_plain_kernel<SurfX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
         else
             hipLaunchKernelGGL(transport_plain_kernel<SurfX<Fp64>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
-    } else if (m->kin_mode == 1) {
+    } else if (m->kin_mode != 0) {
         hipLaunchKernelGGL(transport_plain_kernel<Fp64X>, dim3(grid), dim3(bs), 0, stream, *m, *a);
     } else {
         hipLaunchKernelGGL(transport_plain_kernel<Fp64>, dim3(grid), dim3(bs), 0, stream, *m, *a);

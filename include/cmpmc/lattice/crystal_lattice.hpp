@@ -1,7 +1,7 @@
 // Drop-in CrystalLattice (lattice/crystal_lattice.hpp of the reference). The tables are
-// built on the host by cmpmc_lattice_build (bit-identical to the reference's) and
-// uploaded to the GPU by SimulationEngine. Octahedral (default) and Exact kinematics are
-// available; the Bilinear table is not.
+// built on the host by cmpmc_lattice_build and cmpmc_lattice_build_bilinear (bit-identical
+// to the reference's) and uploaded to the GPU by SimulationEngine. Octahedral (default),
+// Exact and Bilinear kinematics are available.
 #pragma once
 
 #include <memory>
@@ -16,6 +16,8 @@ public:
     using Config = CrystalConfig;
     static constexpr int N_OCT = 256;
     static constexpr int N_CDF = 1024;
+    static constexpr int N_CTHETA = 181;   // cos(theta) in [-1, 1]
+    static constexpr int N_PHI = 361;      // phi in [0, 2 pi]
     struct Kinematics {
         Vec3 vg_dir;
         double vg_mag;
@@ -24,12 +26,13 @@ public:
 
     // crystal_lattice.hpp:45-55
     explicit CrystalLattice(const Config& cfg)
-        : cfg_(cfg), oct_(3 * N_OCT * N_OCT * 4), cdf_lt_(N_CDF), cdf_tt_(N_CDF) {
+        : cfg_(cfg), oct_(3 * N_OCT * N_OCT * 4), bil_(3 * N_CTHETA * N_PHI * 3), cdf_lt_(N_CDF), cdf_tt_(N_CDF) {
         if (cfg.group != BravaisGroup::Cubic || cfg.has_custom_C)
             throw std::invalid_argument("cmpmc_amd: only cubic crystals without a custom stiffness matrix are supported");
         if (cfg.valley_dirs.size() > 8) throw std::invalid_argument("cmpmc_amd: at most 8 valleys");
         const cmpmc_crystal pod = cfg.to_pod();
         check_status(cmpmc_lattice_build(&pod, oct_.data(), cdf_lt_.data(), cdf_tt_.data(), &lat_));
+        check_status(cmpmc_lattice_build_bilinear(&lat_, bil_.data()));
     }
     // The table pointers inside lat_ refer to this object's vectors.
     CrystalLattice(const CrystalLattice&) = delete;
@@ -41,17 +44,17 @@ public:
     // crystal_lattice.hpp:73-75. Engines built on this lattice follow the mode on each call.
     void set_kinematics_mode(KinematicsMode m) noexcept {
         mode_ = m;
-        lat_.kinematics_mode = (m == KinematicsMode::Exact) ? 1 : 0;
+        lat_.kinematics_mode = static_cast<int32_t>(m);
     }
     KinematicsMode kinematics_mode() const noexcept { return mode_; }
 
-    // crystal_lattice.hpp:77-176: the octahedral table or the exact Christoffel solve
-    Kinematics map_k_to_vg(PhononMode mode, const Vec3& k_det) const {
-        if (mode_ == KinematicsMode::Bilinear)
-            throw std::invalid_argument("cmpmc_amd: KinematicsMode::Bilinear is not available (Octahedral, Exact are)");
+    // crystal_lattice.hpp:77-176: the octahedral table, the exact Christoffel solve or the
+    // bilinear table. The host call fails only for a mode outside PhononMode; with a valid
+    // mode it cannot, so this is noexcept as in the reference.
+    Kinematics map_k_to_vg(PhononMode mode, const Vec3& k_det) const noexcept {
         const double k[3] = {k_det.x, k_det.y, k_det.z};
-        double d[3], mag = 0.0;
-        check_status(cmpmc_map_k_to_vg(&lat_, static_cast<int>(mode), 1, k, d, &mag));
+        double d[3] = {0.0, 0.0, 1.0}, mag = 0.0;
+        (void)cmpmc_map_k_to_vg(&lat_, static_cast<int>(mode), 1, k, d, &mag);
         return {Vec3{d[0], d[1], d[2]}, mag};
     }
     Kinematics map_k_to_vg_oct(PhononMode mode, const Vec3& k) const {
@@ -118,6 +121,7 @@ public:
 private:
     Config cfg_;
     std::vector<float> oct_;
+    std::vector<double> bil_;
     std::vector<double> cdf_lt_, cdf_tt_;
     cmpmc_lattice lat_{};
     KinematicsMode mode_ = KinematicsMode::Octahedral;

@@ -3,6 +3,7 @@
 // transport path uses.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <string>
 #include <vector>
@@ -71,6 +72,14 @@ struct CrystalConfig {
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) c.orient[i][j] = orient.m[i][j];
         c.atomic_Z = atomic_Z; c.atomic_A = atomic_A; c.pair_energy = pair_energy; c.fano_factor = fano_factor;
+        const size_t n_op = std::min<size_t>({iv_energy_eV.size(), iv_deform_eV_m.size(), 8});
+        c.n_iv_op = static_cast<int32_t>(n_op);
+        for (size_t i = 0; i < n_op; ++i) {
+            c.iv_deform_eV_m[i] = iv_deform_eV_m[i];
+            c.iv_energy_eV[i] = iv_energy_eV[i];
+        }
+        c.neutral_impurity_cm3 = neutral_impurity_cm3; c.epsilon_rel = epsilon_rel;
+        c.alpha_nonparabolic = alpha_nonparabolic;
         return c;
     }
 };
@@ -93,6 +102,7 @@ struct SuperconductorFilmConfig {
         f.absorber_gap = absorber_gap; f.absorber_eff = absorber_eff; f.absorber_eff_slope = absorber_eff_slope;
         f.temperature = temperature; f.keep_all_phonons = keep_all_phonons ? 1 : 0;
         f.loss_model = static_cast<int32_t>(loss_model);
+        f.phonon_loss_time = phonon_loss_time;
         return f;
     }
 };

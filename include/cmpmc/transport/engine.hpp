@@ -110,6 +110,7 @@ struct DetectorModel {
         d.fermi_level_above_Ev_eV = fermi_level_above_Ev_eV;
         d.russian_roulette_survival = russian_roulette_survival;
         d.charge_stepping = charge_stepping == ChargeStepping::LemanSecondOrder ? 1 : 0;
+        d.allow_unvalidated = allow_unvalidated ? 1 : 0;
         d.charge_sampling_dk_frac = charge_sampling_dk_frac;
         d.mesh = mesh_field_3d.has_value() ? mesh_field_3d->handle()
                  : mesh_field_2d.has_value() ? mesh_field_2d->handle() : nullptr;
@@ -163,15 +164,13 @@ public:
     // The device model, following the lattice's current kinematics mode (the reference
     // reads the mode at every map_k_to_vg call, crystal_lattice.hpp:87-88).
     cmpmc_model* model() const {
-        if (lattice_.kinematics_mode() == CrystalLattice::KinematicsMode::Bilinear)
-            throw std::invalid_argument("cmpmc_amd: KinematicsMode::Bilinear is not available (Octahedral, Exact are)");
         check_status(cmpmc_model_set_kinematics(model_.get(), lattice_.pod().kinematics_mode));
         return model_.get();
     }
 
     // engine.hpp:171-187
     PhononTrack make_track(const Vec3& pos, const Vec3& k_dir, PhononMode mode, double energy_eV, double time_s = 0.0,
-                           double weight = 1.0) const {
+                           double weight = 1.0) const noexcept {
         const Vec3 k = k_dir.unit();
         const auto kin = lattice_.map_k_to_vg(mode, k);
         PhononTrack t;

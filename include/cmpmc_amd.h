@@ -97,7 +97,7 @@ typedef struct {
     double bandgap, h_mass_ratio;
     double e_mass[3];
     int32_t n_valleys;          /* <= 8 */
-    int32_t iv_model;           /* 0 Quadratic, 1 Linear, 2 Matrix (guarded) */
+    int32_t iv_model;           /* 0 Quadratic, 1 Linear, 2 Matrix (runs with detector allow_unvalidated) */
     double valley_dirs[8][3];
     double ac_deform_e, ac_deform_h;
     double iv_min_energy, iv_quad_rate, iv_quad_field, iv_quad_power;
@@ -108,6 +108,13 @@ typedef struct {
     /* energy partition (crystal_config.hpp:46-47, 87-88): target Z and A [g/mol], mean
      * energy per e-/h+ pair [eV], Fano factor */
     double atomic_Z, atomic_A, pair_energy, fano_factor;
+    /* IVRateModel::Matrix (crystal_config.hpp:89-91, 104-105; charge_kernels.hpp:36-79):
+     * n_iv_op optical intervalley phonons (deformation potential [eV/m], threshold energy
+     * [eV]), neutral impurity density [1/cm^3], relative permittivity, conduction-band
+     * non-parabolicity [1/eV] */
+    int32_t n_iv_op, _pad_iv;
+    double iv_deform_eV_m[8], iv_energy_eV[8];
+    double neutral_impurity_cm3, epsilon_rel, alpha_nonparabolic;
 } cmpmc_crystal;
 
 #define CMPMC_N_OCT 256
@@ -125,8 +132,10 @@ typedef struct {
     double mass_ratio[3], hv_sqrt[3], hv_sqrt_inv[3];
     double valley_rot[8][3][3];
     double l0_e, l0_h, k_sound_e, k_sound_h;
-    int32_t kinematics_mode;      /* CrystalLattice::KinematicsMode: 0 Octahedral (default), 1 Exact */
+    int32_t kinematics_mode;      /* CrystalLattice::KinematicsMode: 0 Octahedral (default), 1 Exact,
+                                     2 Bilinear (needs bil_lut) */
     int32_t _pad;
+    const double* bil_lut;        /* bilinear (cos theta, phi) v_g table [3][181][361][3] (NULL: not built) */
 } cmpmc_lattice;
 
 /* superconductor/kaplan_film.hpp:29 KaplanFilm (crystal_config.hpp:240-257 fields) */
@@ -135,7 +144,9 @@ typedef struct {
            phonon_lifetime_slope, low_qp_limit, high_qp_limit, subgap_absorption,
            absorber_gap, absorber_eff, absorber_eff_slope, temperature;
     int32_t keep_all_phonons;
-    int32_t loss_model;         /* 0 MeanFreePath (1 KaplanRateCompetition: not supported) */
+    int32_t loss_model;         /* FilmLossModel: 0 MeanFreePath, 1 KaplanRateCompetition
+                                   (kaplan_film.hpp:91-99, 183-191) */
+    double phonon_loss_time;    /* tau_l [s] of KaplanRateCompetition (<= 0: no loss) */
 } cmpmc_film;
 
 #define CMPMC_MAX_POLY 8
@@ -196,7 +207,8 @@ typedef struct {
     double russian_roulette_survival;
     int32_t charge_stepping;      /* DetectorModel::ChargeStepping (engine.hpp:131-133): 0 FirstOrderDkCap,
                                      1 LemanSecondOrder (engine.hpp:234-254, 337-342) */
-    int32_t _pad_cs;
+    int32_t allow_unvalidated;    /* DetectorModel::allow_unvalidated (engine.hpp:146-147): 1 runs the
+                                     IVRateModel::Matrix rate; 0 rejects it with CMPMC_ERR_UNVALIDATED */
     double charge_sampling_dk_frac; /* LemanSecondOrder sampling-step cap (default 0.25) */
     const cmpmc_mesh* mesh;       /* NULL: the uniform efield. Else the charge step's field is the
                                      mesh's (zero outside it), read at model creation (the model
@@ -429,10 +441,18 @@ double cmpmc_mesh_potential(const cmpmc_mesh* mesh, const double p[3], int32_t* 
 int cmpmc_lattice_build(const cmpmc_crystal* cfg, float* oct_lut, double* cdf_lt, double* cdf_tt,
                         cmpmc_lattice* out);
 
+/* The bilinear (cos theta, phi) group-velocity table of CrystalLattice
+ * (crystal_lattice.hpp:28-29, 350-368): 181 x 361 Jacobi solves per mode into `bil_lut`
+ * (3*181*361*3 doubles, caller-owned), which lat->bil_lut then points to. The lattice
+ * needs it for KinematicsMode::Bilinear (map_k_to_vg, crystal_lattice.hpp:89-114). */
+#define CMPMC_N_CTHETA 181
+#define CMPMC_N_PHI 361
+int cmpmc_lattice_build_bilinear(cmpmc_lattice* lat, double* bil_lut);
+
 /* CrystalLattice::map_k_to_vg on the host (crystal_lattice.hpp:87-88): the octahedral
- * table (:144-176) or, with lat->kinematics_mode == 1, the exact Christoffel solve
- * (:77-85), for n directions k[3*n] -> vg_dir[3*n], vg_mag[n]. Used by make_track in
- * generators. */
+ * table (:144-176), with lat->kinematics_mode == 1 the exact Christoffel solve (:77-85),
+ * with 2 the bilinear table (:89-114), for n directions k[3*n] -> vg_dir[3*n], vg_mag[n].
+ * Used by make_track in generators. */
 int cmpmc_map_k_to_vg(const cmpmc_lattice* lat, int mode, uint64_t n, const double* k,
                       double* vg_dir, double* vg_mag);
 
@@ -453,8 +473,9 @@ int cmpmc_model_trim(cmpmc_model* m);
 /* CrystalLattice::set_kinematics_mode (crystal_lattice.hpp:73-75) for an existing model:
  * 0 Octahedral (the table, default), 1 Exact (per-call Christoffel solve: FP64 Cardano
  * with the Jacobi fallback in the plain form; FP32 Cardano with that FP64 solve near
- * degenerate directions in the batched form). The model takes the lattice's mode at
- * creation. */
+ * degenerate directions in the batched form), 2 Bilinear (the (cos theta, phi) table:
+ * FP64 in the plain form, an FP32 copy in the batched form; the model must have been
+ * created from a lattice with bil_lut). The model takes the lattice's mode at creation. */
 int cmpmc_model_set_kinematics(cmpmc_model* m, int kinematics_mode);
 
 /* map_k_to_vg evaluated on the device with the model's kinematics, in the arithmetic of

@@ -75,13 +75,19 @@ typedef struct {
     double valley_dirs[8][3];
     double ac_deform_e, ac_deform_h;
     double iv_min_energy;
-    int    iv_model;                 /* 0 Quadratic, 1 Linear, 2 Matrix (unsupported) */
+    int    iv_model;                 /* 0 Quadratic, 1 Linear, 2 Matrix */
     double iv_quad_rate, iv_quad_field, iv_quad_power;
     double iv_lin_rate0, iv_lin_rate1, iv_lin_power;
     double e_trapping_mfp_m, h_trapping_mfp_m;
     double e_trap_ion_d0_mfp_m, e_trap_ion_a_mfp_m, h_trap_ion_d_mfp_m, h_trap_ion_a0_mfp_m;
     double orient[3][3];
     double atomic_Z, atomic_A, pair_energy, fano_factor;   /* crystal_config.hpp:46-47, 87-88 */
+    /* IVRateModel::Matrix (crystal_config.hpp:89-91, 104-105): optical intervalley phonons
+     * (deformation potential [eV/m], energy [eV]), neutral impurities, permittivity and
+     * conduction-band non-parabolicity [1/eV] */
+    int n_iv_op, _pad_iv;
+    double iv_deform_eV_m[8], iv_energy_eV[8];
+    double neutral_impurity_cm3, epsilon_rel, alpha_nonparabolic;
 } orc_crystal;
 
 typedef struct {
@@ -89,6 +95,8 @@ typedef struct {
            phonon_lifetime_slope, low_qp_limit, high_qp_limit, subgap_absorption,
            absorber_gap, absorber_eff, absorber_eff_slope, temperature;
     int keep_all_phonons;
+    int loss_model;                  /* FilmLossModel: 0 MeanFreePath, 1 KaplanRateCompetition */
+    double phonon_loss_time;         /* tau_l [s] of KaplanRateCompetition (<= 0: no loss) */
 } orc_film;
 
 void orc_crystal_germanium(orc_crystal* c);
@@ -97,17 +105,20 @@ void orc_film_aluminum(orc_film* f);
 /* ---- Lattice: lattice/crystal_lattice.hpp:24-478 ------------------------------------ */
 #define ORC_N_OCT 256
 #define ORC_N_CDF 1024
+#define ORC_N_CTHETA 181             /* bilinear (cos theta, phi) table: crystal_lattice.hpp:28-29 */
+#define ORC_N_PHI 361
 typedef struct {
     orc_crystal cfg;
     double C[3][3][3][3];
     float* oct;                      /* [3][256][256][4] = vx, vy, vz, |v| */
+    double* bil;                     /* [3][181][361][3] = v_g (KinematicsMode::Bilinear) */
     double cdf_lt[ORC_N_CDF], cdf_tt[ORC_N_CDF];
     /* charge tensors (crystal_lattice.hpp:394-438) */
     double e_cond_mass_eV, h_mass_eV;
     double mass_ratio[3], hv_sqrt[3], hv_sqrt_inv[3];
     double valley_rot[8][3][3];
     double l0_e, l0_h, k_sound_e, k_sound_h;
-    int kin_mode;                    /* KinematicsMode: 0 Octahedral (default), 1 Exact */
+    int kin_mode;                    /* KinematicsMode: 0 Octahedral (default), 1 Exact, 2 Bilinear */
 } orc_lattice;
 
 int  orc_lattice_init(orc_lattice* L, const orc_crystal* cfg);
@@ -119,7 +130,9 @@ void orc_solve_christoffel_cardano(const double C[3][3][3][3], double rho, const
 void orc_map_k_to_vg(const orc_lattice* L, int mode, const double k[3],
                      double vg_dir[3], double* vg_mag);          /* crystal_lattice.hpp:87-88, 144-176 */
 void orc_map_k_to_vg_exact(const orc_lattice* L, int mode, const double k[3],
-                           double vg_dir[3], double* vg_mag);    /* crystal_lattice.hpp:77-85 */
+                           double vg_dir[3], double* vg_mag);
+void orc_map_k_to_vg_bilinear(const orc_lattice* L, int mode, const double k[3],
+                              double vg_dir[3], double* vg_mag);    /* crystal_lattice.hpp:77-85 */
 double orc_tamura_lt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:98-106 */
 double orc_tamura_tt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:115-129 */
 double orc_theta_L_to_LT(const orc_crystal* c, double x);
@@ -268,7 +281,32 @@ void orc_stats_add(orc_stats* a, const orc_stats* b);   /* engine.hpp:64-92 */
 
 /* ---- Kaplan film: superconductor/kaplan_film.hpp:106-323 ------------------------------ */
 /* Returns edep; writes up to 256 re-emitted energies into refl[], count into *n_refl. */
+/* KaplanFilm::kaplan_pair_breaking_rate (kaplan_film.hpp:69-84): the exact T = 0 Kaplan
+ * pair-breaking rate by 48-point Gauss-Chebyshev quadrature */
+/* IVRateModel::Matrix intervalley rate (charge_kernels.hpp:36-79) [1/s] */
+double orc_intervalley_matrix_rate(const orc_crystal* c, double ekin_eV);
+double orc_kaplan_pair_breaking_rate(const orc_film* f, double omega);
+/* KaplanFilm::loss_probability (kaplan_film.hpp:91-99) */
+double orc_film_loss_probability(const orc_film* f, double energy, double thickness_frac);
 double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_refl, orc_rng* r);
+
+/* KaplanSurrogateLUT (kaplan_film.hpp:325-383): N_E = 128 energy nodes on [2 Delta, 40 Delta],
+ * N_S = 512 stored cascade outcomes per node (absorbed fraction, re-emitted energies as
+ * fractions of the incident energy), drawn from Rng(seed); lookup picks a bracketing node
+ * with linear-interpolation probability and an outcome, and falls back to the exact
+ * cascade outside the table. */
+#define ORC_SURR_NE 128
+#define ORC_SURR_NS 512
+typedef struct {
+    orc_film film;
+    float* abs_frac;                 /* [N_E * N_S] */
+    uint32_t* off;                   /* [N_E * N_S + 1] into refl */
+    float* refl;                     /* re-emitted fractions, all outcomes */
+    size_t n_refl, cap_refl;
+} orc_surrogate;
+int orc_surrogate_build(orc_surrogate* s, const orc_film* f, uint64_t seed);
+void orc_surrogate_free(orc_surrogate* s);
+double orc_surrogate_absorb(const orc_surrogate* s, double energy, double* refl, int* n_refl, orc_rng* r);
 double orc_sample_qp_energy(const orc_film* f, double E, orc_rng* r);
 double orc_sample_phonon_energy(const orc_film* f, double E, orc_rng* r);
 
@@ -400,6 +438,7 @@ void orc_engine_free(orc_engine* e);
 const orc_lattice* orc_engine_lattice(const orc_engine* e);
 void orc_engine_set_kinematics(orc_engine* e, int kin_mode);   /* CrystalLattice::set_kinematics_mode */
 const float* orc_engine_oct(const orc_engine* e);
+const double* orc_engine_bil(const orc_engine* e);   /* the bilinear table [3][181][361][3] */
 size_t orc_sizeof(int which);   /* struct sizes, for binding checks */
 
 #ifdef __cplusplus

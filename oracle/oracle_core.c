@@ -143,6 +143,10 @@ void orc_crystal_germanium(orc_crystal* c) {
     c->iv_quad_rate = 6.72e-2; c->iv_quad_field = 217.0; c->iv_quad_power = 3.24;
     c->iv_lin_rate0 = 1.5e6; c->iv_lin_rate1 = 1.5; c->iv_lin_power = 4.0;
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) c->orient[i][j] = (i == j) ? 1.0 : 0.0;
+    c->n_iv_op = 2;                        /* crystal_config.hpp:89-91, 104-105 defaults */
+    c->iv_deform_eV_m[0] = 3.0e10; c->iv_deform_eV_m[1] = 0.2e10;
+    c->iv_energy_eV[0] = 27.3 * 1e-3; c->iv_energy_eV[1] = 10.3 * 1e-3;
+    c->neutral_impurity_cm3 = 1.0e11; c->epsilon_rel = 16.2; c->alpha_nonparabolic = 0.3;
 }
 
 void orc_film_aluminum(orc_film* f) {            /* crystal_config.hpp:240-257 */
@@ -160,6 +164,8 @@ void orc_film_aluminum(orc_film* f) {            /* crystal_config.hpp:240-257 *
     f->absorber_eff_slope = 0.0;
     f->temperature = 0.05e-3;
     f->keep_all_phonons = 1;
+    f->loss_model = 0;
+    f->phonon_loss_time = 0.0;
 }
 
 /* ---------------- Christoffel solver: phonon_kernels.hpp:134-247 ---------------- */
@@ -423,6 +429,29 @@ int orc_lattice_init(orc_lattice* L, const orc_crystal* cfg) {
             }
         }
     }
+    /* bilinear (cos theta, phi) table: crystal_lattice.hpp:350-368 */
+    L->bil = (double*)malloc(sizeof(double) * 3 * ORC_N_CTHETA * ORC_N_PHI * 3);
+    if (!L->bil) return -1;
+    for (int i = 0; i < ORC_N_CTHETA; ++i) {
+        const double ct = -1.0 + 2.0 * ((double)i / (ORC_N_CTHETA - 1));
+        const double st = sqrt(dmax(0.0, 1.0 - ct * ct));
+        for (int j = 0; j < ORC_N_PHI; ++j) {
+            const double phi = ORC_TWO_PI * ((double)j / (ORC_N_PHI - 1));
+            double k[3] = {st * cos(phi), st * sin(phi), ct};
+            if (st < 1e-7) {
+                const double t[3] = {1e-6, 0.0, ct};
+                v3_unit(t, k);
+            }
+            double vg[3][3], vp[3];
+            orc_solve_christoffel((const double(*)[3][3][3])L->C, cfg->density, k, vg, vp);
+            for (int m = 0; m < 3; ++m) {
+                double* e = L->bil + (((size_t)m * ORC_N_CTHETA + i) * ORC_N_PHI + j) * 3;
+                e[0] = vg[m][0];
+                e[1] = vg[m][1];
+                e[2] = vg[m][2];
+            }
+        }
+    }
     /* decay tables: crystal_lattice.hpp:440-446 */
     const double m = cfg->v_sound / cfg->v_trans;
     build_single_cdf(L->cdf_lt, (m - 1.0) / (m + 1.0), 1.0, cfg, 0);
@@ -476,6 +505,8 @@ int orc_lattice_init(orc_lattice* L, const orc_crystal* cfg) {
 void orc_lattice_free(orc_lattice* L) {
     free(L->oct);
     L->oct = NULL;
+    free(L->bil);
+    L->bil = NULL;
 }
 
 /* crystal_lattice.hpp:117-130 */
@@ -506,9 +537,39 @@ void orc_map_k_to_vg_exact(const orc_lattice* L, int mode, const double kdet[3],
     *vg_mag = mag;
 }
 
+/* KinematicsMode::Bilinear: crystal_lattice.hpp:89-114 */
+void orc_map_k_to_vg_bilinear(const orc_lattice* L, int mode, const double kdet[3],
+                              double vg_dir[3], double* vg_mag) {
+    double kc[3], k[3];
+    m3_mul_t(L->cfg.orient, kdet, kc);
+    v3_unit(kc, k);
+    const double ct = dclamp(k[2], -1.0, 1.0);
+    double phi = atan2(k[1], k[0]);
+    if (phi < 0.0) phi += ORC_TWO_PI;
+    const double u = (ct + 1.0) * 0.5 * (ORC_N_CTHETA - 1);
+    const double v = (phi / ORC_TWO_PI) * (ORC_N_PHI - 1);
+    const int i0 = iclamp((int)u, 0, ORC_N_CTHETA - 2);
+    const int j0 = iclamp((int)v, 0, ORC_N_PHI - 2);
+    const double du = u - i0, dv = v - j0;
+    const double w00 = (1.0 - du) * (1.0 - dv), w10 = du * (1.0 - dv), w01 = (1.0 - du) * dv, w11 = du * dv;
+    const double* e00 = L->bil + (((size_t)mode * ORC_N_CTHETA + i0) * ORC_N_PHI + j0) * 3;
+    const double* e10 = e00 + 3 * ORC_N_PHI;
+    const double* e01 = e00 + 3;
+    const double* e11 = e10 + 3;
+    double vc[3];
+    for (int c = 0; c < 3; ++c) vc[c] = e00[c] * w00 + e10[c] * w10 + e01[c] * w01 + e11[c] * w11;
+    const double mag = v3_norm(vc);
+    double t[3], vd[3];
+    v3_div(vc, dmax(mag, 1e-12), t);
+    m3_mul(L->cfg.orient, t, vd);
+    v3_unit(vd, vg_dir);
+    *vg_mag = mag;
+}
+
 void orc_map_k_to_vg(const orc_lattice* L, int mode, const double kdet[3],
                      double vg_dir[3], double* vg_mag) {   /* crystal_lattice.hpp:87-88, 144-176 */
     if (L->kin_mode == 1) { orc_map_k_to_vg_exact(L, mode, kdet, vg_dir, vg_mag); return; }
+    if (L->kin_mode == 2) { orc_map_k_to_vg_bilinear(L, mode, kdet, vg_dir, vg_mag); return; }
     double k[3];
     m3_mul_t(L->cfg.orient, kdet, k);
     double uo, vo;
@@ -660,6 +721,37 @@ static double escape_probability(const orc_film* f, double E, double thickness_f
     return exp(-path / mfp);
 }
 
+/* kaplan_film.hpp:69-84 */
+static double kaplan_pb_integral(const orc_film* f, double omega) {
+    const double d = f->gap_energy;
+    if (omega < 2.0 * d) return 0.0;
+    const double s = omega - 2.0 * d;
+    double acc = 0.0;
+    for (int i = 0; i < 48; ++i) {
+        const double t = ORC_PI * (i + 0.5) / 48;
+        const double E = d + 0.5 * s * (1.0 - cos(t));
+        acc += (E * (omega - E) + d * d) / sqrt((E + d) * (omega - E + d));
+    }
+    return acc * ORC_PI / 48;
+}
+
+double orc_kaplan_pair_breaking_rate(const orc_film* f, double omega) {
+    if (f->gap_energy <= 0.0 || f->phonon_lifetime <= 0.0 || omega < 2.0 * f->gap_energy) return 0.0;
+    return kaplan_pb_integral(f, omega) / (ORC_PI * f->gap_energy * f->phonon_lifetime);
+}
+
+/* kaplan_film.hpp:91-99: rate competition (tau_pb / (tau_pb + tau_l)) or the mean-free-path
+ * escape probability */
+double orc_film_loss_probability(const orc_film* f, double energy, double thickness_frac) {
+    if (f->loss_model == 1) {
+        if (f->phonon_loss_time <= 0.0) return 0.0;
+        const double g = orc_kaplan_pair_breaking_rate(f, energy);
+        if (g <= 0.0) return 1.0;
+        return 1.0 / (1.0 + g * f->phonon_loss_time);
+    }
+    return escape_probability(f, energy, thickness_frac);
+}
+
 static int do_direct_absorption(const orc_film* f, double E, orc_rng* r) {
     if (E < 2.0 * f->absorber_gap) return 0;
     return orc_uniform(r) < f->subgap_absorption;
@@ -726,7 +818,7 @@ double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_
     kbuf out; out.n = 0;
     double edep = 0.0;
     if (do_direct_absorption(f, energy, r)) { *n_refl = 0; return energy; }
-    if (is_subgap(f, energy) || orc_uniform(r) <= escape_probability(f, energy, 4.0)) {
+    if (is_subgap(f, energy) || orc_uniform(r) <= orc_film_loss_probability(f, energy, 4.0)) {
         refl[0] = energy; *n_refl = 1;
         return 0.0;
     }
@@ -766,6 +858,11 @@ double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_
             for (int i = 0; i < phon.n; ++i) {
                 const double E = phon.v[i];
                 if (is_thermalized(f, E, r)) continue;
+                if (f->loss_model == 1) {            /* kaplan_film.hpp:183-191 */
+                    if (is_subgap(f, E) || orc_uniform(r) < orc_film_loss_probability(f, E, 1.0)) kpush(&out, E);
+                    else kpush(&next_phon, E);
+                    continue;
+                }
                 /* kaplan_film.hpp:193-194: the two draws of this expression are taken in
                  * the order the reference build evaluates them (pinned against it). */
                 const double u_side = orc_uniform(r);
@@ -780,6 +877,66 @@ double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_
     for (int i = 0; i < out.n; ++i) refl[i] = out.v[i];
     *n_refl = out.n;
     return edep;
+}
+
+/* ---------------- KaplanSurrogateLUT: kaplan_film.hpp:325-383 ---------------- */
+static double surrogate_node_energy(int ie, double two_delta) {   /* :377-379 */
+    return two_delta * (1.0 + (20.0 - 1.0) * ((double)ie / (ORC_SURR_NE - 1)));
+}
+
+int orc_surrogate_build(orc_surrogate* s, const orc_film* f, uint64_t seed) {   /* :342-356 */
+    memset(s, 0, sizeof(*s));
+    s->film = *f;
+    s->abs_frac = (float*)malloc(sizeof(float) * ORC_SURR_NE * ORC_SURR_NS);
+    s->off = (uint32_t*)calloc(ORC_SURR_NE * ORC_SURR_NS + 1, sizeof(uint32_t));
+    s->cap_refl = 1u << 20;
+    s->refl = (float*)malloc(sizeof(float) * s->cap_refl);
+    if (!s->abs_frac || !s->off || !s->refl) return -1;
+    orc_rng rng;
+    orc_rng_seed(&rng, seed);
+    const double two_delta = 2.0 * dmax(f->gap_energy, 1e-12);
+    double refl[256];
+    int nr = 0;
+    for (int ie = 0; ie < ORC_SURR_NE; ++ie) {
+        const double E = surrogate_node_energy(ie, two_delta);
+        for (int k = 0; k < ORC_SURR_NS; ++k) {
+            const int idx = ie * ORC_SURR_NS + k;
+            s->abs_frac[idx] = (float)(orc_absorb_phonon(f, E, refl, &nr, &rng) / E);
+            for (int i = 0; i < nr; ++i) {
+                if (s->n_refl == s->cap_refl) {
+                    s->cap_refl *= 2;
+                    float* g = (float*)realloc(s->refl, sizeof(float) * s->cap_refl);
+                    if (!g) return -1;
+                    s->refl = g;
+                }
+                s->refl[s->n_refl++] = (float)(refl[i] / E);
+            }
+            s->off[idx + 1] = (uint32_t)s->n_refl;
+        }
+    }
+    return 0;
+}
+
+void orc_surrogate_free(orc_surrogate* s) {
+    free(s->abs_frac); free(s->off); free(s->refl);
+    memset(s, 0, sizeof(*s));
+}
+
+double orc_surrogate_absorb(const orc_surrogate* s, double energy, double* refl, int* n_refl,
+                            orc_rng* r) {   /* absorb_phonon_fast, :359-373 */
+    const orc_film* f = &s->film;
+    const double two_delta = 2.0 * dmax(f->gap_energy, 1e-12);
+    const double u = (energy / two_delta - 1.0) / (20.0 - 1.0) * (ORC_SURR_NE - 1);
+    if (is_subgap(f, energy) || u < 0.0 || u > ORC_SURR_NE - 1)
+        return orc_absorb_phonon(f, energy, refl, n_refl, r);
+    int ie = (int)u < ORC_SURR_NE - 2 ? (int)u : ORC_SURR_NE - 2;
+    if (orc_uniform(r) < (u - ie)) ++ie;
+    const int ks = (int)(orc_uniform(r) * ORC_SURR_NS);
+    const int k = ie * ORC_SURR_NS + (ks < ORC_SURR_NS - 1 ? ks : ORC_SURR_NS - 1);
+    int n = 0;
+    for (uint32_t i = s->off[k]; i < s->off[k + 1] && n < 256; ++i) refl[n++] = energy * s->refl[i];
+    *n_refl = n;
+    return energy * (double)s->abs_frac[k];
 }
 
 void orc_stats_add(orc_stats* a, const orc_stats* b) {

@@ -49,6 +49,8 @@ class Crystal(C.Structure):
         ("h_trap_ion_d_mfp_m", dbl), ("h_trap_ion_a0_mfp_m", dbl),
         ("orient", (dbl * 3) * 3),
         ("atomic_Z", dbl), ("atomic_A", dbl), ("pair_energy", dbl), ("fano_factor", dbl),
+        ("n_iv_op", C.c_int), ("_pad_iv", C.c_int), ("iv_deform_eV_m", dbl * 8), ("iv_energy_eV", dbl * 8),
+        ("neutral_impurity_cm3", dbl), ("epsilon_rel", dbl), ("alpha_nonparabolic", dbl),
     ]
 
 
@@ -57,7 +59,7 @@ class Film(C.Structure):
         "film_absorption", "film_thickness", "v_sound", "gap_energy", "phonon_lifetime",
         "phonon_lifetime_slope", "low_qp_limit", "high_qp_limit", "subgap_absorption",
         "absorber_gap", "absorber_eff", "absorber_eff_slope", "temperature")] + [
-        ("keep_all_phonons", C.c_int)]
+        ("keep_all_phonons", C.c_int), ("loss_model", C.c_int), ("phonon_loss_time", C.c_double)]
 
 
 class Pad(C.Structure):
@@ -227,6 +229,8 @@ def lib():
         L.orc_engine_free.argtypes = [C.c_void_p]
         L.orc_engine_oct.restype = C.POINTER(C.c_float)
         L.orc_engine_oct.argtypes = [C.c_void_p]
+        L.orc_engine_bil.restype = C.POINTER(C.c_double)
+        L.orc_engine_bil.argtypes = [C.c_void_p]
         L.orc_engine_lattice.restype = C.c_void_p
         L.orc_engine_lattice.argtypes = [C.c_void_p]
         L.orc_sizeof.restype = C.c_size_t
@@ -258,6 +262,13 @@ def lib():
         L.orc_next_bits.argtypes = [C.POINTER(Rng)]
         L.orc_map_k_to_vg.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_map_k_to_vg_exact.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_map_k_to_vg_bilinear.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_intervalley_matrix_rate.argtypes = [C.POINTER(Crystal), C.c_double]
+        L.orc_intervalley_matrix_rate.restype = C.c_double
+        L.orc_kaplan_pair_breaking_rate.argtypes = [C.POINTER(Film), C.c_double]
+        L.orc_kaplan_pair_breaking_rate.restype = C.c_double
+        L.orc_film_loss_probability.argtypes = [C.POINTER(Film), C.c_double, C.c_double]
+        L.orc_film_loss_probability.restype = C.c_double
         L.orc_solve_christoffel_cardano.argtypes = [C.c_void_p, dbl, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_engine_set_kinematics.argtypes = [C.c_void_p, C.c_int]
         L.orc_solve_christoffel.argtypes = [C.c_void_p, dbl, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
@@ -269,6 +280,10 @@ def lib():
                   "orc_tamura_lt_pdf", "orc_tamura_tt_pdf"):
             getattr(L, n).restype = dbl
             getattr(L, n).argtypes = [C.POINTER(Crystal), dbl]
+        L.orc_surrogate_build.argtypes = [C.c_void_p, C.POINTER(Film), C.c_uint64]
+        L.orc_surrogate_free.argtypes = [C.c_void_p]
+        L.orc_surrogate_absorb.restype = dbl
+        L.orc_surrogate_absorb.argtypes = [C.c_void_p, dbl, C.POINTER(dbl), C.POINTER(C.c_int), C.POINTER(Rng)]
         L.orc_absorb_phonon.restype = dbl
         L.orc_absorb_phonon.argtypes = [C.POINTER(Film), dbl, C.POINTER(dbl), C.POINTER(C.c_int), C.POINTER(Rng)]
         L.orc_sample_qp_energy.restype = dbl
@@ -364,6 +379,11 @@ def ref():
         R.ref_decay_fractions.argtypes = [u64, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
         R.ref_map_k_to_vg.argtypes = [C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
         R.ref_map_k_to_vg_exact.argtypes = [C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_map_k_to_vg_mode.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_iv_matrix_rate.argtypes = [C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
+        R.ref_surrogate.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.POINTER(dbl), C.POINTER(dbl),
+                                    C.POINTER(C.c_int), C.POINTER(dbl), C.POINTER(C.c_uint64)]
+        R.ref_film_rate_competition.argtypes = [C.c_double, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         R.ref_set_kinematics_mode.argtypes = [C.c_int]
         R.ref_christoffel.argtypes = [C.c_int, C.POINTER(dbl), C.POINTER(dbl)]
         R.ref_absorb_phonon.argtypes = [dbl, u64, C.c_int, C.POINTER(dbl), C.POINTER(C.c_int),
@@ -419,6 +439,30 @@ def trap_ion_crystal() -> Crystal:
     c.e_trapping_mfp_m = c.h_trapping_mfp_m = 0.5
     c.e_trap_ion_d0_mfp_m = c.e_trap_ion_a_mfp_m = c.h_trap_ion_d_mfp_m = c.h_trap_ion_a0_mfp_m = 0.2
     return c
+
+
+def matrix_iv_crystal() -> Crystal:
+    """Germanium with the guarded IVRateModel::Matrix intervalley rate (optical
+    deformation-potential phonons plus neutral impurities, charge_kernels.hpp:36-79) and
+    its default inputs."""
+    c = germanium()
+    c.iv_model = 2
+    return c
+
+
+# tau_l of the rate-competition film runs: loss_time_from_geometry(600 nm, 3260 m/s,
+# eta_t = 0.5) = 4 d / (eta_t v_s) (kaplan_film.hpp:85-90)
+RC_LOSS_TIME = 4.0 * 600e-9 / (0.5 * 3260.0)
+
+
+def rate_competition_detector(loss_time: float = RC_LOSS_TIME) -> Detector:
+    """The default detector with both Al films on FilmLossModel::KaplanRateCompetition
+    (kaplan_film.hpp:91-99, 183-191) and phonon loss time `loss_time`."""
+    d = default_detector()
+    for i in (0, 1):
+        d.surf[i].film.loss_model = 1
+        d.surf[i].film.phonon_loss_time = loss_time
+    return d
 
 
 def default_detector() -> Detector:
@@ -856,21 +900,25 @@ class Engine:
         p = lib().orc_engine_oct(self.ptr)
         return np.ctypeslib.as_array(p, shape=(3, 256, 256, 4)).copy()
 
+    def bil_table(self) -> np.ndarray:
+        p = lib().orc_engine_bil(self.ptr)
+        return np.ctypeslib.as_array(p, shape=(3, 181, 361, 3)).copy()
+
     def lattice_ptr(self):
         return lib().orc_engine_lattice(self.ptr)
 
     def set_kinematics(self, kin_mode: int) -> None:
-        """CrystalLattice::set_kinematics_mode: 0 Octahedral (default), 1 Exact."""
+        """CrystalLattice::set_kinematics_mode: 0 Octahedral (default), 1 Exact, 2 Bilinear."""
         lib().orc_engine_set_kinematics(self.ptr, kin_mode)
 
-    def map_k_to_vg(self, mode: int, k, exact: bool = False) -> np.ndarray:
+    def map_k_to_vg(self, mode: int, k, exact: bool = False, bilinear: bool = False) -> np.ndarray:
         k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
         out = np.empty((k.shape[0], 4))
         vg = np.empty(3)
         mag = C.c_double()
         L = lib()
         lp = self.lattice_ptr()
-        fn = L.orc_map_k_to_vg_exact if exact else L.orc_map_k_to_vg
+        fn = L.orc_map_k_to_vg_exact if exact else (L.orc_map_k_to_vg_bilinear if bilinear else L.orc_map_k_to_vg)
         for i in range(k.shape[0]):
             row = np.ascontiguousarray(k[i])
             fn(lp, mode, dptr(row), dptr(vg), C.byref(mag))

@@ -395,8 +395,48 @@ static double luke_rate(const orc_lattice* L, const orc_charge* t) {   /* charge
     return (tau > 0.0) ? (1.0 / tau) : 0.0;
 }
 
+/* charge_kernels.hpp:29-34: gamma(E) = sqrt(E (1 + alpha E)) (1 + 2 alpha E), E in J */
+static double nonparabolic_energy_func_SI(double e_eV, double alpha_inv_eV) {
+    if (e_eV <= 0.0) return 0.0;
+    const double e_J = e_eV * ORC_E_SI;
+    const double aE = alpha_inv_eV * e_eV;
+    return sqrt(e_J * (1.0 + aE)) * (1.0 + 2.0 * aE);
+}
+
+/* IVRateModel::Matrix, charge_kernels.hpp:36-79: optical deformation-potential intervalley
+ * scattering over the configured phonons plus Erginsoy neutral-impurity scattering [1/s] */
+double orc_intervalley_matrix_rate(const orc_crystal* c, double ekin_eV) {
+    if (c->n_valleys < 2 || ekin_eV <= 0.0) return 0.0;
+    const double hbar_SI = ORC_HBAR * ORC_E_SI;
+    const double hbar2_SI = hbar_SI * hbar_SI;
+    const double m_prod_ratio = c->e_mass[0] * c->e_mass[1] * c->e_mass[2];
+    const double m_dos_ratio = cbrt(m_prod_ratio);
+    const double m_dos32_SI = sqrt(m_prod_ratio) * pow(ORC_M_E_KG, 1.5);
+    const double n_final_valleys = (double)(2 * c->n_valleys - 1);
+    const double opt_scale = (n_final_valleys * m_dos32_SI) / (sqrt(2.0) * ORC_PI * hbar2_SI * c->density);
+    double opt_rate = 0.0;
+    for (int i = 0; i < c->n_iv_op; ++i) {
+        const double e_min_eV = c->iv_energy_eV[i];
+        if (ekin_eV <= e_min_eV || e_min_eV <= 0.0) continue;
+        const double d_op_SI = c->iv_deform_eV_m[i] * ORC_E_SI;
+        const double e_min_SI = e_min_eV * ORC_E_SI;
+        const double e_func = nonparabolic_energy_func_SI(ekin_eV - e_min_eV, c->alpha_nonparabolic);
+        opt_rate += opt_scale * ((d_op_SI * d_op_SI) / e_min_SI) * e_func;
+    }
+    double imp_rate = 0.0;
+    if (c->neutral_impurity_cm3 > 0.0 && c->epsilon_rel > 0.0) {
+        const double n_I_m3 = c->neutral_impurity_cm3 * 1.0e6;
+        const double e_T_eV = 0.75 * m_dos_ratio / c->epsilon_rel;
+        const double e_trk_J = ekin_eV * ORC_E_SI;
+        const double e_T_J = e_T_eV * ORC_E_SI;
+        imp_rate = (4.0 * sqrt(2.0) * n_I_m3 * hbar2_SI * sqrt(e_trk_J)) / (m_dos32_SI * (e_trk_J + e_T_J));
+    }
+    return opt_rate + imp_rate;
+}
+
 static double iv_rate(const orc_crystal* c, double ekin, const double E[3]) {   /* charge_kernels.hpp:82-99 */
     if (c->n_valleys < 2) return 0.0;
+    if (c->iv_model == 2) return orc_intervalley_matrix_rate(c, ekin);
     if (ekin < c->iv_min_energy) return 0.0;
     if (c->iv_model == 0) {
         const double e0 = c->iv_quad_field;
@@ -982,3 +1022,5 @@ size_t orc_sizeof(int which) {
         default: return 0;
     }
 }
+const double* orc_engine_bil(const orc_engine* e) { return e->L->bil; }
+

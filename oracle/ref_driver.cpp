@@ -47,6 +47,8 @@ SurfaceProperty to_surface(const orc_surface& s) {
         f.absorber_eff_slope = s.film.absorber_eff_slope;
         f.temperature = s.film.temperature;
         f.keep_all_phonons = s.film.keep_all_phonons != 0;
+        f.loss_model = static_cast<FilmLossModel>(s.film.loss_model);
+        f.phonon_loss_time = s.film.phonon_loss_time;
         p.kaplan_sensor = KaplanFilm(f);
     }
     if (s.n_pads > 0) {
@@ -141,6 +143,9 @@ DetectorModel to_detector(const orc_detector* d) {
     m.charge_stepping = d->charge_stepping ? DetectorModel::ChargeStepping::LemanSecondOrder
                                            : DetectorModel::ChargeStepping::FirstOrderDkCap;
     m.charge_sampling_dk_frac = d->charge_sampling_dk_frac;
+    // the oracle's runs include the guarded IVRateModel::Matrix (engine.hpp:197-199); the
+    // guard itself is the product's to enforce
+    m.allow_unvalidated = true;
     if (d->mesh) {
         const orc_mesh* g = d->mesh;
         if (g->dim == 3) m.mesh_field_3d = mesh3_from(g->x, static_cast<long>(g->n_vertices), g->v, g->cells,
@@ -180,6 +185,12 @@ CrystalConfig to_crystal(const orc_crystal* c) {
     g.h_trap_ion_a0_mfp_m = c->h_trap_ion_a0_mfp_m;
     g.B_iso = c->B_iso;
     g.A_anh = c->A_anh;
+    g.iv_model = static_cast<IVRateModel>(c->iv_model);
+    g.iv_deform_eV_m.assign(c->iv_deform_eV_m, c->iv_deform_eV_m + c->n_iv_op);
+    g.iv_energy_eV.assign(c->iv_energy_eV, c->iv_energy_eV + c->n_iv_op);
+    g.neutral_impurity_cm3 = c->neutral_impurity_cm3;
+    g.epsilon_rel = c->epsilon_rel;
+    g.alpha_nonparabolic = c->alpha_nonparabolic;
     return g;
 }
 
@@ -283,7 +294,57 @@ void ref_map_k_to_vg_exact(int mode, int n, const double* k, double* out) {
     }
 }
 
-// Kinematics mode of the lattices that ref_run_events builds (0 Octahedral, 1 Exact).
+// map_k_to_vg with a given KinematicsMode (0 Octahedral, 1 Exact, 2 Bilinear) for n
+// directions: out = n*4 (vx, vy, vz, |v|).
+void ref_map_k_to_vg_mode(int kin_mode, int mode, int n, const double* k, double* out) {
+    static CrystalLattice lat(materials::Germanium());
+    lat.set_kinematics_mode(static_cast<CrystalLattice::KinematicsMode>(kin_mode));
+    for (int i = 0; i < n; ++i) {
+        const auto kin = lat.map_k_to_vg(static_cast<PhononMode>(mode), Vec3{k[3 * i], k[3 * i + 1], k[3 * i + 2]});
+        out[4 * i] = kin.vg_dir.x; out[4 * i + 1] = kin.vg_dir.y; out[4 * i + 2] = kin.vg_dir.z; out[4 * i + 3] = kin.vg_mag;
+    }
+}
+
+// physics::intervalley_matrix_rate (IVRateModel::Matrix) of Germanium's defaults at n
+// kinetic energies [eV].
+void ref_iv_matrix_rate(int n, const double* E, double* out) {
+    const CrystalConfig cfg = materials::Germanium();
+    for (int i = 0; i < n; ++i) out[i] = physics::intervalley_matrix_rate(E[i], cfg);
+}
+
+// KaplanFilm (the Al preset) with FilmLossModel::KaplanRateCompetition and tau_l = loss_time:
+// kaplan_pair_breaking_rate and loss_probability(E, 1.0) at n energies [eV].
+void ref_film_rate_competition(double loss_time, int n, const double* E, double* rate, double* loss) {
+    SuperconductorFilmConfig f = materials::AluminumFilm();
+    f.loss_model = FilmLossModel::KaplanRateCompetition;
+    f.phonon_loss_time = loss_time;
+    const KaplanFilm film(f);
+    for (int i = 0; i < n; ++i) {
+        rate[i] = film.kaplan_pair_breaking_rate(E[i]);
+        loss[i] = film.loss_probability(E[i], 1.0);
+    }
+}
+
+// KaplanSurrogateLUT of the Al sensor film (seed `table_seed`), then n calls of
+// absorb_phonon_fast at energies E[i] from one Rng(seed): edep, reflected count and sum;
+// the final generator state.
+void ref_surrogate(uint64_t table_seed, uint64_t seed, int n, const double* E, double* edep, int* nrefl,
+                   double* refl_sum, uint64_t* state) {
+    const KaplanSurrogateLUT lut(KaplanFilm::aluminum_sensor(), table_seed);
+    Rng r(seed);
+    KaplanBuffer refl;
+    for (int i = 0; i < n; ++i) {
+        edep[i] = lut.absorb_phonon_fast(E[i], refl, r);
+        nrefl[i] = static_cast<int>(refl.size());
+        double sm = 0.0;
+        for (double x : refl) sm += x;
+        refl_sum[i] = sm;
+    }
+    for (int k = 0; k < 4; ++k) state[k] = r.s[k];
+}
+
+// Kinematics mode of the lattices that ref_run_events builds (0 Octahedral, 1 Exact,
+// 2 Bilinear).
 static int g_kin_mode = 0;
 void ref_set_kinematics_mode(int mode) { g_kin_mode = mode; }
 

@@ -126,6 +126,47 @@ int main(int argc, char** argv) {
         check("transport(): deposited + escaped/killed = injected", S.total_deposited_eV + S.escaped_or_killed_eV, Ein, 1e-9);
         check("transport(): stack consumed", static_cast<double>(st.size()), 0.0, 0.0);
     }
+    {   // KinematicsMode::Bilinear: the (cos theta, phi) table on the host and in both forms
+        CrystalLattice lb(materials::Germanium());
+        lb.set_kinematics_mode(CrystalLattice::KinematicsMode::Bilinear);
+        DetectorModel db;
+        const SimulationEngine eb(lb, db);
+        const auto a = lb.map_k_to_vg(PhononMode::L, Vec3{0.3, -0.5, 0.8}.unit());
+        const auto b = lb.map_k_to_vg_oct(PhononMode::L, Vec3{0.3, -0.5, 0.8}.unit());
+        check("bilinear vs octahedral v_g magnitude (table errors < 1e-2)", a.vg_mag, b.vg_mag, 1e-2);
+        auto gen = [&](EventInput& in) {
+            in.phonons.push_back(eb.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::L, 7.5e-3));
+            in.injected_eV += 7.5e-3;
+        };
+        RunOptions o;
+        o.seed = 9;
+        const RunResult r = run_events(eb, 300, gen, o);
+        check("bilinear kinematics: deposited + escaped = injected",
+              r.stats.total_deposited_eV + r.stats.escaped_or_killed_eV, r.stats.total_injected_eV, 1e-9);
+        o.form = CMPMC_FORM_BATCHED;
+        const RunResult rb = run_events(eb, 300, gen, o);
+        check("bilinear kinematics: batched steps vs plain (5%)", static_cast<double>(rb.stats.total_steps),
+              static_cast<double>(r.stats.total_steps), 0.05);
+    }
+    {   // FilmLossModel::KaplanRateCompetition on both films
+        DetectorModel dr;
+        for (int i = 0; i < 2; ++i) {
+            SuperconductorFilmConfig f = materials::AluminumFilm();
+            f.loss_model = FilmLossModel::KaplanRateCompetition;
+            f.phonon_loss_time = 1.47e-9;
+            dr.surfaces[i].kaplan_sensor = KaplanFilm(f);
+        }
+        const SimulationEngine er(lattice, dr);
+        auto gen = [&](EventInput& in) {
+            in.phonons.push_back(er.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::L, 7.5e-3));
+            in.injected_eV += 7.5e-3;
+        };
+        RunOptions o;
+        o.seed = 5;
+        const RunResult r = run_events(er, 300, gen, o);
+        check("rate-competition films: deposited + escaped = injected",
+              r.stats.total_deposited_eV + r.stats.escaped_or_killed_eV, r.stats.total_injected_eV, 1e-9);
+    }
     {   // KinematicsMode::Exact: per-call Christoffel solve on the host and in the kernels
         CrystalLattice lx(materials::Germanium());
         lx.set_kinematics_mode(CrystalLattice::KinematicsMode::Exact);
@@ -290,6 +331,16 @@ int main(int argc, char** argv) {
         bool caught = false;
         try { (void)run_events(guarded, 64, g, o); } catch (const UnvalidatedModuleError&) { caught = true; }
         check("guarded model: UnvalidatedModuleError on the caller", caught ? 1 : 0, 1, 0.0);
+        // with allow_unvalidated the Matrix rate runs (engine.hpp:197-199)
+        DetectorModel da = det;
+        da.allow_unvalidated = true;
+        const SimulationEngine allowed(guarded_lattice, da);
+        const MixedGenerator ga{&allowed, &gc, cfg.valley_dirs.size()};
+        const RunResult ra = run_events(allowed, 64, ga, o);
+        check("Matrix IV with allow_unvalidated: every carrier of the 32 pairs collected",
+              static_cast<double>(ra.stats.charges_collected), 64.0, 0.0);
+        check("Matrix IV with allow_unvalidated: intervalley scatters happen",
+              ra.stats.intervalley_scatters > 0 ? 1 : 0, 1, 0.0);
         bool caught2 = false;
         auto throwing = [](EventInput& in) { if (in.event_id == 17) throw std::runtime_error("generator failure"); };
         try { (void)run_events(engine, 64, throwing, o); } catch (const std::runtime_error&) { caught2 = true; }

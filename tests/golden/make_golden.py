@@ -61,6 +61,23 @@ def main():
         R.ref_map_k_to_vg_exact(m, len(ke), O.dptr(np.ascontiguousarray(ke)), O.dptr(tmp))
         vge[m] = tmp
     out["map_exact_vg"] = vge
+    # 3c. KinematicsMode::Bilinear on the same directions (poles, axes, random)
+    vgb = np.zeros((3, len(ke), 4))
+    for m in range(3):
+        tmp = np.zeros((len(ke), 4))
+        R.ref_map_k_to_vg_mode(2, m, len(ke), O.dptr(np.ascontiguousarray(ke)), O.dptr(tmp))
+        vgb[m] = tmp
+    out["map_bil_vg"] = vgb
+    # 3d. IVRateModel::Matrix rate of Germanium's defaults, through the optical thresholds
+    eiv = np.concatenate([np.geomspace(1e-6, 0.2, 60), [10.3e-3, 27.3e-3, 10.3e-3 + 1e-9, 27.3e-3 + 1e-9]])
+    riv = np.zeros(len(eiv))
+    R.ref_iv_matrix_rate(len(eiv), O.dptr(eiv), O.dptr(riv))
+    out["iv_matrix_E"], out["iv_matrix_rate"] = eiv, riv
+    # 3e. KaplanRateCompetition: pair-breaking rate and loss probability of the Al film
+    erc = 173.715e-6 * np.concatenate([[1.0, 1.999, 2.0, 2.0001], np.linspace(2.05, 200.0, 60)])
+    rrc, lrc = np.zeros(len(erc)), np.zeros(len(erc))
+    R.ref_film_rate_competition(O.RC_LOSS_TIME, len(erc), O.dptr(erc), O.dptr(rrc), O.dptr(lrc))
+    out["rc_E"], out["rc_rate"], out["rc_loss"] = erc, rrc, lrc
     # 4. decay energy-fraction draws from Rng(99)
     lt, tt = np.zeros(1000), np.zeros(1000)
     R.ref_decay_fractions(99, 1000, O.dptr(lt), O.dptr(tt))
@@ -72,6 +89,15 @@ def main():
         R.ref_absorb_phonon(E, seed, 200, O.dptr(edep), ns.ctypes.data_as(O.C.POINTER(O.C.c_int)), O.dptr(sm), st)
         out[f"{tag}_edep"], out[f"{tag}_nrefl"], out[f"{tag}_reflsum"] = edep, ns, sm
         out[f"{tag}_state"] = np.array(list(st), dtype=np.uint64)
+    # 5b. KaplanSurrogateLUT (kaplan_film.hpp:325-383) of the Al film, table seed 1337:
+    # absorb_phonon_fast over [1.5, 45] Delta (the exact cascade outside [2, 40] Delta) from Rng(21)
+    es = 173.715e-6 * np.random.default_rng(5).uniform(1.5, 45.0, 3000)
+    edep, ns, sm = np.zeros(len(es)), np.zeros(len(es), dtype=np.int32), np.zeros(len(es))
+    st = (O.u64 * 4)()
+    R.ref_surrogate(1337, 21, len(es), O.dptr(es), O.dptr(edep), ns.ctypes.data_as(O.C.POINTER(O.C.c_int)),
+                    O.dptr(sm), st)
+    out["surr_E"], out["surr_edep"], out["surr_nrefl"], out["surr_reflsum"] = es, edep, ns, sm
+    out["surr_state"] = np.array(list(st), dtype=np.uint64)
     # 6. intersect_exit of the default cylinder
     g = np.random.default_rng(7)
     p = np.c_[g.uniform(-0.026, 0.026, (300, 2)), g.uniform(-0.0125, 0.0125, 300)]
@@ -150,6 +176,14 @@ def main():
     dm.luke_sample_fraction = 0.0
     runs["mask_pairs"] = (O.workload(O.WL_PAIR), 6, 42, dm)
     runs["c1_exact"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector())
+    # KinematicsMode::Bilinear; the KaplanRateCompetition film-loss model on both films;
+    # the guarded IVRateModel::Matrix intervalley rate for pairs at 4 V
+    runs["c1_bil"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector())
+    runs["film_rc"] = (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.rate_competition_detector())
+    dmx = O.default_detector()
+    O.set_bias(dmx, 4.0)
+    dmx.luke_sample_fraction = 0.0
+    runs["iv_matrix"] = (O.workload(O.WL_PAIR), 6, 42, dmx, O.matrix_iv_crystal())
     # energy partition (charge/energy_partition.hpp, niel_models.hpp): deposits partitioned
     # into pairs and prompt phonons, transported at 4 V with 5 % of the Luke phonons
     dp = O.default_detector()
@@ -165,7 +199,7 @@ def main():
     for tag, spec in runs.items():
         wl, n, seed, d = spec[:4]
         cr = spec[4] if len(spec) > 4 else None
-        R.ref_set_kinematics_mode(1 if tag.endswith("_exact") else 0)
+        R.ref_set_kinematics_mode(1 if tag.endswith("_exact") else (2 if tag.endswith("_bil") else 0))
         r = O.ref_run(wl, n, seed, detector=d, crystal=cr, threads=4)
         R.ref_set_kinematics_mode(0)
         out[f"run_{tag}_stats"] = np.frombuffer(r["stats_raw"], dtype=np.uint8)

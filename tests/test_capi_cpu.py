@@ -129,6 +129,26 @@ def test_host_exact_kinematics_bit_exact(lib_cmpmc):
         assert np.array_equal(np.c_[d, mag], gold["map_exact_vg"][m])
 
 
+def test_host_bilinear_kinematics_bit_exact(lib_cmpmc, oracle):
+    """cmpmc_lattice_build_bilinear + cmpmc_map_k_to_vg with KinematicsMode::Bilinear
+    (crystal_lattice.hpp:89-114, 350-368) on the host: the table equals the oracle's, and the
+    lookups equal the reference build's values (golden, bitwise), poles and axes included."""
+    import os
+    cm = lib_cmpmc
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz"))
+    lat = cm.Lattice()
+    with pytest.raises(cm.CmpmcError):        # no table yet: mode 2 is refused
+        lat.c.kinematics_mode = 2
+        lat.map_k_to_vg(0, np.array([[0.0, 0.0, 1.0]]))
+    lat.c.kinematics_mode = 0
+    lat.set_kinematics_mode(lat.KINEMATICS_BILINEAR)
+    k = gold["map_exact_k"]
+    for m in range(3):
+        d, mag = lat.map_k_to_vg(m, k)
+        assert np.array_equal(np.c_[d, mag], gold["map_bil_vg"][m])
+    assert np.array_equal(lat.bil, oracle.Engine().bil_table())
+
+
 @pytest.mark.parametrize("tag", ["amm_geal", "amm_siw"])
 def test_host_amm_equals_reference(lib_cmpmc, tag):
     """The library's AMM model (the code the plain kernel runs, compiled for the host)

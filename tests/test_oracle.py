@@ -100,6 +100,81 @@ def test_map_k_to_vg_exact_golden(oracle, gold):
         assert np.array_equal(eng.map_k_to_vg(m, gold["map_exact_k"], exact=True), gold["map_exact_vg"][m])
 
 
+def test_map_k_to_vg_bilinear_golden(oracle, gold):
+    """KinematicsMode::Bilinear (crystal_lattice.hpp:89-114, table 350-368): bit-equal to the
+    reference on poles, axes, near-degenerate and random directions."""
+    eng = oracle.Engine()
+    for m in range(3):
+        assert np.array_equal(eng.map_k_to_vg(m, gold["map_exact_k"], bilinear=True), gold["map_bil_vg"][m])
+
+
+def test_bilinear_nodes_are_christoffel_solutions(oracle):
+    """At a node of the (cos theta, phi) grid the bilinear weights select that node alone, so
+    the lookup is the normalised Jacobi solve of the node's direction: it agrees with the
+    closed-form (Cardano) solve there. A transposed or shifted table index fails this."""
+    eng = oracle.Engine()
+    g = np.random.default_rng(3)
+    i = g.integers(1, 180, 40)
+    j = g.integers(0, 360, 40)
+    ct = -1.0 + 2.0 * i / 180.0
+    phi = 2.0 * np.pi * j / 360.0
+    st = np.sqrt(1.0 - ct * ct)
+    k = np.c_[st * np.cos(phi), st * np.sin(phi), ct]
+    for m in range(3):
+        b = eng.map_k_to_vg(m, k, bilinear=True)
+        e = eng.map_k_to_vg(m, k, exact=True)
+        # L everywhere; the transverse modes away from their degeneracies (where the
+        # mode labels of two solvers may differ)
+        ok = np.ones(len(k), bool) if m == 0 else np.abs(e[:, 3] - eng.map_k_to_vg(3 - m, k, exact=True)[:, 3]) > 20.0
+        assert ok.sum() >= 20
+        assert np.allclose(b[ok, 3], e[ok, 3], rtol=1e-7, atol=0)
+        assert np.allclose(b[ok, :3], e[ok, :3], atol=1e-6)
+
+
+def test_iv_matrix_rate_golden(oracle, gold):
+    """IVRateModel::Matrix (charge_kernels.hpp:36-79): bit-equal to the reference's rate of
+    Germanium's defaults on both sides of each optical threshold; below the lowest threshold
+    only the neutral-impurity term remains, which rises as sqrt(E) at small E."""
+    O = oracle
+    c = O.germanium()
+    E = gold["iv_matrix_E"]
+    r = np.array([O.lib().orc_intervalley_matrix_rate(C.byref(c), float(e)) for e in E])
+    assert np.array_equal(r, gold["iv_matrix_rate"])
+    small = E < 1e-4
+    assert np.allclose(r[small] / np.sqrt(E[small]), r[small][0] / np.sqrt(E[small][0]), rtol=0.02)
+    # the optical terms switch on above their thresholds: the rate jumps no more than the
+    # impurity slope across them (e_func(0) = 0), and grows beyond
+    assert np.all(np.diff(r[E < 0.2][np.argsort(E[E < 0.2])]) > -1e-6 * r.max())
+
+
+def test_rate_competition_golden_and_pair_breaking_limit(oracle, gold):
+    """FilmLossModel::KaplanRateCompetition (kaplan_film.hpp:69-99): the pair-breaking rate
+    and loss probability are bit-equal to the reference's; 1/tau_pb(2 Delta) = 1/tau0
+    exactly (the integral's limit); the 48-point quadrature agrees with an adaptive
+    integration of the original singular integral; the loss probability is
+    1 / (1 + tau_l / tau_pb) and 1 below the gap edge."""
+    from scipy.integrate import quad
+    O = oracle
+    d = O.default_detector()
+    f = d.surf[0].film
+    f.loss_model, f.phonon_loss_time = 1, O.RC_LOSS_TIME
+    L = O.lib()
+    E = gold["rc_E"]
+    rate = np.array([L.orc_kaplan_pair_breaking_rate(C.byref(f), float(e)) for e in E])
+    loss = np.array([L.orc_film_loss_probability(C.byref(f), float(e), 1.0) for e in E])
+    assert np.array_equal(rate, gold["rc_rate"]) and np.array_equal(loss, gold["rc_loss"])
+    D, tau0 = f.gap_energy, f.phonon_lifetime
+    assert abs(L.orc_kaplan_pair_breaking_rate(C.byref(f), 2.0 * D) * tau0 - 1.0) < 1e-12
+    for W in (3.0, 10.0, 50.0):
+        om = W * D
+        num, _ = quad(lambda x: (x * (om - x) + D * D) / np.sqrt((x * x - D * D) * ((om - x) ** 2 - D * D)),
+                      D, om - D, limit=400, epsabs=0, epsrel=1e-11, points=[0.5 * om])
+        assert abs(L.orc_kaplan_pair_breaking_rate(C.byref(f), om) / (num / (np.pi * D * tau0)) - 1.0) < 1e-7
+    above = E >= 2.0 * D
+    assert np.allclose(loss[above], 1.0 / (1.0 + rate[above] * O.RC_LOSS_TIME), rtol=1e-15, atol=0)
+    assert np.all(loss[~above] == 1.0)
+
+
 def test_cardano_equals_jacobi(oracle):
     """tests/test_christoffel_cardano.cpp:52-53: the closed-form solve agrees with the
     Jacobi sweeps (group velocities and phase speeds) to 1e-9 relative."""
@@ -164,6 +239,71 @@ def test_kaplan_cascade_golden(oracle, gold, tag, E, seed):
     assert [r.s[i] for i in range(4)] == [int(x) for x in gold[f"{tag}_state"]]
 
 
+class _Surrogate:
+    """The oracle's KaplanSurrogateLUT (test helper)."""
+
+    def __init__(self, O, f, seed=1337):
+        self.O, self.L = O, O.lib()
+        self.buf = C.create_string_buffer(256)           # room for orc_surrogate
+        if self.L.orc_surrogate_build(self.buf, C.byref(f), seed) != 0:
+            raise MemoryError("orc_surrogate_build")
+
+    def absorb(self, E, r, refl, nr):
+        return self.L.orc_surrogate_absorb(self.buf, E, self.O.dptr(refl), C.byref(nr), C.byref(r))
+
+    def close(self):
+        self.L.orc_surrogate_free(self.buf)
+
+
+def test_kaplan_surrogate_golden_and_reentry(oracle, gold):
+    """KaplanSurrogateLUT (kaplan_film.hpp:325-383): the table built from Rng(1337) and its
+    lookup are bit-equal to the reference's (3,000 absorb_phonon_fast calls over [1.5, 45]
+    Delta from one stream: each edep, reflected count and sum, and the final state; energies
+    outside [2, 40] Delta take the exact cascade). And tests/test_kaplan.cpp:95-107's own
+    check: in a re-entry cavity (each re-emitted phonon returns with probability p_ret) the
+    surrogate's collected fraction is the full cascade's to 3 %."""
+    O = oracle
+    L = O.lib()
+    f = O.Film()
+    L.orc_film_aluminum(C.byref(f))
+    s = _Surrogate(O, f)
+    try:
+        r = O.Rng()
+        L.orc_rng_seed(C.byref(r), 21)
+        refl = np.zeros(256)
+        nr = C.c_int()
+        for i, E in enumerate(gold["surr_E"]):
+            edep = s.absorb(float(E), r, refl, nr)
+            assert edep == gold["surr_edep"][i] and nr.value == gold["surr_nrefl"][i], i
+            sm = 0.0
+            for x in refl[:nr.value]:
+                sm += x
+            assert sm == gold["surr_reflsum"][i], i
+        assert [r.s[i] for i in range(4)] == [int(x) for x in gold["surr_state"]]
+        D = f.gap_energy
+        rng = O.Rng()
+        L.orc_rng_seed(C.byref(rng), 31)
+
+        def cavity(E, p_ret, lut):
+            todo, edep = [E], 0.0
+            while todo:
+                e = todo.pop()
+                edep += s.absorb(e, rng, refl, nr) if lut else L.orc_absorb_phonon(C.byref(f), e, O.dptr(refl),
+                                                                                       C.byref(nr), C.byref(rng))
+                for x in refl[:nr.value].copy():
+                    if L.orc_uniform(C.byref(rng)) < p_ret:
+                        todo.append(x)
+            return edep / E
+
+        for p_ret in (0.0, 0.9):
+            for W in (3.0, 15.0, 35.0):
+                a = np.mean([cavity(W * D, p_ret, False) for _ in range(4000)])
+                b = np.mean([cavity(W * D, p_ret, True) for _ in range(4000)])
+                assert b == pytest.approx(a, rel=0.03), (p_ret, W, a, b)
+    finally:
+        s.close()
+
+
 def test_intersect_golden(oracle, gold):
     O = oracle
     L = O.lib()
@@ -190,6 +330,9 @@ def _gold_runs(O):
     dr.russian_roulette_bounces, dr.russian_roulette_survival = 32, 0.5
     ds = O.default_detector()
     O.constant_split_surfaces(ds, 0.5)
+    dmx = O.default_detector()
+    O.set_bias(dmx, 4.0)
+    dmx.luke_sample_fraction = 0.0
     dt = O.default_detector()
     O.set_bias(dt, 4.0)
     dt.luke_sample_fraction = 0.0
@@ -221,15 +364,21 @@ def _gold_runs(O):
         "leman": (O.workload(O.WL_PAIR), 6, 42, dl),
         "leman_ion": (O.workload(O.WL_PAIR), 6, 43, dl2, O.trap_ion_crystal()),
         "c1_exact": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector()),
+        "c1_bil": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector()),
+        "film_rc": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.rate_competition_detector()),
+        "iv_matrix": (O.workload(O.WL_PAIR), 6, 42, dmx, O.matrix_iv_crystal()),
     }
 
 
 @pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo", "bench_dos", "roulette",
                                  "specular", "trap_ion", "leman", "leman_ion", "c1_exact", "deposit_nr",
-                                 "deposit_er", "mesh3_pairs", "mesh2_pairs", "mesh3_leman_ion"])
+                                 "deposit_er", "mesh3_pairs", "mesh2_pairs", "mesh3_leman_ion", "c1_bil",
+                                 "film_rc", "iv_matrix"])
 def test_run_events_golden(oracle, gold, tag):
     """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's
-    (c1_exact: with KinematicsMode::Exact; bench_dos: the reference benchmark's own phonon
+    (c1_exact: with KinematicsMode::Exact; c1_bil: KinematicsMode::Bilinear; film_rc: both
+    films on FilmLossModel::KaplanRateCompetition; iv_matrix: the IVRateModel::Matrix
+    intervalley rate for pairs at 4 V; bench_dos: the reference benchmark's own phonon
     workload; roulette, specular, trap_ion: Russian roulette, half-specular surfaces, trap
     impact ionisation; leman, leman_ion: Leman second-order charge stepping; deposit_nr,
     deposit_er: generators that partition a nuclear / electron recoil with
@@ -242,6 +391,8 @@ def test_run_events_golden(oracle, gold, tag):
     eng = O.Engine(crystal=spec[4] if len(spec) > 4 else None, detector=det)
     if tag.endswith("_exact"):
         eng.set_kinematics(1)
+    elif tag.endswith("_bil"):
+        eng.set_kinematics(2)
     r = eng.run(wl, n, seed, threads=4)
     assert r["stats_raw"] == gold[f"run_{tag}_stats"].tobytes()
     assert np.array_equal(r["hit_offsets"], gold[f"run_{tag}_offsets"])
