@@ -107,6 +107,71 @@ def default_detector() -> capi.Detector:
     return d
 
 
+def aluminum_film() -> capi.Film:
+    """SuperconductorFilmConfig of the Al sensor film (crystal_config.hpp:240-257)."""
+    f = capi.Film()
+    check(capi.load().cmpmc_film_aluminum(C.byref(f)))
+    return f
+
+
+def _film_call(fn, energy: float, rng: "Rng", *args) -> tuple[float, np.ndarray]:
+    st = (capi.u64 * 4)(*rng.s)
+    refl = np.zeros(256)
+    n = capi.i32()
+    edep = C.c_double()
+    check(fn(*args[:1], energy, *args[1:], st, refl.ctypes.data, C.byref(n), C.byref(edep)))
+    rng.s = [int(x) for x in st]
+    return edep.value, refl[:n.value].copy()
+
+
+def film_absorb(film: capi.Film, energy: float, rng: "Rng", photon: bool = False) -> tuple[float, np.ndarray]:
+    """KaplanFilm::absorb_phonon (photon=False) / absorb_quantum with a photon seed on the
+    host (kaplan_film.hpp:106-206): (absorbed energy, re-emitted energies); advances rng."""
+    return _film_call(capi.load().cmpmc_film_absorb_quantum, energy, rng, C.byref(film), int(photon))
+
+
+def film_loss_probability(film: capi.Film, energy: float, thickness_frac: float = 1.0) -> tuple[float, float]:
+    """KaplanFilm::loss_probability and kaplan_pair_breaking_rate (kaplan_film.hpp:69-99)."""
+    p, g = C.c_double(), C.c_double()
+    check(capi.load().cmpmc_film_loss_probability(C.byref(film), energy, thickness_frac, C.byref(p), C.byref(g)))
+    return p.value, g.value
+
+
+class KaplanSurrogate:
+    """KaplanSurrogateLUT(film, seed) (kaplan_film.hpp:325-383): pre-sampled cascade
+    outcomes on [2 Delta, 40 Delta]; absorb() is absorb_phonon_fast."""
+
+    def __init__(self, film: capi.Film | None = None, seed: int = 1337):
+        self.film = film if film is not None else aluminum_film()
+        self.ptr = C.c_void_p()
+        check(capi.load().cmpmc_surrogate_create(C.byref(self.film), seed, C.byref(self.ptr)))
+
+    def absorb(self, energy: float, rng: "Rng") -> tuple[float, np.ndarray]:
+        return _film_call(capi.load().cmpmc_surrogate_absorb, energy, rng, self.ptr)
+
+    def tables(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        a, o, r = C.POINTER(C.c_float)(), C.POINTER(capi.u32)(), C.POINTER(C.c_float)()
+        n = capi.u64()
+        check(capi.load().cmpmc_surrogate_tables(self.ptr, C.byref(a), C.byref(o), C.byref(r), C.byref(n)))
+        k = CMPMC_SURR_NE * CMPMC_SURR_NS
+        return (np.ctypeslib.as_array(a, shape=(k,)).copy(), np.ctypeslib.as_array(o, shape=(k + 1,)).copy(),
+                np.ctypeslib.as_array(r, shape=(max(n.value, 1),))[:n.value].copy())
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None):
+            capi.load().cmpmc_surrogate_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+CMPMC_SURR_NE, CMPMC_SURR_NS = 128, 512
+
+
 def apply_diff_params(crystal: capi.Crystal, det: capi.Detector, params) -> None:
     """Writes (B_iso, A_anh, f_diffuse, p_film) into a crystal and detector the way the
     differentiable form applies them: every surface diffuse with probability f and

@@ -207,6 +207,14 @@ def load():
     L.cmpmc_lattice_build.argtypes = [C.POINTER(Crystal), C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Lattice)]
     L.cmpmc_map_k_to_vg.argtypes = [C.POINTER(Lattice), C.c_int, u64, C.c_void_p, C.c_void_p, C.c_void_p]
     L.cmpmc_lattice_build_bilinear.argtypes = [C.POINTER(Lattice), C.c_void_p]
+    L.cmpmc_film_absorb_quantum.argtypes = [C.POINTER(Film), dbl, C.c_int, C.POINTER(u64), C.c_void_p,
+                                            C.POINTER(i32), C.POINTER(dbl)]
+    L.cmpmc_film_loss_probability.argtypes = [C.POINTER(Film), dbl, dbl, C.POINTER(dbl), C.POINTER(dbl)]
+    L.cmpmc_surrogate_create.argtypes = [C.POINTER(Film), u64, C.POINTER(C.c_void_p)]
+    L.cmpmc_surrogate_absorb.argtypes = [C.c_void_p, dbl, C.POINTER(u64), C.c_void_p, C.POINTER(i32), C.POINTER(dbl)]
+    L.cmpmc_surrogate_tables.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(u32)),
+                                         C.POINTER(C.POINTER(C.c_float)), C.POINTER(u64)]
+    L.cmpmc_surrogate_destroy.argtypes = [C.c_void_p]
     L.cmpmc_model_create.argtypes = [C.POINTER(Lattice), C.POINTER(Detector), C.POINTER(C.c_void_p)]
     L.cmpmc_model_destroy.argtypes = [C.c_void_p]
     L.cmpmc_model_cancel.argtypes = [C.c_void_p, C.c_int]
@@ -311,5 +319,6 @@ EXPORTED_SYMBOLS = [
     "cmpmc_sample_fano_binomial", "cmpmc_partition_deposit", "cmpmc_mesh3d_create", "cmpmc_mesh2d_create",
     "cmpmc_mesh3d_grid_vertices", "cmpmc_mesh3d_grid", "cmpmc_mesh2d_grid_vertices", "cmpmc_mesh2d_grid",
     "cmpmc_mesh3d_load", "cmpmc_mesh_destroy", "cmpmc_mesh_info", "cmpmc_mesh_locate", "cmpmc_mesh_field",
-    "cmpmc_mesh_potential", "cmpmc_model_field_at",
+    "cmpmc_mesh_potential", "cmpmc_model_field_at", "cmpmc_film_absorb_quantum", "cmpmc_film_loss_probability",
+    "cmpmc_surrogate_create", "cmpmc_surrogate_absorb", "cmpmc_surrogate_tables", "cmpmc_surrogate_destroy",
 ]

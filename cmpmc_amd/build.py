@@ -38,7 +38,8 @@ UNITS = [
     ("capi_plan.hip", []),
 ]
 HOST_UNITS = [("lattice_build.cpp", ["-ffp-contract=off"]), ("surfaces.cpp", ["-ffp-contract=off"]),
-              ("partition.cpp", ["-ffp-contract=off"]), ("mesh_field.cpp", ["-ffp-contract=off"])]
+              ("partition.cpp", ["-ffp-contract=off"]), ("mesh_field.cpp", ["-ffp-contract=off"]),
+              ("kaplan_host.cpp", ["-ffp-contract=off"])]
 
 
 def _run(cmd):

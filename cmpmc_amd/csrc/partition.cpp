@@ -9,44 +9,15 @@
 #include "capi_internal.h"
 #include "cmpmc_amd.h"
 #include "energy_partition.hpp"
+#include "host_rng.hpp"
 
 namespace {
 
+using cmpmc_internal::HostRng;
+
 using cmpmc_internal::fail;
 
-constexpr double kTwoPi = 6.283185307179586;
 constexpr double kHPlanck = 4.135667696e-15;     // eV s
-
-// xoshiro256** with the reference's uniform / normal / isotropic_dir (rng.hpp:49-83).
-struct HostRng {
-    uint64_t s[4];
-    static uint64_t rotl(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
-    uint64_t next_u64() {
-        const uint64_t r = rotl(s[1] * 5, 7) * 9;
-        const uint64_t t = s[1] << 17;
-        s[2] ^= s[0];
-        s[3] ^= s[1];
-        s[1] ^= s[2];
-        s[0] ^= s[3];
-        s[2] ^= t;
-        s[3] = rotl(s[3], 45);
-        return r;
-    }
-    double uniform() { return static_cast<double>((next_u64() >> 11) | 1ull) * 0x1.0p-53; }
-    double normal() {
-        const double u1 = uniform();
-        const double u2 = uniform();
-        return std::sqrt(-2.0 * std::log(u1)) * std::cos(kTwoPi * u2);
-    }
-    void isotropic_dir(double d[3]) {
-        const double ct = 2.0 * uniform() - 1.0;
-        const double st = std::sqrt(std::max(0.0, 1.0 - ct * ct));
-        const double ph = kTwoPi * uniform();
-        d[0] = st * std::cos(ph);
-        d[1] = st * std::sin(ph);
-        d[2] = ct;
-    }
-};
 
 cmpmc_part::Material material(const cmpmc_crystal& c) {
     return cmpmc_part::Material{c.atomic_Z,  c.atomic_A,   c.density, c.bandgap, c.pair_energy, c.fano_factor,

@@ -9,6 +9,7 @@
 #include "cmpmc/geometry/mesh_field.hpp"
 #include "cmpmc/phonon/interface_transmission.hpp"
 #include "cmpmc/geometry/surface_property.hpp"
+#include "cmpmc/superconductor/kaplan_film.hpp"
 #include "cmpmc/transport/engine.hpp"
 #include "cmpmc/transport/parallel.hpp"
 #include "cmpmc/charge/niel_models.hpp"

@@ -1,8 +1,7 @@
 // Drop-in geometry and sensor-film API (geometry/solids.hpp, geometry/surface_property.hpp,
 // superconductor/kaplan_film.hpp of the reference): the cylinder solid, per-surface
 // boundary models (with the optional electrode mask and AMM film entry) and the Kaplan
-// film configuration. The film cascade itself runs on the GPU inside the transport
-// kernels.
+// film (configuration and host cascade; the transport kernels run the cascade on the GPU).
 #pragma once
 
 #include <limits>
@@ -57,12 +56,58 @@ struct CylinderSolid {
     }
 };
 
-// kaplan_film.hpp:29-37
+// kaplan_film.hpp:29-323: the film configuration and, on the host, its cascade (the
+// library's cmpmc_film_* entry points, bit-exact to the reference for a given stream); the
+// transport kernels run the same cascade on the GPU.
 struct KaplanFilm : public SuperconductorFilmConfig {
+    enum class Seed : uint8_t { Phonon, Photon };
+
     constexpr KaplanFilm() noexcept = default;
     constexpr explicit KaplanFilm(const SuperconductorFilmConfig& c) noexcept : SuperconductorFilmConfig(c) {}
     static constexpr KaplanFilm aluminum_sensor() noexcept { return KaplanFilm(materials::AluminumFilm()); }
     constexpr bool is_subgap(double e) const noexcept { return e < 2.0 * gap_energy; }
+
+    // kaplan_film.hpp:50-57
+    double calc_escape_probability(double energy, double thickness_frac) const noexcept {
+        cmpmc_film f = to_pod();
+        f.loss_model = 0;
+        double p = 1.0;
+        (void)cmpmc_film_loss_probability(&f, energy, thickness_frac, &p, nullptr);
+        return p;
+    }
+    // kaplan_film.hpp:69-84
+    double kaplan_pair_breaking_rate(double omega) const noexcept {
+        const cmpmc_film f = to_pod();
+        double p = 0.0, g = 0.0;
+        (void)cmpmc_film_loss_probability(&f, omega, 1.0, &p, &g);
+        return g;
+    }
+    // kaplan_film.hpp:85-90
+    static double loss_time_from_geometry(double d_film, double v_s, double eta_t) noexcept {
+        return (eta_t > 0.0 && v_s > 0.0) ? 4.0 * d_film / (eta_t * v_s) : 0.0;
+    }
+    // kaplan_film.hpp:91-99
+    double loss_probability(double energy, double thickness_frac) const noexcept {
+        const cmpmc_film f = to_pod();
+        double p = 0.0;
+        (void)cmpmc_film_loss_probability(&f, energy, thickness_frac, &p, nullptr);
+        return p;
+    }
+    // kaplan_film.hpp:106-206: absorbed energy; `reflected_energies` receives the phonons
+    // that leave the film; `rng` advances as the reference's does
+    double absorb_quantum(double energy, KaplanBuffer& reflected_energies, Rng& rng, Seed seed) const noexcept {
+        const cmpmc_film f = to_pod();
+        double refl[256], edep = 0.0;
+        int32_t n = 0;
+        reflected_energies.clear();
+        if (cmpmc_film_absorb_quantum(&f, energy, seed == Seed::Photon ? 1 : 0, rng.s, refl, &n, &edep) != CMPMC_OK)
+            return 0.0;
+        for (int32_t i = 0; i < n; ++i) reflected_energies.push_back(refl[i]);
+        return edep;
+    }
+    double absorb_phonon(double energy, KaplanBuffer& reflected_energies, Rng& rng) const noexcept {
+        return absorb_quantum(energy, reflected_energies, rng, Seed::Phonon);
+    }
 };
 
 // surface_property.hpp:23-273

@@ -435,6 +435,35 @@ int cmpmc_mesh_field(const cmpmc_mesh* mesh, const double p[3], int32_t* hint, d
 /* potential_at (mesh_field.hpp:96-104, 389-396): V at a 3D position, 0 outside */
 double cmpmc_mesh_potential(const cmpmc_mesh* mesh, const double p[3], int32_t* hint);
 
+/* KaplanFilm on the host (superconductor/kaplan_film.hpp:106-323): absorb_quantum of one
+ * phonon (photon = 0, absorb_phonon) or photon (photon = 1) of `energy` [eV] with the
+ * stream rng_state (advanced in place, the reference's draw order); `reflected` (room for
+ * 256) receives the energies that leave the film, *edep the absorbed energy. */
+int cmpmc_film_absorb_quantum(const cmpmc_film* film, double energy, int photon, uint64_t rng_state[4],
+                              double* reflected, int32_t* n_reflected, double* edep);
+/* KaplanFilm::loss_probability(energy, thickness_frac) and kaplan_pair_breaking_rate(energy)
+ * (kaplan_film.hpp:69-99); pair_breaking_rate may be NULL. */
+int cmpmc_film_loss_probability(const cmpmc_film* film, double energy, double thickness_frac, double* p,
+                                double* pair_breaking_rate);
+
+/* KaplanSurrogateLUT (kaplan_film.hpp:325-383): CMPMC_SURR_NE energy nodes on
+ * [2 Delta, 2 Delta CMPMC_SURR_EMAX_2GAP], CMPMC_SURR_NS exact-cascade outcomes per node
+ * drawn from Rng(seed) (the reference's default seed is 1337). cmpmc_surrogate_absorb is
+ * absorb_phonon_fast: a stored outcome of a bracketing node (linear-interpolation
+ * probability) rescaled to `energy`, the exact cascade outside the table.
+ * cmpmc_surrogate_tables exposes the stored outcomes (absorbed fraction per outcome,
+ * offsets [NE * NS + 1] into the re-emitted fractions). */
+#define CMPMC_SURR_NE 128
+#define CMPMC_SURR_NS 512
+#define CMPMC_SURR_EMAX_2GAP 20.0
+typedef struct cmpmc_surrogate cmpmc_surrogate;
+int cmpmc_surrogate_create(const cmpmc_film* film, uint64_t seed, cmpmc_surrogate** out);
+int cmpmc_surrogate_absorb(const cmpmc_surrogate* s, double energy, uint64_t rng_state[4], double* reflected,
+                           int32_t* n_reflected, double* edep);
+int cmpmc_surrogate_tables(const cmpmc_surrogate* s, const float** abs_frac, const uint32_t** offsets,
+                           const float** reflected_frac, uint64_t* n_reflected);
+int cmpmc_surrogate_destroy(cmpmc_surrogate* s);
+
 /* CrystalLattice(const CrystalConfig&) table build: crystal_lattice.hpp:45-55, 340-477.
  * oct_lut: 3*256*256*4 floats; cdf_lt, cdf_tt: 1024 doubles each (caller-owned);
  * `out` receives cfg, the derived charge tensors and the three table pointers. */

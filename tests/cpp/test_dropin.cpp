@@ -148,6 +148,43 @@ int main(int argc, char** argv) {
         check("bilinear kinematics: batched steps vs plain (5%)", static_cast<double>(rb.stats.total_steps),
               static_cast<double>(r.stats.total_steps), 0.05);
     }
+    {   // KaplanFilm on the host and KaplanSurrogateLUT (tests/test_kaplan.cpp:95-107: the
+        // surrogate's collected fraction in a re-entry cavity within 3 % of the full cascade)
+        const KaplanFilm film = KaplanFilm::aluminum_sensor();
+        const KaplanSurrogateLUT lut(film);
+        const double D = film.gap_energy;
+        Rng rr(31);
+        auto cavity = [&](const KaplanSurrogateLUT* l, double E, double p_ret) {
+            std::vector<double> todo{E};
+            double edep = 0.0;
+            KaplanBuffer refl;
+            while (!todo.empty()) {
+                const double e = todo.back();
+                todo.pop_back();
+                edep += l ? l->absorb_phonon_fast(e, refl, rr) : film.absorb_phonon(e, refl, rr);
+                for (double x : refl)
+                    if (rr.uniform() < p_ret) todo.push_back(x);
+            }
+            return edep / E;
+        };
+        for (double p_ret : {0.0, 0.9})
+            for (double W : {3.0, 15.0, 35.0}) {
+                double a = 0.0, b = 0.0;
+                for (int i = 0; i < 5000; ++i) {
+                    a += cavity(nullptr, W * D, p_ret);
+                    b += cavity(&lut, W * D, p_ret);
+                }
+                check("surrogate / full collected fraction, p_ret " + std::to_string(p_ret).substr(0, 3) + ", " +
+                          std::to_string(static_cast<int>(W)) + " Delta", b, a, 0.03);
+            }
+        check("kaplan_pair_breaking_rate(2 Delta) tau0 = 1", film.kaplan_pair_breaking_rate(2.0 * D) * film.phonon_lifetime,
+              1.0, 1e-12);
+        KaplanBuffer refl;
+        const double e = film.absorb_phonon(7.5e-3, refl, rr);
+        double out = 0.0;
+        for (double x : refl) out += x;
+        check("absorb_phonon: absorbed + re-emitted <= incident", e + out <= 7.5e-3 * (1.0 + 1e-12) ? 1 : 0, 1, 0.0);
+    }
     {   // FilmLossModel::KaplanRateCompetition on both films
         DetectorModel dr;
         for (int i = 0; i < 2; ++i) {

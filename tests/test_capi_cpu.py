@@ -149,6 +149,68 @@ def test_host_bilinear_kinematics_bit_exact(lib_cmpmc, oracle):
     assert np.array_equal(lat.bil, oracle.Engine().bil_table())
 
 
+def _gold():
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz"))
+
+
+def test_host_film_cascade_equals_reference(lib_cmpmc):
+    """KaplanFilm::absorb_phonon on the host (cmpmc_film_absorb_quantum) equals the reference
+    build's cascades bit for bit: 200 cascades from Rng(seed) at 7.5, 15 and 31 meV (each
+    edep, reflected count and sum, and the final stream state); the rate-competition loss
+    probability and pair-breaking rate equal the reference's on the golden energies; a
+    photon seed conserves energy exactly (absorbed + re-emitted = quantum, no thermal loss
+    at T = 0)."""
+    cm = lib_cmpmc
+    gold = _gold()
+    f = cm.aluminum_film()
+    for tag, E, seed in (("k75", 7.5e-3, 11), ("k15", 15e-3, 12), ("k31", 31e-3, 13)):
+        r = cm.Rng(seed)
+        for i in range(200):
+            edep, refl = cm.film_absorb(f, E, r)
+            sm = 0.0
+            for x in refl:
+                sm += x
+            assert edep == gold[f"{tag}_edep"][i] and len(refl) == gold[f"{tag}_nrefl"][i]
+            assert sm == gold[f"{tag}_reflsum"][i]
+        assert r.s == [int(x) for x in gold[f"{tag}_state"]]
+    frc = cm.aluminum_film()
+    frc.loss_model, frc.phonon_loss_time = 1, 4.0 * 600e-9 / (0.5 * 3260.0)
+    for E, g_ref, p_ref in zip(gold["rc_E"], gold["rc_rate"], gold["rc_loss"]):
+        p, g = cm.film_loss_probability(frc, float(E), 1.0)
+        assert p == p_ref and g == g_ref
+    f0 = cm.aluminum_film()
+    f0.temperature = 0.0
+    r = cm.Rng(3)
+    for E in (0.5e-3, 2.0e-3, 13.9e-3):
+        for _ in range(50):
+            edep, refl = cm.film_absorb(f0, E, r, photon=True)
+            assert abs(edep + refl.sum() - E) <= 1e-12 * E
+
+
+def test_host_surrogate_equals_reference(lib_cmpmc):
+    """KaplanSurrogateLUT (cmpmc_surrogate_*): the table built from Rng(1337) and
+    absorb_phonon_fast equal the reference's on 3,000 calls from one stream (energies over
+    [1.5, 45] Delta, so the exact cascade outside the table as well); each stored outcome
+    conserves energy to FP32 rounding."""
+    cm = lib_cmpmc
+    gold = _gold()
+    s = cm.KaplanSurrogate()
+    r = cm.Rng(21)
+    for i, E in enumerate(gold["surr_E"]):
+        edep, refl = s.absorb(float(E), r)
+        sm = 0.0
+        for x in refl:
+            sm += x
+        assert edep == gold["surr_edep"][i] and len(refl) == gold["surr_nrefl"][i] and sm == gold["surr_reflsum"][i], i
+    assert r.s == [int(x) for x in gold["surr_state"]]
+    a, o, fr = s.tables()
+    assert len(a) == 128 * 512 and o[0] == 0 and o[-1] == len(fr)
+    tot = a.astype(np.float64) + np.add.reduceat(np.r_[fr, 0.0].astype(np.float64), o[:-1]) * (np.diff(o) > 0)
+    # fractions of the incident energy: absorbed + re-emitted <= 1 (thermalised energy leaves)
+    assert np.all(tot <= 1.0 + 1e-6) and np.mean(tot) > 0.5
+    s.close()
+
+
 @pytest.mark.parametrize("tag", ["amm_geal", "amm_siw"])
 def test_host_amm_equals_reference(lib_cmpmc, tag):
     """The library's AMM model (the code the plain kernel runs, compiled for the host)
