@@ -77,7 +77,7 @@ class Detector(C.Structure):
                     "min_phonon_eV", "luke_sample_fraction", "charge_max_dk_frac", "charge_dt_cap_s",
                     "fermi_level_above_Ev_eV", "russian_roulette_survival")] + [
                 ("charge_stepping", i32), ("allow_unvalidated", i32), ("charge_sampling_dk_frac", dbl),
-                ("mesh", C.c_void_p)]
+                ("mesh", C.c_void_p), ("charge_release", i32), ("_pad_cr", i32), ("electrode_seed_max_gap", dbl)]
 
 
 class Lattice(C.Structure):

@@ -148,6 +148,7 @@ extern "C" int cmpmc_detector_default(cmpmc_detector* d) {
     d->charge_max_dk_frac = 0.05; d->charge_dt_cap_s = 100.0 * 1e-12;
     d->fermi_level_above_Ev_eV = -1.0; d->russian_roulette_survival = 0.5;
     d->charge_stepping = 0; d->charge_sampling_dk_frac = 0.25;
+    d->charge_release = 0; d->electrode_seed_max_gap = 80.0;   // RecordOnly (engine.hpp:138-140)
     return CMPMC_OK;
 }
 
@@ -307,6 +308,9 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     m.max_bounces = det->max_bounces; m.rr_bounces = det->russian_roulette_bounces;
     m.min_phonon_eV = det->min_phonon_eV; m.luke_fraction = det->luke_sample_fraction;
     m.charge_stepping = det->charge_stepping;
+    m.charge_release = det->charge_release;
+    m._pad_cr = 0;
+    m.electrode_seed_max_gap = det->electrode_seed_max_gap;
     m.dk_frac = det->charge_stepping == 1 ? det->charge_sampling_dk_frac : det->charge_max_dk_frac;
     m.dt_cap = det->charge_dt_cap_s;
     m.fermi = det->fermi_level_above_Ev_eV; m.rr_survival = det->russian_roulette_survival;
@@ -360,6 +364,10 @@ static int check_detector(const cmpmc_detector* d) {
                     d->charge_stepping);
     if (d->charge_stepping == 1 && !(d->charge_sampling_dk_frac > 0.0))
         return fail(CMPMC_ERR_INVALID, "detector: Leman stepping needs charge_sampling_dk_frac > 0");
+    if (d->charge_release < 0 || d->charge_release > 2 ||
+        (d->charge_release == 2 && !(d->electrode_seed_max_gap > 0.0 && std::isfinite(d->electrode_seed_max_gap))))
+        return fail(CMPMC_ERR_INVALID, "detector: charge_release %d is not 0 (RecordOnly), 1 (BulkDebye) or 2 "
+                    "(ElectrodeFilm with a finite electrode_seed_max_gap > 0)", d->charge_release);
     for (int s = 0; s < 3; ++s) {
         const cmpmc_surface& a = d->surf[s];
         if (a.n_spec < 0 || a.n_spec > CMPMC_MAX_POLY || a.n_diff < 0 || a.n_diff > CMPMC_MAX_POLY || a.n_anh < 0 ||
@@ -810,7 +818,9 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
     // deposits are generated by their own pass; the other workloads on the event's lane
     const bool dep = J.wl && J.wl->kind == CMPMC_WL_DEPOSIT;
     const bool lane_gen = J.wl && !dep;
-    uint64_t headroom = C.headroom ? C.headroom : (any_charges && M->dm.luke_fraction > 0.0 ? 8192 : 128);
+    uint64_t headroom = C.headroom ? C.headroom
+                                   : (any_charges && M->dm.luke_fraction > 0.0 ? 8192
+                                                                               : (any_charges && M->dm.charge_release ? 512 : 128));
     uint64_t c_headroom = 16;   // knock-on carriers per event beyond the initial ones
     const uint64_t max_lanes = 262144;
     const uint64_t region_budget = 24ull << 30;
@@ -1089,6 +1099,9 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
     const double lf = std::min(std::max(M->dm.luke_fraction, 0.0), 1.0);
     uint64_t luke_cap =
         (n_charges && lf > 0.0) ? n_charges * static_cast<uint64_t>(std::ceil(lf * kLukePerCarrier)) + 1024 : 0;
+    // phonons released at collection share the Luke queue (about E_rel / h nu_D of them per
+    // carrier with BulkDebye; the escaping cascade phonons with ElectrodeFilm)
+    if (n_charges && M->dm.charge_release != 0) luke_cap += n_charges * (M->dm.charge_release == 1 ? 16 : 64) + 1024;
     // knock-on carriers per lane of the charge kernel: the options' stack_headroom, or the
     // default; grown on a rerun when a lane's stack overflowed
     uint32_t cdepth = C.headroom ? static_cast<uint32_t>(std::min<uint64_t>(C.headroom, 1u << 16)) : kChargeDepth;
@@ -1174,6 +1187,10 @@ int run_batched(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O,
             ca.ledger = d_acc->ledger;
             ca.status = d_acc->status;
             ca.cancel = run_cancel(M, C);
+            if (M->dm.charge_release == 2) {      // the film cascades of ElectrodeFilm release
+                CHK(ensure(M->ws, static_cast<size_t>(5) * CMPMC_KCAP * std::max(lanes, clanes) * sizeof(double)));
+                ca.kaplan_ws = as<double>(M->ws);
+            }
             HIPCHK(cmpmc_launch_charges(&M->dm, &ca, block, cgrid, waves, s));
             HIPCHK(hipMemcpyAsync(&n_luke, &d_acc->luke_count, 8, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));

@@ -404,6 +404,54 @@ __device__ inline typename P::R leman_event_time(const DevModel& m, const CTrack
     }
 }
 
+// DetectorModel::charge_release at collection (engine.hpp:299-312, 888-927): the phonons a
+// collected carrier releases at its contact join the phonon work (ctx.push_luke) from 1e-11 m
+// inside the face, each with a DOS mode and an inward direction. BulkDebye: max(1,
+// round(E_rel / h nu_D)) phonons of equal energy, nothing recorded in the hit. ElectrodeFilm
+// (a face with a Kaplan film): max(1, ceil(E_rel / q_max)) photon-seeded cascades of equal
+// quanta; the film keeps what it absorbs (the hit), the phonons that escape it are released,
+// and the rest (thermalised) is booked as escaped. Returns the energy the hit records. Out of
+// line: the default kernels keep their registers.
+template <class P, class G, class Ctx>
+__device__ __noinline__ double release_at_contact(const DevModel& m, CTrack<P>& trk, double e_rel, V3<typename P::R> n_in,
+                                                  int sid, G& r, Ctx& ctx) {
+    using R = typename P::R;
+    const V3<R> p0 = trk.pos + n_in * R(1e-11);
+    auto emit = [&](double E) {
+        const int md = sample_dos_mode<P>(m, r);
+        const V3<R> kd = sample_inward_wavevector<P>(m, md, n_in, r);
+        PTrack<P> ph;
+        make_track<P>(m, ph, p0, kd, md, E, trk.time, trk.weight);
+        ctx.push_luke(ph, trk);
+    };
+    if (m.charge_release == 1) {
+        const int n = static_cast<int>(tmax(1.0, round(e_rel / m.debye_eV)));
+        for (int i = 0; i < n; ++i) emit(e_rel / n);
+        return 0.0;
+    }
+    const DevSurface& s = m.surf[sid];
+    if (m.charge_release != 2 || !s.has_film) return e_rel;
+    const double qmax = tmax(4.0, m.electrode_seed_max_gap) * s.film.gap;
+    const int n = static_cast<int>(tmax(1.0, ceil(e_rel / qmax)));
+    const double q = e_rel / n;
+    double* wb = ctx.kaplan_base();
+    const uint32_t ws = ctx.kaplan_stride();
+    KList L[5];
+    for (int i = 0; i < 5; ++i) L[i] = KList{wb + static_cast<size_t>(i) * CMPMC_KCAP * ws, ws, 0, CMPMC_KCAP};
+    KList& out = L[4];
+    double absorbed = 0.0, emitted = 0.0;
+    for (int i = 0; i < n; ++i) {
+        absorbed += film_absorb_phonon<P>(s.film, q, out, L, r, true);
+        for (int k = 0; k < out.n; ++k) {
+            const double E = out.get(k);
+            emit(E);
+            emitted += E;
+        }
+    }
+    ctx.escaped(tmax(0.0, e_rel - absorbed - emitted) * trk.weight);
+    return absorbed;
+}
+
 // One pass of the charge step loop. Returns false when the carrier ends (collected,
 // trapped) or exceeds the 20,000,000-step cap (the caller counts steps). The stepping is
 // the detector's: first order with the wavevector-change cap (the default), or Leman's
@@ -485,7 +533,8 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             const double muF = (m.fermi >= 0.0) ? m.fermi : 0.5 * m.bandgap;
             const double e_rel = static_cast<double>(trk.ekin) + ((trk.type == 1) ? (m.bandgap - muF) : muF);
             ctx.ccount(CC_COLLECTED);
-            const double recomb = e_rel * trk.weight;
+            const double hit_eV = (m.charge_release != 0) ? release_at_contact<P>(m, trk, e_rel, n_in, sid, r, ctx) : e_rel;
+            const double recomb = hit_eV * trk.weight;
             ctx.recomb(recomb);
             ctx.deposited(recomb);
             ctx.charge_hit(trk, recomb, e_rel, sid, ch);

@@ -146,6 +146,10 @@ struct DevModel {
     // second order (dk_frac = charge_sampling_dk_frac)
     int32_t charge_stepping;
     int32_t surf_x;             // some surface has an electrode mask or AMM film entry
+    // DetectorModel::ChargeRelease at collection: 0 RecordOnly, 1 BulkDebye, 2 ElectrodeFilm
+    // (quanta of at most electrode_seed_max_gap gaps)
+    int32_t charge_release, _pad_cr;
+    double electrode_seed_max_gap;
     double min_phonon_eV, luke_fraction, dk_frac, dt_cap, fermi, rr_survival;
     // energy partition of deposit workloads: target Z, A, mean pair energy, Fano factor,
     // primary (Debye) phonon energy
@@ -1151,6 +1155,21 @@ template <class P, class G> __device__ inline double film_sample_qp(const DevFil
     }
 }
 
+// Case-II (photon) pair split (kaplan_film.hpp:229-244): weight [x(E-x) - D^2] / sqrt((x+D)(E-x+D))
+// after the same sin^2 substitution, in FP64 in both forms (a release quantum splits once).
+template <class P, class G> __device__ inline double film_sample_qp_photon(const DevFilm& f, double E, G& r) {
+    const double d = f.gap;
+    const double span = E - 2.0 * d;
+    if (span <= 1e-16) return 0.5 * E;
+    auto w = [E, d](double x) { return tmax(0.0, x * (E - x) - d * d) / sqrt((x + d) * (E - x + d)); };
+    const double w_max = tmax(w(d), w(0.5 * E)) * 1.0001;
+    for (uint32_t trial = 1;; ++trial) {
+        const double s = sin(kHalfPi * static_cast<double>(P::uniform(r)));
+        const double x = d + span * (s * s);
+        if (static_cast<double>(P::uniform(r)) * w_max <= w(x) || trial >= kMaxTrials) return x;
+    }
+}
+
 template <class P, class G> __device__ inline double film_sample_phonon(const DevFilm& f, double E, G& r) {   // :269-289
     const double d = f.gap;
     const double span = E - d;
@@ -1192,16 +1211,17 @@ template <class P, class G> __device__ inline double film_sample_phonon(const De
     }
 }
 
-// KaplanFilm::absorb_phonon (Seed::Phonon; MeanFreePath or KaplanRateCompetition loss):
-// returns edep, fills `out`.
+// KaplanFilm::absorb_quantum (Seed::Phonon, or Seed::Photon with `photon`; MeanFreePath or
+// KaplanRateCompetition loss): returns edep, fills `out`.
 // lists[0..3] provide the storage of the qp / phon / next_phon / next_qp buffers; the
 // generation swaps are done on named copies so that no list is indexed at run time.
 template <class P, class G>
-__device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KList& out, const KList* lists, G& r) {
+__device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KList& out, const KList* lists, G& r,
+                                            bool photon = false) {
     using R = typename P::R;
     out.n = 0;
-    if (film_direct<P>(f, energy, r)) return energy;
-    if (energy < 2.0 * f.gap || P::uniform(r) <= R(film_loss_probability<P>(f, energy, 4.0))) {
+    if (!photon && film_direct<P>(f, energy, r)) return energy;
+    if (energy < 2.0 * f.gap || (!photon && P::uniform(r) <= R(film_loss_probability<P>(f, energy, 4.0)))) {
         out.push(energy);
         return 0.0;
     }
@@ -1210,8 +1230,16 @@ __device__ inline double film_absorb_phonon(const DevFilm& f, double energy, KLi
     KList qp = lists[0], ph = lists[1], nph = lists[2], nqp = lists[3];
     qp.n = ph.n = nph.n = nqp.n = 0;
     const double init_e = energy / n_pairs;
-    for (int i = 0; i < n_pairs; ++i) ph.push(init_e);
     double edep = 0.0;
+    for (int i = 0; i < n_pairs; ++i) {
+        if (photon) {                                 // case-II seed: one pair split (kaplan_film.hpp:133-139)
+            const double qpE = film_sample_qp_photon<P>(f, init_e, r);
+            edep += film_qp_absorption<P>(f, qpE, ph, qp, r);
+            edep += film_qp_absorption<P>(f, init_e - qpE, ph, qp, r);
+        } else {
+            ph.push(init_e);
+        }
+    }
     // a generation loop of the cascade (the reference's is unbounded); past kMaxTrials
     // generations the phonons still in the film escape, which keeps the ledger closed
     for (uint32_t gen = 1; qp.n > 0 || ph.n > 0; ++gen) {

@@ -596,6 +596,7 @@ template <class P> struct ChargeCtx {
     unsigned long long* hit_valid;
     uint64_t hit_cap;
     bool keep_hits;
+    double* kws;                         // Kaplan workspace of this lane (ElectrodeFilm release)
 
     __device__ void count(Counter c) { bs_count(*bs, c); }
     __device__ void ccount(CCounter c) { bs_count(*bs, C_NCOUNTERS + c); }
@@ -604,6 +605,8 @@ template <class P> struct ChargeCtx {
     __device__ void luke_eV(double v) { bs_ledger(*bs, L_LUKE, v); }
     __device__ void recomb(double v) { bs_ledger(*bs, L_RECOMB, v); }
     __device__ void branch(int) {}
+    __device__ double* kaplan_base() const { return kws; }
+    __device__ uint32_t kaplan_stride() const { return 1; }
     // A knock-on carrier (trap ionisation) goes on the lane's stack; a full stack turns it
     // away, and the host reruns the batch with a stack deep enough for depth + lost.
     __device__ void push_charge(CTrack<P>& c, CTrack<P>& parent) {
@@ -658,6 +661,7 @@ transport_charges_kernel(const DevModel m, const ChargeArgs a) {
     ctx.hit_valid = a.hit_valid;
     ctx.hit_cap = a.hit_cap;
     ctx.keep_hits = a.keep_hits != 0;
+    ctx.kws = a.kaplan_ws ? a.kaplan_ws + static_cast<size_t>(lane) * 5 * CMPMC_KCAP : nullptr;
 
     CTrack<P> c;
     bool alive = false, exhausted = false;

@@ -203,6 +203,7 @@ struct ChargeArgs {
     unsigned long long* ledger;
     unsigned long long* status;
     const volatile int32_t* cancel;
+    double* kaplan_ws;                 // 5 * CMPMC_KCAP doubles per lane (ElectrodeFilm release; NULL: none)
 };
 
 }  // namespace cmpmc_dev

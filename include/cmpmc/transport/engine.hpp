@@ -94,8 +94,6 @@ struct DetectorModel {
     // (which must outlive the result until the model is created), and the mesh field is
     // passed by its library handle (held by this DetectorModel).
     cmpmc_detector to_pod(std::array<std::vector<cmpmc_pad>, 3>* pads = nullptr) const {
-        if (charge_release != ChargeRelease::RecordOnly)
-            throw std::invalid_argument("cmpmc_amd: only ChargeRelease::RecordOnly runs on the GPU path");
         cmpmc_detector d{};
         d.radius = solid.radius;
         d.half_z = solid.half_z;
@@ -111,6 +109,8 @@ struct DetectorModel {
         d.russian_roulette_survival = russian_roulette_survival;
         d.charge_stepping = charge_stepping == ChargeStepping::LemanSecondOrder ? 1 : 0;
         d.allow_unvalidated = allow_unvalidated ? 1 : 0;
+        d.charge_release = static_cast<int32_t>(charge_release);
+        d.electrode_seed_max_gap = electrode_seed_max_gap;
         d.charge_sampling_dk_frac = charge_sampling_dk_frac;
         d.mesh = mesh_field_3d.has_value() ? mesh_field_3d->handle()
                  : mesh_field_2d.has_value() ? mesh_field_2d->handle() : nullptr;

@@ -213,6 +213,12 @@ typedef struct {
     const cmpmc_mesh* mesh;       /* NULL: the uniform efield. Else the charge step's field is the
                                      mesh's (zero outside it), read at model creation (the model
                                      keeps its own copy; the mesh may be destroyed afterwards) */
+    int32_t charge_release;       /* DetectorModel::ChargeRelease (engine.hpp:127-140): 0 RecordOnly,
+                                     1 BulkDebye (E_rel released as h nu_D phonons at the contact),
+                                     2 ElectrodeFilm (photon-seeded cascades in the face's film) */
+    int32_t _pad_cr;
+    double electrode_seed_max_gap; /* ElectrodeFilm: E_rel enters the film as quanta of at most this
+                                      many gaps (default 80) */
 } cmpmc_detector;
 
 typedef struct cmpmc_model cmpmc_model;

@@ -223,6 +223,8 @@ typedef struct {
     int32_t charge_stepping;             /* DetectorModel::ChargeStepping: 0 FirstOrderDkCap, 1 LemanSecondOrder */
     double charge_sampling_dk_frac;      /* LemanSecondOrder: sampling-step cap on |dk| / max(k_s, |k|) */
     const orc_mesh* mesh;                /* mesh_field_3d / mesh_field_2d; NULL: the uniform efield */
+    int32_t charge_release;              /* DetectorModel::ChargeRelease: 0 RecordOnly, 1 BulkDebye, 2 ElectrodeFilm */
+    double electrode_seed_max_gap;       /* ElectrodeFilm: E_rel enters as quanta of <= this x Delta */
 } orc_detector;
 
 void orc_detector_default(orc_detector* d);
@@ -289,6 +291,9 @@ double orc_kaplan_pair_breaking_rate(const orc_film* f, double omega);
 /* KaplanFilm::loss_probability (kaplan_film.hpp:91-99) */
 double orc_film_loss_probability(const orc_film* f, double energy, double thickness_frac);
 double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_refl, orc_rng* r);
+/* KaplanFilm::absorb_quantum (kaplan_film.hpp:110-206): photon != 0 seeds the cascade with a
+ * case-II pair split (no direct absorption, no entry escape) */
+double orc_absorb_quantum(const orc_film* f, double energy, int photon, double* refl, int* n_refl, orc_rng* r);
 
 /* KaplanSurrogateLUT (kaplan_film.hpp:325-383): N_E = 128 energy nodes on [2 Delta, 40 Delta],
  * N_S = 512 stored cascade outcomes per node (absorbed fraction, re-emitted energies as

@@ -641,6 +641,8 @@ void orc_detector_default(orc_detector* d) {
     d->russian_roulette_survival = 0.5;
     d->charge_stepping = 0;
     d->charge_sampling_dk_frac = 0.25;
+    d->charge_release = 0;                  /* RecordOnly */
+    d->electrode_seed_max_gap = 80.0;       /* engine.hpp:140 */
 }
 
 static double eval_poly(const double* c, int n, double f) {   /* surface_property.hpp:182-189 */
@@ -814,11 +816,32 @@ double orc_sample_phonon_energy(const orc_film* f, double E, orc_rng* r) {   /* 
     }
 }
 
+/* case-II (photon) pair split: kaplan_film.hpp:229-244 */
+static inline double qp2_weight(double E, double d, double x) {
+    return dmax(0.0, x * (E - x) - d * d) / sqrt((x + d) * (E - x + d));
+}
+static double sample_qp_energy_case2(const orc_film* f, double E, orc_rng* r) {
+    const double d = f->gap_energy;
+    const double span = E - 2.0 * d;
+    if (span <= 1e-16) return 0.5 * E;
+    const double w_max = dmax(qp2_weight(E, d, d), qp2_weight(E, d, 0.5 * E)) * 1.0001;
+    for (;;) {
+        const double u = orc_uniform(r);
+        const double s = sin(ORC_HALF_PI * u);
+        const double x = d + span * (s * s);
+        if (orc_uniform(r) * w_max <= qp2_weight(E, d, x)) return x;
+    }
+}
+
 double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_refl, orc_rng* r) {
+    return orc_absorb_quantum(f, energy, 0, refl, n_refl, r);
+}
+
+double orc_absorb_quantum(const orc_film* f, double energy, int photon, double* refl, int* n_refl, orc_rng* r) {
     kbuf out; out.n = 0;
     double edep = 0.0;
-    if (do_direct_absorption(f, energy, r)) { *n_refl = 0; return energy; }
-    if (is_subgap(f, energy) || orc_uniform(r) <= orc_film_loss_probability(f, energy, 4.0)) {
+    if (!photon && do_direct_absorption(f, energy, r)) { *n_refl = 0; return energy; }
+    if (is_subgap(f, energy) || (!photon && orc_uniform(r) <= orc_film_loss_probability(f, energy, 4.0))) {
         refl[0] = energy; *n_refl = 1;
         return 0.0;
     }
@@ -827,7 +850,15 @@ double orc_absorb_phonon(const orc_film* f, double energy, double* refl, int* n_
     static __thread kbuf qp, phon, next_phon, next_qp;
     qp.n = 0; phon.n = 0; next_phon.n = 0; next_qp.n = 0;
     const double init_e = energy / n_pairs;
-    for (int i = 0; i < n_pairs; ++i) kpush(&phon, init_e);
+    for (int i = 0; i < n_pairs; ++i) {
+        if (photon) {
+            const double qpE = sample_qp_energy_case2(f, init_e, r);
+            edep += calc_qp_absorption(f, qpE, &phon, &qp, r);
+            edep += calc_qp_absorption(f, init_e - qpE, &phon, &qp, r);
+        } else {
+            kpush(&phon, init_e);
+        }
+    }
 
     while (qp.n > 0 || phon.n > 0) {
         if (phon.n > 0) {                            /* A: pair breaking */

@@ -94,6 +94,7 @@ class Detector(C.Structure):
         ("russian_roulette_bounces", u32), ("russian_roulette_survival", dbl),
         ("charge_stepping", C.c_int), ("charge_sampling_dk_frac", dbl),
         ("mesh", C.c_void_p),
+        ("charge_release", C.c_int32), ("electrode_seed_max_gap", dbl),
     ]
 
 

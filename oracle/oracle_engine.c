@@ -571,6 +571,55 @@ static void reflect_charge(const orc_engine* e, orc_charge* t, const double n[3]
     }
 }
 
+/* Phonons a collected carrier releases at its contact (DetectorModel::charge_release,
+ * engine.hpp:888-927), pushed to `phonons` from 1e-11 m inside the face with a DOS mode and
+ * an inward direction: BulkDebye (film == NULL) max(1, round(E_rel / h nu_D)) phonons of
+ * equal energy; ElectrodeFilm max(1, ceil(E_rel / q_max)) photon-seeded film cascades of
+ * equal quanta (q_max = max(4, electrode_seed_max_gap) Delta), whose escaping phonons are
+ * released. Returns the energy absorbed in the film (0 for BulkDebye); *emitted receives
+ * the released energy. */
+static void push_release(const orc_engine* e, orc_charge* trk, const double p0[3], const double n_in[3], double E,
+                         orc_track_vec* phonons, int per_track, uint64_t ev_base, orc_rng* R) {
+    const int m = sample_dos_mode(&e->L->cfg, R);
+    double kd[3];
+    sample_inward_wavevector(e, m, n_in, R, kd);
+    orc_track ph;
+    orc_make_track(e, p0, kd, m, E, trk->time, trk->weight, &ph);
+    if (per_track) {
+        ph.key = orc_child_key(trk->key, ++trk->spawn_no);
+        orc_rng32_for_track(&ph.rng, ev_base, ph.key);
+    }
+    orc_track_push(phonons, &ph);
+}
+
+static double release_phonons(const orc_engine* e, orc_charge* trk, double e_rel, const double n_in[3],
+                              const orc_film* film, orc_track_vec* phonons, int per_track, uint64_t ev_base,
+                              orc_rng* R, double* emitted) {
+    double p0[3];
+    for (int i = 0; i < 3; ++i) p0[i] = trk->pos[i] + n_in[i] * 1e-11;
+    *emitted = 0.0;
+    if (!film) {                                           /* emit_debye_phonons (:894-904) */
+        const double ed = e->L->cfg.debye_freq * ORC_H_PLANCK;
+        const int n = (int)dmax(1.0, round(e_rel / ed));
+        for (int i = 0; i < n; ++i) push_release(e, trk, p0, n_in, e_rel / n, phonons, per_track, ev_base, R);
+        return 0.0;
+    }
+    const double qmax = dmax(4.0, e->D->electrode_seed_max_gap) * film->gap_energy;   /* :906-927 */
+    const int n = (int)dmax(1.0, ceil(e_rel / qmax));
+    const double q = e_rel / n;
+    double absorbed = 0.0, out = 0.0, refl[256];
+    int nr = 0;
+    for (int i = 0; i < n; ++i) {
+        absorbed += orc_absorb_quantum(film, q, 1, refl, &nr, R);
+        for (int k = 0; k < nr; ++k) {
+            push_release(e, trk, p0, n_in, refl[k], phonons, per_track, ev_base, R);
+            out += refl[k];
+        }
+    }
+    *emitted = out;
+    return absorbed;
+}
+
 /* ---------------- SimulationEngine::transport_charges: engine.hpp:192-380 ---------------- */
 void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_track_vec* phonons,
                            orc_hit_vec* hits, orc_stats* st, orc_rng* rng,
@@ -668,7 +717,18 @@ void orc_transport_charges(const orc_engine* e, orc_charge_vec* charges, orc_tra
                     const double muF = (D->fermi_level_above_Ev_eV >= 0.0) ? D->fermi_level_above_Ev_eV : 0.5 * e_gap;
                     const double e_rel = trk.ekin + ((trk.type == 1) ? (e_gap - muF) : muF);
                     st->charges_collected++;
-                    const double recomb = e_rel * trk.weight;
+                    double hit_eV = e_rel;
+                    if (D->charge_release == 1) {              /* BulkDebye (engine.hpp:302-305) */
+                        double emitted;
+                        hit_eV = release_phonons(e, &trk, e_rel, n_in, NULL, phonons, per_track, ev_base, R, &emitted);
+                    } else if (D->charge_release == 2 && s->has_film) {   /* ElectrodeFilm (:306-312) */
+                        double emitted;
+                        hit_eV = release_phonons(e, &trk, e_rel, n_in, &s->film, phonons, per_track, ev_base, R,
+                                                 &emitted);
+                        const double lost = dmax(0.0, e_rel - hit_eV - emitted);
+                        st->escaped_or_killed_eV += lost * trk.weight;
+                    }
+                    const double recomb = hit_eV * trk.weight;
                     st->charge_recomb_edep_eV += recomb;
                     st->total_deposited_eV += recomb;
                     orc_hit h;

@@ -146,6 +146,8 @@ DetectorModel to_detector(const orc_detector* d) {
     // the oracle's runs include the guarded IVRateModel::Matrix (engine.hpp:197-199); the
     // guard itself is the product's to enforce
     m.allow_unvalidated = true;
+    m.charge_release = static_cast<DetectorModel::ChargeRelease>(d->charge_release);
+    m.electrode_seed_max_gap = d->electrode_seed_max_gap;
     if (d->mesh) {
         const orc_mesh* g = d->mesh;
         if (g->dim == 3) m.mesh_field_3d = mesh3_from(g->x, static_cast<long>(g->n_vertices), g->v, g->cells,

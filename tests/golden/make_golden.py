@@ -184,6 +184,15 @@ def main():
     O.set_bias(dmx, 4.0)
     dmx.luke_sample_fraction = 0.0
     runs["iv_matrix"] = (O.workload(O.WL_PAIR), 6, 42, dmx, O.matrix_iv_crystal())
+    # ChargeRelease at carrier collection: BulkDebye (h nu_D phonons at the contact) and
+    # ElectrodeFilm (photon-seeded film cascades, escaping phonons re-injected), pairs at
+    # 4 V, Luke phonons counted
+    for tag, mode in (("release_debye", 1), ("release_film", 2)):
+        dr_ = O.default_detector()
+        O.set_bias(dr_, 4.0)
+        dr_.luke_sample_fraction = 0.0
+        dr_.charge_release = mode
+        runs[tag] = (O.workload(O.WL_PAIR), 6, 42, dr_)
     # energy partition (charge/energy_partition.hpp, niel_models.hpp): deposits partitioned
     # into pairs and prompt phonons, transported at 4 V with 5 % of the Luke phonons
     dp = O.default_detector()

@@ -173,3 +173,54 @@ def test_matrix_iv_guard_and_both_forms(lib_cmpmc, oracle):
         assert gs["intervalley_scatters"] > 100 * (2 * n)
     q = cm.Engine(cm.Lattice(), d).run_workload(wl, n, seed=seed, form=cm.FORM_BATCHED)
     assert q["stats"]["intervalley_scatters"] < 5 * (2 * n)
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["bulk_debye", "electrode_film"])
+def test_charge_release_both_forms(lib_cmpmc, oracle, mode):
+    """DetectorModel::ChargeRelease at carrier collection (engine.hpp:299-312, 888-927):
+    BulkDebye releases E_rel as h nu_D phonons at the contact and records nothing in the
+    carrier's hit; ElectrodeFilm sends E_rel into the face's Kaplan film as photon-seeded
+    quanta (the hit keeps what the film absorbs, the escaping phonons are released). Pairs at
+    4 V with Luke phonons counted, both forms against the oracle (pinned to the reference by
+    the golden runs release_debye and release_film): every carrier collected, the carriers'
+    recorded energy, the released phonons' transport (steps, film hits, deposited energy)
+    and hit distributions."""
+    cm, O = lib_cmpmc, oracle
+    d = cm.default_detector()
+    cm.set_bias(d, 4.0)
+    d.luke_sample_fraction = 0.0
+    d.charge_release = mode
+    eng = cm.Engine(detector=d)
+    od = O.default_detector()
+    O.set_bias(od, 4.0)
+    od.luke_sample_fraction = 0.0
+    od.charge_release = mode
+    oeng = O.Engine(detector=od)
+    wl, owl = cm.workload(cm.WL_PAIR), O.workload(O.WL_PAIR)
+    n, seed = 400, 42
+    for form, batched in ((cm.FORM_PLAIN, False), (cm.FORM_BATCHED, True)):
+        name = f"release {mode} {'batched' if batched else 'plain'}"
+        print(f"case: {name}, n={n} seed={seed}", flush=True)
+        g = eng.run_workload(wl, n, seed=seed, form=form)
+        o = oeng.run(owl, n, seed, batched=batched, threads=8)
+        gs, os_ = g["stats"], o["stats"]
+        print(f"{name}: deposited {gs['total_deposited_eV']:.4f} vs {os_['total_deposited_eV']:.4f} eV, hits "
+              f"{gs['sensor_hits']} vs {os_['sensor_hits']}, steps {gs['total_steps']} vs {os_['total_steps']}", flush=True)
+        assert gs["charges_collected"] == os_["charges_collected"] == 2 * n
+        assert gs["total_steps"] > 0, f"{name}: the released phonons are transported"
+        for k in ("total_deposited_eV", "sensor_hits", "total_steps"):
+            assert gs[k] == pytest.approx(os_[k], rel=0.05), f"{name} {k}"
+        if mode == 1:
+            assert gs["charge_recomb_edep_eV"] == 0.0 and os_["charge_recomb_edep_eV"] == 0.0
+        else:
+            assert gs["charge_recomb_edep_eV"] == pytest.approx(os_["charge_recomb_edep_eV"], rel=0.03)
+        carrier = g["hits"]["excitation"] != 0
+        assert carrier.sum() == 2 * n
+        ph_g = g["hits"][~carrier]
+        ph_o = o["hits"][o["hits"]["excitation"] == 0]
+        for f in ("time", "edep"):
+            _ks_ok(ph_g[f], ph_o[f], f"{name} phonon hit {f}")
+    bad = cm.default_detector()
+    bad.charge_release = 5
+    with pytest.raises(cm.CmpmcError):
+        cm.Engine(detector=bad)

@@ -367,18 +367,29 @@ def _gold_runs(O):
         "c1_bil": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.default_detector()),
         "film_rc": (O.workload(O.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 200, 42, O.rate_competition_detector()),
         "iv_matrix": (O.workload(O.WL_PAIR), 6, 42, dmx, O.matrix_iv_crystal()),
+        "release_debye": (O.workload(O.WL_PAIR), 6, 42, _release_detector(O, 1)),
+        "release_film": (O.workload(O.WL_PAIR), 6, 42, _release_detector(O, 2)),
     }
+
+
+def _release_detector(O, mode):
+    d = O.default_detector()
+    O.set_bias(d, 4.0)
+    d.luke_sample_fraction = 0.0
+    d.charge_release = mode
+    return d
 
 
 @pytest.mark.parametrize("tag", ["c1", "lognormal", "mixed", "pairs", "bench_demo", "bench_dos", "roulette",
                                  "specular", "trap_ion", "leman", "leman_ion", "c1_exact", "deposit_nr",
                                  "deposit_er", "mesh3_pairs", "mesh2_pairs", "mesh3_leman_ion", "c1_bil",
-                                 "film_rc", "iv_matrix"])
+                                 "film_rc", "iv_matrix", "release_debye", "release_film"])
 def test_run_events_golden(oracle, gold, tag):
     """Whole run_events results (SimStats bytes, hit_offsets, hits) equal the reference's
     (c1_exact: with KinematicsMode::Exact; c1_bil: KinematicsMode::Bilinear; film_rc: both
     films on FilmLossModel::KaplanRateCompetition; iv_matrix: the IVRateModel::Matrix
-    intervalley rate for pairs at 4 V; bench_dos: the reference benchmark's own phonon
+    intervalley rate for pairs at 4 V; release_debye, release_film: ChargeRelease BulkDebye and
+    ElectrodeFilm at carrier collection; bench_dos: the reference benchmark's own phonon
     workload; roulette, specular, trap_ion: Russian roulette, half-specular surfaces, trap
     impact ionisation; leman, leman_ion: Leman second-order charge stepping; deposit_nr,
     deposit_er: generators that partition a nuclear / electron recoil with
