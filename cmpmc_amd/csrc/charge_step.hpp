@@ -1,6 +1,6 @@
 // One step of SimulationEngine::transport_charges (engine.hpp:210-377), first-order
 // (FirstOrderDkCap) or Leman second-order (LemanSecondOrder) stepping, in the uniform field
-// or the FEM mesh field (MeshX policies), RecordOnly release at collection, with the
+// or the FEM mesh field (MeshX policies), any ChargeRelease at collection, with the
 // ChargePhysics leaves of charge/charge_transport.hpp:49-241 and the Herring-Vogt
 // kinematics of lattice/crystal_lattice.hpp:239-293.
 //
