@@ -558,7 +558,14 @@ class Lattice:
         check(capi.load().cmpmc_lattice_build(C.byref(self.crystal), self.oct.ctypes.data, self.cdf_lt.ctypes.data,
                                                self.cdf_tt.ctypes.data, C.byref(self.c)))
 
-    KINEMATICS_OCTAHEDRAL, KINEMATICS_EXACT, KINEMATICS_BILINEAR = 0, 1, 2
+    KINEMATICS_OCTAHEDRAL, KINEMATICS_EXACT, KINEMATICS_BILINEAR, KINEMATICS_SYMMETRIC = 0, 1, 2, 3
+
+    def build_symmetric(self) -> None:
+        """The cubic-symmetry v_g table (kinematics mode 3, cmpmc_lattice_build_symmetric),
+        which the batched kernels stage in LDS; engines created afterwards carry it."""
+        if getattr(self, "sym", None) is None:
+            self.sym = np.zeros((3, 3, 33, 33), dtype=np.float32)
+            check(capi.load().cmpmc_lattice_build_symmetric(C.byref(self.c), self.sym.ctypes.data))
 
     def build_bilinear(self) -> None:
         """The bilinear (cos theta, phi) v_g table (crystal_lattice.hpp:350-368), which
@@ -569,12 +576,14 @@ class Lattice:
 
     def set_kinematics_mode(self, kin_mode: int) -> None:
         """CrystalLattice::set_kinematics_mode: 0 Octahedral (LUT, default), 1 Exact,
-        2 Bilinear (builds the bilinear table). Engines take the lattice's mode when they
-        are created."""
-        if kin_mode not in (0, 1, 2):
-            raise ValueError("kinematics mode: 0 (Octahedral), 1 (Exact) or 2 (Bilinear)")
+        2 Bilinear (builds the bilinear table), 3 the cubic-symmetry table (this build's;
+        builds it). Engines take the lattice's mode when they are created."""
+        if kin_mode not in (0, 1, 2, 3):
+            raise ValueError("kinematics mode: 0 (Octahedral), 1 (Exact), 2 (Bilinear) or 3 (cubic-symmetry table)")
         if kin_mode == 2:
             self.build_bilinear()
+        if kin_mode == 3:
+            self.build_symmetric()
         self.c.kinematics_mode = kin_mode
 
     def map_k_to_vg(self, mode: int, k) -> tuple[np.ndarray, np.ndarray]:
@@ -637,7 +646,8 @@ class Engine:
 
     def set_kinematics(self, kin_mode: int) -> None:
         """Kinematics of this engine's device model: 0 Octahedral LUT, 1 Exact, 2 Bilinear
-        (for an engine whose lattice had its bilinear table built)."""
+        (for an engine whose lattice had that table built), 3 cubic-symmetry table (every
+        model carries it)."""
         check(capi.load().cmpmc_model_set_kinematics(self.ptr, kin_mode))
 
     def device_map_k_to_vg(self, mode: int, k, form: int = FORM_PLAIN) -> tuple[np.ndarray, np.ndarray]:

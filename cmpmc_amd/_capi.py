@@ -85,7 +85,8 @@ class Lattice(C.Structure):
                 ("cdf_tt", C.POINTER(dbl)), ("e_cond_mass_eV", dbl), ("h_mass_eV", dbl),
                 ("mass_ratio", dbl * 3), ("hv_sqrt", dbl * 3), ("hv_sqrt_inv", dbl * 3),
                 ("valley_rot", ((dbl * 3) * 3) * 8), ("l0_e", dbl), ("l0_h", dbl), ("k_sound_e", dbl),
-                ("k_sound_h", dbl), ("kinematics_mode", i32), ("_pad", i32), ("bil_lut", C.POINTER(dbl))]
+                ("k_sound_h", dbl), ("kinematics_mode", i32), ("_pad", i32), ("bil_lut", C.POINTER(dbl)),
+                ("sym_lut", C.POINTER(C.c_float))]
 
 
 PHONON_DTYPE = np.dtype({"names": ["pos", "k_dir", "vg_dir", "energy", "vg_mag", "time", "weight", "bounces", "mode"],
@@ -207,6 +208,7 @@ def load():
     L.cmpmc_lattice_build.argtypes = [C.POINTER(Crystal), C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(Lattice)]
     L.cmpmc_map_k_to_vg.argtypes = [C.POINTER(Lattice), C.c_int, u64, C.c_void_p, C.c_void_p, C.c_void_p]
     L.cmpmc_lattice_build_bilinear.argtypes = [C.POINTER(Lattice), C.c_void_p]
+    L.cmpmc_lattice_build_symmetric.argtypes = [C.POINTER(Lattice), C.c_void_p]
     L.cmpmc_film_absorb_quantum.argtypes = [C.POINTER(Film), dbl, C.c_int, C.POINTER(u64), C.c_void_p,
                                             C.POINTER(i32), C.POINTER(dbl)]
     L.cmpmc_film_loss_probability.argtypes = [C.POINTER(Film), dbl, dbl, C.POINTER(dbl), C.POINTER(dbl)]
@@ -307,7 +309,8 @@ def check(code: int) -> None:
 
 EXPORTED_SYMBOLS = [
     "cmpmc_last_error", "cmpmc_device_info", "cmpmc_abi_sizeof", "cmpmc_crystal_germanium", "cmpmc_film_aluminum",
-    "cmpmc_detector_default", "cmpmc_lattice_build", "cmpmc_lattice_build_bilinear", "cmpmc_map_k_to_vg",
+    "cmpmc_detector_default", "cmpmc_lattice_build", "cmpmc_lattice_build_bilinear", "cmpmc_lattice_build_symmetric",
+    "cmpmc_map_k_to_vg",
     "cmpmc_model_create",
     "cmpmc_model_destroy", "cmpmc_model_cancel", "cmpmc_model_trim", "cmpmc_model_set_kinematics",
     "cmpmc_model_map_k_to_vg", "cmpmc_transport_serial",

@@ -244,6 +244,7 @@ static void fill_dev_model(cmpmc_model* M, const cmpmc_lattice* lat, const cmpmc
     m.ivm = M->d_ivm;
     m.bil = M->d_bil;
     m.bil_f = M->d_bil ? reinterpret_cast<const float4*>(M->d_bil + kBilDoubles) : nullptr;
+    m.sym = M->d_sym;
     m.kin_mode = lat->kinematics_mode;
     m.density = lat->cfg.density;
     std::memcpy(m.orient, c.orient, sizeof(m.orient));
@@ -432,8 +433,9 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
         return fail(CMPMC_ERR_INVALID, "cmpmc_model_create: null argument");
     *out = nullptr;
     CHK(check_detector(det));
-    if (lat->kinematics_mode < 0 || lat->kinematics_mode > 2)
-        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0), Exact (1) and Bilinear (2) are available",
+    if (lat->kinematics_mode < 0 || lat->kinematics_mode > 3)
+        return fail(CMPMC_ERR_INVALID,
+                    "kinematics mode %d: Octahedral (0), Exact (1), Bilinear (2) and Symmetric (3) are available",
                     lat->kinematics_mode);
     if (lat->kinematics_mode == 2 && !lat->bil_lut)
         return fail(CMPMC_ERR_INVALID, "KinematicsMode::Bilinear needs the lattice's bilinear table "
@@ -487,6 +489,27 @@ extern "C" int cmpmc_model_create(const cmpmc_lattice* lat, const cmpmc_detector
         HIPCHK(hipMemcpy(M->d_bil, lat->bil_lut, kBilDoubles * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(M->d_bil + kBilDoubles, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
     }
+    {
+        // the cubic-symmetry table: the lattice's, or built here (3,267 solves) so that every
+        // model can switch to kinematics mode 3
+        constexpr size_t n_sym = static_cast<size_t>(3) * 3 * CMPMC_N_SYM * CMPMC_N_SYM;
+        std::vector<float> own;
+        const float* sym = lat->sym_lut;
+        if (!sym) {
+            own.resize(n_sym);
+            cmpmc_lattice tmp = *lat;
+            if (cmpmc_lattice_build_symmetric(&tmp, own.data()) != CMPMC_OK) {
+                cmpmc_model_destroy(M);
+                return CMPMC_ERR_INVALID;
+            }
+            sym = own.data();
+        }
+        if (hipMalloc(&M->d_sym, n_sym * sizeof(float)) != hipSuccess) {
+            cmpmc_model_destroy(M);
+            return fail(CMPMC_ERR_HIP, "cmpmc_model_create: symmetry table allocation failed");
+        }
+        HIPCHK(hipMemcpy(M->d_sym, sym, n_sym * sizeof(float), hipMemcpyHostToDevice));
+    }
     void* hc = nullptr;
     HIPCHK(hipHostMalloc(&hc, sizeof(int32_t), hipHostMallocMapped));
     M->h_cancel = static_cast<volatile int32_t*>(hc);
@@ -534,6 +557,7 @@ extern "C" int cmpmc_model_destroy(cmpmc_model* M) {
     if (M->d_cdf) (void)hipFree(M->d_cdf);
     if (M->d_stiff) (void)hipFree(M->d_stiff);
     if (M->d_bil) (void)hipFree(M->d_bil);
+    if (M->d_sym) (void)hipFree(M->d_sym);
     if (M->d_ivm) (void)hipFree(M->d_ivm);
     if (M->d_pads) (void)hipFree(M->d_pads);
     if (M->d_amm) (void)hipFree(M->d_amm);
@@ -610,9 +634,9 @@ extern "C" int cmpmc_model_trim(cmpmc_model* M) {
 
 extern "C" int cmpmc_model_set_kinematics(cmpmc_model* M, int mode) {
     if (!M) return fail(CMPMC_ERR_INVALID, "null model");
-    if (mode < 0 || mode > 2)
-        return fail(CMPMC_ERR_INVALID, "kinematics mode %d: Octahedral (0), Exact (1) and Bilinear (2) are available",
-                    mode);
+    if (mode < 0 || mode > 3)
+        return fail(CMPMC_ERR_INVALID,
+                    "kinematics mode %d: Octahedral (0), Exact (1), Bilinear (2) and Symmetric (3) are available", mode);
     if (mode == 2 && !M->d_bil)
         return fail(CMPMC_ERR_INVALID, "KinematicsMode::Bilinear: the model was created from a lattice without the "
                                        "bilinear table (cmpmc_lattice_build_bilinear)");
@@ -1027,7 +1051,8 @@ int run_plain(cmpmc_model* M, const Job& J, const RunCfg& C, const Outputs& O, h
 int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
-    const int flags = (M->dm.kin_mode != 0 ? 1 : 0) | (M->dm.surf_x ? 2 : 0) | (M->dm.mesh.dim != 0 ? 4 : 0);
+    const int flags = (M->dm.kin_mode != 0 ? 1 : 0) | (M->dm.surf_x ? 2 : 0) | (M->dm.mesh.dim != 0 ? 4 : 0) |
+                      (M->dm.kin_mode == 3 ? 8 : 0);
     const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, flags, block, &per_cu)
                                       : cmpmc_occupancy(which, waves, flags, block, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;

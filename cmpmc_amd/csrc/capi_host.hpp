@@ -96,6 +96,7 @@ struct cmpmc_model {
     double* d_cdf = nullptr;
     double* d_stiff = nullptr;              // C_ijlm: 81 doubles, then 81 floats
     double* d_bil = nullptr;                // bilinear v_g table (FP64), then its FP32 float4 copy
+    float* d_sym = nullptr;                 // cubic-symmetry v_g table [3][3][33][33]
     double* d_ivm = nullptr;                // IVRateModel::Matrix constants (kIvmDoubles)
     cmpmc_pad* d_pads = nullptr;            // electrode pads of all surfaces
     float* d_amm = nullptr;                 // FP32 AMM film-entry tables [3][2][kAmmTab + 1]

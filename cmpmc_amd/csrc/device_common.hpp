@@ -23,6 +23,11 @@
 #define CMPMC_KCAP 256
 #endif
 
+// An LDS (address space 3) pointer: loads and stores through it are ds_* instructions.
+#ifndef CMPMC_LDS
+#define CMPMC_LDS __attribute__((address_space(3)))
+#endif
+
 namespace cmpmc_dev {
 
 // ------------------------------------------------------------------ constants (core/units.hpp)
@@ -118,6 +123,7 @@ struct DevModel {
     const float* stiff_f;       // the same in FP32
     const double* ivm;          // IVRateModel::Matrix constants (kIvm* layout; NULL: other models)
     const double* bil;          // bilinear (cos theta, phi) v_g table [3][181][361][3] (NULL: none)
+    const float* sym;           // cubic-symmetry v_g table [3][3][33][33] (kinematics mode 3)
     const float4* bil_f;        // the same in FP32, one (vx, vy, vz, 0) per node
     int32_t kin_mode;           // CrystalLattice::KinematicsMode: 0 table (the default kernels), 1 exact,
                                 // 2 bilinear (both in the kin_exact kernel instantiations)
@@ -296,6 +302,7 @@ struct Fp64 {
     using TRng = Rng;                            // a track's transport stream
     static constexpr bool exact = true;
     static constexpr bool kin_exact = false;     // octahedral LUT kinematics
+    static constexpr bool lds_sym = false;       // the cubic-symmetry table is read from HBM, not LDS
     static constexpr bool surf_x = false;        // no electrode masks or AMM film entry
     static constexpr bool mesh = false;          // uniform field (no FEM mesh)
     __device__ static Rng track_stream(uint64_t base, uint64_t key) { return rng_for_track(base, key); }
@@ -327,6 +334,7 @@ struct Fp32 {
     using TRng = Rng32;                          // a track's transport stream
     static constexpr bool exact = false;
     static constexpr bool kin_exact = false;
+    static constexpr bool lds_sym = false;
     static constexpr bool surf_x = false;
     static constexpr bool mesh = false;
     __device__ static Rng32 track_stream(uint64_t base, uint64_t key) { return rng32_for_track(base, key); }
@@ -364,6 +372,12 @@ struct Fp64X : Fp64 {
 };
 struct Fp32X : Fp32 {
     static constexpr bool kin_exact = true;
+};
+// Kinematics mode 3 in the batched form: the cubic-symmetry table staged in LDS by each
+// block (39 KiB) instead of the octahedral table read through the L2 (3 MiB), with the decay
+// tables read from HBM to make room.
+struct Fp32S : Fp32 {
+    static constexpr bool lds_sym = true;
 };
 // The same with the surface extras (electrode masks, AMM film entry) compiled in: the
 // kernels of a model whose surfaces have neither are built without that code, so the
@@ -744,16 +758,76 @@ __device__ inline void map_k_to_vg_bilinear(const DevModel& m, int mode, V3<type
     vg_mag = mag;
 }
 
+// Kinematics mode 3: the cubic-symmetry table (no counterpart in the reference). A cubic
+// crystal's v_g(R k) = R v_g(k) for the 48 signed axis permutations R, so the table covers
+// |k_x| >= |k_y| >= |k_z| >= 0 on nodes (1, s, s t), s, t = 0, 1/32, .., 1: the lookup ranks
+// the axes by |k| (largest first, ties to the lower axis), interpolates the canonical v_g
+// bilinearly in (s, t) = (q / p, r / q) of the sorted magnitudes and puts each component back
+// on its axis with the sign of k there. `tab` is the table in HBM or its LDS copy.
+constexpr int kNSym = CMPMC_N_SYM;
+constexpr int kSymFloats = 3 * 3 * kNSym * kNSym;
+template <class P, class T>
+__device__ inline void map_k_to_vg_sym(const T* tab, const DevModel& m, int mode, V3<typename P::R> kdet,
+                                       V3<typename P::R>& vg_dir, typename P::R& vg_mag) {
+    using R = typename P::R;
+    const V3<R> k = m.orient_identity ? kdet : mat_mul_t(m.orient, kdet);
+    const R a = fabs(k.x), b = fabs(k.y), c = fabs(k.z);
+    const int rx = (a < b) + (a < c), ry = (b <= a) + (b < c), rz = (c <= a) + (c <= b);
+    R p, q, r;
+    if constexpr (sizeof(R) == 4) {            // v_max3 / v_med3 / v_min3
+        p = __builtin_fmaxf(__builtin_fmaxf(a, b), c);
+        r = __builtin_fminf(__builtin_fminf(a, b), c);
+        q = __builtin_amdgcn_fmed3f(a, b, c);
+    } else {
+        R srt[3];
+        srt[rx] = a;
+        srt[ry] = b;
+        srt[rz] = c;
+        p = srt[0];
+        q = srt[1];
+        r = srt[2];
+    }
+    const R sv = (p > R(0)) ? P::div(q, p) : R(0);
+    const R tv = (q > R(0)) ? P::div(r, q) : R(0);
+    const R u = sv * R(kNSym - 1), w = tv * R(kNSym - 1);
+    const int i0 = tmin(static_cast<int>(u), kNSym - 2), j0 = tmin(static_cast<int>(w), kNSym - 2);
+    const R du = u - R(i0), dv = w - R(j0);
+    const R w00 = (R(1) - du) * (R(1) - dv), w10 = du * (R(1) - dv), w01 = (R(1) - du) * dv, w11 = du * dv;
+    R vc[3];
+    for (int cc = 0; cc < 3; ++cc) {
+        const T* t = tab + ((mode * 3 + cc) * kNSym + i0) * kNSym + j0;
+        vc[cc] = R(t[0]) * w00 + R(t[kNSym]) * w10 + R(t[1]) * w01 + R(t[kNSym + 1]) * w11;
+    }
+    auto pick = [&vc](int rank) { return rank == 0 ? vc[0] : (rank == 1 ? vc[1] : vc[2]); };
+    const R vx = pick(rx), vy = pick(ry), vz = pick(rz);
+    const V3<R> v{(k.x < R(0)) ? -vx : vx, (k.y < R(0)) ? -vy : vy, (k.z < R(0)) ? -vz : vz};
+    const R mag = norm<P>(v);
+    const V3<R> t = vdiv<P>(v, tmax(mag, R(1e-12)));
+    vg_dir = unit<P>(m.orient_identity ? t : mat_mul(m.orient, t));
+    vg_mag = mag;
+}
+
+// The LDS copy of the cubic-symmetry table (allocated in the kernels whose policy stages it).
+__device__ inline CMPMC_LDS float* sym_lds_table() {
+    __shared__ float t[kSymFloats];
+    return (CMPMC_LDS float*)t;
+}
+
 // CrystalLattice::map_k_to_vg: the octahedral LUT (default) or, for the kin_exact
-// policies, the per-call solve or the bilinear table (the model's kinematics mode).
+// policies, the per-call solve or the bilinear or cubic-symmetry table (the model's
+// kinematics mode); the lds_sym policies read the symmetry table from LDS.
 template <class P>
 __device__ inline void map_k_to_vg(const DevModel& m, int mode, V3<typename P::R> kdet, V3<typename P::R>& vg_dir,
                                    typename P::R& vg_mag) {
     using R = typename P::R;
-    if constexpr (!P::kin_exact) {
+    if constexpr (P::lds_sym) {
+        map_k_to_vg_sym<P>(sym_lds_table(), m, mode, kdet, vg_dir, vg_mag);
+    } else if constexpr (!P::kin_exact) {
         map_k_to_vg_oct<P>(m, mode, kdet, vg_dir, vg_mag);
     } else if (m.kin_mode == 2) {
         map_k_to_vg_bilinear<P>(m, mode, kdet, vg_dir, vg_mag);
+    } else if (m.kin_mode == 3) {
+        map_k_to_vg_sym<P>(m.sym, m, mode, kdet, vg_dir, vg_mag);
     } else {
         const R kk[3] = {kdet.x, kdet.y, kdet.z};
         R d[3];

@@ -11,7 +11,9 @@ namespace cmpmc_dev {
 // compiles to ds_read / ds_write; through a generic pointer (a pointer stored in a context
 // struct loses the address space) it compiles to flat accesses that wait for all
 // outstanding memory operations. lds() converts the address of a __shared__ object.
+#ifndef CMPMC_LDS
 #define CMPMC_LDS __attribute__((address_space(3)))
+#endif
 template <class T> __device__ inline CMPMC_LDS T* lds(T* p) { return (CMPMC_LDS T*)p; }
 
 // Global counters (uint64): phonon counters C_* then charge counters CC_* .

@@ -346,11 +346,88 @@ extern "C" int cmpmc_lattice_build_bilinear(cmpmc_lattice* lat, double* bil) {
     return CMPMC_OK;
 }
 
+// The cubic-symmetry table (KinematicsMode 3): the Jacobi solve at k = (1, s, s t) / |.|.
+extern "C" int cmpmc_lattice_build_symmetric(cmpmc_lattice* lat, float* sym) {
+    if (!lat || !sym) return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_lattice_build_symmetric: null argument");
+    Stiff C;
+    stiffness(lat->cfg, C);
+    constexpr int N = CMPMC_N_SYM;
+    for (int i = 0; i < N; ++i) {
+        const double sv = static_cast<double>(i) / (N - 1);
+        for (int j = 0; j < N; ++j) {
+            const double tv = static_cast<double>(j) / (N - 1);
+            const V k = unit(V{1.0, sv, sv * tv});
+            V vg[3];
+            christoffel(C, lat->cfg.density, k, vg);
+            for (int m = 0; m < 3; ++m)
+                for (int c = 0; c < 3; ++c)
+                    sym[((m * 3 + c) * N + i) * N + j] = static_cast<float>(comp(vg[m], c));
+        }
+    }
+    lat->sym_lut = sym;
+    return CMPMC_OK;
+}
+
+namespace cmpmc_internal {
+// Lookup in the cubic-symmetry table (crystal frame, any |k| > 0): the axis ranks of |k|
+// (largest first; ties to the lower axis), s = q / p and t = r / q of the sorted magnitudes,
+// bilinear weights over the node square, and the canonical components mapped back to their
+// axes with the signs of k. FP64 arithmetic on the FP32 table.
+void sym_lookup(const float* tab, int mode, const double k[3], double v[3]) {
+    constexpr int N = CMPMC_N_SYM;
+    const double a = std::fabs(k[0]), b = std::fabs(k[1]), c = std::fabs(k[2]);
+    const int rk[3] = {(a < b) + (a < c), (b <= a) + (b < c), (c <= a) + (c <= b)};
+    double srt[3];
+    srt[rk[0]] = a;
+    srt[rk[1]] = b;
+    srt[rk[2]] = c;
+    const double sv = (srt[0] > 0.0) ? srt[1] / srt[0] : 0.0;
+    const double tv = (srt[1] > 0.0) ? srt[2] / srt[1] : 0.0;
+    const double u = sv * (N - 1), w = tv * (N - 1);
+    int i0 = static_cast<int>(u), j0 = static_cast<int>(w);
+    i0 = i0 > N - 2 ? N - 2 : i0;
+    j0 = j0 > N - 2 ? N - 2 : j0;
+    const double du = u - i0, dv = w - j0;
+    const double w00 = (1.0 - du) * (1.0 - dv), w10 = du * (1.0 - dv), w01 = (1.0 - du) * dv, w11 = du * dv;
+    double vc[3];
+    for (int q = 0; q < 3; ++q) {
+        const float* T = tab + ((mode * 3 + q) * N + i0) * N + j0;
+        vc[q] = static_cast<double>(T[0]) * w00 + static_cast<double>(T[N]) * w10 + static_cast<double>(T[1]) * w01 +
+                static_cast<double>(T[N + 1]) * w11;
+    }
+    for (int d = 0; d < 3; ++d) v[d] = (k[d] < 0.0) ? -vc[rk[d]] : vc[rk[d]];
+}
+}  // namespace cmpmc_internal
+
 extern "C" int cmpmc_map_k_to_vg(const cmpmc_lattice* lat, int mode, uint64_t n, const double* k, double* vg_dir,
                                  double* vg_mag) {
     if (!lat || !lat->oct_lut || (n && (!k || !vg_dir || !vg_mag)) || mode < 0 || mode > 2)
         return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_map_k_to_vg: bad argument");
     const double (*O)[3] = lat->cfg.orient;
+    if (lat->kinematics_mode == 3) {              // the cubic-symmetry table
+        if (!lat->sym_lut)
+            return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_map_k_to_vg: kinematics mode 3 needs the symmetry "
+                                                           "table (cmpmc_lattice_build_symmetric)");
+        for (uint64_t q = 0; q < n; ++q) {
+            const V kd{k[3 * q], k[3 * q + 1], k[3 * q + 2]};
+            const double kc[3] = {O[0][0] * kd.x + O[1][0] * kd.y + O[2][0] * kd.z,
+                                  O[0][1] * kd.x + O[1][1] * kd.y + O[2][1] * kd.z,
+                                  O[0][2] * kd.x + O[1][2] * kd.y + O[2][2] * kd.z};
+            double v[3];
+            cmpmc_internal::sym_lookup(lat->sym_lut, mode, kc, v);
+            const V vc{v[0], v[1], v[2]};
+            const double mag = std::sqrt(nrm2(vc));
+            const double inv = 1.0 / dmax(mag, 1e-12);
+            const V t{vc.x * inv, vc.y * inv, vc.z * inv};
+            const V d = unit(V{O[0][0] * t.x + O[0][1] * t.y + O[0][2] * t.z, O[1][0] * t.x + O[1][1] * t.y + O[1][2] * t.z,
+                               O[2][0] * t.x + O[2][1] * t.y + O[2][2] * t.z});
+            vg_dir[3 * q] = d.x;
+            vg_dir[3 * q + 1] = d.y;
+            vg_dir[3 * q + 2] = d.z;
+            vg_mag[q] = mag;
+        }
+        return CMPMC_OK;
+    }
     if (lat->kinematics_mode == 2) {              // KinematicsMode::Bilinear (crystal_lattice.hpp:89-114)
         if (!lat->bil_lut)
             return cmpmc_internal::fail(CMPMC_ERR_INVALID, "cmpmc_map_k_to_vg: KinematicsMode::Bilinear needs the "

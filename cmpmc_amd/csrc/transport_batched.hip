@@ -297,7 +297,8 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     using Ctx = BatchCtx<Diff, P>;
     using Ops = typename Ctx::Ops;
     using Rec = typename Ctx::Rec;
-    __shared__ double s_cdf[2 * CMPMC_N_CDF];
+    // the decay CDFs in LDS, except where the cubic-symmetry table takes their room
+    __shared__ double s_cdf[P::lds_sym ? 1 : 2 * CMPMC_N_CDF];
     __shared__ BlockStats s_stats;
     __shared__ DevModel::SurfF s_surf[3];
     __shared__ uint32_t s_top[kMaxBlock / 64];
@@ -305,9 +306,14 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     __shared__ unsigned long long s_hbase[kMaxBlock / 64], s_tbase[kMaxBlock / 64];
     __shared__ uint32_t s_hfill[kMaxBlock / 64], s_tfill[kMaxBlock / 64], s_hvalid[kMaxBlock / 64],
         s_tvalid[kMaxBlock / 64];
-    for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
-        s_cdf[i] = m.cdf_lt[i];
-        s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
+    if constexpr (P::lds_sym) {
+        CMPMC_LDS float* t = sym_lds_table();
+        for (int i = threadIdx.x; i < kSymFloats; i += blockDim.x) t[i] = m.sym[i];
+    } else {
+        for (int i = threadIdx.x; i < CMPMC_N_CDF; i += blockDim.x) {
+            s_cdf[i] = m.cdf_lt[i];
+            s_cdf[CMPMC_N_CDF + i] = m.cdf_tt[i];
+        }
     }
     if (threadIdx.x < 3) s_surf[threadIdx.x] = m.surf_f[threadIdx.x];
     if (threadIdx.x < kMaxBlock / 64) {
@@ -354,8 +360,13 @@ transport_batched_kernel(const DevModel m, const BatchedArgs a) {
     ctx.first_event = a.dyn ? a.dyn->first_event : a.first_event;
     ctx.rec = false;
     ctx.kws = a.kaplan_ws + static_cast<size_t>(lane) * 5 * CMPMC_KCAP;   // this lane's lists, contiguous
-    ctx.lt = s_cdf;
-    ctx.tt = s_cdf + CMPMC_N_CDF;
+    if constexpr (P::lds_sym) {
+        ctx.lt = m.cdf_lt;
+        ctx.tt = m.cdf_tt;
+    } else {
+        ctx.lt = s_cdf;
+        ctx.tt = s_cdf + CMPMC_N_CDF;
+    }
     for (int i = 0; i < 4; ++i) ctx.sc[i] = 0.0;
     ctx.k_f = a.k_f;
     ctx.k_p = a.k_p;
@@ -1008,9 +1019,12 @@ template <bool Diff, class P> static const void* batched_fn_p(int waves) {
                              reinterpret_cast<const void*>(&transport_batched_kernel<3, Diff, P>),
                              reinterpret_cast<const void*>(&transport_batched_kernel<4, Diff, P>));
 }
-// flags: bit 0 exact kinematics, bit 1 surface extras (the differentiable form has none)
+// flags: bit 0 a table or solve other than the octahedral table, bit 1 surface extras (the
+// differentiable form has none), bit 3 the cubic-symmetry table (staged in LDS by the plain
+// batched kernel; the others read it from HBM)
 static const void* batched_fn(int waves, bool diff, int flags) {
     const bool exact = (flags & 1) != 0, surf = (flags & 2) != 0 && !diff;
+    if ((flags & 8) && !surf && !diff) return batched_fn_p<false, Fp32S>(waves);
     if (surf) return exact ? batched_fn_p<false, SurfX<Fp32X>>(waves) : batched_fn_p<false, SurfX<Fp32>>(waves);
     if (exact) return diff ? batched_fn_p<true, Fp32X>(waves) : batched_fn_p<false, Fp32X>(waves);
     return diff ? batched_fn_p<true, Fp32>(waves) : batched_fn_p<false, Fp32>(waves);
@@ -1028,7 +1042,7 @@ static const void* charges_fn(int waves, int flags) {
 }
 // bit 2: the FEM mesh field (charge kernels only; the phonon kernels do not read the field)
 static int model_flags(const DevModel* m) {
-    return (m->kin_mode != 0 ? 1 : 0) | (m->surf_x ? 2 : 0) | (m->mesh.dim != 0 ? 4 : 0);
+    return (m->kin_mode != 0 ? 1 : 0) | (m->surf_x ? 2 : 0) | (m->mesh.dim != 0 ? 4 : 0) | (m->kin_mode == 3 ? 8 : 0);
 }
 
 extern "C" hipError_t cmpmc_launch_mesh_query32(const DevModel* m, uint64_t n, const double* pts, const int32_t* start,

@@ -1,7 +1,8 @@
 // Drop-in CrystalLattice (lattice/crystal_lattice.hpp of the reference). The tables are
 // built on the host by cmpmc_lattice_build and cmpmc_lattice_build_bilinear (bit-identical
 // to the reference's) and uploaded to the GPU by SimulationEngine. Octahedral (default),
-// Exact and Bilinear kinematics are available.
+// Exact and Bilinear kinematics are available, and this library's Symmetric mode (a 39 KiB
+// cubic-symmetry table built by cmpmc_lattice_build_symmetric).
 #pragma once
 
 #include <memory>
@@ -22,17 +23,21 @@ public:
         Vec3 vg_dir;
         double vg_mag;
     };
-    enum class KinematicsMode : uint8_t { Octahedral = 0, Exact = 1, Bilinear = 2 };
+    // Symmetric: this library's cubic-symmetry table (cmpmc_lattice_build_symmetric; no
+    // counterpart in the reference), which the batched kernels stage in LDS.
+    enum class KinematicsMode : uint8_t { Octahedral = 0, Exact = 1, Bilinear = 2, Symmetric = 3 };
 
     // crystal_lattice.hpp:45-55
     explicit CrystalLattice(const Config& cfg)
-        : cfg_(cfg), oct_(3 * N_OCT * N_OCT * 4), bil_(3 * N_CTHETA * N_PHI * 3), cdf_lt_(N_CDF), cdf_tt_(N_CDF) {
+        : cfg_(cfg), oct_(3 * N_OCT * N_OCT * 4), bil_(3 * N_CTHETA * N_PHI * 3), sym_(3 * 3 * CMPMC_N_SYM * CMPMC_N_SYM),
+          cdf_lt_(N_CDF), cdf_tt_(N_CDF) {
         if (cfg.group != BravaisGroup::Cubic || cfg.has_custom_C)
             throw std::invalid_argument("cmpmc_amd: only cubic crystals without a custom stiffness matrix are supported");
         if (cfg.valley_dirs.size() > 8) throw std::invalid_argument("cmpmc_amd: at most 8 valleys");
         const cmpmc_crystal pod = cfg.to_pod();
         check_status(cmpmc_lattice_build(&pod, oct_.data(), cdf_lt_.data(), cdf_tt_.data(), &lat_));
         check_status(cmpmc_lattice_build_bilinear(&lat_, bil_.data()));
+        check_status(cmpmc_lattice_build_symmetric(&lat_, sym_.data()));
     }
     // The table pointers inside lat_ refer to this object's vectors.
     CrystalLattice(const CrystalLattice&) = delete;
@@ -122,6 +127,7 @@ private:
     Config cfg_;
     std::vector<float> oct_;
     std::vector<double> bil_;
+    std::vector<float> sym_;
     std::vector<double> cdf_lt_, cdf_tt_;
     cmpmc_lattice lat_{};
     KinematicsMode mode_ = KinematicsMode::Octahedral;

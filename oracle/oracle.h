@@ -107,18 +107,21 @@ void orc_film_aluminum(orc_film* f);
 #define ORC_N_CDF 1024
 #define ORC_N_CTHETA 181             /* bilinear (cos theta, phi) table: crystal_lattice.hpp:28-29 */
 #define ORC_N_PHI 361
+#define ORC_N_SYM 33                 /* cubic-symmetry table nodes per side */
 typedef struct {
     orc_crystal cfg;
     double C[3][3][3][3];
     float* oct;                      /* [3][256][256][4] = vx, vy, vz, |v| */
     double* bil;                     /* [3][181][361][3] = v_g (KinematicsMode::Bilinear) */
+    float* sym;                      /* [3][3][33][33] cubic-symmetry table (this build's mode 3) */
     double cdf_lt[ORC_N_CDF], cdf_tt[ORC_N_CDF];
     /* charge tensors (crystal_lattice.hpp:394-438) */
     double e_cond_mass_eV, h_mass_eV;
     double mass_ratio[3], hv_sqrt[3], hv_sqrt_inv[3];
     double valley_rot[8][3][3];
     double l0_e, l0_h, k_sound_e, k_sound_h;
-    int kin_mode;                    /* KinematicsMode: 0 Octahedral (default), 1 Exact, 2 Bilinear */
+    int kin_mode;                    /* KinematicsMode: 0 Octahedral (default), 1 Exact, 2 Bilinear,
+                                        3 cubic-symmetry table (no reference counterpart) */
 } orc_lattice;
 
 int  orc_lattice_init(orc_lattice* L, const orc_crystal* cfg);
@@ -132,7 +135,9 @@ void orc_map_k_to_vg(const orc_lattice* L, int mode, const double k[3],
 void orc_map_k_to_vg_exact(const orc_lattice* L, int mode, const double k[3],
                            double vg_dir[3], double* vg_mag);
 void orc_map_k_to_vg_bilinear(const orc_lattice* L, int mode, const double k[3],
-                              double vg_dir[3], double* vg_mag);    /* crystal_lattice.hpp:77-85 */
+                              double vg_dir[3], double* vg_mag);
+/* the cubic-symmetry table lookup (this build's kinematics mode 3; no reference counterpart) */
+void orc_map_k_to_vg_sym(const orc_lattice* L, int mode, const double k[3], double vg_dir[3], double* vg_mag);    /* crystal_lattice.hpp:77-85 */
 double orc_tamura_lt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:98-106 */
 double orc_tamura_tt_pdf(const orc_crystal* c, double x);         /* phonon_kernels.hpp:115-129 */
 double orc_theta_L_to_LT(const orc_crystal* c, double x);
@@ -444,6 +449,7 @@ const orc_lattice* orc_engine_lattice(const orc_engine* e);
 void orc_engine_set_kinematics(orc_engine* e, int kin_mode);   /* CrystalLattice::set_kinematics_mode */
 const float* orc_engine_oct(const orc_engine* e);
 const double* orc_engine_bil(const orc_engine* e);   /* the bilinear table [3][181][361][3] */
+const float* orc_engine_sym(const orc_engine* e);    /* the cubic-symmetry table [3][3][33][33] */
 size_t orc_sizeof(int which);   /* struct sizes, for binding checks */
 
 #ifdef __cplusplus

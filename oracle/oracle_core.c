@@ -452,6 +452,23 @@ int orc_lattice_init(orc_lattice* L, const orc_crystal* cfg) {
             }
         }
     }
+    /* cubic-symmetry table (this build's mode 3): v_g of the Jacobi solve at k = (1, s, s t) / |.|,
+     * s = i / 32, t = j / 32, components in the canonical frame |k_x| >= |k_y| >= |k_z| */
+    L->sym = (float*)malloc(sizeof(float) * 3 * 3 * ORC_N_SYM * ORC_N_SYM);
+    if (!L->sym) return -1;
+    for (int i = 0; i < ORC_N_SYM; ++i) {
+        const double sv = (double)i / (ORC_N_SYM - 1);
+        for (int j = 0; j < ORC_N_SYM; ++j) {
+            const double tv = (double)j / (ORC_N_SYM - 1);
+            const double t[3] = {1.0, sv, sv * tv};
+            double k[3], vg[3][3], vp[3];
+            v3_unit(t, k);
+            orc_solve_christoffel((const double(*)[3][3][3])L->C, cfg->density, k, vg, vp);
+            for (int m = 0; m < 3; ++m)
+                for (int c = 0; c < 3; ++c)
+                    L->sym[((m * 3 + c) * ORC_N_SYM + i) * ORC_N_SYM + j] = (float)vg[m][c];
+        }
+    }
     /* decay tables: crystal_lattice.hpp:440-446 */
     const double m = cfg->v_sound / cfg->v_trans;
     build_single_cdf(L->cdf_lt, (m - 1.0) / (m + 1.0), 1.0, cfg, 0);
@@ -507,6 +524,8 @@ void orc_lattice_free(orc_lattice* L) {
     L->oct = NULL;
     free(L->bil);
     L->bil = NULL;
+    free(L->sym);
+    L->sym = NULL;
 }
 
 /* crystal_lattice.hpp:117-130 */
@@ -566,8 +585,41 @@ void orc_map_k_to_vg_bilinear(const orc_lattice* L, int mode, const double kdet[
     *vg_mag = mag;
 }
 
+/* Cubic-symmetry table lookup: v_g(R k) = R v_g(k) for the signed axis permutations R of a
+ * cubic crystal. Sort |k| (largest first, ties to the lower axis), interpolate the canonical
+ * v_g bilinearly in (s, t) = (q / p, r / q), and put each component back on its axis with
+ * the sign of k there. */
+void orc_map_k_to_vg_sym(const orc_lattice* L, int mode, const double kdet[3], double vg_dir[3], double* vg_mag) {
+    const int N = ORC_N_SYM;
+    double k[3];
+    m3_mul_t(L->cfg.orient, kdet, k);
+    const double a = fabs(k[0]), b = fabs(k[1]), c = fabs(k[2]);
+    const int rk[3] = {(a < b) + (a < c), (b <= a) + (b < c), (c <= a) + (c <= b)};
+    double srt[3];
+    srt[rk[0]] = a; srt[rk[1]] = b; srt[rk[2]] = c;
+    const double sv = (srt[0] > 0.0) ? srt[1] / srt[0] : 0.0;
+    const double tv = (srt[1] > 0.0) ? srt[2] / srt[1] : 0.0;
+    const double u = sv * (N - 1), w = tv * (N - 1);
+    const int i0 = iclamp((int)u, 0, N - 2), j0 = iclamp((int)w, 0, N - 2);
+    const double du = u - i0, dv = w - j0;
+    const double w00 = (1.0 - du) * (1.0 - dv), w10 = du * (1.0 - dv), w01 = (1.0 - du) * dv, w11 = du * dv;
+    double vc[3], v[3];
+    for (int q = 0; q < 3; ++q) {
+        const float* T = L->sym + ((mode * 3 + q) * N + i0) * N + j0;
+        vc[q] = (double)T[0] * w00 + (double)T[N] * w10 + (double)T[1] * w01 + (double)T[N + 1] * w11;
+    }
+    for (int d = 0; d < 3; ++d) v[d] = (k[d] < 0.0) ? -vc[rk[d]] : vc[rk[d]];
+    const double mag = v3_norm(v);
+    double t[3], vd[3];
+    v3_div(v, dmax(mag, 1e-12), t);
+    m3_mul(L->cfg.orient, t, vd);
+    v3_unit(vd, vg_dir);
+    *vg_mag = mag;
+}
+
 void orc_map_k_to_vg(const orc_lattice* L, int mode, const double kdet[3],
                      double vg_dir[3], double* vg_mag) {   /* crystal_lattice.hpp:87-88, 144-176 */
+    if (L->kin_mode == 3) { orc_map_k_to_vg_sym(L, mode, kdet, vg_dir, vg_mag); return; }
     if (L->kin_mode == 1) { orc_map_k_to_vg_exact(L, mode, kdet, vg_dir, vg_mag); return; }
     if (L->kin_mode == 2) { orc_map_k_to_vg_bilinear(L, mode, kdet, vg_dir, vg_mag); return; }
     double k[3];

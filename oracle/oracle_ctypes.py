@@ -264,6 +264,9 @@ def lib():
         L.orc_map_k_to_vg.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_map_k_to_vg_exact.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
         L.orc_map_k_to_vg_bilinear.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_map_k_to_vg_sym.argtypes = [C.c_void_p, C.c_int, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(dbl)]
+        L.orc_engine_sym.restype = C.POINTER(C.c_float)
+        L.orc_engine_sym.argtypes = [C.c_void_p]
         L.orc_intervalley_matrix_rate.argtypes = [C.POINTER(Crystal), C.c_double]
         L.orc_intervalley_matrix_rate.restype = C.c_double
         L.orc_kaplan_pair_breaking_rate.argtypes = [C.POINTER(Film), C.c_double]
@@ -901,6 +904,10 @@ class Engine:
         p = lib().orc_engine_oct(self.ptr)
         return np.ctypeslib.as_array(p, shape=(3, 256, 256, 4)).copy()
 
+    def sym_table(self) -> np.ndarray:
+        p = lib().orc_engine_sym(self.ptr)
+        return np.ctypeslib.as_array(p, shape=(3, 3, 33, 33)).copy()
+
     def bil_table(self) -> np.ndarray:
         p = lib().orc_engine_bil(self.ptr)
         return np.ctypeslib.as_array(p, shape=(3, 181, 361, 3)).copy()
@@ -909,17 +916,20 @@ class Engine:
         return lib().orc_engine_lattice(self.ptr)
 
     def set_kinematics(self, kin_mode: int) -> None:
-        """CrystalLattice::set_kinematics_mode: 0 Octahedral (default), 1 Exact, 2 Bilinear."""
+        """CrystalLattice::set_kinematics_mode: 0 Octahedral (default), 1 Exact, 2 Bilinear; 3 the
+        cubic-symmetry table (this build's, no reference counterpart)."""
         lib().orc_engine_set_kinematics(self.ptr, kin_mode)
 
-    def map_k_to_vg(self, mode: int, k, exact: bool = False, bilinear: bool = False) -> np.ndarray:
+    def map_k_to_vg(self, mode: int, k, exact: bool = False, bilinear: bool = False,
+                    symmetric: bool = False) -> np.ndarray:
         k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
         out = np.empty((k.shape[0], 4))
         vg = np.empty(3)
         mag = C.c_double()
         L = lib()
         lp = self.lattice_ptr()
-        fn = L.orc_map_k_to_vg_exact if exact else (L.orc_map_k_to_vg_bilinear if bilinear else L.orc_map_k_to_vg)
+        fn = (L.orc_map_k_to_vg_exact if exact else L.orc_map_k_to_vg_bilinear if bilinear
+              else L.orc_map_k_to_vg_sym if symmetric else L.orc_map_k_to_vg)
         for i in range(k.shape[0]):
             row = np.ascontiguousarray(k[i])
             fn(lp, mode, dptr(row), dptr(vg), C.byref(mag))

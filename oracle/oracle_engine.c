@@ -1083,4 +1083,5 @@ size_t orc_sizeof(int which) {
     }
 }
 const double* orc_engine_bil(const orc_engine* e) { return e->L->bil; }
+const float* orc_engine_sym(const orc_engine* e) { return e->L->sym; }
 

@@ -148,6 +148,31 @@ int main(int argc, char** argv) {
         check("bilinear kinematics: batched steps vs plain (5%)", static_cast<double>(rb.stats.total_steps),
               static_cast<double>(r.stats.total_steps), 0.05);
     }
+    {   // Symmetric kinematics (this library's cubic-symmetry table, staged in LDS by the batched
+        // kernel): close to the exact solve, and a run closes its ledger in both forms
+        CrystalLattice ls(materials::Germanium());
+        ls.set_kinematics_mode(CrystalLattice::KinematicsMode::Symmetric);
+        DetectorModel ds;
+        const SimulationEngine es(ls, ds);
+        const auto a = ls.map_k_to_vg(PhononMode::L, Vec3{0.3, -0.5, 0.8}.unit());
+        const auto b = ls.map_k_to_vg_exact(PhononMode::L, Vec3{0.3, -0.5, 0.8}.unit());
+        check("symmetric vs exact v_g magnitude (table errors < 1e-3)", a.vg_mag, b.vg_mag, 1e-3);
+        auto gen = [&](EventInput& in) {
+            in.phonons.push_back(es.make_track({0, 0, 0}, in.rng.isotropic_dir(), PhononMode::L, 7.5e-3));
+            in.injected_eV += 7.5e-3;
+        };
+        RunOptions o;
+        o.seed = 9;
+        const RunResult r = run_events(es, 300, gen, o);
+        check("symmetric kinematics: deposited + escaped = injected",
+              r.stats.total_deposited_eV + r.stats.escaped_or_killed_eV, r.stats.total_injected_eV, 1e-9);
+        o.form = CMPMC_FORM_BATCHED;
+        const RunResult rb = run_events(es, 300, gen, o);
+        check("symmetric kinematics: batched steps vs plain (5%)", static_cast<double>(rb.stats.total_steps),
+              static_cast<double>(r.stats.total_steps), 0.05);
+        check("symmetric kinematics: batched ledger closes",
+              rb.stats.total_deposited_eV + rb.stats.escaped_or_killed_eV, rb.stats.total_injected_eV, 1e-9);
+    }
     {   // KaplanFilm on the host and KaplanSurrogateLUT (tests/test_kaplan.cpp:95-107: the
         // surrogate's collected fraction in a re-entry cavity within 3 % of the full cascade)
         const KaplanFilm film = KaplanFilm::aluminum_sensor();

@@ -149,6 +149,27 @@ def test_host_bilinear_kinematics_bit_exact(lib_cmpmc, oracle):
     assert np.array_equal(lat.bil, oracle.Engine().bil_table())
 
 
+def test_host_symmetric_kinematics_equals_oracle(lib_cmpmc, oracle):
+    """cmpmc_lattice_build_symmetric + cmpmc_map_k_to_vg in kinematics mode 3 (this build's
+    cubic-symmetry table; no reference counterpart): the table and the lookups equal the
+    oracle's bitwise, on random directions, the axes, the diagonals and a tie."""
+    cm = lib_cmpmc
+    lat = cm.Lattice()
+    with pytest.raises(cm.CmpmcError):        # no table yet: mode 3 is refused
+        lat.c.kinematics_mode = 3
+        lat.map_k_to_vg(0, np.array([[0.0, 0.0, 1.0]]))
+    lat.c.kinematics_mode = 0
+    lat.set_kinematics_mode(lat.KINEMATICS_SYMMETRIC)
+    eng = oracle.Engine()
+    assert np.array_equal(lat.sym, eng.sym_table())
+    g = np.random.default_rng(41)
+    k = np.vstack([g.normal(size=(2000, 3)), [[0, 0, 1], [0, -1, 0], [1, 0, 0], [1, 1, 1], [-1, 1, -1], [1, 1, 0],
+                                              [0.3, -0.3, 0.9]]])
+    for m in range(3):
+        d, mag = lat.map_k_to_vg(m, k)
+        assert np.array_equal(np.c_[d, mag], eng.map_k_to_vg(m, k, symmetric=True))
+
+
 def _gold():
     return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_vectors.npz"))
 

@@ -7,7 +7,10 @@ seeds, in both forms:
   with probability tau_pb / (tau_pb + tau_l) from the exact Kaplan pair-breaking rate;
 * IVRateModel::Matrix (charge_kernels.hpp:36-79; engine.hpp:197-199): the guarded
   deformation-potential + neutral-impurity intervalley rate, refused without
-  allow_unvalidated and run with it.
+  allow_unvalidated and run with it;
+* kinematics mode 3, this build's cubic-symmetry v_g table (no reference counterpart; its
+  oracle is pinned by the node, invariance and accuracy checks of tests/test_oracle.py),
+  which the batched kernel stages in LDS.
 
 The oracle is pinned bit for bit to the reference on each (tests/test_oracle.py: golden
 lookups and rates, runs c1_bil, film_rc, iv_matrix). Bars as for the other branch tests:
@@ -107,6 +110,56 @@ def test_bilinear_transport_both_forms(lib_cmpmc, oracle):
     eng.set_kinematics(0)
     t2 = eng.run_workload(cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3), 2000, seed=42, form=cm.FORM_PLAIN)
     assert t2["stats_raw"] == t["stats_raw"], "switching the engine back to the table gives the table's run"
+
+
+def test_symmetric_device_queries(lib_cmpmc, oracle):
+    """Kinematics mode 3 (this build's cubic-symmetry table; no reference counterpart) on the
+    device: FP64 lookups (plain form) equal the host's (the oracle's, bitwise) to 1e-12,
+    FP32 lookups (batched form) within FP32 rounding of the interpolation."""
+    cm = lib_cmpmc
+    lat = cm.Lattice()
+    lat.set_kinematics_mode(lat.KINEMATICS_SYMMETRIC)
+    eng = cm.Engine(lat)
+    g = np.random.default_rng(23)
+    k = np.vstack([g.normal(size=(4000, 3)), [[0, 0, 1], [0, 0, -1], [1, 0, 0], [0, 1, 0], [-1, 0, 0], [1, 1, 1],
+                                              [1, -1, 0]]])
+    k /= np.linalg.norm(k, axis=1)[:, None]
+    print(f"case: symmetric device queries, n={len(k)}", flush=True)
+    for m in range(3):
+        hd, hm = lat.map_k_to_vg(m, k)
+        d64, m64 = eng.device_map_k_to_vg(m, k, form=cm.FORM_PLAIN)
+        assert np.allclose(m64, hm, rtol=1e-12, atol=0) and np.allclose(d64, hd, rtol=0, atol=1e-12)
+        d32, m32 = eng.device_map_k_to_vg(m, k, form=cm.FORM_BATCHED)
+        assert np.allclose(m32, hm, rtol=2e-5, atol=0)
+        ang = np.arctan2(np.linalg.norm(np.cross(d32, hd), axis=1), np.sum(d32 * hd, axis=1))
+        assert np.quantile(ang, 0.99) < 2e-5 and ang.max() < 1e-3, f"mode {m}: FP32 direction error {ang.max():.2e}"
+
+
+def test_symmetric_transport_both_forms(lib_cmpmc, oracle):
+    """Kinematics mode 3 in both forms against the oracle in mode 3 (the batched form stages
+    the table in LDS); the engine really runs the table (the octahedral table's run on the
+    same seed differs) and switching back gives the octahedral run."""
+    cm, O = lib_cmpmc, oracle
+    lat = cm.Lattice()
+    lat.set_kinematics_mode(lat.KINEMATICS_SYMMETRIC)
+    eng = cm.Engine(lat)
+    oeng = O.Engine()
+    oeng.set_kinematics(3)
+    r = _phonon_parity(cm, O, eng, oeng, 2000, 42, "symmetric kinematics")
+    wl = cm.workload(cm.WL_PHONON_POINT, mode=0, energy_eV=7.5e-3)
+    t = cm.Engine().run_workload(wl, 2000, seed=42, form=cm.FORM_PLAIN)
+    assert r[cm.FORM_PLAIN]["stats_raw"] != t["stats_raw"]
+    eng.set_kinematics(0)
+    t2 = eng.run_workload(wl, 2000, seed=42, form=cm.FORM_PLAIN)
+    assert t2["stats_raw"] == t["stats_raw"], "switching the engine back to the table gives the table's run"
+    # a model created without a symmetry table builds its own: switching a default engine
+    # to mode 3 gives the same batched run as the engine built on the mode-3 lattice
+    e3 = cm.Engine()
+    e3.set_kinematics(3)
+    eng.set_kinematics(3)
+    a = e3.run_workload(wl, 2000, seed=43, form=cm.FORM_BATCHED)
+    b = eng.run_workload(wl, 2000, seed=43, form=cm.FORM_BATCHED)
+    assert a["stats_raw"] == b["stats_raw"]
 
 
 def test_rate_competition_both_forms(lib_cmpmc, oracle):
