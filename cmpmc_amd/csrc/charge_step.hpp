@@ -502,9 +502,11 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
     const R d_step = vmag * dt;
     if (vmag > R(1e-6)) {
         const V3<R> vdir = vdiv<P>(vm, vmag);
-        V3<R> n_in;
-        int sid;
-        const R dist = intersect_exit<P>(m, trk.pos, vdir, n_in, sid);
+        V3<R> n_in{R(0), R(0), R(1)};
+        int sid = 2;
+        R dist = P::inf();
+        // the FP32 form skips the exit computation for a step that cannot reach the surface
+        if (P::exact || !flight_stays_inside<P>(m, trk.pos, d_step)) dist = intersect_exit<P>(m, trk.pos, vdir, n_in, sid);
         if (dist <= d_step) {
             const R dtb = P::div(dist, vmag);
             trk.pos = trk.pos + vdir * dist;

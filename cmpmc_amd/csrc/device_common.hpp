@@ -948,6 +948,20 @@ __device__ inline typename P::R intersect_exit(const DevModel& m, V3<typename P:
     return t_min;
 }
 
+// A lower bound on the distance from a point inside the crystal to its surface: the smaller
+// of the end-cap gap and (R^2 - r^2) / 2R (<= R - r). A flight from the point shorter than
+// this, with the margin the FP32 exit computation needs (1 % and 10 nm), ends inside the
+// crystal whatever its direction, so the batched steps skip intersect_exit for it and take
+// the branch the full test would take (the exit point and normal are not used on it).
+template <class P>
+__device__ inline bool flight_stays_inside(const DevModel& m, V3<typename P::R> p, typename P::R d) {
+    using R = typename P::R;
+    const R rad = R(m.radius);
+    const R gz = R(m.half_z) - fabs(p.z);
+    const R gr = (rad * rad - (p.x * p.x + p.y * p.y)) * R(0.5 / m.radius);
+    return d < R(0.99) * tmin(gz, gr) - R(1e-8);
+}
+
 // surface_property.hpp:182-220
 __host__ __device__ inline double eval_poly(const double* c, int n, double f) {
     if (n <= 0) return 0.0;
