@@ -1052,7 +1052,7 @@ int grid_for(cmpmc_model* M, int which, const RunCfg& C, int block, int waves) {
     if (C.grid_blocks) return static_cast<int>(C.grid_blocks);
     int per_cu = 0;
     const int flags = (M->dm.kin_mode != 0 ? 1 : 0) | (M->dm.surf_x ? 2 : 0) | (M->dm.mesh.dim != 0 ? 4 : 0) |
-                      (M->dm.kin_mode == 3 ? 8 : 0);
+                      (M->dm.kin_mode == 3 ? 8 : 0) | (M->dm.charge_release != 0 ? 16 : 0);
     const hipError_t e = (which == 2) ? cmpmc_occupancy_diff(waves, flags, block, &per_cu)
                                       : cmpmc_occupancy(which, waves, flags, block, &per_cu);
     if (e != hipSuccess || per_cu <= 0) per_cu = 4;

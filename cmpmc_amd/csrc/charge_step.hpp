@@ -535,7 +535,10 @@ __device__ inline bool charge_step(const DevModel& m, CTrack<P>& trk, G& r, Ctx&
             const double muF = (m.fermi >= 0.0) ? m.fermi : 0.5 * m.bandgap;
             const double e_rel = static_cast<double>(trk.ekin) + ((trk.type == 1) ? (m.bandgap - muF) : muF);
             ctx.ccount(CC_COLLECTED);
-            const double hit_eV = (m.charge_release != 0) ? release_at_contact<P>(m, trk, e_rel, n_in, sid, r, ctx) : e_rel;
+            double hit_eV = e_rel;
+            if constexpr (P::release) {
+                if (m.charge_release != 0) hit_eV = release_at_contact<P>(m, trk, e_rel, n_in, sid, r, ctx);
+            }
             const double recomb = hit_eV * trk.weight;
             ctx.recomb(recomb);
             ctx.deposited(recomb);

@@ -1034,15 +1034,20 @@ template <class P> static const void* charges_fn_p(int waves) {
                              reinterpret_cast<const void*>(&transport_charges_kernel<3, P>),
                              reinterpret_cast<const void*>(&transport_charges_kernel<4, P>));
 }
+// bit 4: charge release at collection (the SurfX and MeshRX kernels carry it)
 static const void* charges_fn(int waves, int flags) {
-    const bool exact = (flags & 1) != 0;
-    if (flags & 4) return exact ? charges_fn_p<MeshX<Fp32X>>(waves) : charges_fn_p<MeshX<Fp32>>(waves);
-    if (flags & 2) return exact ? charges_fn_p<SurfX<Fp32X>>(waves) : charges_fn_p<SurfX<Fp32>>(waves);
+    const bool exact = (flags & 1) != 0, rel = (flags & 16) != 0;
+    if (flags & 4) {
+        if (rel) return exact ? charges_fn_p<MeshRX<Fp32X>>(waves) : charges_fn_p<MeshRX<Fp32>>(waves);
+        return exact ? charges_fn_p<MeshX<Fp32X>>(waves) : charges_fn_p<MeshX<Fp32>>(waves);
+    }
+    if ((flags & 2) || rel) return exact ? charges_fn_p<SurfX<Fp32X>>(waves) : charges_fn_p<SurfX<Fp32>>(waves);
     return exact ? charges_fn_p<Fp32X>(waves) : charges_fn_p<Fp32>(waves);
 }
 // bit 2: the FEM mesh field (charge kernels only; the phonon kernels do not read the field)
 static int model_flags(const DevModel* m) {
-    return (m->kin_mode != 0 ? 1 : 0) | (m->surf_x ? 2 : 0) | (m->mesh.dim != 0 ? 4 : 0) | (m->kin_mode == 3 ? 8 : 0);
+    return (m->kin_mode != 0 ? 1 : 0) | (m->surf_x ? 2 : 0) | (m->mesh.dim != 0 ? 4 : 0) | (m->kin_mode == 3 ? 8 : 0) |
+           (m->charge_release != 0 ? 16 : 0);
 }
 
 extern "C" hipError_t cmpmc_launch_mesh_query32(const DevModel* m, uint64_t n, const double* pts, const int32_t* start,
