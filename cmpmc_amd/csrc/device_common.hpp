@@ -669,7 +669,28 @@ __device__ inline bool christoffel_cardano(const T* C, T rho, const T k[3], int 
 // degenerate directions ([100], [111]) the transverse polarisation is set by rounding
 // alone, so the FP32 form's FP64 fallback must round like the plain form and the
 // reference.
-__device__ __noinline__ void map_k_to_vg_exact64(const DevModel& m, int mode, const double kdet[3], double vg_dir[3],
+// What the exact solvers read of the model. They are out of line, and a kernel that hands
+// them its by-value DevModel parameter copies all 3.6 KB of it into per-lane scratch
+// memory; this small copy is made at the call instead.
+struct ExactModel {
+    double orient[3][3];
+    float orient_f[9];
+    const double* stiff;
+    const float* stiff_f;
+    double density;
+};
+__device__ inline ExactModel exact_model(const DevModel& m) {
+    ExactModel e;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) e.orient[i][j] = m.orient[i][j];
+    for (int i = 0; i < 9; ++i) e.orient_f[i] = m.orient_f[i];
+    e.stiff = m.stiff;
+    e.stiff_f = m.stiff_f;
+    e.density = m.density;
+    return e;
+}
+
+__device__ __noinline__ void map_k_to_vg_exact64(const ExactModel& m, int mode, const double kdet[3], double vg_dir[3],
                                                  double* vg_mag) {
 #pragma clang fp contract(off)
     const double (*O)[3] = m.orient;
@@ -702,7 +723,7 @@ __device__ __noinline__ void map_k_to_vg_exact64(const DevModel& m, int mode, co
 // The same in FP32: the cubic, eigenvectors and group velocity in single precision, with
 // the FP64 solve (above) for directions whose relative eigenvalue gap is below 1e-2,
 // where FP32 eigenvectors would lose more than ~1e-5.
-__device__ __noinline__ void map_k_to_vg_exact32(const DevModel& m, int mode, const float kdet[3], float vg_dir[3],
+__device__ __noinline__ void map_k_to_vg_exact32(const ExactModel& m, int mode, const float kdet[3], float vg_dir[3],
                                                  float* vg_mag) {
     const float* O = m.orient_f;
     const V3<float> kc{O[0] * kdet[0] + O[3] * kdet[1] + O[6] * kdet[2], O[1] * kdet[0] + O[4] * kdet[1] + O[7] * kdet[2],
@@ -841,8 +862,9 @@ __device__ inline void map_k_to_vg(const DevModel& m, int mode, V3<typename P::R
     } else {
         const R kk[3] = {kdet.x, kdet.y, kdet.z};
         R d[3];
-        if constexpr (P::exact) map_k_to_vg_exact64(m, mode, kk, d, &vg_mag);
-        else map_k_to_vg_exact32(m, mode, kk, d, &vg_mag);
+        const ExactModel em = exact_model(m);
+        if constexpr (P::exact) map_k_to_vg_exact64(em, mode, kk, d, &vg_mag);
+        else map_k_to_vg_exact32(em, mode, kk, d, &vg_mag);
         vg_dir = V3<R>{d[0], d[1], d[2]};
     }
 }
