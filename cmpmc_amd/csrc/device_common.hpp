@@ -305,7 +305,7 @@ struct Fp64 {
     static constexpr bool lds_sym = false;       // the cubic-symmetry table is read from HBM, not LDS
     static constexpr bool surf_x = false;        // no electrode masks or AMM film entry
     static constexpr bool mesh = false;          // uniform field (no FEM mesh)
-    static constexpr bool release = true;        // charge release at collection compiled in
+    static constexpr bool release = false;       // no charge release at collection (SurfX, MeshRX)
     __device__ static Rng track_stream(uint64_t base, uint64_t key) { return rng_for_track(base, key); }
     __device__ static double uniform(Rng& r) { return static_cast<double>((next_u64(r) >> 11) | 1ULL) * 0x1.0p-53; }
     __device__ static double neg_log(double u) { return -log(u); }
@@ -340,7 +340,7 @@ struct Fp32 {
     static constexpr bool mesh = false;
     // charge release at collection is an out-of-line call whose argument copy and register
     // saves would cost the default charge kernels their occupancy: only the SurfX and MeshRX
-    // kernels carry it
+    // kernels carry it (in both forms)
     static constexpr bool release = false;
     __device__ static Rng32 track_stream(uint64_t base, uint64_t key) { return rng32_for_track(base, key); }
     // the top 24 bits of a draw: (0, 1), never 0 or 1 (64-bit event streams and 32-bit

@@ -381,12 +381,19 @@ extern "C" hipError_t cmpmc_launch_plain(const cmpmc_dev::DevModel* m, const cmp
     const int bs = (block > 0 && block <= cmpmc_dev::kMaxBlock) ? block : 256;
     const unsigned grid = static_cast<unsigned>((n + bs - 1) / bs);
     using namespace cmpmc_dev;
-    if (m->mesh.dim != 0) {
+    // charge release at collection is compiled into the SurfX and MeshRX kernels only
+    const bool rel = m->charge_release != 0;
+    if (m->mesh.dim != 0 && rel) {
+        if (m->kin_mode != 0)
+            hipLaunchKernelGGL(transport_plain_kernel<MeshRX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+        else
+            hipLaunchKernelGGL(transport_plain_kernel<MeshRX<Fp64>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
+    } else if (m->mesh.dim != 0) {
         if (m->kin_mode != 0)
             hipLaunchKernelGGL(transport_plain_kernel<MeshX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
         else
             hipLaunchKernelGGL(transport_plain_kernel<MeshX<Fp64>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
-    } else if (m->surf_x) {
+    } else if (m->surf_x || rel) {
         if (m->kin_mode != 0)
             hipLaunchKernelGGL(transport_plain_kernel<SurfX<Fp64X>>, dim3(grid), dim3(bs), 0, stream, *m, *a);
         else
