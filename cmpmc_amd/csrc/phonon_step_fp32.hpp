@@ -18,8 +18,6 @@
 //    absorption, or re-emission with a DOS mode and an inward direction); the step does
 //    that itself, re-emission through the converged tail like a diffuse reflection.
 // Rare branches (anharmonic decay, surface down-conversion) keep their own lookups.
-// A free flight that cannot reach the surface from where it starts skips the exit
-// computation (flight_stays_inside); a wave runs it only for lanes near a wall.
 //
 // The context also receives the likelihood terms of every random choice that depends on
 // the differentiable parameters (no-ops outside the differentiable form):
@@ -71,9 +69,8 @@ __device__ inline int phonon_step_fp32(const DevModel& m, PTrack<P>& trk, Ctx& c
         const float rate_down = (trk.mode != 0) ? 0.0f : m.c_anh_f * (e2 * e2 * e);
         const float rate_tot = rate_iso + rate_poly + rate_down;
         const float d_phys = (rate_tot > 1e-12f) ? (P::neg_log(P::uniform(r)) * P::div(trk.vg_mag, rate_tot)) : P::inf();
-        int sid = 2;
-        float dist = P::inf();
-        if (!flight_stays_inside<P>(m, trk.pos, d_phys)) dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
+        int sid;
+        const float dist = intersect_exit<P>(m, trk.pos, trk.vg, n_in, sid);
         if (d_phys < dist) {                      // bulk event
             const float tau = P::div(d_phys, trk.vg_mag);
             trk.pos = trk.pos + trk.vg * d_phys;
